@@ -1,0 +1,334 @@
+#!/usr/bin/env python3
+"""Posterior Fourier points/s of the CRAVA Fourier-domain inversion core on one MI355X.
+
+Workload (default): 512x512x320 FFT grid (interior 448x448x256), 6 angle
+stacks at 4-40 deg, per-angle S/N 100..20, seeded synthetic float32 inputs
+generated on the GPU.  One step is the whole posterior path of
+Crava::computePostMeanResidAndFFTCov with its transforms:
+
+  16 forward r2c (6 data, 3 prior means, 6 prior covariances, errCorr)
+  + the per-point posterior over all P = 257*512*320 Fourier points
+  + 15 inverse c2r (3 posterior means, 6 posterior covariances, 6 residuals)
+
+run twice: the MIXED form (float grids and transforms, FP64 solve where
+noise/signal < threshold) and the plain FP64 form (double grids, double
+transforms, double solve).  value = P * steps / time of the MIXED form.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="precision")
+    ap.add_argument("--threshold", type=float, default=1e-3, help="noise/signal below which MIXED uses FP64")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=5)
+    return ap.parse_args()
+
+
+class Pipeline:
+    """Working grids + posterior handle for one precision form."""
+
+    def __init__(self, ctx, prob, params, tables, fp64: bool):
+        import torch
+        from crava_amd import _lib as L
+        from crava_amd.core import Posterior
+        self.ctx, self.prob, self.fp64 = ctx, prob, fp64
+        dt = torch.float64 if fp64 else torch.float32
+        self.mu = [g.to(dt).clone() for g in prob.mu]
+        self.data = [g.to(dt).clone() for g in prob.data]
+        self.cov = [g.to(dt).clone() for g in prob.cov]
+        self.ec0 = prob.errcorr.to(dt).clone()
+        self.ec = self.ec0.clone()
+        self.post = Posterior(ctx, params, tables)
+        self.precision = L.F64 if fp64 else L.MIXED
+        self.L = L
+
+    def step(self, counts=False):
+        L, c = self.L, self.prob.cfg
+        n = (c.nxp, c.nyp, c.nzp)
+        self.ec.copy_(self.ec0)
+        for g in self.data:
+            self.ctx.r2c(g, *n, L.DATA)
+        for g in self.mu:
+            self.ctx.r2c(g, *n, L.PARAMETER)
+        for g in self.cov:
+            self.ctx.r2c(g, *n, L.COVARIANCE)
+        self.ctx.r2c(self.ec, *n, L.COVARIANCE)
+        r = self.post.run(self.precision, self.mu, self.data, self.cov, self.ec, counts=counts)
+        for g in self.mu:
+            self.ctx.c2r(g, *n, L.PARAMETER)
+        for g in self.cov:
+            self.ctx.c2r(g, *n, L.COVARIANCE)
+        for g in self.data:
+            self.ctx.c2r(g, *n, L.DATA)
+        return r
+
+    def reset(self):
+        for dst, src in zip(self.mu + self.data + self.cov, self.prob.mu + self.prob.data + self.prob.cov):
+            dst.copy_(src)
+
+
+def rel_err(a, b, nxp, ref_base=None):
+    import torch
+    a = a[..., :nxp].double()
+    b = b[..., :nxp].double()
+    den = (b - ref_base[..., :nxp].double()) if ref_base is not None else b
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(den))
+
+
+def timed_kernel(fn, reps):
+    """Average duration (ms) of fn() over reps launches, HIP events on torch's stream."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for s, e in ev:
+        s.record()
+        fn()
+        e.record()
+    torch.cuda.synchronize()
+    return sum(s.elapsed_time(e) for s, e in ev) / reps
+
+
+def cpu_baseline(prob, tabs, budget_s):
+    """Reference CPU path (FFTW 2.1.2 float + lib_matr.c built from the
+    reference sources) on one core, on a bounded sample: one full-size 3-D
+    r2c, one c2r, and the per-point loop over a few in-band and out-of-band
+    k-planes; the step time is composed from the measured per-unit costs."""
+    import numpy as np
+    import oracle
+    cfg = prob.cfg
+    nxp, nyp, nzp, nt = cfg.nxp, cfg.nyp, cfg.nzp, prob.ntheta
+    kind = "reference" if oracle.ref_available() else "port"
+    which = "ref" if kind == "reference" else "oracle"
+    cn = nxp // 2 + 1
+    P = cn * nyp * nzp
+    g = prob.data[0].cpu().numpy().copy()
+    t0 = time.perf_counter()
+    oracle.fft_r2c(g, nxp, nyp, nzp, False, which)
+    t_r2c = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.fft_c2r(g, nxp, nyp, nzp, False, which)
+    t_c2r = time.perf_counter() - t0
+    # loop sample: planes of Fourier-domain inputs (random but representative magnitudes)
+    inv = tabs["invert"]
+    kin = [k for k in range(nzp) if inv[k]]
+    kout = [k for k in range(nzp) if not inv[k]]
+    rng = np.random.default_rng(0)
+
+    def plane_inputs(ks):
+        m = len(ks)
+        shp = (m, nyp, 2 * cn)
+        mk = lambda s: (rng.standard_normal(shp) * s).astype(np.float32)
+        mu = [mk(1e-2) for _ in range(3)]
+        d = [mk(1e-2) for _ in range(nt)]
+        base = np.abs(rng.standard_normal(shp)).astype(np.float32) * 50 + 1
+        cv = [(base * np.float32(prob.sigma0[a, b])).astype(np.float32)
+              for a, b in [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]]
+        ec = np.abs(rng.standard_normal(shp)).astype(np.float32) * 10 + 1
+        sub = {k: np.ascontiguousarray(v[ks]) for k, v in tabs.items()}
+        return mu, d, cv, ec, sub
+
+    per_pt = {}
+    remaining = max(2.0, budget_s - 2 * (t_r2c + t_c2r))
+    for label, ks in (("in", kin), ("out", kout)):
+        if not ks:
+            per_pt[label] = 0.0
+            continue
+        m = max(1, min(len(ks), 4))
+        mu, d, cv, ec, sub = plane_inputs(ks[:m])
+        t0 = time.perf_counter()
+        oracle.posterior(nxp, nyp, m, prob.A, prob.err_theta_cov, cfg.wnc, prob.sigma0, sub, mu, d, cv, ec, which)
+        dtm = time.perf_counter() - t0
+        per_pt[label] = dtm / (m * cn * nyp)
+        remaining -= dtm
+    n_in = len(kin) * cn * nyp
+    n_out = len(kout) * cn * nyp
+    t_step = 16 * t_r2c + 15 * t_c2r + per_pt["in"] * n_in + per_pt["out"] * n_out
+    return {
+        "value": P / t_step,
+        "unit": "Fourier pts/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": (f"1 full-grid r2c ({t_r2c:.2f}s) + 1 c2r ({t_c2r:.2f}s) of {nxp}x{nyp}x{nzp} and the per-point "
+                   f"loop over up to 4 in-band + 4 out-of-band k-planes ({per_pt['in']*1e6:.2f}/"
+                   f"{per_pt['out']*1e6:.3f} us per point); step = 16 r2c + 15 c2r + loop over all P points"),
+        "seconds_per_step": t_step,
+        "cpu": platform.processor() or platform.machine(),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from crava_amd import _lib as L
+    from crava_amd import synth
+    from crava_amd.core import Context, problem_tables
+
+    torch.cuda.init()
+    dev = torch.cuda.current_device()
+    cfg = synth.config(args.config)
+    log(f"config {cfg.name}: grid {cfg.nxp}x{cfg.nyp}x{cfg.nzp}, ntheta={len(cfg.thetas_deg)}")
+    t0 = time.time()
+    prob = synth.make_problem(cfg, "cuda")
+    torch.cuda.synchronize()
+    log(f"generated inputs on the GPU in {time.time()-t0:.1f}s")
+    tabs = problem_tables(prob)
+    params = synth.post_params(prob, args.threshold)
+    ctx = Context()
+    nxp, nyp, nzp = cfg.nxp, cfg.nyp, cfg.nzp
+    cn = nxp // 2 + 1
+    P = cn * nyp * nzp
+    n_in = int(tabs["invert"].sum()) * cn * nyp
+    nt = prob.ntheta
+
+    mixed = Pipeline(ctx, prob, params, tabs, fp64=False)
+    res = {}
+    # accuracy: one step of each form from the same inputs
+    nfail, n64 = mixed.step(counts=True)
+    torch.cuda.synchronize()
+    log(f"mixed step: {nfail} failed factorisations, {n64} of {n_in} in-band points in FP64")
+    acc = {}
+    if not args.no_fp64:
+        f64 = Pipeline(ctx, prob, params, tabs, fp64=True)
+        nfail64, _ = f64.step(counts=True)
+        torch.cuda.synchronize()
+        acc["mean_update"] = max(rel_err(a, b, nxp, p) for a, b, p in zip(mixed.mu, f64.mu, prob.mu))
+        acc["mean"] = max(rel_err(a, b, nxp) for a, b in zip(mixed.mu, f64.mu))
+        acc["cov"] = max(rel_err(a, b, nxp) for a, b in zip(mixed.cov, f64.cov))
+        acc["residual"] = max(rel_err(a, b, nxp) for a, b in zip(mixed.data, f64.data))
+        log(f"rel err mixed vs FP64: {acc}")
+        f64.reset()
+    mixed.reset()
+
+    def run(pipe, steps, warmup):
+        for _ in range(warmup):
+            pipe.step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            pipe.step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    ms_mixed = run(mixed, args.steps, args.warmup) * 1e3
+    log(f"mixed: {ms_mixed:.3f} ms/step, {P/ms_mixed*1e3:.4g} pts/s")
+    ms_f64 = None
+    if not args.no_fp64:
+        ms_f64 = run(f64, max(1, args.steps), 1) * 1e3
+        log(f"fp64: {ms_f64:.3f} ms/step, {P/ms_f64*1e3:.4g} pts/s")
+
+    # per-kernel timing (HIP events on the launch stream)
+    g = mixed.data[0]
+    n3 = (nxp, nyp, nzp)
+    t_x = timed_kernel(lambda: ctx.fft_pass(g, *n3, 0, False, 1.0), args.kernel_reps)
+    t_y = timed_kernel(lambda: ctx.fft_pass(g, *n3, 1, False), args.kernel_reps)
+    t_z = timed_kernel(lambda: ctx.fft_pass(g, *n3, 2, False), args.kernel_reps)
+    grid_c_bytes = 8 * cn * nyp * nzp
+    real_bytes = 4 * nxp * nyp * nzp
+    mixed.reset()
+    for gg in mixed.data + mixed.mu:
+        ctx.r2c(gg, *n3, L.DATA)
+    for gg in mixed.cov:
+        ctx.r2c(gg, *n3, L.COVARIANCE)
+    ctx.r2c(mixed.ec, *n3, L.COVARIANCE)
+    t_post = timed_kernel(lambda: mixed.post.run(L.MIXED, mixed.mu, mixed.data, mixed.cov, mixed.ec),
+                          args.kernel_reps)
+    t_post32 = timed_kernel(lambda: mixed.post.run(L.F32, mixed.mu, mixed.data, mixed.cov, mixed.ec),
+                            args.kernel_reps)
+    t_post64 = None
+    if not args.no_fp64:
+        for gg in f64.data + f64.mu:
+            ctx.r2c(gg, *n3, L.DATA)
+        for gg in f64.cov:
+            ctx.r2c(gg, *n3, L.COVARIANCE)
+        ctx.r2c(f64.ec, *n3, L.COVARIANCE)
+        t_post64 = timed_kernel(lambda: f64.post.run(L.F64, f64.mu, f64.data, f64.cov, f64.ec), args.kernel_reps)
+    post_bytes = n_in * 8 * (19 + 2 * nt) + (P - n_in) * 96
+    kernels = {
+        "fft_x_r2c": {"ms": t_x, "alg_bytes": real_bytes + grid_c_bytes},
+        "fft_y": {"ms": t_y, "alg_bytes": 2 * grid_c_bytes},
+        "fft_z": {"ms": t_z, "alg_bytes": 2 * grid_c_bytes},
+        "posterior_mixed": {"ms": t_post, "alg_bytes": post_bytes},
+        "posterior_f32": {"ms": t_post32, "alg_bytes": post_bytes},
+    }
+    if t_post64:
+        kernels["posterior_f64"] = {"ms": t_post64, "alg_bytes": 2 * post_bytes}
+    for k in kernels.values():
+        k["GBps"] = k["alg_bytes"] / (k["ms"] * 1e-3) / 1e9
+    # dominant kernel of the step by total time: 31 transforms x 3 passes vs one posterior
+    fft_total = 31 * (t_x + t_y + t_z)
+    if fft_total >= t_post:
+        dom_name = max(("fft_x_r2c", "fft_y", "fft_z"), key=lambda k: kernels[k]["ms"])
+    else:
+        dom_name = "posterior_mixed"
+    dom = kernels[dom_name]
+    roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(dom["GBps"], 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(dom["GBps"] / PEAK_HBM_GBS, 4), "traffic": None}
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(prob, tabs, args.cpu_seconds)
+        except Exception as e:  # the baseline is reported, never the target
+            log(f"cpu baseline failed: {e}")
+    value = P / (ms_mixed * 1e-3)
+    out = {
+        "metric": "posterior Fourier pts/s (512x512x320, 6 angles): mixed vs FP64, rel err<=1e-5",
+        "value": value,
+        "unit": "Fourier pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_mixed,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 (mixed)",
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": f"posterior path {nxp}x{nyp}x{nzp} FFT grid, ntheta={nt}", "fourier_points": P,
+                   "in_band_points": n_in, "transforms_per_step": 31, "parallelism": "single GPU"},
+        "fp64_value": (P / (ms_f64 * 1e-3)) if ms_f64 else None,
+        "fp64_ms_per_step": ms_f64,
+        "speedup_vs_fp64": (ms_f64 / ms_mixed) if ms_f64 else None,
+        "rel_err_vs_fp64": acc,
+        "fp64_points": n64,
+        "posterior_kernel_pts_per_s": {"mixed": P / (t_post * 1e-3), "f32": P / (t_post32 * 1e-3),
+                                       "f64": (P / (t_post64 * 1e-3)) if t_post64 else None},
+        "kernels": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kernels.items()},
+        "fft_3d_alg_frac": (real_bytes + grid_c_bytes) / ((t_x + t_y + t_z) * 1e-3) / 1e9 / PEAK_HBM_GBS,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,103 @@
+"""ctypes binding of libcrava_hip.so (the C ABI in include/crava_hip.h).
+
+The library is loaded from this package directory.  If it is missing the
+import fails loudly: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcrava_hip.so")
+
+OK = 0
+E_INVALID = -1
+E_OOM = -2
+E_HIP = -3
+E_STATE = -4
+
+DATA, PARAMETER, COVARIANCE = 0, 1, 2
+F32, MIXED, F64 = 0, 1, 2
+
+_lib = None
+
+
+class CravaHipError(RuntimeError):
+    pass
+
+
+class PostParams(C.Structure):
+    _fields_ = [
+        ("nxp", C.c_int), ("nyp", C.c_int), ("nzp", C.c_int), ("ntheta", C.c_int),
+        ("wnc", C.c_float),
+        ("sigma0", C.c_double * 9),
+        ("A", C.c_float * 24),
+        ("err_theta_cov", C.c_double * 64),
+        ("mixed_threshold", C.c_double),
+    ]
+
+
+VP = C.c_void_p
+I = C.c_int
+LL = C.c_longlong
+
+# name -> (restype, argtypes); every function declared in include/crava_hip.h
+SIGNATURES = {
+    "crava_hip_version": (I, []),
+    "crava_hip_last_error": (C.c_char_p, []),
+    "crava_hip_ctx_create": (I, [C.POINTER(VP), VP]),
+    "crava_hip_ctx_destroy": (I, [VP]),
+    "crava_hip_ctx_set_stream": (I, [VP, VP]),
+    "crava_hip_synchronize": (I, [VP]),
+    "crava_hip_fft_r2c": (I, [VP, VP, I, I, I, I, I]),
+    "crava_hip_fft_c2r": (I, [VP, VP, I, I, I, I, I]),
+    "crava_hip_fft_pass": (I, [VP, VP, I, I, I, I, I, I, C.c_double]),
+    "crava_hip_perk_tables": (I, [I, I, I, C.c_double, C.c_float, C.c_float, C.c_float, VP, VP, VP, VP, VP, VP, VP]),
+    "crava_hip_post_create": (I, [VP, C.POINTER(PostParams), VP, VP, VP, VP, VP, C.POINTER(VP)]),
+    "crava_hip_post_destroy": (I, [VP]),
+    "crava_hip_post_run": (I, [VP, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP,
+                               C.POINTER(LL), C.POINTER(LL)]),
+    "crava_hip_simulate": (I, [VP, I, I, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(LL)]),
+    "crava_hip_fill_complex_noise": (I, [VP, VP, I, I, I, C.c_uint64, C.c_uint64]),
+    "crava_hip_facies": (I, [VP, I, I, I, I, I, I, VP, VP, VP, I, VP, I, I, I, VP, VP, VP, C.c_float,
+                             C.POINTER(VP), VP, VP]),
+    "crava_hip_nll_grad": (I, [VP, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP, VP, VP, VP,
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "crava_hip_batch_create": (I, [VP, I, VP, C.POINTER(VP)]),
+    "crava_hip_batch_run": (I, [VP, I]),
+    "crava_hip_batch_destroy": (I, [VP]),
+    "crava_hip_autotune_fft": (I, [VP, I, I, I, I, VP]),
+}
+
+
+def lib():
+    """Load libcrava_hip.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CravaHipError(
+                f"{LIB_PATH} is missing; build it with `python -m crava_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != OK:
+        msg = lib().crava_hip_last_error()
+        raise CravaHipError(f"{what} failed with {rc}: {msg.decode() if msg else ''}")
+
+
+def ptr_array(ptrs):
+    arr = (VP * max(1, len(ptrs)))()
+    for i, p in enumerate(ptrs):
+        arr[i] = p
+    return arr

@@ -1,0 +1,124 @@
+"""torch-facing wrappers over the C ABI (device memory and streams only).
+
+Every call goes to libcrava_hip.so; tensors must live on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._lib import check, lib, ptr_array
+
+
+def _stream_ptr(stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def _dev_ptr(t: torch.Tensor) -> int:
+    if not t.is_cuda:
+        raise L.CravaHipError("tensor must be on the GPU")
+    if not t.is_contiguous():
+        raise L.CravaHipError("tensor must be contiguous")
+    return int(t.data_ptr())
+
+
+def precision_of(t: torch.Tensor) -> int:
+    if t.dtype == torch.float64:
+        return L.F64
+    if t.dtype == torch.float32:
+        return L.F32
+    raise L.CravaHipError(f"unsupported grid dtype {t.dtype}")
+
+
+class Context:
+    """A crava_hip_ctx bound to a torch stream."""
+
+    def __init__(self, stream=None):
+        self.h = C.c_void_p()
+        check(lib().crava_hip_ctx_create(C.byref(self.h), _stream_ptr(stream)), "crava_hip_ctx_create")
+
+    def close(self):
+        if self.h:
+            lib().crava_hip_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream):
+        check(lib().crava_hip_ctx_set_stream(self.h, _stream_ptr(stream)), "crava_hip_ctx_set_stream")
+
+    def synchronize(self):
+        check(lib().crava_hip_synchronize(self.h), "crava_hip_synchronize")
+
+    def r2c(self, grid: torch.Tensor, nxp: int, nyp: int, nzp: int, cubetype: int):
+        check(lib().crava_hip_fft_r2c(self.h, _dev_ptr(grid), nxp, nyp, nzp, cubetype, precision_of(grid)),
+              "crava_hip_fft_r2c")
+
+    def fft_pass(self, grid: torch.Tensor, nxp, nyp, nzp, axis: int, inverse: bool, scale: float = 1.0):
+        check(lib().crava_hip_fft_pass(self.h, _dev_ptr(grid), nxp, nyp, nzp, axis, int(inverse),
+                                       precision_of(grid), scale), "crava_hip_fft_pass")
+
+    def c2r(self, grid: torch.Tensor, nxp: int, nyp: int, nzp: int, cubetype: int):
+        check(lib().crava_hip_fft_c2r(self.h, _dev_ptr(grid), nxp, nyp, nzp, cubetype, precision_of(grid)),
+              "crava_hip_fft_c2r")
+
+
+def perk_tables(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wavelets: np.ndarray, wnorm: np.ndarray):
+    """Host per-k vectors (crava_hip_perk_tables)."""
+    wav = np.ascontiguousarray(wavelets, dtype=np.float32)
+    wn = np.ascontiguousarray(wnorm, dtype=np.float32)
+    inv = np.zeros(nzp, dtype=np.uint8)
+    out = [np.zeros((nzp, ntheta, 2), dtype=np.float32) for _ in range(4)]
+    check(lib().crava_hip_perk_tables(ntheta, nz, nzp, float(lz), float(low_cut), float(high_cut),
+                                      float(min_rel_thick), wav.ctypes.data, wn.ctypes.data, inv.ctypes.data,
+                                      *[o.ctypes.data for o in out]), "crava_hip_perk_tables")
+    return {"invert": inv, "kWD": out[0], "e1": out[1], "e2": out[2], "e3": out[3]}
+
+
+def problem_tables(prob):
+    cfg = prob.cfg
+    return perk_tables(prob.ntheta, cfg.nz, cfg.nzp, prob.lz, cfg.low_cut, cfg.high_cut, 1.0,
+                       prob.wavelets, prob.wnorm)
+
+
+class Posterior:
+    """crava_hip_post: per-k tables of one problem resident on the GPU."""
+
+    def __init__(self, ctx: Context, params, tables):
+        self.ctx = ctx
+        self.params = params
+        self._tabs = {k: np.ascontiguousarray(v) for k, v in tables.items()}
+        self.h = C.c_void_p()
+        t = self._tabs
+        check(lib().crava_hip_post_create(ctx.h, C.byref(params), t["invert"].ctypes.data, t["kWD"].ctypes.data,
+                                          t["e1"].ctypes.data, t["e2"].ctypes.data, t["e3"].ctypes.data,
+                                          C.byref(self.h)), "crava_hip_post_create")
+
+    def close(self):
+        if self.h:
+            lib().crava_hip_post_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, precision: int, mu, data, cov, errcorr, counts: bool = False):
+        mu_p = ptr_array([_dev_ptr(t) for t in mu])
+        d_p = ptr_array([_dev_ptr(t) for t in data])
+        c_p = ptr_array([_dev_ptr(t) for t in cov])
+        nf, n64 = C.c_longlong(0), C.c_longlong(0)
+        check(lib().crava_hip_post_run(self.h, precision, mu_p, d_p, c_p, _dev_ptr(errcorr),
+                                       C.byref(nf) if counts else None, C.byref(n64) if counts else None),
+              "crava_hip_post_run")
+        return (nf.value, n64.value) if counts else None

@@ -1,0 +1,19 @@
+// Host-side per-k preparation of the posterior loop (constant-thickness case).
+#pragma once
+
+namespace crava {
+
+// Per-k vectors of Crava::computePostMeanResidAndFFTCov (src/crava.cpp:1042-1057):
+//   invert[k] : 5*minRelThick < f_k < 55-style frequency gate (:1044,1098)
+//   kWD[k][l] : kD(k) * conj(W_l(k))            (:1050-1051, fillkW :1337-1345)
+//   e1[k][l]  : conj(W_l(k)) / |w_l|            (fillkWNorm :1347-1356)
+//   e2[k][l]  : conj(E3_l(k)) / |e3_l|, e3 = central(backward(forward(w_l)))
+//   e3[k][l]  : 1                                (:1057)
+// Spectra are 1-D DFTs with FFTW's forward sign, computed in double and
+// rounded to float; k >= nzp/2+1 uses the Hermitian extension of
+// Wavelet::getCAmp (src/wavelet.cpp:309-323).
+void perk_vectors(int ntheta, int nz, int nzp, double lz, float low_cut, float high_cut, float min_rel_thick,
+                  const float *wavelets, const float *wnorm, unsigned char *invert, float *kWD, float *e1,
+                  float *e2, float *e3);
+
+}  // namespace crava

@@ -1,0 +1,323 @@
+"""Seeded synthetic inversion inputs (no field data ships with the reference).
+
+Builds, from one seed, everything the posterior path consumes, following the
+formulas of the reference where it defines them:
+
+* prior covariance Sigma0 from standard deviations and correlations;
+* lateral correlation corrXY on the padded grid (ModelGeneral::findCorrXYGrid,
+  src/modelgeneral.cpp:4073-4108) with a spherical variogram
+  (SphericalVario::corr, src/vario.cpp:66-73);
+* circular temporal correlation (SeismicParametersHolder::computeCircCorrT /
+  makeCircCorrTPosDef, src/seismicparametersholder.cpp:431-499) with the
+  low-cut integer of src/modelgeneral.cpp:3961;
+* the six prior covariance grids (FFTGrid::fillInParamCorr with zero
+  gradients, src/fftgrid.cpp:1119-1164, times Sigma0_ab,
+  src/seismicparametersholder.cpp:214-219) and the error-correlation grid
+  (FFTGrid::fillInErrCorr, src/fftgrid.cpp:1065-1116);
+* Ricker wavelets in FFT order (Wavelet::Ricker, src/wavelet.cpp:1354-1358;
+  Wavelet::resample ordering and norm, :416-463; Wavelet::scale, :477-486);
+* the PP reflection matrix (ModelAVODynamic::setupDefaultReflectionMatrix,
+  src/modelavodynamic.cpp:876-928);
+* angular noise correlation Sigma_e,lm = s_l s_m exp(-3 |dtheta| / 10 deg);
+* background trends, a Gaussian perturbation drawn from the prior and seismic
+  d = K m + white noise at the requested S/N.
+
+Arrays are torch tensors on the requested device (CPU for tests, the GPU for
+large benchmarks).  Grids use the FFTGrid real layout [nzp][nyp][rnxp].
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+RMISSING = -99999.0
+
+
+@dataclass
+class Config:
+    name: str
+    nx: int
+    ny: int
+    nz: int
+    nxp: int
+    nyp: int
+    nzp: int
+    thetas_deg: list
+    peak_hz: list
+    wavelet_scale: list
+    snr: list
+    dx: float = 25.0
+    dy: float = 25.0
+    dt: float = 4.0
+    lateral_range: float = 800.0
+    vertical_range_ms: float = 40.0
+    std: tuple = (0.08, 0.12, 0.04)
+    corr: tuple = (0.7, 0.5, 0.4)   # (ab, ar, br)
+    wnc: float = 0.1
+    low_cut: float = 5.0
+    high_cut: float = 55.0
+    ang_range_deg: float = 10.0
+    seed: int = 1
+
+
+def find_closest_factorable(n: int) -> int:
+    """Smallest 2^a 3^b 5^c 7^d >= n (FFTGrid::findClosestFactorableNumber,
+    src/fftgrid.cpp:2837-2866)."""
+    m = max(1, n)
+    while True:
+        k = m
+        for p in (2, 3, 5, 7):
+            while k % p == 0:
+                k //= p
+        if k == 1:
+            return m
+        m += 1
+
+
+def config(name: str, seed: int = 1) -> Config:
+    rng = np.random.default_rng(seed + 1000)
+    if name == "plain_cpu":      # 64x64x128 padded 80x80x160, 3 angles
+        return Config(name, 64, 64, 128, 80, 80, 160, [10.0, 20.0, 30.0], [30.0] * 3, [1.0] * 3,
+                      [5.0] * 3, seed=seed)
+    if name == "plain_gpu":      # 256^3 padded 288^3, 5 angles
+        th = list(np.linspace(5.0, 35.0, 5))
+        return Config(name, 256, 256, 256, 288, 288, 288, th, [25.0] * 5,
+                      list(rng.uniform(0.8, 1.2, 5)), list(rng.uniform(3.0, 10.0, 5)), seed=seed)
+    if name == "precision":      # 512x512x320 FFT grid (interior 448x448x256), 6 angles
+        th = list(np.linspace(4.0, 40.0, 6))
+        return Config(name, 448, 448, 256, 512, 512, 320, th, [25.0] * 6,
+                      list(rng.uniform(0.8, 1.2, 6)), [100.0, 80.0, 60.0, 40.0, 30.0, 20.0], seed=seed)
+    if name == "differentiation":  # 256x256x192, 5 angles
+        th = list(np.linspace(5.0, 35.0, 5))
+        nxp, nzp = find_closest_factorable(int(math.ceil(256 * 1.1))), find_closest_factorable(int(math.ceil(192 * 1.1)))
+        return Config(name, 256, 256, 192, nxp, nxp, nzp, th, [25.0] * 5,
+                      list(rng.uniform(0.8, 1.2, 5)), list(rng.uniform(3.0, 10.0, 5)), seed=seed)
+    if name == "tiny":           # brute-force sized
+        return Config(name, 4, 4, 8, 4, 4, 8, [10.0, 30.0], [60.0] * 2, [1.0] * 2, [4.0] * 2,
+                      lateral_range=60.0, vertical_range_ms=12.0, low_cut=0.0, high_cut=1e9, seed=seed)
+    if name == "small":          # quick parity size, odd nxp on purpose
+        return Config(name, 12, 10, 20, 15, 12, 24, [8.0, 18.0, 28.0, 38.0], [35.0] * 4, [1.0, 0.9, 1.1, 1.05],
+                      [6.0, 5.0, 4.0, 3.0], low_cut=5.0, high_cut=100.0, seed=seed)
+    raise KeyError(name)
+
+
+def random_config(rng: np.random.Generator, idx: int, seed: int = 1) -> Config:
+    """One volume of the batching workload: nx, ny in [32,192], nz in
+    [64,256], ntheta in [3,8], Ricker 20-40 Hz, ranges 300-1500 m /
+    20-60 ms, S/N 2-10; padding as ModelGeneral::setPaddingSize with 10%."""
+    nx, ny = int(rng.integers(32, 193)), int(rng.integers(32, 193))
+    nz = int(rng.integers(64, 257))
+    nt = int(rng.integers(3, 9))
+    nxp = find_closest_factorable(int(math.ceil(nx * 1.1)))
+    nyp = find_closest_factorable(int(math.ceil(ny * 1.1)))
+    nzp = find_closest_factorable(int(math.ceil(nz * 1.1)))
+    th = list(np.sort(rng.uniform(2.0, 45.0, nt)))
+    f = float(rng.uniform(20.0, 40.0))
+    return Config(f"batch{idx}", nx, ny, nz, nxp, nyp, nzp, th, [f] * nt, list(rng.uniform(0.8, 1.2, nt)),
+                  list(rng.uniform(2.0, 10.0, nt)), lateral_range=float(rng.uniform(300, 1500)),
+                  vertical_range_ms=float(rng.uniform(20, 60)), seed=seed + idx)
+
+
+@dataclass
+class Problem:
+    cfg: Config
+    sigma0: np.ndarray          # (3,3) float64
+    A: np.ndarray               # (nt,3) float32
+    err_theta_cov: np.ndarray   # (nt,nt) float64
+    wavelets: np.ndarray        # (nt,nzp) float32, FFT order, scaled
+    wnorm: np.ndarray           # (nt,) float32
+    corr_xy: np.ndarray         # (nyp,nxp) float32
+    circ_corr_t: np.ndarray     # (nzp,) float32
+    mu: list = field(default_factory=list)      # 3 real grids [nzp][nyp][rnxp] float32
+    data: list = field(default_factory=list)    # nt grids
+    cov: list = field(default_factory=list)     # 6 grids (aa, bb, rr, ab, ar, br)
+    errcorr: torch.Tensor | None = None
+    true_m: list = field(default_factory=list)
+
+    @property
+    def ntheta(self):
+        return len(self.cfg.thetas_deg)
+
+    @property
+    def lz(self) -> float:
+        return float(self.cfg.nz * self.cfg.dt)
+
+
+def sigma0_of(cfg: Config) -> np.ndarray:
+    s = np.array(cfg.std, dtype=np.float64)
+    rab, rar, rbr = cfg.corr
+    R = np.array([[1.0, rab, rar], [rab, 1.0, rbr], [rar, rbr, 1.0]])
+    return (s[:, None] * R * s[None, :]).astype(np.float64)
+
+
+def reflection_matrix(thetas_deg, vsvp: float = 0.5) -> np.ndarray:
+    A = np.zeros((len(thetas_deg), 3), dtype=np.float32)
+    v2 = vsvp * vsvp
+    for l, th in enumerate(thetas_deg):
+        a = math.radians(float(th))
+        s2 = math.sin(a) ** 2
+        t2 = math.tan(a) * math.tan(a)
+        A[l] = [np.float32((1.0 + t2) / 2.0), np.float32(-4 * v2 * s2), np.float32((1.0 - 4.0 * v2 * s2) / 2.0)]
+    return A
+
+
+def spherical(d: np.ndarray) -> np.ndarray:
+    d = d.astype(np.float32)
+    out = (1.0 + d.astype(np.float64) ** 3 / 2.0 - 1.5 * d.astype(np.float64)).astype(np.float32)
+    return np.where(d >= 1.0, np.float32(0.0), out).astype(np.float32)
+
+
+def corr_xy_grid(cfg: Config) -> np.ndarray:
+    i = np.arange(cfg.nxp)
+    j = np.arange(cfg.nyp)
+    ri = np.where(i < cfg.nxp // 2 + 1, i, i - cfg.nxp).astype(np.float32)
+    rj = np.where(j < cfg.nyp // 2 + 1, j, j - cfg.nyp).astype(np.float32)
+    dx = (ri[None, :] * np.float32(cfg.dx)) / np.float32(cfg.lateral_range)
+    dy = (rj[:, None] * np.float32(cfg.dy)) / np.float32(cfg.lateral_range)
+    d = np.sqrt(dx * dx + dy * dy).astype(np.float32)
+    return spherical(d)
+
+
+def circ_corr_t(cfg: Config) -> np.ndarray:
+    nzp = cfg.nzp
+    lags = np.arange(nzp)
+    refk = np.where(lags < nzp // 2 + 1, lags, nzp - lags)
+    corr_t = spherical((refk * cfg.dt / cfg.vertical_range_ms).astype(np.float32))
+    low_int_cut = int(math.floor(cfg.low_cut * (nzp * 0.001 * cfg.dt)))
+    F = np.fft.rfft(corr_t.astype(np.float64))
+    mag = np.abs(F)
+    mag[: low_int_cut + 1] = 0.0
+    c = np.fft.irfft(mag, n=nzp) * nzp
+    if c[0] <= 1e-5:
+        raise ValueError("circular temporal correlation undefined (increase nz)")
+    return (c * (1.0 / c[0])).astype(np.float32)
+
+
+def ricker_fft_order(nzp: int, dt: float, peak: float) -> np.ndarray:
+    k = np.arange(nzp)
+    t = np.where(k < nzp // 2 + 1, dt * k, dt * (k - nzp)).astype(np.float64)
+    c = math.pi * math.pi * peak * peak * t * t * 1e-6
+    return ((1 - 2 * c) * np.exp(-c)).astype(np.float32)
+
+
+def kwd_table(wavelets: np.ndarray, nzp: int) -> np.ndarray:
+    """kD(k) conj(W_l(k)), complex128 [nzp][nt] (generator use only)."""
+    k = np.arange(nzp)
+    kD = -1.0 + np.exp(2j * np.pi * k / nzp)
+    W = np.fft.fft(wavelets.astype(np.float64), axis=1)  # [nt][nzp]
+    return (kD[:, None] * np.conj(W.T))
+
+
+def _padded(real_zyx: torch.Tensor, rnxp: int, fill: float) -> torch.Tensor:
+    nzp, nyp, nxp = real_zyx.shape
+    out = torch.full((nzp, nyp, rnxp), fill, dtype=torch.float32, device=real_zyx.device)
+    out[:, :, :nxp] = real_zyx.to(torch.float32)
+    return out
+
+
+def make_problem(cfg: Config, device: str = "cpu", grids: bool = True) -> Problem:
+    nt = len(cfg.thetas_deg)
+    nxp, nyp, nzp = cfg.nxp, cfg.nyp, cfg.nzp
+    rnxp = 2 * (nxp // 2 + 1)
+    sig0 = sigma0_of(cfg)
+    A = reflection_matrix(cfg.thetas_deg)
+    cxy = corr_xy_grid(cfg)
+    cct = circ_corr_t(cfg)
+    wav = np.zeros((nt, nzp), dtype=np.float32)
+    wnorm = np.zeros(nt, dtype=np.float32)
+    for l in range(nt):
+        w = ricker_fft_order(nzp, cfg.dt, float(cfg.peak_hz[l]))
+        wnorm[l] = np.float32(math.sqrt(float(np.sum((w * w).astype(np.float64)))))
+        wav[l] = (w * np.float32(cfg.wavelet_scale[l])).astype(np.float32)
+    th = np.array(cfg.thetas_deg, dtype=np.float64)
+    ang = np.exp(-3.0 * np.abs(th[:, None] - th[None, :]) / cfg.ang_range_deg)
+    prob = Problem(cfg, sig0, A, np.zeros((nt, nt)), wav, wnorm, cxy, cct)
+    if not grids:
+        prob.err_theta_cov = ang * 1e-4
+        return prob
+
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(cfg.seed))
+    dev = torch.device(device)
+    f64 = torch.float64
+    base = torch.from_numpy(cct.astype(np.float64)).to(dev)[:, None, None] * \
+        torch.from_numpy(cxy.astype(np.float64)).to(dev)[None, :, :]
+    # covariance grids: (circCorrT x corrXY) in float, times Sigma0_ab in float
+    base32 = (torch.from_numpy(cct).to(dev)[:, None, None] * torch.from_numpy(cxy).to(dev)[None, :, :])
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    prob.cov = []
+    for (a, b) in pairs:
+        s = np.float32(sig0[a, b])
+        g = _padded(base32, rnxp, RMISSING)
+        prob.cov.append(g * float(s))
+    ec = torch.zeros((nzp, nyp, nxp), dtype=torch.float32, device=dev)
+    ec[0] = torch.from_numpy(cxy).to(dev)
+    prob.errcorr = _padded(ec, rnxp, RMISSING)
+
+    # background trends (constant laterally), cosine return through the z padding
+    k = torch.arange(nzp, device=dev, dtype=f64)
+    nz = cfg.nz
+    frac = torch.clamp(k / max(1, nz - 1), max=1.0)
+    back = torch.where(k < nz, frac, 0.5 * (1.0 + torch.cos(math.pi * (k - nz + 1) / (nzp - nz + 1))))
+    vp = 2500.0 + 1000.0 * back
+    rho = 2.2 + 0.3 * back
+    trends = [torch.log(vp), torch.log(0.5 * vp), torch.log(rho)]
+    mu = [t[:, None, None].expand(nzp, nyp, nxp).contiguous() for t in trends]
+
+    # Gaussian perturbation drawn from the prior: sqrt(eig) * chol(Sigma0) * white
+    lam = torch.fft.rfftn(base, dim=(0, 1, 2)).real.abs()
+    L0 = torch.from_numpy(np.linalg.cholesky(sig0)).to(dev)
+    z = [torch.randn((nzp, nyp, nxp), generator=gen, device=dev, dtype=f64) for _ in range(3)]
+    zh = [torch.fft.rfftn(zz, dim=(0, 1, 2), norm="ortho") for zz in z]
+    del z
+    sq = torch.sqrt(lam)
+    m_true = []
+    for a in range(3):
+        acc = L0[a, 0] * zh[0]
+        for b in range(1, a + 1):
+            acc = acc + L0[a, b] * zh[b]
+        dm = torch.fft.irfftn(acc * sq, s=(nzp, nyp, nxp), dim=(0, 1, 2), norm="ortho")
+        m_true.append(mu[a] + dm)
+    del zh, sq, lam
+    mh = [torch.fft.rfftn(m, dim=(0, 1, 2), norm="ortho") for m in m_true]
+    K = torch.from_numpy(kwd_table(wav, nzp)).to(dev)  # [nzp][nt]
+    data = []
+    sig_e = np.zeros(nt)
+    for l in range(nt):
+        acc = None
+        for c in range(3):
+            term = mh[c] * float(A[l, c])
+            acc = term if acc is None else acc + term
+        acc = acc * K[:, l][:, None, None]
+        d = torch.fft.irfftn(acc, s=(nzp, nyp, nxp), dim=(0, 1, 2), norm="ortho")
+        var = float(d[: cfg.nz, : cfg.ny, : cfg.nx].var())
+        s = math.sqrt(var / float(cfg.snr[l]))
+        sig_e[l] = s
+        d = d + s * torch.randn((nzp, nyp, nxp), generator=gen, device=dev, dtype=f64)
+        data.append(_padded(d, rnxp, 0.0))
+    del mh
+    prob.err_theta_cov = (sig_e[:, None] * sig_e[None, :]) * ang
+    prob.mu = [_padded(m, rnxp, 0.0) for m in mu]
+    prob.data = data
+    prob.true_m = [_padded(m, rnxp, 0.0) for m in m_true]
+    return prob
+
+
+def post_params(prob: Problem, mixed_threshold: float = 1e-3):
+    """Fill the C-ABI parameter struct of a problem."""
+    from ._lib import PostParams
+    cfg = prob.cfg
+    p = PostParams()
+    p.nxp, p.nyp, p.nzp, p.ntheta = cfg.nxp, cfg.nyp, cfg.nzp, prob.ntheta
+    p.wnc = cfg.wnc
+    for i, v in enumerate(prob.sigma0.reshape(-1)):
+        p.sigma0[i] = float(v)
+    for i, v in enumerate(prob.A.reshape(-1)):
+        p.A[i] = float(v)
+    for i, v in enumerate(prob.err_theta_cov.reshape(-1)):
+        p.err_theta_cov[i] = float(v)
+    p.mixed_threshold = mixed_threshold
+    return p

@@ -1,0 +1,124 @@
+/*
+ * crava_hip.h -- C ABI of the MI355X-native CRAVA Fourier-domain inversion core.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device memory; the
+ * stream is a hipStream_t passed as void*.  Every entry point returns
+ * CRAVA_HIP_OK (0) or a negative error code and never exits the process;
+ * crava_hip_last_error() gives the message of the calling thread's last error.
+ *
+ * Grid layout is the reference's FFTGrid layout (src/fftgrid.cpp:51-66):
+ *   real    [nzp][nyp][rnxp] float, rnxp = 2*(nxp/2+1)
+ *   complex [nzp][nyp][cnxp] {re,im}, cnxp = nxp/2+1, aliasing the same memory.
+ * FP64 grids use the same layout with double.
+ */
+#ifndef CRAVA_HIP_H
+#define CRAVA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference's int-status convention, lib_matr.c / FFTGrid) */
+#define CRAVA_HIP_OK 0
+#define CRAVA_HIP_E_INVALID (-1)
+#define CRAVA_HIP_E_OOM (-2)
+#define CRAVA_HIP_E_HIP (-3)
+#define CRAVA_HIP_E_STATE (-4)
+
+/* FFTGrid::cubetype_ (src/fftgrid.h:167) decides the transform scaling */
+#define CRAVA_HIP_DATA 0
+#define CRAVA_HIP_PARAMETER 1
+#define CRAVA_HIP_COVARIANCE 2
+
+/* arithmetic of a posterior / transform call */
+#define CRAVA_HIP_F32 0   /* float grids, float solve (the reference's arithmetic) */
+#define CRAVA_HIP_MIXED 1 /* float grids, FP64 solve below the noise/signal threshold */
+#define CRAVA_HIP_F64 2   /* double grids, double transforms and solve */
+
+typedef struct crava_hip_ctx crava_hip_ctx;
+typedef struct crava_hip_post crava_hip_post;
+
+/* Library version (major*10000 + minor*100 + patch).  No reference counterpart. */
+int crava_hip_version(void);
+
+/* Message of the calling thread's last error ("" if none). */
+const char *crava_hip_last_error(void);
+
+/* Execution context: a stream, a cache of transform plans and small device
+ * counters.  Replaces the implicit global state of FFTGrid/Crava
+ * (src/fftgrid.cpp:2869-2876).  stream = NULL uses the null stream. */
+int crava_hip_ctx_create(crava_hip_ctx **ctx, void *stream);
+int crava_hip_ctx_destroy(crava_hip_ctx *ctx);
+int crava_hip_ctx_set_stream(crava_hip_ctx *ctx, void *stream);
+int crava_hip_synchronize(crava_hip_ctx *ctx);
+
+/* In-place 3-D real-to-half-complex transform of one device grid.
+ * Replaces FFTGrid::fftInPlace (src/fftgrid.cpp:1804-1829): grids that are
+ * not CRAVA_HIP_COVARIANCE are scaled by 1/sqrt(nxp*nyp*nzp); FFTW's forward
+ * sign (-1).  precision: CRAVA_HIP_F32 (float grid) or CRAVA_HIP_F64. */
+int crava_hip_fft_r2c(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int cubetype, int precision);
+
+/* In-place 3-D half-complex-to-real transform.  Replaces
+ * FFTGrid::invFFTInPlace (src/fftgrid.cpp:1832-1862): scaled by 1/N for
+ * CRAVA_HIP_COVARIANCE, 1/sqrt(N) otherwise; Im of the self-conjugate x bins
+ * is ignored as in FFTW's halfcomplex c2r. */
+int crava_hip_fft_c2r(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int cubetype, int precision);
+
+/* One pass of a 3-D transform (axis 0 = x rows incl. the r2c/c2r split,
+ * 1 = y, 2 = z), for profiling and per-pass timing; `scale` multiplies the
+ * x-pass output.  No reference counterpart (FFTW does all passes inside
+ * rfftwnd_one_*). */
+int crava_hip_fft_pass(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int axis, int inverse,
+                       int precision, double scale);
+
+/* Per-k vectors of the constant-thickness posterior loop, host side.
+ * Replaces the per-k preparation of Crava::computePostMeanResidAndFFTCov
+ * (src/crava.cpp:938-1057 with fillkW/fillkWNorm :1337-1359,
+ * Wavelet::getCAmp src/wavelet.cpp:309-323, Wavelet1D difference operators
+ * src/wavelet1D.cpp:368-479).
+ *   wavelets [ntheta][nzp] scaled wavelets in FFT order; wnorm[ntheta] their
+ *   Wavelet::norm_.  Outputs: invert[nzp]; kWD, e1, e2, e3 [nzp][ntheta]
+ *   complex float ({re,im} pairs). */
+int crava_hip_perk_tables(int ntheta, int nz, int nzp, double lz, float low_cut, float high_cut,
+                          float min_rel_thick, const float *wavelets, const float *wnorm,
+                          unsigned char *invert, float *kWD, float *e1, float *e2, float *e3);
+
+/* Posterior problem: everything that is not a grid. */
+typedef struct {
+  int nxp, nyp, nzp, ntheta;   /* ntheta in 1..8 */
+  float wnc;                   /* white-noise component (ModelSettings, default 0.1) */
+  double sigma0[9];            /* prior 3x3 covariance Sigma0, row-major (ln Vp, ln Vs, ln rho) */
+  float A[8 * 3];              /* reflection matrix [ntheta][3] (ModelAVODynamic A_) */
+  double err_theta_cov[8 * 8]; /* Sigma_e [ntheta][ntheta] (Crava::errThetaCov_) */
+  double mixed_threshold;      /* CRAVA_HIP_MIXED: FP64 where tr(E)/tr(KSK^H) < this */
+} crava_hip_post_params;
+
+/* Upload the per-k tables of one posterior problem.  The per-k vectors are
+ * those of crava_hip_perk_tables (or the caller's own). */
+int crava_hip_post_create(crava_hip_ctx *ctx, const crava_hip_post_params *params, const unsigned char *invert,
+                          const float *kWD, const float *e1, const float *e2, const float *e3,
+                          crava_hip_post **post);
+int crava_hip_post_destroy(crava_hip_post *post);
+
+/* Per-point posterior, in place, on Fourier-domain device grids.  Replaces
+ * the loop of Crava::computePostMeanResidAndFFTCov (src/crava.cpp:1042-1160):
+ *   mu[3]      prior mean (ln Vp, ln Vs, ln rho)  -> posterior mean
+ *   data[nt]   seismic data                       -> residual d - K mu_post
+ *   cov[6]     prior covariance grids, order (aa, bb, rr, ab, ar, br),
+ *              unscaled transforms                -> posterior covariance
+ *   errcorr    error-correlation grid (read only)
+ * Out of band, or where the Cholesky factorisation fails, the prior passes
+ * through (src/crava.cpp:1098,1109).  n_fail / n_fp64 (host, may be NULL):
+ * if given, the call synchronises and reports the number of failed
+ * factorisations / points solved in FP64. */
+int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, void *const *data, void *const *cov,
+                       const void *errcorr, long long *n_fail, long long *n_fp64);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRAVA_HIP_H */

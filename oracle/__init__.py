@@ -1,0 +1,219 @@
+"""ctypes binding of the CPU checker libraries.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, and there only as the checker.  The product
+never imports this package.
+
+* liboracle.so       -- crava_oracle.c, the C restatement (always built);
+* _ref/libcrava_ref.so -- the reference's own FFTW 2.1.2 + lib_matr.c, built
+  from /root/reference sources by `make ref` (optional; absent on machines
+  without the reference sources unless copied with the tree).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libcrava_ref.so")
+
+_orc = None
+_ref = None
+
+FP = C.POINTER(C.c_float)
+DP = C.POINTER(C.c_double)
+VP = C.c_void_p
+
+
+def build(ref: bool = True) -> None:
+    """Compile liboracle.so (and _ref when the reference sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+    if ref and os.path.isdir("/root/reference/libs/fft"):
+        subprocess.run(["make", "-s", "-C", HERE, "-B", "ref"], check=True)
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            build(ref=False)
+        L = C.CDLL(ORACLE_SO)
+        I, F, D, LL, PP = C.c_int, C.c_float, C.c_double, C.c_longlong, C.POINTER(VP)
+        L.orc_fft3d_r2c.argtypes = [VP, I, I, I, I]
+        L.orc_fft3d_c2r.argtypes = [VP, I, I, I, I]
+        L.orc_fft3d_r2c_f64.argtypes = [VP, I, I, I, I]
+        L.orc_fft3d_c2r_f64.argtypes = [VP, I, I, I, I]
+        L.orc_dft1d.argtypes = [VP, I, I]
+        L.orc_perk_tables.argtypes = [I, I, I, F, D, F, F, F, VP, VP, VP, VP, VP, VP, VP]
+        L.orc_posterior.argtypes = [I, I, I, I, VP, VP, F, VP, VP, VP, VP, VP, VP, PP, PP, PP, VP]
+        L.orc_posterior_f64.argtypes = [I, I, I, I, VP, VP, D, VP, VP, VP, VP, VP, VP, PP, PP, PP, VP]
+        L.orc_simulate_points.argtypes = [LL, PP, PP]
+        L.orc_rnorm01.argtypes = [C.POINTER(C.c_uint32)]
+        L.orc_unif01.argtypes = [C.POINTER(C.c_uint32)]
+        L.orc_fill_complex_noise.argtypes = [VP, I, I, I, C.POINTER(C.c_uint32)]
+        L.orc_chol_cpx.argtypes = [I, VP]
+        L.orc_axeqb_mat_cpx.argtypes = [I, VP, VP, I]
+        L.orc_facies.argtypes = [I, I, I, I, I, I, VP, VP, VP, I, VP, I, I, I, VP, VP, VP, F, PP]
+        L.orc_posterior.restype = C.c_longlong
+        L.orc_posterior_f64.restype = C.c_longlong
+        L.orc_simulate_points.restype = C.c_longlong
+        L.orc_rnorm01.restype = C.c_double
+        L.orc_unif01.restype = C.c_double
+        L.orc_chol_cpx.restype = C.c_int
+        _orc = L
+    return _orc
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise FileNotFoundError(REF_SO)
+        L = C.CDLL(REF_SO)
+        I, F, LL, PP = C.c_int, C.c_float, C.c_longlong, C.POINTER(VP)
+        L.ref_fft3d_r2c.argtypes = [VP, I, I, I, I]
+        L.ref_fft3d_c2r.argtypes = [VP, I, I, I, I]
+        L.ref_fft1d_r2c.argtypes = [VP, I]
+        L.ref_chol_cpx.argtypes = [I, VP]
+        L.ref_axeqb_mat_cpx.argtypes = [I, VP, VP, I]
+        L.ref_posterior.argtypes = [I, I, I, I, VP, VP, F, VP, VP, VP, VP, VP, VP, PP, PP, PP, VP]
+        L.ref_simulate_points.argtypes = [LL, PP, PP]
+        L.ref_posterior.restype = C.c_longlong
+        L.ref_simulate_points.restype = C.c_longlong
+        L.ref_wall_seconds.restype = C.c_double
+        L.ref_chol_cpx.restype = C.c_int
+        _ref = L
+    return _ref
+
+
+def _ptrs(arrs):
+    a = (VP * len(arrs))()
+    for i, x in enumerate(arrs):
+        assert x.flags["C_CONTIGUOUS"]
+        a[i] = x.ctypes.data
+    return a
+
+
+# ---------------------------------------------------------------- transforms
+def fft_r2c(grid: np.ndarray, nxp, nyp, nzp, is_cov: bool, which: str = "oracle") -> np.ndarray:
+    """In-place r2c on a float32 [nzp][nyp][rnxp] array; returns complex64 view."""
+    assert grid.dtype == np.float32 and grid.flags["C_CONTIGUOUS"]
+    if which == "ref":
+        ref().ref_fft3d_r2c(grid.ctypes.data, nxp, nyp, nzp, int(is_cov))
+    else:
+        orc().orc_fft3d_r2c(grid.ctypes.data, nxp, nyp, nzp, int(is_cov))
+    return grid.view(np.complex64)
+
+
+def fft_c2r(grid: np.ndarray, nxp, nyp, nzp, is_cov: bool, which: str = "oracle") -> np.ndarray:
+    assert grid.dtype == np.float32 and grid.flags["C_CONTIGUOUS"]
+    if which == "ref":
+        ref().ref_fft3d_c2r(grid.ctypes.data, nxp, nyp, nzp, int(is_cov))
+    else:
+        orc().orc_fft3d_c2r(grid.ctypes.data, nxp, nyp, nzp, int(is_cov))
+    return grid
+
+
+def fft_r2c_f64(grid: np.ndarray, nxp, nyp, nzp, is_cov: bool) -> np.ndarray:
+    assert grid.dtype == np.float64 and grid.flags["C_CONTIGUOUS"]
+    orc().orc_fft3d_r2c_f64(grid.ctypes.data, nxp, nyp, nzp, int(is_cov))
+    return grid.view(np.complex128)
+
+
+def fft_c2r_f64(grid: np.ndarray, nxp, nyp, nzp, is_cov: bool) -> np.ndarray:
+    assert grid.dtype == np.float64 and grid.flags["C_CONTIGUOUS"]
+    orc().orc_fft3d_c2r_f64(grid.ctypes.data, nxp, nyp, nzp, int(is_cov))
+    return grid
+
+
+# ----------------------------------------------------------------- tables
+def perk_tables(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wavelets, wnorm):
+    wav = np.ascontiguousarray(wavelets, dtype=np.float32)
+    wn = np.ascontiguousarray(wnorm, dtype=np.float32)
+    inv = np.zeros(nzp, dtype=np.uint8)
+    out = [np.zeros((nzp, ntheta, 2), dtype=np.float32) for _ in range(4)]
+    orc().orc_perk_tables(ntheta, nz, nzp, 0.0, lz, low_cut,
+                          high_cut, min_rel_thick, wav.ctypes.data, wn.ctypes.data,
+                          inv.ctypes.data, *[o.ctypes.data for o in out])
+    return {"invert": inv, "kWD": out[0], "e1": out[1], "e2": out[2], "e3": out[3]}
+
+
+# -------------------------------------------------------------- posterior
+def posterior(nxp, nyp, nzp, A, etc, wnc, sigma0, tabs, mu, data, cov, errcorr, which: str = "oracle") -> int:
+    """Float per-point loop on complex64 grids (in place).  Returns #failed."""
+    nt = A.shape[0]
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    etc = np.ascontiguousarray(etc, dtype=np.float64)
+    s0 = np.ascontiguousarray(sigma0, dtype=np.float64)
+    args = (nxp, nyp, nzp, nt, A.ctypes.data, etc.ctypes.data, wnc, s0.ctypes.data,
+            tabs["invert"].ctypes.data, tabs["kWD"].ctypes.data, tabs["e1"].ctypes.data, tabs["e2"].ctypes.data,
+            tabs["e3"].ctypes.data, _ptrs(mu), _ptrs(data), _ptrs(cov), errcorr.ctypes.data)
+    if which == "ref":
+        return int(ref().ref_posterior(*args))
+    return int(orc().orc_posterior(*args))
+
+
+def posterior_f64(nxp, nyp, nzp, A, etc, wnc, sigma0, tabs, mu, data, cov, errcorr) -> int:
+    """FP64 per-point posterior on complex128 grids (in place)."""
+    nt = A.shape[0]
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    etc = np.ascontiguousarray(etc, dtype=np.float64)
+    s0 = np.ascontiguousarray(sigma0, dtype=np.float64)
+    return int(orc().orc_posterior_f64(nxp, nyp, nzp, nt, A.ctypes.data, etc.ctypes.data, wnc,
+                                       s0.ctypes.data, tabs["invert"].ctypes.data, tabs["kWD"].ctypes.data,
+                                       tabs["e1"].ctypes.data, tabs["e2"].ctypes.data, tabs["e3"].ctypes.data,
+                                       _ptrs(mu), _ptrs(data), _ptrs(cov), errcorr.ctypes.data))
+
+
+def chol_cpx(mat: np.ndarray, which: str = "oracle") -> int:
+    """lib_matrCholCpx on a complex64 (n,n) array, in place."""
+    assert mat.dtype == np.complex64 and mat.flags["C_CONTIGUOUS"]
+    n = mat.shape[0]
+    if which == "ref":
+        return int(ref().ref_chol_cpx(n, mat.ctypes.data))
+    return int(orc().orc_chol_cpx(n, mat.ctypes.data))
+
+
+# ------------------------------------------------------------- simulation
+def rnorm01_stream(seed: int, n: int) -> tuple[np.ndarray, int]:
+    s = C.c_uint32(seed)
+    out = np.array([orc().orc_rnorm01(C.byref(s)) for _ in range(n)])
+    return out, s.value
+
+
+def fill_complex_noise(nxp, nyp, nzp, seed: int) -> tuple[np.ndarray, int]:
+    cn = nxp // 2 + 1
+    g = np.zeros((nzp, nyp, 2 * cn), dtype=np.float32)
+    s = C.c_uint32(seed)
+    orc().orc_fill_complex_noise(g.ctypes.data, nxp, nyp, nzp, C.byref(s))
+    return g, s.value
+
+
+def simulate_points(cov, seed, which: str = "oracle") -> int:
+    npts = cov[0].size
+    if which == "ref":
+        return int(ref().ref_simulate_points(npts, _ptrs(cov), _ptrs(seed)))
+    return int(orc().orc_simulate_points(npts, _ptrs(cov), _ptrs(seed)))
+
+
+# ----------------------------------------------------------------- facies
+def facies(nx, ny, nz, nxp, nyp, nzp, alpha, beta, rho, tables, tn, vol, vn, prior, p_undef):
+    nfac = len(prior)
+    sm = 2 * (nx // 2 + 1)
+    out = [np.zeros((nz, ny, sm), dtype=np.float32) for _ in range(nfac + 1)]
+    tab = np.ascontiguousarray(tables, dtype=np.float32)
+    v = np.ascontiguousarray(vol, dtype=np.float64)
+    n = np.ascontiguousarray(vn, dtype=np.int32)
+    pr = np.ascontiguousarray(prior, dtype=np.float32)
+    orc().orc_facies(nx, ny, nz, nxp, nyp, nzp, alpha.ctypes.data, beta.ctypes.data, rho.ctypes.data, nfac,
+                     tab.ctypes.data, tn[0], tn[1], tn[2], v.ctypes.data, n.ctypes.data, pr.ctypes.data,
+                     p_undef, _ptrs(out))
+    return out

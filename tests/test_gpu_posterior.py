@@ -1,0 +1,79 @@
+"""GPU per-point posterior against the float oracle and the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from crava_amd import _lib as L
+from crava_amd import synth
+from crava_amd.core import Posterior, problem_tables
+from tests.helpers import as_f64, fourier_inputs, rel_l2
+
+
+def _run_gpu(ctx, prob, grids, precision, thr=1e-3):
+    mu, d, cv, ec = grids
+    want = np.float64 if precision == L.F64 else np.float32
+    def up(g):
+        assert g.dtype == want
+        return torch.from_numpy(g.copy()).cuda()
+    tm = [up(g) for g in mu]
+    td = [up(g) for g in d]
+    tc = [up(g) for g in cv]
+    te = up(ec)
+    post = Posterior(ctx, synth.post_params(prob, thr), problem_tables(prob))
+    nf, n64 = post.run(precision, tm, td, tc, te, counts=True)
+    torch.cuda.synchronize()
+    post.close()
+    return [t.cpu().numpy() for t in tm], [t.cpu().numpy() for t in td], [t.cpu().numpy() for t in tc], nf, n64
+
+
+def _cplx(g):
+    return g.view(np.complex64 if g.dtype == np.float32 else np.complex128)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["small", "plain_cpu"])
+def test_posterior_f32_vs_oracle(ctx, name):
+    print(f"posterior f32 {name}", flush=True)
+    prob = synth.make_problem(synth.config(name), "cpu")
+    cfg = prob.cfg
+    grids = fourier_inputs(prob)
+    tabs = problem_tables(prob)
+    mu, d, cv, ec = [x if not isinstance(x, list) else [g.copy() for g in x] for x in grids]
+    ec = ec.copy()
+    nf_ref = oracle.posterior(cfg.nxp, cfg.nyp, cfg.nzp, prob.A, prob.err_theta_cov, cfg.wnc, prob.sigma0,
+                              tabs, mu, d, cv, ec)
+    gm, gd, gc, nf, _ = _run_gpu(ctx, prob, grids, L.F32)
+    assert nf == nf_ref
+    for a, b in zip(gm, mu):
+        assert rel_l2(_cplx(a), _cplx(b)) < 1e-6
+    for a, b in zip(gd, d):
+        assert rel_l2(_cplx(a), _cplx(b)) < 2e-5
+    for a, b in zip(gc, cv):
+        assert rel_l2(_cplx(a), _cplx(b)) < 2e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["small", "plain_cpu"])
+def test_posterior_f64_and_mixed_vs_fp64_oracle(ctx, name):
+    print(f"posterior f64/mixed {name}", flush=True)
+    prob = synth.make_problem(synth.config(name), "cpu")
+    cfg = prob.cfg
+    grids = fourier_inputs(prob)
+    tabs = problem_tables(prob)
+    m64, d64, c64 = as_f64(grids[0]), as_f64(grids[1]), as_f64(grids[2])
+    e64 = as_f64([grids[3]])[0]
+    g64 = (m64, d64, c64, e64)
+    om, od, oc = [g.copy() for g in m64], [g.copy() for g in d64], [g.copy() for g in c64]
+    nf_ref = oracle.posterior_f64(cfg.nxp, cfg.nyp, cfg.nzp, prob.A, prob.err_theta_cov, cfg.wnc, prob.sigma0,
+                                  tabs, om, od, oc, e64.copy())
+    gm, gd, gc, nf, _ = _run_gpu(ctx, prob, g64, L.F64)
+    assert nf == nf_ref
+    for a, b in zip(gm + gd + gc, om + od + oc):
+        assert rel_l2(_cplx(a), _cplx(b)) < 1e-12
+    # mixed: float grids, every point in FP64 (threshold huge) -> FP64 accuracy up to float rounding
+    gm, gd, gc, nf, n64 = _run_gpu(ctx, prob, grids, L.MIXED, thr=1e30)
+    assert nf == nf_ref
+    assert n64 > 0
+    for a, b in zip(gc, oc):
+        assert rel_l2(_cplx(a), _cplx(b)) < 1e-6
