@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=5)
+    ap.add_argument("--sweep", action="store_true", help="report accuracy over a range of FP64 thresholds")
     return ap.parse_args()
 
 
@@ -65,6 +66,7 @@ class Pipeline:
         self.ec = self.ec0.clone()
         self.post = Posterior(ctx, params, tables)
         self.precision = L.F64 if fp64 else L.MIXED
+        self.cov_precision = L.F64 if fp64 else L.MIXED
         self.L = L
 
     def step(self, counts=False):
@@ -75,9 +77,11 @@ class Pipeline:
             self.ctx.r2c(g, *n, L.DATA)
         for g in self.mu:
             self.ctx.r2c(g, *n, L.PARAMETER)
+        # covariance spectra span many decades: the mixed form transforms them
+        # with double arithmetic (float storage) so small eigenvalues stay exact
         for g in self.cov:
-            self.ctx.r2c(g, *n, L.COVARIANCE)
-        self.ctx.r2c(self.ec, *n, L.COVARIANCE)
+            self.ctx.r2c(g, *n, L.COVARIANCE, self.cov_precision)
+        self.ctx.r2c(self.ec, *n, L.COVARIANCE, self.cov_precision)
         r = self.post.run(self.precision, self.mu, self.data, self.cov, self.ec, counts=counts)
         for g in self.mu:
             self.ctx.c2r(g, *n, L.PARAMETER)
@@ -211,20 +215,50 @@ def main():
     nt = prob.ntheta
 
     mixed = Pipeline(ctx, prob, params, tabs, fp64=False)
-    res = {}
+    # autotune the transform launch shapes (float and double) on a working grid
+    from crava_amd.core import autotune_fft
+    tune = {"f32": autotune_fft(ctx, mixed.data[0], nxp, nyp, nzp),
+            "f32_store_f64_math": autotune_fft(ctx, mixed.data[0], nxp, nyp, nzp, L.MIXED)}
+    log(f"autotuned variants {tune}")
+    mixed.reset()
+    f64 = None
+    if not args.no_fp64:
+        f64 = Pipeline(ctx, prob, params, tabs, fp64=True)
+        tune["f64"] = autotune_fft(ctx, f64.data[0], nxp, nyp, nzp)
+        log(f"autotuned f64 variants {tune['f64']}")
+        f64.reset()
+
+    def errors(m, ref):
+        return {"mean_update": max(rel_err(a, b, nxp, p) for a, b, p in zip(m.mu, ref.mu, prob.mu)),
+                "mean": max(rel_err(a, b, nxp) for a, b in zip(m.mu, ref.mu)),
+                "cov": max(rel_err(a, b, nxp) for a, b in zip(m.cov, ref.cov)),
+                "residual": max(rel_err(a, b, nxp) for a, b in zip(m.data, ref.data))}
+
     # accuracy: one step of each form from the same inputs
+    acc, sweep = {}, []
+    if f64 is not None:
+        f64.step(counts=True)
+        torch.cuda.synchronize()
+        if args.sweep:
+            from crava_amd.core import Posterior
+            for thr in (1e-3, 1e-2, 3e-2, 1e-1, 3e-1, 1.0, 1e30):
+                mixed.post.close()
+                mixed.post = Posterior(ctx, synth.post_params(prob, thr), tabs)
+                mixed.reset()
+                nf_s, n64_s = mixed.step(counts=True)
+                torch.cuda.synchronize()
+                e = errors(mixed, f64)
+                t_k = None
+                sweep.append({"threshold": thr, "fp64_points": n64_s, "errors": e})
+                log(f"threshold {thr:g}: {n64_s} FP64 points, errors {e}")
+            mixed.post.close()
+            mixed.post = Posterior(ctx, params, tabs)
+        mixed.reset()
     nfail, n64 = mixed.step(counts=True)
     torch.cuda.synchronize()
     log(f"mixed step: {nfail} failed factorisations, {n64} of {n_in} in-band points in FP64")
-    acc = {}
-    if not args.no_fp64:
-        f64 = Pipeline(ctx, prob, params, tabs, fp64=True)
-        nfail64, _ = f64.step(counts=True)
-        torch.cuda.synchronize()
-        acc["mean_update"] = max(rel_err(a, b, nxp, p) for a, b, p in zip(mixed.mu, f64.mu, prob.mu))
-        acc["mean"] = max(rel_err(a, b, nxp) for a, b in zip(mixed.mu, f64.mu))
-        acc["cov"] = max(rel_err(a, b, nxp) for a, b in zip(mixed.cov, f64.cov))
-        acc["residual"] = max(rel_err(a, b, nxp) for a, b in zip(mixed.data, f64.data))
+    if f64 is not None:
+        acc = errors(mixed, f64)
         log(f"rel err mixed vs FP64: {acc}")
         f64.reset()
     mixed.reset()
@@ -242,7 +276,7 @@ def main():
     ms_mixed = run(mixed, args.steps, args.warmup) * 1e3
     log(f"mixed: {ms_mixed:.3f} ms/step, {P/ms_mixed*1e3:.4g} pts/s")
     ms_f64 = None
-    if not args.no_fp64:
+    if f64 is not None:
         ms_f64 = run(f64, max(1, args.steps), 1) * 1e3
         log(f"fp64: {ms_f64:.3f} ms/step, {P/ms_f64*1e3:.4g} pts/s")
 
@@ -265,7 +299,7 @@ def main():
     t_post32 = timed_kernel(lambda: mixed.post.run(L.F32, mixed.mu, mixed.data, mixed.cov, mixed.ec),
                             args.kernel_reps)
     t_post64 = None
-    if not args.no_fp64:
+    if f64 is not None:
         for gg in f64.data + f64.mu:
             ctx.r2c(gg, *n3, L.DATA)
         for gg in f64.cov:
@@ -319,6 +353,9 @@ def main():
         "fp64_ms_per_step": ms_f64,
         "speedup_vs_fp64": (ms_f64 / ms_mixed) if ms_f64 else None,
         "rel_err_vs_fp64": acc,
+        "threshold": args.threshold,
+        "threshold_sweep": sweep or None,
+        "fft_variants": tune,
         "fp64_points": n64,
         "posterior_kernel_pts_per_s": {"mixed": P / (t_post * 1e-3), "f32": P / (t_post32 * 1e-3),
                                        "f64": (P / (t_post64 * 1e-3)) if t_post64 else None},

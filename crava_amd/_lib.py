@@ -59,6 +59,7 @@ SIGNATURES = {
     "crava_hip_post_run": (I, [VP, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP,
                                C.POINTER(LL), C.POINTER(LL)]),
     "crava_hip_simulate": (I, [VP, I, I, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(LL)]),
+    "crava_hip_simulate_points": (I, [VP, LL, C.POINTER(VP), C.POINTER(VP), C.POINTER(LL)]),
     "crava_hip_fill_complex_noise": (I, [VP, VP, I, I, I, C.c_uint64, C.c_uint64]),
     "crava_hip_facies": (I, [VP, I, I, I, I, I, I, VP, VP, VP, I, VP, I, I, I, VP, VP, VP, C.c_float,
                              C.POINTER(VP), VP, VP]),
@@ -67,7 +68,8 @@ SIGNATURES = {
     "crava_hip_batch_create": (I, [VP, I, VP, C.POINTER(VP)]),
     "crava_hip_batch_run": (I, [VP, I]),
     "crava_hip_batch_destroy": (I, [VP]),
-    "crava_hip_autotune_fft": (I, [VP, I, I, I, I, VP]),
+    "crava_hip_autotune_fft": (I, [VP, VP, I, I, I, I, C.POINTER(I), C.POINTER(C.c_float)]),
+    "crava_hip_fft_set_variants": (I, [VP, I, I, I, I, C.POINTER(I)]),
 }
 
 

@@ -23,6 +23,7 @@ ARCH = os.environ.get("CRAVA_HIP_ARCH", "gfx950")
 # roundings bit for bit, so contraction into FMA is off there.
 SOURCES = {
     "capi.hip": [],
+    "fft_static.hip": [],
     "fft.hip": [],
     "posterior.hip": [],
     "tables.cpp": [],

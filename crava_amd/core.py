@@ -58,17 +58,18 @@ class Context:
     def synchronize(self):
         check(lib().crava_hip_synchronize(self.h), "crava_hip_synchronize")
 
-    def r2c(self, grid: torch.Tensor, nxp: int, nyp: int, nzp: int, cubetype: int):
-        check(lib().crava_hip_fft_r2c(self.h, _dev_ptr(grid), nxp, nyp, nzp, cubetype, precision_of(grid)),
-              "crava_hip_fft_r2c")
+    def r2c(self, grid: torch.Tensor, nxp: int, nyp: int, nzp: int, cubetype: int, precision: int | None = None):
+        """precision: None -> from the dtype; L.MIXED -> float grid, double arithmetic."""
+        p = precision_of(grid) if precision is None else precision
+        check(lib().crava_hip_fft_r2c(self.h, _dev_ptr(grid), nxp, nyp, nzp, cubetype, p), "crava_hip_fft_r2c")
 
     def fft_pass(self, grid: torch.Tensor, nxp, nyp, nzp, axis: int, inverse: bool, scale: float = 1.0):
         check(lib().crava_hip_fft_pass(self.h, _dev_ptr(grid), nxp, nyp, nzp, axis, int(inverse),
                                        precision_of(grid), scale), "crava_hip_fft_pass")
 
-    def c2r(self, grid: torch.Tensor, nxp: int, nyp: int, nzp: int, cubetype: int):
-        check(lib().crava_hip_fft_c2r(self.h, _dev_ptr(grid), nxp, nyp, nzp, cubetype, precision_of(grid)),
-              "crava_hip_fft_c2r")
+    def c2r(self, grid: torch.Tensor, nxp: int, nyp: int, nzp: int, cubetype: int, precision: int | None = None):
+        p = precision_of(grid) if precision is None else precision
+        check(lib().crava_hip_fft_c2r(self.h, _dev_ptr(grid), nxp, nyp, nzp, cubetype, p), "crava_hip_fft_c2r")
 
 
 def perk_tables(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wavelets: np.ndarray, wnorm: np.ndarray):
@@ -122,3 +123,18 @@ class Posterior:
                                        C.byref(nf) if counts else None, C.byref(n64) if counts else None),
               "crava_hip_post_run")
         return (nf.value, n64.value) if counts else None
+
+
+def autotune_fft(ctx: Context, scratch: torch.Tensor, nxp: int, nyp: int, nzp: int, precision: int | None = None):
+    """Run the device autotuner for one transform size; returns (variants, ms)."""
+    v = (C.c_int * 3)()
+    ms = (C.c_float * 3)()
+    p = precision_of(scratch) if precision is None else precision
+    check(lib().crava_hip_autotune_fft(ctx.h, _dev_ptr(scratch), nxp, nyp, nzp, p, v, ms),
+          "crava_hip_autotune_fft")
+    return list(v), list(ms)
+
+
+def set_fft_variants(ctx: Context, nxp: int, nyp: int, nzp: int, precision: int, variants):
+    v = (C.c_int * 3)(*variants)
+    check(lib().crava_hip_fft_set_variants(ctx.h, nxp, nyp, nzp, precision, v), "crava_hip_fft_set_variants")

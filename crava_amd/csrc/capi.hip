@@ -3,7 +3,9 @@
 #include "common.hpp"
 #include "context.hpp"
 #include "fft.hpp"
+#include "facies.hpp"
 #include "posterior.hpp"
+#include "simulate.hpp"
 #include "tables.hpp"
 
 #include <cmath>
@@ -20,11 +22,11 @@ static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 const char *get_error() { return g_err.c_str(); }
 
-int Context::plan(int nxp, int nyp, int nzp, bool dbl, Fft3D **out) {
-  auto key = std::make_tuple(nxp, nyp, nzp, dbl);
+int Context::plan(int nxp, int nyp, int nzp, int prec, Fft3D **out) {
+  auto key = std::make_tuple(nxp, nyp, nzp, prec);
   auto it = plans.find(key);
   if (it == plans.end()) {
-    std::unique_ptr<Fft3D> f(new (std::nothrow) Fft3D(nxp, nyp, nzp, dbl));
+    std::unique_ptr<Fft3D> f(new (std::nothrow) Fft3D(nxp, nyp, nzp, prec));
     if (!f) return CRAVA_HIP_E_OOM;
     int rc = f->init();
     if (rc) return rc;
@@ -46,6 +48,11 @@ struct crava_hip_post {
 };
 
 using crava::set_error;
+
+// transform arithmetic/storage code of a C-ABI precision
+static int fft_prec(int precision) {
+  return precision == CRAVA_HIP_F64 ? 2 : (precision == CRAVA_HIP_MIXED ? 1 : 0);
+}
 
 extern "C" {
 
@@ -93,12 +100,12 @@ static int fft_common(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp,
   if (!ctx || !grid) { set_error("NULL ctx or grid"); return CRAVA_HIP_E_INVALID; }
   if (nxp < 2 || nyp < 1 || nzp < 1) { set_error("invalid grid size"); return CRAVA_HIP_E_INVALID; }
   if (cubetype < CRAVA_HIP_DATA || cubetype > CRAVA_HIP_COVARIANCE) { set_error("invalid cubetype"); return CRAVA_HIP_E_INVALID; }
-  if (precision != CRAVA_HIP_F32 && precision != CRAVA_HIP_F64) {
-    set_error("transform precision must be CRAVA_HIP_F32 or CRAVA_HIP_F64");
+  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) {
+    set_error("invalid transform precision");
     return CRAVA_HIP_E_INVALID;
   }
   crava::Fft3D *f = nullptr;
-  int rc = ctx->plan(nxp, nyp, nzp, precision == CRAVA_HIP_F64, &f);
+  int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
   if (rc) return rc;
   const long long N = (long long)nxp * nyp * nzp;
   double scale;
@@ -126,13 +133,40 @@ int crava_hip_fft_c2r(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp,
 int crava_hip_fft_pass(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int axis, int inverse,
                        int precision, double scale) {
   if (!ctx || !grid || axis < 0 || axis > 2) { set_error("invalid arguments to crava_hip_fft_pass"); return CRAVA_HIP_E_INVALID; }
-  if (precision != CRAVA_HIP_F32 && precision != CRAVA_HIP_F64) { set_error("invalid precision"); return CRAVA_HIP_E_INVALID; }
+  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) { set_error("invalid precision"); return CRAVA_HIP_E_INVALID; }
   crava::Fft3D *f = nullptr;
-  int rc = ctx->plan(nxp, nyp, nzp, precision == CRAVA_HIP_F64, &f);
+  int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
   if (rc) return rc;
   if (axis == 0) return f->pass_x(grid, ctx->stream, inverse != 0, scale);
   if (axis == 1) return f->pass_y(grid, ctx->stream, inverse != 0);
   return f->pass_z(grid, ctx->stream, inverse != 0);
+}
+
+int crava_hip_autotune_fft(crava_hip_ctx *ctx, void *scratch, int nxp, int nyp, int nzp, int precision,
+                           int *variants, float *ms) {
+  if (!ctx || !scratch) { set_error("NULL argument to crava_hip_autotune_fft"); return CRAVA_HIP_E_INVALID; }
+  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) { set_error("invalid precision"); return CRAVA_HIP_E_INVALID; }
+  crava::Fft3D *f = nullptr;
+  int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
+  if (rc) return rc;
+  float t[3] = {0, 0, 0};
+  if ((rc = f->autotune(scratch, ctx->stream, 5, t))) return rc;
+  for (int a = 0; a < 3; ++a) {
+    if (variants) variants[a] = f->var[a];
+    if (ms) ms[a] = t[a];
+  }
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_fft_set_variants(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, int precision, const int *variants) {
+  if (!ctx || !variants) { set_error("NULL argument to crava_hip_fft_set_variants"); return CRAVA_HIP_E_INVALID; }
+  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) { set_error("invalid precision"); return CRAVA_HIP_E_INVALID; }
+  crava::Fft3D *f = nullptr;
+  int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
+  if (rc) return rc;
+  for (int a = 0; a < 3; ++a)
+    if ((rc = f->set_variant(a, variants[a]))) return rc;
+  return CRAVA_HIP_OK;
 }
 
 int crava_hip_perk_tables(int ntheta, int nz, int nzp, double lz, float low_cut, float high_cut,
@@ -217,6 +251,74 @@ int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, voi
     if (n_fp64) *n_fp64 = (long long)h[1];
   }
   return CRAVA_HIP_OK;
+}
+
+int crava_hip_fill_complex_noise(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, uint64_t seed,
+                                 uint64_t stream) {
+  if (!ctx || !grid || nxp < 2 || nyp < 1 || nzp < 1) { set_error("invalid arguments to crava_hip_fill_complex_noise"); return CRAVA_HIP_E_INVALID; }
+  return crava::launch_fill_noise(grid, nxp, nyp, nzp, seed, stream, ctx->stream);
+}
+
+static int sync_counter(crava_hip_ctx *ctx, long long *out) {
+  unsigned long long h = 0;
+  CRAVA_HIP_CHECK(hipMemcpyAsync(&h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  *out = (long long)h;
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_simulate_points(crava_hip_ctx *ctx, long long npts, void *const *cov, void *const *seed,
+                              long long *n_fail) {
+  if (!ctx || !cov || !seed || npts < 0) { set_error("invalid arguments to crava_hip_simulate_points"); return CRAVA_HIP_E_INVALID; }
+  for (int q = 0; q < 6; ++q) if (!cov[q]) { set_error("NULL covariance grid"); return CRAVA_HIP_E_INVALID; }
+  for (int c = 0; c < 3; ++c) if (!seed[c]) { set_error("NULL seed grid"); return CRAVA_HIP_E_INVALID; }
+  if (npts == 0) { if (n_fail) *n_fail = 0; return CRAVA_HIP_OK; }
+  if (n_fail) CRAVA_HIP_CHECK(hipMemsetAsync(ctx->counters, 0, sizeof(unsigned long long), ctx->stream));
+  int rc = crava::launch_simulate_points(npts, cov, seed, n_fail ? ctx->counters : nullptr, ctx->stream);
+  if (rc) return rc;
+  return n_fail ? sync_counter(ctx, n_fail) : CRAVA_HIP_OK;
+}
+
+int crava_hip_simulate(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, void *const *cov, void *const *seed,
+                       void *const *mean, long long *n_fail) {
+  if (!ctx || !cov || !seed || nxp < 2 || nyp < 1 || nzp < 1) { set_error("invalid arguments to crava_hip_simulate"); return CRAVA_HIP_E_INVALID; }
+  const long long npts = (long long)(nxp / 2 + 1) * nyp * nzp;
+  long long nf = 0;
+  int rc = crava_hip_simulate_points(ctx, npts, cov, seed, n_fail ? &nf : nullptr);
+  if (rc) return rc;
+  for (int c = 0; c < 3; ++c) {
+    if ((rc = crava_hip_fft_c2r(ctx, seed[c], nxp, nyp, nzp, CRAVA_HIP_PARAMETER, CRAVA_HIP_F32))) return rc;
+    if (mean && mean[c]) {
+      const long long rsize = (long long)2 * (nxp / 2 + 1) * nyp * nzp;
+      if ((rc = crava::launch_add_real(reinterpret_cast<float *>(seed[c]), reinterpret_cast<const float *>(mean[c]),
+                                       rsize, ctx->stream)))
+        return rc;
+    }
+  }
+  if (n_fail) *n_fail = nf;
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_facies(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, const float *alpha,
+                     const float *beta, const float *rho, int nfac, const float *tables, int tn1, int tn2, int tn3,
+                     const double *volume, const int *vn, const float *prior, float p_undefined, float *const *prob,
+                     int *argmax, unsigned long long *counts) {
+  if (!ctx || !alpha || !beta || !rho || !tables || !volume || !vn || !prior || !prob) {
+    set_error("NULL argument to crava_hip_facies");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (nx < 1 || ny < 1 || nz < 1 || nx > nxp || ny > nyp || nz > nzp || tn1 < 1 || tn2 < 1 || tn3 < 1 ||
+      vn[0] < 1 || vn[1] < 1 || vn[2] < 1 || vn[0] > tn1 || vn[1] > tn2 || vn[2] > tn3) {
+    set_error("invalid sizes in crava_hip_facies");
+    return CRAVA_HIP_E_INVALID;
+  }
+  for (int l = 0; l <= nfac && l <= crava::kMaxFacies; ++l)
+    if (!prob[l]) { set_error("NULL probability grid"); return CRAVA_HIP_E_INVALID; }
+  crava::FaciesCall c{nx, ny, nz, nxp, nyp, nzp, alpha, beta, rho, nfac, tables, tn1, tn2, tn3,
+                      volume, vn, prior, p_undefined, prob, argmax, counts};
+  if (counts && nfac >= 1 && nfac <= crava::kMaxFacies)
+    CRAVA_HIP_CHECK(hipMemsetAsync(counts, 0, sizeof(unsigned long long) * nfac, ctx->stream));
+  return crava::launch_facies(c, ctx->stream);
 }
 
 }  // extern "C"

@@ -11,8 +11,9 @@ namespace crava {
 struct Context {
   hipStream_t stream = nullptr;
   unsigned long long *counters = nullptr;  // small device counters
-  std::map<std::tuple<int, int, int, bool>, std::unique_ptr<Fft3D>> plans;
-  int plan(int nxp, int nyp, int nzp, bool dbl, Fft3D **out);
+  std::map<std::tuple<int, int, int, int>, std::unique_ptr<Fft3D>> plans;
+  // prec: 0 float, 1 float storage / double arithmetic, 2 double
+  int plan(int nxp, int nyp, int nzp, int prec, Fft3D **out);
   ~Context();
 };
 }  // namespace crava

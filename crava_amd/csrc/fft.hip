@@ -9,109 +9,13 @@
 // whole contiguous rows for the x pass.
 #include "fft.hpp"
 #include "../../include/crava_hip.h"
+#include "fft_dft.hpp"
+#include "fft_static.hpp"
 
 #include <cmath>
 #include <vector>
 
 namespace crava {
-
-// ---------------------------------------------------------------------------
-// Small DFTs.  DIR = -1 forward (FFTW_FORWARD), +1 backward.
-// ---------------------------------------------------------------------------
-template <int DIR, typename T>
-__device__ __forceinline__ cx<T> rot_i(cx<T> a) {  // multiply by DIR * i
-  return DIR < 0 ? mk(a.y, -a.x) : mk(-a.y, a.x);
-}
-
-template <int R, int DIR, typename T>
-struct Dft;
-
-template <int DIR, typename T>
-struct Dft<2, DIR, T> {
-  static __device__ __forceinline__ void run(cx<T> *v) {
-    cx<T> a = v[0], b = v[1];
-    v[0] = add(a, b);
-    v[1] = sub(a, b);
-  }
-};
-
-template <int DIR, typename T>
-struct Dft<4, DIR, T> {
-  static __device__ __forceinline__ void run(cx<T> *v) {
-    cx<T> t0 = add(v[0], v[2]), t1 = sub(v[0], v[2]);
-    cx<T> t2 = add(v[1], v[3]), t3 = rot_i<DIR>(sub(v[1], v[3]));
-    v[0] = add(t0, t2);
-    v[2] = sub(t0, t2);
-    v[1] = add(t1, t3);
-    v[3] = sub(t1, t3);
-  }
-};
-
-template <int DIR, typename T>
-struct Dft<8, DIR, T> {
-  static __device__ __forceinline__ void run(cx<T> *v) {
-    const T h = (T)0.70710678118654752440084436210484903928;
-    cx<T> e[4] = {v[0], v[2], v[4], v[6]};
-    cx<T> o[4] = {v[1], v[3], v[5], v[7]};
-    Dft<4, DIR, T>::run(e);
-    Dft<4, DIR, T>::run(o);
-    // W8^1 = (1 + DIR i)/sqrt2, W8^2 = DIR i, W8^3 = (-1 + DIR i)/sqrt2
-    cx<T> o1 = DIR < 0 ? mk(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x))
-                       : mk(h * (o[1].x - o[1].y), h * (o[1].y + o[1].x));
-    cx<T> o2 = rot_i<DIR>(o[2]);
-    cx<T> o3 = DIR < 0 ? mk(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y))
-                       : mk(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y));
-    v[0] = add(e[0], o[0]);
-    v[4] = sub(e[0], o[0]);
-    v[1] = add(e[1], o1);
-    v[5] = sub(e[1], o1);
-    v[2] = add(e[2], o2);
-    v[6] = sub(e[2], o2);
-    v[3] = add(e[3], o3);
-    v[7] = sub(e[3], o3);
-  }
-};
-
-// Odd prime radix by symmetric pairs: a_r = v_r + v_{R-r}, b_r = v_r - v_{R-r}.
-template <int R, int DIR, typename T>
-struct DftOdd {
-  static __device__ __forceinline__ void run(cx<T> *v) {
-    constexpr int H = (R - 1) / 2;
-    cx<T> a[H], b[H];
-#pragma unroll
-    for (int r = 1; r <= H; ++r) {
-      a[r - 1] = add(v[r], v[R - r]);
-      b[r - 1] = sub(v[r], v[R - r]);
-    }
-    cx<T> x0 = v[0];
-    cx<T> s0 = x0;
-#pragma unroll
-    for (int r = 0; r < H; ++r) s0 = add(s0, a[r]);
-#pragma unroll
-    for (int m = 1; m <= H; ++m) {
-      cx<T> re = x0, im = mk((T)0, (T)0);
-#pragma unroll
-      for (int r = 1; r <= H; ++r) {
-        const int q = (r * m) % R;
-        const T c = (T)cos(2.0 * M_PI * q / R);
-        const T s = (T)sin(2.0 * M_PI * q / R);
-        re = add(re, scl(a[r - 1], c));
-        im = add(im, scl(b[r - 1], s));
-      }
-      // X_m = re + DIR*i*im, X_{R-m} = re - DIR*i*im
-      cx<T> t = rot_i<DIR>(im);
-      v[m] = add(re, t);
-      v[R - m] = sub(re, t);
-    }
-    v[0] = s0;
-  }
-};
-template <int DIR, typename T>
-struct Dft<3, DIR, T> : DftOdd<3, DIR, T> {};
-template <int DIR, typename T>
-struct Dft<5, DIR, T> : DftOdd<5, DIR, T> {};
-template <int DIR, typename T>
-struct Dft<7, DIR, T> : DftOdd<7, DIR, T> {};
 
 // One radix-R Stockham pass over the tx pencils held in LDS.
 template <int R, int DIR, typename T>
@@ -169,8 +73,8 @@ __device__ void run_passes(cx<T> *buf, const cx<T> *tw, const PassPlan &pl) {
 // Strided complex pass (y or z).  Columns are contiguous; elements `es`
 // apart; blockIdx.y selects a batch `bstride` apart.
 // ---------------------------------------------------------------------------
-template <int DIR, typename T>
-__global__ void __launch_bounds__(1024) k_fft_strided(cx<T> *__restrict__ data, long long es, long long ncols,
+template <int DIR, typename T, typename S>
+__global__ void __launch_bounds__(1024) k_fft_strided(cx<S> *__restrict__ data, long long es, long long ncols,
                                                       long long bstride, PassPlan pl,
                                                       const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
@@ -179,20 +83,26 @@ __global__ void __launch_bounds__(1024) k_fft_strided(cx<T> *__restrict__ data, 
   const int nthr = blockDim.x;
   const long long col0 = (long long)blockIdx.x * pl.tx;
   const int ncol = (int)min((long long)pl.tx, ncols - col0);
-  cx<T> *base = data + (long long)blockIdx.y * bstride + col0;
+  cx<S> *base = data + (long long)blockIdx.y * bstride + col0;
   for (int q = threadIdx.x; q < pl.n; q += nthr) tw[q] = twg[q];
   const int tot = pl.tx * pl.n;
   for (int q = threadIdx.x; q < tot; q += nthr) {
     const int e = q / pl.tx;
     const int c = q - e * pl.tx;
-    if (c < ncol) buf[c * pl.ldsp + e] = base[c + e * es];
+    if (c < ncol) {
+      const cx<S> v = base[c + e * es];
+      buf[c * pl.ldsp + e] = mk((T)v.x, (T)v.y);
+    }
   }
   __syncthreads();
   run_passes<DIR>(buf, tw, pl);
   for (int q = threadIdx.x; q < tot; q += nthr) {
     const int e = q / pl.tx;
     const int c = q - e * pl.tx;
-    if (c < ncol) base[c + e * es] = buf[c * pl.ldsp + e];
+    if (c < ncol) {
+      const cx<T> v = buf[c * pl.ldsp + e];
+      base[c + e * es] = mk((S)v.x, (S)v.y);
+    }
   }
 }
 
@@ -202,8 +112,8 @@ __global__ void __launch_bounds__(1024) k_fft_strided(cx<T> *__restrict__ data, 
 // 2c+1 (imaginary part); the two spectra are separated afterwards with
 // A_k = (Z_k + conj Z_{n-k})/2, B_k = (Z_k - conj Z_{n-k})/(2i).
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(1024) k_fft_rows_r2c(T *__restrict__ data, long long nrows, int rnxp,
+template <typename T, typename S>
+__global__ void __launch_bounds__(1024) k_fft_rows_r2c(S *__restrict__ data, long long nrows, int rnxp,
                                                        PassPlan pl, const cx<T> *__restrict__ twg, T scale) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -217,7 +127,7 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(T *__restrict__ data, lon
     const int rl = q / n;
     const int e = q - rl * n;
     const long long row = row0 + rl;
-    T v = row < nrows ? data[row * rnxp + e] : (T)0;
+    T v = row < nrows ? (T)data[row * rnxp + e] : (T)0;
     T *dst = reinterpret_cast<T *>(buf + (rl >> 1) * pl.ldsp + e);
     dst[rl & 1] = v;
   }
@@ -233,12 +143,12 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(T *__restrict__ data, lon
     const cx<T> zm = buf[c * pl.ldsp + (k == 0 ? 0 : n - k)];
     const long long ra = row0 + 2 * c;
     if (ra < nrows) {
-      cx<T> *pa = reinterpret_cast<cx<T> *>(data + ra * rnxp);
-      pa[k] = mk(hs * (z.x + zm.x), hs * (z.y - zm.y));
+      cx<S> *pa = reinterpret_cast<cx<S> *>(data + ra * rnxp);
+      pa[k] = mk((S)(hs * (z.x + zm.x)), (S)(hs * (z.y - zm.y)));
     }
     if (ra + 1 < nrows) {
-      cx<T> *pb = reinterpret_cast<cx<T> *>(data + (ra + 1) * rnxp);
-      pb[k] = mk(hs * (z.y + zm.y), hs * (zm.x - z.x));
+      cx<S> *pb = reinterpret_cast<cx<S> *>(data + (ra + 1) * rnxp);
+      pb[k] = mk((S)(hs * (z.y + zm.y)), (S)(hs * (zm.x - z.x)));
     }
   }
 }
@@ -246,8 +156,8 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(T *__restrict__ data, lon
 // x pass, inverse: half spectra of rows a, b -> Z = A + iB on the full length
 // (Hermitian extension, Im of bins 0 and n/2 ignored as in a halfcomplex
 // c2r), inverse DFT, real part -> row a, imaginary part -> row b.
-template <typename T>
-__global__ void __launch_bounds__(1024) k_fft_rows_c2r(T *__restrict__ data, long long nrows, int rnxp,
+template <typename T, typename S>
+__global__ void __launch_bounds__(1024) k_fft_rows_c2r(S *__restrict__ data, long long nrows, int rnxp,
                                                        PassPlan pl, const cx<T> *__restrict__ twg, T scale) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -265,12 +175,12 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(T *__restrict__ data, lon
     const int kk = mirror ? n - k : k;
     const bool selfc = (kk == 0) || ((n & 1) == 0 && kk == n / 2);
     const long long ra = row0 + 2 * c;
-    cx<T> a = mk((T)0, (T)0), b = mk((T)0, (T)0);
-    if (ra < nrows) a = reinterpret_cast<const cx<T> *>(data + ra * rnxp)[kk];
-    if (ra + 1 < nrows) b = reinterpret_cast<const cx<T> *>(data + (ra + 1) * rnxp)[kk];
-    if (selfc) { a.y = (T)0; b.y = (T)0; }
+    cx<S> a = mk((S)0, (S)0), b = mk((S)0, (S)0);
+    if (ra < nrows) a = reinterpret_cast<const cx<S> *>(data + ra * rnxp)[kk];
+    if (ra + 1 < nrows) b = reinterpret_cast<const cx<S> *>(data + (ra + 1) * rnxp)[kk];
+    if (selfc) { a.y = (S)0; b.y = (S)0; }
     if (mirror) { a.y = -a.y; b.y = -b.y; }
-    buf[c * pl.ldsp + k] = mk(a.x - b.y, a.y + b.x);
+    buf[c * pl.ldsp + k] = mk((T)a.x - (T)b.y, (T)a.y + (T)b.x);
   }
   __syncthreads();
   run_passes<+1>(buf, tw, pl);
@@ -281,7 +191,7 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(T *__restrict__ data, lon
     const long long row = row0 + rl;
     if (row < nrows) {
       const T *src = reinterpret_cast<const T *>(buf + (rl >> 1) * pl.ldsp + e);
-      data[row * rnxp + e] = src[rl & 1] * scale;
+      data[row * rnxp + e] = (S)(src[rl & 1] * scale);
     }
   }
 }
@@ -290,26 +200,12 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(T *__restrict__ data, lon
 // Host side: plans and launches.
 // ---------------------------------------------------------------------------
 bool factor_radices(int n, PassPlan &p) {
+  const RadixList L = make_radices(n);
   p.n = n;
-  p.nrad = 0;
-  if (n < 1) return false;
-  int m = n, a = 0;
-  while (m % 2 == 0) { m /= 2; ++a; }
-  std::vector<int> r;
-  const int q = a / 3, rem = a % 3;
-  if (rem == 0) { for (int i = 0; i < q; ++i) r.push_back(8); }
-  else if (rem == 1) {
-    if (q >= 1) { for (int i = 0; i < q - 1; ++i) r.push_back(8); r.push_back(4); r.push_back(4); }
-    else r.push_back(2);
-  } else { for (int i = 0; i < q; ++i) r.push_back(8); r.push_back(4); }
-  for (int pf : {7, 5, 3})
-    while (m % pf == 0) { m /= pf; r.push_back(pf); }
-  if (m != 1 || r.size() > 24) return false;
-  p.nrad = (int)r.size();
-  p.rmin = 8;
-  for (int i = 0; i < p.nrad; ++i) { p.rad[i] = r[i]; p.rmin = std::min(p.rmin, r[i]); }
-  if (p.nrad == 0) p.rmin = 1;
-  return true;
+  p.nrad = L.cnt;
+  p.rmin = L.rmin;
+  for (int i = 0; i < L.cnt; ++i) p.rad[i] = L.r[i];
+  return L.ok;
 }
 
 static int max_lds_per_block() {
@@ -325,7 +221,7 @@ static int max_lds_per_block() {
   return v;
 }
 
-Fft3D::Fft3D(int nx, int ny, int nz, bool d) : nxp(nx), nyp(ny), nzp(nz), dbl(d) {}
+Fft3D::Fft3D(int nx, int ny, int nz, int p) : nxp(nx), nyp(ny), nzp(nz), prec(p), dbl(p >= 1) {}
 
 Fft3D::~Fft3D() {
   if (twx) (void)hipFree(twx);
@@ -395,6 +291,12 @@ int Fft3D::init() {
         (rc = upload_twiddles<float>(nzp, &twz)))
       return rc;
   }
+  // specialised kernels where a length has them (variant 1 until tuned)
+  for (int axis = 0; axis < 3; ++axis) {
+    var[axis] = -1;
+    if (num_variants(axis) > 1 && set_variant(axis, 1) != CRAVA_HIP_OK) (void)set_variant(axis, 0);
+  }
+  set_error("");
   return CRAVA_HIP_OK;
 }
 
@@ -413,7 +315,7 @@ static int set_lds(F kernel, size_t bytes) {
   return CRAVA_HIP_OK;
 }
 
-template <typename T>
+template <typename T, typename S>
 static int launch_x(const Fft3D &f, void *grid, hipStream_t s, bool inverse, double scale) {
   const long long nrows = (long long)f.nyp * f.nzp;
   const int rnxp = 2 * (f.nxp / 2 + 1);
@@ -421,52 +323,124 @@ static int launch_x(const Fft3D &f, void *grid, hipStream_t s, bool inverse, dou
   const unsigned blocks = (unsigned)((nrows + 2LL * p.tx - 1) / (2LL * p.tx));
   int rc;
   if (!inverse) {
-    if ((rc = set_lds(k_fft_rows_r2c<T>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL(k_fft_rows_r2c<T>, dim3(blocks), dim3(p.threads), p.lds_bytes, s,
-                       reinterpret_cast<T *>(grid), nrows, rnxp, p, reinterpret_cast<const cx<T> *>(f.twx), (T)scale);
+    if ((rc = set_lds(k_fft_rows_r2c<T, S>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_rows_r2c<T, S>), dim3(blocks), dim3(p.threads), p.lds_bytes, s,
+                       reinterpret_cast<S *>(grid), nrows, rnxp, p, reinterpret_cast<const cx<T> *>(f.twx), (T)scale);
   } else {
-    if ((rc = set_lds(k_fft_rows_c2r<T>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL(k_fft_rows_c2r<T>, dim3(blocks), dim3(p.threads), p.lds_bytes, s,
-                       reinterpret_cast<T *>(grid), nrows, rnxp, p, reinterpret_cast<const cx<T> *>(f.twx), (T)scale);
+    if ((rc = set_lds(k_fft_rows_c2r<T, S>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_rows_c2r<T, S>), dim3(blocks), dim3(p.threads), p.lds_bytes, s,
+                       reinterpret_cast<S *>(grid), nrows, rnxp, p, reinterpret_cast<const cx<T> *>(f.twx), (T)scale);
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
 
-template <typename T>
+template <typename T, typename S>
 static int launch_strided(const PassPlan &p, const void *tw, void *grid, hipStream_t s, bool inverse,
                           long long es, long long ncols, long long nbatch, long long bstride) {
   if (p.n <= 1) return CRAVA_HIP_OK;
   const unsigned gx = (unsigned)((ncols + p.tx - 1) / p.tx);
   int rc;
   if (!inverse) {
-    if ((rc = set_lds(k_fft_strided<-1, T>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_strided<-1, T>), dim3(gx, (unsigned)nbatch), dim3(p.threads), p.lds_bytes, s,
-                       reinterpret_cast<cx<T> *>(grid), es, ncols, bstride, p, reinterpret_cast<const cx<T> *>(tw));
+    if ((rc = set_lds(k_fft_strided<-1, T, S>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_strided<-1, T, S>), dim3(gx, (unsigned)nbatch), dim3(p.threads), p.lds_bytes, s,
+                       reinterpret_cast<cx<S> *>(grid), es, ncols, bstride, p, reinterpret_cast<const cx<T> *>(tw));
   } else {
-    if ((rc = set_lds(k_fft_strided<1, T>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_strided<1, T>), dim3(gx, (unsigned)nbatch), dim3(p.threads), p.lds_bytes, s,
-                       reinterpret_cast<cx<T> *>(grid), es, ncols, bstride, p, reinterpret_cast<const cx<T> *>(tw));
+    if ((rc = set_lds(k_fft_strided<1, T, S>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_strided<1, T, S>), dim3(gx, (unsigned)nbatch), dim3(p.threads), p.lds_bytes, s,
+                       reinterpret_cast<cx<S> *>(grid), es, ncols, bstride, p, reinterpret_cast<const cx<T> *>(tw));
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
 
 int Fft3D::pass_x(void *grid, hipStream_t s, bool inverse, double scale) {
-  return dbl ? launch_x<double>(*this, grid, s, inverse, scale) : launch_x<float>(*this, grid, s, inverse, scale);
+  if (var[0] >= 0) {
+    const long long nrows = (long long)nyp * nzp;
+    return static_fft_launch(nxp, prec, inverse ? kRowsC2R : kRowsR2C, var[0], grid, 0, 0, 0, 0, nrows,
+                             2 * (nxp / 2 + 1), scale, twx, s);
+  }
+  return prec == 2 ? launch_x<double, double>(*this, grid, s, inverse, scale)
+       : prec == 1 ? launch_x<double, float>(*this, grid, s, inverse, scale)
+                   : launch_x<float, float>(*this, grid, s, inverse, scale);
 }
 
 int Fft3D::pass_y(void *grid, hipStream_t s, bool inverse) {
   const long long cnxp = nxp / 2 + 1;
-  return dbl ? launch_strided<double>(py, twy, grid, s, inverse, cnxp, cnxp, nzp, cnxp * nyp)
-             : launch_strided<float>(py, twy, grid, s, inverse, cnxp, cnxp, nzp, cnxp * nyp);
+  if (nyp <= 1) return CRAVA_HIP_OK;
+  if (var[1] >= 0)
+    return static_fft_launch(nyp, prec, inverse ? kStridedInv : kStridedFwd, var[1], grid, cnxp, cnxp, nzp,
+                             cnxp * nyp, 0, 0, 1.0, twy, s);
+  return prec == 2 ? launch_strided<double, double>(py, twy, grid, s, inverse, cnxp, cnxp, nzp, cnxp * nyp)
+       : prec == 1 ? launch_strided<double, float>(py, twy, grid, s, inverse, cnxp, cnxp, nzp, cnxp * nyp)
+                   : launch_strided<float, float>(py, twy, grid, s, inverse, cnxp, cnxp, nzp, cnxp * nyp);
 }
 
 int Fft3D::pass_z(void *grid, hipStream_t s, bool inverse) {
   const long long cnxp = nxp / 2 + 1;
   const long long plane = cnxp * nyp;
-  return dbl ? launch_strided<double>(pz, twz, grid, s, inverse, plane, plane, 1, 0)
-             : launch_strided<float>(pz, twz, grid, s, inverse, plane, plane, 1, 0);
+  if (nzp <= 1) return CRAVA_HIP_OK;
+  if (var[2] >= 0)
+    return static_fft_launch(nzp, prec, inverse ? kStridedInv : kStridedFwd, var[2], grid, plane, plane, 1, 0, 0, 0,
+                             1.0, twz, s);
+  return prec == 2 ? launch_strided<double, double>(pz, twz, grid, s, inverse, plane, plane, 1, 0)
+       : prec == 1 ? launch_strided<double, float>(pz, twz, grid, s, inverse, plane, plane, 1, 0)
+                   : launch_strided<float, float>(pz, twz, grid, s, inverse, plane, plane, 1, 0);
+}
+
+int Fft3D::set_variant(int axis, int v) {
+  if (axis < 0 || axis > 2) return CRAVA_HIP_E_INVALID;
+  const int n = axis == 0 ? nxp : (axis == 1 ? nyp : nzp);
+  if (v < 0) { var[axis] = -1; return CRAVA_HIP_OK; }
+  const FftKind kind = axis == 0 ? kRowsR2C : kStridedFwd;
+  if (!static_fft_geometry(n, dbl, kind, v, nullptr)) {
+    set_error("variant " + std::to_string(v) + " unavailable for length " + std::to_string(n));
+    return CRAVA_HIP_E_INVALID;
+  }
+  var[axis] = v;
+  return CRAVA_HIP_OK;
+}
+
+int Fft3D::num_variants(int axis) const {
+  const int n = axis == 0 ? nxp : (axis == 1 ? nyp : nzp);
+  if (!static_fft_available(n) || (axis == 0 && n % 2)) return 0;
+  return axis == 0 ? kNumRowVariants : kNumStridedVariants;
+}
+
+// Time every launch variant of every axis on `grid` (contents are destroyed)
+// and keep the fastest.  Forward and inverse share a variant.
+int Fft3D::autotune(void *grid, hipStream_t s, int reps, float *ms_out) {
+  hipEvent_t e0, e1;
+  CRAVA_HIP_CHECK(hipEventCreate(&e0));
+  CRAVA_HIP_CHECK(hipEventCreate(&e1));
+  int rc = CRAVA_HIP_OK;
+  for (int axis = 0; axis < 3 && rc == CRAVA_HIP_OK; ++axis) {
+    const int nv = num_variants(axis);
+    float best = 1e30f;
+    int bestv = var[axis];
+    for (int v = -1; v < nv; ++v) {
+      if (set_variant(axis, v) != CRAVA_HIP_OK) continue;
+      auto run = [&]() {
+        return axis == 0 ? pass_x(grid, s, false, 1.0) : (axis == 1 ? pass_y(grid, s, false) : pass_z(grid, s, false));
+      };
+      if ((rc = run())) break;  // warm-up
+      if (hipEventRecord(e0, s) != hipSuccess) { rc = CRAVA_HIP_E_HIP; break; }
+      for (int r = 0; r < reps && rc == CRAVA_HIP_OK; ++r) rc = run();
+      if (rc) break;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = CRAVA_HIP_E_HIP; break; }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= (float)reps;
+      if (ms < best) { best = ms; bestv = v; }
+    }
+    if (rc == CRAVA_HIP_OK) {
+      var[axis] = bestv;
+      if (ms_out) ms_out[axis] = best;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 int Fft3D::r2c(void *grid, hipStream_t s, double scale) {

@@ -29,7 +29,8 @@ bool factor_radices(int n, PassPlan &p);
 
 class Fft3D {
  public:
-  Fft3D(int nxp, int nyp, int nzp, bool dbl);
+  // prec: 0 float, 1 float storage with double arithmetic, 2 double.
+  Fft3D(int nxp, int nyp, int nzp, int prec);
   ~Fft3D();
   int init();  // CRAVA_HIP_OK or an error code
   // In-place forward r2c; the result is multiplied by `scale`.
@@ -42,11 +43,18 @@ class Fft3D {
   int pass_z(void *grid, hipStream_t s, bool inverse);
 
   int nxp, nyp, nzp;
-  bool dbl;
+  int prec;
+  bool dbl;  // double arithmetic (LDS, twiddles)
   PassPlan px, py, pz;
   void *twx = nullptr, *twy = nullptr, *twz = nullptr;  // device twiddle tables
-  // Plans built with a caller-chosen pencils-per-workgroup (autotuner).
+  // Plans built with a caller-chosen pencils-per-workgroup (generic kernels).
   int set_tx(int tx_x, int tx_y, int tx_z);
+  // Specialised-kernel launch variant per axis (x, y, z); -1 = generic kernel.
+  int var[3] = {-1, -1, -1};
+  int set_variant(int axis, int v);
+  int num_variants(int axis) const;
+  // Time all variants on `grid` (overwritten) and keep the fastest per axis.
+  int autotune(void *grid, hipStream_t s, int reps, float *ms_out);
 
  private:
   int build_plan(int n, PassPlan &p, int want_tx);

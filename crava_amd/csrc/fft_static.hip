@@ -1,0 +1,356 @@
+// Compile-time specialised Stockham passes for gfx950.
+//
+// Same algorithm as the generic kernels in fft.hip, but the length N, the
+// radix sequence, the pencils per workgroup TX and the workgroup size are
+// template constants, so every index division folds into shifts or
+// multiply-highs and the passes unroll.  Each pencil sits in LDS with one pad
+// slot every 32 elements (index x -> x + x/32) and an odd pencil pitch, which
+// removes the bank conflicts of the strided early-pass writes and of the
+// column-major tile loads.
+#include "../../include/crava_hip.h"
+#include "fft_dft.hpp"
+#include "fft_static.hpp"
+
+namespace crava {
+
+__device__ __forceinline__ constexpr int padx(int x) { return x + (x >> 5); }
+
+template <int N>
+struct SP {
+  static constexpr RadixList rl = make_radices(N);
+  static constexpr int len = N + (N - 1) / 32 + 1;
+  static constexpr int pitch = len | 1;
+  static constexpr int ns(int p) {
+    int s = 1;
+    for (int i = 0; i < p; ++i) s *= rl.r[i];
+    return s;
+  }
+};
+
+template <int N, int TX, int THREADS, int DIR, int P, typename T>
+__device__ __forceinline__ void spasses(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw) {
+  if constexpr (P < SP<N>::rl.cnt) {
+    constexpr int R = SP<N>::rl.r[P];
+    constexpr int Ns = SP<N>::ns(P);
+    constexpr int NB = N / R;
+    constexpr int TOT = TX * NB;
+    constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
+    constexpr int TSTEP = N / (Ns * R);
+    constexpr int PITCH = SP<N>::pitch;
+    cx<T> v[TRIPS][R];
+#pragma unroll
+    for (int t = 0; t < TRIPS; ++t) {
+      const int b = (int)threadIdx.x + t * THREADS;
+      if (TOT % THREADS == 0 || b < TOT) {
+        const int c = b / NB, j = b - (b / NB) * NB;
+        const cx<T> *src = buf + c * PITCH;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[t][r] = src[padx(j + r * NB)];
+        if constexpr (Ns > 1) {
+          const int k = j % Ns;
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            cx<T> w = tw[r * k * TSTEP];
+            if (DIR > 0) w.y = -w.y;
+            v[t][r] = mul(v[t][r], w);
+          }
+        }
+        Dft<R, DIR, T>::run(v[t]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < TRIPS; ++t) {
+      const int b = (int)threadIdx.x + t * THREADS;
+      if (TOT % THREADS == 0 || b < TOT) {
+        const int c = b / NB, j = b - (b / NB) * NB;
+        const int k = j % Ns;
+        cx<T> *dst = buf + c * PITCH;
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[padx(base + r * Ns)] = v[t][r];
+      }
+    }
+    __syncthreads();
+    spasses<N, TX, THREADS, DIR, P + 1>(buf, tw);
+  }
+}
+
+template <int N, typename T>
+constexpr size_t lds_bytes(int tx) {
+  return sizeof(cx<T>) * ((size_t)N + (size_t)tx * SP<N>::pitch);
+}
+
+// Strided complex pass: columns contiguous, elements `es` apart.
+template <int N, int TX, int THREADS, int DIR, typename T, typename S>
+__global__ void __launch_bounds__(THREADS) k_sfft_strided(cx<S> *__restrict__ data, long long es, long long ncols,
+                                                          long long bstride, const cx<T> *__restrict__ twg) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
+  cx<T> *buf = tw + N;
+  constexpr int PITCH = SP<N>::pitch;
+  const long long col0 = (long long)blockIdx.x * TX;
+  const int ncol = (int)min((long long)TX, ncols - col0);
+  cx<S> *base = data + (long long)blockIdx.y * bstride + col0;
+  for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
+  constexpr int TOT = TX * N;
+  constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    if (TOT % THREADS == 0 || q < TOT) {
+      const int e = q / TX, c = q % TX;
+      if (c < ncol) {
+        const cx<S> v = base[c + (long long)e * es];
+        buf[c * PITCH + padx(e)] = mk((T)v.x, (T)v.y);
+      }
+    }
+  }
+  __syncthreads();
+  spasses<N, TX, THREADS, DIR, 0>(buf, tw);
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    if (TOT % THREADS == 0 || q < TOT) {
+      const int e = q / TX, c = q % TX;
+      if (c < ncol) {
+        const cx<T> v = buf[c * PITCH + padx(e)];
+        base[c + (long long)e * es] = mk((S)v.x, (S)v.y);
+      }
+    }
+  }
+}
+
+// x pass forward: real rows -> half spectra (two rows per complex pencil).
+template <int N, int TX, int THREADS, typename T, typename S>
+__global__ void __launch_bounds__(THREADS) k_srows_r2c(S *__restrict__ data, long long nrows, int rnxp,
+                                                       const cx<T> *__restrict__ twg, T scale) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
+  cx<T> *buf = tw + N;
+  constexpr int PITCH = SP<N>::pitch;
+  const long long row0 = (long long)blockIdx.x * 2 * TX;
+  for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
+  static_assert(N % 2 == 0, "row kernels need an even length");
+  constexpr int H = N / 2;           // value pairs per row
+  constexpr int TOT = 2 * TX * H;
+  constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    if (TOT % THREADS == 0 || q < TOT) {
+      const int rl = q / H, e2 = q % H;
+      const long long row = row0 + rl;
+      cx<S> v = mk((S)0, (S)0);
+      if (row < nrows) v = reinterpret_cast<const cx<S> *>(data + row * rnxp)[e2];
+      T *d0 = reinterpret_cast<T *>(buf + (rl >> 1) * PITCH + padx(2 * e2));
+      T *d1 = reinterpret_cast<T *>(buf + (rl >> 1) * PITCH + padx(2 * e2 + 1));
+      d0[rl & 1] = (T)v.x;
+      d1[rl & 1] = (T)v.y;
+    }
+  }
+  __syncthreads();
+  spasses<N, TX, THREADS, -1, 0>(buf, tw);
+  constexpr int CN = N / 2 + 1;
+  constexpr int TOT2 = TX * CN;
+  constexpr int TRIPS2 = (TOT2 + THREADS - 1) / THREADS;
+  const T hs = (T)0.5 * scale;
+#pragma unroll
+  for (int t = 0; t < TRIPS2; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    if (TOT2 % THREADS == 0 || q < TOT2) {
+      const int c = q / CN, k = q % CN;
+      const cx<T> z = buf[c * PITCH + padx(k)];
+      const cx<T> zm = buf[c * PITCH + padx(k == 0 ? 0 : N - k)];
+      const long long ra = row0 + 2 * c;
+      if (ra < nrows)
+        reinterpret_cast<cx<S> *>(data + ra * rnxp)[k] = mk((S)(hs * (z.x + zm.x)), (S)(hs * (z.y - zm.y)));
+      if (ra + 1 < nrows)
+        reinterpret_cast<cx<S> *>(data + (ra + 1) * rnxp)[k] = mk((S)(hs * (z.y + zm.y)), (S)(hs * (zm.x - z.x)));
+    }
+  }
+}
+
+// x pass inverse: half spectra of two rows -> Z = A + iB -> inverse -> rows.
+template <int N, int TX, int THREADS, typename T, typename S>
+__global__ void __launch_bounds__(THREADS) k_srows_c2r(S *__restrict__ data, long long nrows, int rnxp,
+                                                       const cx<T> *__restrict__ twg, T scale) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
+  cx<T> *buf = tw + N;
+  constexpr int PITCH = SP<N>::pitch;
+  constexpr int CN = N / 2 + 1;
+  const long long row0 = (long long)blockIdx.x * 2 * TX;
+  for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
+  constexpr int TOT = TX * N;
+  constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    if (TOT % THREADS == 0 || q < TOT) {
+      const int c = q / N, k = q % N;
+      const bool mirror = k >= CN;
+      const int kk = mirror ? N - k : k;
+      const bool selfc = (kk == 0) || (kk == N / 2);
+      const long long ra = row0 + 2 * c;
+      cx<S> a = mk((S)0, (S)0), b = mk((S)0, (S)0);
+      if (ra < nrows) a = reinterpret_cast<const cx<S> *>(data + ra * rnxp)[kk];
+      if (ra + 1 < nrows) b = reinterpret_cast<const cx<S> *>(data + (ra + 1) * rnxp)[kk];
+      if (selfc) { a.y = (S)0; b.y = (S)0; }
+      if (mirror) { a.y = -a.y; b.y = -b.y; }
+      buf[c * PITCH + padx(k)] = mk((T)a.x - (T)b.y, (T)a.y + (T)b.x);
+    }
+  }
+  __syncthreads();
+  spasses<N, TX, THREADS, 1, 0>(buf, tw);
+  constexpr int H = N / 2;
+  constexpr int TOT2 = 2 * TX * H;
+  constexpr int TRIPS2 = (TOT2 + THREADS - 1) / THREADS;
+#pragma unroll
+  for (int t = 0; t < TRIPS2; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    if (TOT2 % THREADS == 0 || q < TOT2) {
+      const int rl = q / H, e2 = q % H;
+      const long long row = row0 + rl;
+      if (row < nrows) {
+        const T *s0 = reinterpret_cast<const T *>(buf + (rl >> 1) * PITCH + padx(2 * e2));
+        const T *s1 = reinterpret_cast<const T *>(buf + (rl >> 1) * PITCH + padx(2 * e2 + 1));
+        reinterpret_cast<cx<S> *>(data + row * rnxp)[e2] = mk((S)(s0[rl & 1] * scale), (S)(s1[rl & 1] * scale));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Variant tables.  Strided: (TX, THREADS); rows: (row pairs, THREADS).
+// ---------------------------------------------------------------------------
+constexpr int kSV[kNumStridedVariants][2] = {{16, 256}, {16, 512}, {8, 256}, {32, 512}};
+constexpr int kRV[kNumRowVariants][2] = {{8, 256}, {16, 512}, {4, 256}};
+
+template <typename F>
+static int set_lds_attr(F kernel, size_t bytes) {
+  if (bytes > 65536)
+    CRAVA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return CRAVA_HIP_OK;
+}
+
+template <int N, int V, typename T, typename S>
+static int launch_strided_v(bool inv, void *data, long long es, long long ncols, long long nbatch, long long bstride,
+                            const void *tw, hipStream_t s) {
+  constexpr int TX = kSV[V][0], TH = kSV[V][1];
+  const size_t lds = lds_bytes<N, T>(TX);
+  const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
+  int rc;
+  if (!inv) {
+    if ((rc = set_lds_attr(k_sfft_strided<N, TX, TH, -1, T, S>, lds))) return rc;
+    hipLaunchKernelGGL((k_sfft_strided<N, TX, TH, -1, T, S>), dim3(gx, (unsigned)nbatch), dim3(TH), lds, s,
+                       reinterpret_cast<cx<S> *>(data), es, ncols, bstride, reinterpret_cast<const cx<T> *>(tw));
+  } else {
+    if ((rc = set_lds_attr(k_sfft_strided<N, TX, TH, 1, T, S>, lds))) return rc;
+    hipLaunchKernelGGL((k_sfft_strided<N, TX, TH, 1, T, S>), dim3(gx, (unsigned)nbatch), dim3(TH), lds, s,
+                       reinterpret_cast<cx<S> *>(data), es, ncols, bstride, reinterpret_cast<const cx<T> *>(tw));
+  }
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
+template <int N, int V, typename T, typename S>
+static int launch_rows_v(bool inv, void *data, long long nrows, int rnxp, double scale, const void *tw,
+                         hipStream_t s) {
+  constexpr int TX = kRV[V][0], TH = kRV[V][1];
+  const size_t lds = lds_bytes<N, T>(TX);
+  const unsigned gx = (unsigned)((nrows + 2LL * TX - 1) / (2LL * TX));
+  int rc;
+  if (!inv) {
+    if ((rc = set_lds_attr(k_srows_r2c<N, TX, TH, T, S>, lds))) return rc;
+    hipLaunchKernelGGL((k_srows_r2c<N, TX, TH, T, S>), dim3(gx), dim3(TH), lds, s, reinterpret_cast<S *>(data), nrows,
+                       rnxp, reinterpret_cast<const cx<T> *>(tw), (T)scale);
+  } else {
+    if ((rc = set_lds_attr(k_srows_c2r<N, TX, TH, T, S>, lds))) return rc;
+    hipLaunchKernelGGL((k_srows_c2r<N, TX, TH, T, S>), dim3(gx), dim3(TH), lds, s, reinterpret_cast<S *>(data), nrows,
+                       rnxp, reinterpret_cast<const cx<T> *>(tw), (T)scale);
+  }
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
+template <int N, typename T, typename S>
+static int launch_n(FftKind kind, int v, void *data, long long es, long long ncols, long long nbatch,
+                    long long bstride, long long nrows, int rnxp, double scale, const void *tw, hipStream_t s) {
+  if (kind == kStridedFwd || kind == kStridedInv) {
+    const bool inv = kind == kStridedInv;
+    switch (v) {
+      case 0: return launch_strided_v<N, 0, T, S>(inv, data, es, ncols, nbatch, bstride, tw, s);
+      case 1: return launch_strided_v<N, 1, T, S>(inv, data, es, ncols, nbatch, bstride, tw, s);
+      case 2: return launch_strided_v<N, 2, T, S>(inv, data, es, ncols, nbatch, bstride, tw, s);
+      default: return launch_strided_v<N, 3, T, S>(inv, data, es, ncols, nbatch, bstride, tw, s);
+    }
+  }
+  const bool inv = kind == kRowsC2R;
+  switch (v) {
+    case 0: return launch_rows_v<N, 0, T, S>(inv, data, nrows, rnxp, scale, tw, s);
+    case 1: return launch_rows_v<N, 1, T, S>(inv, data, nrows, rnxp, scale, tw, s);
+    default: return launch_rows_v<N, 2, T, S>(inv, data, nrows, rnxp, scale, tw, s);
+  }
+}
+
+// Lengths with specialised kernels: the x, y and z lengths of the standard
+// workloads (80/160 plain CPU grid, 288 plain GPU grid, 512/320 large grid,
+// 216 differentiation grid) plus common powers of two.
+#define CRAVA_STATIC_LENGTHS(X) X(64) X(80) X(128) X(160) X(216) X(256) X(288) X(320) X(512)
+
+bool static_fft_available(int n) {
+#define CRAVA_CASE(L) if (n == L) return true;
+  CRAVA_STATIC_LENGTHS(CRAVA_CASE)
+#undef CRAVA_CASE
+  return false;
+}
+
+static size_t max_lds() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0, val = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&val, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && val > 0)
+      v = val;
+    else
+      v = 65536;
+  }
+  return (size_t)v;
+}
+
+bool static_fft_geometry(int n, bool dbl, FftKind kind, int variant, StaticLaunch *out) {
+  if (!static_fft_available(n)) return false;
+  const bool strided = kind == kStridedFwd || kind == kStridedInv;
+  if (strided && (variant < 0 || variant >= kNumStridedVariants)) return false;
+  if (!strided && (variant < 0 || variant >= kNumRowVariants)) return false;
+  const int tx = strided ? kSV[variant][0] : kRV[variant][0];
+  const int th = strided ? kSV[variant][1] : kRV[variant][1];
+  const size_t eb = dbl ? 16 : 8;
+  const size_t len = (size_t)n + (size_t)(n - 1) / 32 + 1;
+  const size_t pitch = len | 1;
+  const size_t lds = eb * ((size_t)n + (size_t)tx * pitch);
+  if (lds > max_lds()) return false;
+  if (out) { out->tx = tx; out->threads = th; out->lds = lds; }
+  return true;
+}
+
+int static_fft_launch(int n, int prec, FftKind kind, int variant, void *data, long long es, long long ncols,
+                      long long nbatch, long long bstride, long long nrows, int rnxp, double scale,
+                      const void *tw, hipStream_t s) {
+  if (!static_fft_geometry(n, prec >= 1, kind, variant, nullptr)) {
+    set_error("no specialised transform for length " + std::to_string(n) + " / variant " + std::to_string(variant));
+    return CRAVA_HIP_E_INVALID;
+  }
+#define CRAVA_CASE(L)                                                                                          \
+  if (n == L)                                                                                                  \
+    return prec == 2 ? launch_n<L, double, double>(kind, variant, data, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s) \
+         : prec == 1 ? launch_n<L, double, float>(kind, variant, data, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s) \
+                     : launch_n<L, float, float>(kind, variant, data, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s);
+  CRAVA_STATIC_LENGTHS(CRAVA_CASE)
+#undef CRAVA_CASE
+  return CRAVA_HIP_E_INVALID;
+}
+
+}  // namespace crava

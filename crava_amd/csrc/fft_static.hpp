@@ -1,0 +1,33 @@
+// Compile-time specialised Stockham kernels for the transform lengths of the
+// standard workloads (see fft_static.hip).  Each length has a few launch
+// variants (pencils per workgroup x threads); the autotuner picks one.
+#pragma once
+#include "common.hpp"
+
+namespace crava {
+
+// Kinds of pass.
+enum FftKind { kStridedFwd = 0, kStridedInv = 1, kRowsR2C = 2, kRowsC2R = 3 };
+
+constexpr int kNumStridedVariants = 4;
+constexpr int kNumRowVariants = 3;
+
+struct StaticLaunch {
+  int tx = 0;        // pencils (strided) or row pairs (rows) per workgroup
+  int threads = 0;
+  size_t lds = 0;
+};
+
+// True if length n has specialised kernels.
+bool static_fft_available(int n);
+// Launch geometry of a variant; false if the variant does not fit.
+bool static_fft_geometry(int n, bool dbl_compute, FftKind kind, int variant, StaticLaunch *out);
+// Launch a specialised pass.  prec: 0 float, 1 float storage with double
+// arithmetic, 2 double.  Strided: data complex, `es` element stride,
+// `ncols` contiguous columns, `nbatch` batches `bstride` apart.  Rows: data
+// real, `nrows` rows of pitch `rnxp`, `scale` applied to the output.
+int static_fft_launch(int n, int prec, FftKind kind, int variant, void *data, long long es, long long ncols,
+                      long long nbatch, long long bstride, long long nrows, int rnxp, double scale,
+                      const void *twiddles, hipStream_t s);
+
+}  // namespace crava

@@ -33,6 +33,9 @@ std::vector<c64> half_spectrum(const float *w, int nzp) {
     out[k].re = (float)re;
     out[k].im = (float)im;
   }
+  // DC and (even length) Nyquist bins of a real series are exactly real
+  out[0].im = 0.0f;
+  if (nzp % 2 == 0) out[nzp / 2].im = 0.0f;
   return out;
 }
 

@@ -87,9 +87,10 @@ def config(name: str, seed: int = 1) -> Config:
         return Config(name, 256, 256, 256, 288, 288, 288, th, [25.0] * 5,
                       list(rng.uniform(0.8, 1.2, 5)), list(rng.uniform(3.0, 10.0, 5)), seed=seed)
     if name == "precision":      # 512x512x320 FFT grid (interior 448x448x256), 6 angles
+        # per-angle S/N from 1e4 down to 20: low noise makes M ill-conditioned
         th = list(np.linspace(4.0, 40.0, 6))
         return Config(name, 448, 448, 256, 512, 512, 320, th, [25.0] * 6,
-                      list(rng.uniform(0.8, 1.2, 6)), [100.0, 80.0, 60.0, 40.0, 30.0, 20.0], seed=seed)
+                      list(rng.uniform(0.8, 1.2, 6)), [1e4, 3e3, 1e3, 300.0, 100.0, 20.0], seed=seed)
     if name == "differentiation":  # 256x256x192, 5 angles
         th = list(np.linspace(5.0, 35.0, 5))
         nxp, nzp = find_closest_factorable(int(math.ceil(256 * 1.1))), find_closest_factorable(int(math.ceil(192 * 1.1)))

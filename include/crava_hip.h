@@ -58,7 +58,10 @@ int crava_hip_synchronize(crava_hip_ctx *ctx);
 /* In-place 3-D real-to-half-complex transform of one device grid.
  * Replaces FFTGrid::fftInPlace (src/fftgrid.cpp:1804-1829): grids that are
  * not CRAVA_HIP_COVARIANCE are scaled by 1/sqrt(nxp*nyp*nzp); FFTW's forward
- * sign (-1).  precision: CRAVA_HIP_F32 (float grid) or CRAVA_HIP_F64. */
+ * sign (-1).  precision: CRAVA_HIP_F32 (float grid, float arithmetic),
+ * CRAVA_HIP_MIXED (float grid, double arithmetic: every output is rounded
+ * once, so tiny spectral values keep their relative accuracy) or
+ * CRAVA_HIP_F64 (double grid). */
 int crava_hip_fft_r2c(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int cubetype, int precision);
 
 /* In-place 3-D half-complex-to-real transform.  Replaces
@@ -73,6 +76,17 @@ int crava_hip_fft_c2r(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp,
  * rfftwnd_one_*). */
 int crava_hip_fft_pass(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int axis, int inverse,
                        int precision, double scale);
+
+/* Autotuner: time every launch variant of each pass of the nxp x nyp x nzp
+ * transform on `scratch` (a grid of that size; its contents are destroyed)
+ * and keep the fastest in the context's plan.  variants[3] / ms[3] (host,
+ * may be NULL) receive the choice per axis (-1 = generic kernel) and its
+ * time.  No reference counterpart (FFTW_ESTIMATE plans are not measured). */
+int crava_hip_autotune_fft(crava_hip_ctx *ctx, void *scratch, int nxp, int nyp, int nzp, int precision,
+                           int *variants, float *ms);
+
+/* Set the launch variant per axis directly (e.g. from a stored tuning). */
+int crava_hip_fft_set_variants(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, int precision, const int *variants);
 
 /* Per-k vectors of the constant-thickness posterior loop, host side.
  * Replaces the per-k preparation of Crava::computePostMeanResidAndFFTCov
@@ -116,6 +130,50 @@ int crava_hip_post_destroy(crava_hip_post *post);
  * factorisations / points solved in FP64. */
 int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, void *const *data, void *const *cov,
                        const void *errcorr, long long *n_fail, long long *n_fp64);
+
+/* Fill a half-complex float grid with white complex Gaussian noise in place
+ * of FFTGrid::fillInComplexNoise (src/fftgrid.cpp:1214-1263): CN(0,1) per
+ * point, real N(0,1) at self-conjugate points, conjugate pairs on the
+ * self-conjugate x planes.  Counter-based (Philox4x32-10) so the result does
+ * not depend on launch shape; (seed, stream) select the realisation.  The
+ * reference's sequential RandomGen stream is not reproduced on the device. */
+int crava_hip_fill_complex_noise(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, uint64_t seed,
+                                 uint64_t stream);
+
+/* One posterior realisation, in place of the per-realisation body of
+ * Crava::simulate (src/crava.cpp:1482-1573):
+ *   seed[3]  in: complex noise grids (float); out: real realisation grids
+ *   cov[6]   posterior covariance in the Fourier domain (read only)
+ *   mean[3]  posterior mean, time domain (read only; NULL adds nothing)
+ * Per point seed <- chol(Sigma_post) seed (0 where the factorisation fails,
+ * :1503-1517), then c2r with PARAMETER scaling and + mean.  n_fail (host,
+ * may be NULL) synchronises and reports failed factorisations. */
+int crava_hip_simulate(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, void *const *cov, void *const *seed,
+                       void *const *mean, long long *n_fail);
+
+/* Only the per-point step of crava_hip_simulate (src/crava.cpp:1482-1521) on
+ * npts contiguous half-complex points. */
+int crava_hip_simulate_points(crava_hip_ctx *ctx, long long npts, void *const *cov, void *const *seed,
+                              long long *n_fail);
+
+/* Facies probabilities from per-facies posterior elastic PDFs, in place of
+ * FaciesProb::CalculateFaciesProbFromPosteriorElasticPDF
+ * (src/faciesprob.cpp:2198-2265, three elastic dimensions, no angle
+ * weights) with PosteriorElasticPDF3D::FindDensity
+ * (src/posteriorelasticpdf3d.cpp:511-604) on an unrotated volume.
+ *   alpha/beta/rho  device, padded real grids [nzp][nyp][rnxp]
+ *   tables          device, nfac tables laid out as FFTGrid(tn1,tn2,tn3)
+ *   volume          host {x0, y0, dx, dy, ztop, zbot}; vn host {nx, ny, nz}
+ *   prior           host [nfac]; p_undefined as the reference
+ *   prob            host array of nfac+1 device grids [nz][ny][2*(nx/2+1)]
+ *                   (last = undefined), RMISSING in the x padding
+ *   argmax          device int [nz][ny][nx] most probable facies, or NULL
+ *   counts          device uint64 [nfac] voxels per most probable facies, or NULL
+ * Bit-exact with the reference's float/double arithmetic.  nfac <= 16. */
+int crava_hip_facies(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp, const float *alpha,
+                     const float *beta, const float *rho, int nfac, const float *tables, int tn1, int tn2, int tn3,
+                     const double *volume, const int *vn, const float *prior, float p_undefined, float *const *prob,
+                     int *argmax, unsigned long long *counts);
 
 #ifdef __cplusplus
 }

@@ -408,6 +408,10 @@ static void spectrum_f(const float *w, int nzp, ocpx *spec)
   for (int t = 0; t < nzp; t++) { x[t].re = w[t]; x[t].im = 0.0; }
   dft_rec(x, nzp, -1, tmp);
   for (int t = 0; t < nzp / 2 + 1; t++) { spec[t].re = (float)x[t].re; spec[t].im = (float)x[t].im; }
+  /* a real series has exactly real DC and Nyquist bins (FFTW's halfcomplex
+   * format stores no imaginary part for them) */
+  spec[0].im = 0.0f;
+  if (nzp % 2 == 0) spec[nzp / 2].im = 0.0f;
   free(x); free(tmp);
 }
 

@@ -1,0 +1,77 @@
+"""C ABI surface, host-side entry points and argument checking (no GPU)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from crava_amd import _lib as L
+from crava_amd import synth
+from tests.helpers import rel_l2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(crava_hip_\w+)\s*\(", text))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 15
+    lib = L.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_declarations():
+    assert set(declared_functions()) <= set(L.SIGNATURES)
+
+
+def test_version_and_error_channel():
+    lib = L.lib()
+    assert lib.crava_hip_version() >= 100
+    assert lib.crava_hip_ctx_create(None, None) == L.E_INVALID
+    assert b"NULL" in lib.crava_hip_last_error()
+    assert lib.crava_hip_fft_r2c(None, None, 8, 8, 8, 0, 0) == L.E_INVALID
+    assert lib.crava_hip_post_run(None, 0, None, None, None, None, None, None) == L.E_INVALID
+    assert lib.crava_hip_facies(None, 1, 1, 1, 1, 1, 1, None, None, None, 1, None, 1, 1, 1, None, None, None,
+                                0.0, None, None, None) == L.E_INVALID
+
+
+@pytest.mark.parametrize("name", ["small", "plain_cpu", "precision"])
+def test_perk_tables_host_code_vs_oracle(name):
+    """The library's host-side per-k preparation against the oracle's."""
+    from crava_amd.core import perk_tables
+    prob = synth.make_problem(synth.config(name), grids=False)
+    cfg = prob.cfg
+    args = (prob.ntheta, cfg.nz, cfg.nzp, prob.lz, cfg.low_cut, cfg.high_cut, 1.0, prob.wavelets, prob.wnorm)
+    got = perk_tables(*args)
+    ref = oracle.perk_tables(*args)
+    assert np.array_equal(got["invert"], ref["invert"])
+    for key in ("kWD", "e1", "e2", "e3"):
+        assert rel_l2(got[key], ref[key]) < 1e-6, key
+
+
+def test_perk_tables_rejects_bad_args():
+    lib = L.lib()
+    z = np.zeros(64, np.float32)
+    rc = lib.crava_hip_perk_tables(9, 8, 8, 32.0, 5.0, 55.0, 1.0, z.ctypes.data, z.ctypes.data, z.ctypes.data,
+                                   z.ctypes.data, z.ctypes.data, z.ctypes.data, z.ctypes.data)
+    assert rc == L.E_INVALID
+
+
+def test_product_never_imports_the_oracle():
+    for path in glob.glob(os.path.join(ROOT, "crava_amd", "**", "*"), recursive=True):
+        if os.path.isfile(path) and path.endswith((".py", ".hip", ".cpp", ".hpp", ".h")):
+            text = open(path).read()
+            assert "import oracle" not in text and "from oracle" not in text, path
+            assert "liboracle" not in text and "libcrava_ref" not in text, path
