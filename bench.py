@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="precision")
-    ap.add_argument("--threshold", type=float, default=1e-3, help="noise/signal below which MIXED uses FP64")
+    ap.add_argument("--threshold", type=float, default=0.1, help="noise/signal below which MIXED uses FP64")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

@@ -73,8 +73,8 @@ __device__ void run_passes(cx<T> *buf, const cx<T> *tw, const PassPlan &pl) {
 // Strided complex pass (y or z).  Columns are contiguous; elements `es`
 // apart; blockIdx.y selects a batch `bstride` apart.
 // ---------------------------------------------------------------------------
-template <int DIR, typename T, typename S>
-__global__ void __launch_bounds__(1024) k_fft_strided(cx<S> *__restrict__ data, long long es, long long ncols,
+template <int DIR, typename T, typename SI, typename SO>
+__global__ void __launch_bounds__(1024) k_fft_strided(const cx<SI> *src, cx<SO> *dst, long long es, long long ncols,
                                                       long long bstride, PassPlan pl,
                                                       const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
@@ -83,14 +83,16 @@ __global__ void __launch_bounds__(1024) k_fft_strided(cx<S> *__restrict__ data, 
   const int nthr = blockDim.x;
   const long long col0 = (long long)blockIdx.x * pl.tx;
   const int ncol = (int)min((long long)pl.tx, ncols - col0);
-  cx<S> *base = data + (long long)blockIdx.y * bstride + col0;
+  const long long off = (long long)blockIdx.y * bstride + col0;
+  const cx<SI> *ibase = src + off;
+  cx<SO> *obase = dst + off;
   for (int q = threadIdx.x; q < pl.n; q += nthr) tw[q] = twg[q];
   const int tot = pl.tx * pl.n;
   for (int q = threadIdx.x; q < tot; q += nthr) {
     const int e = q / pl.tx;
     const int c = q - e * pl.tx;
     if (c < ncol) {
-      const cx<S> v = base[c + e * es];
+      const cx<SI> v = ibase[c + e * es];
       buf[c * pl.ldsp + e] = mk((T)v.x, (T)v.y);
     }
   }
@@ -101,7 +103,7 @@ __global__ void __launch_bounds__(1024) k_fft_strided(cx<S> *__restrict__ data, 
     const int c = q - e * pl.tx;
     if (c < ncol) {
       const cx<T> v = buf[c * pl.ldsp + e];
-      base[c + e * es] = mk((S)v.x, (S)v.y);
+      obase[c + e * es] = mk((SO)v.x, (SO)v.y);
     }
   }
 }
@@ -112,8 +114,8 @@ __global__ void __launch_bounds__(1024) k_fft_strided(cx<S> *__restrict__ data, 
 // 2c+1 (imaginary part); the two spectra are separated afterwards with
 // A_k = (Z_k + conj Z_{n-k})/2, B_k = (Z_k - conj Z_{n-k})/(2i).
 // ---------------------------------------------------------------------------
-template <typename T, typename S>
-__global__ void __launch_bounds__(1024) k_fft_rows_r2c(S *__restrict__ data, long long nrows, int rnxp,
+template <typename T, typename SI, typename SO>
+__global__ void __launch_bounds__(1024) k_fft_rows_r2c(const SI *src, SO *dst, long long nrows, int rnxp,
                                                        PassPlan pl, const cx<T> *__restrict__ twg, T scale) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -127,7 +129,7 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(S *__restrict__ data, lon
     const int rl = q / n;
     const int e = q - rl * n;
     const long long row = row0 + rl;
-    T v = row < nrows ? (T)data[row * rnxp + e] : (T)0;
+    T v = row < nrows ? (T)src[row * rnxp + e] : (T)0;
     T *dst = reinterpret_cast<T *>(buf + (rl >> 1) * pl.ldsp + e);
     dst[rl & 1] = v;
   }
@@ -143,12 +145,12 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(S *__restrict__ data, lon
     const cx<T> zm = buf[c * pl.ldsp + (k == 0 ? 0 : n - k)];
     const long long ra = row0 + 2 * c;
     if (ra < nrows) {
-      cx<S> *pa = reinterpret_cast<cx<S> *>(data + ra * rnxp);
-      pa[k] = mk((S)(hs * (z.x + zm.x)), (S)(hs * (z.y - zm.y)));
+      cx<SO> *pa = reinterpret_cast<cx<SO> *>(dst + ra * rnxp);
+      pa[k] = mk((SO)(hs * (z.x + zm.x)), (SO)(hs * (z.y - zm.y)));
     }
     if (ra + 1 < nrows) {
-      cx<S> *pb = reinterpret_cast<cx<S> *>(data + (ra + 1) * rnxp);
-      pb[k] = mk((S)(hs * (z.y + zm.y)), (S)(hs * (zm.x - z.x)));
+      cx<SO> *pb = reinterpret_cast<cx<SO> *>(dst + (ra + 1) * rnxp);
+      pb[k] = mk((SO)(hs * (z.y + zm.y)), (SO)(hs * (zm.x - z.x)));
     }
   }
 }
@@ -156,8 +158,8 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(S *__restrict__ data, lon
 // x pass, inverse: half spectra of rows a, b -> Z = A + iB on the full length
 // (Hermitian extension, Im of bins 0 and n/2 ignored as in a halfcomplex
 // c2r), inverse DFT, real part -> row a, imaginary part -> row b.
-template <typename T, typename S>
-__global__ void __launch_bounds__(1024) k_fft_rows_c2r(S *__restrict__ data, long long nrows, int rnxp,
+template <typename T, typename SI, typename SO>
+__global__ void __launch_bounds__(1024) k_fft_rows_c2r(const SI *src, SO *dst, long long nrows, int rnxp,
                                                        PassPlan pl, const cx<T> *__restrict__ twg, T scale) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -175,10 +177,10 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(S *__restrict__ data, lon
     const int kk = mirror ? n - k : k;
     const bool selfc = (kk == 0) || ((n & 1) == 0 && kk == n / 2);
     const long long ra = row0 + 2 * c;
-    cx<S> a = mk((S)0, (S)0), b = mk((S)0, (S)0);
-    if (ra < nrows) a = reinterpret_cast<const cx<S> *>(data + ra * rnxp)[kk];
-    if (ra + 1 < nrows) b = reinterpret_cast<const cx<S> *>(data + (ra + 1) * rnxp)[kk];
-    if (selfc) { a.y = (S)0; b.y = (S)0; }
+    cx<SI> a = mk((SI)0, (SI)0), b = mk((SI)0, (SI)0);
+    if (ra < nrows) a = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[kk];
+    if (ra + 1 < nrows) b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[kk];
+    if (selfc) { a.y = (SI)0; b.y = (SI)0; }
     if (mirror) { a.y = -a.y; b.y = -b.y; }
     buf[c * pl.ldsp + k] = mk((T)a.x - (T)b.y, (T)a.y + (T)b.x);
   }
@@ -190,8 +192,8 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(S *__restrict__ data, lon
     const int e = q - rl * n;
     const long long row = row0 + rl;
     if (row < nrows) {
-      const T *src = reinterpret_cast<const T *>(buf + (rl >> 1) * pl.ldsp + e);
-      data[row * rnxp + e] = (S)(src[rl & 1] * scale);
+      const T *sv = reinterpret_cast<const T *>(buf + (rl >> 1) * pl.ldsp + e);
+      dst[row * rnxp + e] = (SO)(sv[rl & 1] * scale);
     }
   }
 }
@@ -199,6 +201,25 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(S *__restrict__ data, lon
 // ---------------------------------------------------------------------------
 // Host side: plans and launches.
 // ---------------------------------------------------------------------------
+// Element-wise storage conversion (used when an axis of length 1 makes a pass
+// the identity but the data still has to move between float and double).
+template <typename SI, typename SO>
+__global__ void k_convert(const SI *src, SO *dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (SO)src[i];
+}
+
+int launch_convert(int io, const void *src, void *dst, long long ncomplex, hipStream_t s) {
+  const long long n = 2 * ncomplex;
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+  if (io == kIoF32toF64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(blocks), dim3(256), 0, s, (const float *)src, (double *)dst, n);
+  else if (io == kIoF64toF32)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks), dim3(256), 0, s, (const double *)src, (float *)dst, n);
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
 bool factor_radices(int n, PassPlan &p) {
   const RadixList L = make_radices(n);
   p.n = n;
@@ -224,6 +245,7 @@ static int max_lds_per_block() {
 Fft3D::Fft3D(int nx, int ny, int nz, int p) : nxp(nx), nyp(ny), nzp(nz), prec(p), dbl(p >= 1) {}
 
 Fft3D::~Fft3D() {
+  if (scratch) (void)hipFree(scratch);
   if (twx) (void)hipFree(twx);
   if (twy) (void)hipFree(twy);
   if (twz) (void)hipFree(twz);
@@ -291,6 +313,10 @@ int Fft3D::init() {
         (rc = upload_twiddles<float>(nzp, &twz)))
       return rc;
   }
+  if (prec == 1) {  // double intermediates of the float-storage transform
+    const size_t bytes = sizeof(cx<double>) * (size_t)(nxp / 2 + 1) * nyp * nzp;
+    CRAVA_HIP_CHECK(hipMalloc(&scratch, bytes));
+  }
   // specialised kernels where a length has them (variant 1 until tuned)
   for (int axis = 0; axis < 3; ++axis) {
     var[axis] = -1;
@@ -315,77 +341,126 @@ static int set_lds(F kernel, size_t bytes) {
   return CRAVA_HIP_OK;
 }
 
-template <typename T, typename S>
-static int launch_x(const Fft3D &f, void *grid, hipStream_t s, bool inverse, double scale) {
+template <typename T, typename SI, typename SO>
+static int launch_x(const Fft3D &f, const void *src, void *dst, hipStream_t s, bool inverse, double scale) {
   const long long nrows = (long long)f.nyp * f.nzp;
   const int rnxp = 2 * (f.nxp / 2 + 1);
   const PassPlan &p = f.px;
   const unsigned blocks = (unsigned)((nrows + 2LL * p.tx - 1) / (2LL * p.tx));
+  const SI *ip = reinterpret_cast<const SI *>(src);
+  SO *op = reinterpret_cast<SO *>(dst);
+  const cx<T> *tw = reinterpret_cast<const cx<T> *>(f.twx);
   int rc;
   if (!inverse) {
-    if ((rc = set_lds(k_fft_rows_r2c<T, S>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_rows_r2c<T, S>), dim3(blocks), dim3(p.threads), p.lds_bytes, s,
-                       reinterpret_cast<S *>(grid), nrows, rnxp, p, reinterpret_cast<const cx<T> *>(f.twx), (T)scale);
+    if ((rc = set_lds(k_fft_rows_r2c<T, SI, SO>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_rows_r2c<T, SI, SO>), dim3(blocks), dim3(p.threads), p.lds_bytes, s, ip, op, nrows,
+                       rnxp, p, tw, (T)scale);
   } else {
-    if ((rc = set_lds(k_fft_rows_c2r<T, S>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_rows_c2r<T, S>), dim3(blocks), dim3(p.threads), p.lds_bytes, s,
-                       reinterpret_cast<S *>(grid), nrows, rnxp, p, reinterpret_cast<const cx<T> *>(f.twx), (T)scale);
+    if ((rc = set_lds(k_fft_rows_c2r<T, SI, SO>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_rows_c2r<T, SI, SO>), dim3(blocks), dim3(p.threads), p.lds_bytes, s, ip, op, nrows,
+                       rnxp, p, tw, (T)scale);
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
 
-template <typename T, typename S>
-static int launch_strided(const PassPlan &p, const void *tw, void *grid, hipStream_t s, bool inverse,
-                          long long es, long long ncols, long long nbatch, long long bstride) {
+template <typename T, typename SI, typename SO>
+static int launch_strided(const PassPlan &p, const void *tw, const void *src, void *dst, hipStream_t s,
+                          bool inverse, long long es, long long ncols, long long nbatch, long long bstride) {
   if (p.n <= 1) return CRAVA_HIP_OK;
   const unsigned gx = (unsigned)((ncols + p.tx - 1) / p.tx);
+  const cx<SI> *ip = reinterpret_cast<const cx<SI> *>(src);
+  cx<SO> *op = reinterpret_cast<cx<SO> *>(dst);
+  const cx<T> *twp = reinterpret_cast<const cx<T> *>(tw);
   int rc;
   if (!inverse) {
-    if ((rc = set_lds(k_fft_strided<-1, T, S>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_strided<-1, T, S>), dim3(gx, (unsigned)nbatch), dim3(p.threads), p.lds_bytes, s,
-                       reinterpret_cast<cx<S> *>(grid), es, ncols, bstride, p, reinterpret_cast<const cx<T> *>(tw));
+    if ((rc = set_lds(k_fft_strided<-1, T, SI, SO>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_strided<-1, T, SI, SO>), dim3(gx, (unsigned)nbatch), dim3(p.threads), p.lds_bytes, s,
+                       ip, op, es, ncols, bstride, p, twp);
   } else {
-    if ((rc = set_lds(k_fft_strided<1, T, S>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_strided<1, T, S>), dim3(gx, (unsigned)nbatch), dim3(p.threads), p.lds_bytes, s,
-                       reinterpret_cast<cx<S> *>(grid), es, ncols, bstride, p, reinterpret_cast<const cx<T> *>(tw));
+    if ((rc = set_lds(k_fft_strided<1, T, SI, SO>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_strided<1, T, SI, SO>), dim3(gx, (unsigned)nbatch), dim3(p.threads), p.lds_bytes, s,
+                       ip, op, es, ncols, bstride, p, twp);
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
+}
+
+// Generic-kernel dispatch on the storage of one pass.
+static int generic_x(const Fft3D &f, int io, const void *src, void *dst, hipStream_t s, bool inv, double scale) {
+  switch (io) {
+    case kIoF32: return launch_x<float, float, float>(f, src, dst, s, inv, scale);
+    case kIoF64: return launch_x<double, double, double>(f, src, dst, s, inv, scale);
+    case kIoF32toF64: return launch_x<double, float, double>(f, src, dst, s, inv, scale);
+    default: return launch_x<double, double, float>(f, src, dst, s, inv, scale);
+  }
+}
+
+static int generic_strided(const PassPlan &p, const void *tw, int io, const void *src, void *dst, hipStream_t s,
+                           bool inv, long long es, long long ncols, long long nbatch, long long bstride) {
+  switch (io) {
+    case kIoF32: return launch_strided<float, float, float>(p, tw, src, dst, s, inv, es, ncols, nbatch, bstride);
+    case kIoF64: return launch_strided<double, double, double>(p, tw, src, dst, s, inv, es, ncols, nbatch, bstride);
+    case kIoF32toF64: return launch_strided<double, float, double>(p, tw, src, dst, s, inv, es, ncols, nbatch, bstride);
+    default: return launch_strided<double, double, float>(p, tw, src, dst, s, inv, es, ncols, nbatch, bstride);
+  }
+}
+
+// Storage of each pass.  Float and double transforms run in place.  The
+// float-storage transform with double arithmetic keeps its intermediates in
+// the plan's double scratch: forward x: grid(f32) -> scratch(f64), y in
+// scratch, z: scratch -> grid(f32); inverse z: grid -> scratch, y in
+// scratch, x: scratch -> grid.
+struct PassIo {
+  int io;
+  const void *src;
+  void *dst;
+};
+
+static PassIo pass_io(const Fft3D &f, void *grid, int axis, bool inverse) {
+  if (f.prec == 0) return {kIoF32, grid, grid};
+  if (f.prec == 2) return {kIoF64, grid, grid};
+  if (axis == 1) return {kIoF64, f.scratch, f.scratch};
+  const bool first = inverse ? axis == 2 : axis == 0;
+  if (first) return {kIoF32toF64, grid, f.scratch};
+  return {kIoF64toF32, f.scratch, grid};
 }
 
 int Fft3D::pass_x(void *grid, hipStream_t s, bool inverse, double scale) {
+  const PassIo io = pass_io(*this, grid, 0, inverse);
   if (var[0] >= 0) {
     const long long nrows = (long long)nyp * nzp;
-    return static_fft_launch(nxp, prec, inverse ? kRowsC2R : kRowsR2C, var[0], grid, 0, 0, 0, 0, nrows,
+    return static_fft_launch(nxp, io.io, inverse ? kRowsC2R : kRowsR2C, var[0], io.src, io.dst, 0, 0, 0, 0, nrows,
                              2 * (nxp / 2 + 1), scale, twx, s);
   }
-  return prec == 2 ? launch_x<double, double>(*this, grid, s, inverse, scale)
-       : prec == 1 ? launch_x<double, float>(*this, grid, s, inverse, scale)
-                   : launch_x<float, float>(*this, grid, s, inverse, scale);
+  return generic_x(*this, io.io, io.src, io.dst, s, inverse, scale);
+}
+
+static int copy_between(const PassIo &io, long long nelem, hipStream_t s) {
+  // a length-1 axis is the identity; still move the data between storages
+  if (io.src == io.dst) return CRAVA_HIP_OK;
+  return launch_convert(io.io, io.src, io.dst, nelem, s);
 }
 
 int Fft3D::pass_y(void *grid, hipStream_t s, bool inverse) {
   const long long cnxp = nxp / 2 + 1;
-  if (nyp <= 1) return CRAVA_HIP_OK;
+  const PassIo io = pass_io(*this, grid, 1, inverse);
+  if (nyp <= 1) return copy_between(io, cnxp * nyp * nzp, s);
   if (var[1] >= 0)
-    return static_fft_launch(nyp, prec, inverse ? kStridedInv : kStridedFwd, var[1], grid, cnxp, cnxp, nzp,
-                             cnxp * nyp, 0, 0, 1.0, twy, s);
-  return prec == 2 ? launch_strided<double, double>(py, twy, grid, s, inverse, cnxp, cnxp, nzp, cnxp * nyp)
-       : prec == 1 ? launch_strided<double, float>(py, twy, grid, s, inverse, cnxp, cnxp, nzp, cnxp * nyp)
-                   : launch_strided<float, float>(py, twy, grid, s, inverse, cnxp, cnxp, nzp, cnxp * nyp);
+    return static_fft_launch(nyp, io.io, inverse ? kStridedInv : kStridedFwd, var[1], io.src, io.dst, cnxp, cnxp,
+                             nzp, cnxp * nyp, 0, 0, 1.0, twy, s);
+  return generic_strided(py, twy, io.io, io.src, io.dst, s, inverse, cnxp, cnxp, nzp, cnxp * nyp);
 }
 
 int Fft3D::pass_z(void *grid, hipStream_t s, bool inverse) {
   const long long cnxp = nxp / 2 + 1;
   const long long plane = cnxp * nyp;
-  if (nzp <= 1) return CRAVA_HIP_OK;
+  const PassIo io = pass_io(*this, grid, 2, inverse);
+  if (nzp <= 1) return copy_between(io, plane * nzp, s);
   if (var[2] >= 0)
-    return static_fft_launch(nzp, prec, inverse ? kStridedInv : kStridedFwd, var[2], grid, plane, plane, 1, 0, 0, 0,
-                             1.0, twz, s);
-  return prec == 2 ? launch_strided<double, double>(pz, twz, grid, s, inverse, plane, plane, 1, 0)
-       : prec == 1 ? launch_strided<double, float>(pz, twz, grid, s, inverse, plane, plane, 1, 0)
-                   : launch_strided<float, float>(pz, twz, grid, s, inverse, plane, plane, 1, 0);
+    return static_fft_launch(nzp, io.io, inverse ? kStridedInv : kStridedFwd, var[2], io.src, io.dst, plane, plane,
+                             1, 0, 0, 0, 1.0, twz, s);
+  return generic_strided(pz, twz, io.io, io.src, io.dst, s, inverse, plane, plane, 1, 0);
 }
 
 int Fft3D::set_variant(int axis, int v) {

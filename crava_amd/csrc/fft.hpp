@@ -27,6 +27,9 @@ struct PassPlan {
 // prime factor > 7.
 bool factor_radices(int n, PassPlan &p);
 
+// Convert ncomplex values between float and double storage (FftIo code).
+int launch_convert(int io, const void *src, void *dst, long long ncomplex, hipStream_t s);
+
 class Fft3D {
  public:
   // prec: 0 float, 1 float storage with double arithmetic, 2 double.
@@ -47,6 +50,7 @@ class Fft3D {
   bool dbl;  // double arithmetic (LDS, twiddles)
   PassPlan px, py, pz;
   void *twx = nullptr, *twy = nullptr, *twz = nullptr;  // device twiddle tables
+  void *scratch = nullptr;  // double intermediates of a float-storage/double-arithmetic plan
   // Plans built with a caller-chosen pencils-per-workgroup (generic kernels).
   int set_tx(int tx_x, int tx_y, int tx_z);
   // Specialised-kernel launch variant per axis (x, y, z); -1 = generic kernel.

@@ -82,8 +82,8 @@ constexpr size_t lds_bytes(int tx) {
 }
 
 // Strided complex pass: columns contiguous, elements `es` apart.
-template <int N, int TX, int THREADS, int DIR, typename T, typename S>
-__global__ void __launch_bounds__(THREADS) k_sfft_strided(cx<S> *__restrict__ data, long long es, long long ncols,
+template <int N, int TX, int THREADS, int DIR, typename T, typename SI, typename SO>
+__global__ void __launch_bounds__(THREADS) k_sfft_strided(const cx<SI> *src, cx<SO> *dst, long long es, long long ncols,
                                                           long long bstride, const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -91,7 +91,9 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(cx<S> *__restrict__ da
   constexpr int PITCH = SP<N>::pitch;
   const long long col0 = (long long)blockIdx.x * TX;
   const int ncol = (int)min((long long)TX, ncols - col0);
-  cx<S> *base = data + (long long)blockIdx.y * bstride + col0;
+  const long long off = (long long)blockIdx.y * bstride + col0;
+  const cx<SI> *ibase = src + off;
+  cx<SO> *obase = dst + off;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
   constexpr int TOT = TX * N;
   constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
@@ -101,7 +103,7 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(cx<S> *__restrict__ da
     if (TOT % THREADS == 0 || q < TOT) {
       const int e = q / TX, c = q % TX;
       if (c < ncol) {
-        const cx<S> v = base[c + (long long)e * es];
+        const cx<SI> v = ibase[c + (long long)e * es];
         buf[c * PITCH + padx(e)] = mk((T)v.x, (T)v.y);
       }
     }
@@ -115,15 +117,15 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(cx<S> *__restrict__ da
       const int e = q / TX, c = q % TX;
       if (c < ncol) {
         const cx<T> v = buf[c * PITCH + padx(e)];
-        base[c + (long long)e * es] = mk((S)v.x, (S)v.y);
+        obase[c + (long long)e * es] = mk((SO)v.x, (SO)v.y);
       }
     }
   }
 }
 
 // x pass forward: real rows -> half spectra (two rows per complex pencil).
-template <int N, int TX, int THREADS, typename T, typename S>
-__global__ void __launch_bounds__(THREADS) k_srows_r2c(S *__restrict__ data, long long nrows, int rnxp,
+template <int N, int TX, int THREADS, typename T, typename SI, typename SO>
+__global__ void __launch_bounds__(THREADS) k_srows_r2c(const SI *src, SO *dst, long long nrows, int rnxp,
                                                        const cx<T> *__restrict__ twg, T scale) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -141,8 +143,8 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(S *__restrict__ data, lon
     if (TOT % THREADS == 0 || q < TOT) {
       const int rl = q / H, e2 = q % H;
       const long long row = row0 + rl;
-      cx<S> v = mk((S)0, (S)0);
-      if (row < nrows) v = reinterpret_cast<const cx<S> *>(data + row * rnxp)[e2];
+      cx<SI> v = mk((SI)0, (SI)0);
+      if (row < nrows) v = reinterpret_cast<const cx<SI> *>(src + row * rnxp)[e2];
       T *d0 = reinterpret_cast<T *>(buf + (rl >> 1) * PITCH + padx(2 * e2));
       T *d1 = reinterpret_cast<T *>(buf + (rl >> 1) * PITCH + padx(2 * e2 + 1));
       d0[rl & 1] = (T)v.x;
@@ -164,16 +166,16 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(S *__restrict__ data, lon
       const cx<T> zm = buf[c * PITCH + padx(k == 0 ? 0 : N - k)];
       const long long ra = row0 + 2 * c;
       if (ra < nrows)
-        reinterpret_cast<cx<S> *>(data + ra * rnxp)[k] = mk((S)(hs * (z.x + zm.x)), (S)(hs * (z.y - zm.y)));
+        reinterpret_cast<cx<SO> *>(dst + ra * rnxp)[k] = mk((SO)(hs * (z.x + zm.x)), (SO)(hs * (z.y - zm.y)));
       if (ra + 1 < nrows)
-        reinterpret_cast<cx<S> *>(data + (ra + 1) * rnxp)[k] = mk((S)(hs * (z.y + zm.y)), (S)(hs * (zm.x - z.x)));
+        reinterpret_cast<cx<SO> *>(dst + (ra + 1) * rnxp)[k] = mk((SO)(hs * (z.y + zm.y)), (SO)(hs * (zm.x - z.x)));
     }
   }
 }
 
 // x pass inverse: half spectra of two rows -> Z = A + iB -> inverse -> rows.
-template <int N, int TX, int THREADS, typename T, typename S>
-__global__ void __launch_bounds__(THREADS) k_srows_c2r(S *__restrict__ data, long long nrows, int rnxp,
+template <int N, int TX, int THREADS, typename T, typename SI, typename SO>
+__global__ void __launch_bounds__(THREADS) k_srows_c2r(const SI *src, SO *dst, long long nrows, int rnxp,
                                                        const cx<T> *__restrict__ twg, T scale) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
@@ -193,10 +195,10 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(S *__restrict__ data, lon
       const int kk = mirror ? N - k : k;
       const bool selfc = (kk == 0) || (kk == N / 2);
       const long long ra = row0 + 2 * c;
-      cx<S> a = mk((S)0, (S)0), b = mk((S)0, (S)0);
-      if (ra < nrows) a = reinterpret_cast<const cx<S> *>(data + ra * rnxp)[kk];
-      if (ra + 1 < nrows) b = reinterpret_cast<const cx<S> *>(data + (ra + 1) * rnxp)[kk];
-      if (selfc) { a.y = (S)0; b.y = (S)0; }
+      cx<SI> a = mk((SI)0, (SI)0), b = mk((SI)0, (SI)0);
+      if (ra < nrows) a = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[kk];
+      if (ra + 1 < nrows) b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[kk];
+      if (selfc) { a.y = (SI)0; b.y = (SI)0; }
       if (mirror) { a.y = -a.y; b.y = -b.y; }
       buf[c * PITCH + padx(k)] = mk((T)a.x - (T)b.y, (T)a.y + (T)b.x);
     }
@@ -215,7 +217,7 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(S *__restrict__ data, lon
       if (row < nrows) {
         const T *s0 = reinterpret_cast<const T *>(buf + (rl >> 1) * PITCH + padx(2 * e2));
         const T *s1 = reinterpret_cast<const T *>(buf + (rl >> 1) * PITCH + padx(2 * e2 + 1));
-        reinterpret_cast<cx<S> *>(data + row * rnxp)[e2] = mk((S)(s0[rl & 1] * scale), (S)(s1[rl & 1] * scale));
+        reinterpret_cast<cx<SO> *>(dst + row * rnxp)[e2] = mk((SO)(s0[rl & 1] * scale), (SO)(s1[rl & 1] * scale));
       }
     }
   }
@@ -235,64 +237,96 @@ static int set_lds_attr(F kernel, size_t bytes) {
   return CRAVA_HIP_OK;
 }
 
-template <int N, int V, typename T, typename S>
-static int launch_strided_v(bool inv, void *data, long long es, long long ncols, long long nbatch, long long bstride,
-                            const void *tw, hipStream_t s) {
+template <int N, int V, int DIR, typename T, typename SI, typename SO>
+static int launch_strided_v(const void *src, void *dst, long long es, long long ncols, long long nbatch,
+                            long long bstride, const void *tw, hipStream_t s) {
   constexpr int TX = kSV[V][0], TH = kSV[V][1];
   const size_t lds = lds_bytes<N, T>(TX);
   const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
   int rc;
-  if (!inv) {
-    if ((rc = set_lds_attr(k_sfft_strided<N, TX, TH, -1, T, S>, lds))) return rc;
-    hipLaunchKernelGGL((k_sfft_strided<N, TX, TH, -1, T, S>), dim3(gx, (unsigned)nbatch), dim3(TH), lds, s,
-                       reinterpret_cast<cx<S> *>(data), es, ncols, bstride, reinterpret_cast<const cx<T> *>(tw));
-  } else {
-    if ((rc = set_lds_attr(k_sfft_strided<N, TX, TH, 1, T, S>, lds))) return rc;
-    hipLaunchKernelGGL((k_sfft_strided<N, TX, TH, 1, T, S>), dim3(gx, (unsigned)nbatch), dim3(TH), lds, s,
-                       reinterpret_cast<cx<S> *>(data), es, ncols, bstride, reinterpret_cast<const cx<T> *>(tw));
-  }
+  if ((rc = set_lds_attr(k_sfft_strided<N, TX, TH, DIR, T, SI, SO>, lds))) return rc;
+  hipLaunchKernelGGL((k_sfft_strided<N, TX, TH, DIR, T, SI, SO>), dim3(gx, (unsigned)nbatch), dim3(TH), lds, s,
+                     reinterpret_cast<const cx<SI> *>(src), reinterpret_cast<cx<SO> *>(dst), es, ncols, bstride,
+                     reinterpret_cast<const cx<T> *>(tw));
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
 
-template <int N, int V, typename T, typename S>
-static int launch_rows_v(bool inv, void *data, long long nrows, int rnxp, double scale, const void *tw,
+template <int N, int V, bool INV, typename T, typename SI, typename SO>
+static int launch_rows_v(const void *src, void *dst, long long nrows, int rnxp, double scale, const void *tw,
                          hipStream_t s) {
   constexpr int TX = kRV[V][0], TH = kRV[V][1];
   const size_t lds = lds_bytes<N, T>(TX);
   const unsigned gx = (unsigned)((nrows + 2LL * TX - 1) / (2LL * TX));
+  const SI *ip = reinterpret_cast<const SI *>(src);
+  SO *op = reinterpret_cast<SO *>(dst);
+  const cx<T> *twp = reinterpret_cast<const cx<T> *>(tw);
   int rc;
-  if (!inv) {
-    if ((rc = set_lds_attr(k_srows_r2c<N, TX, TH, T, S>, lds))) return rc;
-    hipLaunchKernelGGL((k_srows_r2c<N, TX, TH, T, S>), dim3(gx), dim3(TH), lds, s, reinterpret_cast<S *>(data), nrows,
-                       rnxp, reinterpret_cast<const cx<T> *>(tw), (T)scale);
+  if constexpr (!INV) {
+    if ((rc = set_lds_attr(k_srows_r2c<N, TX, TH, T, SI, SO>, lds))) return rc;
+    hipLaunchKernelGGL((k_srows_r2c<N, TX, TH, T, SI, SO>), dim3(gx), dim3(TH), lds, s, ip, op, nrows, rnxp, twp,
+                       (T)scale);
   } else {
-    if ((rc = set_lds_attr(k_srows_c2r<N, TX, TH, T, S>, lds))) return rc;
-    hipLaunchKernelGGL((k_srows_c2r<N, TX, TH, T, S>), dim3(gx), dim3(TH), lds, s, reinterpret_cast<S *>(data), nrows,
-                       rnxp, reinterpret_cast<const cx<T> *>(tw), (T)scale);
+    if ((rc = set_lds_attr(k_srows_c2r<N, TX, TH, T, SI, SO>, lds))) return rc;
+    hipLaunchKernelGGL((k_srows_c2r<N, TX, TH, T, SI, SO>), dim3(gx), dim3(TH), lds, s, ip, op, nrows, rnxp, twp,
+                       (T)scale);
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
 
-template <int N, typename T, typename S>
-static int launch_n(FftKind kind, int v, void *data, long long es, long long ncols, long long nbatch,
-                    long long bstride, long long nrows, int rnxp, double scale, const void *tw, hipStream_t s) {
-  if (kind == kStridedFwd || kind == kStridedInv) {
-    const bool inv = kind == kStridedInv;
-    switch (v) {
-      case 0: return launch_strided_v<N, 0, T, S>(inv, data, es, ncols, nbatch, bstride, tw, s);
-      case 1: return launch_strided_v<N, 1, T, S>(inv, data, es, ncols, nbatch, bstride, tw, s);
-      case 2: return launch_strided_v<N, 2, T, S>(inv, data, es, ncols, nbatch, bstride, tw, s);
-      default: return launch_strided_v<N, 3, T, S>(inv, data, es, ncols, nbatch, bstride, tw, s);
-    }
-  }
-  const bool inv = kind == kRowsC2R;
+template <int N, int DIR, typename T, typename SI, typename SO>
+static int strided_any(int v, const void *src, void *dst, long long es, long long ncols, long long nbatch,
+                       long long bstride, const void *tw, hipStream_t s) {
   switch (v) {
-    case 0: return launch_rows_v<N, 0, T, S>(inv, data, nrows, rnxp, scale, tw, s);
-    case 1: return launch_rows_v<N, 1, T, S>(inv, data, nrows, rnxp, scale, tw, s);
-    default: return launch_rows_v<N, 2, T, S>(inv, data, nrows, rnxp, scale, tw, s);
+    case 0: return launch_strided_v<N, 0, DIR, T, SI, SO>(src, dst, es, ncols, nbatch, bstride, tw, s);
+    case 1: return launch_strided_v<N, 1, DIR, T, SI, SO>(src, dst, es, ncols, nbatch, bstride, tw, s);
+    case 2: return launch_strided_v<N, 2, DIR, T, SI, SO>(src, dst, es, ncols, nbatch, bstride, tw, s);
+    default: return launch_strided_v<N, 3, DIR, T, SI, SO>(src, dst, es, ncols, nbatch, bstride, tw, s);
   }
+}
+
+template <int N, bool INV, typename T, typename SI, typename SO>
+static int rows_any(int v, const void *src, void *dst, long long nrows, int rnxp, double scale, const void *tw,
+                    hipStream_t s) {
+  switch (v) {
+    case 0: return launch_rows_v<N, 0, INV, T, SI, SO>(src, dst, nrows, rnxp, scale, tw, s);
+    case 1: return launch_rows_v<N, 1, INV, T, SI, SO>(src, dst, nrows, rnxp, scale, tw, s);
+    default: return launch_rows_v<N, 2, INV, T, SI, SO>(src, dst, nrows, rnxp, scale, tw, s);
+  }
+}
+
+template <int N, typename T, typename S>
+static int all_kinds(FftKind kind, int v, const void *src, void *dst, long long es, long long ncols, long long nbatch,
+                     long long bstride, long long nrows, int rnxp, double scale, const void *tw, hipStream_t s) {
+  switch (kind) {
+    case kStridedFwd: return strided_any<N, -1, T, S, S>(v, src, dst, es, ncols, nbatch, bstride, tw, s);
+    case kStridedInv: return strided_any<N, 1, T, S, S>(v, src, dst, es, ncols, nbatch, bstride, tw, s);
+    case kRowsR2C: return rows_any<N, false, T, S, S>(v, src, dst, nrows, rnxp, scale, tw, s);
+    default: return rows_any<N, true, T, S, S>(v, src, dst, nrows, rnxp, scale, tw, s);
+  }
+}
+
+// Storage combinations: float, double, and the two halves of a float-storage
+// transform with double arithmetic and double intermediates: forward = rows
+// float->double, strided double->double, strided double->float; inverse =
+// strided float->double, strided double->double, rows double->float.
+template <int N>
+static int launch_store(int io, FftKind kind, int v, const void *src, void *dst, long long es, long long ncols,
+                        long long nbatch, long long bstride, long long nrows, int rnxp, double scale, const void *tw,
+                        hipStream_t s) {
+  if (io == kIoF32) return all_kinds<N, float, float>(kind, v, src, dst, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s);
+  if (io == kIoF64) return all_kinds<N, double, double>(kind, v, src, dst, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s);
+  if (io == kIoF32toF64 && kind == kRowsR2C)
+    return rows_any<N, false, double, float, double>(v, src, dst, nrows, rnxp, scale, tw, s);
+  if (io == kIoF32toF64 && kind == kStridedInv)
+    return strided_any<N, 1, double, float, double>(v, src, dst, es, ncols, nbatch, bstride, tw, s);
+  if (io == kIoF64toF32 && kind == kStridedFwd)
+    return strided_any<N, -1, double, double, float>(v, src, dst, es, ncols, nbatch, bstride, tw, s);
+  if (io == kIoF64toF32 && kind == kRowsC2R)
+    return rows_any<N, true, double, double, float>(v, src, dst, nrows, rnxp, scale, tw, s);
+  set_error("unsupported storage combination for this pass");
+  return CRAVA_HIP_E_INVALID;
 }
 
 // Lengths with specialised kernels: the x, y and z lengths of the standard
@@ -336,18 +370,16 @@ bool static_fft_geometry(int n, bool dbl, FftKind kind, int variant, StaticLaunc
   return true;
 }
 
-int static_fft_launch(int n, int prec, FftKind kind, int variant, void *data, long long es, long long ncols,
-                      long long nbatch, long long bstride, long long nrows, int rnxp, double scale,
+int static_fft_launch(int n, int io, FftKind kind, int variant, const void *src, void *dst, long long es,
+                      long long ncols, long long nbatch, long long bstride, long long nrows, int rnxp, double scale,
                       const void *tw, hipStream_t s) {
-  if (!static_fft_geometry(n, prec >= 1, kind, variant, nullptr)) {
+  if (!static_fft_geometry(n, io != kIoF32, kind, variant, nullptr)) {
     set_error("no specialised transform for length " + std::to_string(n) + " / variant " + std::to_string(variant));
     return CRAVA_HIP_E_INVALID;
   }
 #define CRAVA_CASE(L)                                                                                          \
   if (n == L)                                                                                                  \
-    return prec == 2 ? launch_n<L, double, double>(kind, variant, data, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s) \
-         : prec == 1 ? launch_n<L, double, float>(kind, variant, data, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s) \
-                     : launch_n<L, float, float>(kind, variant, data, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s);
+    return launch_store<L>(io, kind, variant, src, dst, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s);
   CRAVA_STATIC_LENGTHS(CRAVA_CASE)
 #undef CRAVA_CASE
   return CRAVA_HIP_E_INVALID;

@@ -22,12 +22,15 @@ struct StaticLaunch {
 bool static_fft_available(int n);
 // Launch geometry of a variant; false if the variant does not fit.
 bool static_fft_geometry(int n, bool dbl_compute, FftKind kind, int variant, StaticLaunch *out);
-// Launch a specialised pass.  prec: 0 float, 1 float storage with double
-// arithmetic, 2 double.  Strided: data complex, `es` element stride,
-// `ncols` contiguous columns, `nbatch` batches `bstride` apart.  Rows: data
-// real, `nrows` rows of pitch `rnxp`, `scale` applied to the output.
-int static_fft_launch(int n, int prec, FftKind kind, int variant, void *data, long long es, long long ncols,
-                      long long nbatch, long long bstride, long long nrows, int rnxp, double scale,
+// Storage of a pass's input/output (arithmetic is double unless kIoF32).
+enum FftIo { kIoF32 = 0, kIoF64 = 1, kIoF32toF64 = 2, kIoF64toF32 = 3 };
+
+// Launch a specialised pass from `src` to `dst` (may alias for in place).
+// Strided: complex, `es` element stride, `ncols` contiguous columns,
+// `nbatch` batches `bstride` apart.  Rows: `nrows` rows of pitch `rnxp`
+// reals, `scale` applied to the output.
+int static_fft_launch(int n, int io, FftKind kind, int variant, const void *src, void *dst, long long es,
+                      long long ncols, long long nbatch, long long bstride, long long nrows, int rnxp, double scale,
                       const void *twiddles, hipStream_t s);
 
 }  // namespace crava
