@@ -63,7 +63,7 @@ SIGNATURES = {
     "crava_hip_fill_complex_noise": (I, [VP, VP, I, I, I, C.c_uint64, C.c_uint64]),
     "crava_hip_facies": (I, [VP, I, I, I, I, I, I, VP, VP, VP, I, VP, I, I, I, VP, VP, VP, C.c_float,
                              C.POINTER(VP), VP, VP]),
-    "crava_hip_nll_grad": (I, [VP, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP, VP, VP, VP,
+    "crava_hip_nll_grad": (I, [VP, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP, VP, VP, VP,
                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "crava_hip_batch_create": (I, [VP, I, VP, C.POINTER(VP)]),
     "crava_hip_batch_run": (I, [VP, I]),

@@ -4,6 +4,7 @@
 #include "context.hpp"
 #include "fft.hpp"
 #include "facies.hpp"
+#include "nll.hpp"
 #include "posterior.hpp"
 #include "simulate.hpp"
 #include "tables.hpp"
@@ -45,6 +46,14 @@ Context::~Context() {
 struct crava_hip_post {
   crava_hip_ctx *ctx;
   crava::PostDevice dev;
+  // host copies of the per-k vectors (the likelihood rebuilds its tables)
+  std::vector<unsigned char> invert;
+  std::vector<float> kWD, e1, e2, e3;
+  // likelihood workspace, allocated on first use
+  crava::NllTab *nll_tab = nullptr;
+  int *nll_klist = nullptr;
+  double *nll_partial = nullptr;
+  int nll_blocks = 0;
 };
 
 using crava::set_error;
@@ -207,6 +216,12 @@ int crava_hip_post_create(crava_hip_ctx *ctx, const crava_hip_post_params *param
   std::vector<crava::PerK<float>> tf(p.nzp);
   std::vector<crava::PerK<double>> td(p.nzp);
   crava::build_perk(pp, invert, kWD, e1, e2, e3, tf.data(), td.data());
+  const size_t nv = (size_t)p.nzp * p.ntheta * 2;
+  h->invert.assign(invert, invert + p.nzp);
+  h->kWD.assign(kWD, kWD + nv);
+  h->e1.assign(e1, e1 + nv);
+  h->e2.assign(e2, e2 + nv);
+  h->e3.assign(e3, e3 + nv);
   hipError_t e = hipMalloc(&h->dev.tabf, sizeof(crava::PerK<float>) * p.nzp);
   if (e == hipSuccess) e = hipMalloc(&h->dev.tabd, sizeof(crava::PerK<double>) * p.nzp);
   if (e == hipSuccess)
@@ -226,6 +241,9 @@ int crava_hip_post_destroy(crava_hip_post *post) {
   if (!post) return CRAVA_HIP_OK;
   if (post->dev.tabf) (void)hipFree(post->dev.tabf);
   if (post->dev.tabd) (void)hipFree(post->dev.tabd);
+  if (post->nll_tab) (void)hipFree(post->nll_tab);
+  if (post->nll_klist) (void)hipFree(post->nll_klist);
+  if (post->nll_partial) (void)hipFree(post->nll_partial);
   delete post;
   return CRAVA_HIP_OK;
 }
@@ -249,6 +267,125 @@ int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, voi
     CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (n_fail) *n_fail = (long long)h[0];
     if (n_fp64) *n_fp64 = (long long)h[1];
+  }
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_nll_grad(crava_hip_post *post, int precision, void *const *mu, void *const *data, void *const *cov,
+                       const void *errcorr, const double *scales, const double *noise_var, const double *chol0,
+                       double *nll, double *grad) {
+  if (!post || !mu || !data || !cov || !errcorr || !scales || !noise_var || !chol0 || !nll || !grad) {
+    set_error("NULL argument to crava_hip_nll_grad");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (precision != CRAVA_HIP_F32 && precision != CRAVA_HIP_F64) { set_error("grids must be float or double"); return CRAVA_HIP_E_INVALID; }
+  const crava::PostProblem &pp = post->dev.pp;
+  const int nt = pp.ntheta;
+  for (int l = 0; l < nt; ++l)
+    if (!(scales[l] > 0.0) || !(noise_var[l] > 0.0)) { set_error("scales and noise variances must be positive"); return CRAVA_HIP_E_INVALID; }
+  crava_hip_ctx *ctx = post->ctx;
+  // tables at the current parameters
+  std::vector<crava::NllTab> tab(pp.nzp);
+  std::vector<int> klist;
+  const double c0 = 0.5 * (1.0 - (double)pp.wnc);
+  double sig[crava::kMaxTheta * crava::kMaxTheta];
+  for (int l = 0; l < nt; ++l)
+    for (int m = 0; m < nt; ++m) {
+      const double bl = pp.err_theta_cov[l * nt + l], bm = pp.err_theta_cov[m * nt + m];
+      const double rho = (bl > 0 && bm > 0) ? pp.err_theta_cov[l * nt + m] / std::sqrt(bl * bm) : (l == m ? 1.0 : 0.0);
+      sig[l * nt + m] = std::sqrt(noise_var[l] * noise_var[m]) * rho;
+    }
+  for (int k = 0; k < pp.nzp; ++k) {
+    crava::NllTab t{};
+    if (post->invert[k]) klist.push_back(k);
+    crava::cx<double> w[crava::kMaxTheta], a1[crava::kMaxTheta], a2[crava::kMaxTheta];
+    for (int l = 0; l < nt; ++l) {
+      const size_t o = ((size_t)k * nt + l) * 2;
+      w[l] = crava::mk((double)post->kWD[o] * scales[l], (double)post->kWD[o + 1] * scales[l]);
+      a1[l] = crava::mk((double)post->e1[o], (double)post->e1[o + 1]);
+      a2[l] = crava::mk((double)post->e2[o], (double)post->e2[o + 1]);
+      t.kwd[l] = w[l];
+      const double e3 = post->e3[o];
+      t.wd[l] = (double)pp.wnc * e3 * e3;
+      t.Ed[l] = t.wd[l] * sig[l * nt + l];
+    }
+    for (int l = 0; l < nt; ++l)
+      for (int m = 0; m <= l; ++m) {
+        const int q = l * (l + 1) / 2 + m;
+        crava::cx<double> g = crava::add(crava::mulc(a1[l], a1[m]), crava::mulc(a2[l], a2[m]));
+        if (l == m) g.y = 0.0;
+        t.g[q] = g;
+        t.Ec[q] = crava::scl(g, c0 * sig[l * nt + m]);
+      }
+    t.c0 = c0;
+    tab[k] = t;
+  }
+  // Sigma0 = L0 L0^T
+  const double L00 = chol0[0], L10 = chol0[1], L11 = chol0[2], L20 = chol0[3], L21 = chol0[4], L22 = chol0[5];
+  const double S[3][3] = {{L00 * L00, L00 * L10, L00 * L20},
+                          {L10 * L00, L10 * L10 + L11 * L11, L10 * L20 + L11 * L21},
+                          {L20 * L00, L20 * L10 + L21 * L11, L20 * L20 + L21 * L21 + L22 * L22}};
+  crava::NllCall c{};
+  c.nt = nt;
+  c.nxp = pp.nxp;
+  c.nyp = pp.nyp;
+  c.dbl = precision == CRAVA_HIP_F64;
+  for (int q = 0; q < 3; ++q) c.mu[q] = mu[q];
+  for (int l = 0; l < nt; ++l) c.data[l] = data[l];
+  c.cov0 = cov[0];
+  c.errcorr = errcorr;
+  c.n_inband = (int)klist.size();
+  for (int i = 0; i < crava::kMaxTheta * 3; ++i) c.A[i] = pp.A[i];
+  c.sig0[0] = S[0][0]; c.sig0[1] = S[1][1]; c.sig0[2] = S[2][2];
+  c.sig0[3] = S[0][1]; c.sig0[4] = S[0][2]; c.sig0[5] = S[1][2];
+  c.base00 = pp.sigma0[0];
+  const int na = crava::nll_accumulator_count(nt);
+  if (!post->nll_tab) {
+    CRAVA_HIP_CHECK(hipMalloc(&post->nll_tab, sizeof(crava::NllTab) * pp.nzp));
+    CRAVA_HIP_CHECK(hipMalloc(&post->nll_klist, sizeof(int) * pp.nzp));
+    post->nll_blocks = 1024;
+    CRAVA_HIP_CHECK(hipMalloc(&post->nll_partial, sizeof(double) * post->nll_blocks * 64));
+  }
+  CRAVA_HIP_CHECK(hipMemcpyAsync(post->nll_tab, tab.data(), sizeof(crava::NllTab) * pp.nzp, hipMemcpyHostToDevice, ctx->stream));
+  if (!klist.empty())
+    CRAVA_HIP_CHECK(hipMemcpyAsync(post->nll_klist, klist.data(), sizeof(int) * klist.size(), hipMemcpyHostToDevice, ctx->stream));
+  c.tab = post->nll_tab;
+  c.klist = post->nll_klist;
+  c.partial = post->nll_partial;
+  const long long total = (long long)c.n_inband * (pp.nxp / 2 + 1) * pp.nyp;
+  c.blocks = (int)std::max<long long>(1, std::min<long long>(post->nll_blocks, (total + 255) / 256));
+  std::vector<double> part((size_t)c.blocks * na, 0.0);
+  if (total > 0) {
+    int rc = crava::launch_nll(c, ctx->stream);
+    if (rc) return rc;
+    CRAVA_HIP_CHECK(hipMemcpyAsync(part.data(), post->nll_partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  std::vector<double> acc(na, 0.0);
+  for (int b = 0; b < c.blocks; ++b)
+    for (int v = 0; v < na; ++v) acc[v] += part[(size_t)b * na + v];
+  const int kS = 1, kN = 1 + nt, kP = 1 + nt + nt * (nt + 1) / 2;
+  *nll = 0.5 * acc[0];
+  for (int l = 0; l < nt; ++l) grad[l] = 0.5 * acc[kS + l] / scales[l];
+  for (int l = 0; l < nt; ++l) {
+    double gsum = acc[kN + l * (l + 1) / 2 + l];
+    for (int m = 0; m < nt; ++m) {
+      if (m == l) continue;
+      const int hi = l > m ? l : m, lo = l > m ? m : l;
+      gsum += acc[kN + hi * (hi + 1) / 2 + lo] * sig[l * nt + m] / (2.0 * noise_var[l]);
+    }
+    grad[nt + l] = 0.5 * gsum;
+  }
+  // dNLL/dSigma0 (symmetric P) -> dNLL/dL0 = 2 * (1/2) P L0 on the lower triangle
+  const double P[3][3] = {{acc[kP + 0], acc[kP + 3], acc[kP + 4]},
+                          {acc[kP + 3], acc[kP + 1], acc[kP + 5]},
+                          {acc[kP + 4], acc[kP + 5], acc[kP + 2]}};
+  const double Lm[3][3] = {{L00, 0, 0}, {L10, L11, 0}, {L20, L21, L22}};
+  const int li[6] = {0, 1, 1, 2, 2, 2}, lj[6] = {0, 0, 1, 0, 1, 2};
+  for (int q = 0; q < 6; ++q) {
+    double v = 0.0;
+    for (int k = 0; k < 3; ++k) v += P[li[q]][k] * Lm[k][lj[q]];
+    grad[2 * nt + q] = v;
   }
   return CRAVA_HIP_OK;
 }

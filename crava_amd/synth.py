@@ -322,3 +322,47 @@ def post_params(prob: Problem, mixed_threshold: float = 1e-3):
         p.err_theta_cov[i] = float(v)
     p.mixed_threshold = mixed_threshold
     return p
+
+
+def model_data(prob: Problem, tables: dict, mu_hat: np.ndarray, cov0_hat: np.ndarray, ec_hat: np.ndarray,
+               scales, noise_var, chol0, seed: int = 7) -> list:
+    """Fourier-domain data drawn from the likelihood model at given parameters:
+    d(p) = K mu(p) + chol(M(p)) zeta(p), zeta ~ CN(0, I), at the in-band
+    points (others are left at zero).  mu_hat [3][nzp][nyp][cnxp] complex,
+    cov0_hat / ec_hat the Fourier-domain covariance grid 0 and errCorr.
+    Returns nt complex128 arrays [nzp][nyp][cnxp]."""
+    cfg = prob.cfg
+    nt = prob.ntheta
+    rng = np.random.default_rng(seed)
+    shape = cov0_hat.shape
+    out = [np.zeros(shape, np.complex128) for _ in range(nt)]
+    L0 = np.zeros((3, 3))
+    for q, (a, b) in enumerate([(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]):
+        L0[a, b] = chol0[q]
+    Sig0 = L0 @ L0.T
+    bl = np.sqrt(np.diag(prob.err_theta_cov))
+    rho = prob.err_theta_cov / np.outer(bl, bl)
+    sd = np.sqrt(np.asarray(noise_var, np.float64))
+    Se = np.outer(sd, sd) * rho
+    c0 = 0.5 * (1 - cfg.wnc)
+    for k in np.nonzero(tables["invert"])[0]:
+        kw = (tables["kWD"][k, :, 0] + 1j * tables["kWD"][k, :, 1]) * np.asarray(scales)
+        e1 = tables["e1"][k, :, 0] + 1j * tables["e1"][k, :, 1]
+        e2 = tables["e2"][k, :, 0] + 1j * tables["e2"][k, :, 1]
+        e3 = tables["e3"][k, :, 0].astype(np.float64)
+        K = kw[:, None] * prob.A.astype(np.float64)
+        g = np.outer(e1, e1.conj()) + np.outer(e2, e2.conj())
+        Ec = c0 * Se * g
+        Ec = Ec - 1j * np.diag(np.diag(Ec).imag)
+        Ed = np.diag(cfg.wnc * np.diag(Se) * e3 ** 2)
+        cabs = np.abs(cov0_hat[k].real / prob.sigma0[0, 0]).reshape(-1)
+        lam = np.abs(ec_hat[k].real).reshape(-1)
+        M = (cabs[:, None, None] * (K @ Sig0 @ K.conj().T)[None] + lam[:, None, None] * Ec[None] + Ed[None])
+        M = 0.5 * (M + np.conj(np.swapaxes(M, 1, 2)))
+        L = np.linalg.cholesky(M)
+        zeta = (rng.standard_normal((M.shape[0], nt)) + 1j * rng.standard_normal((M.shape[0], nt))) / np.sqrt(2)
+        mu_k = np.stack([m[k].reshape(-1) for m in mu_hat], -1)
+        d = (mu_k @ K.T) + np.einsum("plm,pm->pl", L, zeta)
+        for l in range(nt):
+            out[l][k] = d[:, l].reshape(shape[1:])
+    return out

@@ -131,6 +131,27 @@ int crava_hip_post_destroy(crava_hip_post *post);
 int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, void *const *data, void *const *cov,
                        const void *errcorr, long long *n_fail, long long *n_fp64);
 
+/* Negative log marginal likelihood of the data and its gradient (no
+ * reference counterpart; the model is the one the posterior loop assumes,
+ * src/crava.cpp:1087-1130):
+ *   NLL = 1/2 sum over in-band points of w [r^H M^-1 r + log det M] (+ const),
+ *   r = d - K mu,  M = K S K^H + E,  w = 1 on self-conjugate x planes, else 2,
+ * with parameters
+ *   scales[nt]    per-angle wavelet scale s_l (K row l = s_l K0_l, K0 from the
+ *                 per-k vectors given to crava_hip_post_create),
+ *   noise_var[nt] per-angle noise variance v_l (Sigma_e,lm = sqrt(v_l v_m) rho_lm,
+ *                 rho from the params' err_theta_cov),
+ *   chol0[6]      lower Cholesky factor of Sigma0 (L00, L10, L11, L20, L21, L22);
+ *                 S(p) = |c(p)| L0 L0^T with |c(p)| = |Re cov[0](p)| / params Sigma0_aa.
+ * Grids are the Fourier-domain mu[3], data[nt], cov[6] (only cov[0] is read)
+ * and errcorr, float (precision CRAVA_HIP_F32) or double (CRAVA_HIP_F64).
+ * Outputs (host): *nll and grad[2*nt+6] = dNLL/d(scales, noise_var, chol0).
+ * Arithmetic is FP64; Cholesky factors are recomputed per point.  Partial
+ * sums are reduced in a fixed order, so repeated calls give identical results. */
+int crava_hip_nll_grad(crava_hip_post *post, int precision, void *const *mu, void *const *data, void *const *cov,
+                       const void *errcorr, const double *scales, const double *noise_var, const double *chol0,
+                       double *nll, double *grad);
+
 /* Fill a half-complex float grid with white complex Gaussian noise in place
  * of FFTGrid::fillInComplexNoise (src/fftgrid.cpp:1214-1263): CN(0,1) per
  * point, real N(0,1) at self-conjugate points, conjugate pairs on the
