@@ -344,7 +344,8 @@ def model_data(prob: Problem, tables: dict, mu_hat: np.ndarray, cov0_hat: np.nda
     rho = prob.err_theta_cov / np.outer(bl, bl)
     sd = np.sqrt(np.asarray(noise_var, np.float64))
     Se = np.outer(sd, sd) * rho
-    c0 = 0.5 * (1 - cfg.wnc)
+    wnc = float(np.float32(cfg.wnc))     # the interface carries wnc as a float
+    c0 = 0.5 * (1 - wnc)
     for k in np.nonzero(tables["invert"])[0]:
         kw = (tables["kWD"][k, :, 0] + 1j * tables["kWD"][k, :, 1]) * np.asarray(scales)
         e1 = tables["e1"][k, :, 0] + 1j * tables["e1"][k, :, 1]
@@ -354,7 +355,7 @@ def model_data(prob: Problem, tables: dict, mu_hat: np.ndarray, cov0_hat: np.nda
         g = np.outer(e1, e1.conj()) + np.outer(e2, e2.conj())
         Ec = c0 * Se * g
         Ec = Ec - 1j * np.diag(np.diag(Ec).imag)
-        Ed = np.diag(cfg.wnc * np.diag(Se) * e3 ** 2)
+        Ed = np.diag(wnc * np.diag(Se) * e3 ** 2)
         cabs = np.abs(cov0_hat[k].real / prob.sigma0[0, 0]).reshape(-1)
         lam = np.abs(ec_hat[k].real).reshape(-1)
         M = (cabs[:, None, None] * (K @ Sig0 @ K.conj().T)[None] + lam[:, None, None] * Ec[None] + Ed[None])

@@ -43,7 +43,8 @@ def prepare(tabs, A, err_theta_cov, wnc, base00, mu, data, cov0, errcorr, nxp):
     rho = err_theta_cov / np.outer(bl, bl)
     return dict(d=d, m=m, cabs=cabs, lam=lam, w=w, kwd=kw("kWD"), e1=kw("e1"), e2=kw("e2"),
                 e3=torch.from_numpy(tabs["e3"][ks, :, 0].astype(np.float64)),
-                A=torch.from_numpy(A.astype(np.float64)), rho=torch.from_numpy(rho), wnc=float(wnc), nt=nt)
+                A=torch.from_numpy(A.astype(np.float64)), rho=torch.from_numpy(rho),
+                wnc=float(np.float32(wnc)), nt=nt)   # wnc is a float in the interface
 
 
 def nll(P, scales: torch.Tensor, noise_var: torch.Tensor, chol0: torch.Tensor) -> torch.Tensor:
@@ -57,8 +58,10 @@ def nll(P, scales: torch.Tensor, noise_var: torch.Tensor, chol0: torch.Tensor) -
     Se = sd[:, None] * sd[None, :] * P["rho"]
     K = (P["kwd"] * scales)[..., :, None] * P["A"]                       # [nk, nt, 3]
     S = P["cabs"][..., None, None] * Sig0                                 # [nk, plane, 3, 3]
-    KS = torch.einsum("kla,kpab->kplb", K.to(torch.complex128), S.to(torch.complex128))
-    Ms = torch.einsum("kplb,kmb->kplm", KS, K.conj())
+    # elementwise products and sums only: batched complex BLAS paths lose ~1e-9
+    Kc = K.to(torch.complex128)
+    KS = (Kc[:, None, :, :, None] * S.to(torch.complex128)[:, :, None, :, :]).sum(-2)      # [nk, p, nt, 3]
+    Ms = (KS[:, :, :, None, :] * Kc.conj()[:, None, None, :, :]).sum(-1)                  # [nk, p, nt, nt]
     g = P["e1"][..., :, None] * P["e1"].conj()[..., None, :] + P["e2"][..., :, None] * P["e2"].conj()[..., None, :]
     c0 = 0.5 * (1.0 - wnc)
     Ec = c0 * Se * g                                                      # [nk, nt, nt]
@@ -67,7 +70,7 @@ def nll(P, scales: torch.Tensor, noise_var: torch.Tensor, chol0: torch.Tensor) -
     Ed = wnc * torch.diagonal(Se) * P["e3"] ** 2                          # [nk, nt]
     M = Ms + P["lam"][..., None, None] * Ec[:, None] + torch.diag_embed(Ed)[:, None].to(torch.complex128)
     M = 0.5 * (M + M.conj().transpose(-1, -2))
-    r = P["d"] - torch.einsum("klc,kpc->kpl", K.to(torch.complex128), P["m"])
+    r = P["d"] - (Kc[:, None, :, :] * P["m"][:, :, None, :]).sum(-1)
     L = torch.linalg.cholesky(M)
     z = torch.linalg.solve_triangular(L, r[..., None], upper=False)[..., 0]
     quad = (z.abs() ** 2).sum(-1)

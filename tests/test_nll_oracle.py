@@ -55,8 +55,9 @@ def test_nll_oracle_equals_dense_gaussian_tiny():
     Kb = lambda l, c: Fi @ np.diag(kw[:, l] * prob.A[l, c]) @ F
     G = np.block([[Kb(l, c) for c in range(3)] for l in range(nt)])
     Cm = np.block([[Fi @ np.diag(cabs * Sig0[a, b]) @ F for b in range(3)] for a in range(3)])
-    Eb = lambda l, m: (0.5 * (1 - cfg.wnc) * Se[l, m] * lam * (e1[:, l] * e1[:, m].conj() + e2[:, l] * e2[:, m].conj())
-                       + (cfg.wnc * Se[l, l] if l == m else 0.0))
+    wnc = float(np.float32(cfg.wnc))
+    Eb = lambda l, m: (0.5 * (1 - wnc) * Se[l, m] * lam * (e1[:, l] * e1[:, m].conj() + e2[:, l] * e2[:, m].conj())
+                       + (wnc * Se[l, l] if l == m else 0.0))
     Ce = np.block([[Fi @ np.diag(Eb(l, m)) @ F for m in range(nt)] for l in range(nt)])
     Cd = G @ Cm @ G.conj().T + Ce
     Cd = 0.5 * (Cd + Cd.conj().T)
