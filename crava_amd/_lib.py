@@ -42,6 +42,16 @@ VP = C.c_void_p
 I = C.c_int
 LL = C.c_longlong
 
+
+class BatchVolume(C.Structure):
+    _fields_ = [
+        ("post", VP),
+        ("mu", VP * 3),
+        ("data", VP * 8),
+        ("cov", VP * 6),
+        ("errcorr", VP),
+    ]
+
 # name -> (restype, argtypes); every function declared in include/crava_hip.h
 SIGNATURES = {
     "crava_hip_version": (I, []),
@@ -65,8 +75,13 @@ SIGNATURES = {
                              C.POINTER(VP), VP, VP]),
     "crava_hip_nll_grad": (I, [VP, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP, VP, VP, VP,
                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
-    "crava_hip_batch_create": (I, [VP, I, VP, C.POINTER(VP)]),
+    "crava_hip_fft_r2c_many": (I, [VP, C.POINTER(VP), I, I, I, I, I, I]),
+    "crava_hip_fft_c2r_many": (I, [VP, C.POINTER(VP), I, I, I, I, I, I]),
+    "crava_hip_batch_create": (I, [VP, I, I, C.POINTER(BatchVolume), C.POINTER(VP)]),
+    "crava_hip_batch_autotune": (I, [VP, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(C.c_float)]),
+    "crava_hip_batch_set_launch": (I, [VP, I, I, I]),
     "crava_hip_batch_run": (I, [VP, I]),
+    "crava_hip_batch_counts": (I, [VP, C.POINTER(LL), C.POINTER(LL)]),
     "crava_hip_batch_destroy": (I, [VP]),
     "crava_hip_autotune_fft": (I, [VP, VP, I, I, I, I, C.POINTER(I), C.POINTER(C.c_float)]),
     "crava_hip_fft_set_variants": (I, [VP, I, I, I, I, C.POINTER(I)]),

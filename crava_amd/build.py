@@ -1,9 +1,11 @@
 """Build libcrava_hip.so in-tree with hipcc for gfx950.
 
     python -m crava_amd.build            # or crava_amd.build.build()
+    python -m crava_amd.build --incremental
 
 Objects go to crava_amd/_build/ (git-ignored); the shared library is
-crava_amd/libcrava_hip.so.  Every call recompiles.
+crava_amd/libcrava_hip.so.  build() recompiles every source; --incremental
+skips objects newer than their source and every header.
 """
 from __future__ import annotations
 
@@ -54,8 +56,17 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src: str, extra: list[str]) -> str:
+def _newest_dependency(src: str) -> float:
+    deps = [os.path.join(CSRC, src), os.path.abspath(__file__),
+            os.path.join(HERE, "..", "include", "crava_hip.h")]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    return max(os.path.getmtime(d) for d in deps if os.path.exists(d))
+
+
+def _compile(src: str, extra: list[str], incremental: bool = False) -> str:
     obj = os.path.join(BUILD, src + ".o")
+    if incremental and os.path.exists(obj) and os.path.getmtime(obj) > _newest_dependency(src):
+        return obj
     cmd = [_hipcc(), *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -63,12 +74,12 @@ def _compile(src: str, extra: list[str]) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, incremental: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     jobs = int(os.environ.get("MAX_JOBS", "8"))
     with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(srcs)))) as ex:
-        objs = list(ex.map(lambda s: _compile(s, SOURCES[s]), srcs))
+        objs = list(ex.map(lambda s: _compile(s, SOURCES[s], incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -81,5 +92,5 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

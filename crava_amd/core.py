@@ -71,6 +71,17 @@ class Context:
         p = precision_of(grid) if precision is None else precision
         check(lib().crava_hip_fft_c2r(self.h, _dev_ptr(grid), nxp, nyp, nzp, cubetype, p), "crava_hip_fft_c2r")
 
+    def r2c_many(self, grids, nxp: int, nyp: int, nzp: int, cubetype: int, precision: int | None = None):
+        """One transform launch per pass for a list of same-size grids."""
+        p = precision_of(grids[0]) if precision is None else precision
+        check(lib().crava_hip_fft_r2c_many(self.h, ptr_array([_dev_ptr(g) for g in grids]), len(grids), nxp, nyp,
+                                           nzp, cubetype, p), "crava_hip_fft_r2c_many")
+
+    def c2r_many(self, grids, nxp: int, nyp: int, nzp: int, cubetype: int, precision: int | None = None):
+        p = precision_of(grids[0]) if precision is None else precision
+        check(lib().crava_hip_fft_c2r_many(self.h, ptr_array([_dev_ptr(g) for g in grids]), len(grids), nxp, nyp,
+                                           nzp, cubetype, p), "crava_hip_fft_c2r_many")
+
 
 def perk_tables(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wavelets: np.ndarray, wnorm: np.ndarray):
     """Host per-k vectors (crava_hip_perk_tables)."""
@@ -138,3 +149,57 @@ def autotune_fft(ctx: Context, scratch: torch.Tensor, nxp: int, nyp: int, nzp: i
 def set_fft_variants(ctx: Context, nxp: int, nyp: int, nzp: int, precision: int, variants):
     v = (C.c_int * 3)(*variants)
     check(lib().crava_hip_fft_set_variants(ctx.h, nxp, nyp, nzp, precision, v), "crava_hip_fft_set_variants")
+
+
+class Batch:
+    """crava_hip_batch over volumes given as (Posterior, mu[3], data[nt], cov[6], errcorr)
+    of time-domain GPU grids.  The grids are registered: keep them alive."""
+
+    def __init__(self, ctx: Context, precision: int, volumes):
+        self.ctx = ctx
+        self.volumes = list(volumes)
+        arr = (L.BatchVolume * len(self.volumes))()
+        for i, (post, mu, data, cov, ec) in enumerate(self.volumes):
+            v = arr[i]
+            v.post = post.h.value
+            for c in range(3):
+                v.mu[c] = _dev_ptr(mu[c])
+            for l, g in enumerate(data):
+                v.data[l] = _dev_ptr(g)
+            for q in range(6):
+                v.cov[q] = _dev_ptr(cov[q])
+            v.errcorr = _dev_ptr(ec)
+        self.h = C.c_void_p()
+        check(lib().crava_hip_batch_create(ctx.h, precision, len(self.volumes), arr, C.byref(self.h)),
+              "crava_hip_batch_create")
+
+    def close(self):
+        if self.h:
+            lib().crava_hip_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self):
+        """Returns (points_per_block, blocks, per_class, posterior ms)."""
+        p, b, c, ms = C.c_int(0), C.c_int(0), C.c_int(0), C.c_float(0)
+        check(lib().crava_hip_batch_autotune(self.h, C.byref(p), C.byref(b), C.byref(c), C.byref(ms)),
+              "crava_hip_batch_autotune")
+        return p.value, b.value, bool(c.value), ms.value
+
+    def set_launch(self, points_per_block: int, blocks: int, per_class: bool = True):
+        check(lib().crava_hip_batch_set_launch(self.h, points_per_block, blocks, int(per_class)),
+              "crava_hip_batch_set_launch")
+
+    def run(self, graph: bool = True):
+        check(lib().crava_hip_batch_run(self.h, int(graph)), "crava_hip_batch_run")
+
+    def counts(self):
+        n = len(self.volumes)
+        nf, n64 = (C.c_longlong * n)(), (C.c_longlong * n)()
+        check(lib().crava_hip_batch_counts(self.h, nf, n64), "crava_hip_batch_counts")
+        return list(nf), list(n64)

@@ -1,0 +1,75 @@
+// Many inversion volumes of uneven size and ntheta as one captured sequence:
+// per volume the forward transforms of (data, mean) and (covariances, error
+// correlation) as two grid sets, one persistent posterior launch over every
+// volume, then the inverse transforms of (mean, residuals) and covariances.
+// The sequence runs eagerly or as a hipGraph captured once and replayed.
+#pragma once
+#include "common.hpp"
+#include "context.hpp"
+#include "posterior.hpp"
+
+#include <vector>
+
+namespace crava {
+
+struct BatchVolume {
+  const PostDevice *pd = nullptr;
+  void *mu[3] = {nullptr, nullptr, nullptr};
+  void *data[kMaxTheta] = {nullptr};
+  void *cov[6] = {nullptr};
+  void *errcorr = nullptr;
+};
+
+class BatchExec {
+ public:
+  static constexpr int kStreams = 4;
+  BatchExec(Context *ctx, int precision) : ctx_(ctx), precision_(precision) {}
+  ~BatchExec();
+  int init(const BatchVolume *vols, int n);
+  // One pass of the whole sequence on `s`; graph: capture on first use, replay after.
+  int run(hipStream_t s, bool graph);
+  // Choose FFT variants per grid shape and the posterior launch (one launch
+  // or one per ntheta class x points per block x workgroups) by timing; the
+  // registered grids are restored.
+  int autotune(hipStream_t s, int *ppb, int *blocks, int *per_class, float *post_ms);
+  int set_launch(int ppb, int blocks, bool per_class);
+  // Per-volume counters of the last run (synchronises `s`).
+  int counts(hipStream_t s, long long *n_fail, long long *n_fp64);
+  int nvol() const { return (int)vols_.size(); }
+
+ private:
+  struct Set {
+    Fft3D *plan = nullptr;
+    int ng = 0;
+    void *grids[kMaxGrids] = {nullptr};
+    double scale[kMaxGrids] = {0};
+  };
+  struct Work {
+    Set fwd[2], inv[2];
+    int stream = 0;
+  };
+  int build_sets();
+  int enqueue(hipStream_t origin, bool transforms_only, bool forward_only);
+  int launch_post(hipStream_t origin);
+  int fork(hipStream_t origin);
+  int join(hipStream_t origin);
+  int backup_grids(hipStream_t s, bool restore);
+  void drop_graph();
+
+  Context *ctx_;
+  int precision_;
+  std::vector<BatchVolume> vols_;
+  std::vector<Work> work_;
+  PostBatch post_;
+  unsigned long long *counters_ = nullptr;  // [nvol][2]
+  hipStream_t side_[kStreams] = {nullptr};
+  hipEvent_t fork_ev_ = nullptr;
+  hipEvent_t join_ev_[kStreams] = {nullptr};
+  hipStream_t cap_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  void *backup_ = nullptr;
+  size_t backup_bytes_ = 0;
+};
+
+}  // namespace crava

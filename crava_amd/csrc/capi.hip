@@ -1,5 +1,6 @@
 // C ABI of libcrava_hip.so (declared in include/crava_hip.h).
 #include "../../include/crava_hip.h"
+#include "batch.hpp"
 #include "common.hpp"
 #include "context.hpp"
 #include "fft.hpp"
@@ -104,39 +105,52 @@ int crava_hip_synchronize(crava_hip_ctx *ctx) {
   return CRAVA_HIP_OK;
 }
 
-static int fft_common(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int cubetype, int precision,
-                      bool inverse) {
-  if (!ctx || !grid) { set_error("NULL ctx or grid"); return CRAVA_HIP_E_INVALID; }
+static double fft_scale(int cubetype, int precision, long long N, bool inverse) {
+  return crava::grid_fft_scale(cubetype == CRAVA_HIP_COVARIANCE, precision == CRAVA_HIP_F64, N, inverse);
+}
+
+static int fft_common(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp, int nyp, int nzp, int cubetype,
+                      int precision, bool inverse) {
+  if (!ctx || !grids) { set_error("NULL ctx or grid list"); return CRAVA_HIP_E_INVALID; }
+  if (ngrid < 1 || ngrid > crava::kMaxGrids) {
+    set_error("ngrid must be 1.." + std::to_string(crava::kMaxGrids));
+    return CRAVA_HIP_E_INVALID;
+  }
   if (nxp < 2 || nyp < 1 || nzp < 1) { set_error("invalid grid size"); return CRAVA_HIP_E_INVALID; }
   if (cubetype < CRAVA_HIP_DATA || cubetype > CRAVA_HIP_COVARIANCE) { set_error("invalid cubetype"); return CRAVA_HIP_E_INVALID; }
   if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) {
     set_error("invalid transform precision");
     return CRAVA_HIP_E_INVALID;
   }
+  for (int i = 0; i < ngrid; ++i)
+    if (!grids[i]) { set_error("NULL grid"); return CRAVA_HIP_E_INVALID; }
   crava::Fft3D *f = nullptr;
   int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
   if (rc) return rc;
-  const long long N = (long long)nxp * nyp * nzp;
-  double scale;
-  if (!inverse) {
-    // FFTGrid::fftInPlace multiplies by 1.0f/sqrt(float(N)) before the transform
-    scale = cubetype == CRAVA_HIP_COVARIANCE ? 1.0
-            : (precision == CRAVA_HIP_F64 ? 1.0 / std::sqrt((double)N) : (double)(1.0f / std::sqrt((float)N)));
-    return f->r2c(grid, ctx->stream, scale);
-  }
-  if (cubetype == CRAVA_HIP_COVARIANCE)
-    scale = precision == CRAVA_HIP_F64 ? 1.0 / (double)N : (double)(float)(1.0 / (double)N);
-  else
-    scale = precision == CRAVA_HIP_F64 ? 1.0 / std::sqrt((double)N) : (double)(float)(1.0 / std::sqrt((float)N));
-  return f->c2r(grid, ctx->stream, scale);
+  double scale[crava::kMaxGrids];
+  const double sc = fft_scale(cubetype, precision, (long long)nxp * nyp * nzp, inverse);
+  for (int i = 0; i < ngrid; ++i) scale[i] = sc;
+  return inverse ? f->c2r(grids, ngrid, scale, ctx->stream) : f->r2c(grids, ngrid, scale, ctx->stream);
 }
 
 int crava_hip_fft_r2c(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int cubetype, int precision) {
-  return fft_common(ctx, grid, nxp, nyp, nzp, cubetype, precision, false);
+  if (!grid) { set_error("NULL grid"); return CRAVA_HIP_E_INVALID; }
+  return fft_common(ctx, &grid, 1, nxp, nyp, nzp, cubetype, precision, false);
 }
 
 int crava_hip_fft_c2r(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int cubetype, int precision) {
-  return fft_common(ctx, grid, nxp, nyp, nzp, cubetype, precision, true);
+  if (!grid) { set_error("NULL grid"); return CRAVA_HIP_E_INVALID; }
+  return fft_common(ctx, &grid, 1, nxp, nyp, nzp, cubetype, precision, true);
+}
+
+int crava_hip_fft_r2c_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp, int nyp, int nzp,
+                           int cubetype, int precision) {
+  return fft_common(ctx, grids, ngrid, nxp, nyp, nzp, cubetype, precision, false);
+}
+
+int crava_hip_fft_c2r_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp, int nyp, int nzp,
+                           int cubetype, int precision) {
+  return fft_common(ctx, grids, ngrid, nxp, nyp, nzp, cubetype, precision, true);
 }
 
 int crava_hip_fft_pass(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int axis, int inverse,
@@ -146,9 +160,7 @@ int crava_hip_fft_pass(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp
   crava::Fft3D *f = nullptr;
   int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
   if (rc) return rc;
-  if (axis == 0) return f->pass_x(grid, ctx->stream, inverse != 0, scale);
-  if (axis == 1) return f->pass_y(grid, ctx->stream, inverse != 0);
-  return f->pass_z(grid, ctx->stream, inverse != 0);
+  return f->pass(axis, &grid, 1, inverse != 0, &scale, ctx->stream);
 }
 
 int crava_hip_autotune_fft(crava_hip_ctx *ctx, void *scratch, int nxp, int nyp, int nzp, int precision,
@@ -159,7 +171,7 @@ int crava_hip_autotune_fft(crava_hip_ctx *ctx, void *scratch, int nxp, int nyp, 
   int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
   if (rc) return rc;
   float t[3] = {0, 0, 0};
-  if ((rc = f->autotune(scratch, ctx->stream, 5, t))) return rc;
+  if ((rc = f->autotune(&scratch, 1, ctx->stream, 5, t))) return rc;
   for (int a = 0; a < 3; ++a) {
     if (variants) variants[a] = f->var[a];
     if (ms) ms[a] = t[a];
@@ -423,8 +435,8 @@ int crava_hip_simulate(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, void *cons
   long long nf = 0;
   int rc = crava_hip_simulate_points(ctx, npts, cov, seed, n_fail ? &nf : nullptr);
   if (rc) return rc;
+  if ((rc = crava_hip_fft_c2r_many(ctx, seed, 3, nxp, nyp, nzp, CRAVA_HIP_PARAMETER, CRAVA_HIP_F32))) return rc;
   for (int c = 0; c < 3; ++c) {
-    if ((rc = crava_hip_fft_c2r(ctx, seed[c], nxp, nyp, nzp, CRAVA_HIP_PARAMETER, CRAVA_HIP_F32))) return rc;
     if (mean && mean[c]) {
       const long long rsize = (long long)2 * (nxp / 2 + 1) * nyp * nzp;
       if ((rc = crava::launch_add_real(reinterpret_cast<float *>(seed[c]), reinterpret_cast<const float *>(mean[c]),
@@ -459,3 +471,59 @@ int crava_hip_facies(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int ny
 }
 
 }  // extern "C"
+
+struct crava_hip_batch {
+  crava_hip_ctx *ctx;
+  crava::BatchExec exec;
+  crava_hip_batch(crava_hip_ctx *c, int precision) : ctx(c), exec(c, precision) {}
+};
+
+int crava_hip_batch_create(crava_hip_ctx *ctx, int precision, int nvol, const crava_hip_batch_volume *vols,
+                           crava_hip_batch **batch) {
+  if (!ctx || !vols || !batch || nvol < 1) { set_error("invalid arguments to crava_hip_batch_create"); return CRAVA_HIP_E_INVALID; }
+  *batch = nullptr;
+  std::vector<crava::BatchVolume> bv((size_t)nvol);
+  for (int v = 0; v < nvol; ++v) {
+    if (!vols[v].post) { set_error("volume " + std::to_string(v) + " has no problem"); return CRAVA_HIP_E_INVALID; }
+    bv[v].pd = &vols[v].post->dev;
+    for (int c = 0; c < 3; ++c) bv[v].mu[c] = vols[v].mu[c];
+    for (int l = 0; l < crava::kMaxTheta; ++l) bv[v].data[l] = vols[v].data[l];
+    for (int q = 0; q < 6; ++q) bv[v].cov[q] = vols[v].cov[q];
+    bv[v].errcorr = vols[v].errcorr;
+  }
+  crava_hip_batch *b = new (std::nothrow) crava_hip_batch(ctx, precision);
+  if (!b) { set_error("out of host memory"); return CRAVA_HIP_E_OOM; }
+  const int rc = b->exec.init(bv.data(), nvol);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *batch = b;
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_batch_autotune(crava_hip_batch *batch, int *points_per_block, int *blocks, int *per_class,
+                             float *post_ms) {
+  if (!batch) { set_error("batch is NULL"); return CRAVA_HIP_E_INVALID; }
+  return batch->exec.autotune(batch->ctx->stream, points_per_block, blocks, per_class, post_ms);
+}
+
+int crava_hip_batch_set_launch(crava_hip_batch *batch, int points_per_block, int blocks, int per_class) {
+  if (!batch) { set_error("batch is NULL"); return CRAVA_HIP_E_INVALID; }
+  return batch->exec.set_launch(points_per_block, blocks, per_class != 0);
+}
+
+int crava_hip_batch_run(crava_hip_batch *batch, int use_graph) {
+  if (!batch) { set_error("batch is NULL"); return CRAVA_HIP_E_INVALID; }
+  return batch->exec.run(batch->ctx->stream, use_graph != 0);
+}
+
+int crava_hip_batch_counts(crava_hip_batch *batch, long long *n_fail, long long *n_fp64) {
+  if (!batch) { set_error("batch is NULL"); return CRAVA_HIP_E_INVALID; }
+  return batch->exec.counts(batch->ctx->stream, n_fail, n_fp64);
+}
+
+int crava_hip_batch_destroy(crava_hip_batch *batch) {
+  delete batch;
+  return CRAVA_HIP_OK;
+}

@@ -74,10 +74,11 @@ __device__ void run_passes(cx<T> *buf, const cx<T> *tw, const PassPlan &pl) {
 // apart; blockIdx.y selects a batch `bstride` apart.
 // ---------------------------------------------------------------------------
 template <int DIR, typename T, typename SI, typename SO>
-__global__ void __launch_bounds__(1024) k_fft_strided(const cx<SI> *src, cx<SO> *dst, long long es, long long ncols,
-                                                      long long bstride, PassPlan pl,
-                                                      const cx<T> *__restrict__ twg) {
+__global__ void __launch_bounds__(1024) k_fft_strided(GridSet gs, long long es, long long ncols, long long bstride,
+                                                      PassPlan pl, const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
+  const cx<SI> *src = reinterpret_cast<const cx<SI> *>(gs.src[blockIdx.z]);
+  cx<SO> *dst = reinterpret_cast<cx<SO> *>(gs.dst[blockIdx.z]);
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + pl.n;
   const int nthr = blockDim.x;
@@ -115,9 +116,12 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const cx<SI> *src, cx<SO> 
 // A_k = (Z_k + conj Z_{n-k})/2, B_k = (Z_k - conj Z_{n-k})/(2i).
 // ---------------------------------------------------------------------------
 template <typename T, typename SI, typename SO>
-__global__ void __launch_bounds__(1024) k_fft_rows_r2c(const SI *src, SO *dst, long long nrows, int rnxp,
-                                                       PassPlan pl, const cx<T> *__restrict__ twg, T scale) {
+__global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nrows, int rnxp, PassPlan pl,
+                                                       const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
+  const SI *src = reinterpret_cast<const SI *>(gs.src[blockIdx.z]);
+  SO *dst = reinterpret_cast<SO *>(gs.dst[blockIdx.z]);
+  const T scale = (T)gs.scale[blockIdx.z];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + pl.n;
   const int n = pl.n;
@@ -159,9 +163,12 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(const SI *src, SO *dst, l
 // (Hermitian extension, Im of bins 0 and n/2 ignored as in a halfcomplex
 // c2r), inverse DFT, real part -> row a, imaginary part -> row b.
 template <typename T, typename SI, typename SO>
-__global__ void __launch_bounds__(1024) k_fft_rows_c2r(const SI *src, SO *dst, long long nrows, int rnxp,
-                                                       PassPlan pl, const cx<T> *__restrict__ twg, T scale) {
+__global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nrows, int rnxp, PassPlan pl,
+                                                       const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
+  const SI *src = reinterpret_cast<const SI *>(gs.src[blockIdx.z]);
+  SO *dst = reinterpret_cast<SO *>(gs.dst[blockIdx.z]);
+  const T scale = (T)gs.scale[blockIdx.z];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + pl.n;
   const int n = pl.n;
@@ -220,12 +227,12 @@ int launch_convert(int io, const void *src, void *dst, long long ncomplex, hipSt
   return CRAVA_HIP_OK;
 }
 
-bool factor_radices(int n, PassPlan &p) {
+bool factor_radices(int n, PassPlan &p, int order) {
   const RadixList L = make_radices(n);
   p.n = n;
   p.nrad = L.cnt;
   p.rmin = L.rmin;
-  for (int i = 0; i < L.cnt; ++i) p.rad[i] = L.r[i];
+  for (int i = 0; i < L.cnt; ++i) p.rad[i] = order ? L.r[L.cnt - 1 - i] : L.r[i];
   return L.ok;
 }
 
@@ -242,6 +249,10 @@ static int max_lds_per_block() {
   return v;
 }
 
+// Generic-kernel variant g: radix order (g / 3: 0 large radices first, 1
+// small first) x cap on pencils per workgroup (g % 3).
+static const int kGenericTxCap[3] = {64, 16, 4};
+
 Fft3D::Fft3D(int nx, int ny, int nz, int p) : nxp(nx), nyp(ny), nzp(nz), prec(p), dbl(p >= 1) {}
 
 Fft3D::~Fft3D() {
@@ -251,8 +262,8 @@ Fft3D::~Fft3D() {
   if (twz) (void)hipFree(twz);
 }
 
-int Fft3D::build_plan(int n, PassPlan &p, int want_tx) {
-  if (!factor_radices(n, p)) {
+int Fft3D::build_plan(int n, PassPlan &p, int g) {
+  if (!factor_radices(n, p, g / 3)) {
     set_error("FFT length " + std::to_string(n) + " is not 7-smooth");
     return CRAVA_HIP_E_INVALID;
   }
@@ -265,17 +276,15 @@ int Fft3D::build_plan(int n, PassPlan &p, int want_tx) {
   p.ldsp = (n % 2 == 0) ? n + 1 : n;
   // keep two workgroups per CU: at most 80 KiB (or the device limit)
   const size_t budget = std::min<size_t>((size_t)max_lds_per_block(), 80 * 1024);
-  int tx_lds = (int)((budget / eb - (size_t)n) / (size_t)p.ldsp);
-  int tx_thr = 1024 / tpp;
-  int tx = std::min(tx_lds, tx_thr);
-  if (want_tx > 0) tx = std::min(tx, want_tx);
-  tx = std::min(tx, 64);
+  const int tx_lds = (int)((budget / eb - (size_t)n) / (size_t)p.ldsp);
+  const int tx_thr = 1024 / tpp;
+  const int tx = std::min(std::min(tx_lds, tx_thr), kGenericTxCap[g % 3]);
   if (tx < 1) {
     set_error("FFT length " + std::to_string(n) + " does not fit in LDS");
     return CRAVA_HIP_E_INVALID;
   }
   p.tx = tx;
-  int threads = tx * tpp;
+  const int threads = tx * tpp;
   p.threads = ((threads + 63) / 64) * 64;
   p.lds_bytes = eb * ((size_t)n + (size_t)tx * p.ldsp);
   return CRAVA_HIP_OK;
@@ -285,7 +294,7 @@ template <typename T>
 static int upload_twiddles(int n, void **dst) {
   std::vector<cx<T>> h(n);
   for (int m = 0; m < n; ++m) {
-    // exp(-2 pi i m / n), argument reduced to the first octant for accuracy
+    // exp(-2 pi i m / n) evaluated in long double
     long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
     h[m].x = (T)cosl(a);
     h[m].y = (T)sinl(a);
@@ -313,10 +322,7 @@ int Fft3D::init() {
         (rc = upload_twiddles<float>(nzp, &twz)))
       return rc;
   }
-  if (prec == 1) {  // double intermediates of the float-storage transform
-    const size_t bytes = sizeof(cx<double>) * (size_t)(nxp / 2 + 1) * nyp * nzp;
-    CRAVA_HIP_CHECK(hipMalloc(&scratch, bytes));
-  }
+  if (prec == 1 && (rc = ensure_scratch(1))) return rc;
   // specialised kernels where a length has them (variant 1 until tuned)
   for (int axis = 0; axis < 3; ++axis) {
     var[axis] = -1;
@@ -326,9 +332,15 @@ int Fft3D::init() {
   return CRAVA_HIP_OK;
 }
 
-int Fft3D::set_tx(int tx_x, int tx_y, int tx_z) {
-  int rc;
-  if ((rc = build_plan(nxp, px, tx_x)) || (rc = build_plan(nyp, py, tx_y)) || (rc = build_plan(nzp, pz, tx_z))) return rc;
+long long Fft3D::complex_elems() const { return (long long)(nxp / 2 + 1) * nyp * nzp; }
+
+int Fft3D::ensure_scratch(int ngrids) {
+  if (prec != 1 || ngrids <= scratch_grids) return CRAVA_HIP_OK;
+  if (scratch) (void)hipFree(scratch);
+  scratch = nullptr;
+  scratch_grids = 0;
+  CRAVA_HIP_CHECK(hipMalloc(&scratch, sizeof(cx<double>) * (size_t)complex_elems() * (size_t)ngrids));
+  scratch_grids = ngrids;
   return CRAVA_HIP_OK;
 }
 
@@ -342,131 +354,158 @@ static int set_lds(F kernel, size_t bytes) {
 }
 
 template <typename T, typename SI, typename SO>
-static int launch_x(const Fft3D &f, const void *src, void *dst, hipStream_t s, bool inverse, double scale) {
+static int launch_x(const Fft3D &f, const GridSet &gs, hipStream_t s, bool inverse) {
   const long long nrows = (long long)f.nyp * f.nzp;
   const int rnxp = 2 * (f.nxp / 2 + 1);
   const PassPlan &p = f.px;
   const unsigned blocks = (unsigned)((nrows + 2LL * p.tx - 1) / (2LL * p.tx));
-  const SI *ip = reinterpret_cast<const SI *>(src);
-  SO *op = reinterpret_cast<SO *>(dst);
   const cx<T> *tw = reinterpret_cast<const cx<T> *>(f.twx);
   int rc;
   if (!inverse) {
     if ((rc = set_lds(k_fft_rows_r2c<T, SI, SO>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_rows_r2c<T, SI, SO>), dim3(blocks), dim3(p.threads), p.lds_bytes, s, ip, op, nrows,
-                       rnxp, p, tw, (T)scale);
+    hipLaunchKernelGGL((k_fft_rows_r2c<T, SI, SO>), dim3(blocks, 1, (unsigned)gs.n), dim3(p.threads), p.lds_bytes, s,
+                       gs, nrows, rnxp, p, tw);
   } else {
     if ((rc = set_lds(k_fft_rows_c2r<T, SI, SO>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_rows_c2r<T, SI, SO>), dim3(blocks), dim3(p.threads), p.lds_bytes, s, ip, op, nrows,
-                       rnxp, p, tw, (T)scale);
+    hipLaunchKernelGGL((k_fft_rows_c2r<T, SI, SO>), dim3(blocks, 1, (unsigned)gs.n), dim3(p.threads), p.lds_bytes, s,
+                       gs, nrows, rnxp, p, tw);
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
 
 template <typename T, typename SI, typename SO>
-static int launch_strided(const PassPlan &p, const void *tw, const void *src, void *dst, hipStream_t s,
-                          bool inverse, long long es, long long ncols, long long nbatch, long long bstride) {
-  if (p.n <= 1) return CRAVA_HIP_OK;
+static int launch_strided(const PassPlan &p, const void *tw, const GridSet &gs, hipStream_t s, bool inverse,
+                          long long es, long long ncols, long long nbatch, long long bstride) {
   const unsigned gx = (unsigned)((ncols + p.tx - 1) / p.tx);
-  const cx<SI> *ip = reinterpret_cast<const cx<SI> *>(src);
-  cx<SO> *op = reinterpret_cast<cx<SO> *>(dst);
+  const dim3 grid(gx, (unsigned)nbatch, (unsigned)gs.n);
   const cx<T> *twp = reinterpret_cast<const cx<T> *>(tw);
   int rc;
   if (!inverse) {
     if ((rc = set_lds(k_fft_strided<-1, T, SI, SO>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_strided<-1, T, SI, SO>), dim3(gx, (unsigned)nbatch), dim3(p.threads), p.lds_bytes, s,
-                       ip, op, es, ncols, bstride, p, twp);
+    hipLaunchKernelGGL((k_fft_strided<-1, T, SI, SO>), grid, dim3(p.threads), p.lds_bytes, s, gs, es, ncols, bstride,
+                       p, twp);
   } else {
     if ((rc = set_lds(k_fft_strided<1, T, SI, SO>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_strided<1, T, SI, SO>), dim3(gx, (unsigned)nbatch), dim3(p.threads), p.lds_bytes, s,
-                       ip, op, es, ncols, bstride, p, twp);
+    hipLaunchKernelGGL((k_fft_strided<1, T, SI, SO>), grid, dim3(p.threads), p.lds_bytes, s, gs, es, ncols, bstride,
+                       p, twp);
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
 
 // Generic-kernel dispatch on the storage of one pass.
-static int generic_x(const Fft3D &f, int io, const void *src, void *dst, hipStream_t s, bool inv, double scale) {
+static int generic_x(const Fft3D &f, int io, const GridSet &gs, hipStream_t s, bool inv) {
   switch (io) {
-    case kIoF32: return launch_x<float, float, float>(f, src, dst, s, inv, scale);
-    case kIoF64: return launch_x<double, double, double>(f, src, dst, s, inv, scale);
-    case kIoF32toF64: return launch_x<double, float, double>(f, src, dst, s, inv, scale);
-    default: return launch_x<double, double, float>(f, src, dst, s, inv, scale);
+    case kIoF32: return launch_x<float, float, float>(f, gs, s, inv);
+    case kIoF64: return launch_x<double, double, double>(f, gs, s, inv);
+    case kIoF32toF64: return launch_x<double, float, double>(f, gs, s, inv);
+    default: return launch_x<double, double, float>(f, gs, s, inv);
   }
 }
 
-static int generic_strided(const PassPlan &p, const void *tw, int io, const void *src, void *dst, hipStream_t s,
-                           bool inv, long long es, long long ncols, long long nbatch, long long bstride) {
+static int generic_strided(const PassPlan &p, const void *tw, int io, const GridSet &gs, hipStream_t s, bool inv,
+                           long long es, long long ncols, long long nbatch, long long bstride) {
   switch (io) {
-    case kIoF32: return launch_strided<float, float, float>(p, tw, src, dst, s, inv, es, ncols, nbatch, bstride);
-    case kIoF64: return launch_strided<double, double, double>(p, tw, src, dst, s, inv, es, ncols, nbatch, bstride);
-    case kIoF32toF64: return launch_strided<double, float, double>(p, tw, src, dst, s, inv, es, ncols, nbatch, bstride);
-    default: return launch_strided<double, double, float>(p, tw, src, dst, s, inv, es, ncols, nbatch, bstride);
+    case kIoF32: return launch_strided<float, float, float>(p, tw, gs, s, inv, es, ncols, nbatch, bstride);
+    case kIoF64: return launch_strided<double, double, double>(p, tw, gs, s, inv, es, ncols, nbatch, bstride);
+    case kIoF32toF64: return launch_strided<double, float, double>(p, tw, gs, s, inv, es, ncols, nbatch, bstride);
+    default: return launch_strided<double, double, float>(p, tw, gs, s, inv, es, ncols, nbatch, bstride);
   }
 }
 
 // Storage of each pass.  Float and double transforms run in place.  The
-// float-storage transform with double arithmetic keeps its intermediates in
-// the plan's double scratch: forward x: grid(f32) -> scratch(f64), y in
-// scratch, z: scratch -> grid(f32); inverse z: grid -> scratch, y in
-// scratch, x: scratch -> grid.
-struct PassIo {
-  int io;
-  const void *src;
-  void *dst;
-};
-
-static PassIo pass_io(const Fft3D &f, void *grid, int axis, bool inverse) {
-  if (f.prec == 0) return {kIoF32, grid, grid};
-  if (f.prec == 2) return {kIoF64, grid, grid};
-  if (axis == 1) return {kIoF64, f.scratch, f.scratch};
-  const bool first = inverse ? axis == 2 : axis == 0;
-  if (first) return {kIoF32toF64, grid, f.scratch};
-  return {kIoF64toF32, f.scratch, grid};
-}
-
-int Fft3D::pass_x(void *grid, hipStream_t s, bool inverse, double scale) {
-  const PassIo io = pass_io(*this, grid, 0, inverse);
-  if (var[0] >= 0) {
-    const long long nrows = (long long)nyp * nzp;
-    return static_fft_launch(nxp, io.io, inverse ? kRowsC2R : kRowsR2C, var[0], io.src, io.dst, 0, 0, 0, 0, nrows,
-                             2 * (nxp / 2 + 1), scale, twx, s);
+// float-storage transform with double arithmetic keeps the intermediates of
+// grid i in slot i of the plan's double scratch: forward x: grid(f32) ->
+// scratch(f64), y in scratch, z: scratch -> grid(f32); inverse z: grid ->
+// scratch, y in scratch, x: scratch -> grid.
+int Fft3D::make_set(int axis, bool inverse, void *const *grids, int ng, const double *scale, GridSet &gs,
+                    int &io) const {
+  if (ng < 1 || ng > kMaxGrids) {
+    set_error("a transform set holds 1.." + std::to_string(kMaxGrids) + " grids");
+    return CRAVA_HIP_E_INVALID;
   }
-  return generic_x(*this, io.io, io.src, io.dst, s, inverse, scale);
+  if (prec == 1 && ng > scratch_grids) {
+    set_error("FFT scratch holds fewer grids than the set");
+    return CRAVA_HIP_E_STATE;
+  }
+  const bool first = inverse ? axis == 2 : axis == 0;
+  io = prec == 0 ? kIoF32 : prec == 2 ? kIoF64 : axis == 1 ? kIoF64 : first ? kIoF32toF64 : kIoF64toF32;
+  gs.n = ng;
+  for (int i = 0; i < ng; ++i) {
+    if (!grids[i]) {
+      set_error("NULL grid in a transform set");
+      return CRAVA_HIP_E_INVALID;
+    }
+    void *slot = prec == 1 ? static_cast<void *>(static_cast<cx<double> *>(scratch) + (size_t)i * complex_elems())
+                           : grids[i];
+    const void *src = grids[i];
+    void *dst = grids[i];
+    if (prec == 1) {
+      if (axis == 1) src = dst = slot;
+      else if (first) dst = slot;
+      else src = slot;
+    }
+    gs.src[i] = src;
+    gs.dst[i] = dst;
+    gs.scale[i] = scale ? scale[i] : 1.0;
+  }
+  for (int i = ng; i < kMaxGrids; ++i) {
+    gs.src[i] = nullptr;
+    gs.dst[i] = nullptr;
+    gs.scale[i] = 0.0;
+  }
+  return CRAVA_HIP_OK;
 }
 
-static int copy_between(const PassIo &io, long long nelem, hipStream_t s) {
+static int copy_between(int io, const GridSet &gs, long long nelem, hipStream_t s) {
   // a length-1 axis is the identity; still move the data between storages
-  if (io.src == io.dst) return CRAVA_HIP_OK;
-  return launch_convert(io.io, io.src, io.dst, nelem, s);
+  int rc;
+  for (int i = 0; i < gs.n; ++i)
+    if (gs.src[i] != gs.dst[i] && (rc = launch_convert(io, gs.src[i], gs.dst[i], nelem, s))) return rc;
+  return CRAVA_HIP_OK;
 }
 
-int Fft3D::pass_y(void *grid, hipStream_t s, bool inverse) {
+int Fft3D::pass(int axis, void *const *grids, int ng, bool inverse, const double *scale, hipStream_t s) {
+  GridSet gs;
+  int io, rc;
+  if ((rc = make_set(axis, inverse, grids, ng, scale, gs, io))) return rc;
   const long long cnxp = nxp / 2 + 1;
-  const PassIo io = pass_io(*this, grid, 1, inverse);
-  if (nyp <= 1) return copy_between(io, cnxp * nyp * nzp, s);
-  if (var[1] >= 0)
-    return static_fft_launch(nyp, io.io, inverse ? kStridedInv : kStridedFwd, var[1], io.src, io.dst, cnxp, cnxp,
-                             nzp, cnxp * nyp, 0, 0, 1.0, twy, s);
-  return generic_strided(py, twy, io.io, io.src, io.dst, s, inverse, cnxp, cnxp, nzp, cnxp * nyp);
-}
-
-int Fft3D::pass_z(void *grid, hipStream_t s, bool inverse) {
-  const long long cnxp = nxp / 2 + 1;
+  if (axis == 0) {
+    if (var[0] >= 0)
+      return static_fft_launch(nxp, io, inverse ? kRowsC2R : kRowsR2C, var[0], gs, 0, 0, 0, 0, (long long)nyp * nzp,
+                               (int)(2 * cnxp), twx, s);
+    return generic_x(*this, io, gs, s, inverse);
+  }
+  const FftKind kind = inverse ? kStridedInv : kStridedFwd;
+  if (axis == 1) {
+    if (nyp <= 1) return copy_between(io, gs, cnxp * nyp * nzp, s);
+    if (var[1] >= 0) return static_fft_launch(nyp, io, kind, var[1], gs, cnxp, cnxp, nzp, cnxp * nyp, 0, 0, twy, s);
+    return generic_strided(py, twy, io, gs, s, inverse, cnxp, cnxp, nzp, cnxp * nyp);
+  }
   const long long plane = cnxp * nyp;
-  const PassIo io = pass_io(*this, grid, 2, inverse);
-  if (nzp <= 1) return copy_between(io, plane * nzp, s);
-  if (var[2] >= 0)
-    return static_fft_launch(nzp, io.io, inverse ? kStridedInv : kStridedFwd, var[2], io.src, io.dst, plane, plane,
-                             1, 0, 0, 0, 1.0, twz, s);
-  return generic_strided(pz, twz, io.io, io.src, io.dst, s, inverse, plane, plane, 1, 0);
+  if (nzp <= 1) return copy_between(io, gs, plane * nzp, s);
+  if (var[2] >= 0) return static_fft_launch(nzp, io, kind, var[2], gs, plane, plane, 1, 0, 0, 0, twz, s);
+  return generic_strided(pz, twz, io, gs, s, inverse, plane, plane, 1, 0);
 }
 
 int Fft3D::set_variant(int axis, int v) {
   if (axis < 0 || axis > 2) return CRAVA_HIP_E_INVALID;
   const int n = axis == 0 ? nxp : (axis == 1 ? nyp : nzp);
-  if (v < 0) { var[axis] = -1; return CRAVA_HIP_OK; }
+  PassPlan &p = axis == 0 ? px : (axis == 1 ? py : pz);
+  if (v < 0) {
+    const int g = -1 - v;
+    if (g >= kNumGenericVariants) {
+      set_error("generic variant " + std::to_string(g) + " does not exist");
+      return CRAVA_HIP_E_INVALID;
+    }
+    PassPlan q;
+    int rc;
+    if ((rc = build_plan(n, q, g))) return rc;
+    p = q;
+    var[axis] = v;
+    return CRAVA_HIP_OK;
+  }
   const FftKind kind = axis == 0 ? kRowsR2C : kStridedFwd;
   if (!static_fft_geometry(n, dbl, kind, v, nullptr)) {
     set_error("variant " + std::to_string(v) + " unavailable for length " + std::to_string(n));
@@ -482,22 +521,24 @@ int Fft3D::num_variants(int axis) const {
   return axis == 0 ? kNumRowVariants : kNumStridedVariants;
 }
 
-// Time every launch variant of every axis on `grid` (contents are destroyed)
-// and keep the fastest.  Forward and inverse share a variant.
-int Fft3D::autotune(void *grid, hipStream_t s, int reps, float *ms_out) {
+// Time every launch variant of every axis (generic radix orders and pencil
+// counts, and the specialised kernels where the length has them) on the set
+// `grids` (contents are destroyed) and keep the fastest.  Forward and inverse
+// share a variant.
+int Fft3D::autotune(void *const *grids, int ng, hipStream_t s, int reps, float *ms_out) {
+  int rc;
+  if ((rc = ensure_scratch(ng))) return rc;
   hipEvent_t e0, e1;
   CRAVA_HIP_CHECK(hipEventCreate(&e0));
   CRAVA_HIP_CHECK(hipEventCreate(&e1));
-  int rc = CRAVA_HIP_OK;
+  rc = CRAVA_HIP_OK;
   for (int axis = 0; axis < 3 && rc == CRAVA_HIP_OK; ++axis) {
     const int nv = num_variants(axis);
     float best = 1e30f;
     int bestv = var[axis];
-    for (int v = -1; v < nv; ++v) {
+    for (int v = -kNumGenericVariants; v < nv; ++v) {
       if (set_variant(axis, v) != CRAVA_HIP_OK) continue;
-      auto run = [&]() {
-        return axis == 0 ? pass_x(grid, s, false, 1.0) : (axis == 1 ? pass_y(grid, s, false) : pass_z(grid, s, false));
-      };
+      auto run = [&]() { return pass(axis, grids, ng, false, nullptr, s); };
       if ((rc = run())) break;  // warm-up
       if (hipEventRecord(e0, s) != hipSuccess) { rc = CRAVA_HIP_E_HIP; break; }
       for (int r = 0; r < reps && rc == CRAVA_HIP_OK; ++r) rc = run();
@@ -509,27 +550,30 @@ int Fft3D::autotune(void *grid, hipStream_t s, int reps, float *ms_out) {
       if (ms < best) { best = ms; bestv = v; }
     }
     if (rc == CRAVA_HIP_OK) {
-      var[axis] = bestv;
+      rc = set_variant(axis, bestv);
       if (ms_out) ms_out[axis] = best;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  set_error("");
   return rc;
 }
 
-int Fft3D::r2c(void *grid, hipStream_t s, double scale) {
+int Fft3D::r2c(void *const *grids, int ng, const double *scale, hipStream_t s) {
   int rc;
-  if ((rc = pass_x(grid, s, false, scale))) return rc;
-  if ((rc = pass_y(grid, s, false))) return rc;
-  return pass_z(grid, s, false);
+  if ((rc = ensure_scratch(ng))) return rc;
+  if ((rc = pass(0, grids, ng, false, scale, s))) return rc;
+  if ((rc = pass(1, grids, ng, false, nullptr, s))) return rc;
+  return pass(2, grids, ng, false, nullptr, s);
 }
 
-int Fft3D::c2r(void *grid, hipStream_t s, double scale) {
+int Fft3D::c2r(void *const *grids, int ng, const double *scale, hipStream_t s) {
   int rc;
-  if ((rc = pass_z(grid, s, true))) return rc;
-  if ((rc = pass_y(grid, s, true))) return rc;
-  return pass_x(grid, s, true, scale);
+  if ((rc = ensure_scratch(ng))) return rc;
+  if ((rc = pass(2, grids, ng, true, nullptr, s))) return rc;
+  if ((rc = pass(1, grids, ng, true, nullptr, s))) return rc;
+  return pass(0, grids, ng, true, scale, s);
 }
 
 }  // namespace crava

@@ -8,6 +8,9 @@
 // z, y, then the c2r rows.  Each pass stages whole pencils in LDS.
 #pragma once
 #include "common.hpp"
+#include "fft_static.hpp"
+
+#include <cmath>
 
 namespace crava {
 
@@ -23,12 +26,27 @@ struct PassPlan {
 };
 
 // Radix sequence for n (7-smooth), preferring radix 8 and keeping the
-// smallest radix >= 4 for powers of two >= 4.  Returns false if n has a
-// prime factor > 7.
-bool factor_radices(int n, PassPlan &p);
+// smallest radix >= 4 for powers of two >= 4; order 1 reverses it.  Returns
+// false if n has a prime factor > 7.
+bool factor_radices(int n, PassPlan &p, int order);
 
 // Convert ncomplex values between float and double storage (FftIo code).
 int launch_convert(int io, const void *src, void *dst, long long ncomplex, hipStream_t s);
+
+// Scale of a transform (src/fftgrid.cpp:1817,1848-1859): forward 1/sqrt(N)
+// except covariance grids (unscaled); inverse 1/sqrt(N), covariance 1/N.
+// Float transforms round the factor to float as FFTGrid does.
+inline double grid_fft_scale(bool covariance, bool f64, long long N, bool inverse) {
+  if (!inverse) {
+    if (covariance) return 1.0;
+    return f64 ? 1.0 / std::sqrt((double)N) : (double)(1.0f / std::sqrt((float)N));
+  }
+  if (covariance) return f64 ? 1.0 / (double)N : (double)(float)(1.0 / (double)N);
+  return f64 ? 1.0 / std::sqrt((double)N) : (double)(float)(1.0 / std::sqrt((float)N));
+}
+
+// Variants of the generic (runtime-length) kernels: radix order x pencils per workgroup.
+constexpr int kNumGenericVariants = 6;
 
 class Fft3D {
  public:
@@ -36,32 +54,37 @@ class Fft3D {
   Fft3D(int nxp, int nyp, int nzp, int prec);
   ~Fft3D();
   int init();  // CRAVA_HIP_OK or an error code
-  // In-place forward r2c; the result is multiplied by `scale`.
-  int r2c(void *grid, hipStream_t s, double scale);
-  // In-place inverse c2r; the result is multiplied by `scale`.
-  int c2r(void *grid, hipStream_t s, double scale);
-  // Individual passes (exposed for profiling and the batch executor).
-  int pass_x(void *grid, hipStream_t s, bool inverse, double scale);
-  int pass_y(void *grid, hipStream_t s, bool inverse);
-  int pass_z(void *grid, hipStream_t s, bool inverse);
+  // In-place forward r2c of `ng` (<= kMaxGrids) grids of this size in one
+  // launch per pass; grid i's result is multiplied by scale[i] (1 if NULL).
+  int r2c(void *const *grids, int ng, const double *scale, hipStream_t s);
+  // In-place inverse c2r of a set of grids, scaled as r2c.
+  int c2r(void *const *grids, int ng, const double *scale, hipStream_t s);
+  int r2c(void *grid, hipStream_t s, double scale) { return r2c(&grid, 1, &scale, s); }
+  int c2r(void *grid, hipStream_t s, double scale) { return c2r(&grid, 1, &scale, s); }
+  // One pass (axis 0 x rows, 1 y, 2 z) over a set; `scale` applies on the x pass.
+  int pass(int axis, void *const *grids, int ng, bool inverse, const double *scale, hipStream_t s);
+  // Make the double scratch of a prec-1 plan hold `ngrids` grids (no-op otherwise).
+  int ensure_scratch(int ngrids);
+  long long complex_elems() const;
 
   int nxp, nyp, nzp;
   int prec;
   bool dbl;  // double arithmetic (LDS, twiddles)
   PassPlan px, py, pz;
   void *twx = nullptr, *twy = nullptr, *twz = nullptr;  // device twiddle tables
-  void *scratch = nullptr;  // double intermediates of a float-storage/double-arithmetic plan
-  // Plans built with a caller-chosen pencils-per-workgroup (generic kernels).
-  int set_tx(int tx_x, int tx_y, int tx_z);
-  // Specialised-kernel launch variant per axis (x, y, z); -1 = generic kernel.
+  void *scratch = nullptr;  // double intermediates of a prec-1 plan, one slot per grid
+  int scratch_grids = 0;
+  // Launch variant per axis (x, y, z): v >= 0 specialised kernel variant v,
+  // v < 0 generic kernel variant -1 - v.
   int var[3] = {-1, -1, -1};
   int set_variant(int axis, int v);
-  int num_variants(int axis) const;
-  // Time all variants on `grid` (overwritten) and keep the fastest per axis.
-  int autotune(void *grid, hipStream_t s, int reps, float *ms_out);
+  int num_variants(int axis) const;  // specialised variants available
+  // Time all variants on the set `grids` (overwritten) and keep the fastest per axis.
+  int autotune(void *const *grids, int ng, hipStream_t s, int reps, float *ms_out);
 
  private:
-  int build_plan(int n, PassPlan &p, int want_tx);
+  int build_plan(int n, PassPlan &p, int generic_variant);
+  int make_set(int axis, bool inverse, void *const *grids, int ng, const double *scale, GridSet &gs, int &io) const;
 };
 
 }  // namespace crava

@@ -83,8 +83,10 @@ constexpr size_t lds_bytes(int tx) {
 
 // Strided complex pass: columns contiguous, elements `es` apart.
 template <int N, int TX, int THREADS, int DIR, typename T, typename SI, typename SO>
-__global__ void __launch_bounds__(THREADS) k_sfft_strided(const cx<SI> *src, cx<SO> *dst, long long es, long long ncols,
-                                                          long long bstride, const cx<T> *__restrict__ twg) {
+__global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long es, long long ncols, long long bstride,
+                                                          const cx<T> *__restrict__ twg) {
+  const cx<SI> *src = reinterpret_cast<const cx<SI> *>(gs.src[blockIdx.z]);
+  cx<SO> *dst = reinterpret_cast<cx<SO> *>(gs.dst[blockIdx.z]);
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
@@ -125,8 +127,11 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(const cx<SI> *src, cx<
 
 // x pass forward: real rows -> half spectra (two rows per complex pencil).
 template <int N, int TX, int THREADS, typename T, typename SI, typename SO>
-__global__ void __launch_bounds__(THREADS) k_srows_r2c(const SI *src, SO *dst, long long nrows, int rnxp,
-                                                       const cx<T> *__restrict__ twg, T scale) {
+__global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nrows, int rnxp,
+                                                       const cx<T> *__restrict__ twg) {
+  const SI *src = reinterpret_cast<const SI *>(gs.src[blockIdx.z]);
+  SO *dst = reinterpret_cast<SO *>(gs.dst[blockIdx.z]);
+  const T scale = (T)gs.scale[blockIdx.z];
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
@@ -175,8 +180,11 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(const SI *src, SO *dst, l
 
 // x pass inverse: half spectra of two rows -> Z = A + iB -> inverse -> rows.
 template <int N, int TX, int THREADS, typename T, typename SI, typename SO>
-__global__ void __launch_bounds__(THREADS) k_srows_c2r(const SI *src, SO *dst, long long nrows, int rnxp,
-                                                       const cx<T> *__restrict__ twg, T scale) {
+__global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nrows, int rnxp,
+                                                       const cx<T> *__restrict__ twg) {
+  const SI *src = reinterpret_cast<const SI *>(gs.src[blockIdx.z]);
+  SO *dst = reinterpret_cast<SO *>(gs.dst[blockIdx.z]);
+  const T scale = (T)gs.scale[blockIdx.z];
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
@@ -238,72 +246,67 @@ static int set_lds_attr(F kernel, size_t bytes) {
 }
 
 template <int N, int V, int DIR, typename T, typename SI, typename SO>
-static int launch_strided_v(const void *src, void *dst, long long es, long long ncols, long long nbatch,
-                            long long bstride, const void *tw, hipStream_t s) {
+static int launch_strided_v(const GridSet &gs, long long es, long long ncols, long long nbatch, long long bstride,
+                            const void *tw, hipStream_t s) {
   constexpr int TX = kSV[V][0], TH = kSV[V][1];
   const size_t lds = lds_bytes<N, T>(TX);
   const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
   int rc;
   if ((rc = set_lds_attr(k_sfft_strided<N, TX, TH, DIR, T, SI, SO>, lds))) return rc;
-  hipLaunchKernelGGL((k_sfft_strided<N, TX, TH, DIR, T, SI, SO>), dim3(gx, (unsigned)nbatch), dim3(TH), lds, s,
-                     reinterpret_cast<const cx<SI> *>(src), reinterpret_cast<cx<SO> *>(dst), es, ncols, bstride,
-                     reinterpret_cast<const cx<T> *>(tw));
+  hipLaunchKernelGGL((k_sfft_strided<N, TX, TH, DIR, T, SI, SO>), dim3(gx, (unsigned)nbatch, (unsigned)gs.n), dim3(TH),
+                     lds, s, gs, es, ncols, bstride, reinterpret_cast<const cx<T> *>(tw));
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
 
 template <int N, int V, bool INV, typename T, typename SI, typename SO>
-static int launch_rows_v(const void *src, void *dst, long long nrows, int rnxp, double scale, const void *tw,
-                         hipStream_t s) {
+static int launch_rows_v(const GridSet &gs, long long nrows, int rnxp, const void *tw, hipStream_t s) {
   constexpr int TX = kRV[V][0], TH = kRV[V][1];
   const size_t lds = lds_bytes<N, T>(TX);
   const unsigned gx = (unsigned)((nrows + 2LL * TX - 1) / (2LL * TX));
-  const SI *ip = reinterpret_cast<const SI *>(src);
-  SO *op = reinterpret_cast<SO *>(dst);
   const cx<T> *twp = reinterpret_cast<const cx<T> *>(tw);
   int rc;
   if constexpr (!INV) {
     if ((rc = set_lds_attr(k_srows_r2c<N, TX, TH, T, SI, SO>, lds))) return rc;
-    hipLaunchKernelGGL((k_srows_r2c<N, TX, TH, T, SI, SO>), dim3(gx), dim3(TH), lds, s, ip, op, nrows, rnxp, twp,
-                       (T)scale);
+    hipLaunchKernelGGL((k_srows_r2c<N, TX, TH, T, SI, SO>), dim3(gx, 1, (unsigned)gs.n), dim3(TH), lds, s, gs, nrows,
+                       rnxp, twp);
   } else {
     if ((rc = set_lds_attr(k_srows_c2r<N, TX, TH, T, SI, SO>, lds))) return rc;
-    hipLaunchKernelGGL((k_srows_c2r<N, TX, TH, T, SI, SO>), dim3(gx), dim3(TH), lds, s, ip, op, nrows, rnxp, twp,
-                       (T)scale);
+    hipLaunchKernelGGL((k_srows_c2r<N, TX, TH, T, SI, SO>), dim3(gx, 1, (unsigned)gs.n), dim3(TH), lds, s, gs, nrows,
+                       rnxp, twp);
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
 
 template <int N, int DIR, typename T, typename SI, typename SO>
-static int strided_any(int v, const void *src, void *dst, long long es, long long ncols, long long nbatch,
-                       long long bstride, const void *tw, hipStream_t s) {
+static int strided_any(int v, const GridSet &gs, long long es, long long ncols, long long nbatch, long long bstride,
+                       const void *tw, hipStream_t s) {
   switch (v) {
-    case 0: return launch_strided_v<N, 0, DIR, T, SI, SO>(src, dst, es, ncols, nbatch, bstride, tw, s);
-    case 1: return launch_strided_v<N, 1, DIR, T, SI, SO>(src, dst, es, ncols, nbatch, bstride, tw, s);
-    case 2: return launch_strided_v<N, 2, DIR, T, SI, SO>(src, dst, es, ncols, nbatch, bstride, tw, s);
-    default: return launch_strided_v<N, 3, DIR, T, SI, SO>(src, dst, es, ncols, nbatch, bstride, tw, s);
+    case 0: return launch_strided_v<N, 0, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    case 1: return launch_strided_v<N, 1, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    case 2: return launch_strided_v<N, 2, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    default: return launch_strided_v<N, 3, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
   }
 }
 
 template <int N, bool INV, typename T, typename SI, typename SO>
-static int rows_any(int v, const void *src, void *dst, long long nrows, int rnxp, double scale, const void *tw,
-                    hipStream_t s) {
+static int rows_any(int v, const GridSet &gs, long long nrows, int rnxp, const void *tw, hipStream_t s) {
   switch (v) {
-    case 0: return launch_rows_v<N, 0, INV, T, SI, SO>(src, dst, nrows, rnxp, scale, tw, s);
-    case 1: return launch_rows_v<N, 1, INV, T, SI, SO>(src, dst, nrows, rnxp, scale, tw, s);
-    default: return launch_rows_v<N, 2, INV, T, SI, SO>(src, dst, nrows, rnxp, scale, tw, s);
+    case 0: return launch_rows_v<N, 0, INV, T, SI, SO>(gs, nrows, rnxp, tw, s);
+    case 1: return launch_rows_v<N, 1, INV, T, SI, SO>(gs, nrows, rnxp, tw, s);
+    default: return launch_rows_v<N, 2, INV, T, SI, SO>(gs, nrows, rnxp, tw, s);
   }
 }
 
 template <int N, typename T, typename S>
-static int all_kinds(FftKind kind, int v, const void *src, void *dst, long long es, long long ncols, long long nbatch,
-                     long long bstride, long long nrows, int rnxp, double scale, const void *tw, hipStream_t s) {
+static int all_kinds(FftKind kind, int v, const GridSet &gs, long long es, long long ncols, long long nbatch,
+                     long long bstride, long long nrows, int rnxp, const void *tw, hipStream_t s) {
   switch (kind) {
-    case kStridedFwd: return strided_any<N, -1, T, S, S>(v, src, dst, es, ncols, nbatch, bstride, tw, s);
-    case kStridedInv: return strided_any<N, 1, T, S, S>(v, src, dst, es, ncols, nbatch, bstride, tw, s);
-    case kRowsR2C: return rows_any<N, false, T, S, S>(v, src, dst, nrows, rnxp, scale, tw, s);
-    default: return rows_any<N, true, T, S, S>(v, src, dst, nrows, rnxp, scale, tw, s);
+    case kStridedFwd: return strided_any<N, -1, T, S, S>(v, gs, es, ncols, nbatch, bstride, tw, s);
+    case kStridedInv: return strided_any<N, 1, T, S, S>(v, gs, es, ncols, nbatch, bstride, tw, s);
+    case kRowsR2C: return rows_any<N, false, T, S, S>(v, gs, nrows, rnxp, tw, s);
+    default: return rows_any<N, true, T, S, S>(v, gs, nrows, rnxp, tw, s);
   }
 }
 
@@ -312,19 +315,16 @@ static int all_kinds(FftKind kind, int v, const void *src, void *dst, long long 
 // float->double, strided double->double, strided double->float; inverse =
 // strided float->double, strided double->double, rows double->float.
 template <int N>
-static int launch_store(int io, FftKind kind, int v, const void *src, void *dst, long long es, long long ncols,
-                        long long nbatch, long long bstride, long long nrows, int rnxp, double scale, const void *tw,
-                        hipStream_t s) {
-  if (io == kIoF32) return all_kinds<N, float, float>(kind, v, src, dst, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s);
-  if (io == kIoF64) return all_kinds<N, double, double>(kind, v, src, dst, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s);
-  if (io == kIoF32toF64 && kind == kRowsR2C)
-    return rows_any<N, false, double, float, double>(v, src, dst, nrows, rnxp, scale, tw, s);
+static int launch_store(int io, FftKind kind, int v, const GridSet &gs, long long es, long long ncols,
+                        long long nbatch, long long bstride, long long nrows, int rnxp, const void *tw, hipStream_t s) {
+  if (io == kIoF32) return all_kinds<N, float, float>(kind, v, gs, es, ncols, nbatch, bstride, nrows, rnxp, tw, s);
+  if (io == kIoF64) return all_kinds<N, double, double>(kind, v, gs, es, ncols, nbatch, bstride, nrows, rnxp, tw, s);
+  if (io == kIoF32toF64 && kind == kRowsR2C) return rows_any<N, false, double, float, double>(v, gs, nrows, rnxp, tw, s);
   if (io == kIoF32toF64 && kind == kStridedInv)
-    return strided_any<N, 1, double, float, double>(v, src, dst, es, ncols, nbatch, bstride, tw, s);
+    return strided_any<N, 1, double, float, double>(v, gs, es, ncols, nbatch, bstride, tw, s);
   if (io == kIoF64toF32 && kind == kStridedFwd)
-    return strided_any<N, -1, double, double, float>(v, src, dst, es, ncols, nbatch, bstride, tw, s);
-  if (io == kIoF64toF32 && kind == kRowsC2R)
-    return rows_any<N, true, double, double, float>(v, src, dst, nrows, rnxp, scale, tw, s);
+    return strided_any<N, -1, double, double, float>(v, gs, es, ncols, nbatch, bstride, tw, s);
+  if (io == kIoF64toF32 && kind == kRowsC2R) return rows_any<N, true, double, double, float>(v, gs, nrows, rnxp, tw, s);
   set_error("unsupported storage combination for this pass");
   return CRAVA_HIP_E_INVALID;
 }
@@ -370,16 +370,15 @@ bool static_fft_geometry(int n, bool dbl, FftKind kind, int variant, StaticLaunc
   return true;
 }
 
-int static_fft_launch(int n, int io, FftKind kind, int variant, const void *src, void *dst, long long es,
-                      long long ncols, long long nbatch, long long bstride, long long nrows, int rnxp, double scale,
-                      const void *tw, hipStream_t s) {
+int static_fft_launch(int n, int io, FftKind kind, int variant, const GridSet &gs, long long es, long long ncols,
+                      long long nbatch, long long bstride, long long nrows, int rnxp, const void *tw, hipStream_t s) {
   if (!static_fft_geometry(n, io != kIoF32, kind, variant, nullptr)) {
     set_error("no specialised transform for length " + std::to_string(n) + " / variant " + std::to_string(variant));
     return CRAVA_HIP_E_INVALID;
   }
 #define CRAVA_CASE(L)                                                                                          \
   if (n == L)                                                                                                  \
-    return launch_store<L>(io, kind, variant, src, dst, es, ncols, nbatch, bstride, nrows, rnxp, scale, tw, s);
+    return launch_store<L>(io, kind, variant, gs, es, ncols, nbatch, bstride, nrows, rnxp, tw, s);
   CRAVA_STATIC_LENGTHS(CRAVA_CASE)
 #undef CRAVA_CASE
   return CRAVA_HIP_E_INVALID;

@@ -22,15 +22,24 @@ struct StaticLaunch {
 bool static_fft_available(int n);
 // Launch geometry of a variant; false if the variant does not fit.
 bool static_fft_geometry(int n, bool dbl_compute, FftKind kind, int variant, StaticLaunch *out);
+// Up to kMaxGrids same-size grids transformed by one launch (blockIdx.z).
+constexpr int kMaxGrids = 16;
+struct GridSet {
+  const void *src[kMaxGrids];
+  void *dst[kMaxGrids];
+  double scale[kMaxGrids];
+  int n;
+};
+
 // Storage of a pass's input/output (arithmetic is double unless kIoF32).
 enum FftIo { kIoF32 = 0, kIoF64 = 1, kIoF32toF64 = 2, kIoF64toF32 = 3 };
 
-// Launch a specialised pass from `src` to `dst` (may alias for in place).
-// Strided: complex, `es` element stride, `ncols` contiguous columns,
-// `nbatch` batches `bstride` apart.  Rows: `nrows` rows of pitch `rnxp`
-// reals, `scale` applied to the output.
-int static_fft_launch(int n, int io, FftKind kind, int variant, const void *src, void *dst, long long es,
-                      long long ncols, long long nbatch, long long bstride, long long nrows, int rnxp, double scale,
-                      const void *twiddles, hipStream_t s);
+// Launch a specialised pass over every grid of `gs` (src -> dst, may alias
+// for in place).  Strided: complex, `es` element stride, `ncols` contiguous
+// columns, `nbatch` batches `bstride` apart.  Rows: `nrows` rows of pitch
+// `rnxp` reals, gs.scale applied to the output.
+int static_fft_launch(int n, int io, FftKind kind, int variant, const GridSet &gs, long long es, long long ncols,
+                      long long nbatch, long long bstride, long long nrows, int rnxp, const void *twiddles,
+                      hipStream_t s);
 
 }  // namespace crava

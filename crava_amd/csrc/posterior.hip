@@ -18,7 +18,10 @@
 #include "posterior.hpp"
 #include "../../include/crava_hip.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 namespace crava {
 
@@ -154,9 +157,7 @@ __device__ __forceinline__ cx<D> cvt(cx<S_> a) { return mk((D)a.x, (D)a.y); }
 // noise/signal ratio tr(E)/tr(K S K^H) is below thr, which are solved
 // end-to-end in FP64 from the FP32 inputs and rounded once at the end.
 template <typename G, int NT, int MODE>
-__global__ void __launch_bounds__(256) k_posterior(PostArgs<G> a) {
-  const int k = blockIdx.y;
-  const long long ip = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void post_point(const PostArgs<G> &a, int k, long long ip) {
   const bool live = ip < a.plane;
   const long long p = (long long)k * a.plane + (live ? ip : 0);
   const PerK<G> *tb = pick_tab<G>(a.tabf, a.tabd) + k;
@@ -255,6 +256,47 @@ __global__ void __launch_bounds__(256) k_posterior(PostArgs<G> a) {
     if ((threadIdx.x & 63) == 0) {
       if (fails) atomicAdd(&a.counters[0], (unsigned long long)__popcll(fails));
       if (wide) atomicAdd(&a.counters[1], (unsigned long long)__popcll(wide));
+    }
+  }
+}
+
+template <typename G, int NT, int MODE>
+__global__ void __launch_bounds__(256) k_posterior(PostArgs<G> a) {
+  post_point<G, NT, MODE>(a, blockIdx.y, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Many volumes in one persistent launch.  Tile t = {volume, k, first point,
+// ntheta} covers blockDim.x consecutive points of one k-plane of one
+// volume; workgroups stride over the tile list.  Everything a tile reads
+// from `vols` and `tiles` is wave-uniform.  This form holds the registers of
+// the largest ntheta for every tile; k_posterior_class below is the same
+// loop over the tiles of one ntheta.
+template <typename G, int NT, int MODE>
+__global__ void __launch_bounds__(256) k_posterior_class(const PostArgs<G> *__restrict__ vols,
+                                                         const int4 *__restrict__ tiles, long long t0,
+                                                         long long t1) {
+  for (long long t = t0 + blockIdx.x; t < t1; t += gridDim.x) {
+    const int4 tl = tiles[t];
+    post_point<G, NT, MODE>(vols[tl.x], tl.y, (long long)tl.z + threadIdx.x);
+  }
+}
+
+template <typename G, int MODE>
+__global__ void __launch_bounds__(256) k_posterior_batch(const PostArgs<G> *__restrict__ vols,
+                                                         const int4 *__restrict__ tiles, long long ntiles) {
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int4 tl = tiles[t];
+    const PostArgs<G> &a = vols[tl.x];
+    const long long ip = (long long)tl.z + threadIdx.x;
+    switch (tl.w) {
+      case 1: post_point<G, 1, MODE>(a, tl.y, ip); break;
+      case 2: post_point<G, 2, MODE>(a, tl.y, ip); break;
+      case 3: post_point<G, 3, MODE>(a, tl.y, ip); break;
+      case 4: post_point<G, 4, MODE>(a, tl.y, ip); break;
+      case 5: post_point<G, 5, MODE>(a, tl.y, ip); break;
+      case 6: post_point<G, 6, MODE>(a, tl.y, ip); break;
+      case 7: post_point<G, 7, MODE>(a, tl.y, ip); break;
+      default: post_point<G, 8, MODE>(a, tl.y, ip); break;
     }
   }
 }
@@ -376,6 +418,136 @@ int launch_posterior(const PostDevice &pd, int precision, void *const *mu, void 
   fill_args(a, pd, mu, data, cov, errcorr, counters);
   if (precision == CRAVA_HIP_MIXED) return dispatch<float, 1>(a, nt, pd.pp.nzp, s);
   return dispatch<float, 0>(a, nt, pd.pp.nzp, s);
+}
+
+// ---------------------------------------------------------------------------
+PostBatch::~PostBatch() {
+  if (args) (void)hipFree(args);
+  if (tiles) (void)hipFree(tiles);
+}
+
+int PostBatch::init(int prec, const PostVolume *vols, int n) {
+  if (n < 1 || !vols) {
+    set_error("a posterior batch needs at least one volume");
+    return CRAVA_HIP_E_INVALID;
+  }
+  precision = prec;
+  nvol = n;
+  nt.resize(n);
+  nzp.resize(n);
+  plane.resize(n);
+  const size_t asz = prec == CRAVA_HIP_F64 ? sizeof(PostArgs<double>) : sizeof(PostArgs<float>);
+  std::vector<unsigned char> host(asz * (size_t)n);
+  for (int v = 0; v < n; ++v) {
+    const PostVolume &pv = vols[v];
+    const PostProblem &pp = pv.pd->pp;
+    nt[v] = pp.ntheta;
+    nzp[v] = pp.nzp;
+    plane[v] = (long long)(pp.nxp / 2 + 1) * pp.nyp;
+    if (prec == CRAVA_HIP_F64) {
+      PostArgs<double> a{};
+      fill_args(a, *pv.pd, pv.mu, pv.data, pv.cov, pv.errcorr, pv.counters);
+      std::memcpy(host.data() + asz * v, &a, sizeof(a));
+    } else {
+      PostArgs<float> a{};
+      fill_args(a, *pv.pd, pv.mu, pv.data, pv.cov, pv.errcorr, pv.counters);
+      std::memcpy(host.data() + asz * v, &a, sizeof(a));
+    }
+  }
+  CRAVA_HIP_CHECK(hipMalloc(&args, host.size()));
+  CRAVA_HIP_CHECK(hipMemcpy(args, host.data(), host.size(), hipMemcpyHostToDevice));
+  return set_launch(256, 0);
+}
+
+int PostBatch::set_launch(int points_per_block, int nblocks) {
+  if (points_per_block != 64 && points_per_block != 128 && points_per_block != 256) {
+    set_error("points per block must be 64, 128 or 256");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (nblocks < 0) {
+    set_error("negative block count");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (points_per_block != ppb || !tiles) {
+    // tiles ordered by ntheta, so each ntheta class is one contiguous segment
+    std::vector<int4> h;
+    for (int c = 1; c <= kMaxTheta; ++c) {
+      seg[c - 1] = (long long)h.size();
+      for (int v = 0; v < nvol; ++v) {
+        if (nt[v] != c) continue;
+        for (int k = 0; k < nzp[v]; ++k)
+          for (long long p0 = 0; p0 < plane[v]; p0 += points_per_block) h.push_back(make_int4(v, k, (int)p0, c));
+      }
+    }
+    seg[kMaxTheta] = (long long)h.size();
+    if (tiles) (void)hipFree(tiles);
+    tiles = nullptr;
+    CRAVA_HIP_CHECK(hipMalloc(&tiles, sizeof(int4) * h.size()));
+    CRAVA_HIP_CHECK(hipMemcpy(tiles, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
+    ntiles = (long long)h.size();
+    ppb = points_per_block;
+  }
+  blocks = nblocks;
+  return CRAVA_HIP_OK;
+}
+
+int PostBatch::num_classes() const {
+  int n = 0;
+  for (int c = 0; c < kMaxTheta; ++c) n += seg[c + 1] > seg[c];
+  return n;
+}
+
+template <typename G, int NT, int MODE>
+static void launch_class(const PostBatch &b, long long t0, long long t1, hipStream_t s) {
+  const long long n = t1 - t0;
+  const long long g = b.blocks > 0 ? std::min<long long>(b.blocks, n) : n;
+  hipLaunchKernelGGL((k_posterior_class<G, NT, MODE>), dim3((unsigned)g), dim3((unsigned)b.ppb), 0, s,
+                     reinterpret_cast<const PostArgs<G> *>(b.args), b.tiles, t0, t1);
+}
+
+template <typename G, int MODE>
+static int launch_classes(const PostBatch &b, const hipStream_t *streams, int nstreams) {
+  int i = 0;
+  for (int c = 1; c <= kMaxTheta; ++c) {
+    const long long t0 = b.seg[c - 1], t1 = b.seg[c];
+    if (t1 <= t0) continue;
+    hipStream_t s = streams[i++ % nstreams];
+    switch (c) {
+      case 1: launch_class<G, 1, MODE>(b, t0, t1, s); break;
+      case 2: launch_class<G, 2, MODE>(b, t0, t1, s); break;
+      case 3: launch_class<G, 3, MODE>(b, t0, t1, s); break;
+      case 4: launch_class<G, 4, MODE>(b, t0, t1, s); break;
+      case 5: launch_class<G, 5, MODE>(b, t0, t1, s); break;
+      case 6: launch_class<G, 6, MODE>(b, t0, t1, s); break;
+      case 7: launch_class<G, 7, MODE>(b, t0, t1, s); break;
+      default: launch_class<G, 8, MODE>(b, t0, t1, s); break;
+    }
+    CRAVA_HIP_CHECK(hipGetLastError());
+  }
+  return CRAVA_HIP_OK;
+}
+
+int PostBatch::launch(const hipStream_t *streams, int nstreams) const {
+  if (ntiles < 1) return CRAVA_HIP_OK;
+  if (per_class) {
+    if (precision == CRAVA_HIP_F64) return launch_classes<double, 0>(*this, streams, nstreams);
+    if (precision == CRAVA_HIP_MIXED) return launch_classes<float, 1>(*this, streams, nstreams);
+    return launch_classes<float, 0>(*this, streams, nstreams);
+  }
+  const long long g = blocks > 0 ? std::min<long long>(blocks, ntiles) : ntiles;
+  const dim3 grid((unsigned)g), block((unsigned)ppb);
+  hipStream_t s = streams[0];
+  if (precision == CRAVA_HIP_F64)
+    hipLaunchKernelGGL((k_posterior_batch<double, 0>), grid, block, 0, s,
+                       reinterpret_cast<const PostArgs<double> *>(args), tiles, ntiles);
+  else if (precision == CRAVA_HIP_MIXED)
+    hipLaunchKernelGGL((k_posterior_batch<float, 1>), grid, block, 0, s,
+                       reinterpret_cast<const PostArgs<float> *>(args), tiles, ntiles);
+  else
+    hipLaunchKernelGGL((k_posterior_batch<float, 0>), grid, block, 0, s,
+                       reinterpret_cast<const PostArgs<float> *>(args), tiles, ntiles);
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
 }
 
 }  // namespace crava

@@ -3,6 +3,8 @@
 #pragma once
 #include "common.hpp"
 
+#include <vector>
+
 namespace crava {
 
 constexpr int kMaxTheta = 8;
@@ -51,5 +53,37 @@ struct PostDevice {
 // point below the noise/signal threshold), 2 = FP64 (complex128 grids).
 int launch_posterior(const PostDevice &pd, int precision, void *const *mu, void *const *data, void *const *cov,
                      const void *errcorr, unsigned long long *counters, hipStream_t s);
+
+// One volume of a batched posterior: its problem and device grids.
+struct PostVolume {
+  const PostDevice *pd = nullptr;
+  void *mu[3] = {nullptr, nullptr, nullptr};
+  void *data[kMaxTheta] = {nullptr};
+  void *cov[6] = {nullptr};
+  const void *errcorr = nullptr;
+  unsigned long long *counters = nullptr;  // device [2]: failed Cholesky, FP64 points
+};
+
+// The posterior of many volumes (uneven sizes and ntheta) over a tile list:
+// ppb points of one k-plane per tile, `blocks` persistent workgroups
+// striding over the tiles (0: one workgroup per tile).  per_class = false:
+// one launch over every tile, sized for the largest ntheta; true: one launch
+// per ntheta class (tiles are ordered by ntheta), each with the registers
+// of its own ntheta, spread over the given streams.
+struct PostBatch {
+  int precision = 0, nvol = 0, ppb = 0, blocks = 0;
+  bool per_class = true;
+  void *args = nullptr;       // device PostArgs per volume
+  int4 *tiles = nullptr;      // device {volume, k, first point, ntheta}
+  long long ntiles = 0;
+  long long seg[kMaxTheta + 1] = {0};  // tiles of ntheta c: [seg[c-1], seg[c])
+  std::vector<int> nt, nzp;
+  std::vector<long long> plane;
+  ~PostBatch();
+  int init(int precision, const PostVolume *vols, int nvol);
+  int set_launch(int points_per_block, int blocks);
+  int num_classes() const;
+  int launch(const hipStream_t *streams, int nstreams) const;
+};
 
 }  // namespace crava

@@ -105,12 +105,13 @@ def config(name: str, seed: int = 1) -> Config:
     raise KeyError(name)
 
 
-def random_config(rng: np.random.Generator, idx: int, seed: int = 1) -> Config:
+def random_config(rng: np.random.Generator, idx: int, seed: int = 1, nxy=(32, 192), nzr=(64, 256)) -> Config:
     """One volume of the batching workload: nx, ny in [32,192], nz in
-    [64,256], ntheta in [3,8], Ricker 20-40 Hz, ranges 300-1500 m /
-    20-60 ms, S/N 2-10; padding as ModelGeneral::setPaddingSize with 10%."""
-    nx, ny = int(rng.integers(32, 193)), int(rng.integers(32, 193))
-    nz = int(rng.integers(64, 257))
+    [64,256] (or the given inclusive ranges), ntheta in [3,8], Ricker
+    20-40 Hz, ranges 300-1500 m / 20-60 ms, S/N 2-10; padding as
+    ModelGeneral::setPaddingSize with 10%."""
+    nx, ny = int(rng.integers(nxy[0], nxy[1] + 1)), int(rng.integers(nxy[0], nxy[1] + 1))
+    nz = int(rng.integers(nzr[0], nzr[1] + 1))
     nt = int(rng.integers(3, 9))
     nxp = find_closest_factorable(int(math.ceil(nx * 1.1)))
     nyp = find_closest_factorable(int(math.ceil(ny * 1.1)))

@@ -40,6 +40,7 @@ extern "C" {
 
 typedef struct crava_hip_ctx crava_hip_ctx;
 typedef struct crava_hip_post crava_hip_post;
+typedef struct crava_hip_batch crava_hip_batch;
 
 /* Library version (major*10000 + minor*100 + patch).  No reference counterpart. */
 int crava_hip_version(void);
@@ -70,6 +71,17 @@ int crava_hip_fft_r2c(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp,
  * is ignored as in FFTW's halfcomplex c2r. */
 int crava_hip_fft_c2r(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int cubetype, int precision);
 
+/* The same transforms over `ngrid` (1..16) device grids of one size and
+ * cubetype, one kernel launch per pass for the whole set.  Replaces the
+ * grid-by-grid loops of SeismicParametersHolder::invFFTAllGrids /
+ * FFTAllGrids (src/seismicparametersholder.cpp:262-298, with the
+ * covariance grids of FFTCovGrids / invFFTCovGrids) and the per-grid calls
+ * of Crava (src/crava.cpp:177,188-190,1012,1181-1183,1201,1538-1545). */
+int crava_hip_fft_r2c_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp, int nyp, int nzp,
+                           int cubetype, int precision);
+int crava_hip_fft_c2r_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp, int nyp, int nzp,
+                           int cubetype, int precision);
+
 /* One pass of a 3-D transform (axis 0 = x rows incl. the r2c/c2r split,
  * 1 = y, 2 = z), for profiling and per-pass timing; `scale` multiplies the
  * x-pass output.  No reference counterpart (FFTW does all passes inside
@@ -79,8 +91,11 @@ int crava_hip_fft_pass(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp
 
 /* Autotuner: time every launch variant of each pass of the nxp x nyp x nzp
  * transform on `scratch` (a grid of that size; its contents are destroyed)
- * and keep the fastest in the context's plan.  variants[3] / ms[3] (host,
- * may be NULL) receive the choice per axis (-1 = generic kernel) and its
+ * and keep the fastest in the context's plan.  Variants: v >= 0 a
+ * compile-time specialised kernel (lengths 64, 80, 128, 160, 216, 256, 288,
+ * 320, 512), v = -1..-6 the runtime-length kernel with radix order
+ * (large-first / small-first) x pencils per workgroup (<=64, <=16, <=4).
+ * variants[3] / ms[3] (host, may be NULL) receive the choice per axis and its
  * time.  No reference counterpart (FFTW_ESTIMATE plans are not measured). */
 int crava_hip_autotune_fft(crava_hip_ctx *ctx, void *scratch, int nxp, int nyp, int nzp, int precision,
                            int *variants, float *ms);
@@ -195,6 +210,56 @@ int crava_hip_facies(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int ny
                      const float *beta, const float *rho, int nfac, const float *tables, int tn1, int tn2, int tn3,
                      const double *volume, const int *vn, const float *prior, float p_undefined, float *const *prob,
                      int *argmax, unsigned long long *counts);
+
+/* One inversion volume of a batch: its problem (from crava_hip_post_create,
+ * which fixes nxp, nyp, nzp and ntheta) and its device grids in the FFTGrid
+ * layout, float (CRAVA_HIP_F32 / CRAVA_HIP_MIXED) or double (CRAVA_HIP_F64). */
+typedef struct {
+  crava_hip_post *post;
+  void *mu[3];      /* prior mean, time domain         -> posterior mean, time domain */
+  void *data[8];    /* seismic data, time domain        -> residuals, time domain (first ntheta used) */
+  void *cov[6];     /* prior covariance (aa,bb,rr,ab,ar,br), time domain -> posterior covariance, time domain */
+  void *errcorr;    /* error correlation, time domain   -> its spectrum (consumed) */
+} crava_hip_batch_volume;
+
+/* Many volumes of uneven size and ntheta inverted as one sequence.  No
+ * reference counterpart: the reference inverts one volume per Crava object
+ * (main.cpp:279-308).  Per volume, as Crava::computePostMeanResidAndFFTCov
+ * (src/crava.cpp:887-1204): forward transforms of data and mean (scaled as
+ * DATA/PARAMETER) and of the covariance and error-correlation grids
+ * (COVARIANCE; double arithmetic for CRAVA_HIP_MIXED), the posterior, then
+ * inverse transforms of mean, residuals and covariances.  All transforms of
+ * one volume and kind are one launch per pass; the posterior of every volume
+ * is one persistent launch.  The grids are registered here and must stay
+ * allocated, like `ctx` and every `post`, until crava_hip_batch_destroy. */
+int crava_hip_batch_create(crava_hip_ctx *ctx, int precision, int nvol, const crava_hip_batch_volume *vols,
+                           crava_hip_batch **batch);
+
+/* Time the transform variants of every grid shape of the batch and the
+ * posterior launch and keep the fastest.  Posterior launch forms: one
+ * persistent launch over every volume (registers of the largest ntheta), or
+ * one persistent launch per ntheta class running concurrently; each with
+ * 64/128/256 points per block and 0 (one workgroup per tile) or
+ * 1024..8192 workgroups.  The registered grids' contents are saved to a
+ * temporary device buffer and restored.  Outputs (host, may be NULL): the
+ * chosen points per block, workgroups, per-class flag and posterior time. */
+int crava_hip_batch_autotune(crava_hip_batch *batch, int *points_per_block, int *blocks, int *per_class,
+                             float *post_ms);
+
+/* Set the posterior launch directly (points_per_block 64, 128 or 256;
+ * blocks 0 = one workgroup per tile; per_class 0 = one launch). */
+int crava_hip_batch_set_launch(crava_hip_batch *batch, int points_per_block, int blocks, int per_class);
+
+/* Run the whole batch on the context's stream, asynchronously.  use_graph:
+ * 0 enqueues the launches directly; 1 captures them into a hipGraph on first
+ * use (and after autotune / set_launch) and replays it. */
+int crava_hip_batch_run(crava_hip_batch *batch, int use_graph);
+
+/* Per-volume counters of the last run (host arrays [nvol], may be NULL):
+ * failed factorisations and points solved in FP64.  Synchronises. */
+int crava_hip_batch_counts(crava_hip_batch *batch, long long *n_fail, long long *n_fp64);
+
+int crava_hip_batch_destroy(crava_hip_batch *batch);
 
 #ifdef __cplusplus
 }

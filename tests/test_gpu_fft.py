@@ -86,7 +86,8 @@ def test_fft_rejects_bad_sizes(ctx):
 @pytest.mark.parametrize("n", [(512, 16, 320), (288, 12, 216), (80, 64, 160)], ids=lambda n: "x".join(map(str, n)))
 @pytest.mark.parametrize("dbl", [False, True])
 def test_every_launch_variant_and_autotuner(ctx, n, dbl):
-    """Each specialised launch variant (and the generic kernel) against the oracle."""
+    """Each specialised launch variant and each runtime-length kernel variant
+    (radix order x pencils per workgroup, codes -1..-6) against the oracle."""
     from crava_amd.core import autotune_fft, set_fft_variants
     nxp, nyp, nzp = n
     print(f"variants {n} dbl={dbl}", flush=True)
@@ -96,7 +97,7 @@ def test_every_launch_variant_and_autotuner(ctx, n, dbl):
     (oracle.fft_r2c_f64 if dbl else lambda a, *r: oracle.fft_r2c(a, *r))(ref, nxp, nyp, nzp, False)
     tol = 1e-13 if dbl else 1e-6
     prec = L.F64 if dbl else L.F32
-    for v in (-1, 0, 1, 2, 3):
+    for v in (-6, -5, -4, -3, -2, -1, 0, 1, 2, 3):
         try:
             set_fft_variants(ctx, nxp, nyp, nzp, prec, [min(v, 2), v, v])
         except L.CravaHipError:
@@ -110,7 +111,7 @@ def test_every_launch_variant_and_autotuner(ctx, n, dbl):
         assert rel_l2(t.cpu().numpy()[:, :, :nxp], g[:, :, :nxp]) < tol, v
     scratch = torch.zeros(g.shape, dtype=torch.float64 if dbl else torch.float32, device="cuda")
     var, ms = autotune_fft(ctx, scratch, nxp, nyp, nzp)
-    assert all(-1 <= x <= 3 for x in var) and all(m > 0 for m in ms)
+    assert all(-6 <= x <= 3 for x in var) and all(m > 0 for m in ms)
     t = torch.from_numpy(g.copy()).cuda()
     ctx.r2c(t, nxp, nyp, nzp, L.DATA)
     torch.cuda.synchronize()
@@ -142,3 +143,52 @@ def test_r2c_float_storage_double_arithmetic(ctx, n):
     mask = np.abs(want) > 1e-9 * big
     relerr = np.abs(got[mask] - want[mask]) / np.abs(want[mask])
     assert np.percentile(relerr, 99) < 1e-6 and relerr.max() < 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [(105, 45, 98), (36, 126, 75)], ids=lambda n: "x".join(map(str, n)))
+def test_generic_variants_odd_lengths(ctx, n):
+    """Runtime-length kernels on odd and 3/5/7-rich lengths: every generic
+    variant, forward against the oracle and the round trip."""
+    from crava_amd.core import set_fft_variants
+    nxp, nyp, nzp = n
+    print(f"generic variants {n}", flush=True)
+    g = seeded_real_grid(nxp, nyp, nzp, seed=5)
+    ref = g.copy()
+    oracle.fft_r2c(ref, nxp, nyp, nzp, False)
+    for v in (-6, -5, -4, -3, -2, -1):
+        set_fft_variants(ctx, nxp, nyp, nzp, L.F32, [v, v, v])
+        t = torch.from_numpy(g.copy()).cuda()
+        ctx.r2c(t, nxp, nyp, nzp, L.DATA)
+        torch.cuda.synchronize()
+        assert rel_l2(to_complex(t.cpu().numpy()), to_complex(ref)) < 1e-6, v
+        ctx.c2r(t, nxp, nyp, nzp, L.DATA)
+        torch.cuda.synchronize()
+        assert rel_l2(t.cpu().numpy()[:, :, :nxp], g[:, :, :nxp]) < 1e-6, v
+
+
+@pytest.mark.gpu
+def test_many_grids_one_launch_equals_one_by_one(ctx):
+    """crava_hip_fft_r2c_many / c2r_many: a set of grids per launch gives the
+    same bits as transforming the grids one at a time (float, float storage
+    with double arithmetic, double)."""
+    nxp, nyp, nzp = 80, 45, 64
+    print("many grids", flush=True)
+    for prec, dt in ((L.F32, np.float32), (L.MIXED, np.float32), (L.F64, np.float64)):
+        gs = [seeded_real_grid(nxp, nyp, nzp, seed=s, dtype=dt) for s in range(7)]
+        one = [torch.from_numpy(a.copy()).cuda() for a in gs]
+        many = [torch.from_numpy(a.copy()).cuda() for a in gs]
+        for t in one:
+            ctx.r2c(t, nxp, nyp, nzp, L.COVARIANCE, prec)
+        ctx.r2c_many(many, nxp, nyp, nzp, L.COVARIANCE, prec)
+        torch.cuda.synchronize()
+        for a, b in zip(one, many):
+            assert torch.equal(a, b), prec
+        for t in one:
+            ctx.c2r(t, nxp, nyp, nzp, L.DATA, prec)
+        ctx.c2r_many(many, nxp, nyp, nzp, L.DATA, prec)
+        torch.cuda.synchronize()
+        for a, b in zip(one, many):
+            assert torch.equal(a, b), prec
+    with pytest.raises(L.CravaHipError):
+        ctx.r2c_many([torch.zeros(2 * 41 * 45 * 64, device="cuda")] * 17, nxp, nyp, nzp, L.DATA)
