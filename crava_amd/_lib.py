@@ -63,6 +63,7 @@ SIGNATURES = {
     "crava_hip_fft_r2c": (I, [VP, VP, I, I, I, I, I]),
     "crava_hip_fft_c2r": (I, [VP, VP, I, I, I, I, I]),
     "crava_hip_fft_pass": (I, [VP, VP, I, I, I, I, I, I, C.c_double]),
+    "crava_hip_fft_pass_many": (I, [VP, C.POINTER(VP), I, I, I, I, I, I, I, C.c_double]),
     "crava_hip_perk_tables": (I, [I, I, I, C.c_double, C.c_float, C.c_float, C.c_float, VP, VP, VP, VP, VP, VP, VP]),
     "crava_hip_post_create": (I, [VP, C.POINTER(PostParams), VP, VP, VP, VP, VP, C.POINTER(VP)]),
     "crava_hip_post_destroy": (I, [VP]),

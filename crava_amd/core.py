@@ -67,6 +67,12 @@ class Context:
         check(lib().crava_hip_fft_pass(self.h, _dev_ptr(grid), nxp, nyp, nzp, axis, int(inverse),
                                        precision_of(grid), scale), "crava_hip_fft_pass")
 
+    def fft_pass_many(self, grids, nxp, nyp, nzp, axis: int, inverse: bool, scale: float = 1.0,
+                      precision: int | None = None):
+        p = precision_of(grids[0]) if precision is None else precision
+        check(lib().crava_hip_fft_pass_many(self.h, ptr_array([_dev_ptr(g) for g in grids]), len(grids), nxp, nyp,
+                                            nzp, axis, int(inverse), p, scale), "crava_hip_fft_pass_many")
+
     def c2r(self, grid: torch.Tensor, nxp: int, nyp: int, nzp: int, cubetype: int, precision: int | None = None):
         p = precision_of(grid) if precision is None else precision
         check(lib().crava_hip_fft_c2r(self.h, _dev_ptr(grid), nxp, nyp, nzp, cubetype, p), "crava_hip_fft_c2r")

@@ -153,14 +153,26 @@ int crava_hip_fft_c2r_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, in
   return fft_common(ctx, grids, ngrid, nxp, nyp, nzp, cubetype, precision, true);
 }
 
-int crava_hip_fft_pass(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int axis, int inverse,
-                       int precision, double scale) {
-  if (!ctx || !grid || axis < 0 || axis > 2) { set_error("invalid arguments to crava_hip_fft_pass"); return CRAVA_HIP_E_INVALID; }
+int crava_hip_fft_pass_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp, int nyp, int nzp, int axis,
+                            int inverse, int precision, double scale) {
+  if (!ctx || !grids || axis < 0 || axis > 2 || ngrid < 1 || ngrid > crava::kMaxGrids) {
+    set_error("invalid arguments to crava_hip_fft_pass");
+    return CRAVA_HIP_E_INVALID;
+  }
   if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) { set_error("invalid precision"); return CRAVA_HIP_E_INVALID; }
   crava::Fft3D *f = nullptr;
   int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
   if (rc) return rc;
-  return f->pass(axis, &grid, 1, inverse != 0, &scale, ctx->stream);
+  if ((rc = f->ensure_scratch(ngrid))) return rc;
+  double sc[crava::kMaxGrids];
+  for (int i = 0; i < ngrid; ++i) sc[i] = scale;
+  return f->pass(axis, grids, ngrid, inverse != 0, sc, ctx->stream);
+}
+
+int crava_hip_fft_pass(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int axis, int inverse,
+                       int precision, double scale) {
+  if (!grid) { set_error("NULL grid"); return CRAVA_HIP_E_INVALID; }
+  return crava_hip_fft_pass_many(ctx, &grid, 1, nxp, nyp, nzp, axis, inverse, precision, scale);
 }
 
 int crava_hip_autotune_fft(crava_hip_ctx *ctx, void *scratch, int nxp, int nyp, int nzp, int precision,

@@ -31,6 +31,16 @@ __host__ __device__ __forceinline__ cx<T> scl(cx<T> a, T s) { return mk(a.x * s,
 template <typename T>
 __host__ __device__ __forceinline__ T norm2(cx<T> a) { return a.x * a.x + a.y * a.y; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Map dispatch-order id L (of N workgroups) to a work id so that every
+// XCD receives one contiguous range of work ids: neighbouring tiles, which
+// share HBM lines, then meet in the same L2.  A bijection on [0, N).
+__device__ __forceinline__ unsigned xcd_work_id(unsigned L, unsigned N) {
+  constexpr unsigned kXcd = 8;
+  const unsigned x = L % kXcd, i = L / kXcd, q = N / kXcd, r = N % kXcd;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // Last error message of the calling thread (exposed through the C ABI).
 void set_error(const std::string &msg);
 const char *get_error();

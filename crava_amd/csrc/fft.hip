@@ -77,14 +77,18 @@ template <int DIR, typename T, typename SI, typename SO>
 __global__ void __launch_bounds__(1024) k_fft_strided(GridSet gs, long long es, long long ncols, long long bstride,
                                                       PassPlan pl, const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
-  const cx<SI> *src = reinterpret_cast<const cx<SI> *>(gs.src[blockIdx.z]);
-  cx<SO> *dst = reinterpret_cast<cx<SO> *>(gs.dst[blockIdx.z]);
+  // (column block, batch, grid) of this workgroup, XCD-contiguous
+  const unsigned W = xcd_work_id(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                                 gridDim.x * gridDim.y * gridDim.z);
+  const unsigned bx = W % gridDim.x, by = (W / gridDim.x) % gridDim.y, bz = W / (gridDim.x * gridDim.y);
+  const cx<SI> *src = reinterpret_cast<const cx<SI> *>(gs.src[bz]);
+  cx<SO> *dst = reinterpret_cast<cx<SO> *>(gs.dst[bz]);
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + pl.n;
   const int nthr = blockDim.x;
-  const long long col0 = (long long)blockIdx.x * pl.tx;
+  const long long col0 = (long long)bx * pl.tx;
   const int ncol = (int)min((long long)pl.tx, ncols - col0);
-  const long long off = (long long)blockIdx.y * bstride + col0;
+  const long long off = (long long)by * bstride + col0;
   const cx<SI> *ibase = src + off;
   cx<SO> *obase = dst + off;
   for (int q = threadIdx.x; q < pl.n; q += nthr) tw[q] = twg[q];

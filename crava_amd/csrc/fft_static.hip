@@ -85,15 +85,19 @@ constexpr size_t lds_bytes(int tx) {
 template <int N, int TX, int THREADS, int DIR, typename T, typename SI, typename SO>
 __global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long es, long long ncols, long long bstride,
                                                           const cx<T> *__restrict__ twg) {
-  const cx<SI> *src = reinterpret_cast<const cx<SI> *>(gs.src[blockIdx.z]);
-  cx<SO> *dst = reinterpret_cast<cx<SO> *>(gs.dst[blockIdx.z]);
+  // (column block, batch, grid) of this workgroup, XCD-contiguous
+  const unsigned W = xcd_work_id(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                                 gridDim.x * gridDim.y * gridDim.z);
+  const unsigned bx = W % gridDim.x, by = (W / gridDim.x) % gridDim.y, bz = W / (gridDim.x * gridDim.y);
+  const cx<SI> *src = reinterpret_cast<const cx<SI> *>(gs.src[bz]);
+  cx<SO> *dst = reinterpret_cast<cx<SO> *>(gs.dst[bz]);
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
   constexpr int PITCH = SP<N>::pitch;
-  const long long col0 = (long long)blockIdx.x * TX;
+  const long long col0 = (long long)bx * TX;
   const int ncol = (int)min((long long)TX, ncols - col0);
-  const long long off = (long long)blockIdx.y * bstride + col0;
+  const long long off = (long long)by * bstride + col0;
   const cx<SI> *ibase = src + off;
   cx<SO> *obase = dst + off;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];

@@ -88,6 +88,9 @@ int crava_hip_fft_c2r_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, in
  * rfftwnd_one_*). */
 int crava_hip_fft_pass(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int axis, int inverse,
                        int precision, double scale);
+/* The same pass over a set of 1..16 grids in one launch. */
+int crava_hip_fft_pass_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp, int nyp, int nzp, int axis,
+                            int inverse, int precision, double scale);
 
 /* Autotuner: time every launch variant of each pass of the nxp x nyp x nzp
  * transform on `scratch` (a grid of that size; its contents are destroyed)
