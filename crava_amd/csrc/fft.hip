@@ -123,14 +123,17 @@ template <typename T, typename SI, typename SO>
 __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nrows, int rnxp, PassPlan pl,
                                                        const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
-  const SI *src = reinterpret_cast<const SI *>(gs.src[blockIdx.z]);
-  SO *dst = reinterpret_cast<SO *>(gs.dst[blockIdx.z]);
-  const T scale = (T)gs.scale[blockIdx.z];
+  // (row block, grid) of this workgroup, XCD-contiguous
+  const unsigned W = xcd_work_id(blockIdx.x + gridDim.x * blockIdx.z, gridDim.x * gridDim.z);
+  const unsigned bx = W % gridDim.x, bz = W / gridDim.x;
+  const SI *src = reinterpret_cast<const SI *>(gs.src[bz]);
+  SO *dst = reinterpret_cast<SO *>(gs.dst[bz]);
+  const T scale = (T)gs.scale[bz];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + pl.n;
   const int n = pl.n;
   const int nthr = blockDim.x;
-  const long long row0 = (long long)blockIdx.x * 2 * pl.tx;
+  const long long row0 = (long long)bx * 2 * pl.tx;
   for (int q = threadIdx.x; q < n; q += nthr) tw[q] = twg[q];
   const int tot = 2 * pl.tx * n;
   for (int q = threadIdx.x; q < tot; q += nthr) {
@@ -170,15 +173,18 @@ template <typename T, typename SI, typename SO>
 __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nrows, int rnxp, PassPlan pl,
                                                        const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
-  const SI *src = reinterpret_cast<const SI *>(gs.src[blockIdx.z]);
-  SO *dst = reinterpret_cast<SO *>(gs.dst[blockIdx.z]);
-  const T scale = (T)gs.scale[blockIdx.z];
+  // (row block, grid) of this workgroup, XCD-contiguous
+  const unsigned W = xcd_work_id(blockIdx.x + gridDim.x * blockIdx.z, gridDim.x * gridDim.z);
+  const unsigned bx = W % gridDim.x, bz = W / gridDim.x;
+  const SI *src = reinterpret_cast<const SI *>(gs.src[bz]);
+  SO *dst = reinterpret_cast<SO *>(gs.dst[bz]);
+  const T scale = (T)gs.scale[bz];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + pl.n;
   const int n = pl.n;
   const int cn = n / 2 + 1;
   const int nthr = blockDim.x;
-  const long long row0 = (long long)blockIdx.x * 2 * pl.tx;
+  const long long row0 = (long long)bx * 2 * pl.tx;
   for (int q = threadIdx.x; q < n; q += nthr) tw[q] = twg[q];
   const int tot = pl.tx * n;
   for (int q = threadIdx.x; q < tot; q += nthr) {

@@ -133,14 +133,17 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long 
 template <int N, int TX, int THREADS, typename T, typename SI, typename SO>
 __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nrows, int rnxp,
                                                        const cx<T> *__restrict__ twg) {
-  const SI *src = reinterpret_cast<const SI *>(gs.src[blockIdx.z]);
-  SO *dst = reinterpret_cast<SO *>(gs.dst[blockIdx.z]);
-  const T scale = (T)gs.scale[blockIdx.z];
+  // (row block, grid) of this workgroup, XCD-contiguous
+  const unsigned W = xcd_work_id(blockIdx.x + gridDim.x * blockIdx.z, gridDim.x * gridDim.z);
+  const unsigned bx = W % gridDim.x, bz = W / gridDim.x;
+  const SI *src = reinterpret_cast<const SI *>(gs.src[bz]);
+  SO *dst = reinterpret_cast<SO *>(gs.dst[bz]);
+  const T scale = (T)gs.scale[bz];
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
   constexpr int PITCH = SP<N>::pitch;
-  const long long row0 = (long long)blockIdx.x * 2 * TX;
+  const long long row0 = (long long)bx * 2 * TX;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
   static_assert(N % 2 == 0, "row kernels need an even length");
   constexpr int H = N / 2;           // value pairs per row
@@ -186,15 +189,18 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nro
 template <int N, int TX, int THREADS, typename T, typename SI, typename SO>
 __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nrows, int rnxp,
                                                        const cx<T> *__restrict__ twg) {
-  const SI *src = reinterpret_cast<const SI *>(gs.src[blockIdx.z]);
-  SO *dst = reinterpret_cast<SO *>(gs.dst[blockIdx.z]);
-  const T scale = (T)gs.scale[blockIdx.z];
+  // (row block, grid) of this workgroup, XCD-contiguous
+  const unsigned W = xcd_work_id(blockIdx.x + gridDim.x * blockIdx.z, gridDim.x * gridDim.z);
+  const unsigned bx = W % gridDim.x, bz = W / gridDim.x;
+  const SI *src = reinterpret_cast<const SI *>(gs.src[bz]);
+  SO *dst = reinterpret_cast<SO *>(gs.dst[bz]);
+  const T scale = (T)gs.scale[bz];
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
   constexpr int PITCH = SP<N>::pitch;
   constexpr int CN = N / 2 + 1;
-  const long long row0 = (long long)blockIdx.x * 2 * TX;
+  const long long row0 = (long long)bx * 2 * TX;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
   constexpr int TOT = TX * N;
   constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;

@@ -262,6 +262,8 @@ __device__ __forceinline__ void post_point(const PostArgs<G> &a, int k, long lon
 
 template <typename G, int NT, int MODE>
 __global__ void __launch_bounds__(256) k_posterior(PostArgs<G> a) {
+  // consecutive workgroups go to different XCDs, which spreads the in-band
+  // k-planes (a contiguous range of k) over all of them
   post_point<G, NT, MODE>(a, blockIdx.y, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
