@@ -18,6 +18,8 @@ E_HIP = -3
 E_STATE = -4
 
 DATA, PARAMETER, COVARIANCE = 0, 1, 2
+OP_ADD, OP_SUBTRACT, OP_MULTIPLY, OP_CHANGE_SIGN, OP_CONJUGATE, OP_MULTIPLY_SCALAR, OP_ADD_SCALAR, \
+    OP_FILL_CONSTANT = range(8)
 F32, MIXED, F64 = 0, 1, 2
 
 _lib = None
@@ -78,6 +80,12 @@ SIGNATURES = {
                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "crava_hip_fft_r2c_many": (I, [VP, C.POINTER(VP), I, I, I, I, I, I]),
     "crava_hip_fft_c2r_many": (I, [VP, C.POINTER(VP), I, I, I, I, I, I]),
+    "crava_hip_fill_param_corr": (I, [VP, C.POINTER(VP), I, C.POINTER(C.c_float), I, I, I, VP, VP, C.c_float,
+                                      C.c_float]),
+    "crava_hip_fill_err_corr": (I, [VP, VP, I, I, I, VP, C.c_float, C.c_float]),
+    "crava_hip_grid_op": (I, [VP, I, VP, VP, I, I, I, I, C.c_float]),
+    "crava_hip_data_variance": (I, [VP, VP, I, I, I, I, I, I, C.POINTER(C.c_float), C.POINTER(LL)]),
+    "crava_hip_facies_geomodel": (I, [VP, I, I, I, I, C.POINTER(C.c_float), C.POINTER(VP)]),
     "crava_hip_batch_create": (I, [VP, I, I, C.POINTER(BatchVolume), C.POINTER(VP)]),
     "crava_hip_batch_autotune": (I, [VP, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(C.c_float)]),
     "crava_hip_batch_set_launch": (I, [VP, I, I, I]),

@@ -21,8 +21,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcrava_hip.so")
 ARCH = os.environ.get("CRAVA_HIP_ARCH", "gfx950")
 
-# source -> extra flags.  The facies kernel must reproduce float/double
-# roundings bit for bit, so contraction into FMA is off there.
+# source -> extra flags.  The facies and grid kernels must reproduce
+# float/double roundings bit for bit, so contraction into FMA is off there.
 SOURCES = {
     "capi.hip": [],
     "fft_static.hip": [],
@@ -33,7 +33,7 @@ SOURCES = {
     "facies.hip": ["-ffp-contract=off"],
     "nll.hip": [],
     "batch.hip": [],
-    "fftgrid.cpp": [],
+    "grid.hip": ["-ffp-contract=off"],
 }
 
 COMMON = [

@@ -89,6 +89,39 @@ class Context:
                                            nzp, cubetype, p), "crava_hip_fft_c2r_many")
 
 
+def fill_param_corr(ctx: Context, grids, scales, nxp, nyp, nzp, corr_xy: torch.Tensor, circ_corr_t: torch.Tensor,
+                    grad_i: float = 0.0, grad_j: float = 0.0):
+    """crava_hip_fill_param_corr: the prior correlation times scales[g] into every grid."""
+    sc = (C.c_float * len(grids))(*[float(x) for x in scales])
+    check(lib().crava_hip_fill_param_corr(ctx.h, ptr_array([_dev_ptr(g) for g in grids]), len(grids), sc, nxp, nyp,
+                                          nzp, _dev_ptr(corr_xy), _dev_ptr(circ_corr_t), grad_i, grad_j),
+          "crava_hip_fill_param_corr")
+
+
+def fill_err_corr(ctx: Context, grid, nxp, nyp, nzp, corr_xy, grad_i: float = 0.0, grad_j: float = 0.0):
+    check(lib().crava_hip_fill_err_corr(ctx.h, _dev_ptr(grid), nxp, nyp, nzp, _dev_ptr(corr_xy), grad_i, grad_j),
+          "crava_hip_fill_err_corr")
+
+
+def grid_op(ctx: Context, op: int, grid, other, nxp, nyp, nzp, transformed: bool, scalar: float = 0.0):
+    check(lib().crava_hip_grid_op(ctx.h, op, _dev_ptr(grid), None if other is None else _dev_ptr(other), nxp, nyp,
+                                  nzp, int(transformed), scalar), "crava_hip_grid_op")
+
+
+def data_variance(ctx: Context, grid, nx, ny, nz, nxp, nyp, nzp):
+    """Returns (float variance, ndata)."""
+    v, n = C.c_float(0), C.c_longlong(0)
+    check(lib().crava_hip_data_variance(ctx.h, _dev_ptr(grid), nx, ny, nz, nxp, nyp, nzp, C.byref(v), C.byref(n)),
+          "crava_hip_data_variance")
+    return v.value, n.value
+
+
+def facies_geomodel(ctx: Context, nx, ny, nz, prior, prob):
+    pr = (C.c_float * len(prior))(*[float(x) for x in prior])
+    check(lib().crava_hip_facies_geomodel(ctx.h, nx, ny, nz, len(prior), pr, ptr_array([_dev_ptr(g) for g in prob])),
+          "crava_hip_facies_geomodel")
+
+
 def perk_tables(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wavelets: np.ndarray, wnorm: np.ndarray):
     """Host per-k vectors (crava_hip_perk_tables)."""
     wav = np.ascontiguousarray(wavelets, dtype=np.float32)

@@ -5,6 +5,7 @@
 #include "context.hpp"
 #include "fft.hpp"
 #include "facies.hpp"
+#include "grid.hpp"
 #include "nll.hpp"
 #include "posterior.hpp"
 #include "simulate.hpp"
@@ -38,9 +39,22 @@ int Context::plan(int nxp, int nyp, int nzp, int prec, Fft3D **out) {
   return CRAVA_HIP_OK;
 }
 
+int Context::workspace(size_t bytes, void **out) {
+  if (bytes > work_bytes) {
+    if (work) (void)hipFree(work);
+    work = nullptr;
+    work_bytes = 0;
+    CRAVA_HIP_CHECK(hipMalloc(&work, bytes));
+    work_bytes = bytes;
+  }
+  *out = work;
+  return CRAVA_HIP_OK;
+}
+
 Context::~Context() {
   plans.clear();
   if (counters) (void)hipFree(counters);
+  if (work) (void)hipFree(work);
 }
 }  // namespace crava
 
@@ -538,4 +552,97 @@ int crava_hip_batch_counts(crava_hip_batch *batch, long long *n_fail, long long 
 int crava_hip_batch_destroy(crava_hip_batch *batch) {
   delete batch;
   return CRAVA_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grid fills, element-wise operators, data variance, facies geomodel.
+static bool grid_dims_ok(int nxp, int nyp, int nzp) { return nxp >= 1 && nyp >= 1 && nzp >= 1; }
+
+int crava_hip_fill_param_corr(crava_hip_ctx *ctx, void *const *grids, int ngrid, const float *scale, int nxp, int nyp,
+                              int nzp, const float *corr_xy, const float *circ_corr_t, float grad_i, float grad_j) {
+  if (!ctx || !grids || !scale || !corr_xy || !circ_corr_t || ngrid < 1 || ngrid > crava::kMaxGrids ||
+      !grid_dims_ok(nxp, nyp, nzp)) {
+    set_error("invalid arguments to crava_hip_fill_param_corr");
+    return CRAVA_HIP_E_INVALID;
+  }
+  crava::GridSet gs{};
+  gs.n = ngrid;
+  for (int g = 0; g < ngrid; ++g) {
+    if (!grids[g]) { set_error("NULL grid"); return CRAVA_HIP_E_INVALID; }
+    gs.dst[g] = grids[g];
+    gs.src[g] = grids[g];
+    gs.scale[g] = (double)scale[g];
+  }
+  return crava::launch_fill_param_corr(gs, nxp, nyp, nzp, corr_xy, circ_corr_t, grad_i, grad_j, ctx->stream);
+}
+
+int crava_hip_fill_err_corr(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, const float *corr_xy,
+                            float grad_i, float grad_j) {
+  if (!ctx || !grid || !corr_xy || !grid_dims_ok(nxp, nyp, nzp)) {
+    set_error("invalid arguments to crava_hip_fill_err_corr");
+    return CRAVA_HIP_E_INVALID;
+  }
+  return crava::launch_fill_err_corr(static_cast<float *>(grid), nxp, nyp, nzp, corr_xy, grad_i, grad_j, ctx->stream);
+}
+
+int crava_hip_grid_op(crava_hip_ctx *ctx, int op, void *grid, const void *other, int nxp, int nyp, int nzp,
+                      int transformed, float scalar) {
+  if (!ctx || !grid || !grid_dims_ok(nxp, nyp, nzp) || op < CRAVA_HIP_OP_ADD || op > CRAVA_HIP_OP_FILL_CONSTANT) {
+    set_error("invalid arguments to crava_hip_grid_op");
+    return CRAVA_HIP_E_INVALID;
+  }
+  const bool binary = op == CRAVA_HIP_OP_ADD || op == CRAVA_HIP_OP_SUBTRACT || op == CRAVA_HIP_OP_MULTIPLY;
+  if (binary && !other) { set_error("this operator needs a second grid"); return CRAVA_HIP_E_INVALID; }
+  // the reference asserts the domain of these operators (src/fftgrid.cpp:1903,1976,1984)
+  if (transformed && (op == CRAVA_HIP_OP_MULTIPLY_SCALAR || op == CRAVA_HIP_OP_ADD_SCALAR ||
+                      op == CRAVA_HIP_OP_FILL_CONSTANT)) {
+    set_error("scalar operators apply in the real domain only");
+    return CRAVA_HIP_E_STATE;
+  }
+  if (!transformed && op == CRAVA_HIP_OP_CONJUGATE) {
+    set_error("conjugate applies to transformed grids only");
+    return CRAVA_HIP_E_STATE;
+  }
+  const long long csize = (long long)(nxp / 2 + 1) * nyp * nzp;
+  const long long n = transformed ? csize : 2 * csize;
+  return crava::launch_grid_op(op, static_cast<float *>(grid), static_cast<const float *>(other), n, transformed != 0,
+                               scalar, ctx->stream);
+}
+
+int crava_hip_data_variance(crava_hip_ctx *ctx, const void *grid, int nx, int ny, int nz, int nxp, int nyp, int nzp,
+                            float *variance, long long *ndata) {
+  if (!ctx || !grid || !variance || !grid_dims_ok(nxp, nyp, nzp) || nx < 0 || ny < 0 || nz < 0 || nx > nxp ||
+      ny > nyp || nz > nzp) {
+    set_error("invalid arguments to crava_hip_data_variance");
+    return CRAVA_HIP_E_INVALID;
+  }
+  void *work = nullptr;
+  int rc = ctx->workspace(sizeof(double) * crava::data_variance_work_doubles(nyp, nzp), &work);
+  if (rc) return rc;
+  double out[2] = {0.0, 0.0};
+  if ((rc = crava::data_variance(static_cast<const float *>(grid), nx, ny, nz, nxp, nyp, nzp,
+                                 static_cast<double *>(work), out, ctx->stream)))
+    return rc;
+  const long long cnt = (long long)out[1];
+  *variance = cnt == 0 ? 0.0f : static_cast<float>(out[0] / static_cast<double>(cnt));
+  if (ndata) *ndata = cnt;
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_facies_geomodel(crava_hip_ctx *ctx, int nx, int ny, int nz, int nfac, const float *prior,
+                              float *const *prob) {
+  if (!ctx || !prior || !prob || nfac < 1 || nfac > crava::kMaxGrids - 1 || nx < 1 || ny < 1 || nz < 1) {
+    set_error("invalid arguments to crava_hip_facies_geomodel");
+    return CRAVA_HIP_E_INVALID;
+  }
+  crava::GridSet gs{};
+  gs.n = nfac;
+  for (int l = 0; l <= nfac; ++l)
+    if (!prob[l]) { set_error("NULL probability grid"); return CRAVA_HIP_E_INVALID; }
+  for (int l = 0; l < nfac; ++l) {
+    gs.dst[l] = prob[l];
+    gs.src[l] = prob[l];
+    gs.scale[l] = (double)prior[l];
+  }
+  return crava::launch_facies_geomodel(gs, prob[nfac], nfac, nx, ny, nz, ctx->stream);
 }

@@ -14,6 +14,10 @@ struct Context {
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<Fft3D>> plans;
   // prec: 0 float, 1 float storage / double arithmetic, 2 double
   int plan(int nxp, int nyp, int nzp, int prec, Fft3D **out);
+  // Device workspace of at least `bytes`, kept and grown across calls.
+  void *work = nullptr;
+  size_t work_bytes = 0;
+  int workspace(size_t bytes, void **out);
   ~Context();
 };
 }  // namespace crava

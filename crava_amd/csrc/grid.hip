@@ -1,0 +1,260 @@
+// Grid-wide kernels of FFTGrid and its callers on the padded FFTGrid layout:
+// the prior and error correlation fills, the element-wise operators, the
+// data-variance reduction and the facies geomodel update.  Each reproduces
+// the reference's float arithmetic operation by operation (this file is
+// compiled with contraction off), so results are bit-identical.
+//
+// Every kernel is one pass over HBM: a grid-stride loop over the rsize (or
+// csize) values of the grid, 256 threads per workgroup.
+#include "grid.hpp"
+#include "../../include/crava_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace crava {
+
+constexpr float kRMissing = -99999.0f;  // RMISSING, src/definitions.h:35
+
+static unsigned blocks_for(long long n) {
+  return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 1 << 20));
+}
+
+// FFTGrid::fillInParamCorr (src/fftgrid.cpp:1119-1164) followed by
+// multiplyByScalar(scale[g]) (src/seismicparametersholder.cpp:210-215) for
+// each grid g of the set: the same correlation written to up to 16 grids.
+__global__ void k_fill_param_corr(GridSet gs, int nxp, int nyp, int nzp, const float *__restrict__ corr_xy,
+                                  const float *__restrict__ circ_t, float grad_i, float grad_j) {
+  const int rnxp = 2 * (nxp / 2 + 1);
+  const long long rsize = (long long)rnxp * nyp * nzp;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < rsize;
+       p += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(p % rnxp);
+    const long long r = p / rnxp;
+    const int j = (int)(r % nyp);
+    const int k = (int)(r / nyp);
+    float value;
+    if (i < nxp) {
+      int cycle_i = i - nxp;
+      if (i < -cycle_i) cycle_i = i;
+      int cycle_j = j - nyp;
+      if (j < -cycle_j) cycle_j = j;
+      float sub_k = (float)k + (float)cycle_i * grad_i + (float)cycle_j * grad_j;
+      int base_k = (int)floorf(sub_k);
+      sub_k -= (float)base_k;
+      while (base_k < 0) base_k += nzp;
+      while (base_k >= nzp) base_k -= nzp;
+      value = (1.0f - sub_k) * circ_t[base_k];
+      if (base_k != nzp - 1) value += sub_k * circ_t[base_k + 1];
+      else value += sub_k * circ_t[0];
+      value *= corr_xy[i + nxp * j];
+    } else {
+      value = kRMissing;
+    }
+    for (int g = 0; g < gs.n; ++g) static_cast<float *>(gs.dst[g])[p] = value * (float)gs.scale[g];
+  }
+}
+
+// FFTGrid::fillInErrCorr (src/fftgrid.cpp:1065-1116), range 1.
+__global__ void k_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *__restrict__ corr_xy,
+                                float grad_i, float grad_j) {
+  const int rnxp = 2 * (nxp / 2 + 1);
+  const long long rsize = (long long)rnxp * nyp * nzp;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < rsize;
+       p += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(p % rnxp);
+    const long long r = p / rnxp;
+    const int j = (int)(r % nyp);
+    const int k = (int)(r / nyp);
+    int cycle_i = i - nxp;
+    if (i < -cycle_i) cycle_i = i;
+    int cycle_j = j - nyp;
+    if (j < -cycle_j) cycle_j = j;
+    float sub_k = (float)k + (float)cycle_i * grad_i + (float)cycle_j * grad_j;
+    float value;
+    if (i < nxp) {
+      if (fabsf(sub_k) < 1.0f || fabsf(sub_k - (float)nzp) < 1.0f) {
+        int base_k = (int)sub_k;
+        sub_k -= (float)base_k;
+        while (base_k < -1) base_k += nzp;
+        while (base_k >= 1) base_k -= nzp;
+        if (base_k < 0) {
+          base_k = -1 - base_k;
+          sub_k = 1.0f - sub_k;
+        }
+        value = (1.0f - sub_k) * corr_xy[i + nxp * j];
+      } else {
+        value = 0.0f;
+      }
+    } else {
+      value = kRMissing;
+    }
+    grid[p] = value;
+  }
+}
+
+// Element-wise operators (src/fftgrid.cpp:1877-1999), over n floats (real
+// domain) or n complex values (transformed).
+__global__ void k_grid_op(int op, float *a, const float *__restrict__ b, long long n, int transformed, float s) {
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+    if (transformed) {
+      float2 *ac = reinterpret_cast<float2 *>(a);
+      const float2 *bc = reinterpret_cast<const float2 *>(b);
+      float2 v = ac[p];
+      switch (op) {
+        case CRAVA_HIP_OP_ADD: { const float2 w = bc[p]; v.x += w.x; v.y += w.y; break; }
+        case CRAVA_HIP_OP_SUBTRACT: { const float2 w = bc[p]; v.x -= w.x; v.y -= w.y; break; }
+        case CRAVA_HIP_OP_MULTIPLY: {
+          const float2 w = bc[p], t = v;
+          v.x = w.x * t.x - w.y * t.y;
+          v.y = w.y * t.x + w.x * t.y;
+          break;
+        }
+        case CRAVA_HIP_OP_CHANGE_SIGN: v.x = -v.x; v.y = -v.y; break;
+        case CRAVA_HIP_OP_CONJUGATE: v.y = -v.y; break;
+        default: break;
+      }
+      ac[p] = v;
+    } else {
+      float v = a[p];
+      switch (op) {
+        case CRAVA_HIP_OP_ADD: v += b[p]; break;
+        case CRAVA_HIP_OP_SUBTRACT: v -= b[p]; break;
+        case CRAVA_HIP_OP_MULTIPLY: v *= b[p]; break;
+        case CRAVA_HIP_OP_CHANGE_SIGN: v = -v; break;
+        case CRAVA_HIP_OP_MULTIPLY_SCALAR: v *= s; break;
+        case CRAVA_HIP_OP_ADD_SCALAR: v += s; break;
+        case CRAVA_HIP_OP_FILL_CONSTANT: v = s; break;
+        default: break;
+      }
+      a[p] = v;
+    }
+  }
+}
+
+// Crava::computeDataVariance (src/crava.cpp:322-363) in the reference's
+// summation order: per (j,k) row a double sum over i (one thread per row),
+// per k a sum over rows, then over k.
+__global__ void k_row_sumsq(const float *grid, int nx, int ny, int nz, int rnxp, int nyp, int nzp, double *rowsum,
+                            long long *rowcnt) {
+  const long long nrows = (long long)nyp * nzp;
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+       r += (long long)gridDim.x * blockDim.x) {
+    const int j = (int)(r % nyp), k = (int)(r / nyp);
+    double s = 0.0;
+    long long c = 0;
+    if (k < nz && j < ny) {
+      const float *row = grid + r * rnxp;
+      for (int i = 0; i < nx; ++i) {
+        const float t = row[i];
+        if (t != 0.0f) {
+          s += (double)(t * t);
+          ++c;
+        }
+      }
+    }
+    rowsum[r] = s;
+    rowcnt[r] = c;
+  }
+}
+
+__global__ void k_plane_sum(const double *rowsum, const long long *rowcnt, int nyp, int nzp, double *planesum,
+                            long long *planecnt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nzp) return;
+  double s = 0.0;
+  long long c = 0;
+  for (int j = 0; j < nyp; ++j) {
+    s += rowsum[(long long)k * nyp + j];
+    c += rowcnt[(long long)k * nyp + j];
+  }
+  planesum[k] = s;
+  planecnt[k] = c;
+}
+
+__global__ void k_total_sum(const double *planesum, const long long *planecnt, int nzp, double *out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double s = 0.0;
+  long long c = 0;
+  for (int k = 0; k < nzp; ++k) {
+    s += planesum[k];
+    c += planecnt[k];
+  }
+  out[0] = s;
+  out[1] = (double)c;
+}
+
+// FaciesProb::calculateFaciesProbGeomodel (src/faciesprob.cpp:2286-2330),
+// constant prior probabilities (probs.scale[l]): prob_l <- prior_l * undef +
+// prob_l inside the volume, RMISSING in the x padding, on grids of
+// nz x ny x 2*(nx/2+1).
+__global__ void k_facies_geomodel(GridSet probs, const float *__restrict__ undef, int nfac, int nx, int ny, int nz) {
+  const int rnx = 2 * (nx / 2 + 1);
+  const long long n = (long long)rnx * ny * nz;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(p % rnx);
+    const float u = undef[p];
+    for (int l = 0; l < nfac; ++l) {
+      float *g = static_cast<float *>(probs.dst[l]);
+      const float v = (float)probs.scale[l] * u + g[p];
+      g[p] = i < nx ? v : kRMissing;
+    }
+  }
+}
+
+int launch_fill_param_corr(const GridSet &gs, int nxp, int nyp, int nzp, const float *corr_xy, const float *circ_t,
+                           float grad_i, float grad_j, hipStream_t s) {
+  const long long rsize = 2LL * (nxp / 2 + 1) * nyp * nzp;
+  hipLaunchKernelGGL(k_fill_param_corr, dim3(blocks_for(rsize)), dim3(256), 0, s, gs, nxp, nyp, nzp, corr_xy,
+                     circ_t, grad_i, grad_j);
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
+int launch_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *corr_xy, float grad_i, float grad_j,
+                         hipStream_t s) {
+  const long long rsize = 2LL * (nxp / 2 + 1) * nyp * nzp;
+  hipLaunchKernelGGL(k_fill_err_corr, dim3(blocks_for(rsize)), dim3(256), 0, s, grid, nxp, nyp, nzp, corr_xy,
+                     grad_i, grad_j);
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
+int launch_grid_op(int op, float *a, const float *b, long long n, bool transformed, float scalar, hipStream_t s) {
+  hipLaunchKernelGGL(k_grid_op, dim3(blocks_for(n)), dim3(256), 0, s, op, a, b, n, transformed ? 1 : 0, scalar);
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
+int data_variance(const float *grid, int nx, int ny, int nz, int nxp, int nyp, int nzp, double *work,
+                  double *out2, hipStream_t s) {
+  const long long nrows = (long long)nyp * nzp;
+  double *rowsum = work;
+  long long *rowcnt = reinterpret_cast<long long *>(work + nrows);
+  double *planesum = work + 2 * nrows;
+  long long *planecnt = reinterpret_cast<long long *>(work + 2 * nrows + nzp);
+  double *tot = work + 2 * nrows + 2 * nzp;
+  const int rnxp = 2 * (nxp / 2 + 1);
+  hipLaunchKernelGGL(k_row_sumsq, dim3(blocks_for(nrows)), dim3(256), 0, s, grid, nx, ny, nz, rnxp, nyp, nzp, rowsum,
+                     rowcnt);
+  hipLaunchKernelGGL(k_plane_sum, dim3((nzp + 255) / 256), dim3(256), 0, s, rowsum, rowcnt, nyp, nzp, planesum,
+                     planecnt);
+  hipLaunchKernelGGL(k_total_sum, dim3(1), dim3(64), 0, s, planesum, planecnt, nzp, tot);
+  CRAVA_HIP_CHECK(hipGetLastError());
+  CRAVA_HIP_CHECK(hipMemcpyAsync(out2, tot, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  CRAVA_HIP_CHECK(hipStreamSynchronize(s));
+  return CRAVA_HIP_OK;
+}
+
+size_t data_variance_work_doubles(int nyp, int nzp) { return 2 * (size_t)nyp * nzp + 2 * (size_t)nzp + 2; }
+
+int launch_facies_geomodel(const GridSet &probs, const float *undef, int nfac, int nx, int ny, int nz,
+                           hipStream_t s) {
+  const long long n = 2LL * (nx / 2 + 1) * ny * nz;
+  hipLaunchKernelGGL(k_facies_geomodel, dim3(blocks_for(n)), dim3(256), 0, s, probs, undef, nfac, nx, ny, nz);
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
+}  // namespace crava

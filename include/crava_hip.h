@@ -214,6 +214,57 @@ int crava_hip_facies(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int ny
                      const double *volume, const int *vn, const float *prior, float p_undefined, float *const *prob,
                      int *argmax, unsigned long long *counts);
 
+/* Prior correlation of the parameter covariance grids, in place of
+ * FFTGrid::fillInParamCorr (src/fftgrid.cpp:1119-1164) followed by
+ * multiplyByScalar(Sigma0_ab) as in
+ * SeismicParametersHolder::initializeCorrelations
+ * (src/seismicparametersholder.cpp:192-225): one pass writes every grid of
+ * the set (1..16 real padded float grids, time domain):
+ *   grid_g(i,j,k) = [(1-f) c(b) + f c(b+1 mod nzp)] * corr_xy[i + nxp*j] * scale[g]
+ * with b + f = k + ci*grad_i + cj*grad_j (ci, cj the signed lateral lags),
+ * RMISSING*scale[g] in the x padding.  corr_xy: device float [nyp*nxp];
+ * circ_corr_t: device float [nzp] (computeCircCorrT); scale: host. */
+int crava_hip_fill_param_corr(crava_hip_ctx *ctx, void *const *grids, int ngrid, const float *scale, int nxp, int nyp,
+                              int nzp, const float *corr_xy, const float *circ_corr_t, float grad_i, float grad_j);
+
+/* Error correlation grid, in place of FFTGrid::fillInErrCorr
+ * (src/fftgrid.cpp:1065-1116): the lateral correlation at the (shifted) zero
+ * time lag, linearly split between the two nearest samples; 0 elsewhere;
+ * RMISSING in the x padding. */
+int crava_hip_fill_err_corr(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, const float *corr_xy,
+                            float grad_i, float grad_j);
+
+/* Element-wise operators of FFTGrid (src/fftgrid.cpp:1877-1999) on a float
+ * grid in the real domain (transformed = 0: rsize floats incl. padding) or
+ * transformed (csize complex).  `other` is the second grid of the binary
+ * operators; `scalar` the value of the scalar ones.  Scalar operators and
+ * the fill are real-domain only and conjugate is transformed-only, as the
+ * reference asserts (CRAVA_HIP_E_STATE otherwise). */
+#define CRAVA_HIP_OP_ADD 0             /* add(grid)          */
+#define CRAVA_HIP_OP_SUBTRACT 1        /* subtract(grid)     */
+#define CRAVA_HIP_OP_MULTIPLY 2        /* multiply(grid)     */
+#define CRAVA_HIP_OP_CHANGE_SIGN 3     /* changeSign()       */
+#define CRAVA_HIP_OP_CONJUGATE 4       /* conjugate()        */
+#define CRAVA_HIP_OP_MULTIPLY_SCALAR 5 /* multiplyByScalar() */
+#define CRAVA_HIP_OP_ADD_SCALAR 6      /* addScalar()        */
+#define CRAVA_HIP_OP_FILL_CONSTANT 7   /* fillInConstant()   */
+int crava_hip_grid_op(crava_hip_ctx *ctx, int op, void *grid, const void *other, int nxp, int nyp, int nzp,
+                      int transformed, float scalar);
+
+/* Variance of one seismic stack, in place of Crava::computeDataVariance
+ * (src/crava.cpp:322-363): mean of d^2 over the nonzero samples inside
+ * nx x ny x nz, summed in the reference's order (double per row, rows, then
+ * planes), so the float result is identical.  Synchronises. */
+int crava_hip_data_variance(crava_hip_ctx *ctx, const void *grid, int nx, int ny, int nz, int nxp, int nyp, int nzp,
+                            float *variance, long long *ndata);
+
+/* Facies probabilities with constant prior facies probabilities, in place
+ * of FaciesProb::calculateFaciesProbGeomodel (src/faciesprob.cpp:2286-2330):
+ * prob[l] <- prior[l] * prob[nfac] + prob[l] inside nx, RMISSING in the x
+ * padding; prob[] as in crava_hip_facies (nfac + 1 grids, last = undefined). */
+int crava_hip_facies_geomodel(crava_hip_ctx *ctx, int nx, int ny, int nz, int nfac, const float *prior,
+                              float *const *prob);
+
 /* One inversion volume of a batch: its problem (from crava_hip_post_create,
  * which fixes nxp, nyp, nzp and ntheta) and its device grids in the FFTGrid
  * layout, float (CRAVA_HIP_F32 / CRAVA_HIP_MIXED) or double (CRAVA_HIP_F64). */

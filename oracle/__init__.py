@@ -58,6 +58,12 @@ def orc():
         L.orc_chol_cpx.argtypes = [I, VP]
         L.orc_axeqb_mat_cpx.argtypes = [I, VP, VP, I]
         L.orc_facies.argtypes = [I, I, I, I, I, I, VP, VP, VP, I, VP, I, I, I, VP, VP, VP, F, PP]
+        L.orc_fill_param_corr.argtypes = [VP, I, I, I, VP, VP, F, F, F]
+        L.orc_fill_err_corr.argtypes = [VP, I, I, I, VP, F, F]
+        L.orc_grid_op.argtypes = [I, VP, VP, LL, I, F]
+        L.orc_data_variance.argtypes = [VP, I, I, I, I, I, I, C.POINTER(C.c_longlong)]
+        L.orc_data_variance.restype = C.c_float
+        L.orc_facies_geomodel.argtypes = [I, I, I, I, VP, PP]
         L.orc_posterior.restype = C.c_longlong
         L.orc_posterior_f64.restype = C.c_longlong
         L.orc_simulate_points.restype = C.c_longlong
@@ -217,3 +223,41 @@ def facies(nx, ny, nz, nxp, nyp, nzp, alpha, beta, rho, tables, tn, vol, vn, pri
                      tab.ctypes.data, tn[0], tn[1], tn[2], v.ctypes.data, n.ctypes.data, pr.ctypes.data,
                      p_undef, _ptrs(out))
     return out
+
+
+# ------------------------------------------------------------ grid fills / operators
+def fill_param_corr(nxp, nyp, nzp, corr_xy, circ_t, grad_i, grad_j, scale) -> np.ndarray:
+    rnxp = 2 * (nxp // 2 + 1)
+    g = np.zeros((nzp, nyp, rnxp), dtype=np.float32)
+    cxy = np.ascontiguousarray(corr_xy, dtype=np.float32)
+    ct = np.ascontiguousarray(circ_t, dtype=np.float32)
+    orc().orc_fill_param_corr(g.ctypes.data, nxp, nyp, nzp, cxy.ctypes.data, ct.ctypes.data, grad_i, grad_j, scale)
+    return g
+
+
+def fill_err_corr(nxp, nyp, nzp, corr_xy, grad_i, grad_j) -> np.ndarray:
+    rnxp = 2 * (nxp // 2 + 1)
+    g = np.zeros((nzp, nyp, rnxp), dtype=np.float32)
+    cxy = np.ascontiguousarray(corr_xy, dtype=np.float32)
+    orc().orc_fill_err_corr(g.ctypes.data, nxp, nyp, nzp, cxy.ctypes.data, grad_i, grad_j)
+    return g
+
+
+def grid_op(op, a: np.ndarray, b, transformed: bool, scalar: float = 0.0) -> np.ndarray:
+    """In place on the float32 array `a` (all of it: rsize floats or csize complex)."""
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    n = a.size // 2 if transformed else a.size
+    orc().orc_grid_op(op, a.ctypes.data, None if b is None else b.ctypes.data, n, int(transformed), scalar)
+    return a
+
+
+def data_variance(grid, nx, ny, nz, nxp, nyp, nzp):
+    n = C.c_longlong(0)
+    v = orc().orc_data_variance(grid.ctypes.data, nx, ny, nz, nxp, nyp, nzp, C.byref(n))
+    return float(np.float32(v)), n.value
+
+
+def facies_geomodel(nx, ny, nz, prior, prob):
+    pr = np.ascontiguousarray(prior, dtype=np.float32)
+    orc().orc_facies_geomodel(nx, ny, nz, len(prior), pr.ctypes.data, _ptrs(prob))
+    return prob

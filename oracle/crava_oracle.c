@@ -971,3 +971,147 @@ void orc_facies(int nx, int ny, int nz, int nxp, int nyp, int nzp,
         }
       }
 }
+
+/* ========================================================================
+ * Grid fills and operators (the callers either side of the transforms).
+ * ====================================================================== */
+
+/* FFTGrid::fillInParamCorr (src/fftgrid.cpp:1119-1164), then
+ * multiplyByScalar(scale) over all rsize values (src/fftgrid.cpp:1992-1999,
+ * as SeismicParametersHolder::initializeCorrelations applies it). */
+void orc_fill_param_corr(float *grid, int nxp, int nyp, int nzp, const float *corr_xy,
+                         const float *circ_t, float grad_i, float grad_j, float scale)
+{
+  int rnxp = 2 * (nxp / 2 + 1);
+  long long p = 0;
+  for (int k = 0; k < nzp; k++)
+    for (int j = 0; j < nyp; j++)
+      for (int i = 0; i < rnxp; i++, p++) {
+        float value;
+        if (i < nxp) {
+          int ci = (i < nxp - i) ? i : i - nxp;   /* signed lateral lag */
+          int cj = (j < nyp - j) ? j : j - nyp;
+          float t = (float)k + (float)ci * grad_i;
+          t = t + (float)cj * grad_j;
+          int b = (int)floorf(t);
+          float f = t - (float)b;
+          b = ((b % nzp) + nzp) % nzp;
+          value = (1.0f - f) * circ_t[b];
+          value = value + f * circ_t[b == nzp - 1 ? 0 : b + 1];
+          value = value * corr_xy[i + nxp * j];
+        } else {
+          value = -99999.0f;                       /* RMISSING */
+        }
+        grid[p] = value * scale;
+      }
+}
+
+/* FFTGrid::fillInErrCorr (src/fftgrid.cpp:1065-1116) with range 1: the
+ * lateral correlation lands on the time samples within one sample of the
+ * shifted zero lag (cyclically), weighted by 1 - distance. */
+void orc_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *corr_xy,
+                       float grad_i, float grad_j)
+{
+  int rnxp = 2 * (nxp / 2 + 1);
+  long long p = 0;
+  for (int k = 0; k < nzp; k++)
+    for (int j = 0; j < nyp; j++)
+      for (int i = 0; i < rnxp; i++, p++) {
+        int ci = (i < nxp - i) ? i : i - nxp;
+        int cj = (j < nyp - j) ? j : j - nyp;
+        float t = (float)k + (float)ci * grad_i;
+        t = t + (float)cj * grad_j;
+        float value = 0.0f;
+        if (i >= nxp) {
+          value = -99999.0f;
+        } else if (fabsf(t) < 1.0f || fabsf(t - (float)nzp) < 1.0f) {
+          int b = (int)t;                 /* truncation toward zero */
+          float f = t - (float)b;
+          /* b is 0 or nzp-1 (or -1 / nzp for tiny negative t); reduce to
+           * {-1, 0} and weight by the distance to the zero lag */
+          while (b < -1) b += nzp;
+          while (b >= 1) b -= nzp;
+          if (b < 0) f = 1.0f - f;
+          value = (1.0f - f) * corr_xy[i + nxp * j];
+        }
+        grid[p] = value;
+      }
+}
+
+/* Element-wise operators of FFTGrid (src/fftgrid.cpp:1877-1999); op codes as
+ * CRAVA_HIP_OP_*; n floats (real) or n complex values (transformed). */
+void orc_grid_op(int op, float *a, const float *b, long long n, int transformed, float s)
+{
+  for (long long p = 0; p < n; p++) {
+    if (transformed) {
+      float ar = a[2 * p], ai = a[2 * p + 1];
+      float br = b ? b[2 * p] : 0.0f, bi = b ? b[2 * p + 1] : 0.0f;
+      switch (op) {
+        case 0: ar = ar + br; ai = ai + bi; break;
+        case 1: ar = ar - br; ai = ai - bi; break;
+        case 2: { float r = br * ar - bi * ai; float im = bi * ar + br * ai; ar = r; ai = im; break; }
+        case 3: ar = -ar; ai = -ai; break;
+        case 4: ai = -ai; break;
+        default: break;
+      }
+      a[2 * p] = ar;
+      a[2 * p + 1] = ai;
+    } else {
+      float v = a[p];
+      switch (op) {
+        case 0: v = v + b[p]; break;
+        case 1: v = v - b[p]; break;
+        case 2: v = v * b[p]; break;
+        case 3: v = -v; break;
+        case 5: v = v * s; break;
+        case 6: v = v + s; break;
+        case 7: v = s; break;
+        default: break;
+      }
+      a[p] = v;
+    }
+  }
+}
+
+/* Crava::computeDataVariance (src/crava.cpp:322-363) for one stack. */
+float orc_data_variance(const float *grid, int nx, int ny, int nz, int nxp, int nyp, int nzp,
+                        long long *ndata)
+{
+  int rnxp = 2 * (nxp / 2 + 1);
+  double tot = 0.0;
+  long long cnt = 0, p = 0;
+  for (int k = 0; k < nzp; k++) {
+    double plane = 0.0;
+    for (int j = 0; j < nyp; j++) {
+      double row = 0.0;
+      for (int i = 0; i < rnxp; i++, p++) {
+        float t = grid[p];
+        if (k < nz && j < ny && i < nx && t != 0.0f) {
+          row += (double)(t * t);
+          cnt++;
+        }
+      }
+      plane += row;
+    }
+    tot += plane;
+  }
+  if (ndata) *ndata = cnt;
+  return cnt == 0 ? 0.0f : (float)(tot / (double)cnt);
+}
+
+/* FaciesProb::calculateFaciesProbGeomodel (src/faciesprob.cpp:2286-2330),
+ * constant priors; grids nz x ny x 2*(nx/2+1), prob[nfac] = undefined. */
+void orc_facies_geomodel(int nx, int ny, int nz, int nfac, const float *prior, float **prob)
+{
+  int rnx = 2 * (nx / 2 + 1);
+  long long p = 0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < rnx; i++, p++) {
+        float u = prob[nfac][p];
+        for (int l = 0; l < nfac; l++) {
+          float v = prior[l] * u + prob[l][p];
+          prob[l][p] = (i < nx) ? v : -99999.0f;
+        }
+      }
+}
