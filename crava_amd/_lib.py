@@ -62,6 +62,10 @@ SIGNATURES = {
     "crava_hip_ctx_destroy": (I, [VP]),
     "crava_hip_ctx_set_stream": (I, [VP, VP]),
     "crava_hip_synchronize": (I, [VP]),
+    "crava_hip_grid_alloc": (I, [VP, I, I, I, I, C.POINTER(VP)]),
+    "crava_hip_grid_free": (I, [VP]),
+    "crava_hip_grid_upload": (I, [VP, VP, VP, C.c_size_t]),
+    "crava_hip_grid_download": (I, [VP, VP, VP, C.c_size_t]),
     "crava_hip_fft_r2c": (I, [VP, VP, I, I, I, I, I]),
     "crava_hip_fft_c2r": (I, [VP, VP, I, I, I, I, I]),
     "crava_hip_fft_pass": (I, [VP, VP, I, I, I, I, I, I, C.c_double]),

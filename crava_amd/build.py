@@ -19,6 +19,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcrava_hip.so")
+HOST_LIB = os.path.join(HERE, "libcrava_host.so")
+HOST_SRC = os.path.join(HERE, "host")
+HOST_TEST_SRC = os.path.join(HERE, "..", "tests", "cpp", "host_mirror_main.cpp")
+HOST_TEST_BIN = os.path.join(HERE, "..", "tests", "cpp", "host_mirror_main")
 ARCH = os.environ.get("CRAVA_HIP_ARCH", "gfx950")
 
 # source -> extra flags.  The facies and grid kernels must reproduce
@@ -88,7 +92,29 @@ def build(verbose: bool = False, incremental: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose: bool = False) -> None:
+    """The C++ host mirror (plain g++, over the C ABI only) and its test driver."""
+    gxx = shutil.which("g++") or "g++"
+    srcs = sorted(os.path.join(HOST_SRC, f) for f in os.listdir(HOST_SRC) if f.endswith(".cpp"))
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(HERE, "..", "include")]
+    cmd = [gxx, *flags, "-shared", "-o", HOST_LIB + ".tmp", *srcs, "-L" + HERE, "-lcrava_hip",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host mirror build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    if os.path.exists(HOST_TEST_SRC):
+        cmd = [gxx, *flags, "-o", HOST_TEST_BIN, HOST_TEST_SRC, "-L" + HERE, "-lcrava_host", "-lcrava_hip",
+               "-Wl,-rpath,$ORIGIN/../../crava_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host mirror test driver build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", HOST_LIB)
 
 
 if __name__ == "__main__":

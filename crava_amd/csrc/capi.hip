@@ -646,3 +646,40 @@ int crava_hip_facies_geomodel(crava_hip_ctx *ctx, int nx, int ny, int nz, int nf
   }
   return crava::launch_facies_geomodel(gs, prob[nfac], nfac, nx, ny, nz, ctx->stream);
 }
+
+// ---------------------------------------------------------------------------
+// Device grid storage for host code that holds no HIP types.
+int crava_hip_grid_alloc(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, int precision, void **grid) {
+  if (!ctx || !grid || nxp < 1 || nyp < 1 || nzp < 1 || precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) {
+    set_error("invalid arguments to crava_hip_grid_alloc");
+    return CRAVA_HIP_E_INVALID;
+  }
+  *grid = nullptr;
+  const size_t bytes = (size_t)2 * (nxp / 2 + 1) * nyp * nzp * (precision == CRAVA_HIP_F64 ? 8 : 4);
+  const hipError_t e = hipMalloc(grid, bytes);
+  if (e != hipSuccess) {
+    *grid = nullptr;
+    set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? CRAVA_HIP_E_OOM : CRAVA_HIP_E_HIP;
+  }
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_grid_free(void *grid) {
+  if (grid) CRAVA_HIP_CHECK(hipFree(grid));
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_grid_upload(crava_hip_ctx *ctx, void *grid, const void *host, size_t bytes) {
+  if (!ctx || !grid || !host) { set_error("invalid arguments to crava_hip_grid_upload"); return CRAVA_HIP_E_INVALID; }
+  CRAVA_HIP_CHECK(hipMemcpyAsync(grid, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_grid_download(crava_hip_ctx *ctx, void *host, const void *grid, size_t bytes) {
+  if (!ctx || !grid || !host) { set_error("invalid arguments to crava_hip_grid_download"); return CRAVA_HIP_E_INVALID; }
+  CRAVA_HIP_CHECK(hipMemcpyAsync(host, grid, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return CRAVA_HIP_OK;
+}

@@ -1,0 +1,142 @@
+// Crava (posterior core) of the C++ host mirror (see crava_host.hpp).
+#include <cmath>
+#include <cstring>
+
+#include "crava_host.hpp"
+
+namespace crava_hip {
+
+Crava::Crava(Context *ctx, const Settings &s, std::vector<FFTGrid *> seisData)
+    : ctx_(ctx), s_(s), seisData_(std::move(seisData)) {
+  const int nt = s_.ntheta;
+  if (nt < 1 || nt > 8 || (int)seisData_.size() != nt || (int)s_.A.size() != 3 * nt ||
+      (int)s_.wavelets.size() != nt * s_.nzp || (int)s_.waveletNorm.size() != nt || (int)s_.snr.size() != nt ||
+      (int)s_.thetaDeg.size() != nt)
+    throw Error(CRAVA_HIP_E_INVALID, "inconsistent Crava settings");
+  dataVariance_.assign(nt, 0.0f);
+  errorVariance_.assign(nt, 0.0f);
+  errThetaCov_.assign((size_t)nt * nt, 0.0);
+}
+
+Crava::~Crava() = default;
+
+void Crava::computeDataVariance() {
+  for (int l = 0; l < s_.ntheta; ++l) {
+    FFTGrid *g = seisData_[l];
+    long long n = 0;
+    check(crava_hip_data_variance(ctx_->handle(), g->devicePtr(), s_.nx, s_.ny, s_.nz, s_.nxp, s_.nyp, s_.nzp,
+                                  &dataVariance_[l], &n),
+          "computeDataVariance");
+  }
+}
+
+// src/crava.cpp:366-401 without local noise scales.  S/N < 1.1 is an error
+// (the reference exits).
+void Crava::setupErrorCorrelation(const std::function<double(float)> &angularCorr) {
+  const int nt = s_.ntheta;
+  for (int l = 0; l < nt; ++l) {
+    if (s_.snr[l] < 1.1f) throw Error(CRAVA_HIP_E_INVALID, "signal-to-noise ratio below 1.1");
+    errorVariance_[l] = dataVariance_[l] / s_.snr[l];
+  }
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nt; ++j) {
+      const float dTheta = s_.thetaDeg[i] - s_.thetaDeg[j];
+      errThetaCov_[(size_t)i * nt + j] = static_cast<float>(std::sqrt(errorVariance_[i]) *
+                                                            std::sqrt(errorVariance_[j]) * angularCorr(dTheta));
+    }
+}
+
+long long Crava::computePostMeanResidAndFFTCov(SeismicParametersHolder &sph) {
+  const int nt = s_.ntheta, nzp = s_.nzp;
+  crava_hip_ctx *h = ctx_->handle();
+  // per-k vectors (src/crava.cpp:938-1057)
+  std::vector<unsigned char> invert(nzp);
+  std::vector<float> kWD((size_t)nzp * nt * 2), e1(kWD.size()), e2(kWD.size()), e3(kWD.size());
+  check(crava_hip_perk_tables(nt, s_.nz, nzp, (double)s_.nz * s_.dz, s_.lowCut, s_.highCut, 1.0f, s_.wavelets.data(),
+                              s_.waveletNorm.data(), invert.data(), kWD.data(), e1.data(), e2.data(), e3.data()),
+        "perk tables");
+  crava_hip_post_params p;
+  std::memset(&p, 0, sizeof(p));
+  p.nxp = s_.nxp;
+  p.nyp = s_.nyp;
+  p.nzp = nzp;
+  p.ntheta = nt;
+  p.wnc = s_.wnc;
+  const double(&v0)[3][3] = sph.getPriorVar0();
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) p.sigma0[a * 3 + b] = v0[a][b];
+  for (int i = 0; i < 3 * nt; ++i) p.A[i] = s_.A[i];
+  for (int i = 0; i < nt * nt; ++i) p.err_theta_cov[i] = errThetaCov_[i];
+  p.mixed_threshold = s_.mixedThreshold;
+  crava_hip_post *post = nullptr;
+  check(crava_hip_post_create(h, &p, invert.data(), kWD.data(), e1.data(), e2.data(), e3.data(), &post),
+        "crava_hip_post_create");
+
+  // error correlation grid (src/crava.cpp:154-157,1012)
+  FFTGrid errCorr(ctx_, s_.nx, s_.ny, s_.nz, s_.nxp, s_.nyp, nzp);
+  errCorr.setType(FFTGrid::COVARIANCE);
+  errCorr.setTransformPrecision(sph.GetCovAlpha()->getTransformPrecision());
+  errCorr.createRealGrid();
+  long long nfail = 0;
+  try {
+    errCorr.fillInErrCorr(s_.corrXY, s_.corrGradI, s_.corrGradJ);
+    // forward transforms: data and means share the 1/sqrt(N) scaling, one set
+    std::vector<void *> dm;
+    for (FFTGrid *g : seisData_)
+      if (!g->getIsTransformed()) dm.push_back(g->devicePtr());
+    for (int c = 0; c < 3; ++c)
+      if (!sph.mu(c)->getIsTransformed()) dm.push_back(sph.mu(c)->devicePtr());
+    if (!dm.empty())
+      check(crava_hip_fft_r2c_many(h, dm.data(), (int)dm.size(), s_.nxp, s_.nyp, nzp, CRAVA_HIP_DATA, CRAVA_HIP_F32),
+            "forward transforms");
+    for (FFTGrid *g : seisData_) g->setIsTransformed(true);
+    for (int c = 0; c < 3; ++c) sph.mu(c)->setIsTransformed(true);
+    sph.FFTCovGrids();
+    errCorr.fftInPlace();
+
+    void *mu[3], *data[8], *cov[6];
+    for (int c = 0; c < 3; ++c) mu[c] = sph.mu(c)->devicePtr();
+    for (int l = 0; l < nt; ++l) data[l] = seisData_[l]->devicePtr();
+    for (int q = 0; q < 6; ++q) cov[q] = sph.cov(q)->devicePtr();
+    long long n64 = 0;
+    check(crava_hip_post_run(post, s_.precision, mu, data, cov, errCorr.devicePtr(), &nfail, &n64), "posterior");
+    nFP64_ = n64;
+
+    // posterior mean and residuals back to the time domain (src/crava.cpp:1181-1201)
+    std::vector<void *> back;
+    for (int c = 0; c < 3; ++c) back.push_back(mu[c]);
+    for (int l = 0; l < nt; ++l) back.push_back(data[l]);
+    check(crava_hip_fft_c2r_many(h, back.data(), (int)back.size(), s_.nxp, s_.nyp, nzp, CRAVA_HIP_DATA,
+                                 CRAVA_HIP_F32),
+          "inverse transforms");
+    for (int c = 0; c < 3; ++c) sph.mu(c)->setIsTransformed(false);
+    for (FFTGrid *g : seisData_) g->setIsTransformed(false);
+  } catch (...) {
+    crava_hip_post_destroy(post);
+    throw;
+  }
+  crava_hip_post_destroy(post);
+  ctx_->synchronize();
+  return nfail;
+}
+
+long long Crava::simulate(SeismicParametersHolder &sph, unsigned long long seed, unsigned long long stream,
+                          FFTGrid *real[3]) {
+  crava_hip_ctx *h = ctx_->handle();
+  void *seeds[3], *cov[6], *mean[3];
+  for (int c = 0; c < 3; ++c) {
+    seeds[c] = real[c]->devicePtr();
+    check(crava_hip_fill_complex_noise(h, seeds[c], s_.nxp, s_.nyp, s_.nzp, seed, stream * 3 + c), "noise");
+    mean[c] = sph.mu(c)->getIsTransformed() ? nullptr : sph.mu(c)->devicePtr();
+  }
+  for (int q = 0; q < 6; ++q) {
+    if (!sph.cov(q)->getIsTransformed()) throw Error(CRAVA_HIP_E_STATE, "simulate needs transformed covariances");
+    cov[q] = sph.cov(q)->devicePtr();
+  }
+  long long nfail = 0;
+  check(crava_hip_simulate(h, s_.nxp, s_.nyp, s_.nzp, cov, seeds, mean, &nfail), "simulate");
+  for (int c = 0; c < 3; ++c) real[c]->setIsTransformed(false);
+  return nfail;
+}
+
+}  // namespace crava_hip

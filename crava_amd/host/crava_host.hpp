@@ -1,0 +1,229 @@
+// C++ host mirror of the reference's FFTGrid / SeismicParametersHolder /
+// Crava interface on the hot path, with device-resident grids behind the C
+// ABI of include/crava_hip.h (no HIP or torch type appears here).
+//
+//   FFTGrid                  src/fftgrid.h:22-295
+//   SeismicParametersHolder  src/seismicparametersholder.h
+//   Crava (posterior path)   src/crava.h; src/crava.cpp:322-473,887-1295,1416-1599
+//
+// Names, argument meaning and call order follow the reference, so code
+// written against it ports by changing the namespace.  Differences that
+// follow from the storage living in HBM:
+//   * A grid's values live on the GPU.  As for the reference's disk-backed
+//     FFTFileGrid, setAccessMode(READ / READANDWRITE / RANDOMACCESS) brings
+//     them into a host copy for the cursor and (i,j,k) accessors, and
+//     endAccess() writes the host copy back after WRITE / READANDWRITE /
+//     RANDOMACCESS.  Transforms, fills and element-wise operators run on the
+//     device and need no access mode.
+//   * Errors are reported as exceptions (crava_hip::Error) carrying the C
+//     ABI's status and message, where the reference asserts or exits.
+#pragma once
+
+#include <complex>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/crava_hip.h"
+
+namespace crava_hip {
+
+class Error : public std::runtime_error {
+ public:
+  Error(int status, const std::string &what) : std::runtime_error(what), status_(status) {}
+  int status() const { return status_; }
+
+ private:
+  int status_;
+};
+
+// Throw Error if rc is not CRAVA_HIP_OK.
+void check(int rc, const char *what);
+
+struct fftw_complex_f {   // layout of the reference's fftw_complex (float)
+  float re, im;
+};
+
+// A crava_hip_ctx: one stream and the transform plans.
+class Context {
+ public:
+  explicit Context(void *stream = nullptr);
+  ~Context();
+  Context(const Context &) = delete;
+  Context &operator=(const Context &) = delete;
+  crava_hip_ctx *handle() const { return ctx_; }
+  void synchronize();
+
+ private:
+  crava_hip_ctx *ctx_ = nullptr;
+};
+
+class FFTGrid {
+ public:
+  enum gridTypes { CTMISSING, DATA, PARAMETER, COVARIANCE, VELOCITY };   // src/fftgrid.h:167
+  enum accessMode { NONE, READ, WRITE, READANDWRITE, RANDOMACCESS };     // src/fftgrid.h:168
+
+  FFTGrid(Context *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp);
+  virtual ~FFTGrid();
+  FFTGrid(const FFTGrid &) = delete;
+  FFTGrid &operator=(const FFTGrid &) = delete;
+
+  void createRealGrid(bool add = true);
+  void createComplexGrid();
+  void setType(int cubeType) { cubetype_ = cubeType; }
+  int getType() const { return cubetype_; }
+  // CRAVA_HIP_F32 (default) or CRAVA_HIP_MIXED: arithmetic of this grid's
+  // transforms (float storage either way).
+  void setTransformPrecision(int precision) { precision_ = precision; }
+  int getTransformPrecision() const { return precision_; }
+
+  int fftInPlace();      // src/fftgrid.cpp:1804-1829
+  int invFFTInPlace();   // src/fftgrid.cpp:1832-1862
+  bool getIsTransformed() const { return istransformed_; }
+  void setIsTransformed(bool v) { istransformed_ = v; }
+
+  void fillInConstant(float value);
+  void fillInParamCorr(const std::vector<float> &corrXY, const float *circCorrT, float gradI, float gradJ);
+  void fillInErrCorr(const std::vector<float> &corrXY, float gradI, float gradJ);
+
+  void multiplyByScalar(float scalar);
+  void addScalar(float scalar);
+  void add(FFTGrid *g);
+  void subtract(FFTGrid *g);
+  void multiply(FFTGrid *g);
+  void conjugate();
+  void changeSign();
+
+  virtual void setAccessMode(int mode);
+  virtual void endAccess();
+  float getNextReal();
+  int setNextReal(float value);
+  fftw_complex_f getNextComplex();
+  int setNextComplex(fftw_complex_f value);
+  float getRealValue(int i, int j, int k, bool extSimbox = false) const;
+  int setRealValue(int i, int j, int k, float value, bool extSimbox = false);
+  fftw_complex_f getComplexValue(int i, int j, int k, bool extSimbox = false) const;
+
+  int getNx() const { return nx_; }
+  int getNy() const { return ny_; }
+  int getNz() const { return nz_; }
+  int getNxp() const { return nxp_; }
+  int getNyp() const { return nyp_; }
+  int getNzp() const { return nzp_; }
+  int getRNxp() const { return rnxp_; }
+  int getCNxp() const { return cnxp_; }
+  long long getrsize() const { return rsize_; }
+  long long getcsize() const { return csize_; }
+
+  void *devicePtr() const { return dev_; }
+  Context *context() const { return ctx_; }
+
+ private:
+  int abiCubeType() const;
+  void op(int code, const FFTGrid *other, float scalar);
+
+  Context *ctx_;
+  int nx_, ny_, nz_, nxp_, nyp_, nzp_, rnxp_, cnxp_;
+  long long rsize_, csize_;
+  int cubetype_ = CTMISSING;
+  int precision_ = CRAVA_HIP_F32;
+  bool istransformed_ = false;
+  void *dev_ = nullptr;
+  int mode_ = NONE;
+  std::vector<float> host_;
+  long long counterForGet_ = 0, counterForSet_ = 0;
+};
+
+class SeismicParametersHolder {
+ public:
+  explicit SeismicParametersHolder(Context *ctx) : ctx_(ctx) {}
+  ~SeismicParametersHolder();
+
+  void allocateGrids(int nx, int ny, int nz, int nxPad, int nyPad, int nzPad);   // :227-260
+  void setPriorVar0(const double priorVar0[3][3]);
+  const double (&getPriorVar0() const)[3][3] { return priorVar0_; }
+  // :192-225: fill the six covariance grids from the lateral correlation
+  // corrXY [nyp*nxp] and the temporal correlation priorCorrT, times Sigma0_ab.
+  void initializeCorrelations(const std::vector<float> &priorCorrXY, const std::vector<float> &priorCorrT,
+                              float corrGradI, float corrGradJ, int lowIntCut, int nzp);
+  // :431-499 (host; the 1-D transform in double).
+  std::vector<float> computeCircCorrT(const std::vector<float> &priorCorrT, int minIntFq, int nzp) const;
+  void makeCircCorrTPosDef(std::vector<float> &circCorrT, int minIntFq, int nzp) const;
+
+  void FFTAllGrids();       // :282-298, one launch per pass for each set
+  void invFFTAllGrids();    // :262-278
+  void FFTCovGrids();
+  void invFFTCovGrids();
+  // Arithmetic of the covariance grids' forward transforms (CRAVA_HIP_F32 or
+  // CRAVA_HIP_MIXED): their spectra span many decades.
+  void setCovTransformPrecision(int precision);
+
+  FFTGrid *GetMuAlpha() const { return mu_[0]; }
+  FFTGrid *GetMuBeta() const { return mu_[1]; }
+  FFTGrid *GetMuRho() const { return mu_[2]; }
+  FFTGrid *GetCovAlpha() const { return cov_[0]; }
+  FFTGrid *GetCovBeta() const { return cov_[1]; }
+  FFTGrid *GetCovRho() const { return cov_[2]; }
+  FFTGrid *GetCrCovAlphaBeta() const { return cov_[3]; }
+  FFTGrid *GetCrCovAlphaRho() const { return cov_[4]; }
+  FFTGrid *GetCrCovBetaRho() const { return cov_[5]; }
+  FFTGrid *mu(int c) const { return mu_[c]; }
+  FFTGrid *cov(int q) const { return cov_[q]; }
+
+ private:
+  void transformSet(FFTGrid *const *grids, int n, bool inverse);
+  Context *ctx_;
+  FFTGrid *mu_[3] = {nullptr, nullptr, nullptr};
+  FFTGrid *cov_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double priorVar0_[3][3] = {{0}};
+};
+
+// The posterior core of Crava on the constant-thickness path.
+class Crava {
+ public:
+  struct Settings {
+    int ntheta = 0;
+    int nx = 0, ny = 0, nz = 0, nxp = 0, nyp = 0, nzp = 0;
+    float dz = 4.0f;                    // time sample (ms): lz = nz * dz
+    float lowCut = 5.0f, highCut = 55.0f;
+    float wnc = 0.1f;                   // white-noise component
+    std::vector<float> thetaDeg;        // [ntheta]
+    std::vector<float> snr;             // empirical S/N per angle
+    std::vector<float> A;               // reflection matrix [ntheta][3]
+    std::vector<float> wavelets;        // [ntheta][nzp] scaled wavelets in FFT order
+    std::vector<float> waveletNorm;     // [ntheta]
+    std::vector<float> corrXY;          // lateral correlation [nyp*nxp] (error correlation)
+    float corrGradI = 0.0f, corrGradJ = 0.0f;
+    int precision = CRAVA_HIP_F32;      // posterior solve: F32 or MIXED
+    double mixedThreshold = 0.1;
+  };
+
+  Crava(Context *ctx, const Settings &s, std::vector<FFTGrid *> seisData);
+  ~Crava();
+
+  void computeDataVariance();                                               // :322-363
+  void setupErrorCorrelation(const std::function<double(float)> &angularCorr);  // :366-401
+  // :887-1204: forward transforms, per-point posterior, inverse transforms
+  // of the mean and the residuals (the covariance stays transformed).
+  // Returns the number of failed factorisations.
+  long long computePostMeanResidAndFFTCov(SeismicParametersHolder &sph);
+  // :1416-1599 one realisation: noise (seed, stream) -> real grids real[3].
+  long long simulate(SeismicParametersHolder &sph, unsigned long long seed, unsigned long long stream,
+                     FFTGrid *real[3]);
+
+  const std::vector<float> &getDataVariance() const { return dataVariance_; }
+  const std::vector<float> &getErrorVariance() const { return errorVariance_; }
+  const std::vector<double> &getErrThetaCov() const { return errThetaCov_; }
+  long long getFP64Points() const { return nFP64_; }
+
+ private:
+  Context *ctx_;
+  Settings s_;
+  std::vector<FFTGrid *> seisData_;
+  std::vector<float> dataVariance_, errorVariance_;
+  std::vector<double> errThetaCov_;   // [ntheta][ntheta]
+  long long nFP64_ = 0;
+};
+
+}  // namespace crava_hip

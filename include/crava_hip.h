@@ -56,6 +56,17 @@ int crava_hip_ctx_destroy(crava_hip_ctx *ctx);
 int crava_hip_ctx_set_stream(crava_hip_ctx *ctx, void *stream);
 int crava_hip_synchronize(crava_hip_ctx *ctx);
 
+/* Device storage of one grid in the FFTGrid layout (rsize = 2*(nxp/2+1)*nyp*nzp
+ * floats, or doubles for CRAVA_HIP_F64), in place of FFTGrid::createRealGrid /
+ * createComplexGrid (src/fftgrid.cpp:1283-1316, fftw_malloc).  Upload and
+ * download are synchronous copies on the context's stream; with them a
+ * host caller needs no HIP type (the FFTGrid cursor access of
+ * src/fftgrid.cpp:1419-1653 runs on a host copy between them). */
+int crava_hip_grid_alloc(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, int precision, void **grid);
+int crava_hip_grid_free(void *grid);
+int crava_hip_grid_upload(crava_hip_ctx *ctx, void *grid, const void *host, size_t bytes);
+int crava_hip_grid_download(crava_hip_ctx *ctx, void *host, const void *grid, size_t bytes);
+
 /* In-place 3-D real-to-half-complex transform of one device grid.
  * Replaces FFTGrid::fftInPlace (src/fftgrid.cpp:1804-1829): grids that are
  * not CRAVA_HIP_COVARIANCE are scaled by 1/sqrt(nxp*nyp*nzp); FFTW's forward

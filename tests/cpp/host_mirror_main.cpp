@@ -1,0 +1,162 @@
+// Drives the C++ host mirror (crava_amd/host) the way the reference's
+// Crava::Crava / computePostMeanResidAndFFTCov sequence drives FFTGrid and
+// SeismicParametersHolder, on inputs written by tests/test_gpu_host_mirror.py,
+// and writes the outputs for that test to compare with the oracle.
+//
+//   host_mirror_main <case dir>
+//
+// Inputs in <dir>: params.txt, wavelets.bin, corrxy.bin, corrt.bin,
+// mu{0,1,2}.bin, data{l}.bin (raw float32).  Outputs: circ.bin,
+// out_mu{c}.bin, out_res{l}.bin (time domain), out_cov{q}.bin (transformed),
+// out_sim{c}.bin, out_scalars.txt.
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../crava_amd/host/crava_host.hpp"
+
+using crava_hip::FFTGrid;
+
+static std::vector<float> read_bin(const std::string &path, size_t n) {
+  std::vector<float> v(n);
+  FILE *f = std::fopen(path.c_str(), "rb");
+  if (!f || std::fread(v.data(), sizeof(float), n, f) != n) throw std::runtime_error("cannot read " + path);
+  std::fclose(f);
+  return v;
+}
+
+static void write_bin(const std::string &path, const std::vector<float> &v) {
+  FILE *f = std::fopen(path.c_str(), "wb");
+  if (!f || std::fwrite(v.data(), sizeof(float), v.size(), f) != v.size()) throw std::runtime_error("cannot write " + path);
+  std::fclose(f);
+}
+
+// Read a real grid with the cursor, as the reference's writers do.
+static std::vector<float> drain_real(FFTGrid *g) {
+  g->setAccessMode(FFTGrid::READ);
+  std::vector<float> v((size_t)g->getrsize());
+  for (auto &x : v) x = g->getNextReal();
+  g->endAccess();
+  return v;
+}
+
+static std::vector<float> drain_complex(FFTGrid *g) {
+  g->setAccessMode(FFTGrid::READ);
+  std::vector<float> v(2 * (size_t)g->getcsize());
+  for (size_t i = 0; i < v.size() / 2; ++i) {
+    const crava_hip::fftw_complex_f c = g->getNextComplex();
+    v[2 * i] = c.re;
+    v[2 * i + 1] = c.im;
+  }
+  g->endAccess();
+  return v;
+}
+
+static void fill_real(FFTGrid *g, const std::vector<float> &v) {
+  g->setAccessMode(FFTGrid::WRITE);
+  for (float x : v) g->setNextReal(x);
+  g->endAccess();
+}
+
+int main(int argc, char **argv) {
+  if (argc != 2) {
+    std::cerr << "usage: host_mirror_main <case dir>\n";
+    return 2;
+  }
+  const std::string dir = argv[1];
+  try {
+    std::ifstream in(dir + "/params.txt");
+    crava_hip::Crava::Settings s;
+    int ncorrt = 0, lowIntCut = 0;
+    double angRange = 10.0;
+    unsigned long long seed = 0;
+    in >> s.nx >> s.ny >> s.nz >> s.nxp >> s.nyp >> s.nzp >> s.ntheta >> s.dz >> s.lowCut >> s.highCut >> s.wnc >>
+        s.precision >> s.mixedThreshold >> s.corrGradI >> s.corrGradJ >> lowIntCut >> ncorrt >> angRange >> seed;
+    const int nt = s.ntheta;
+    s.thetaDeg.resize(nt);
+    s.snr.resize(nt);
+    s.A.resize(3 * nt);
+    s.waveletNorm.resize(nt);
+    double sigma0[3][3];
+    for (auto &x : s.thetaDeg) in >> x;
+    for (auto &x : s.snr) in >> x;
+    for (auto &x : s.A) in >> x;
+    for (auto &x : s.waveletNorm) in >> x;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) in >> sigma0[a][b];
+    if (!in) throw std::runtime_error("bad params.txt");
+    s.wavelets = read_bin(dir + "/wavelets.bin", (size_t)nt * s.nzp);
+    s.corrXY = read_bin(dir + "/corrxy.bin", (size_t)s.nxp * s.nyp);
+    const std::vector<float> corrT = read_bin(dir + "/corrt.bin", (size_t)ncorrt);
+    const size_t rsize = (size_t)2 * (s.nxp / 2 + 1) * s.nyp * s.nzp;
+
+    crava_hip::Context ctx;
+    crava_hip::SeismicParametersHolder sph(&ctx);
+    sph.allocateGrids(s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
+    sph.setPriorVar0(sigma0);
+    if (s.precision == CRAVA_HIP_MIXED) sph.setCovTransformPrecision(CRAVA_HIP_MIXED);
+    sph.initializeCorrelations(s.corrXY, corrT, s.corrGradI, s.corrGradJ, lowIntCut, s.nzp);
+    write_bin(dir + "/circ.bin", sph.computeCircCorrT(corrT, lowIntCut, s.nzp));
+    for (int c = 0; c < 3; ++c) fill_real(sph.mu(c), read_bin(dir + "/mu" + std::to_string(c) + ".bin", rsize));
+
+    std::vector<FFTGrid *> seis;
+    for (int l = 0; l < nt; ++l) {
+      FFTGrid *g = new FFTGrid(&ctx, s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
+      g->setType(FFTGrid::DATA);
+      g->createRealGrid();
+      fill_real(g, read_bin(dir + "/data" + std::to_string(l) + ".bin", rsize));
+      seis.push_back(g);
+    }
+
+    crava_hip::Crava crava(&ctx, s, seis);
+    crava.computeDataVariance();
+    crava.setupErrorCorrelation([angRange](float dTheta) { return std::exp(-3.0 * std::fabs(dTheta) / angRange); });
+    const long long nfail = crava.computePostMeanResidAndFFTCov(sph);
+
+    for (int c = 0; c < 3; ++c) write_bin(dir + "/out_mu" + std::to_string(c) + ".bin", drain_real(sph.mu(c)));
+    for (int l = 0; l < nt; ++l) write_bin(dir + "/out_res" + std::to_string(l) + ".bin", drain_real(seis[l]));
+    for (int q = 0; q < 6; ++q) write_bin(dir + "/out_cov" + std::to_string(q) + ".bin", drain_complex(sph.cov(q)));
+
+    // one posterior realisation (posterior mean added in the time domain)
+    FFTGrid *real[3];
+    for (int c = 0; c < 3; ++c) {
+      real[c] = new FFTGrid(&ctx, s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
+      real[c]->setType(FFTGrid::PARAMETER);
+      real[c]->createComplexGrid();
+    }
+    const long long simfail = crava.simulate(sph, seed, 0, real);
+    for (int c = 0; c < 3; ++c) {
+      write_bin(dir + "/out_sim" + std::to_string(c) + ".bin", drain_real(real[c]));
+      delete real[c];
+    }
+
+    // a few cursor / random-access checks against the values just written
+    FFTGrid *m0 = sph.mu(0);
+    m0->setAccessMode(FFTGrid::RANDOMACCESS);
+    const float v = m0->getRealValue(1, 2, 3);
+    const float outside = m0->getRealValue(s.nx, 0, 0);
+    m0->setRealValue(1, 2, 3, v);
+    m0->endAccess();
+
+    std::ofstream out(dir + "/out_scalars.txt");
+    out.precision(9);
+    out << nfail << " " << simfail << " " << crava.getFP64Points() << "\n";
+    for (float x : crava.getDataVariance()) out << x << " ";
+    out << "\n";
+    for (float x : crava.getErrorVariance()) out << x << " ";
+    out << "\n";
+    for (double x : crava.getErrThetaCov()) out << x << " ";
+    out << "\n" << v << " " << outside << "\n";
+    for (FFTGrid *g : seis) delete g;
+  } catch (const crava_hip::Error &e) {
+    std::cerr << "crava_hip error " << e.status() << ": " << e.what() << "\n";
+    return 1;
+  } catch (const std::exception &e) {
+    std::cerr << "error: " << e.what() << "\n";
+    return 1;
+  }
+  return 0;
+}

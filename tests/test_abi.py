@@ -75,3 +75,19 @@ def test_product_never_imports_the_oracle():
             text = open(path).read()
             assert "import oracle" not in text and "from oracle" not in text, path
             assert "liboracle" not in text and "libcrava_ref" not in text, path
+
+
+def test_host_mirror_links_only_the_c_abi():
+    """The C++ host mirror (FFTGrid / SeismicParametersHolder / Crava) calls
+    the GPU only through include/crava_hip.h: every undefined symbol it needs
+    from libcrava_hip.so is a declared crava_hip_* entry point, and it needs
+    nothing from the HIP runtime directly."""
+    import subprocess
+    so = os.path.join(ROOT, "crava_amd", "libcrava_host.so")
+    assert os.path.exists(so), "built by crava_amd.build"
+    out = subprocess.run(["nm", "-D", "--undefined-only", so], capture_output=True, text=True, check=True).stdout
+    undef = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    ours = {u for u in undef if u.startswith("crava_hip_")}
+    assert ours and ours <= set(declared_functions())
+    assert not any(u.startswith("hip") for u in undef)
+    C.CDLL(so)   # loads with its rpath to libcrava_hip.so
