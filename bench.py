@@ -31,6 +31,52 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 
+# Launch geometry of the specialised transform kernels' variants
+# (pencils or row pairs per workgroup, threads), as in fft_static.hip.
+STRIDED_VARIANTS = [(16, 256), (16, 512), (8, 256), (32, 512)]
+ROW_VARIANTS = [(8, 256), (16, 512), (4, 256)]
+
+
+def kernel_identity(key, variants, nxp, nyp, nzp, ngrid, nt):
+    """(kernel name as profiled, total work-items per launch) of a bench kernel key."""
+    cn = nxp // 2 + 1
+    vx, vy, vz = variants
+    if key == "fft_x_r2c":
+        if vx < 0:
+            return "crava::k_fft_rows_r2c<float, float, float>", None
+        tx, th = ROW_VARIANTS[vx]
+        rows = nyp * nzp
+        return (f"crava::k_srows_r2c<{nxp}, {tx}, {th}, float, float, float>",
+                -(-rows // (2 * tx)) * ngrid * th)
+    if key in ("fft_y", "fft_z"):
+        n, v = (nyp, vy) if key == "fft_y" else (nzp, vz)
+        ncols, nbatch = (cn, nzp) if key == "fft_y" else (cn * nyp, 1)
+        if v < 0:
+            return "crava::k_fft_strided<-1, float, float, float>", None
+        tx, th = STRIDED_VARIANTS[v]
+        return (f"crava::k_sfft_strided<{n}, {tx}, {th}, -1, float, float, float>",
+                -(-ncols // tx) * nbatch * ngrid * th)
+    mode = {"posterior_mixed": ("float", 1), "posterior_f32": ("float", 0), "posterior_f64": ("double", 0)}[key]
+    return f"crava::k_posterior<{mode[0]}, {nt}, {mode[1]}>", -(-(cn * nyp) // 256) * 256 * nzp
+
+
+def profiled_traffic(name, grid):
+    """HBM bytes per launch from the committed counter summaries (TRAFFIC lines
+    of profiles/hbm_traffic_*.md), or None."""
+    import glob
+    best = None
+    for f in glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.md")):
+        for line in open(f):
+            if not line.startswith("TRAFFIC "):
+                continue
+            body = line[len("TRAFFIC "):].rsplit(" ", 2)
+            if len(body) != 3 or body[0] != name:
+                continue
+            g = int(body[1].split("=")[1])
+            if grid is None or g == grid:
+                best = float(body[2])
+    return best
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -76,7 +122,11 @@ class Pipeline:
         """One pass of the path; each group of same-kind grids is one launch per transform pass."""
         L, c = self.L, self.prob.cfg
         n = (c.nxp, c.nyp, c.nzp)
-        self.ec.copy_(self.ec0)
+        if self.fp64:
+            self.ec.copy_(self.ec0)
+        else:   # the forward transform consumes the error-correlation grid
+            from crava_amd.core import grid_op
+            grid_op(self.ctx, L.OP_COPY, self.ec, self.ec0, *n, False)
         # data and prior means share the 1/sqrt(N) scaling (DATA / PARAMETER)
         self.ctx.r2c_many(self.data + self.mu, *n, L.DATA)
         # covariance spectra span many decades: the mixed form transforms them
@@ -88,8 +138,15 @@ class Pipeline:
         return r
 
     def reset(self):
+        """Reload the inputs with the library's grid copy (the plain copy the
+        HBM counters are calibrated against)."""
+        from crava_amd.core import grid_op
+        c = self.prob.cfg
         for dst, src in zip(self.mu + self.data + self.cov, self.prob.mu + self.prob.data + self.prob.cov):
-            dst.copy_(src)
+            if self.fp64:
+                dst.copy_(src)
+            else:
+                grid_op(self.ctx, self.L.OP_COPY, dst, src, c.nxp, c.nyp, c.nzp, False)
 
 
 def rel_err(a, b, nxp, ref_base=None):
@@ -391,8 +448,11 @@ def main():
     else:
         dom_name = "posterior_mixed"
     dom = kernels[dom_name]
-    roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(dom["GBps"], 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(dom["GBps"] / PEAK_HBM_GBS, 4), "traffic": None}
+    kname, kgrid = kernel_identity(dom_name, tune["f32"][0], nxp, nyp, nzp, ng, nt)
+    traffic = profiled_traffic(kname, kgrid)
+    roofline = {"bound": "hbm", "kernel": dom_name, "kernel_name": kname, "achieved": round(dom["GBps"], 1),
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(dom["GBps"] / PEAK_HBM_GBS, 4),
+                "alg_bytes_per_launch": dom["alg_bytes"], "traffic": traffic}
     batch = None
     if args.batch_volumes > 0:
         del mixed, f64

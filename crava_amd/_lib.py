@@ -19,7 +19,7 @@ E_STATE = -4
 
 DATA, PARAMETER, COVARIANCE = 0, 1, 2
 OP_ADD, OP_SUBTRACT, OP_MULTIPLY, OP_CHANGE_SIGN, OP_CONJUGATE, OP_MULTIPLY_SCALAR, OP_ADD_SCALAR, \
-    OP_FILL_CONSTANT = range(8)
+    OP_FILL_CONSTANT, OP_COPY = range(9)
 F32, MIXED, F64 = 0, 1, 2
 
 _lib = None

@@ -587,11 +587,12 @@ int crava_hip_fill_err_corr(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, in
 
 int crava_hip_grid_op(crava_hip_ctx *ctx, int op, void *grid, const void *other, int nxp, int nyp, int nzp,
                       int transformed, float scalar) {
-  if (!ctx || !grid || !grid_dims_ok(nxp, nyp, nzp) || op < CRAVA_HIP_OP_ADD || op > CRAVA_HIP_OP_FILL_CONSTANT) {
+  if (!ctx || !grid || !grid_dims_ok(nxp, nyp, nzp) || op < CRAVA_HIP_OP_ADD || op > CRAVA_HIP_OP_COPY) {
     set_error("invalid arguments to crava_hip_grid_op");
     return CRAVA_HIP_E_INVALID;
   }
-  const bool binary = op == CRAVA_HIP_OP_ADD || op == CRAVA_HIP_OP_SUBTRACT || op == CRAVA_HIP_OP_MULTIPLY;
+  const bool binary = op == CRAVA_HIP_OP_ADD || op == CRAVA_HIP_OP_SUBTRACT || op == CRAVA_HIP_OP_MULTIPLY ||
+                      op == CRAVA_HIP_OP_COPY;
   if (binary && !other) { set_error("this operator needs a second grid"); return CRAVA_HIP_E_INVALID; }
   // the reference asserts the domain of these operators (src/fftgrid.cpp:1903,1976,1984)
   if (transformed && (op == CRAVA_HIP_OP_MULTIPLY_SCALAR || op == CRAVA_HIP_OP_ADD_SCALAR ||

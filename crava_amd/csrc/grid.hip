@@ -133,6 +133,14 @@ __global__ void k_grid_op(int op, float *a, const float *__restrict__ b, long lo
   }
 }
 
+// Copy of a whole grid (rsize floats, an even count), as the FFTGrid copy
+// constructor copies rvalue_ (src/fftgrid.cpp:69-100): 8-byte vector loads
+// and stores, the plain device copy the HBM counters are calibrated with.
+__global__ void k_grid_copy(float2 *__restrict__ dst, const float2 *__restrict__ src, long long n2) {
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n2; p += (long long)gridDim.x * blockDim.x)
+    dst[p] = src[p];
+}
+
 // Crava::computeDataVariance (src/crava.cpp:322-363) in the reference's
 // summation order: per (j,k) row a double sum over i (one thread per row),
 // per k a sum over rows, then over k.
@@ -222,6 +230,13 @@ int launch_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *co
 }
 
 int launch_grid_op(int op, float *a, const float *b, long long n, bool transformed, float scalar, hipStream_t s) {
+  if (op == CRAVA_HIP_OP_COPY) {
+    const long long n2 = transformed ? n : n / 2;   // float2 elements
+    hipLaunchKernelGGL(k_grid_copy, dim3(blocks_for(n2)), dim3(256), 0, s, reinterpret_cast<float2 *>(a),
+                       reinterpret_cast<const float2 *>(b), n2);
+    CRAVA_HIP_CHECK(hipGetLastError());
+    return CRAVA_HIP_OK;
+  }
   hipLaunchKernelGGL(k_grid_op, dim3(blocks_for(n)), dim3(256), 0, s, op, a, b, n, transformed ? 1 : 0, scalar);
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;

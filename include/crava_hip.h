@@ -259,6 +259,7 @@ int crava_hip_fill_err_corr(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, in
 #define CRAVA_HIP_OP_MULTIPLY_SCALAR 5 /* multiplyByScalar() */
 #define CRAVA_HIP_OP_ADD_SCALAR 6      /* addScalar()        */
 #define CRAVA_HIP_OP_FILL_CONSTANT 7   /* fillInConstant()   */
+#define CRAVA_HIP_OP_COPY 8            /* grid <- other, the FFTGrid(FFTGrid*) copy (src/fftgrid.cpp:69-100) */
 int crava_hip_grid_op(crava_hip_ctx *ctx, int op, void *grid, const void *other, int nxp, int nyp, int nzp,
                       int transformed, float scalar);
 

@@ -1038,8 +1038,9 @@ void orc_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *corr
       }
 }
 
-/* Element-wise operators of FFTGrid (src/fftgrid.cpp:1877-1999); op codes as
- * CRAVA_HIP_OP_*; n floats (real) or n complex values (transformed). */
+/* Element-wise operators of FFTGrid (src/fftgrid.cpp:1877-1999) and the
+ * copy (op 8); op codes as CRAVA_HIP_OP_*; n floats (real) or n complex
+ * values (transformed). */
 void orc_grid_op(int op, float *a, const float *b, long long n, int transformed, float s)
 {
   for (long long p = 0; p < n; p++) {
@@ -1052,6 +1053,7 @@ void orc_grid_op(int op, float *a, const float *b, long long n, int transformed,
         case 2: { float r = br * ar - bi * ai; float im = bi * ar + br * ai; ar = r; ai = im; break; }
         case 3: ar = -ar; ai = -ai; break;
         case 4: ai = -ai; break;
+        case 8: ar = br; ai = bi; break;
         default: break;
       }
       a[2 * p] = ar;
@@ -1066,6 +1068,7 @@ void orc_grid_op(int op, float *a, const float *b, long long n, int transformed,
         case 5: v = v * s; break;
         case 6: v = v + s; break;
         case 7: v = s; break;
+        case 8: v = b[p]; break;
         default: break;
       }
       a[p] = v;

@@ -74,7 +74,7 @@ def test_grid_ops_bitexact(ctx, transformed):
     rng = np.random.default_rng(3)
     a0 = rng.standard_normal((nzp, nyp, rnxp)).astype(np.float32)
     b = rng.standard_normal((nzp, nyp, rnxp)).astype(np.float32)
-    ops = [L.OP_ADD, L.OP_SUBTRACT, L.OP_MULTIPLY, L.OP_CHANGE_SIGN]
+    ops = [L.OP_ADD, L.OP_SUBTRACT, L.OP_MULTIPLY, L.OP_CHANGE_SIGN, L.OP_COPY]
     ops += [L.OP_CONJUGATE] if transformed else [L.OP_MULTIPLY_SCALAR, L.OP_ADD_SCALAR, L.OP_FILL_CONSTANT]
     tb = torch.from_numpy(b).cuda()
     for op in ops:

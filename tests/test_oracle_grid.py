@@ -117,7 +117,7 @@ def test_grid_ops_match_float32_arithmetic(transformed):
     a0 = rng.standard_normal(2 * 60).astype(np.float32)
     b = rng.standard_normal(2 * 60).astype(np.float32)
     s = np.float32(1.7)
-    ops = [0, 1, 2, 3, 4] if transformed else [0, 1, 2, 3, 5, 6, 7]
+    ops = [0, 1, 2, 3, 4, 8] if transformed else [0, 1, 2, 3, 5, 6, 7, 8]
     for op in ops:
         a = a0.copy()
         oracle.grid_op(op, a, b, transformed, float(s))
@@ -127,11 +127,11 @@ def test_grid_ops_match_float32_arithmetic(transformed):
             # (numpy's own complex64 product may fuse operations)
             xr, xi, yr, yi = x.real, x.imag, y.real, y.imag
             prod = (xr * yr - xi * yi) + 1j * (xi * yr + xr * yi).astype(np.float32)
-            exp = {0: x + y, 1: x - y, 2: prod, 3: -x, 4: np.conj(x)}[op]
+            exp = {0: x + y, 1: x - y, 2: prod, 3: -x, 4: np.conj(x), 8: y}[op]
             assert np.array_equal(a.view(np.complex64), exp.astype(np.complex64)), op
         else:
             exp = {0: a0 + b, 1: a0 - b, 2: a0 * b, 3: -a0, 5: a0 * s, 6: a0 + s,
-                   7: np.full_like(a0, s)}[op]
+                   7: np.full_like(a0, s), 8: b}[op]
             assert np.array_equal(a, exp.astype(np.float32)), op
 
 
