@@ -80,3 +80,36 @@ def test_posterior_f64_and_mixed_vs_fp64_oracle(ctx, name):
     assert n64 > 0
     for a, b in zip(gc, oc):
         assert rel_l2(_cplx(a), _cplx(b)) < 1e-6
+
+
+@pytest.mark.gpu
+def test_posterior_f32_vs_golden_reference_outputs(ctx):
+    """GPU posterior on the golden fixture's inputs against the reference's
+    lib_matr outputs stored with them (tests/golden)."""
+    import os
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_cases.npz"))
+    nxp, nyp, nzp, nt = (int(x) for x in G["post_dims"])
+    p = L.PostParams()
+    p.nxp, p.nyp, p.nzp, p.ntheta = nxp, nyp, nzp, nt
+    p.wnc = float(G["post_wnc"])
+    for i, v in enumerate(G["post_sigma0"].reshape(-1)):
+        p.sigma0[i] = float(v)
+    for i, v in enumerate(G["post_A"].reshape(-1)):
+        p.A[i] = float(v)
+    for i, v in enumerate(G["post_etc"].reshape(-1)):
+        p.err_theta_cov[i] = float(v)
+    p.mixed_threshold = 1e-3
+    tabs = {k[len("post_tab_"):]: G[k] for k in G.files if k.startswith("post_tab_")}
+    post = Posterior(ctx, p, tabs)
+    tm = [torch.from_numpy(g.copy()).cuda() for g in G["post_in_mu"]]
+    td = [torch.from_numpy(g.copy()).cuda() for g in G["post_in_d"]]
+    tc = [torch.from_numpy(g.copy()).cuda() for g in G["post_in_cov"]]
+    te = torch.from_numpy(G["post_in_ec"].copy()).cuda()
+    nf, _ = post.run(L.F32, tm, td, tc, te, counts=True)
+    torch.cuda.synchronize()
+    post.close()
+    assert nf == int(G["post_nfail"])
+    for got, ref in zip(tm, G["post_out_mu"]):
+        assert rel_l2(_cplx(got.cpu().numpy()), _cplx(ref)) < 1e-6
+    for got, ref in zip(td + tc, list(G["post_out_d"]) + list(G["post_out_cov"])):
+        assert rel_l2(_cplx(got.cpu().numpy()), _cplx(ref)) < 2e-5
