@@ -17,61 +17,83 @@
 
 namespace crava {
 
-// One radix-R Stockham pass over the tx pencils held in LDS.
+__device__ __forceinline__ int gpadx(int x) { return x + (x >> 5); }
+
+// Global loads issued back to back per thread in the staging loops.
+constexpr int kUnroll = 8;
+
+// x / d with the plan's multiplier (d == 1 when mul == 0).
+__device__ __forceinline__ int fdiv(int x, unsigned mul) { return mul ? (int)__umulhi((unsigned)x, mul) : x; }
+
+// One radix-R Stockham pass over the tx pencils held in LDS.  Butterfly b
+// (pencil b / NB, index j = b mod NB) runs on thread b mod nthr; a workgroup
+// has half as many threads as the widest pass has butterflies, so each
+// thread runs up to two independent butterflies and the active ones fill
+// whole waves.
 template <int R, int DIR, typename T>
-__device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw, int n, int ldsp, int Ns,
-                                              int c, int j, bool in_range) {
-  const int nb = n / R;
-  const bool act = in_range && j < nb;
-  cx<T> v[R];
-  int k = 0;
-  if (act) {
-    k = j % Ns;
-    const int tstep = k * (n / (Ns * R));
-    const cx<T> *src = buf + c * ldsp + j;
+__device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw, const PassStep &st, int tx, int ldsp) {
+  const int NB = st.NB, Ns = st.Ns;
+  const int nthr = blockDim.x;
+  const int tot = tx * NB;
+  cx<T> v[2][R];
+  int base[2] = {0, 0}, cb[2] = {0, 0};
+  bool act[2];
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = src[r * nb];
+  for (int t = 0; t < 2; ++t) {
+    const int b = (int)threadIdx.x + t * nthr;
+    act[t] = b < tot;
+    if (act[t]) {
+      const int c = fdiv(b, st.nb_mul);
+      const int j = b - c * NB;
+      const cx<T> *src = buf + c * ldsp;
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-      cx<T> w = tw[r * tstep];
-      if (DIR > 0) w.y = -w.y;
-      v[r] = mul(v[r], w);
+      for (int r = 0; r < R; ++r) v[t][r] = src[gpadx(j + r * NB)];
+      int k = 0;
+      if (Ns > 1) {
+        k = j - fdiv(j, st.ns_mul) * Ns;
+        const int ts = k * st.tstep;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cx<T> w = tw[r * ts];
+          if (DIR > 0) w.y = -w.y;
+          v[t][r] = mul(v[t][r], w);
+        }
+      }
+      Dft<R, DIR, T>::run(v[t]);
+      base[t] = (j - k) * R + k;   // (j / Ns) * Ns * R + k
+      cb[t] = c * ldsp;
     }
-    Dft<R, DIR, T>::run(v);
   }
   __syncthreads();
-  if (act) {
-    cx<T> *dst = buf + c * ldsp + (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
+  for (int t = 0; t < 2; ++t) {
+    if (act[t]) {
+      cx<T> *dst = buf + cb[t];
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[gpadx(base[t] + r * Ns)] = v[t][r];
+    }
   }
   __syncthreads();
 }
 
 template <int DIR, typename T>
 __device__ void run_passes(cx<T> *buf, const cx<T> *tw, const PassPlan &pl) {
-  const int tpp = pl.n / pl.rmin;
-  const int c = threadIdx.x / tpp;
-  const int j = threadIdx.x - c * tpp;
-  const bool in_range = c < pl.tx;
-  int Ns = 1;
   for (int p = 0; p < pl.nrad; ++p) {
-    const int R = pl.rad[p];
-    switch (R) {
-      case 2: stockham_pass<2, DIR>(buf, tw, pl.n, pl.ldsp, Ns, c, j, in_range); break;
-      case 3: stockham_pass<3, DIR>(buf, tw, pl.n, pl.ldsp, Ns, c, j, in_range); break;
-      case 4: stockham_pass<4, DIR>(buf, tw, pl.n, pl.ldsp, Ns, c, j, in_range); break;
-      case 5: stockham_pass<5, DIR>(buf, tw, pl.n, pl.ldsp, Ns, c, j, in_range); break;
-      case 7: stockham_pass<7, DIR>(buf, tw, pl.n, pl.ldsp, Ns, c, j, in_range); break;
-      default: stockham_pass<8, DIR>(buf, tw, pl.n, pl.ldsp, Ns, c, j, in_range); break;
+    const PassStep &st = pl.step[p];
+    switch (st.R) {
+      case 2: stockham_pass<2, DIR>(buf, tw, st, pl.tx, pl.ldsp); break;
+      case 3: stockham_pass<3, DIR>(buf, tw, st, pl.tx, pl.ldsp); break;
+      case 4: stockham_pass<4, DIR>(buf, tw, st, pl.tx, pl.ldsp); break;
+      case 5: stockham_pass<5, DIR>(buf, tw, st, pl.tx, pl.ldsp); break;
+      case 7: stockham_pass<7, DIR>(buf, tw, st, pl.tx, pl.ldsp); break;
+      default: stockham_pass<8, DIR>(buf, tw, st, pl.tx, pl.ldsp); break;
     }
-    Ns *= R;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Strided complex pass (y or z).  Columns are contiguous; elements `es`
-// apart; blockIdx.y selects a batch `bstride` apart.
+// apart; a batch dimension `bstride` apart; blockIdx.z selects the grid.
 // ---------------------------------------------------------------------------
 template <int DIR, typename T, typename SI, typename SO>
 __global__ void __launch_bounds__(1024) k_fft_strided(GridSet gs, long long es, long long ncols, long long bstride,
@@ -93,21 +115,35 @@ __global__ void __launch_bounds__(1024) k_fft_strided(GridSet gs, long long es, 
   cx<SO> *obase = dst + off;
   for (int q = threadIdx.x; q < pl.n; q += nthr) tw[q] = twg[q];
   const int tot = pl.tx * pl.n;
-  for (int q = threadIdx.x; q < tot; q += nthr) {
-    const int e = q / pl.tx;
-    const int c = q - e * pl.tx;
-    if (c < ncol) {
-      const cx<SI> v = ibase[c + e * es];
-      buf[c * pl.ldsp + e] = mk((T)v.x, (T)v.y);
+  // kUnroll loads in flight per thread before they are stored to LDS
+  for (int q0 = threadIdx.x; q0 < tot; q0 += kUnroll * nthr) {
+    cx<SI> v[kUnroll];
+    int slot[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int q = q0 + u * nthr;
+      slot[u] = -1;
+      if (q < tot) {
+        const int e = fdiv(q, pl.tx_mul);
+        const int c = q - e * pl.tx;
+        if (c < ncol) {
+          v[u] = ibase[c + e * es];
+          slot[u] = c * pl.ldsp + gpadx(e);
+        }
+      }
     }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (slot[u] >= 0) buf[slot[u]] = mk((T)v[u].x, (T)v[u].y);
   }
   __syncthreads();
   run_passes<DIR>(buf, tw, pl);
+#pragma unroll 4
   for (int q = threadIdx.x; q < tot; q += nthr) {
-    const int e = q / pl.tx;
+    const int e = fdiv(q, pl.tx_mul);
     const int c = q - e * pl.tx;
     if (c < ncol) {
-      const cx<T> v = buf[c * pl.ldsp + e];
+      const cx<T> v = buf[c * pl.ldsp + gpadx(e)];
       obase[c + e * es] = mk((SO)v.x, (SO)v.y);
     }
   }
@@ -136,13 +172,24 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nro
   const long long row0 = (long long)bx * 2 * pl.tx;
   for (int q = threadIdx.x; q < n; q += nthr) tw[q] = twg[q];
   const int tot = 2 * pl.tx * n;
-  for (int q = threadIdx.x; q < tot; q += nthr) {
-    const int rl = q / n;
-    const int e = q - rl * n;
-    const long long row = row0 + rl;
-    T v = row < nrows ? (T)src[row * rnxp + e] : (T)0;
-    T *dst = reinterpret_cast<T *>(buf + (rl >> 1) * pl.ldsp + e);
-    dst[rl & 1] = v;
+  for (int q0 = threadIdx.x; q0 < tot; q0 += kUnroll * nthr) {
+    T v[kUnroll];
+    int slot[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int q = q0 + u * nthr;
+      slot[u] = -1;
+      if (q < tot) {
+        const int rl = fdiv(q, pl.n_mul);
+        const int e = q - rl * n;
+        const long long row = row0 + rl;
+        v[u] = row < nrows ? (T)src[row * rnxp + e] : (T)0;
+        slot[u] = 2 * ((rl >> 1) * pl.ldsp + gpadx(e)) + (rl & 1);   // in units of T
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (slot[u] >= 0) reinterpret_cast<T *>(buf)[slot[u]] = v[u];
   }
   __syncthreads();
   run_passes<-1>(buf, tw, pl);
@@ -150,10 +197,10 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nro
   const T hs = (T)0.5 * scale;
   const int tot2 = pl.tx * cn;
   for (int q = threadIdx.x; q < tot2; q += nthr) {
-    const int c = q / cn;
+    const int c = fdiv(q, pl.cn_mul);
     const int k = q - c * cn;
-    const cx<T> z = buf[c * pl.ldsp + k];
-    const cx<T> zm = buf[c * pl.ldsp + (k == 0 ? 0 : n - k)];
+    const cx<T> z = buf[c * pl.ldsp + gpadx(k)];
+    const cx<T> zm = buf[c * pl.ldsp + gpadx(k == 0 ? 0 : n - k)];
     const long long ra = row0 + 2 * c;
     if (ra < nrows) {
       cx<SO> *pa = reinterpret_cast<cx<SO> *>(dst + ra * rnxp);
@@ -187,29 +234,48 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nro
   const long long row0 = (long long)bx * 2 * pl.tx;
   for (int q = threadIdx.x; q < n; q += nthr) tw[q] = twg[q];
   const int tot = pl.tx * n;
-  for (int q = threadIdx.x; q < tot; q += nthr) {
-    const int c = q / n;
-    const int k = q - c * n;
-    const bool mirror = k >= cn;
-    const int kk = mirror ? n - k : k;
-    const bool selfc = (kk == 0) || ((n & 1) == 0 && kk == n / 2);
-    const long long ra = row0 + 2 * c;
-    cx<SI> a = mk((SI)0, (SI)0), b = mk((SI)0, (SI)0);
-    if (ra < nrows) a = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[kk];
-    if (ra + 1 < nrows) b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[kk];
-    if (selfc) { a.y = (SI)0; b.y = (SI)0; }
-    if (mirror) { a.y = -a.y; b.y = -b.y; }
-    buf[c * pl.ldsp + k] = mk((T)a.x - (T)b.y, (T)a.y + (T)b.x);
+  for (int q0 = threadIdx.x; q0 < tot; q0 += kUnroll * nthr) {
+    cx<SI> a[kUnroll], b[kUnroll];
+    int slot[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int q = q0 + u * nthr;
+      slot[u] = -1;
+      if (q < tot) {
+        const int c = fdiv(q, pl.n_mul);
+        const int k = q - c * n;
+        const bool mirror = k >= cn;
+        const int kk = mirror ? n - k : k;
+        const long long ra = row0 + 2 * c;
+        a[u] = mk((SI)0, (SI)0);
+        b[u] = mk((SI)0, (SI)0);
+        if (ra < nrows) a[u] = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[kk];
+        if (ra + 1 < nrows) b[u] = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[kk];
+        slot[u] = c * pl.ldsp + gpadx(k);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (slot[u] < 0) continue;
+      const int k = (q0 + u * nthr) - fdiv(q0 + u * nthr, pl.n_mul) * n;
+      const bool mirror = k >= cn;
+      const int kk = mirror ? n - k : k;
+      const bool selfc = (kk == 0) || ((n & 1) == 0 && kk == n / 2);
+      cx<SI> x = a[u], y = b[u];
+      if (selfc) { x.y = (SI)0; y.y = (SI)0; }
+      if (mirror) { x.y = -x.y; y.y = -y.y; }
+      buf[slot[u]] = mk((T)x.x - (T)y.y, (T)x.y + (T)y.x);
+    }
   }
   __syncthreads();
   run_passes<+1>(buf, tw, pl);
   const int tot2 = 2 * pl.tx * n;
   for (int q = threadIdx.x; q < tot2; q += nthr) {
-    const int rl = q / n;
+    const int rl = fdiv(q, pl.n_mul);
     const int e = q - rl * n;
     const long long row = row0 + rl;
     if (row < nrows) {
-      const T *sv = reinterpret_cast<const T *>(buf + (rl >> 1) * pl.ldsp + e);
+      const T *sv = reinterpret_cast<const T *>(buf + (rl >> 1) * pl.ldsp + gpadx(e));
       dst[row * rnxp + e] = (SO)(sv[rl & 1] * scale);
     }
   }
@@ -260,8 +326,8 @@ static int max_lds_per_block() {
 }
 
 // Generic-kernel variant g: radix order (g / 3: 0 large radices first, 1
-// small first) x cap on pencils per workgroup (g % 3).
-static const int kGenericTxCap[3] = {64, 16, 4};
+// small first) x cap on pencils per workgroup (g % 3: 64, 16, 8).
+static const int kGenericTxCap[3] = {64, 16, 8};
 
 Fft3D::Fft3D(int nx, int ny, int nz, int p) : nxp(nx), nyp(ny), nzp(nz), prec(p), dbl(p >= 1) {}
 
@@ -278,25 +344,43 @@ int Fft3D::build_plan(int n, PassPlan &p, int g) {
     return CRAVA_HIP_E_INVALID;
   }
   const size_t eb = dbl ? 16 : 8;
-  const int tpp = std::max(1, n / p.rmin);
+  const int tpp = std::max(1, n / p.rmin);   // butterflies of the widest pass, per pencil
   if (tpp > 1024) {
     set_error("FFT length " + std::to_string(n) + " too long for one workgroup");
     return CRAVA_HIP_E_INVALID;
   }
-  p.ldsp = (n % 2 == 0) ? n + 1 : n;
-  // keep two workgroups per CU: at most 80 KiB (or the device limit)
+  // one pad slot every 32 elements, odd pitch (as the specialised kernels)
+  p.ldsp = (n + (n - 1) / 32 + 1) | 1;
+  // keep two workgroups per CU: at most 80 KiB (or the device limit); up to
+  // two butterflies per thread; a power-of-two pencil count keeps the
+  // strided passes' row segments aligned
   const size_t budget = std::min<size_t>((size_t)max_lds_per_block(), 80 * 1024);
   const int tx_lds = (int)((budget / eb - (size_t)n) / (size_t)p.ldsp);
-  const int tx_thr = 1024 / tpp;
-  const int tx = std::min(std::min(tx_lds, tx_thr), kGenericTxCap[g % 3]);
+  const int tx_thr = 2048 / tpp;
+  int tx = std::min(std::min(tx_lds, tx_thr), kGenericTxCap[g % 3]);
   if (tx < 1) {
     set_error("FFT length " + std::to_string(n) + " does not fit in LDS");
     return CRAVA_HIP_E_INVALID;
   }
+  while (tx & (tx - 1)) tx &= tx - 1;
   p.tx = tx;
-  const int threads = tx * tpp;
-  p.threads = ((threads + 63) / 64) * 64;
+  const int threads = (tx * tpp + 1) / 2;
+  p.threads = std::max(64, ((threads + 63) / 64) * 64);
   p.lds_bytes = eb * ((size_t)n + (size_t)tx * p.ldsp);
+  p.tx_mul = div_mul(tx);
+  p.n_mul = div_mul(n);
+  p.cn_mul = div_mul(n / 2 + 1);
+  int ns = 1;
+  for (int i = 0; i < p.nrad; ++i) {
+    PassStep &st = p.step[i];
+    st.R = p.rad[i];
+    st.Ns = ns;
+    st.NB = n / st.R;
+    st.tstep = n / (ns * st.R);
+    st.ns_mul = div_mul(ns);
+    st.nb_mul = div_mul(st.NB);
+    ns *= st.R;
+  }
   return CRAVA_HIP_OK;
 }
 

@@ -14,16 +14,35 @@
 
 namespace crava {
 
+// One radix pass of a runtime-length plan, with its index constants
+// precomputed on the host: ns_mul / nb_mul give j / Ns and b / NB as
+// __umulhi(x, mul) (exact for the index ranges of a workgroup).
+struct PassStep {
+  int R = 2;           // radix
+  int Ns = 1;          // product of the radices before this pass
+  int NB = 1;          // butterflies per pencil, n / R
+  int tstep = 1;       // twiddle stride n / (Ns R)
+  unsigned ns_mul = 0, nb_mul = 0;
+};
+
 struct PassPlan {
   int n = 1;          // transform length
   int nrad = 0;       // number of radix passes
   int rad[24] = {0};  // radices in execution order
-  int rmin = 1;       // smallest radix (sets threads per pencil = n / rmin)
+  int rmin = 1;       // smallest radix (sets the butterflies of the widest pass)
   int tx = 1;         // pencils per workgroup
-  int ldsp = 1;       // LDS pitch of one pencil, in complex elements
+  int ldsp = 1;       // LDS pitch of one pencil (padded), in complex elements
   int threads = 64;   // workgroup size
+  unsigned tx_mul = 0, n_mul = 0, cn_mul = 0;  // division by tx, n, n/2+1
   size_t lds_bytes = 0;
+  PassStep step[24];
 };
+
+// Multiplier m with x / d == __umulhi(x, m) for 0 <= x < 2^32 / d (d >= 2);
+// 0 for d == 1.
+inline unsigned div_mul(int d) {
+  return d <= 1 ? 0u : (unsigned)(((1ull << 32) + (unsigned long long)d - 1) / (unsigned long long)d);
+}
 
 // Radix sequence for n (7-smooth), preferring radix 8 and keeping the
 // smallest radix >= 4 for powers of two >= 4; order 1 reverses it.  Returns
