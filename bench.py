@@ -13,8 +13,10 @@ Crava::computePostMeanResidAndFFTCov with its transforms:
 run twice: the MIXED form (float grids and transforms, FP64 solve where
 noise/signal < threshold) and the plain FP64 form (double grids, double
 transforms, double solve).  value = P * steps / time of the MIXED form.
-The batching workload (48 uneven volumes as one replayed hipGraph) is
-measured after it and reported under "batching".  Prints one JSON line.
+The batching workload (48 uneven volumes as one replayed hipGraph) and the
+differentiation workload (likelihood adjoint at 288x288x216, 5 angles, and
+50 L-BFGS iterations) are measured after it and reported under "batching"
+and "differentiation".  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -96,6 +98,8 @@ def parse():
     ap.add_argument("--sweep", action="store_true", help="report accuracy over a range of FP64 thresholds")
     ap.add_argument("--batch-volumes", type=int, default=48, help="volumes of the batching workload (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--diff-steps", type=int, default=50, help="L-BFGS iterations of the differentiation workload "
+                    "(0: skip it)")
     return ap.parse_args()
 
 
@@ -313,6 +317,90 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
     return out
 
 
+def differentiation(ctx, steps: int = 50):
+    """The differentiation workload: 256x256x192 cells (FFT grid 288x288x216),
+    5 angles.  Seismic data drawn from the likelihood model at known
+    parameters; the hand-written adjoint (crava_hip_nll_grad, FP64, Cholesky
+    recomputed per point) is timed, checked against FP64 central
+    differences (h = 1e-6) on three parameters, and drives L-BFGS from
+    perturbed wavelet scales and noise variances (Sigma0 held)."""
+    import numpy as np
+    import torch
+    from crava_amd import _lib as L
+    from crava_amd import synth
+    from crava_amd.autograd import nll_grad
+    from crava_amd.core import Posterior, problem_tables
+    from crava_amd.estimate import fit
+    t0 = time.time()
+    prob = synth.make_problem(synth.config("differentiation"), "cuda")
+    c = prob.cfg
+    n3 = (c.nxp, c.nyp, c.nzp)
+    nt = prob.ntheta
+    cn = c.nxp // 2 + 1
+    tabs = problem_tables(prob)
+    mu = [g.double() for g in prob.mu]
+    cv = [g.double() for g in prob.cov]
+    ec = prob.errcorr.double()
+    ctx.r2c_many(mu, *n3, L.DATA)
+    ctx.r2c_many(cv + [ec], *n3, L.COVARIANCE)
+    torch.cuda.synchronize()
+    cplx = lambda t: t.cpu().numpy().view(np.complex128)
+    L0 = np.linalg.cholesky(prob.sigma0)
+    truth_s = np.ones(nt)
+    truth_v = np.diag(prob.err_theta_cov).copy()
+    truth_l = np.array([L0[0, 0], L0[1, 0], L0[1, 1], L0[2, 0], L0[2, 1], L0[2, 2]])
+    d = synth.model_data(prob, tabs, [cplx(g) for g in mu], cplx(cv[0]), cplx(ec), truth_s, truth_v, truth_l, seed=3)
+    data = [torch.from_numpy(np.ascontiguousarray(x).view(np.float64)).cuda() for x in d]
+    del d
+    post = Posterior(ctx, synth.post_params(prob), tabs)
+    problem = (post, L.F64, mu, data, cv, ec)
+    n_in = int(tabs["invert"].sum()) * cn * c.nyp
+    log(f"differentiation: {c.nxp}x{c.nyp}x{c.nzp}, {n_in} in-band points, set up in {time.time()-t0:.1f}s")
+    nll_grad(post, L.F64, mu, data, cv, ec, truth_s, truth_v, truth_l)
+    reps = 5
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        f0, g0 = nll_grad(post, L.F64, mu, data, cv, ec, truth_s, truth_v, truth_l)
+    ms = (time.perf_counter() - t1) / reps * 1e3
+    # FP64 central differences (relative step 1e-6) on a scale, a noise
+    # variance and a Sigma0 factor entry
+    fd = {}
+    theta = [truth_s.copy(), truth_v.copy(), truth_l.copy()]
+    for name, (which, i) in {"scale0": (0, 0), "noise_var2": (1, 2), "chol0_L11": (2, 2)}.items():
+        h = 1e-6 * abs(theta[which][i])      # relative step (noise variances are ~1e-4)
+        vals = []
+        for sgn in (1, -1):
+            th = [x.copy() for x in theta]
+            th[which][i] += sgn * h
+            vals.append(nll_grad(post, L.F64, mu, data, cv, ec, *th)[0])
+        num = (vals[0] - vals[1]) / (2 * h)
+        ana = g0[[0, nt, 2 * nt][which] + i]
+        # the difference quotient itself resolves the derivative to about
+        # eps |f| / (h |f'|): the likelihood sums millions of points
+        fd[name] = {"adjoint": ana, "central_difference": num, "h": h,
+                    "rel_err": abs(ana - num) / max(abs(num), 1e-300),
+                    "fd_resolution": 2.2e-16 * abs(f0) / (h * max(abs(num), 1e-300))}
+    # L-BFGS from perturbed scales and noise variances, Sigma0 held at the truth
+    s0 = truth_s * np.array([1.15, 0.85, 1.1, 0.9, 1.05])[:nt]
+    v0 = truth_v * 2.0
+    free = [1.0] * (2 * nt) + [0.0] * 6
+    t1 = time.perf_counter()
+    s, v, _, trace = fit(problem, nt, s0, v0, truth_l, steps=steps, free=free)
+    t_fit = time.perf_counter() - t1
+    post.close()
+    out = {"grid": list(n3), "ntheta": nt, "in_band_points": n_in, "ms_per_nll_and_gradient": ms,
+           "pts_per_s": n_in / (ms * 1e-3), "gradient_vs_fd": fd,
+           "lbfgs": {"iterations": steps, "evaluations": len(trace), "seconds": t_fit,
+                     "nll_start": trace[0][0], "nll_end": trace[-1][0],
+                     "max_rel_err_scales": float(np.abs(s.numpy() / truth_s - 1).max()),
+                     "max_rel_err_noise_var": float(np.abs(v.numpy() / truth_v - 1).max()),
+                     "start_rel_err_scales": 0.15, "start_rel_err_noise_var": 1.0}}
+    log(f"differentiation: {ms:.2f} ms per NLL+gradient; FD {[round(x['rel_err'], 9) for x in fd.values()]}; "
+        f"L-BFGS {t_fit:.1f}s, scales err {out['lbfgs']['max_rel_err_scales']:.3g}, "
+        f"noise err {out['lbfgs']['max_rel_err_noise_var']:.3g}")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -458,6 +546,10 @@ def main():
         del mixed, f64
         torch.cuda.empty_cache()
         batch = batching(ctx, args.batch_volumes, args.batch_reps)
+    diff = None
+    if args.diff_steps > 0:
+        torch.cuda.empty_cache()
+        diff = differentiation(ctx, args.diff_steps)
     cpu = None
     if not args.no_cpu:
         try:
@@ -499,6 +591,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batching": batch,
+        "differentiation": diff,
     }
     print(json.dumps(out))
 

@@ -55,7 +55,7 @@ def fit(problem, nt: int, scales0, noise_var0, chol00, steps: int = 50, history:
         f.backward()
         if mask is not None:
             x.grad.mul_(mask)
-        trace.append((float(f), float(x.grad.norm())))
+        trace.append((float(f.detach()), float(x.grad.norm())))
         return f
 
     opt.step(closure)
