@@ -171,25 +171,28 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nro
   const int nthr = blockDim.x;
   const long long row0 = (long long)bx * 2 * pl.tx;
   for (int q = threadIdx.x; q < n; q += nthr) tw[q] = twg[q];
-  const int tot = 2 * pl.tx * n;
+  // element e of pencil c is (row 2c, row 2c+1) at e: one whole complex LDS
+  // store per lane
+  const int tot = pl.tx * n;
   for (int q0 = threadIdx.x; q0 < tot; q0 += kUnroll * nthr) {
-    T v[kUnroll];
+    T va[kUnroll], vb[kUnroll];
     int slot[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int q = q0 + u * nthr;
       slot[u] = -1;
       if (q < tot) {
-        const int rl = fdiv(q, pl.n_mul);
-        const int e = q - rl * n;
-        const long long row = row0 + rl;
-        v[u] = row < nrows ? (T)src[row * rnxp + e] : (T)0;
-        slot[u] = 2 * ((rl >> 1) * pl.ldsp + gpadx(e)) + (rl & 1);   // in units of T
+        const int c = fdiv(q, pl.n_mul);
+        const int e = q - c * n;
+        const long long ra = row0 + 2 * c;
+        va[u] = ra < nrows ? (T)src[ra * rnxp + e] : (T)0;
+        vb[u] = ra + 1 < nrows ? (T)src[(ra + 1) * rnxp + e] : (T)0;
+        slot[u] = c * pl.ldsp + gpadx(e);
       }
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u)
-      if (slot[u] >= 0) reinterpret_cast<T *>(buf)[slot[u]] = v[u];
+      if (slot[u] >= 0) buf[slot[u]] = mk(va[u], vb[u]);
   }
   __syncthreads();
   run_passes<-1>(buf, tw, pl);
@@ -269,15 +272,15 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nro
   }
   __syncthreads();
   run_passes<+1>(buf, tw, pl);
-  const int tot2 = 2 * pl.tx * n;
+  // element e of pencil c -> real part to row 2c, imaginary part to row 2c+1
+  const int tot2 = pl.tx * n;
   for (int q = threadIdx.x; q < tot2; q += nthr) {
-    const int rl = fdiv(q, pl.n_mul);
-    const int e = q - rl * n;
-    const long long row = row0 + rl;
-    if (row < nrows) {
-      const T *sv = reinterpret_cast<const T *>(buf + (rl >> 1) * pl.ldsp + gpadx(e));
-      dst[row * rnxp + e] = (SO)(sv[rl & 1] * scale);
-    }
+    const int c = fdiv(q, pl.n_mul);
+    const int e = q - c * n;
+    const long long ra = row0 + 2 * c;
+    const cx<T> v = buf[c * pl.ldsp + gpadx(e)];
+    if (ra < nrows) dst[ra * rnxp + e] = (SO)(v.x * scale);
+    if (ra + 1 < nrows) dst[(ra + 1) * rnxp + e] = (SO)(v.y * scale);
   }
 }
 
@@ -336,6 +339,8 @@ Fft3D::~Fft3D() {
   if (twx) (void)hipFree(twx);
   if (twy) (void)hipFree(twy);
   if (twz) (void)hipFree(twz);
+  for (void *t : stw)
+    if (t) (void)hipFree(t);
 }
 
 int Fft3D::build_plan(int n, PassPlan &p, int g) {
@@ -385,6 +390,17 @@ int Fft3D::build_plan(int n, PassPlan &p, int g) {
 }
 
 template <typename T>
+static int upload_static_twiddles(int n, void **dst) {
+  std::vector<double> h(2 * (size_t)n);
+  if (!static_twiddle_table(n, h.data())) return CRAVA_HIP_OK;
+  std::vector<cx<T>> t(n);
+  for (int i = 0; i < n; ++i) t[i] = mk((T)h[2 * i], (T)h[2 * i + 1]);
+  CRAVA_HIP_CHECK(hipMalloc(dst, sizeof(cx<T>) * n));
+  CRAVA_HIP_CHECK(hipMemcpy(*dst, t.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
+  return CRAVA_HIP_OK;
+}
+
+template <typename T>
 static int upload_twiddles(int n, void **dst) {
   std::vector<cx<T>> h(n);
   for (int m = 0; m < n; ++m) {
@@ -409,11 +425,13 @@ int Fft3D::init() {
   if ((rc = build_plan(nxp, px, 0)) || (rc = build_plan(nyp, py, 0)) || (rc = build_plan(nzp, pz, 0))) return rc;
   if (dbl) {
     if ((rc = upload_twiddles<double>(nxp, &twx)) || (rc = upload_twiddles<double>(nyp, &twy)) ||
-        (rc = upload_twiddles<double>(nzp, &twz)))
+        (rc = upload_twiddles<double>(nzp, &twz)) || (rc = upload_static_twiddles<double>(nxp, &stw[0])) ||
+        (rc = upload_static_twiddles<double>(nyp, &stw[1])) || (rc = upload_static_twiddles<double>(nzp, &stw[2])))
       return rc;
   } else {
     if ((rc = upload_twiddles<float>(nxp, &twx)) || (rc = upload_twiddles<float>(nyp, &twy)) ||
-        (rc = upload_twiddles<float>(nzp, &twz)))
+        (rc = upload_twiddles<float>(nzp, &twz)) || (rc = upload_static_twiddles<float>(nxp, &stw[0])) ||
+        (rc = upload_static_twiddles<float>(nyp, &stw[1])) || (rc = upload_static_twiddles<float>(nzp, &stw[2])))
       return rc;
   }
   if (prec == 1 && (rc = ensure_scratch(1))) return rc;
@@ -568,18 +586,18 @@ int Fft3D::pass(int axis, void *const *grids, int ng, bool inverse, const double
   if (axis == 0) {
     if (var[0] >= 0)
       return static_fft_launch(nxp, io, inverse ? kRowsC2R : kRowsR2C, var[0], gs, 0, 0, 0, 0, (long long)nyp * nzp,
-                               (int)(2 * cnxp), twx, s);
+                               (int)(2 * cnxp), stw[0], s);
     return generic_x(*this, io, gs, s, inverse);
   }
   const FftKind kind = inverse ? kStridedInv : kStridedFwd;
   if (axis == 1) {
     if (nyp <= 1) return copy_between(io, gs, cnxp * nyp * nzp, s);
-    if (var[1] >= 0) return static_fft_launch(nyp, io, kind, var[1], gs, cnxp, cnxp, nzp, cnxp * nyp, 0, 0, twy, s);
+    if (var[1] >= 0) return static_fft_launch(nyp, io, kind, var[1], gs, cnxp, cnxp, nzp, cnxp * nyp, 0, 0, stw[1], s);
     return generic_strided(py, twy, io, gs, s, inverse, cnxp, cnxp, nzp, cnxp * nyp);
   }
   const long long plane = cnxp * nyp;
   if (nzp <= 1) return copy_between(io, gs, plane * nzp, s);
-  if (var[2] >= 0) return static_fft_launch(nzp, io, kind, var[2], gs, plane, plane, 1, 0, 0, 0, twz, s);
+  if (var[2] >= 0) return static_fft_launch(nzp, io, kind, var[2], gs, plane, plane, 1, 0, 0, 0, stw[2], s);
   return generic_strided(pz, twz, io, gs, s, inverse, plane, plane, 1, 0);
 }
 

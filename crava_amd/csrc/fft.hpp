@@ -91,6 +91,7 @@ class Fft3D {
   bool dbl;  // double arithmetic (LDS, twiddles)
   PassPlan px, py, pz;
   void *twx = nullptr, *twy = nullptr, *twz = nullptr;  // device twiddle tables
+  void *stw[3] = {nullptr, nullptr, nullptr};  // per-pass tables of the specialised kernels, per axis
   void *scratch = nullptr;  // double intermediates of a prec-1 plan, one slot per grid
   int scratch_grids = 0;
   // Launch variant per axis (x, y, z): v >= 0 specialised kernel variant v,

@@ -4,39 +4,52 @@
 // radix sequence, the pencils per workgroup TX and the workgroup size are
 // template constants, so every index division folds into shifts or
 // multiply-highs and the passes unroll.  Each pencil sits in LDS with one pad
-// slot every 32 elements (index x -> x + x/32) and an odd pencil pitch, which
-// removes the bank conflicts of the strided early-pass writes and of the
-// column-major tile loads.
+// slot every 32 complex floats or 16 complex doubles and an odd pencil pitch,
+// which spreads the strided early-pass writes and the column-major tile
+// loads over the banks.
 #include "../../include/crava_hip.h"
 #include "fft_dft.hpp"
 #include "fft_static.hpp"
 
+#include <cmath>
+
 namespace crava {
 
-__device__ __forceinline__ constexpr int padx(int x) { return x + (x >> 5); }
+// One pad slot every 32 complex floats, or every 16 complex doubles (a
+// 16-byte element spans four banks; the denser padding keeps the strided
+// Stockham writes spread over the banks).
+template <typename T>
+constexpr int kPadShift = sizeof(T) == 8 ? 4 : 5;
+template <typename T>
+__device__ __forceinline__ constexpr int padx(int x) { return x + (x >> kPadShift<T>); }
 
-template <int N>
+template <int N, typename T>
 struct SP {
   static constexpr RadixList rl = make_radices(N);
-  static constexpr int len = N + (N - 1) / 32 + 1;
+  static constexpr int len = N + ((N - 1) >> kPadShift<T>) + 1;
   static constexpr int pitch = len | 1;
   static constexpr int ns(int p) {
     int s = 1;
     for (int i = 0; i < p; ++i) s *= rl.r[i];
     return s;
   }
+  // start of pass p's twiddles in the per-pass table (static_twiddle_table)
+  static constexpr int off(int p) {
+    int o = 0;
+    for (int i = 0; i < p; ++i) o += ns(i) * (rl.r[i] - 1);
+    return o;
+  }
 };
 
 template <int N, int TX, int THREADS, int DIR, int P, typename T>
 __device__ __forceinline__ void spasses(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw) {
-  if constexpr (P < SP<N>::rl.cnt) {
-    constexpr int R = SP<N>::rl.r[P];
-    constexpr int Ns = SP<N>::ns(P);
+  if constexpr (P < SP<N, T>::rl.cnt) {
+    constexpr int R = SP<N, T>::rl.r[P];
+    constexpr int Ns = SP<N, T>::ns(P);
     constexpr int NB = N / R;
     constexpr int TOT = TX * NB;
     constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
-    constexpr int TSTEP = N / (Ns * R);
-    constexpr int PITCH = SP<N>::pitch;
+    constexpr int PITCH = SP<N, T>::pitch;
     cx<T> v[TRIPS][R];
 #pragma unroll
     for (int t = 0; t < TRIPS; ++t) {
@@ -45,12 +58,13 @@ __device__ __forceinline__ void spasses(cx<T> *__restrict__ buf, const cx<T> *__
         const int c = b / NB, j = b - (b / NB) * NB;
         const cx<T> *src = buf + c * PITCH;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[t][r] = src[padx(j + r * NB)];
+        for (int r = 0; r < R; ++r) v[t][r] = src[padx<T>(j + r * NB)];
         if constexpr (Ns > 1) {
+          // per-pass table [r-1][k]: the lanes of a wave read consecutive k
           const int k = j % Ns;
 #pragma unroll
           for (int r = 1; r < R; ++r) {
-            cx<T> w = tw[r * k * TSTEP];
+            cx<T> w = tw[SP<N, T>::off(P) + (r - 1) * Ns + k];
             if (DIR > 0) w.y = -w.y;
             v[t][r] = mul(v[t][r], w);
           }
@@ -68,7 +82,7 @@ __device__ __forceinline__ void spasses(cx<T> *__restrict__ buf, const cx<T> *__
         cx<T> *dst = buf + c * PITCH;
         const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[padx(base + r * Ns)] = v[t][r];
+        for (int r = 0; r < R; ++r) dst[padx<T>(base + r * Ns)] = v[t][r];
       }
     }
     __syncthreads();
@@ -78,7 +92,7 @@ __device__ __forceinline__ void spasses(cx<T> *__restrict__ buf, const cx<T> *__
 
 template <int N, typename T>
 constexpr size_t lds_bytes(int tx) {
-  return sizeof(cx<T>) * ((size_t)N + (size_t)tx * SP<N>::pitch);
+  return sizeof(cx<T>) * ((size_t)N + (size_t)tx * SP<N, T>::pitch);
 }
 
 // Strided complex pass: columns contiguous, elements `es` apart.
@@ -94,7 +108,7 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long 
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
-  constexpr int PITCH = SP<N>::pitch;
+  constexpr int PITCH = SP<N, T>::pitch;
   const long long col0 = (long long)bx * TX;
   const int ncol = (int)min((long long)TX, ncols - col0);
   const long long off = (long long)by * bstride + col0;
@@ -110,7 +124,7 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long 
       const int e = q / TX, c = q % TX;
       if (c < ncol) {
         const cx<SI> v = ibase[c + (long long)e * es];
-        buf[c * PITCH + padx(e)] = mk((T)v.x, (T)v.y);
+        buf[c * PITCH + padx<T>(e)] = mk((T)v.x, (T)v.y);
       }
     }
   }
@@ -122,7 +136,7 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long 
     if (TOT % THREADS == 0 || q < TOT) {
       const int e = q / TX, c = q % TX;
       if (c < ncol) {
-        const cx<T> v = buf[c * PITCH + padx(e)];
+        const cx<T> v = buf[c * PITCH + padx<T>(e)];
         obase[c + (long long)e * es] = mk((SO)v.x, (SO)v.y);
       }
     }
@@ -142,25 +156,27 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nro
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
-  constexpr int PITCH = SP<N>::pitch;
+  constexpr int PITCH = SP<N, T>::pitch;
   const long long row0 = (long long)bx * 2 * TX;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
   static_assert(N % 2 == 0, "row kernels need an even length");
-  constexpr int H = N / 2;           // value pairs per row
-  constexpr int TOT = 2 * TX * H;
+  // lane (c, e2) reads the value pair e2 of rows 2c and 2c+1 and stores the
+  // two whole complex elements 2e2, 2e2+1 of pencil c (a = Re, b = Im)
+  constexpr int H = N / 2;
+  constexpr int TOT = TX * H;
   constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
 #pragma unroll
   for (int t = 0; t < TRIPS; ++t) {
     const int q = (int)threadIdx.x + t * THREADS;
     if (TOT % THREADS == 0 || q < TOT) {
-      const int rl = q / H, e2 = q % H;
-      const long long row = row0 + rl;
-      cx<SI> v = mk((SI)0, (SI)0);
-      if (row < nrows) v = reinterpret_cast<const cx<SI> *>(src + row * rnxp)[e2];
-      T *d0 = reinterpret_cast<T *>(buf + (rl >> 1) * PITCH + padx(2 * e2));
-      T *d1 = reinterpret_cast<T *>(buf + (rl >> 1) * PITCH + padx(2 * e2 + 1));
-      d0[rl & 1] = (T)v.x;
-      d1[rl & 1] = (T)v.y;
+      const int c = q / H, e2 = q % H;
+      const long long ra = row0 + 2 * c;
+      cx<SI> a = mk((SI)0, (SI)0), b = mk((SI)0, (SI)0);
+      if (ra < nrows) a = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[e2];
+      if (ra + 1 < nrows) b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[e2];
+      cx<T> *d = buf + c * PITCH;
+      d[padx<T>(2 * e2)] = mk((T)a.x, (T)b.x);
+      d[padx<T>(2 * e2 + 1)] = mk((T)a.y, (T)b.y);
     }
   }
   __syncthreads();
@@ -174,8 +190,8 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nro
     const int q = (int)threadIdx.x + t * THREADS;
     if (TOT2 % THREADS == 0 || q < TOT2) {
       const int c = q / CN, k = q % CN;
-      const cx<T> z = buf[c * PITCH + padx(k)];
-      const cx<T> zm = buf[c * PITCH + padx(k == 0 ? 0 : N - k)];
+      const cx<T> z = buf[c * PITCH + padx<T>(k)];
+      const cx<T> zm = buf[c * PITCH + padx<T>(k == 0 ? 0 : N - k)];
       const long long ra = row0 + 2 * c;
       if (ra < nrows)
         reinterpret_cast<cx<SO> *>(dst + ra * rnxp)[k] = mk((SO)(hs * (z.x + zm.x)), (SO)(hs * (z.y - zm.y)));
@@ -198,7 +214,7 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nro
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
-  constexpr int PITCH = SP<N>::pitch;
+  constexpr int PITCH = SP<N, T>::pitch;
   constexpr int CN = N / 2 + 1;
   const long long row0 = (long long)bx * 2 * TX;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
@@ -218,25 +234,27 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nro
       if (ra + 1 < nrows) b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[kk];
       if (selfc) { a.y = (SI)0; b.y = (SI)0; }
       if (mirror) { a.y = -a.y; b.y = -b.y; }
-      buf[c * PITCH + padx(k)] = mk((T)a.x - (T)b.y, (T)a.y + (T)b.x);
+      buf[c * PITCH + padx<T>(k)] = mk((T)a.x - (T)b.y, (T)a.y + (T)b.x);
     }
   }
   __syncthreads();
   spasses<N, TX, THREADS, 1, 0>(buf, tw);
+  // elements 2e2, 2e2+1 of pencil c -> value pair e2 of rows 2c (Re) and 2c+1 (Im)
   constexpr int H = N / 2;
-  constexpr int TOT2 = 2 * TX * H;
+  constexpr int TOT2 = TX * H;
   constexpr int TRIPS2 = (TOT2 + THREADS - 1) / THREADS;
 #pragma unroll
   for (int t = 0; t < TRIPS2; ++t) {
     const int q = (int)threadIdx.x + t * THREADS;
     if (TOT2 % THREADS == 0 || q < TOT2) {
-      const int rl = q / H, e2 = q % H;
-      const long long row = row0 + rl;
-      if (row < nrows) {
-        const T *s0 = reinterpret_cast<const T *>(buf + (rl >> 1) * PITCH + padx(2 * e2));
-        const T *s1 = reinterpret_cast<const T *>(buf + (rl >> 1) * PITCH + padx(2 * e2 + 1));
-        reinterpret_cast<cx<SO> *>(dst + row * rnxp)[e2] = mk((SO)(s0[rl & 1] * scale), (SO)(s1[rl & 1] * scale));
-      }
+      const int c = q / H, e2 = q % H;
+      const long long ra = row0 + 2 * c;
+      const cx<T> *s0 = buf + c * PITCH;
+      const cx<T> v0 = s0[padx<T>(2 * e2)], v1 = s0[padx<T>(2 * e2 + 1)];
+      if (ra < nrows)
+        reinterpret_cast<cx<SO> *>(dst + ra * rnxp)[e2] = mk((SO)(v0.x * scale), (SO)(v1.x * scale));
+      if (ra + 1 < nrows)
+        reinterpret_cast<cx<SO> *>(dst + (ra + 1) * rnxp)[e2] = mk((SO)(v0.y * scale), (SO)(v1.y * scale));
     }
   }
 }
@@ -364,6 +382,30 @@ static size_t max_lds() {
   return (size_t)v;
 }
 
+// Twiddles of the specialised kernels, laid out per radix pass: pass p
+// (radix R, Ns = product of the earlier radices) holds W_n^(r k n/(Ns R)) for
+// r = 1..R-1, k = 0..Ns-1 at off(p) + (r-1) Ns + k.  n entries in total
+// (the last is unused).  re_im receives 2n values.
+bool static_twiddle_table(int n, double *re_im) {
+  if (!static_fft_available(n)) return false;
+  const RadixList rl = make_radices(n);
+  int o = 0, ns = 1;
+  for (int i = 0; i < 2 * n; ++i) re_im[i] = 0.0;
+  for (int p = 0; p < rl.cnt; ++p) {
+    const int R = rl.r[p];
+    for (int r = 1; r < R; ++r)
+      for (int k = 0; k < ns; ++k) {
+        const long long m = ((long long)r * k * (n / (ns * R))) % n;
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+        re_im[2 * (o + (r - 1) * ns + k)] = (double)cosl(a);
+        re_im[2 * (o + (r - 1) * ns + k) + 1] = (double)sinl(a);
+      }
+    o += ns * (R - 1);
+    ns *= R;
+  }
+  return true;
+}
+
 bool static_fft_geometry(int n, bool dbl, FftKind kind, int variant, StaticLaunch *out) {
   if (!static_fft_available(n)) return false;
   const bool strided = kind == kStridedFwd || kind == kStridedInv;
@@ -372,7 +414,7 @@ bool static_fft_geometry(int n, bool dbl, FftKind kind, int variant, StaticLaunc
   const int tx = strided ? kSV[variant][0] : kRV[variant][0];
   const int th = strided ? kSV[variant][1] : kRV[variant][1];
   const size_t eb = dbl ? 16 : 8;
-  const size_t len = (size_t)n + (size_t)(n - 1) / 32 + 1;
+  const size_t len = (size_t)n + (size_t)((n - 1) >> (dbl ? 4 : 5)) + 1;
   const size_t pitch = len | 1;
   const size_t lds = eb * ((size_t)n + (size_t)tx * pitch);
   if (lds > max_lds()) return false;

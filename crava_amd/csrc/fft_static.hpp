@@ -20,6 +20,9 @@ struct StaticLaunch {
 
 // True if length n has specialised kernels.
 bool static_fft_available(int n);
+// The per-pass twiddle table the specialised kernels of length n read
+// (2n doubles, {re, im} pairs); false if n is not specialised.
+bool static_twiddle_table(int n, double *re_im);
 // Launch geometry of a variant; false if the variant does not fit.
 bool static_fft_geometry(int n, bool dbl_compute, FftKind kind, int variant, StaticLaunch *out);
 // Up to kMaxGrids same-size grids transformed by one launch (blockIdx.z).
