@@ -113,3 +113,37 @@ def test_posterior_f32_vs_golden_reference_outputs(ctx):
         assert rel_l2(_cplx(got.cpu().numpy()), _cplx(ref)) < 1e-6
     for got, ref in zip(td + tc, list(G["post_out_d"]) + list(G["post_out_cov"])):
         assert rel_l2(_cplx(got.cpu().numpy()), _cplx(ref)) < 2e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nt", [1, 2, 5, 6, 7, 8])
+def test_posterior_every_ntheta_vs_oracle(ctx, nt):
+    """Every angle count the kernel is instantiated for (1..8; 3 and 4 are the
+    small/plain_cpu cases above), F32 against the float oracle and MIXED with
+    every point in FP64 against the FP64 oracle, on an odd-x grid."""
+    print(f"posterior ntheta={nt}", flush=True)
+    th = list(np.linspace(5.0, 40.0, nt))
+    cfg = synth.Config(f"nt{nt}", 12, 10, 20, 15, 12, 24, th, [35.0] * nt, list(np.linspace(0.9, 1.1, nt)),
+                       list(np.linspace(8.0, 3.0, nt)), low_cut=5.0, high_cut=100.0, seed=nt)
+    prob = synth.make_problem(cfg, "cpu")
+    grids = fourier_inputs(prob)
+    tabs = problem_tables(prob)
+    mu, d, cv, ec = [g.copy() for g in grids[0]], [g.copy() for g in grids[1]], [g.copy() for g in grids[2]], \
+        grids[3].copy()
+    nf_ref = oracle.posterior(cfg.nxp, cfg.nyp, cfg.nzp, prob.A, prob.err_theta_cov, cfg.wnc, prob.sigma0,
+                              tabs, mu, d, cv, ec)
+    gm, gd, gc, nf, _ = _run_gpu(ctx, prob, grids, L.F32)
+    assert nf == nf_ref
+    for a, b in zip(gm, mu):
+        assert rel_l2(_cplx(a), _cplx(b)) < 1e-6
+    for a, b in zip(gd + gc, d + cv):
+        assert rel_l2(_cplx(a), _cplx(b)) < 2e-5
+    m64, d64, c64 = as_f64(grids[0]), as_f64(grids[1]), as_f64(grids[2])
+    e64 = as_f64([grids[3]])[0]
+    om, od, oc = [g.copy() for g in m64], [g.copy() for g in d64], [g.copy() for g in c64]
+    oracle.posterior_f64(cfg.nxp, cfg.nyp, cfg.nzp, prob.A, prob.err_theta_cov, cfg.wnc, prob.sigma0, tabs,
+                         om, od, oc, e64.copy())
+    gm, gd, gc, nf, n64 = _run_gpu(ctx, prob, grids, L.MIXED, thr=1e30)
+    assert n64 > 0
+    for a, b in zip(gc, oc):
+        assert rel_l2(_cplx(a), _cplx(b)) < 1e-6
