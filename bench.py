@@ -12,7 +12,10 @@ Crava::computePostMeanResidAndFFTCov with its transforms:
 
 run twice: the MIXED form (float grids and transforms, FP64 solve where
 noise/signal < threshold) and the plain FP64 form (double grids, double
-transforms, double solve).  value = P * steps / time of the MIXED form.
+transforms, double solve).  Each form's step runs two ways: as a one-volume
+crava_hip_batch replayed as a hipGraph (the data/mean and covariance
+transform sets on two streams) and as separate launches on one stream;
+both times are reported and the faster is the form's time.  value = P * steps / time of the MIXED form.
 The batching workload (48 uneven volumes as one replayed hipGraph) and the
 differentiation workload (likelihood adjoint at 288x288x216, 5 angles, and
 50 L-BFGS iterations) are measured after it and reported under "batching"
@@ -115,22 +118,45 @@ class Pipeline:
         self.mu = [g.to(dt).clone() for g in prob.mu]
         self.data = [g.to(dt).clone() for g in prob.data]
         self.cov = [g.to(dt).clone() for g in prob.cov]
+        self.cov0 = [g.clone() for g in self.cov]
         self.ec0 = prob.errcorr.to(dt).clone()
         self.ec = self.ec0.clone()
         self.post = Posterior(ctx, params, tables)
         self.precision = L.F64 if fp64 else L.MIXED
         self.cov_precision = L.F64 if fp64 else L.MIXED
         self.L = L
+        self.batch = None
+
+    def use_graph(self):
+        """Run the step as a one-volume batch replayed as a hipGraph: the
+        data/mean and covariance transform sets on two streams (the mixed
+        form's double-arithmetic covariance transforms overlap the float data
+        transforms), the posterior, the inverse sets."""
+        from crava_amd.core import Batch
+        self.batch = Batch(self.ctx, self.precision, [(self.post, self.mu, self.data, self.cov, self.ec)])
+        return self.batch.autotune()
 
     def step(self, counts=False):
         """One pass of the path; each group of same-kind grids is one launch per transform pass."""
         L, c = self.L, self.prob.cfg
         n = (c.nxp, c.nyp, c.nzp)
+        # reload the prior covariance and error-correlation grids: every step
+        # then solves the same fresh problem (the FP64 point count depends on
+        # them; the previous step's means and residuals as inputs do not
+        # change the work)
         if self.fp64:
-            self.ec.copy_(self.ec0)
-        else:   # the forward transform consumes the error-correlation grid
+            for d, s_ in zip(self.cov + [self.ec], self.cov0 + [self.ec0]):
+                d.copy_(s_)
+        else:
             from crava_amd.core import grid_op
-            grid_op(self.ctx, L.OP_COPY, self.ec, self.ec0, *n, False)
+            for d, s_ in zip(self.cov + [self.ec], self.cov0 + [self.ec0]):
+                grid_op(self.ctx, L.OP_COPY, d, s_, *n, False)
+        if self.batch is not None:
+            self.batch.run(graph=True)
+            if counts:
+                nf, n64 = self.batch.counts()
+                return nf[0], n64[0]
+            return None
         # data and prior means share the 1/sqrt(N) scaling (DATA / PARAMETER)
         self.ctx.r2c_many(self.data + self.mu, *n, L.DATA)
         # covariance spectra span many decades: the mixed form transforms them
@@ -161,13 +187,19 @@ def rel_err(a, b, nxp, ref_base=None):
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(den))
 
 
-def timed_kernel(fn, reps):
-    """Average duration (ms) of fn() over reps launches, HIP events on torch's stream."""
+def timed_kernel(fn, reps, setup=None):
+    """Average duration (ms) of fn() over reps launches, HIP events on torch's
+    stream; setup() (e.g. restoring in-place inputs) runs before each launch,
+    outside the events."""
     import torch
+    if setup:
+        setup()
     fn()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
     for s, e in ev:
+        if setup:
+            setup()
         s.record()
         fn()
         e.record()
@@ -276,9 +308,9 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
     log(f"batching: {volumes} volumes generated in {time.time()-t0:.1f}s, {P} Fourier points")
     b = Batch(ctx, L.MIXED, vols)
     t0 = time.time()
-    ppb, blocks, per_class, post_ms = b.autotune()
+    ppb, blocks, form, post_ms = b.autotune()
     log(f"batching autotune {time.time()-t0:.1f}s: {ppb} points/block, {blocks} workgroups, "
-        f"per-class={per_class}, posterior {post_ms:.3f} ms")
+        f"form={form}, posterior {post_ms:.3f} ms")
 
     def reload():
         for pristine, work in src:
@@ -310,7 +342,9 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
            "ntheta_histogram": {str(k): nts.count(k) for k in sorted(set(nts))},
            "ms_per_batch_graph": ms_graph, "ms_per_batch_eager": ms_eager,
            "value": P / (ms_graph * 1e-3), "unit": "Fourier pts/s (whole path: transforms + posterior)",
-           "posterior_launch": {"points_per_block": ppb, "workgroups": blocks, "per_ntheta_class": per_class,
+           "posterior_launch": {"points_per_block": ppb, "workgroups": blocks,
+                                "form": ["one persistent launch", "one persistent launch per ntheta class",
+                                         "one launch per volume"][form],
                                 "ms": post_ms, "pts_per_s": P / (post_ms * 1e-3)},
            "failed_factorisations": int(sum(nf)), "fp64_points": int(sum(n64))}
     log(f"batching: graph {ms_graph:.2f} ms, eager {ms_eager:.2f} ms per batch, {out['value']:.4g} pts/s")
@@ -450,8 +484,10 @@ def main():
                 "residual": max(rel_err(a, b, nxp) for a, b in zip(m.data, ref.data))}
 
     # accuracy: one step of each form from the same inputs
-    acc, sweep = {}, []
+    acc, sweep, graph_tune = {}, [], {}
     if f64 is not None:
+        graph_tune["f64"] = f64.use_graph()
+        f64.reset()
         f64.step(counts=True)
         torch.cuda.synchronize()
         if args.sweep:
@@ -469,6 +505,9 @@ def main():
             mixed.post.close()
             mixed.post = Posterior(ctx, params, tabs)
         mixed.reset()
+    graph_tune["mixed"] = mixed.use_graph()
+    log(f"one-volume graph launch choices (points/block, workgroups, form, posterior ms): {graph_tune}")
+    mixed.reset()
     nfail, n64 = mixed.step(counts=True)
     torch.cuda.synchronize()
     log(f"mixed step: {nfail} failed factorisations, {n64} of {n_in} in-band points in FP64")
@@ -488,12 +527,25 @@ def main():
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / steps
 
-    ms_mixed = run(mixed, args.steps, args.warmup) * 1e3
+    # the step as a one-volume batch replayed as a hipGraph (two transform
+    # streams), then as separate launches on one stream: `value` is the faster
+    ms_mixed_graph = run(mixed, args.steps, args.warmup) * 1e3
+    log(f"mixed, one-volume graph: {ms_mixed_graph:.3f} ms/step")
+    batch_keep, mixed.batch = mixed.batch, None
+    ms_mixed_direct = run(mixed, args.steps, args.warmup) * 1e3
+    log(f"mixed, one stream: {ms_mixed_direct:.3f} ms/step")
+    if ms_mixed_graph < ms_mixed_direct:
+        mixed.batch = batch_keep
+    ms_mixed = min(ms_mixed_graph, ms_mixed_direct)
     log(f"mixed: {ms_mixed:.3f} ms/step, {P/ms_mixed*1e3:.4g} pts/s")
     ms_f64 = None
     if f64 is not None:
-        ms_f64 = run(f64, max(1, args.steps), 1) * 1e3
-        log(f"fp64: {ms_f64:.3f} ms/step, {P/ms_f64*1e3:.4g} pts/s")
+        ms_f64_graph = run(f64, max(1, args.steps), 1) * 1e3
+        f64.batch = None
+        ms_f64_direct = run(f64, max(1, args.steps), 1) * 1e3
+        ms_f64 = min(ms_f64_graph, ms_f64_direct)
+        log(f"fp64: {ms_f64:.3f} ms/step (graph {ms_f64_graph:.3f}, one stream {ms_f64_direct:.3f}), "
+            f"{P/ms_f64*1e3:.4g} pts/s")
 
     # per-kernel timing (HIP events on the launch stream), on the launches the
     # step makes: each forward pass over the set of nt data + 3 mean grids
@@ -508,15 +560,33 @@ def main():
     mixed.reset()
     ctx.r2c_many(mixed.data + mixed.mu, *n3, L.DATA)
     ctx.r2c_many(mixed.cov + [mixed.ec], *n3, L.COVARIANCE, L.MIXED)
+
+    def restorer(pipe):
+        """The posterior works in place: every timed launch starts from the
+        same Fourier-domain prior and data (a launch on its own outputs
+        would see the posterior covariance and take fewer FP64 points)."""
+        grids = pipe.mu + pipe.data + pipe.cov
+        saved = [g.clone() for g in grids]
+
+        def setup():
+            for d, s_ in zip(grids, saved):
+                d.copy_(s_)
+        return setup
+
+    setup32 = restorer(mixed)
     t_post = timed_kernel(lambda: mixed.post.run(L.MIXED, mixed.mu, mixed.data, mixed.cov, mixed.ec),
-                          args.kernel_reps)
+                          args.kernel_reps, setup32)
     t_post32 = timed_kernel(lambda: mixed.post.run(L.F32, mixed.mu, mixed.data, mixed.cov, mixed.ec),
-                            args.kernel_reps)
+                            args.kernel_reps, setup32)
+    del setup32
     t_post64 = None
     if f64 is not None:
         ctx.r2c_many(f64.data + f64.mu, *n3, L.DATA)
         ctx.r2c_many(f64.cov + [f64.ec], *n3, L.COVARIANCE)
-        t_post64 = timed_kernel(lambda: f64.post.run(L.F64, f64.mu, f64.data, f64.cov, f64.ec), args.kernel_reps)
+        setup64 = restorer(f64)
+        t_post64 = timed_kernel(lambda: f64.post.run(L.F64, f64.mu, f64.data, f64.cov, f64.ec), args.kernel_reps,
+                                setup64)
+        del setup64
     post_bytes = n_in * 8 * (19 + 2 * nt) + (P - n_in) * 96
     kernels = {
         "fft_x_r2c": {"ms": t_x, "alg_bytes": real_bytes + grid_c_bytes},
@@ -572,6 +642,9 @@ def main():
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"posterior path {nxp}x{nyp}x{nzp} FFT grid, ntheta={nt}", "fourier_points": P,
                    "in_band_points": n_in, "transforms_per_step": 31, "parallelism": "single GPU"},
+        "ms_per_step_graph": ms_mixed_graph,
+        "ms_per_step_one_stream": ms_mixed_direct,
+        "graph_launch": graph_tune,
         "fp64_value": (P / (ms_f64 * 1e-3)) if ms_f64 else None,
         "fp64_ms_per_step": ms_f64,
         "speedup_vs_fp64": (ms_f64 / ms_mixed) if ms_f64 else None,

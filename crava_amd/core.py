@@ -223,15 +223,18 @@ class Batch:
         except Exception:
             pass
 
+    # posterior launch forms (crava_hip_batch_set_launch)
+    FORM_SINGLE, FORM_PER_CLASS, FORM_PER_VOLUME = 0, 1, 2
+
     def autotune(self):
-        """Returns (points_per_block, blocks, per_class, posterior ms)."""
+        """Returns (points_per_block, blocks, form, posterior ms)."""
         p, b, c, ms = C.c_int(0), C.c_int(0), C.c_int(0), C.c_float(0)
         check(lib().crava_hip_batch_autotune(self.h, C.byref(p), C.byref(b), C.byref(c), C.byref(ms)),
               "crava_hip_batch_autotune")
-        return p.value, b.value, bool(c.value), ms.value
+        return p.value, b.value, c.value, ms.value
 
-    def set_launch(self, points_per_block: int, blocks: int, per_class: bool = True):
-        check(lib().crava_hip_batch_set_launch(self.h, points_per_block, blocks, int(per_class)),
+    def set_launch(self, points_per_block: int, blocks: int, form: int = 1):
+        check(lib().crava_hip_batch_set_launch(self.h, points_per_block, blocks, int(form)),
               "crava_hip_batch_set_launch")
 
     def run(self, graph: bool = True):

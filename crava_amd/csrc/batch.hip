@@ -4,8 +4,12 @@
 // Crava::computePostMeanResidAndFFTCov (src/crava.cpp:887-1204).
 //
 // Volumes are spread over a few side streams forked from the origin stream,
-// so small volumes overlap; volumes of one grid shape share a transform plan
-// (and its double scratch) and therefore always go to the same stream.
+// so small volumes overlap.  Within a volume the data/mean transforms and the
+// covariance transforms go to two different streams: in the mixed form the
+// covariance transforms run double arithmetic (latency-bound) while the float
+// data transforms stream HBM, so the two overlap.  Volumes of one grid shape
+// share transform plans (and the double scratch), so each kind of set of a
+// shape always goes to the same stream.
 #include "batch.hpp"
 #include "../../include/crava_hip.h"
 
@@ -152,7 +156,10 @@ int BatchExec::build_sets() {
   for (auto &g : groups) {
     const int s = (int)(std::min_element(load, load + kStreams) - load);
     load[s] += g.first;
-    for (int v : g.second) work_[v].stream = s;
+    for (int v : g.second) {
+      work_[v].stream[0] = s;
+      work_[v].stream[1] = (s + kStreams / 2) % kStreams;
+    }
   }
   return CRAVA_HIP_OK;
 }
@@ -179,26 +186,36 @@ int BatchExec::enqueue(hipStream_t origin, bool transforms_only, bool forward_on
     CRAVA_HIP_CHECK(hipGetLastError());
   }
   if ((rc = fork(origin))) return rc;
-  for (int v = 0; v < n; ++v) {
-    hipStream_t s = side_[work_[v].stream];
-    for (const Set &st : work_[v].fwd)
-      if ((rc = st.plan->r2c(st.grids, st.ng, st.scale, s))) return rc;
-  }
+  for (int v = 0; v < n; ++v)
+    for (int i = 0; i < 2; ++i) {
+      const Set &st = work_[v].fwd[i];
+      if ((rc = st.plan->r2c(st.grids, st.ng, st.scale, side_[work_[v].stream[i]]))) return rc;
+    }
   if ((rc = join(origin))) return rc;
   if (forward_only) return CRAVA_HIP_OK;
   if (!transforms_only && (rc = launch_post(origin))) return rc;
   if ((rc = fork(origin))) return rc;
-  for (int v = 0; v < n; ++v) {
-    hipStream_t s = side_[work_[v].stream];
-    for (const Set &st : work_[v].inv)
-      if ((rc = st.plan->c2r(st.grids, st.ng, st.scale, s))) return rc;
-  }
+  for (int v = 0; v < n; ++v)
+    for (int i = 0; i < 2; ++i) {
+      const Set &st = work_[v].inv[i];
+      if ((rc = st.plan->c2r(st.grids, st.ng, st.scale, side_[work_[v].stream[i]]))) return rc;
+    }
   return join(origin);
 }
 
 int BatchExec::launch_post(hipStream_t origin) {
-  if (!post_.per_class || post_.num_classes() < 2) return post_.launch(&origin, 1);
   int rc;
+  if (post_.form == kFormPerVolume) {
+    if ((rc = fork(origin))) return rc;
+    for (size_t v = 0; v < vols_.size(); ++v) {
+      const BatchVolume &bv = vols_[v];
+      if ((rc = launch_posterior(*bv.pd, precision_, bv.mu, bv.data, bv.cov, bv.errcorr, counters_ + 2 * v,
+                                 side_[work_[v].stream[0]])))
+        return rc;
+    }
+    return join(origin);
+  }
+  if (post_.form == kFormSingle || post_.num_classes() < 2) return post_.launch(&origin, 1);
   if ((rc = fork(origin))) return rc;
   if ((rc = post_.launch(side_, kStreams))) return rc;
   return join(origin);
@@ -231,10 +248,14 @@ int BatchExec::run(hipStream_t s, bool graph) {
   return CRAVA_HIP_OK;
 }
 
-int BatchExec::set_launch(int ppb, int blocks, bool per_class) {
+int BatchExec::set_launch(int ppb, int blocks, int form) {
+  if (form < kFormSingle || form > kFormPerVolume) {
+    set_error("posterior launch form must be 0, 1 or 2");
+    return CRAVA_HIP_E_INVALID;
+  }
   const int rc = post_.set_launch(ppb, blocks);
   if (rc == CRAVA_HIP_OK) {
-    post_.per_class = per_class;
+    post_.form = form;
     drop_graph();
   }
   return rc;
@@ -270,7 +291,7 @@ int BatchExec::backup_grids(hipStream_t s, bool restore) {
   return CRAVA_HIP_OK;
 }
 
-int BatchExec::autotune(hipStream_t s, int *ppb_out, int *blocks_out, int *per_class_out, float *ms_out) {
+int BatchExec::autotune(hipStream_t s, int *ppb_out, int *blocks_out, int *form_out, float *ms_out) {
   int rc;
   drop_graph();
   if ((rc = backup_grids(s, false))) return rc;
@@ -284,21 +305,22 @@ int BatchExec::autotune(hipStream_t s, int *ppb_out, int *blocks_out, int *per_c
   for (auto &kv : tune)
     if ((rc = kv.first->autotune(kv.second->grids, kv.second->ng, s, 3, nullptr))) return rc;
 
-  // posterior launch: one launch or one per ntheta class x points per block x
-  // persistent workgroups
+  // posterior launch: form x points per block x persistent workgroups (the
+  // per-volume form has one geometry)
   hipEvent_t e0, e1;
   CRAVA_HIP_CHECK(hipEventCreate(&e0));
   CRAVA_HIP_CHECK(hipEventCreate(&e1));
   const int ppbs[3] = {64, 128, 256};
   const int blks[5] = {0, 1024, 2048, 4096, 8192};
   float best = 1e30f;
-  int bp = 256, bb = 0;
-  bool bc = true, warm = false;
-  for (int m = 0; m < 2 && rc == CRAVA_HIP_OK; ++m)
+  int bp = 256, bb = 0, bf = kFormPerClass;
+  bool warm = false;
+  for (int m = 0; m < 3 && rc == CRAVA_HIP_OK; ++m)
     for (int a = 0; a < 3 && rc == CRAVA_HIP_OK; ++a)
       for (int b = 0; b < 5 && rc == CRAVA_HIP_OK; ++b) {
+        if (m == kFormPerVolume && (a > 0 || b > 0)) continue;
         if ((rc = post_.set_launch(ppbs[a], blks[b]))) break;
-        post_.per_class = m == 0;
+        post_.form = m;
         for (int rep = warm ? 0 : 1; rep >= 0 && rc == CRAVA_HIP_OK; --rep) {
           if ((rc = backup_grids(s, true))) break;
           if ((rc = enqueue(s, true, true))) break;
@@ -308,7 +330,7 @@ int BatchExec::autotune(hipStream_t s, int *ppb_out, int *blocks_out, int *per_c
           if (rep == 0) {
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best) { best = ms; bp = ppbs[a]; bb = blks[b]; bc = m == 0; }
+            if (ms < best) { best = ms; bp = ppbs[a]; bb = blks[b]; bf = m; }
           }
         }
         warm = true;
@@ -316,7 +338,7 @@ int BatchExec::autotune(hipStream_t s, int *ppb_out, int *blocks_out, int *per_c
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  if ((rc = set_launch(bp, bb, bc))) return rc;
+  if ((rc = set_launch(bp, bb, bf))) return rc;
   if ((rc = backup_grids(s, true))) return rc;
   CRAVA_HIP_CHECK(hipStreamSynchronize(s));
   if (backup_) (void)hipFree(backup_);
@@ -324,7 +346,7 @@ int BatchExec::autotune(hipStream_t s, int *ppb_out, int *blocks_out, int *per_c
   backup_bytes_ = 0;
   if (ppb_out) *ppb_out = bp;
   if (blocks_out) *blocks_out = bb;
-  if (per_class_out) *per_class_out = bc ? 1 : 0;
+  if (form_out) *form_out = bf;
   if (ms_out) *ms_out = best;
   set_error("");
   return CRAVA_HIP_OK;

@@ -28,11 +28,11 @@ class BatchExec {
   int init(const BatchVolume *vols, int n);
   // One pass of the whole sequence on `s`; graph: capture on first use, replay after.
   int run(hipStream_t s, bool graph);
-  // Choose FFT variants per grid shape and the posterior launch (one launch
-  // or one per ntheta class x points per block x workgroups) by timing; the
-  // registered grids are restored.
-  int autotune(hipStream_t s, int *ppb, int *blocks, int *per_class, float *post_ms);
-  int set_launch(int ppb, int blocks, bool per_class);
+  // Choose FFT variants per grid shape and the posterior launch (form x
+  // points per block x workgroups, see PostForm) by timing; the registered
+  // grids are restored.
+  int autotune(hipStream_t s, int *ppb, int *blocks, int *form, float *post_ms);
+  int set_launch(int ppb, int blocks, int form);
   // Per-volume counters of the last run (synchronises `s`).
   int counts(hipStream_t s, long long *n_fail, long long *n_fp64);
   int nvol() const { return (int)vols_.size(); }
@@ -45,8 +45,8 @@ class BatchExec {
     double scale[kMaxGrids] = {0};
   };
   struct Work {
-    Set fwd[2], inv[2];
-    int stream = 0;
+    Set fwd[2], inv[2];   // [0] data + means, [1] covariances (+ error correlation)
+    int stream[2] = {0, 0};
   };
   int build_sets();
   int enqueue(hipStream_t origin, bool transforms_only, bool forward_only);

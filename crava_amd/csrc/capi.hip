@@ -528,15 +528,15 @@ int crava_hip_batch_create(crava_hip_ctx *ctx, int precision, int nvol, const cr
   return CRAVA_HIP_OK;
 }
 
-int crava_hip_batch_autotune(crava_hip_batch *batch, int *points_per_block, int *blocks, int *per_class,
+int crava_hip_batch_autotune(crava_hip_batch *batch, int *points_per_block, int *blocks, int *form,
                              float *post_ms) {
   if (!batch) { set_error("batch is NULL"); return CRAVA_HIP_E_INVALID; }
-  return batch->exec.autotune(batch->ctx->stream, points_per_block, blocks, per_class, post_ms);
+  return batch->exec.autotune(batch->ctx->stream, points_per_block, blocks, form, post_ms);
 }
 
-int crava_hip_batch_set_launch(crava_hip_batch *batch, int points_per_block, int blocks, int per_class) {
+int crava_hip_batch_set_launch(crava_hip_batch *batch, int points_per_block, int blocks, int form) {
   if (!batch) { set_error("batch is NULL"); return CRAVA_HIP_E_INVALID; }
-  return batch->exec.set_launch(points_per_block, blocks, per_class != 0);
+  return batch->exec.set_launch(points_per_block, blocks, form);
 }
 
 int crava_hip_batch_run(crava_hip_batch *batch, int use_graph) {

@@ -531,7 +531,7 @@ static int launch_classes(const PostBatch &b, const hipStream_t *streams, int ns
 
 int PostBatch::launch(const hipStream_t *streams, int nstreams) const {
   if (ntiles < 1) return CRAVA_HIP_OK;
-  if (per_class) {
+  if (form == kFormPerClass) {
     if (precision == CRAVA_HIP_F64) return launch_classes<double, 0>(*this, streams, nstreams);
     if (precision == CRAVA_HIP_MIXED) return launch_classes<float, 1>(*this, streams, nstreams);
     return launch_classes<float, 0>(*this, streams, nstreams);

@@ -64,15 +64,19 @@ struct PostVolume {
   unsigned long long *counters = nullptr;  // device [2]: failed Cholesky, FP64 points
 };
 
+// Launch forms of a batched posterior: one persistent launch over every
+// tile (registers of the largest ntheta), one persistent launch per ntheta
+// class, or one launch of the single-volume kernel per volume.
+enum PostForm { kFormSingle = 0, kFormPerClass = 1, kFormPerVolume = 2 };
+
 // The posterior of many volumes (uneven sizes and ntheta) over a tile list:
 // ppb points of one k-plane per tile, `blocks` persistent workgroups
-// striding over the tiles (0: one workgroup per tile).  per_class = false:
-// one launch over every tile, sized for the largest ntheta; true: one launch
-// per ntheta class (tiles are ordered by ntheta), each with the registers
-// of its own ntheta, spread over the given streams.
+// striding over the tiles (0: one workgroup per tile).  Tiles are ordered by
+// ntheta, so each class is one contiguous segment.  launch() serves the
+// first two forms; the per-volume form is launched by the batch executor.
 struct PostBatch {
   int precision = 0, nvol = 0, ppb = 0, blocks = 0;
-  bool per_class = true;
+  int form = kFormPerClass;
   void *args = nullptr;       // device PostArgs per volume
   int4 *tiles = nullptr;      // device {volume, k, first point, ntheta}
   long long ntiles = 0;

@@ -302,19 +302,20 @@ int crava_hip_batch_create(crava_hip_ctx *ctx, int precision, int nvol, const cr
                            crava_hip_batch **batch);
 
 /* Time the transform variants of every grid shape of the batch and the
- * posterior launch and keep the fastest.  Posterior launch forms: one
- * persistent launch over every volume (registers of the largest ntheta), or
- * one persistent launch per ntheta class running concurrently; each with
- * 64/128/256 points per block and 0 (one workgroup per tile) or
- * 1024..8192 workgroups.  The registered grids' contents are saved to a
- * temporary device buffer and restored.  Outputs (host, may be NULL): the
- * chosen points per block, workgroups, per-class flag and posterior time. */
-int crava_hip_batch_autotune(crava_hip_batch *batch, int *points_per_block, int *blocks, int *per_class,
+ * posterior launch and keep the fastest.  Posterior launch forms:
+ *   0  one persistent launch over every volume (registers of the largest ntheta),
+ *   1  one persistent launch per ntheta class, the classes running concurrently,
+ *   2  one launch of the single-volume kernel per volume, on the volumes' streams;
+ * forms 0 and 1 with 64/128/256 points per block and 0 (one workgroup per
+ * tile) or 1024..8192 workgroups.  The registered grids' contents are saved
+ * to a temporary device buffer and restored.  Outputs (host, may be NULL):
+ * the chosen points per block, workgroups, form and posterior time. */
+int crava_hip_batch_autotune(crava_hip_batch *batch, int *points_per_block, int *blocks, int *form,
                              float *post_ms);
 
 /* Set the posterior launch directly (points_per_block 64, 128 or 256;
- * blocks 0 = one workgroup per tile; per_class 0 = one launch). */
-int crava_hip_batch_set_launch(crava_hip_batch *batch, int points_per_block, int blocks, int per_class);
+ * blocks 0 = one workgroup per tile; form 0, 1 or 2 as above). */
+int crava_hip_batch_set_launch(crava_hip_batch *batch, int points_per_block, int blocks, int form);
 
 /* Run the whole batch on the context's stream, asynchronously.  use_graph:
  * 0 enqueues the launches directly; 1 captures them into a hipGraph on first

@@ -7,9 +7,11 @@ Checks:
     oracle's forward FFT + float posterior loop + inverse FFT per volume;
   * the batch against the single-volume C-ABI path at the batching
     workload's sizes, for every posterior launch form: the transforms are the
-    same kernels (bit for bit), the posterior is the same per-point code
-    compiled into another kernel, where the compiler may fuse multiply-adds
-    differently (observed: ulp-level differences at ntheta >= 6 only);
+    same kernels (bit for bit); the per-volume form runs the single-volume
+    posterior kernel (bit for bit); the persistent forms run the same
+    per-point code compiled into another kernel, where the compiler may fuse
+    multiply-adds differently (observed: ulp-level differences at ntheta >= 6
+    only);
   * graph replays, each from freshly reloaded inputs with eager work and
     synchronisations between them, bit for bit against the eager run of the
     same launch form;
@@ -129,18 +131,21 @@ def test_batch_matches_single_volume_path_and_graph_replays(ctx, precision):
         a, r = a.double(), r.double()
         return float(torch.linalg.vector_norm(a - r) / torch.linalg.vector_norm(r))
 
-    forms = [(256, 0, True), (64, 512, True), (128, 2048, False)]
-    for ppb, blocks, per_class in forms:
-        b.set_launch(ppb, blocks, per_class)
+    forms = [(256, 0, Batch.FORM_PER_CLASS), (64, 512, Batch.FORM_PER_CLASS), (128, 2048, Batch.FORM_SINGLE),
+             (256, 0, Batch.FORM_PER_VOLUME)]
+    for ppb, blocks, form in forms:
+        b.set_launch(ppb, blocks, form)
         b.run(graph=False)
         torch.cuda.synchronize()
         eager = [v.outputs() for v in vols]
         counts = b.counts()
         for i, v in enumerate(vols):
-            assert (counts[0][i], counts[1][i]) == ref_counts[i], (ppb, blocks, per_class, i)
+            assert (counts[0][i], counts[1][i]) == ref_counts[i], (ppb, blocks, form, i)
             for a, r in zip(eager[i], ref[i]):
+                if form == Batch.FORM_PER_VOLUME:     # the single-volume kernel itself
+                    assert torch.equal(a, r), (form, i)
                 # rounding level: a dropped term or a wrong index is O(1e-2) or more
-                assert rel(a, r) < (1e-12 if precision == L.F64 else 1e-6), (ppb, blocks, per_class, i)
+                assert rel(a, r) < (1e-12 if precision == L.F64 else 1e-6), (ppb, blocks, form, i)
         for rep in range(2):
             for v in vols:
                 v.reload()
@@ -149,7 +154,7 @@ def test_batch_matches_single_volume_path_and_graph_replays(ctx, precision):
             torch.cuda.synchronize()
             for i, v in enumerate(vols):
                 for a, r in zip(v.outputs(), eager[i]):
-                    assert torch.equal(a, r), (ppb, blocks, per_class, rep, i)
+                    assert torch.equal(a, r), (ppb, blocks, form, rep, i)
             assert b.counts() == counts
             _ = torch.randn(1000, device="cuda").sum().item()   # eager work between replays
         for v in vols:
@@ -166,9 +171,9 @@ def test_batch_autotune_restores_grids(ctx):
     vols = [_Vol(ctx, p, 0.1) for p in probs]
     b = Batch(ctx, L.MIXED, [v.entry() for v in vols])
     before = [[g.clone() for g in v.grids] for v in vols]
-    ppb, blocks, per_class, ms = b.autotune()
+    ppb, blocks, form, ms = b.autotune()
     torch.cuda.synchronize()
-    assert ppb in (64, 128, 256) and blocks >= 0 and ms > 0
+    assert ppb in (64, 128, 256) and blocks >= 0 and form in (0, 1, 2) and ms > 0
     for v, bf in zip(vols, before):
         for a, r in zip(v.grids, bf):
             assert torch.equal(a, r)
@@ -188,8 +193,10 @@ def test_batch_rejects_bad_launch(ctx):
     v = _Vol(ctx, probs[0], 0.1)
     b = Batch(ctx, L.F32, [v.entry()])
     with pytest.raises(L.CravaHipError):
-        b.set_launch(100, 0, True)
+        b.set_launch(100, 0, 1)
     with pytest.raises(L.CravaHipError):
-        b.set_launch(64, -1, False)
+        b.set_launch(64, -1, 0)
+    with pytest.raises(L.CravaHipError):
+        b.set_launch(64, 0, 3)
     b.close()
     v.post.close()
