@@ -38,8 +38,9 @@ PEAK_HBM_GBS = 8000.0
 
 # Launch geometry of the specialised transform kernels' variants
 # (pencils or row pairs per workgroup, threads), as in fft_static.hip.
-STRIDED_VARIANTS = [(16, 256), (16, 512), (8, 256), (32, 512)]
-ROW_VARIANTS = [(8, 256), (16, 512), (4, 256)]
+# (pencils or row pairs per workgroup, threads, direct global I/O in the first/last pass)
+STRIDED_VARIANTS = [(tx, th, dio) for dio in ("false", "true") for tx, th in ((16, 256), (16, 512), (8, 256), (32, 512))]
+ROW_VARIANTS = [(tx, th, dio) for dio in ("false", "true") for tx, th in ((8, 256), (16, 512), (4, 256))]
 
 
 def kernel_identity(key, variants, nxp, nyp, nzp, ngrid, nt):
@@ -49,17 +50,17 @@ def kernel_identity(key, variants, nxp, nyp, nzp, ngrid, nt):
     if key == "fft_x_r2c":
         if vx < 0:
             return "crava::k_fft_rows_r2c<float, float, float>", None
-        tx, th = ROW_VARIANTS[vx]
+        tx, th, dio = ROW_VARIANTS[vx]
         rows = nyp * nzp
-        return (f"crava::k_srows_r2c<{nxp}, {tx}, {th}, float, float, float>",
+        return (f"crava::k_srows_r2c<{nxp}, {tx}, {th}, {dio}, float, float, float>",
                 -(-rows // (2 * tx)) * ngrid * th)
     if key in ("fft_y", "fft_z"):
         n, v = (nyp, vy) if key == "fft_y" else (nzp, vz)
         ncols, nbatch = (cn, nzp) if key == "fft_y" else (cn * nyp, 1)
         if v < 0:
             return "crava::k_fft_strided<-1, float, float, float>", None
-        tx, th = STRIDED_VARIANTS[v]
-        return (f"crava::k_sfft_strided<{n}, {tx}, {th}, -1, float, float, float>",
+        tx, th, dio = STRIDED_VARIANTS[v]
+        return (f"crava::k_sfft_strided<{n}, {tx}, {th}, -1, {dio}, float, float, float>",
                 -(-ncols // tx) * nbatch * ngrid * th)
     mode = {"posterior_mixed": ("float", 1), "posterior_f32": ("float", 0), "posterior_f64": ("double", 0)}[key]
     return f"crava::k_posterior<{mode[0]}, {nt}, {mode[1]}>", -(-(cn * nyp) // 256) * 256 * nzp

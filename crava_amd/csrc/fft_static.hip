@@ -12,6 +12,7 @@
 #include "fft_static.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace crava {
 
@@ -41,53 +42,90 @@ struct SP {
   }
 };
 
+// Marks a pass that reads or writes its pencils in LDS (no global hook).
+struct LdsIo {};
+
+template <typename F>
+constexpr bool kLds = std::is_same<F, LdsIo>::value;
+
+// Pass P of the Stockham sequence over TX pencils of length N.
+//   inputs:  element j + r*NB of pencil c, from LDS or from ld(c, e);
+//   outputs: element (j / Ns) * Ns * R + k + r*Ns, to LDS or to st(c, e, v).
+// CF picks the lane mapping: pencil-fastest (c = b % TX, for columns that
+// sit side by side in global memory) or element-fastest (c = b / NB).
+// A pass whose inputs come from global memory needs no barrier before its
+// LDS writes (nothing else reads LDS then); a pass writing global memory
+// needs none after.
+template <int N, int TX, int THREADS, int DIR, int P, bool CF, typename T, typename Ld, typename St>
+__device__ __forceinline__ void spass(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw, const Ld &ld,
+                                      const St &st) {
+  constexpr int R = SP<N, T>::rl.r[P];
+  constexpr int Ns = SP<N, T>::ns(P);
+  constexpr int NB = N / R;
+  constexpr int TOT = TX * NB;
+  constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
+  constexpr int PITCH = SP<N, T>::pitch;
+  cx<T> v[TRIPS][R];
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int b = (int)threadIdx.x + t * THREADS;
+    if (TOT % THREADS == 0 || b < TOT) {
+      const int c = CF ? b % TX : b / NB, j = CF ? b / TX : b % NB;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (kLds<Ld>) v[t][r] = buf[c * PITCH + padx<T>(j + r * NB)];
+        else v[t][r] = ld(c, j + r * NB);
+      }
+      if constexpr (Ns > 1) {
+        // per-pass table [r-1][k]
+        const int k = j % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cx<T> w = tw[SP<N, T>::off(P) + (r - 1) * Ns + k];
+          if (DIR > 0) w.y = -w.y;
+          v[t][r] = mul(v[t][r], w);
+        }
+      }
+      Dft<R, DIR, T>::run(v[t]);
+    }
+  }
+  if constexpr (kLds<Ld> && kLds<St>) __syncthreads();
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int b = (int)threadIdx.x + t * THREADS;
+    if (TOT % THREADS == 0 || b < TOT) {
+      const int c = CF ? b % TX : b / NB, j = CF ? b / TX : b % NB;
+      const int k = j % Ns;
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (kLds<St>) buf[c * PITCH + padx<T>(base + r * Ns)] = v[t][r];
+        else st(c, base + r * Ns, v[t][r]);
+      }
+    }
+  }
+  if constexpr (kLds<St>) __syncthreads();
+}
+
+// All passes.  The first pass takes its inputs from ld and the last writes
+// through st (either may be LdsIo); CF0 / CFL are their lane mappings, the
+// passes between are element-fastest.
+template <int N, int TX, int THREADS, int DIR, int P, bool CF0, bool CFL, typename T, typename Ld, typename St>
+__device__ __forceinline__ void spasses_io(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw, const Ld &ld,
+                                           const St &st) {
+  constexpr int CNT = SP<N, T>::rl.cnt;
+  static_assert(CNT >= 2, "the specialised lengths have at least two passes");
+  if constexpr (P < CNT) {
+    if constexpr (P == 0) spass<N, TX, THREADS, DIR, P, CF0, T>(buf, tw, ld, LdsIo{});
+    else if constexpr (P == CNT - 1) spass<N, TX, THREADS, DIR, P, CFL, T>(buf, tw, LdsIo{}, st);
+    else spass<N, TX, THREADS, DIR, P, false, T>(buf, tw, LdsIo{}, LdsIo{});
+    spasses_io<N, TX, THREADS, DIR, P + 1, CF0, CFL, T>(buf, tw, ld, st);
+  }
+}
+
 template <int N, int TX, int THREADS, int DIR, int P, typename T>
 __device__ __forceinline__ void spasses(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw) {
-  if constexpr (P < SP<N, T>::rl.cnt) {
-    constexpr int R = SP<N, T>::rl.r[P];
-    constexpr int Ns = SP<N, T>::ns(P);
-    constexpr int NB = N / R;
-    constexpr int TOT = TX * NB;
-    constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
-    constexpr int PITCH = SP<N, T>::pitch;
-    cx<T> v[TRIPS][R];
-#pragma unroll
-    for (int t = 0; t < TRIPS; ++t) {
-      const int b = (int)threadIdx.x + t * THREADS;
-      if (TOT % THREADS == 0 || b < TOT) {
-        const int c = b / NB, j = b - (b / NB) * NB;
-        const cx<T> *src = buf + c * PITCH;
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[t][r] = src[padx<T>(j + r * NB)];
-        if constexpr (Ns > 1) {
-          // per-pass table [r-1][k]: the lanes of a wave read consecutive k
-          const int k = j % Ns;
-#pragma unroll
-          for (int r = 1; r < R; ++r) {
-            cx<T> w = tw[SP<N, T>::off(P) + (r - 1) * Ns + k];
-            if (DIR > 0) w.y = -w.y;
-            v[t][r] = mul(v[t][r], w);
-          }
-        }
-        Dft<R, DIR, T>::run(v[t]);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < TRIPS; ++t) {
-      const int b = (int)threadIdx.x + t * THREADS;
-      if (TOT % THREADS == 0 || b < TOT) {
-        const int c = b / NB, j = b - (b / NB) * NB;
-        const int k = j % Ns;
-        cx<T> *dst = buf + c * PITCH;
-        const int base = (j / Ns) * Ns * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) dst[padx<T>(base + r * Ns)] = v[t][r];
-      }
-    }
-    __syncthreads();
-    spasses<N, TX, THREADS, DIR, P + 1>(buf, tw);
-  }
+  spasses_io<N, TX, THREADS, DIR, P, false, false, T>(buf, tw, LdsIo{}, LdsIo{});
 }
 
 template <int N, typename T>
@@ -96,7 +134,11 @@ constexpr size_t lds_bytes(int tx) {
 }
 
 // Strided complex pass: columns contiguous, elements `es` apart.
-template <int N, int TX, int THREADS, int DIR, typename T, typename SI, typename SO>
+// DIO: the first pass loads its butterflies straight from global memory and
+// the last stores straight to it (pencil-fastest lanes, so each row segment
+// of TX columns is one coalesced access), which saves two LDS round trips;
+// otherwise the pencils are staged through LDS before and after the passes.
+template <int N, int TX, int THREADS, int DIR, bool DIO, typename T, typename SI, typename SO>
 __global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long es, long long ncols, long long bstride,
                                                           const cx<T> *__restrict__ twg) {
   // (column block, batch, grid) of this workgroup, XCD-contiguous
@@ -115,6 +157,20 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long 
   const cx<SI> *ibase = src + off;
   cx<SO> *obase = dst + off;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
+  if constexpr (DIO) {
+    const auto ld = [&](int c, int e) -> cx<T> {
+      if (c < ncol) {
+        const cx<SI> v = ibase[c + (long long)e * es];
+        return mk((T)v.x, (T)v.y);
+      }
+      return mk((T)0, (T)0);
+    };
+    const auto st = [&](int c, int e, cx<T> v) {
+      if (c < ncol) obase[c + (long long)e * es] = mk((SO)v.x, (SO)v.y);
+    };
+    spasses_io<N, TX, THREADS, DIR, 0, true, true, T>(buf, tw, ld, st);
+    return;
+  }
   constexpr int TOT = TX * N;
   constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
 #pragma unroll
@@ -144,7 +200,8 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long 
 }
 
 // x pass forward: real rows -> half spectra (two rows per complex pencil).
-template <int N, int TX, int THREADS, typename T, typename SI, typename SO>
+// DIO: the first pass loads rows 2c and 2c+1 straight from global memory.
+template <int N, int TX, int THREADS, bool DIO, typename T, typename SI, typename SO>
 __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nrows, int rnxp,
                                                        const cx<T> *__restrict__ twg) {
   // (row block, grid) of this workgroup, XCD-contiguous
@@ -160,27 +217,38 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nro
   const long long row0 = (long long)bx * 2 * TX;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
   static_assert(N % 2 == 0, "row kernels need an even length");
-  // lane (c, e2) reads the value pair e2 of rows 2c and 2c+1 and stores the
-  // two whole complex elements 2e2, 2e2+1 of pencil c (a = Re, b = Im)
-  constexpr int H = N / 2;
-  constexpr int TOT = TX * H;
-  constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
-#pragma unroll
-  for (int t = 0; t < TRIPS; ++t) {
-    const int q = (int)threadIdx.x + t * THREADS;
-    if (TOT % THREADS == 0 || q < TOT) {
-      const int c = q / H, e2 = q % H;
+  if constexpr (DIO) {
+    const auto ld = [&](int c, int e) -> cx<T> {
       const long long ra = row0 + 2 * c;
-      cx<SI> a = mk((SI)0, (SI)0), b = mk((SI)0, (SI)0);
-      if (ra < nrows) a = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[e2];
-      if (ra + 1 < nrows) b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[e2];
-      cx<T> *d = buf + c * PITCH;
-      d[padx<T>(2 * e2)] = mk((T)a.x, (T)b.x);
-      d[padx<T>(2 * e2 + 1)] = mk((T)a.y, (T)b.y);
+      SI a = (SI)0, b = (SI)0;
+      if (ra < nrows) a = src[ra * rnxp + e];
+      if (ra + 1 < nrows) b = src[(ra + 1) * rnxp + e];
+      return mk((T)a, (T)b);
+    };
+    spasses_io<N, TX, THREADS, -1, 0, false, false, T>(buf, tw, ld, LdsIo{});
+  } else {
+    // lane (c, e2) reads the value pair e2 of rows 2c and 2c+1 and stores the
+    // two whole complex elements 2e2, 2e2+1 of pencil c (a = Re, b = Im)
+    constexpr int H = N / 2;
+    constexpr int TOT = TX * H;
+    constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
+#pragma unroll
+    for (int t = 0; t < TRIPS; ++t) {
+      const int q = (int)threadIdx.x + t * THREADS;
+      if (TOT % THREADS == 0 || q < TOT) {
+        const int c = q / H, e2 = q % H;
+        const long long ra = row0 + 2 * c;
+        cx<SI> a = mk((SI)0, (SI)0), b = mk((SI)0, (SI)0);
+        if (ra < nrows) a = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[e2];
+        if (ra + 1 < nrows) b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[e2];
+        cx<T> *d = buf + c * PITCH;
+        d[padx<T>(2 * e2)] = mk((T)a.x, (T)b.x);
+        d[padx<T>(2 * e2 + 1)] = mk((T)a.y, (T)b.y);
+      }
     }
+    __syncthreads();
+    spasses<N, TX, THREADS, -1, 0>(buf, tw);
   }
-  __syncthreads();
-  spasses<N, TX, THREADS, -1, 0>(buf, tw);
   constexpr int CN = N / 2 + 1;
   constexpr int TOT2 = TX * CN;
   constexpr int TRIPS2 = (TOT2 + THREADS - 1) / THREADS;
@@ -202,7 +270,9 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nro
 }
 
 // x pass inverse: half spectra of two rows -> Z = A + iB -> inverse -> rows.
-template <int N, int TX, int THREADS, typename T, typename SI, typename SO>
+// DIO: the first pass builds Z straight from global memory and the last
+// stores the two rows straight to it.
+template <int N, int TX, int THREADS, bool DIO, typename T, typename SI, typename SO>
 __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nrows, int rnxp,
                                                        const cx<T> *__restrict__ twg) {
   // (row block, grid) of this workgroup, XCD-contiguous
@@ -218,6 +288,27 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nro
   constexpr int CN = N / 2 + 1;
   const long long row0 = (long long)bx * 2 * TX;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
+  if constexpr (DIO) {
+    const auto ld = [&](int c, int k) -> cx<T> {
+      const bool mirror = k >= CN;
+      const int kk = mirror ? N - k : k;
+      const bool selfc = (kk == 0) || (kk == N / 2);
+      const long long ra = row0 + 2 * c;
+      cx<SI> a = mk((SI)0, (SI)0), b = mk((SI)0, (SI)0);
+      if (ra < nrows) a = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[kk];
+      if (ra + 1 < nrows) b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[kk];
+      if (selfc) { a.y = (SI)0; b.y = (SI)0; }
+      if (mirror) { a.y = -a.y; b.y = -b.y; }
+      return mk((T)a.x - (T)b.y, (T)a.y + (T)b.x);
+    };
+    const auto st = [&](int c, int e, cx<T> v) {
+      const long long ra = row0 + 2 * c;
+      if (ra < nrows) dst[ra * rnxp + e] = (SO)(v.x * scale);
+      if (ra + 1 < nrows) dst[(ra + 1) * rnxp + e] = (SO)(v.y * scale);
+    };
+    spasses_io<N, TX, THREADS, 1, 0, false, false, T>(buf, tw, ld, st);
+    return;
+  }
   constexpr int TOT = TX * N;
   constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
 #pragma unroll
@@ -262,8 +353,11 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nro
 // ---------------------------------------------------------------------------
 // Variant tables.  Strided: (TX, THREADS); rows: (row pairs, THREADS).
 // ---------------------------------------------------------------------------
-constexpr int kSV[kNumStridedVariants][2] = {{16, 256}, {16, 512}, {8, 256}, {32, 512}};
-constexpr int kRV[kNumRowVariants][2] = {{8, 256}, {16, 512}, {4, 256}};
+// Variants v and v + half share a shape; the upper half loads the first
+// pass and stores the last straight from and to global memory.
+constexpr int kSV[kNumStridedVariants][2] = {{16, 256}, {16, 512}, {8, 256}, {32, 512},
+                                             {16, 256}, {16, 512}, {8, 256}, {32, 512}};
+constexpr int kRV[kNumRowVariants][2] = {{8, 256}, {16, 512}, {4, 256}, {8, 256}, {16, 512}, {4, 256}};
 
 template <typename F>
 static int set_lds_attr(F kernel, size_t bytes) {
@@ -277,11 +371,12 @@ template <int N, int V, int DIR, typename T, typename SI, typename SO>
 static int launch_strided_v(const GridSet &gs, long long es, long long ncols, long long nbatch, long long bstride,
                             const void *tw, hipStream_t s) {
   constexpr int TX = kSV[V][0], TH = kSV[V][1];
+  constexpr bool DIO = V >= kNumStridedVariants / 2;
   const size_t lds = lds_bytes<N, T>(TX);
   const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
   int rc;
-  if ((rc = set_lds_attr(k_sfft_strided<N, TX, TH, DIR, T, SI, SO>, lds))) return rc;
-  hipLaunchKernelGGL((k_sfft_strided<N, TX, TH, DIR, T, SI, SO>), dim3(gx, (unsigned)nbatch, (unsigned)gs.n), dim3(TH),
+  if ((rc = set_lds_attr(k_sfft_strided<N, TX, TH, DIR, DIO, T, SI, SO>, lds))) return rc;
+  hipLaunchKernelGGL((k_sfft_strided<N, TX, TH, DIR, DIO, T, SI, SO>), dim3(gx, (unsigned)nbatch, (unsigned)gs.n), dim3(TH),
                      lds, s, gs, es, ncols, bstride, reinterpret_cast<const cx<T> *>(tw));
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
@@ -290,17 +385,18 @@ static int launch_strided_v(const GridSet &gs, long long es, long long ncols, lo
 template <int N, int V, bool INV, typename T, typename SI, typename SO>
 static int launch_rows_v(const GridSet &gs, long long nrows, int rnxp, const void *tw, hipStream_t s) {
   constexpr int TX = kRV[V][0], TH = kRV[V][1];
+  constexpr bool DIO = V >= kNumRowVariants / 2;
   const size_t lds = lds_bytes<N, T>(TX);
   const unsigned gx = (unsigned)((nrows + 2LL * TX - 1) / (2LL * TX));
   const cx<T> *twp = reinterpret_cast<const cx<T> *>(tw);
   int rc;
   if constexpr (!INV) {
-    if ((rc = set_lds_attr(k_srows_r2c<N, TX, TH, T, SI, SO>, lds))) return rc;
-    hipLaunchKernelGGL((k_srows_r2c<N, TX, TH, T, SI, SO>), dim3(gx, 1, (unsigned)gs.n), dim3(TH), lds, s, gs, nrows,
+    if ((rc = set_lds_attr(k_srows_r2c<N, TX, TH, DIO, T, SI, SO>, lds))) return rc;
+    hipLaunchKernelGGL((k_srows_r2c<N, TX, TH, DIO, T, SI, SO>), dim3(gx, 1, (unsigned)gs.n), dim3(TH), lds, s, gs, nrows,
                        rnxp, twp);
   } else {
-    if ((rc = set_lds_attr(k_srows_c2r<N, TX, TH, T, SI, SO>, lds))) return rc;
-    hipLaunchKernelGGL((k_srows_c2r<N, TX, TH, T, SI, SO>), dim3(gx, 1, (unsigned)gs.n), dim3(TH), lds, s, gs, nrows,
+    if ((rc = set_lds_attr(k_srows_c2r<N, TX, TH, DIO, T, SI, SO>, lds))) return rc;
+    hipLaunchKernelGGL((k_srows_c2r<N, TX, TH, DIO, T, SI, SO>), dim3(gx, 1, (unsigned)gs.n), dim3(TH), lds, s, gs, nrows,
                        rnxp, twp);
   }
   CRAVA_HIP_CHECK(hipGetLastError());
@@ -314,7 +410,11 @@ static int strided_any(int v, const GridSet &gs, long long es, long long ncols, 
     case 0: return launch_strided_v<N, 0, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
     case 1: return launch_strided_v<N, 1, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
     case 2: return launch_strided_v<N, 2, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
-    default: return launch_strided_v<N, 3, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    case 3: return launch_strided_v<N, 3, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    case 4: return launch_strided_v<N, 4, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    case 5: return launch_strided_v<N, 5, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    case 6: return launch_strided_v<N, 6, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    default: return launch_strided_v<N, 7, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
   }
 }
 
@@ -323,7 +423,10 @@ static int rows_any(int v, const GridSet &gs, long long nrows, int rnxp, const v
   switch (v) {
     case 0: return launch_rows_v<N, 0, INV, T, SI, SO>(gs, nrows, rnxp, tw, s);
     case 1: return launch_rows_v<N, 1, INV, T, SI, SO>(gs, nrows, rnxp, tw, s);
-    default: return launch_rows_v<N, 2, INV, T, SI, SO>(gs, nrows, rnxp, tw, s);
+    case 2: return launch_rows_v<N, 2, INV, T, SI, SO>(gs, nrows, rnxp, tw, s);
+    case 3: return launch_rows_v<N, 3, INV, T, SI, SO>(gs, nrows, rnxp, tw, s);
+    case 4: return launch_rows_v<N, 4, INV, T, SI, SO>(gs, nrows, rnxp, tw, s);
+    default: return launch_rows_v<N, 5, INV, T, SI, SO>(gs, nrows, rnxp, tw, s);
   }
 }
 

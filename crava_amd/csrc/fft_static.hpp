@@ -9,8 +9,8 @@ namespace crava {
 // Kinds of pass.
 enum FftKind { kStridedFwd = 0, kStridedInv = 1, kRowsR2C = 2, kRowsC2R = 3 };
 
-constexpr int kNumStridedVariants = 4;
-constexpr int kNumRowVariants = 3;
+constexpr int kNumStridedVariants = 8;
+constexpr int kNumRowVariants = 6;
 
 struct StaticLaunch {
   int tx = 0;        // pencils (strided) or row pairs (rows) per workgroup

@@ -107,8 +107,11 @@ int crava_hip_fft_pass_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, i
  * transform on `scratch` (a grid of that size; its contents are destroyed)
  * and keep the fastest in the context's plan.  Variants: v >= 0 a
  * compile-time specialised kernel (lengths 64, 80, 128, 160, 216, 256, 288,
- * 320, 512), v = -1..-6 the runtime-length kernel with radix order
- * (large-first / small-first) x pencils per workgroup (<=64, <=16, <=8).
+ * 320, 512): 0..7 for the strided y/z passes, 0..5 for the x (row) pass, the
+ * upper half of each range loading the first radix pass straight from HBM
+ * and storing the last straight to it; v = -1..-6 the runtime-length kernel
+ * with radix order (large-first / small-first) x pencils per workgroup
+ * (<=64, <=16, <=8).
  * variants[3] / ms[3] (host, may be NULL) receive the choice per axis and its
  * time.  No reference counterpart (FFTW_ESTIMATE plans are not measured). */
 int crava_hip_autotune_fft(crava_hip_ctx *ctx, void *scratch, int nxp, int nyp, int nzp, int precision,
