@@ -54,6 +54,20 @@ class BatchVolume(C.Structure):
         ("errcorr", VP),
     ]
 
+class SyntParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("nxp", C.c_int), ("nyp", C.c_int), ("nzp", C.c_int),
+        ("ntheta", C.c_int),
+        ("dz", C.c_float),
+        ("A", VP),
+        ("wavelets", VP),
+        ("shift", VP),
+        ("gain", VP),
+        ("stretch", VP),
+    ]
+
+
 # name -> (restype, argtypes); every function declared in include/crava_hip.h
 SIGNATURES = {
     "crava_hip_version": (I, []),
@@ -98,6 +112,7 @@ SIGNATURES = {
     "crava_hip_batch_destroy": (I, [VP]),
     "crava_hip_autotune_fft": (I, [VP, VP, I, I, I, I, C.POINTER(I), C.POINTER(C.c_float)]),
     "crava_hip_fft_set_variants": (I, [VP, I, I, I, I, C.POINTER(I)]),
+    "crava_hip_synthetic_seismic": (I, [VP, C.POINTER(SyntParams), VP, VP, VP, C.POINTER(VP)]),
 }
 
 

@@ -38,6 +38,7 @@ SOURCES = {
     "nll.hip": [],
     "batch.hip": [],
     "grid.hip": ["-ffp-contract=off"],
+    "forward.hip": ["-ffp-contract=off"],
 }
 
 COMMON = [

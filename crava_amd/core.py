@@ -245,3 +245,33 @@ class Batch:
         nf, n64 = (C.c_longlong * n)(), (C.c_longlong * n)()
         check(lib().crava_hip_batch_counts(self.h, nf, n64), "crava_hip_batch_counts")
         return list(nf), list(n64)
+
+
+def synthetic_seismic(ctx: Context, nx, ny, nz, nxp, nyp, nzp, A, wavelets, dz, alpha, beta, rho, out,
+                      shift=None, gain=None, stretch=None):
+    """Forward modelling (crava_hip_synthetic_seismic) into the float grids
+    `out` (one per angle).  A [ntheta][3], wavelets [ntheta][nzp] and the
+    optional per-angle [ny][nx] tables (lists with None entries allowed) are
+    host arrays."""
+    import numpy as np
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    nt = A.shape[0]
+    wav = np.ascontiguousarray(wavelets, dtype=np.float32)
+    keep = []
+
+    def table(t):
+        if t is None:
+            return None
+        arr = (C.c_void_p * nt)()
+        for l, x in enumerate(t):
+            if x is not None:
+                x = np.ascontiguousarray(x, dtype=np.float32)
+                keep.append(x)
+                arr[l] = x.ctypes.data
+        keep.append(arr)
+        return C.cast(arr, C.c_void_p)
+
+    p = L.SyntParams(nx, ny, nz, nxp, nyp, nzp, nt, dz, A.ctypes.data, wav.ctypes.data, table(shift), table(gain),
+                   table(stretch))
+    check(lib().crava_hip_synthetic_seismic(ctx.h, C.byref(p), _dev_ptr(alpha), _dev_ptr(beta), _dev_ptr(rho),
+                                            ptr_array([_dev_ptr(g) for g in out])), "crava_hip_synthetic_seismic")

@@ -5,6 +5,7 @@
 #include "context.hpp"
 #include "fft.hpp"
 #include "facies.hpp"
+#include "forward.hpp"
 #include "grid.hpp"
 #include "nll.hpp"
 #include "posterior.hpp"
@@ -494,6 +495,90 @@ int crava_hip_facies(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int ny
   if (counts && nfac >= 1 && nfac <= crava::kMaxFacies)
     CRAVA_HIP_CHECK(hipMemsetAsync(counts, 0, sizeof(unsigned long long) * nfac, ctx->stream));
   return crava::launch_facies(c, ctx->stream);
+}
+
+// ---------------------------------------------------------------------------
+// Forward modelling
+int crava_hip_synthetic_seismic(crava_hip_ctx *ctx, const crava_hip_synt_params *p, const void *alpha,
+                                const void *beta, const void *rho, void *const *out) {
+  if (!ctx || !p || !alpha || !beta || !rho || !out || !p->A || !p->wavelets) {
+    set_error("NULL argument to crava_hip_synthetic_seismic");
+    return CRAVA_HIP_E_INVALID;
+  }
+  const int nt = p->ntheta;
+  if (nt < 1 || nt > crava::kFwdMaxTheta) { set_error("ntheta must be 1..8"); return CRAVA_HIP_E_INVALID; }
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1 || p->nx > p->nxp || p->ny > p->nyp || p->nz > p->nzp || p->nzp < 2) {
+    set_error("invalid volume / padded grid sizes");
+    return CRAVA_HIP_E_INVALID;
+  }
+  for (int l = 0; l < nt; ++l)
+    if (!out[l]) { set_error("NULL output grid"); return CRAVA_HIP_E_INVALID; }
+  bool any_shift = false;
+  for (int l = 0; l < nt; ++l) any_shift = any_shift || (p->shift && p->shift[l]);
+  if (p->stretch)
+    for (int l = 0; l < nt; ++l)
+      if (p->stretch[l])
+        for (size_t t = 0; t < (size_t)p->nx * p->ny; ++t)
+          if (!(p->stretch[l][t] > 0.0f) || !std::isfinite(p->stretch[l][t])) {
+            set_error("stretch factors must be positive and finite");
+            return CRAVA_HIP_E_INVALID;
+          }
+  if (any_shift && !(p->dz > 0.0f)) { set_error("dz must be positive when time shifts are given"); return CRAVA_HIP_E_INVALID; }
+  crava::Fft3D *f = nullptr;
+  int rc = ctx->plan(p->nxp, p->nyp, p->nzp, 0, &f);
+  if (rc) return rc;
+  // tables: wavelet half spectra, then the per-trace arrays that are given
+  const int nk = p->nzp / 2 + 1;
+  const size_t ntr = (size_t)p->nx * p->ny;
+  std::vector<float> host((size_t)nt * nk * 2);
+  for (int l = 0; l < nt; ++l)
+    crava::wavelet_half_spectrum(p->wavelets + (size_t)l * p->nzp, p->nzp, host.data() + (size_t)l * nk * 2);
+  const float *const *tabs[3] = {p->stretch, p->shift, p->gain};
+  size_t ntab = 0;
+  for (int t = 0; t < 3; ++t)
+    for (int l = 0; l < nt; ++l) ntab += (tabs[t] && tabs[t][l]) ? 1 : 0;
+  const size_t spec_bytes = ((host.size() * sizeof(float) + 255) / 256) * 256;
+  const size_t tab_bytes = ((ntr * sizeof(float) + 255) / 256) * 256;
+  void *ws = nullptr;
+  if ((rc = ctx->workspace(spec_bytes + ntab * tab_bytes, &ws))) return rc;
+  char *wp = static_cast<char *>(ws);
+  CRAVA_HIP_CHECK(hipMemcpyAsync(wp, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  crava::FwdArgs a{};
+  a.alpha = static_cast<const float *>(alpha);
+  a.beta = static_cast<const float *>(beta);
+  a.rho = static_cast<const float *>(rho);
+  a.W = reinterpret_cast<const crava::cx<float> *>(wp);
+  size_t off = spec_bytes;
+  const float **dst[3] = {a.sf, a.shift, a.gain};
+  for (int t = 0; t < 3; ++t)
+    for (int l = 0; l < nt; ++l) {
+      dst[t][l] = nullptr;
+      if (tabs[t] && tabs[t][l]) {
+        CRAVA_HIP_CHECK(hipMemcpyAsync(wp + off, tabs[t][l], ntr * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        dst[t][l] = reinterpret_cast<const float *>(wp + off);
+        off += tab_bytes;
+      }
+    }
+  // the host tables above go out of scope: finish the uploads first
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  for (int l = 0; l < nt; ++l) {
+    a.out[l] = static_cast<float *>(out[l]);
+    for (int c = 0; c < 3; ++c) a.A[l][c] = p->A[l * 3 + c];
+  }
+  a.dz = p->dz;
+  a.inv_n = (float)(1.0 / (double)p->nzp);
+  a.nt = nt;
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.nz = p->nz;
+  a.nyp = p->nyp;
+  a.nzp = p->nzp;
+  a.rnxp = 2 * (p->nxp / 2 + 1);
+  if ((rc = crava::launch_fm_reflect(a, ctx->stream))) return rc;
+  if ((rc = f->pass(2, out, nt, false, nullptr, ctx->stream))) return rc;
+  if ((rc = crava::launch_fm_filter(a, ctx->stream))) return rc;
+  if ((rc = f->pass(2, out, nt, true, nullptr, ctx->stream))) return rc;
+  return crava::launch_fm_padding(a, ctx->stream);
 }
 
 }  // extern "C"

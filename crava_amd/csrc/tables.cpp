@@ -49,6 +49,14 @@ c64 camp(const std::vector<c64> &s, int nzp, int k) {
 
 }  // namespace
 
+void wavelet_half_spectrum(const float *w, int nzp, float *re_im) {
+  const std::vector<c64> s = half_spectrum(w, nzp);
+  for (int k = 0; k < nzp / 2 + 1; ++k) {
+    re_im[2 * k] = s[k].re;
+    re_im[2 * k + 1] = s[k].im;
+  }
+}
+
 void perk_vectors(int ntheta, int nz, int nzp, double lz, float low_cut, float high_cut, float min_rel_thick,
                   const float *wavelets, const float *wnorm, unsigned char *invert, float *kWD, float *e1,
                   float *e2, float *e3) {

@@ -16,4 +16,10 @@ void perk_vectors(int ntheta, int nz, int nzp, double lz, float low_cut, float h
                   const float *wavelets, const float *wnorm, unsigned char *invert, float *kWD, float *e1,
                   float *e2, float *e3);
 
+// Half spectrum (nzp/2+1 {re, im} pairs) of a real series of nzp samples
+// with FFTW's forward sign and no normalisation (Wavelet::fft1DInPlace,
+// src/wavelet.cpp:246-261): double accumulation rounded to float, DC and
+// Nyquist bins exactly real.
+void wavelet_half_spectrum(const float *w, int nzp, float *re_im);
+
 }  // namespace crava

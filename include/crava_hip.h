@@ -331,6 +331,37 @@ int crava_hip_batch_counts(crava_hip_batch *batch, long long *n_fail, long long 
 
 int crava_hip_batch_destroy(crava_hip_batch *batch);
 
+/* ---- Forward modelling ------------------------------------------------- */
+
+/* Synthetic seismic per angle from the elastic parameter grids, in place of
+ * the trace loop of Crava::computeSyntSeismic (src/crava.cpp:1667-1759) with
+ * Crava::computeSeismicImpedance (:1632-1663) and the local wavelet of
+ * Wavelet1D::createLocalWavelet1D / shiftAndScale (src/wavelet1D.cpp:501-561)
+ * read through Wavelet::getCAmp(k, scale) (src/wavelet.cpp:328-384).
+ * Per trace (i < nx, j < ny): impedance A_l . (alpha, beta, rho) for k < nz,
+ * the linear taper to nzp, forward difference, r2c, times conj(w), c2r, /nzp.
+ * Padding traces receive the impedance, as the grid the reference writes
+ * into holds it there.  Per-trace arrays are [ny][nx] (i fastest); a NULL
+ * table or entry means shift 0, gain 1, stretch 1 for that angle.  All
+ * arrays are host memory. */
+typedef struct {
+  int nx, ny, nz;                 /* volume */
+  int nxp, nyp, nzp;              /* padded FFT grid */
+  int ntheta;                     /* 1..8 */
+  float dz;                       /* wavelet sampling interval, the unit of shift */
+  const float *A;                 /* [ntheta][3] reflection matrix rows (Crava::A_) */
+  const float *wavelets;          /* [ntheta][nzp] wavelets in FFT order (Wavelet::rAmp_) */
+  const float *const *shift;      /* [ntheta] local time shifts (Wavelet::getLocalTimeshift) */
+  const float *const *gain;       /* [ntheta] local gains (Wavelet::getLocalGainFactor) */
+  const float *const *stretch;    /* [ntheta] sf = relThick * localStretch (:1706) */
+} crava_hip_synt_params;
+
+/* alpha, beta, rho: real padded float grids in the time domain (read only);
+ * out[ntheta]: real padded float grids (written).  Asynchronous on the
+ * context's stream once the tables are uploaded. */
+int crava_hip_synthetic_seismic(crava_hip_ctx *ctx, const crava_hip_synt_params *p, const void *alpha,
+                                const void *beta, const void *rho, void *const *out);
+
 #ifdef __cplusplus
 }
 #endif

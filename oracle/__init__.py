@@ -64,6 +64,7 @@ def orc():
         L.orc_data_variance.argtypes = [VP, I, I, I, I, I, I, C.POINTER(C.c_longlong)]
         L.orc_data_variance.restype = C.c_float
         L.orc_facies_geomodel.argtypes = [I, I, I, I, VP, PP]
+        L.orc_synthetic_seismic.argtypes = [I, I, I, I, I, I, I, VP, VP, F, PP, PP, PP, VP, VP, VP, PP]
         L.orc_posterior.restype = C.c_longlong
         L.orc_posterior_f64.restype = C.c_longlong
         L.orc_simulate_points.restype = C.c_longlong
@@ -92,6 +93,7 @@ def ref():
         L.ref_axeqb_mat_cpx.argtypes = [I, VP, VP, I]
         L.ref_posterior.argtypes = [I, I, I, I, VP, VP, F, VP, VP, VP, VP, VP, VP, PP, PP, PP, VP]
         L.ref_simulate_points.argtypes = [LL, PP, PP]
+        L.ref_synthetic_seismic.argtypes = [I, I, I, I, I, I, I, VP, VP, F, PP, PP, PP, VP, VP, VP, PP, I, I]
         L.ref_posterior.restype = C.c_longlong
         L.ref_simulate_points.restype = C.c_longlong
         L.ref_wall_seconds.restype = C.c_double
@@ -261,3 +263,36 @@ def facies_geomodel(nx, ny, nz, prior, prob):
     pr = np.ascontiguousarray(prior, dtype=np.float32)
     orc().orc_facies_geomodel(nx, ny, nz, len(prior), pr.ctypes.data, _ptrs(prob))
     return prob
+
+
+# ------------------------------------------------------------ forward modelling
+def _opt_tables(tabs, nt, nx, ny):
+    """[nt] optional per-trace tables -> (keep-alive list, pointer array or None)."""
+    if tabs is None:
+        return [], None
+    keep = [None if t is None else np.ascontiguousarray(t, dtype=np.float32).reshape(ny, nx) for t in tabs]
+    arr = (VP * nt)()
+    for l, t in enumerate(keep):
+        arr[l] = None if t is None else t.ctypes.data
+    return keep, arr
+
+
+def synthetic_seismic(nx, ny, nz, nxp, nyp, nzp, A, wavelets, dz, alpha, beta, rho, shift=None, gain=None,
+                      stretch=None, which: str = "oracle", i_range=(0, 0)):
+    """Crava::computeSyntSeismic on real padded float grids; returns the
+    ntheta synthetic grids.  shift/gain/stretch: None or a list of ntheta
+    entries, each None or [ny][nx].  which="ref" runs the reference's FFTW
+    (traces with i in i_range only, (0, 0) = all)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    nt = A.shape[0]
+    wav = np.ascontiguousarray(wavelets, dtype=np.float32)
+    grids = [np.ascontiguousarray(g, dtype=np.float32) for g in (alpha, beta, rho)]
+    out = [np.zeros_like(grids[0]) for _ in range(nt)]
+    keep = [_opt_tables(t, nt, nx, ny) for t in (shift, gain, stretch)]
+    args = (nx, ny, nz, nxp, nyp, nzp, nt, A.ctypes.data, wav.ctypes.data, dz, keep[0][1], keep[1][1], keep[2][1],
+            grids[0].ctypes.data, grids[1].ctypes.data, grids[2].ctypes.data, _ptrs(out))
+    if which == "ref":
+        ref().ref_synthetic_seismic(*args, i_range[0], i_range[1])
+    else:
+        orc().orc_synthetic_seismic(*args)
+    return out

@@ -1118,3 +1118,108 @@ void orc_facies_geomodel(int nx, int ny, int nz, int nfac, const float *prior, f
         }
       }
 }
+
+/* ========================================================================
+ * Forward modelling: Crava::computeSyntSeismic (src/crava.cpp:1667-1759)
+ * with computeSeismicImpedance (:1632-1663), the Wavelet1D difference
+ * operator (src/wavelet1D.cpp:420-436), the local wavelet
+ * (Wavelet1D::createLocalWavelet1D / shiftAndScale, src/wavelet1D.cpp:
+ * 501-561) and Wavelet::getCAmp(k, scale) (src/wavelet.cpp:328-384).
+ * The 1-D transforms are double DFTs rounded to float (the reference runs
+ * FFTW 2.1.2 in float); everything else is the reference's float order.
+ *   A [nt][3], wavelets [nt][nzp]; shift/gain/sf: NULL or nt pointers, each
+ *   NULL or [ny][nx]; alpha/beta/rho/out[nt]: real padded grids.
+ * ====================================================================== */
+
+/* c2r of a half spectrum (FFTW's backward sign, unnormalised; the imaginary
+ * parts of the DC and even-length Nyquist bins are not read), rounded to
+ * float. */
+static void c2r_f(const ocpx *spec, int nzp, float *out)
+{
+  dcpx *x = (dcpx *)malloc(sizeof(dcpx) * nzp);
+  dcpx *tmp = (dcpx *)malloc(sizeof(dcpx) * nzp);
+  int cn = nzp / 2 + 1;
+  for (int k = 0; k < nzp; k++) {
+    if (k < cn) { x[k].re = spec[k].re; x[k].im = spec[k].im; }
+    else { x[k].re = spec[nzp - k].re; x[k].im = -spec[nzp - k].im; }
+  }
+  x[0].im = 0.0;
+  if (nzp % 2 == 0) x[nzp / 2].im = 0.0;
+  dft_rec(x, nzp, +1, tmp);
+  for (int t = 0; t < nzp; t++) out[t] = (float)x[t].re;
+  free(x); free(tmp);
+}
+
+void orc_synthetic_seismic(int nx, int ny, int nz, int nxp, int nyp, int nzp, int nt, const float *A,
+                           const float *wavelets, float dz, const float *const *shift,
+                           const float *const *gain, const float *const *sf, const float *alpha,
+                           const float *beta, const float *rho, float **out)
+{
+  const int rnxp = 2 * (nxp / 2 + 1), cn = nzp / 2 + 1;
+  const long long plane = (long long)rnxp * nyp, rsize = plane * nzp;
+  float *imp = (float *)malloc(sizeof(float) * nzp);
+  float *r = (float *)malloc(sizeof(float) * nzp);
+  ocpx *R = (ocpx *)malloc(sizeof(ocpx) * cn);
+  ocpx *W = (ocpx *)malloc(sizeof(ocpx) * cn);
+  ocpx *L = (ocpx *)malloc(sizeof(ocpx) * cn);
+  for (int l = 0; l < nt; l++) {
+    const float *Al = A + 3 * l;
+    float *g = out[l];
+    /* computeSeismicImpedance: every cell of the padded grid */
+    for (long long t = 0; t < rsize; t++) {
+      float v = 0;
+      v += alpha[t] * Al[0];
+      v += beta[t] * Al[1];
+      v += rho[t] * Al[2];
+      g[t] = v;
+    }
+    spectrum_f(wavelets + (long long)l * nzp, nzp, W);
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++) {
+        const long long tr = (long long)j * rnxp + i, ti = (long long)j * nx + i;
+        int k;
+        for (k = 0; k < nz; k++) imp[k] = g[k * plane + tr];
+        float fac = 1.0f / (float)(nzp - nz - 1);                 /* taper, :1692-1696 */
+        for (; k < nzp; k++) imp[k] = fac * ((k - nz) * imp[0] + (nzp - k - 1) * imp[nz - 1]);
+        for (k = 0; k < nzp; k++)                                  /* FIRSTORDERFORWARDDIFF */
+          r[k] = (k == nzp - 1) ? imp[0] - imp[k] : imp[k + 1] - imp[k];
+        spectrum_f(r, nzp, R);
+        /* local wavelet: shiftAndScale of the global spectrum */
+        float ts = (shift && shift[l]) ? shift[l][ti] : 0.0f;
+        float gn = (gain && gain[l]) ? gain[l][ti] : 1.0f;
+        float ishift = ts / dz;
+        if (!(shift && shift[l])) ishift = 0.0f;
+        for (int q = 0; q < cn; q++) {
+          ocpx am, tmp;
+          am.re = (float)(gn * cos(2.0 * (M_PI * (ishift) * q) / (float)nzp));
+          am.im = (float)(gn * sin(-2.0 * (M_PI * (ishift) * q) / (float)nzp));
+          tmp.re = am.re * W[q].re - am.im * W[q].im;
+          tmp.im = am.im * W[q].re + am.re * W[q].im;
+          L[q] = tmp;
+        }
+        float scale = (sf && sf[l]) ? sf[l][ti] : 1.0f;
+        for (k = 0; k < cn; k++) {
+          ocpx w;                                                   /* getCAmp(k, scale), k < cnzp */
+          float omega = (float)k / scale;
+          if (omega >= cn) { w.re = 0.0f; w.im = 0.0f; }
+          else {
+            int omL = (int)floorf(omega), omU = (int)floorf(omega) + 1;
+            if (omU >= cn) omU -= 1;
+            float dO = omega - (float)omL;
+            w.re = L[omL].re * (1.0f - dO) + L[omU].re * dO;
+            w.im = (L[omL].im * (1.0f - dO) + L[omU].im * dO);
+          }
+          w.re /= scale;
+          w.im /= scale;
+          ocpx s;                                                   /* :1711-1714 */
+          s.re = R[k].re * w.re + R[k].im * w.im;
+          s.im = -R[k].re * w.im + R[k].im * w.re;
+          R[k] = s;
+        }
+        c2r_f(R, nzp, r);
+        const double inv = 1.0 / (double)nzp;                       /* invFFT1DInPlace scale */
+        for (k = 0; k < nzp; k++) g[k * plane + tr] = (float)(r[k] * inv);
+      }
+  }
+  free(imp); free(r); free(R); free(W); free(L);
+}

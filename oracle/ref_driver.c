@@ -213,3 +213,86 @@ double ref_wall_seconds(void)
   clock_gettime(CLOCK_MONOTONIC, &ts);
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
+
+/* Crava::computeSyntSeismic trace loop (src/crava.cpp:1667-1759) with
+ * computeSeismicImpedance (:1632-1663), run on the reference's FFTW 2.1.2
+ * 1-D transforms (Wavelet::fft1DInPlace / invFFT1DInPlace,
+ * src/wavelet.cpp:246-280) and the local wavelet of Wavelet1D::shiftAndScale
+ * (src/wavelet1D.cpp:539-561) and Wavelet::getCAmp(k, scale)
+ * (src/wavelet.cpp:328-384).  Arguments as orc_synthetic_seismic.  Traces
+ * [i0, i1) x [0, ny) only (i1 <= 0: all), so the CPU baseline can time a
+ * bounded sample. */
+void ref_synthetic_seismic(int nx, int ny, int nz, int nxp, int nyp, int nzp, int nt, const float *A,
+                           const float *wavelets, float dz, const float *const *shift,
+                           const float *const *gain, const float *const *sf, const float *alpha,
+                           const float *beta, const float *rho, float **out, int i0, int i1)
+{
+  const int rnxp = 2 * (nxp / 2 + 1), cn = nzp / 2 + 1, rn = 2 * cn;
+  const long long plane = (long long)rnxp * nyp, rsize = plane * nzp;
+  float *imp = (float *)malloc(sizeof(float) * rn);
+  float *r = (float *)malloc(sizeof(float) * rn);
+  float *w = (float *)malloc(sizeof(float) * rn);
+  float *loc = (float *)malloc(sizeof(float) * rn);
+  fftw_complex *R = (fftw_complex *)r, *Wc = (fftw_complex *)w, *Lc = (fftw_complex *)loc;
+  rfftwnd_plan p1 = rfftwnd_create_plan(1, &nzp, FFTW_REAL_TO_COMPLEX, FFTW_ESTIMATE | FFTW_IN_PLACE);
+  rfftwnd_plan p2 = rfftwnd_create_plan(1, &nzp, FFTW_COMPLEX_TO_REAL, FFTW_ESTIMATE | FFTW_IN_PLACE);
+  if (i1 <= 0 || i1 > nx) i1 = nx;
+  for (int l = 0; l < nt; l++) {
+    const float *Al = A + 3 * l;
+    float *g = out[l];
+    for (long long t = 0; t < rsize; t++) {
+      float v = 0;
+      v += alpha[t] * Al[0];
+      v += beta[t] * Al[1];
+      v += rho[t] * Al[2];
+      g[t] = v;
+    }
+    memcpy(w, wavelets + (long long)l * nzp, sizeof(float) * nzp);
+    rfftwnd_one_real_to_complex(p1, (fftw_real *)w, Wc);
+    for (int i = i0; i < i1; i++)
+      for (int j = 0; j < ny; j++) {
+        const long long tr = (long long)j * rnxp + i, ti = (long long)j * nx + i;
+        int k;
+        for (k = 0; k < nz; k++) imp[k] = g[k * plane + tr];
+        float fac = 1.0f / (float)(nzp - nz - 1);
+        for (; k < nzp; k++) imp[k] = fac * ((k - nz) * imp[0] + (nzp - k - 1) * imp[nz - 1]);
+        for (k = 0; k < nzp; k++) r[k] = (k == nzp - 1) ? imp[0] - imp[k] : imp[k + 1] - imp[k];
+        rfftwnd_one_real_to_complex(p1, (fftw_real *)r, R);
+        float ts = (shift && shift[l]) ? shift[l][ti] : 0.0f;
+        float gn = (gain && gain[l]) ? gain[l][ti] : 1.0f;
+        float ishift = (shift && shift[l]) ? ts / dz : 0.0f;
+        for (int q = 0; q < cn; q++) {
+          fftw_complex am, tmp;
+          am.re = (float)(gn * cos(2.0 * (M_PI * (ishift) * q) / (float)nzp));
+          am.im = (float)(gn * sin(-2.0 * (M_PI * (ishift) * q) / (float)nzp));
+          tmp.re = am.re * Wc[q].re - am.im * Wc[q].im;
+          tmp.im = am.im * Wc[q].re + am.re * Wc[q].im;
+          Lc[q] = tmp;
+        }
+        float scale = (sf && sf[l]) ? sf[l][ti] : 1.0f;
+        for (k = 0; k < cn; k++) {
+          fftw_complex v, s;
+          float omega = (float)k / scale;
+          if (omega >= cn) { v.re = 0.0f; v.im = 0.0f; }
+          else {
+            int omL = (int)floor(omega), omU = (int)floor(omega) + 1;
+            if (omU >= cn) omU -= 1;
+            float dO = omega - (float)omL;
+            v.re = Lc[omL].re * (1.0f - dO) + Lc[omU].re * dO;
+            v.im = (Lc[omL].im * (1.0f - dO) + Lc[omU].im * dO);
+          }
+          v.re /= scale;
+          v.im /= scale;
+          s.re = R[k].re * v.re + R[k].im * v.im;
+          s.im = -R[k].re * v.im + R[k].im * v.re;
+          R[k] = s;
+        }
+        rfftwnd_one_complex_to_real(p2, R, (fftw_real *)r);
+        const double inv = 1.0 / (double)nzp;
+        for (k = 0; k < nzp; k++) g[k * plane + tr] = (float)(r[k] * inv);
+      }
+  }
+  fftwnd_destroy_plan(p1);
+  fftwnd_destroy_plan(p2);
+  free(imp); free(r); free(w); free(loc);
+}

@@ -5,7 +5,7 @@ library, so it runs where the reference sources are).  The fixtures are
 data only (inputs and the reference's outputs), so the oracle can be
 pinned where the reference sources are absent.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [reference|forward]   (default: both)
 """
 import os
 import sys
@@ -69,5 +69,38 @@ def main():
     print("wrote", os.path.join(HERE, "reference_cases.npz"))
 
 
+def forward():
+    """Crava::computeSyntSeismic's trace loop on the reference's FFTW 2.1.2
+    (oracle/ref_driver.c): a small odd-sized volume, three angles, local
+    shifts, gains and stretches on some angles."""
+    assert oracle.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+    rng = np.random.default_rng(77)
+    nx, ny, nz, nxp, nyp, nzp = 7, 6, 21, 9, 8, 40
+    rnxp = 2 * (nxp // 2 + 1)
+    out = {"dims": np.array([nx, ny, nz, nxp, nyp, nzp], np.int32), "dz": np.float32(4.0)}
+    for name in ("alpha", "beta", "rho"):
+        out[name] = rng.standard_normal((nzp, nyp, rnxp)).astype(np.float32)
+    out["A"] = rng.standard_normal((3, 3)).astype(np.float32)
+    t = np.arange(nzp)
+    t = np.minimum(t, nzp - t).astype(np.float64)
+    out["wavelets"] = np.stack([np.exp(-(t / (2.0 + l)) ** 2) * np.cos(0.7 * t * (l + 1))
+                                for l in range(3)]).astype(np.float32)
+    out["shift0"] = rng.uniform(-10, 10, (ny, nx)).astype(np.float32)
+    out["gain1"] = rng.uniform(0.5, 1.5, (ny, nx)).astype(np.float32)
+    out["stretch0"] = rng.uniform(0.7, 1.3, (ny, nx)).astype(np.float32)
+    out["stretch2"] = rng.uniform(0.9, 1.1, (ny, nx)).astype(np.float32)
+    tabs = {name: [out.get(f"{name}{l}") for l in range(3)] for name in ("shift", "gain", "stretch")}
+    res = oracle.synthetic_seismic(nx, ny, nz, nxp, nyp, nzp, out["A"], out["wavelets"], 4.0, out["alpha"],
+                                   out["beta"], out["rho"], which="ref", **tabs)
+    for l, g in enumerate(res):
+        out[f"out{l}"] = g
+    np.savez_compressed(os.path.join(HERE, "forward_cases.npz"), **out)
+    print("wrote", os.path.join(HERE, "forward_cases.npz"))
+
+
 if __name__ == "__main__":
-    main()
+    which = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if which in ("all", "reference"):
+        main()
+    if which in ("all", "forward"):
+        forward()
