@@ -102,6 +102,8 @@ def parse():
     ap.add_argument("--sweep", action="store_true", help="report accuracy over a range of FP64 thresholds")
     ap.add_argument("--batch-volumes", type=int, default=48, help="volumes of the batching workload (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--forward-reps", type=int, default=5, help="timed calls of the forward modelling "
+                    "workload (0: skip)")
     ap.add_argument("--diff-steps", type=int, default=50, help="L-BFGS iterations of the differentiation workload "
                     "(0: skip it)")
     return ap.parse_args()
@@ -443,6 +445,65 @@ def differentiation(ctx, steps: int = 50):
     return out
 
 
+def forward_modelling(ctx, prob, reps: int, cpu: bool):
+    """Forward modelling (crava_hip_synthetic_seismic, Crava::computeSyntSeismic)
+    of the posterior workload's volume: synthetic seismic for every angle
+    from the elastic parameter grids (the prior means here), timed per call
+    (host wall clock, tables and uploads included).  Algorithmic bytes: one
+    read of the three parameter grids and one write per angle.  The
+    reference's own FFTW path (oracle/_ref) is timed on one core on the
+    first columns of traces and scaled to the volume."""
+    import numpy as np
+    import torch
+    from crava_amd.core import synthetic_seismic
+    c = prob.cfg
+    nt = prob.ntheta
+    n6 = (c.nx, c.ny, c.nz, c.nxp, c.nyp, c.nzp)
+    A = np.asarray(prob.A, dtype=np.float32)
+    wav = np.asarray(prob.wavelets, dtype=np.float32)
+    g = [m.float().contiguous() for m in prob.mu]
+    out = [torch.empty_like(g[0]) for _ in range(nt)]
+    synthetic_seismic(ctx, *n6, A, wav, c.dt, *g, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        synthetic_seismic(ctx, *n6, A, wav, c.dt, *g, out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    rsize = g[0].numel()
+    alg = (12 + 4 * nt) * rsize
+    traces = c.nx * c.ny
+    res = {"volume": [c.nx, c.ny, c.nz], "grid": [c.nxp, c.nyp, c.nzp], "ntheta": nt, "ms_per_call": ms,
+           "traces_per_s": traces * nt / (ms * 1e-3), "alg_bytes": alg,
+           "alg_GBps": alg / (ms * 1e-3) / 1e9, "frac_of_peak": alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+           "cpu_reference": None}
+    log(f"forward modelling: {ms:.2f} ms per call ({nt} angles), {res['alg_GBps']:.0f} GB/s on the minimal bytes")
+    if cpu:
+        try:
+            import oracle
+            if oracle.ref_available():
+                h = [m.cpu().numpy() for m in g]
+                t1 = time.perf_counter()
+                oracle.synthetic_seismic(*n6, A, wav, c.dt, *h, which="ref", i_range=(0, 1))
+                t_one = time.perf_counter() - t1
+                cols = 8
+                t1 = time.perf_counter()
+                oracle.synthetic_seismic(*n6, A, wav, c.dt, *h, which="ref", i_range=(0, 1 + cols))
+                t_more = time.perf_counter() - t1
+                per_col = max(0.0, t_more - t_one) / cols
+                total = t_one + per_col * (c.nx - 1)
+                res["cpu_reference"] = {"seconds_per_call": total, "cores": 1, "kind": "reference",
+                                        "sample": f"impedance of every cell + traces of the first 1 and "
+                                                  f"{1 + cols} x-columns ({t_one:.1f}s, {t_more:.1f}s), "
+                                                  f"scaled to {c.nx} columns",
+                                        "gpu_speedup": total / (ms * 1e-3)}
+                log(f"forward modelling, reference CPU path: {total:.1f} s per call (1 core)")
+        except Exception as e:  # the baseline is reported, never the target
+            log(f"forward modelling cpu baseline failed: {e}")
+    del out, g
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -628,6 +689,10 @@ def main():
     if args.diff_steps > 0:
         torch.cuda.empty_cache()
         diff = differentiation(ctx, args.diff_steps)
+    fwd = None
+    if args.forward_reps > 0:
+        torch.cuda.empty_cache()
+        fwd = forward_modelling(ctx, prob, args.forward_reps, not args.no_cpu)
     cpu = None
     if not args.no_cpu:
         try:
@@ -673,6 +738,7 @@ def main():
         "cpu_baseline": cpu,
         "batching": batch,
         "differentiation": diff,
+        "forward_modelling": fwd,
     }
     print(json.dumps(out))
 

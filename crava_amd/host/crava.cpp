@@ -139,4 +139,53 @@ long long Crava::simulate(SeismicParametersHolder &sph, unsigned long long seed,
   return nfail;
 }
 
+void Crava::computeSyntSeismic(FFTGrid *alpha, FFTGrid *beta, FFTGrid *rho, const std::vector<FFTGrid *> &syn) {
+  const int nt = s_.ntheta;
+  if ((int)syn.size() != nt) throw Error(CRAVA_HIP_E_INVALID, "computeSyntSeismic needs one grid per angle");
+  const bool fftDomain = alpha->getIsTransformed();
+  if (fftDomain) {
+    alpha->invFFTInPlace();
+    beta->invFFTInPlace();
+    rho->invFFTInPlace();
+  }
+  const size_t ntr = (size_t)s_.nx * s_.ny;
+  auto table = [&](const std::vector<std::vector<float>> &t, std::vector<const float *> &ptrs) -> const float *const * {
+    if (t.empty()) return nullptr;
+    if ((int)t.size() != nt) throw Error(CRAVA_HIP_E_INVALID, "local wavelet tables need one entry per angle");
+    ptrs.assign(nt, nullptr);
+    for (int l = 0; l < nt; ++l) {
+      if (t[l].empty()) continue;
+      if (t[l].size() != ntr) throw Error(CRAVA_HIP_E_INVALID, "local wavelet table of the wrong size");
+      ptrs[l] = t[l].data();
+    }
+    return ptrs.data();
+  };
+  std::vector<const float *> pshift, pgain, pstretch;
+  crava_hip_synt_params p{};
+  p.nx = s_.nx;
+  p.ny = s_.ny;
+  p.nz = s_.nz;
+  p.nxp = s_.nxp;
+  p.nyp = s_.nyp;
+  p.nzp = s_.nzp;
+  p.ntheta = nt;
+  p.dz = s_.dz;
+  p.A = s_.A.data();
+  p.wavelets = s_.wavelets.data();
+  p.shift = table(s_.localShift, pshift);
+  p.gain = table(s_.localGain, pgain);
+  p.stretch = table(s_.stretch, pstretch);
+  std::vector<void *> out(nt);
+  for (int l = 0; l < nt; ++l) out[l] = syn[l]->devicePtr();
+  check(crava_hip_synthetic_seismic(ctx_->handle(), &p, alpha->devicePtr(), beta->devicePtr(), rho->devicePtr(),
+                                    out.data()),
+        "synthetic seismic");
+  for (int l = 0; l < nt; ++l) syn[l]->setIsTransformed(false);
+  if (fftDomain) {
+    alpha->fftInPlace();
+    beta->fftInPlace();
+    rho->fftInPlace();
+  }
+}
+
 }  // namespace crava_hip

@@ -197,6 +197,9 @@ class Crava {
     float corrGradI = 0.0f, corrGradJ = 0.0f;
     int precision = CRAVA_HIP_F32;      // posterior solve: F32 or MIXED
     double mixedThreshold = 0.1;
+    // forward modelling: per angle, empty or [ny*nx] (i fastest) local
+    // wavelet time shifts (ms), gains and stretches relThick*localStretch
+    std::vector<std::vector<float>> localShift, localGain, stretch;
   };
 
   Crava(Context *ctx, const Settings &s, std::vector<FFTGrid *> seisData);
@@ -211,6 +214,10 @@ class Crava {
   // :1416-1599 one realisation: noise (seed, stream) -> real grids real[3].
   long long simulate(SeismicParametersHolder &sph, unsigned long long seed, unsigned long long stream,
                      FFTGrid *real[3]);
+  // :1667-1759 forward modelling: synthetic seismic of (alpha, beta, rho)
+  // for every angle into syn[ntheta] (real grids).  Parameters held in the
+  // Fourier domain are transformed back and forth, as the reference does.
+  void computeSyntSeismic(FFTGrid *alpha, FFTGrid *beta, FFTGrid *rho, const std::vector<FFTGrid *> &syn);
 
   const std::vector<float> &getDataVariance() const { return dataVariance_; }
   const std::vector<float> &getErrorVariance() const { return errorVariance_; }

@@ -133,6 +133,34 @@ int main(int argc, char **argv) {
       delete real[c];
     }
 
+    // forward modelling of the posterior means (Crava::computeSyntSeismic),
+    // with a stretch table on angle 0 and a local time shift on the last
+    const size_t ntr = (size_t)s.nx * s.ny;
+    std::vector<float> sf(ntr), ts(ntr);
+    for (int j = 0; j < s.ny; ++j)
+      for (int i = 0; i < s.nx; ++i) {
+        sf[(size_t)j * s.nx + i] = 0.9f + 0.2f * (float)((i * 7 + j * 3) % 11) / 10.0f;
+        ts[(size_t)j * s.nx + i] = (float)((i + 2 * j) % 9) - 4.0f;
+      }
+    s.stretch.assign(nt, {});
+    s.localShift.assign(nt, {});
+    s.stretch[0] = sf;
+    s.localShift[nt - 1] = ts;
+    write_bin(dir + "/stretch.bin", sf);
+    write_bin(dir + "/shift.bin", ts);
+    crava_hip::Crava fwd(&ctx, s, seis);
+    std::vector<FFTGrid *> syn(nt);
+    for (int l = 0; l < nt; ++l) {
+      syn[l] = new FFTGrid(&ctx, s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
+      syn[l]->setType(FFTGrid::DATA);
+      syn[l]->createRealGrid();
+    }
+    fwd.computeSyntSeismic(sph.mu(0), sph.mu(1), sph.mu(2), syn);
+    for (int l = 0; l < nt; ++l) {
+      write_bin(dir + "/out_syn" + std::to_string(l) + ".bin", drain_real(syn[l]));
+      delete syn[l];
+    }
+
     // a few cursor / random-access checks against the values just written
     FFTGrid *m0 = sph.mu(0);
     m0->setAccessMode(FFTGrid::RANDOMACCESS);

@@ -9,7 +9,9 @@ tests/cpp/host_mirror_main, against the oracle:
 
 Tolerances: circCorrT from the mirror's double DFT vs numpy's FFT, 1e-6;
 the inversion outputs as the smoke check (float FFTs of two different
-algorithms plus the float loop), 5e-5 relative L2.
+algorithms plus the float loop), 5e-5 relative L2; the synthetic seismic of
+the mirror's own posterior means against the oracle's forward model, 1e-6
+relative L2 inside the volume and bit-exact impedance in the padding.
 """
 import math
 import os
@@ -131,6 +133,20 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
     caa = np.abs(cv[0].view(np.complex64)[:, :, :cn].real)
     noise_level = int((caa <= 1e-5 * caa.max()).sum())
     assert simfail <= noise_level and sim_ref <= noise_level, (simfail, sim_ref, noise_level)
+    # computeSyntSeismic of the posterior means the mirror wrote
+    means = [_read(os.path.join(d, f"out_mu{i}.bin"), shape) for i in range(3)]
+    sf = np.fromfile(os.path.join(d, "stretch.bin"), dtype=np.float32).reshape(c.ny, c.nx)
+    ts = np.fromfile(os.path.join(d, "shift.bin"), dtype=np.float32).reshape(c.ny, c.nx)
+    stretch = [sf] + [None] * (nt - 1)
+    shift = [None] * (nt - 1) + [ts]
+    syn_ref = oracle.synthetic_seismic(c.nx, c.ny, c.nz, nxp, nyp, nzp, prob.A, prob.wavelets, c.dt, *means,
+                                       shift=shift, stretch=stretch)
+    for l in range(nt):
+        got = _read(os.path.join(d, f"out_syn{l}.bin"), shape)
+        assert rel_l2(got[:, :c.ny, :c.nx], syn_ref[l][:, :c.ny, :c.nx]) < 1e-6, l
+        pad = np.ones(shape[1:], bool)
+        pad[:c.ny, :c.nx] = False
+        assert np.array_equal(got[:, pad], syn_ref[l][:, pad]), l
     for i in range(3):
         sim = _read(os.path.join(d, f"out_sim{i}.bin"), shape)[:, :, :nxp]
         assert np.isfinite(sim).all()
