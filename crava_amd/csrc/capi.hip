@@ -574,6 +574,7 @@ int crava_hip_synthetic_seismic(crava_hip_ctx *ctx, const crava_hip_synt_params 
   a.nyp = p->nyp;
   a.nzp = p->nzp;
   a.rnxp = 2 * (p->nxp / 2 + 1);
+  if (crava::fm_fused_available(p->nzp) && f->stw[2]) return crava::launch_fm_fused(a, f->stw[2], ctx->stream);
   if ((rc = crava::launch_fm_reflect(a, ctx->stream))) return rc;
   if ((rc = f->pass(2, out, nt, false, nullptr, ctx->stream))) return rc;
   if ((rc = crava::launch_fm_filter(a, ctx->stream))) return rc;

@@ -10,128 +10,11 @@
 #include "../../include/crava_hip.h"
 #include "fft_dft.hpp"
 #include "fft_static.hpp"
+#include "stockham.hpp"
 
 #include <cmath>
-#include <type_traits>
 
 namespace crava {
-
-// One pad slot every 32 complex floats, or every 16 complex doubles (a
-// 16-byte element spans four banks; the denser padding keeps the strided
-// Stockham writes spread over the banks).
-template <typename T>
-constexpr int kPadShift = sizeof(T) == 8 ? 4 : 5;
-template <typename T>
-__device__ __forceinline__ constexpr int padx(int x) { return x + (x >> kPadShift<T>); }
-
-template <int N, typename T>
-struct SP {
-  static constexpr RadixList rl = make_radices(N);
-  static constexpr int len = N + ((N - 1) >> kPadShift<T>) + 1;
-  static constexpr int pitch = len | 1;
-  static constexpr int ns(int p) {
-    int s = 1;
-    for (int i = 0; i < p; ++i) s *= rl.r[i];
-    return s;
-  }
-  // start of pass p's twiddles in the per-pass table (static_twiddle_table)
-  static constexpr int off(int p) {
-    int o = 0;
-    for (int i = 0; i < p; ++i) o += ns(i) * (rl.r[i] - 1);
-    return o;
-  }
-};
-
-// Marks a pass that reads or writes its pencils in LDS (no global hook).
-struct LdsIo {};
-
-template <typename F>
-constexpr bool kLds = std::is_same<F, LdsIo>::value;
-
-// Pass P of the Stockham sequence over TX pencils of length N.
-//   inputs:  element j + r*NB of pencil c, from LDS or from ld(c, e);
-//   outputs: element (j / Ns) * Ns * R + k + r*Ns, to LDS or to st(c, e, v).
-// CF picks the lane mapping: pencil-fastest (c = b % TX, for columns that
-// sit side by side in global memory) or element-fastest (c = b / NB).
-// A pass whose inputs come from global memory needs no barrier before its
-// LDS writes (nothing else reads LDS then); a pass writing global memory
-// needs none after.
-template <int N, int TX, int THREADS, int DIR, int P, bool CF, typename T, typename Ld, typename St>
-__device__ __forceinline__ void spass(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw, const Ld &ld,
-                                      const St &st) {
-  constexpr int R = SP<N, T>::rl.r[P];
-  constexpr int Ns = SP<N, T>::ns(P);
-  constexpr int NB = N / R;
-  constexpr int TOT = TX * NB;
-  constexpr int TRIPS = (TOT + THREADS - 1) / THREADS;
-  constexpr int PITCH = SP<N, T>::pitch;
-  cx<T> v[TRIPS][R];
-#pragma unroll
-  for (int t = 0; t < TRIPS; ++t) {
-    const int b = (int)threadIdx.x + t * THREADS;
-    if (TOT % THREADS == 0 || b < TOT) {
-      const int c = CF ? b % TX : b / NB, j = CF ? b / TX : b % NB;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if constexpr (kLds<Ld>) v[t][r] = buf[c * PITCH + padx<T>(j + r * NB)];
-        else v[t][r] = ld(c, j + r * NB);
-      }
-      if constexpr (Ns > 1) {
-        // per-pass table [r-1][k]
-        const int k = j % Ns;
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          cx<T> w = tw[SP<N, T>::off(P) + (r - 1) * Ns + k];
-          if (DIR > 0) w.y = -w.y;
-          v[t][r] = mul(v[t][r], w);
-        }
-      }
-      Dft<R, DIR, T>::run(v[t]);
-    }
-  }
-  if constexpr (kLds<Ld> && kLds<St>) __syncthreads();
-#pragma unroll
-  for (int t = 0; t < TRIPS; ++t) {
-    const int b = (int)threadIdx.x + t * THREADS;
-    if (TOT % THREADS == 0 || b < TOT) {
-      const int c = CF ? b % TX : b / NB, j = CF ? b / TX : b % NB;
-      const int k = j % Ns;
-      const int base = (j / Ns) * Ns * R + k;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if constexpr (kLds<St>) buf[c * PITCH + padx<T>(base + r * Ns)] = v[t][r];
-        else st(c, base + r * Ns, v[t][r]);
-      }
-    }
-  }
-  if constexpr (kLds<St>) __syncthreads();
-}
-
-// All passes.  The first pass takes its inputs from ld and the last writes
-// through st (either may be LdsIo); CF0 / CFL are their lane mappings, the
-// passes between are element-fastest.
-template <int N, int TX, int THREADS, int DIR, int P, bool CF0, bool CFL, typename T, typename Ld, typename St>
-__device__ __forceinline__ void spasses_io(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw, const Ld &ld,
-                                           const St &st) {
-  constexpr int CNT = SP<N, T>::rl.cnt;
-  static_assert(CNT >= 2, "the specialised lengths have at least two passes");
-  if constexpr (P < CNT) {
-    if constexpr (P == 0) spass<N, TX, THREADS, DIR, P, CF0, T>(buf, tw, ld, LdsIo{});
-    else if constexpr (P == CNT - 1) spass<N, TX, THREADS, DIR, P, CFL, T>(buf, tw, LdsIo{}, st);
-    else spass<N, TX, THREADS, DIR, P, false, T>(buf, tw, LdsIo{}, LdsIo{});
-    spasses_io<N, TX, THREADS, DIR, P + 1, CF0, CFL, T>(buf, tw, ld, st);
-  }
-}
-
-template <int N, int TX, int THREADS, int DIR, int P, typename T>
-__device__ __forceinline__ void spasses(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw) {
-  spasses_io<N, TX, THREADS, DIR, P, false, false, T>(buf, tw, LdsIo{}, LdsIo{});
-}
-
-template <int N, typename T>
-constexpr size_t lds_bytes(int tx) {
-  return sizeof(cx<T>) * ((size_t)N + (size_t)tx * SP<N, T>::pitch);
-}
 
 // Strided complex pass: columns contiguous, elements `es` apart.
 // DIO: the first pass loads its butterflies straight from global memory and

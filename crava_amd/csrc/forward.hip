@@ -17,16 +17,23 @@
 // impedance grid the reference writes into does.
 //
 // GPU form.  The traces at x = 2m and 2m+1 are the real and imaginary parts
-// of complex column m of the grid's half-complex view, so the z transforms
-// are the z passes of the grid's transform plan over all angles' grids in
-// one launch each.  Between them, k_fm_filter separates the two traces'
-// spectra from Z(k) and Z(N-k), filters each with its own wavelet and packs
-// them again.  Three grid kernels: k_fm_reflect (impedance, taper,
-// difference), k_fm_filter, k_fm_padding (exact impedance of the padding
-// traces after the transforms).  Built with -ffp-contract=off: the
-// impedance, taper and difference follow the reference's float operations.
+// of complex column m of the grid's half-complex view: one complex z
+// transform serves two traces, and the filter separates their spectra from
+// Z(k) and Z(N-k), filters each with its own wavelet and packs them again.
+//   * nzp with a specialised transform: one fused kernel per (column block,
+//     angle) -- parameters in, impedance, taper and difference into LDS, the
+//     forward Stockham passes, the filter, the inverse passes, seismic out.
+//     One read of the three parameter grids and one write per angle.
+//   * other lengths: k_fm_reflect (impedance, taper, difference into the
+//     output grids), the plan's z passes over all angles in one launch,
+//     k_fm_filter, the inverse z passes, k_fm_padding (exact impedance of the
+//     padding traces after the transforms).
+// Built with -ffp-contract=off: the impedance, taper and difference follow
+// the reference's float operations.
 #include "../../include/crava_hip.h"
+#include "fft_static.hpp"
 #include "forward.hpp"
+#include "stockham.hpp"
 
 #include <cmath>
 
@@ -191,6 +198,189 @@ __global__ void __launch_bounds__(256) k_fm_filter(FwdArgs a) {
   } else {
     g[(long long)k * ncol + q] = mk(S[0].x - S[1].y, S[0].y + S[1].x);
     g[(long long)km * ncol + q] = mk(S[0].x + S[1].y, S[1].x - S[0].y);
+  }
+}
+
+// Local wavelet parameters of one trace, staged in LDS by the fused kernel.
+struct TraceParams {
+  float sf, ishift, gain, inside;
+};
+
+// Filter of one bin pair (k, N-k) of a complex column carrying traces pa, pb
+// (Re, Im): separate the two traces' spectra, filter each trace inside the
+// volume with its local wavelet (spectrum W), scale by 1/N and pack again.
+__device__ __forceinline__ void filter_pair(const cx<float> *W, const TraceParams &pa, const TraceParams &pb, int k,
+                                            int N, float inv_n, cx<float> &Zk, cx<float> &Zm) {
+  const cx<float> Z = Zk, M = Zm;
+  const cx<float> R[2] = {mk(0.5f * (Z.x + M.x), 0.5f * (Z.y - M.y)), mk(0.5f * (Z.y + M.y), 0.5f * (M.x - Z.x))};
+  const TraceParams *P[2] = {&pa, &pb};
+  cx<float> S[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const TraceParams &p = *P[h];
+    if (p.inside != 0.0f) S[h] = scl(filt(R[h], stretched(W, k, p.sf, p.ishift, p.gain, N)), inv_n);
+    else S[h] = scl(R[h], inv_n);
+  }
+  if (k == (N - k) % N) {
+    Zk = mk(S[0].x, S[1].x);
+  } else {
+    Zk = mk(S[0].x - S[1].y, S[0].y + S[1].x);
+    Zm = mk(S[0].x + S[1].y, S[1].x - S[0].y);
+  }
+}
+
+// Fused forward model of TX complex columns (2 TX traces) for one angle,
+// z length N specialised.  Grid: column blocks x angles.  LDS: twiddles,
+// the angle's wavelet half spectrum, the traces' local parameters, the
+// pencils.
+template <int N, int TX, int THREADS>
+__global__ void __launch_bounds__(THREADS) k_fm_fused(FwdArgs a, const cx<float> *__restrict__ twg) {
+  // the angles of one column block are consecutive work ids on one XCD, so
+  // their parameter reads after the first come from that XCD's L2
+  const unsigned W = xcd_work_id(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const unsigned bx = W / (unsigned)a.nt;
+  const int l = (int)(W % (unsigned)a.nt);
+  const int cn = a.rnxp / 2;
+  const long long ncol = (long long)cn * a.nyp;
+  const long long col0 = (long long)bx * TX;
+  const int ncb = (int)min((long long)TX, ncol - col0);
+  constexpr int NK = N / 2 + 1;
+  constexpr int PITCH = SP<N, float>::pitch;
+  extern __shared__ unsigned char smem_raw[];
+  cx<float> *tw = reinterpret_cast<cx<float> *>(smem_raw);
+  cx<float> *wsp = tw + N;
+  TraceParams *tp = reinterpret_cast<TraceParams *>(wsp + NK);
+  cx<float> *buf = reinterpret_cast<cx<float> *>(tp + 2 * TX);
+  for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
+  for (int q = threadIdx.x; q < NK; q += THREADS) wsp[q] = a.W[(long long)l * NK + q];
+  if (threadIdx.x < 2 * TX) {
+    const int c = threadIdx.x >> 1, h = threadIdx.x & 1;
+    const long long q = col0 + c;
+    const int i = (int)(q % cn) * 2 + h, j = (int)(q / cn);
+    TraceParams p{1.0f, 0.0f, 1.0f, 0.0f};
+    if (c < ncb && i < a.nx && j < a.ny) {
+      const long long t = (long long)j * a.nx + i;
+      p.inside = 1.0f;
+      if (a.sf[l]) p.sf = a.sf[l][t];
+      if (a.shift[l]) p.ishift = a.shift[l][t] / a.dz;
+      if (a.gain[l]) p.gain = a.gain[l][t];
+    }
+    tp[threadIdx.x] = p;
+  }
+  const cx<float> *al = reinterpret_cast<const cx<float> *>(a.alpha);
+  const cx<float> *be = reinterpret_cast<const cx<float> *>(a.beta);
+  const cx<float> *rh = reinterpret_cast<const cx<float> *>(a.rho);
+  const auto imp2 = [&](long long o) -> cx<float> {
+    const cx<float> x = al[o], y = be[o], z = rh[o];
+    return mk(impedance(a.A[l], x.x, y.x, z.x), impedance(a.A[l], x.y, y.y, z.y));
+  };
+  // 1. impedance of samples k < nz into LDS (trace 2m = Re, 2m+1 = Im); the
+  // trip count is a constant so every thread's loads are in flight at once
+  constexpr int TRIPS = (TX * N + THREADS - 1) / THREADS;
+  cx<float> r[TRIPS];
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    const int e = q / TX, c = q % TX;
+    r[t] = mk(0.0f, 0.0f);
+    if (e < a.nz && c < ncb) r[t] = imp2((long long)e * ncol + col0 + c);
+  }
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    if (q / TX < a.nz) buf[(q % TX) * PITCH + padx<float>(q / TX)] = r[t];
+  }
+  __syncthreads();
+  // 2. taper to nzp and forward difference (Wavelet1D FIRSTORDERFORWARDDIFF)
+  const float fac = 1.0f / (float)(a.nzp - a.nz - 1);
+  const auto imp_at = [&](int c, int e) -> cx<float> {
+    if (e < a.nz) return buf[c * PITCH + padx<float>(e)];
+    const cx<float> i0 = buf[c * PITCH + padx<float>(0)], il = buf[c * PITCH + padx<float>(a.nz - 1)];
+    const float u = (float)(e - a.nz), w = (float)(a.nzp - e - 1);
+    return mk(fac * (u * i0.x + w * il.x), fac * (u * i0.y + w * il.y));
+  };
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    if ((TX * N) % THREADS == 0 || q < TX * N) {
+      const int e = q / TX, c = q % TX;
+      r[t] = sub(imp_at(c, e == N - 1 ? 0 : e + 1), imp_at(c, e));
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    if ((TX * N) % THREADS == 0 || q < TX * N) buf[(q % TX) * PITCH + padx<float>(q / TX)] = r[t];
+  }
+  __syncthreads();
+  // 3. forward transform of every column
+  spasses<N, TX, THREADS, -1, 0, float>(buf, tw);
+  // 4. filter, bin pairs (k, N-k)
+  constexpr int FTRIPS = (TX * NK + THREADS - 1) / THREADS;
+#pragma unroll
+  for (int t = 0; t < FTRIPS; ++t) {
+    const int q = (int)threadIdx.x + t * THREADS;
+    if ((TX * NK) % THREADS == 0 || q < TX * NK) {
+      const int k = q / TX, c = q % TX;
+      cx<float> *col = buf + c * PITCH;
+      const int km = (N - k) % N;
+      cx<float> zk = col[padx<float>(k)], zm = col[padx<float>(km)];
+      filter_pair(wsp, tp[2 * c], tp[2 * c + 1], k, N, a.inv_n, zk, zm);
+      col[padx<float>(k)] = zk;
+      if (k != km) col[padx<float>(km)] = zm;
+    }
+  }
+  __syncthreads();
+  // 5. inverse transform, the last pass storing the seismic; padding traces
+  // receive their impedance
+  float *out = a.out[l];
+  const auto st = [&](int c, int e, cx<float> v) {
+    if (c >= ncb) return;
+    const long long o = (long long)e * ncol + col0 + c;
+    const bool ia = tp[2 * c].inside != 0.0f, ib = tp[2 * c + 1].inside != 0.0f;
+    if (!ia || !ib) {
+      const cx<float> p = imp2(o);
+      if (!ia) v.x = p.x;
+      if (!ib) v.y = p.y;
+    }
+    reinterpret_cast<cx<float> *>(out)[o] = v;
+  };
+  spasses_io<N, TX, THREADS, 1, 0, false, true, float>(buf, tw, LdsIo{}, st);
+}
+
+template <int N, int TX, int TH>
+static int launch_fused_v(const FwdArgs &a, const void *tw, hipStream_t s) {
+  const size_t lds = sizeof(cx<float>) * ((size_t)N + (size_t)(N / 2 + 1) + (size_t)TX * SP<N, float>::pitch) +
+                     sizeof(TraceParams) * 2 * TX;
+  if (lds > 65536)
+    CRAVA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_fm_fused<N, TX, TH>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const long long ncol = (long long)(a.rnxp / 2) * a.nyp;
+  const unsigned gx = (unsigned)((ncol + TX - 1) / TX);
+  hipLaunchKernelGGL((k_fm_fused<N, TX, TH>), dim3(gx, (unsigned)a.nt), dim3(TH), lds, s, a,
+                     reinterpret_cast<const cx<float> *>(tw));
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
+// 16 columns (32 traces) per workgroup of 512 threads: the fastest of
+// {16, 8, 4} columns x {128, 256, 512} threads measured at 512x512x320.
+template <int N>
+static int launch_fused_n(const FwdArgs &a, const void *tw, hipStream_t s) {
+  return launch_fused_v<N, 16, 512>(a, tw, s);
+}
+
+bool fm_fused_available(int nzp) { return static_fft_available(nzp); }
+
+int launch_fm_fused(const FwdArgs &a, const void *tw, hipStream_t s) {
+  switch (a.nzp) {
+#define CRAVA_FM_N(n) \
+  case n: return launch_fused_n<n>(a, tw, s);
+    CRAVA_FM_N(64) CRAVA_FM_N(80) CRAVA_FM_N(128) CRAVA_FM_N(160) CRAVA_FM_N(216) CRAVA_FM_N(256)
+    CRAVA_FM_N(288) CRAVA_FM_N(320) CRAVA_FM_N(512)
+#undef CRAVA_FM_N
+    default: set_error("no fused forward-modelling kernel for this nzp"); return CRAVA_HIP_E_INVALID;
   }
 }
 

@@ -22,5 +22,9 @@ struct FwdArgs {
 int launch_fm_reflect(const FwdArgs &a, hipStream_t s);
 int launch_fm_filter(const FwdArgs &a, hipStream_t s);
 int launch_fm_padding(const FwdArgs &a, hipStream_t s);
+// One fused kernel for the whole model when nzp has a specialised transform;
+// tw is the plan's per-pass twiddle table of length nzp (float).
+bool fm_fused_available(int nzp);
+int launch_fm_fused(const FwdArgs &a, const void *tw, hipStream_t s);
 
 }  // namespace crava
