@@ -113,6 +113,7 @@ class Pipeline:
     """Working grids + posterior handle for one precision form."""
 
     def __init__(self, ctx, prob, params, tables, fp64: bool):
+        import numpy as np
         import torch
         from crava_amd import _lib as L
         from crava_amd.core import Posterior
@@ -125,6 +126,11 @@ class Pipeline:
         self.ec0 = prob.errcorr.to(dt).clone()
         self.ec = self.ec0.clone()
         self.post = Posterior(ctx, params, tables)
+        # the prior covariance fill (SeismicParametersHolder::initializeCorrelations)
+        pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+        self.cov_scales = [float(np.float32(prob.sigma0[a, b])) for a, b in pairs]
+        self.corr_xy = torch.from_numpy(np.ascontiguousarray(prob.corr_xy, dtype=np.float32)).cuda()
+        self.circ_t = torch.from_numpy(np.ascontiguousarray(prob.circ_corr_t, dtype=np.float32)).cuda()
         self.precision = L.F64 if fp64 else L.MIXED
         self.cov_precision = L.F64 if fp64 else L.MIXED
         self.L = L
@@ -143,17 +149,18 @@ class Pipeline:
         """One pass of the path; each group of same-kind grids is one launch per transform pass."""
         L, c = self.L, self.prob.cfg
         n = (c.nxp, c.nyp, c.nzp)
-        # reload the prior covariance and error-correlation grids: every step
-        # then solves the same fresh problem (the FP64 point count depends on
-        # them; the previous step's means and residuals as inputs do not
-        # change the work)
+        # fill the prior covariance and error-correlation grids afresh
+        # (initializeCorrelations, one write per grid): every step then solves
+        # the same problem (the FP64 point count depends on them; the previous
+        # step's means and residuals as inputs do not change the work).  The
+        # FP64 form copies its double grids back.
         if self.fp64:
             for d, s_ in zip(self.cov + [self.ec], self.cov0 + [self.ec0]):
                 d.copy_(s_)
         else:
-            from crava_amd.core import grid_op
-            for d, s_ in zip(self.cov + [self.ec], self.cov0 + [self.ec0]):
-                grid_op(self.ctx, L.OP_COPY, d, s_, *n, False)
+            from crava_amd.core import fill_err_corr, fill_param_corr
+            fill_param_corr(self.ctx, self.cov, self.cov_scales, *n, self.corr_xy, self.circ_t)
+            fill_err_corr(self.ctx, self.ec, *n, self.corr_xy)
         if self.batch is not None:
             self.batch.run(graph=True)
             if counts:

@@ -650,7 +650,11 @@ int Fft3D::autotune(void *const *grids, int ng, hipStream_t s, int reps, float *
     int bestv = var[axis];
     for (int v = -kNumGenericVariants; v < nv; ++v) {
       if (set_variant(axis, v) != CRAVA_HIP_OK) continue;
-      auto run = [&]() { return pass(axis, grids, ng, false, nullptr, s); };
+      // forward and inverse pass together: one variant serves both directions
+      auto run = [&]() {
+        int r = pass(axis, grids, ng, false, nullptr, s);
+        return r ? r : pass(axis, grids, ng, true, nullptr, s);
+      };
       if ((rc = run())) break;  // warm-up
       if (hipEventRecord(e0, s) != hipSuccess) { rc = CRAVA_HIP_E_HIP; break; }
       for (int r = 0; r < reps && rc == CRAVA_HIP_OK; ++r) rc = run();

@@ -99,7 +99,8 @@ class Fft3D {
   int var[3] = {-1, -1, -1};
   int set_variant(int axis, int v);
   int num_variants(int axis) const;  // specialised variants available
-  // Time all variants on the set `grids` (overwritten) and keep the fastest per axis.
+  // Time all variants (forward + inverse pass) on the set `grids` (overwritten)
+  // and keep the fastest per axis.
   int autotune(void *const *grids, int ng, hipStream_t s, int reps, float *ms_out);
 
  private:

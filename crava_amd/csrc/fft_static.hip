@@ -41,12 +41,11 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long 
   cx<SO> *obase = dst + off;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
   if constexpr (DIO) {
+    // branch-free loads (clamped address, then select) so the compiler can
+    // issue a thread's loads back to back before converting any of them
     const auto ld = [&](int c, int e) -> cx<T> {
-      if (c < ncol) {
-        const cx<SI> v = ibase[c + (long long)e * es];
-        return mk((T)v.x, (T)v.y);
-      }
-      return mk((T)0, (T)0);
+      const cx<SI> v = ibase[min(c, ncol - 1) + (long long)e * es];
+      return c < ncol ? mk((T)v.x, (T)v.y) : mk((T)0, (T)0);
     };
     const auto st = [&](int c, int e, cx<T> v) {
       if (c < ncol) obase[c + (long long)e * es] = mk((SO)v.x, (SO)v.y);
@@ -101,12 +100,12 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nro
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
   static_assert(N % 2 == 0, "row kernels need an even length");
   if constexpr (DIO) {
+    // branch-free loads (clamped rows, then select), as in the strided kernel
     const auto ld = [&](int c, int e) -> cx<T> {
       const long long ra = row0 + 2 * c;
-      SI a = (SI)0, b = (SI)0;
-      if (ra < nrows) a = src[ra * rnxp + e];
-      if (ra + 1 < nrows) b = src[(ra + 1) * rnxp + e];
-      return mk((T)a, (T)b);
+      const SI a = src[min(ra, nrows - 1) * rnxp + e];
+      const SI b = src[min(ra + 1, nrows - 1) * rnxp + e];
+      return mk(ra < nrows ? (T)a : (T)0, ra + 1 < nrows ? (T)b : (T)0);
     };
     spasses_io<N, TX, THREADS, -1, 0, false, false, T>(buf, tw, ld, LdsIo{});
   } else {
@@ -177,9 +176,10 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nro
       const int kk = mirror ? N - k : k;
       const bool selfc = (kk == 0) || (kk == N / 2);
       const long long ra = row0 + 2 * c;
-      cx<SI> a = mk((SI)0, (SI)0), b = mk((SI)0, (SI)0);
-      if (ra < nrows) a = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[kk];
-      if (ra + 1 < nrows) b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[kk];
+      cx<SI> a = reinterpret_cast<const cx<SI> *>(src + min(ra, nrows - 1) * rnxp)[kk];
+      cx<SI> b = reinterpret_cast<const cx<SI> *>(src + min(ra + 1, nrows - 1) * rnxp)[kk];
+      if (ra >= nrows) a = mk((SI)0, (SI)0);
+      if (ra + 1 >= nrows) b = mk((SI)0, (SI)0);
       if (selfc) { a.y = (SI)0; b.y = (SI)0; }
       if (mirror) { a.y = -a.y; b.y = -b.y; }
       return mk((T)a.x - (T)b.y, (T)a.y + (T)b.x);

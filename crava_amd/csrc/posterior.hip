@@ -154,8 +154,9 @@ template <typename D, typename S_>
 __device__ __forceinline__ cx<D> cvt(cx<S_> a) { return mk((D)a.x, (D)a.y); }
 
 // MODE 0: arithmetic in G.  MODE 1 (G = float): FP32, except points whose
-// noise/signal ratio tr(E)/tr(K S K^H) is below thr, which are solved
-// end-to-end in FP64 from the FP32 inputs and rounded once at the end.
+// noise/signal ratio tr(E)/tr(K S K^H) is below thr, and the other points of
+// their 64-lane wave, which are solved end-to-end in FP64 from the FP32
+// inputs and rounded once at the end.
 template <typename G, int NT, int MODE>
 __device__ __forceinline__ void post_point(const PostArgs<G> &a, int k, long long ip) {
   const bool live = ip < a.plane;
@@ -208,7 +209,9 @@ __device__ __forceinline__ void post_point(const PostArgs<G> &a, int k, long lon
       sig += tb->P[l * (l + 1) / 2 + l].x * (b0 * A[l][0] + b1 * A[l][1] + b2 * A[l][2]);
     }
     const G noise = lam * tb->trEc + tb->trEd;
-    use64 = noise < (G)a.thr * sig;
+    // one path per wave: when any of its points needs FP64 the whole wave
+    // solves in FP64, instead of running both paths one after the other
+    use64 = __ballot(noise < (G)a.thr * sig) != 0ull;
   }
 
   bool ok;

@@ -104,8 +104,9 @@ int crava_hip_fft_pass_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, i
                             int inverse, int precision, double scale);
 
 /* Autotuner: time every launch variant of each pass of the nxp x nyp x nzp
- * transform on `scratch` (a grid of that size; its contents are destroyed)
- * and keep the fastest in the context's plan.  Variants: v >= 0 a
+ * transform (forward and inverse pass together) on `scratch` (a grid of that
+ * size; its contents are destroyed) and keep the fastest in the context's
+ * plan; ms[] receives the forward + inverse time per axis.  Variants: v >= 0 a
  * compile-time specialised kernel (lengths 64, 80, 128, 160, 216, 256, 288,
  * 320, 512): 0..7 for the strided y/z passes, 0..5 for the x (row) pass, the
  * upper half of each range loading the first radix pass straight from HBM
@@ -139,7 +140,8 @@ typedef struct {
   double sigma0[9];            /* prior 3x3 covariance Sigma0, row-major (ln Vp, ln Vs, ln rho) */
   float A[8 * 3];              /* reflection matrix [ntheta][3] (ModelAVODynamic A_) */
   double err_theta_cov[8 * 8]; /* Sigma_e [ntheta][ntheta] (Crava::errThetaCov_) */
-  double mixed_threshold;      /* CRAVA_HIP_MIXED: FP64 where tr(E)/tr(KSK^H) < this */
+  double mixed_threshold;      /* CRAVA_HIP_MIXED: FP64 where tr(E)/tr(KSK^H) < this (for the
+                                  whole 64-point wave such a point belongs to) */
 } crava_hip_post_params;
 
 /* Upload the per-k tables of one posterior problem.  The per-k vectors are
