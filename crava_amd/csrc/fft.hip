@@ -115,22 +115,19 @@ __global__ void __launch_bounds__(1024) k_fft_strided(GridSet gs, long long es, 
   cx<SO> *obase = dst + off;
   for (int q = threadIdx.x; q < pl.n; q += nthr) tw[q] = twg[q];
   const int tot = pl.tx * pl.n;
-  // kUnroll loads in flight per thread before they are stored to LDS
+  // kUnroll loads in flight per thread before they are stored to LDS: the
+  // loads are branch-free (clamped addresses, validity kept aside) and stay
+  // in their storage type until the LDS store, so none waits on another
   for (int q0 = threadIdx.x; q0 < tot; q0 += kUnroll * nthr) {
     cx<SI> v[kUnroll];
     int slot[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const int q = q0 + u * nthr;
-      slot[u] = -1;
-      if (q < tot) {
-        const int e = fdiv(q, pl.tx_mul);
-        const int c = q - e * pl.tx;
-        if (c < ncol) {
-          v[u] = ibase[c + e * es];
-          slot[u] = c * pl.ldsp + gpadx(e);
-        }
-      }
+      const int q = min(q0 + u * nthr, tot - 1);
+      const int e = fdiv(q, pl.tx_mul);
+      const int c = q - e * pl.tx;
+      v[u] = ibase[min(c, ncol - 1) + e * es];
+      slot[u] = (q0 + u * nthr < tot && c < ncol) ? c * pl.ldsp + gpadx(e) : -1;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u)
@@ -175,24 +172,25 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nro
   // store per lane
   const int tot = pl.tx * n;
   for (int q0 = threadIdx.x; q0 < tot; q0 += kUnroll * nthr) {
-    T va[kUnroll], vb[kUnroll];
+    SI va[kUnroll], vb[kUnroll];
     int slot[kUnroll];
+    unsigned okb = 0;
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const int q = q0 + u * nthr;
-      slot[u] = -1;
-      if (q < tot) {
-        const int c = fdiv(q, pl.n_mul);
-        const int e = q - c * n;
-        const long long ra = row0 + 2 * c;
-        va[u] = ra < nrows ? (T)src[ra * rnxp + e] : (T)0;
-        vb[u] = ra + 1 < nrows ? (T)src[(ra + 1) * rnxp + e] : (T)0;
-        slot[u] = c * pl.ldsp + gpadx(e);
-      }
+      const int q = min(q0 + u * nthr, tot - 1);
+      const int c = fdiv(q, pl.n_mul);
+      const int e = q - c * n;
+      const long long ra = row0 + 2 * c;
+      va[u] = src[min(ra, nrows - 1) * rnxp + e];
+      vb[u] = src[min(ra + 1, nrows - 1) * rnxp + e];
+      okb |= (ra < nrows ? 1u : 0u) << (2 * u);
+      okb |= (ra + 1 < nrows ? 1u : 0u) << (2 * u + 1);
+      slot[u] = q0 + u * nthr < tot ? c * pl.ldsp + gpadx(e) : -1;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u)
-      if (slot[u] >= 0) buf[slot[u]] = mk(va[u], vb[u]);
+      if (slot[u] >= 0)
+        buf[slot[u]] = mk((okb >> (2 * u)) & 1u ? (T)va[u] : (T)0, (okb >> (2 * u + 1)) & 1u ? (T)vb[u] : (T)0);
   }
   __syncthreads();
   run_passes<-1>(buf, tw, pl);
@@ -240,22 +238,20 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nro
   for (int q0 = threadIdx.x; q0 < tot; q0 += kUnroll * nthr) {
     cx<SI> a[kUnroll], b[kUnroll];
     int slot[kUnroll];
+    unsigned okb = 0;
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const int q = q0 + u * nthr;
-      slot[u] = -1;
-      if (q < tot) {
-        const int c = fdiv(q, pl.n_mul);
-        const int k = q - c * n;
-        const bool mirror = k >= cn;
-        const int kk = mirror ? n - k : k;
-        const long long ra = row0 + 2 * c;
-        a[u] = mk((SI)0, (SI)0);
-        b[u] = mk((SI)0, (SI)0);
-        if (ra < nrows) a[u] = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[kk];
-        if (ra + 1 < nrows) b[u] = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[kk];
-        slot[u] = c * pl.ldsp + gpadx(k);
-      }
+      const int q = min(q0 + u * nthr, tot - 1);
+      const int c = fdiv(q, pl.n_mul);
+      const int k = q - c * n;
+      const bool mirror = k >= cn;
+      const int kk = mirror ? n - k : k;
+      const long long ra = row0 + 2 * c;
+      a[u] = reinterpret_cast<const cx<SI> *>(src + min(ra, nrows - 1) * rnxp)[kk];
+      b[u] = reinterpret_cast<const cx<SI> *>(src + min(ra + 1, nrows - 1) * rnxp)[kk];
+      okb |= (ra < nrows ? 1u : 0u) << (2 * u);
+      okb |= (ra + 1 < nrows ? 1u : 0u) << (2 * u + 1);
+      slot[u] = q0 + u * nthr < tot ? c * pl.ldsp + gpadx(k) : -1;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -265,6 +261,8 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nro
       const int kk = mirror ? n - k : k;
       const bool selfc = (kk == 0) || ((n & 1) == 0 && kk == n / 2);
       cx<SI> x = a[u], y = b[u];
+      if (!((okb >> (2 * u)) & 1u)) x = mk((SI)0, (SI)0);
+      if (!((okb >> (2 * u + 1)) & 1u)) y = mk((SI)0, (SI)0);
       if (selfc) { x.y = (SI)0; y.y = (SI)0; }
       if (mirror) { x.y = -x.y; y.y = -y.y; }
       buf[slot[u]] = mk((T)x.x - (T)y.y, (T)x.y + (T)y.x);
