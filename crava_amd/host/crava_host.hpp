@@ -119,6 +119,20 @@ class FFTGrid {
   void *devicePtr() const { return dev_; }
   Context *context() const { return ctx_; }
 
+  // CRAVA binary grid file (src/fftgrid.cpp:2399-2487): a text line
+  // "crava_fftgrid_binary", the simbox geometry below, rnxp, nyp, nzp and
+  // the rsize padded real values, all big-endian.
+  struct CravaHeader {
+    double x0 = 0.0, y0 = 0.0, dx = 1.0, dy = 1.0;
+    int nx = 0, ny = 0;
+    double il0 = 0.0, xl0 = 0.0, ilStepX = 1.0, ilStepY = 0.0, xlStepX = 0.0, xlStepY = 1.0, angle = 0.0;
+  };
+  // writes fileName + ".crava" (IO::SuffixCrava); throws Error on I/O failure
+  void writeCravaFile(const std::string &fileName, const CravaHeader &h) const;
+  // reads a file written by writeCravaFile; a failure (unreadable file,
+  // other grid dimensions) is appended to errText and leaves the grid as it was
+  void readCravaFile(const std::string &fileName, std::string &errText, bool nopadding = false);
+
  private:
   int abiCubeType() const;
   void op(int code, const FFTGrid *other, float scalar);

@@ -2,7 +2,10 @@
 #include "crava_host.hpp"
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <fstream>
+#include <stdexcept>
 
 namespace crava_hip {
 
@@ -195,6 +198,91 @@ fftw_complex_f FFTGrid::getComplexValue(int i, int j, int k, bool extSimbox) con
   if (!in || i < 0 || j < 0 || k < 0) return fftw_complex_f{kRMissing, 0.0f};
   const size_t o = (size_t)i + (size_t)cnxp_ * ((size_t)j + (size_t)nyp_ * k);
   return fftw_complex_f{host_[2 * o], host_[2 * o + 1]};
+}
+
+// ---------------------------------------------------------------------------
+// CRAVA binary grid files (src/fftgrid.cpp:2399-2487, NRLib big-endian
+// binary numbers)
+namespace {
+
+template <typename T, typename U>
+void put_be(std::ostream &os, T v) {
+  U u;
+  std::memcpy(&u, &v, sizeof(T));
+  unsigned char b[sizeof(T)];
+  for (size_t i = 0; i < sizeof(T); ++i) b[i] = (unsigned char)(u >> (8 * (sizeof(T) - 1 - i)));
+  os.write(reinterpret_cast<const char *>(b), sizeof(T));
+}
+
+template <typename T, typename U>
+T get_be(std::istream &is) {
+  unsigned char b[sizeof(T)];
+  if (!is.read(reinterpret_cast<char *>(b), sizeof(T))) throw std::runtime_error("unexpected end of file");
+  U u = 0;
+  for (size_t i = 0; i < sizeof(T); ++i) u = (U)((u << 8) | b[i]);
+  T v;
+  std::memcpy(&v, &u, sizeof(T));
+  return v;
+}
+
+}  // namespace
+
+void FFTGrid::writeCravaFile(const std::string &fileName, const CravaHeader &h) const {
+  const std::string fname = fileName + ".crava";
+  std::vector<float> v((size_t)rsize_);
+  check(crava_hip_grid_download(ctx_->handle(), v.data(), dev_, sizeof(float) * rsize_), "writeCravaFile");
+  std::ofstream os(fname, std::ios::out | std::ios::binary);
+  if (!os) throw Error(CRAVA_HIP_E_INVALID, "cannot open " + fname + " for writing");
+  os << "crava_fftgrid_binary" << "\n";
+  for (double d : {h.x0, h.y0, h.dx, h.dy}) put_be<double, uint64_t>(os, d);
+  put_be<int32_t, uint32_t>(os, h.nx);
+  put_be<int32_t, uint32_t>(os, h.ny);
+  for (double d : {h.il0, h.xl0, h.ilStepX, h.ilStepY, h.xlStepX, h.xlStepY, h.angle}) put_be<double, uint64_t>(os, d);
+  put_be<int32_t, uint32_t>(os, rnxp_);
+  put_be<int32_t, uint32_t>(os, nyp_);
+  put_be<int32_t, uint32_t>(os, nzp_);
+  std::vector<unsigned char> buf(v.size() * 4);
+  for (size_t i = 0; i < v.size(); ++i) {
+    uint32_t u;
+    std::memcpy(&u, &v[i], 4);
+    buf[4 * i] = (unsigned char)(u >> 24);
+    buf[4 * i + 1] = (unsigned char)(u >> 16);
+    buf[4 * i + 2] = (unsigned char)(u >> 8);
+    buf[4 * i + 3] = (unsigned char)u;
+  }
+  os.write(reinterpret_cast<const char *>(buf.data()), (std::streamsize)buf.size());
+  if (!os) throw Error(CRAVA_HIP_E_INVALID, "write failed on " + fname);
+}
+
+void FFTGrid::readCravaFile(const std::string &fileName, std::string &errText, bool) {
+  try {
+    std::ifstream is(fileName, std::ios::in | std::ios::binary);
+    if (!is) throw std::runtime_error("cannot open '" + fileName + "'");
+    std::string type;
+    std::getline(is, type);
+    for (int i = 0; i < 4; ++i) (void)get_be<double, uint64_t>(is);
+    for (int i = 0; i < 2; ++i) (void)get_be<int32_t, uint32_t>(is);
+    for (int i = 0; i < 7; ++i) (void)get_be<double, uint64_t>(is);
+    const int rnxp = get_be<int32_t, uint32_t>(is);
+    const int nyp = get_be<int32_t, uint32_t>(is);
+    const int nzp = get_be<int32_t, uint32_t>(is);
+    if (rnxp != rnxp_ || nyp != nyp_ || nzp != nzp_)
+      throw std::runtime_error("Grid dimension is wrong for file '" + fileName + "'.");
+    std::vector<unsigned char> buf((size_t)rsize_ * 4);
+    if (!is.read(reinterpret_cast<char *>(buf.data()), (std::streamsize)buf.size()))
+      throw std::runtime_error("unexpected end of file '" + fileName + "'");
+    std::vector<float> v((size_t)rsize_);
+    for (size_t i = 0; i < v.size(); ++i) {
+      const uint32_t u = ((uint32_t)buf[4 * i] << 24) | ((uint32_t)buf[4 * i + 1] << 16) |
+                         ((uint32_t)buf[4 * i + 2] << 8) | (uint32_t)buf[4 * i + 3];
+      std::memcpy(&v[i], &u, 4);
+    }
+    createRealGrid();
+    check(crava_hip_grid_upload(ctx_->handle(), dev_, v.data(), sizeof(float) * rsize_), "readCravaFile");
+    istransformed_ = false;
+  } catch (const std::exception &e) {
+    errText += std::string("Error: ") + e.what() + "\n";
+  }
 }
 
 }  // namespace crava_hip

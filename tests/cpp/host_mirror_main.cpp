@@ -161,6 +161,35 @@ int main(int argc, char **argv) {
       delete syn[l];
     }
 
+    // CRAVA binary files (FFTGrid::writeCravaFile / readCravaFile): write the
+    // posterior mean of alpha, read the file the test wrote, and report a
+    // grid of other dimensions
+    crava_hip::FFTGrid::CravaHeader hdr;
+    hdr.x0 = 456000.5;
+    hdr.y0 = 6780000.25;
+    hdr.dx = 25.0;
+    hdr.dy = 12.5;
+    hdr.nx = s.nx;
+    hdr.ny = s.ny;
+    hdr.il0 = 1001.0;
+    hdr.xl0 = 2002.0;
+    hdr.ilStepX = 0.5;
+    hdr.ilStepY = -0.25;
+    hdr.xlStepX = 0.125;
+    hdr.xlStepY = 2.0;
+    hdr.angle = 0.3;
+    sph.mu(0)->writeCravaFile(dir + "/mu0", hdr);
+    {
+      FFTGrid rb(&ctx, s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
+      std::string err;
+      rb.readCravaFile(dir + "/in.crava", err);
+      write_bin(dir + "/crava_read.bin", drain_real(&rb));
+      FFTGrid other(&ctx, s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp + 2);
+      other.readCravaFile(dir + "/in.crava", err);
+      other.readCravaFile(dir + "/missing.crava", err);
+      std::ofstream(dir + "/crava_err.txt") << err;
+    }
+
     // a few cursor / random-access checks against the values just written
     FFTGrid *m0 = sph.mu(0);
     m0->setAccessMode(FFTGrid::RANDOMACCESS);

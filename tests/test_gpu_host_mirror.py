@@ -15,6 +15,7 @@ relative L2 inside the volume and bit-exact impedance in the padding.
 """
 import math
 import os
+import struct
 import subprocess
 
 import numpy as np
@@ -68,6 +69,15 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
         _write(os.path.join(d, f"mu{i}.bin"), g.numpy())
     for i, g in enumerate(prob.data):
         _write(os.path.join(d, f"data{i}.bin"), g.numpy())
+
+    # a CRAVA binary file as the reference writes it: text line, simbox
+    # geometry (4 doubles, 2 ints, 7 doubles), rnxp, nyp, nzp, the padded
+    # reals; NRLib writes big-endian
+    crava_in = np.random.default_rng(5).standard_normal(shape).astype(np.float32)
+    with open(os.path.join(d, "in.crava"), "wb") as f:
+        f.write(b"crava_fftgrid_binary\n")
+        f.write(struct.pack(">4d2i7d3i", 1.0, 2.0, 25.0, 25.0, c.nx, c.ny, 0, 0, 1, 0, 0, 1, 0.0, rnxp, nyp, nzp))
+        f.write(crava_in.astype(">f4").tobytes())
 
     r = subprocess.run([BIN, d], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -133,6 +143,19 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
     caa = np.abs(cv[0].view(np.complex64)[:, :, :cn].real)
     noise_level = int((caa <= 1e-5 * caa.max()).sum())
     assert simfail <= noise_level and sim_ref <= noise_level, (simfail, sim_ref, noise_level)
+    # CRAVA binary files
+    assert np.array_equal(_read(os.path.join(d, "crava_read.bin"), shape), crava_in)
+    raw = open(os.path.join(d, "mu0.crava"), "rb").read()
+    head = b"crava_fftgrid_binary\n"
+    assert raw.startswith(head)
+    fields = struct.unpack_from(">4d2i7d3i", raw, len(head))
+    assert fields == (456000.5, 6780000.25, 25.0, 12.5, c.nx, c.ny, 1001.0, 2002.0, 0.5, -0.25, 0.125, 2.0, 0.3,
+                      rnxp, nyp, nzp)
+    body = np.frombuffer(raw, dtype=">f4", offset=len(head) + struct.calcsize(">4d2i7d3i"))
+    assert np.array_equal(body.astype(np.float32).reshape(shape), _read(os.path.join(d, "out_mu0.bin"), shape))
+    err = open(os.path.join(d, "crava_err.txt")).read()
+    assert "Grid dimension is wrong" in err and "missing.crava" in err
+
     # computeSyntSeismic of the posterior means the mirror wrote
     means = [_read(os.path.join(d, f"out_mu{i}.bin"), shape) for i in range(3)]
     sf = np.fromfile(os.path.join(d, "stretch.bin"), dtype=np.float32).reshape(c.ny, c.nx)
