@@ -511,6 +511,52 @@ def forward_modelling(ctx, prob, reps: int, cpu: bool):
     return res
 
 
+def time_lapse_merge(ctx, reps: int, cpu: bool):
+    """Time-lapse merge (crava_hip_merge_cov_4d / _mean_4d, State4D::mergeCov /
+    merge) on 512x512x320 transformed grids: 6 + 6 + 9 covariance parts into
+    6 grids, 3 + 3 means into 3.  HIP events on the launch stream; the
+    oracle's restatement (a port: the reference's State4D is C++ that does
+    not build here) timed on one core on the first k-planes."""
+    import numpy as np
+    import torch
+    from crava_amd.core import merge_cov_4d, merge_mean_4d
+    nxp, nyp, nzp = 512, 512, 320
+    cn = nxp // 2 + 1
+    shape = (nzp, nyp, 2 * cn)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    parts = [torch.randn(shape, device="cuda", generator=gen) for _ in range(21)]
+    out = [torch.empty(shape, device="cuda") for _ in range(6)]
+    ms = [torch.randn(shape, device="cuda", generator=gen) for _ in range(6)]
+    mo = [torch.empty(shape, device="cuda") for _ in range(3)]
+    t_cov = timed_kernel(lambda: merge_cov_4d(ctx, nxp, nyp, nzp, parts[:6], parts[6:12], parts[12:], out), reps)
+    t_mu = timed_kernel(lambda: merge_mean_4d(ctx, nxp, nyp, nzp, ms[:3], ms[3:], mo), reps)
+    gb = 8 * cn * nyp * nzp
+    res = {"grid": [nxp, nyp, nzp],
+           "cov": {"ms": t_cov, "alg_bytes": 27 * gb, "GBps": 27 * gb / (t_cov * 1e-3) / 1e9},
+           "mean": {"ms": t_mu, "alg_bytes": 9 * gb, "GBps": 9 * gb / (t_mu * 1e-3) / 1e9},
+           "cpu_port": None}
+    res["cov"]["frac_of_peak"] = res["cov"]["GBps"] / PEAK_HBM_GBS
+    res["mean"]["frac_of_peak"] = res["mean"]["GBps"] / PEAK_HBM_GBS
+    log(f"time-lapse merge: covariance {t_cov:.3f} ms ({res['cov']['GBps']:.0f} GB/s), "
+        f"means {t_mu:.3f} ms ({res['mean']['GBps']:.0f} GB/s)")
+    if cpu:
+        try:
+            import oracle
+            kk = 8
+            h = [p[:kk].cpu().numpy().view(np.complex64) for p in parts]
+            t1 = time.perf_counter()
+            oracle.merge_cov_4d(h[:6], h[6:12], h[12:])
+            dt = time.perf_counter() - t1
+            res["cpu_port"] = {"seconds_per_call": dt * nzp / kk, "cores": 1, "kind": "port",
+                               "sample": f"the oracle's mergeCov over the first {kk} of {nzp} k-planes "
+                                         f"({dt:.2f} s), scaled"}
+        except Exception as e:
+            log(f"time-lapse merge cpu baseline failed: {e}")
+    del parts, out, ms, mo
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -696,6 +742,10 @@ def main():
     if args.diff_steps > 0:
         torch.cuda.empty_cache()
         diff = differentiation(ctx, args.diff_steps)
+    tl = None
+    if args.forward_reps > 0:
+        torch.cuda.empty_cache()
+        tl = time_lapse_merge(ctx, args.kernel_reps, not args.no_cpu)
     fwd = None
     if args.forward_reps > 0:
         torch.cuda.empty_cache()
@@ -746,6 +796,7 @@ def main():
         "batching": batch,
         "differentiation": diff,
         "forward_modelling": fwd,
+        "time_lapse_merge": tl,
     }
     print(json.dumps(out))
 

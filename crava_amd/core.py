@@ -275,3 +275,17 @@ def synthetic_seismic(ctx: Context, nx, ny, nz, nxp, nyp, nzp, A, wavelets, dz, 
                    table(stretch))
     check(lib().crava_hip_synthetic_seismic(ctx.h, C.byref(p), _dev_ptr(alpha), _dev_ptr(beta), _dev_ptr(rho),
                                             ptr_array([_dev_ptr(g) for g in out])), "crava_hip_synthetic_seismic")
+
+
+def merge_mean_4d(ctx: Context, nxp, nyp, nzp, mu_static, mu_dynamic, mu):
+    """crava_hip_merge_mean_4d on transformed float grids (lists of 3)."""
+    P = lambda gs: ptr_array([_dev_ptr(g) for g in gs])
+    check(lib().crava_hip_merge_mean_4d(ctx.h, nxp, nyp, nzp, P(mu_static), P(mu_dynamic), P(mu)),
+          "crava_hip_merge_mean_4d")
+
+
+def merge_cov_4d(ctx: Context, nxp, nyp, nzp, ss, dd, sd, sigma):
+    """crava_hip_merge_cov_4d on transformed float grids: ss[6], dd[6], sd[9] -> sigma[6]."""
+    P = lambda gs: ptr_array([_dev_ptr(g) for g in gs])
+    check(lib().crava_hip_merge_cov_4d(ctx.h, nxp, nyp, nzp, P(ss), P(dd), P(sd), P(sigma)),
+          "crava_hip_merge_cov_4d")

@@ -498,6 +498,50 @@ int crava_hip_facies(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int ny
 }
 
 // ---------------------------------------------------------------------------
+// Time-lapse merge
+static int merge_common(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, crava::Merge4D &a) {
+  if (!ctx || nxp < 1 || nyp < 1 || nzp < 1) { set_error("invalid arguments to the 4D merge"); return CRAVA_HIP_E_INVALID; }
+  a.n = (long long)(nxp / 2 + 1) * nyp * nzp;
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_merge_mean_4d(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, const void *const *mu_static,
+                            const void *const *mu_dynamic, void *const *mu) {
+  crava::Merge4D a{};
+  int rc = merge_common(ctx, nxp, nyp, nzp, a);
+  if (rc) return rc;
+  if (!mu_static || !mu_dynamic || !mu) { set_error("NULL grid list"); return CRAVA_HIP_E_INVALID; }
+  for (int l = 0; l < 3; ++l) {
+    if (!mu_static[l] || !mu_dynamic[l] || !mu[l]) { set_error("NULL grid"); return CRAVA_HIP_E_INVALID; }
+    a.ms[l] = static_cast<const crava::cx<float> *>(mu_static[l]);
+    a.md[l] = static_cast<const crava::cx<float> *>(mu_dynamic[l]);
+    a.mu[l] = static_cast<crava::cx<float> *>(mu[l]);
+  }
+  return crava::launch_merge_4d(a, false, ctx->stream);
+}
+
+int crava_hip_merge_cov_4d(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, const void *const *ss,
+                           const void *const *dd, const void *const *sd, void *const *sigma) {
+  crava::Merge4D a{};
+  int rc = merge_common(ctx, nxp, nyp, nzp, a);
+  if (rc) return rc;
+  if (!ss || !dd || !sd || !sigma) { set_error("NULL grid list"); return CRAVA_HIP_E_INVALID; }
+  for (int q = 0; q < 6; ++q) {
+    if (!ss[q] || !dd[q] || !sigma[q]) { set_error("NULL grid"); return CRAVA_HIP_E_INVALID; }
+    a.ss[q] = static_cast<const crava::cx<float> *>(ss[q]);
+    a.dd[q] = static_cast<const crava::cx<float> *>(dd[q]);
+    a.sig[q] = static_cast<crava::cx<float> *>(sigma[q]);
+  }
+  for (int q = 0; q < 9; ++q) {
+    if (!sd[q]) { set_error("NULL grid"); return CRAVA_HIP_E_INVALID; }
+    a.sd[q] = static_cast<const crava::cx<float> *>(sd[q]);
+  }
+  // each output point depends on the same point of the inputs only, so the
+  // outputs may alias ss or dd
+  return crava::launch_merge_4d(a, true, ctx->stream);
+}
+
+// ---------------------------------------------------------------------------
 // Forward modelling
 int crava_hip_synthetic_seismic(crava_hip_ctx *ctx, const crava_hip_synt_params *p, const void *alpha,
                                 const void *beta, const void *rho, void *const *out) {

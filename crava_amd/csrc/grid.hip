@@ -211,6 +211,54 @@ __global__ void k_facies_geomodel(GridSet probs, const float *__restrict__ undef
   }
 }
 
+// Time-lapse merge of the static and dynamic parts into the current state,
+// transformed grids (State4D::merge / mergeCov, src/state4d.cpp:90-236).
+// Mean: mu_l = mu_static_l + mu_dynamic_l.  Covariance, l <= m:
+//   Sigma_lm = ((F[l][m] + F[l+3][m+3]) + F[l+3][m]) + F[l][m+3],
+// F the 6x6 joint prior: F[a][b] = ss(a,b), F[3+a][3+b] = dd(a,b),
+// F[a][3+b] = sd[3a+b], F[3+b][a] = conj(sd[3a+b]); float, in the
+// reference's order.  One thread per complex point.
+__global__ void k_merge_mean_4d(Merge4D a) {
+  const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.n) return;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const cx<float> s = a.ms[l][p], d = a.md[l][p];
+    a.mu[l][p] = mk(s.x + d.x, s.y + d.y);
+  }
+}
+
+__global__ void k_merge_cov_4d(Merge4D a) {
+  const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.n) return;
+  // upper-triangle order of the six grids: (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)
+  const int ql[6] = {0, 0, 0, 1, 1, 2}, qm[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int l = ql[q], m = qm[q];
+    const cx<float> f0 = a.ss[q][p], f1 = a.dd[q][p];
+    const cx<float> g = a.sd[3 * m + l][p];          // F[m][l+3]; F[l+3][m] is its conjugate
+    const cx<float> f3 = a.sd[3 * l + m][p];         // F[l][m+3]
+    float re = f0.x;
+    re += f1.x;
+    re += g.x;
+    re += f3.x;
+    float im = f0.y;
+    im += f1.y;
+    im += -g.y;
+    im += f3.y;
+    a.sig[q][p] = mk(re, im);
+  }
+}
+
+int launch_merge_4d(const Merge4D &a, bool cov, hipStream_t s) {
+  const unsigned blocks = (unsigned)((a.n + 255) / 256);
+  if (cov) hipLaunchKernelGGL(k_merge_cov_4d, dim3(blocks), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_merge_mean_4d, dim3(blocks), dim3(256), 0, s, a);
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
 int launch_fill_param_corr(const GridSet &gs, int nxp, int nyp, int nzp, const float *corr_xy, const float *circ_t,
                            float grad_i, float grad_j, hipStream_t s) {
   const long long rsize = 2LL * (nxp / 2 + 1) * nyp * nzp;

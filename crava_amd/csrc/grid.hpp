@@ -11,6 +11,15 @@ int launch_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *co
                          hipStream_t s);
 // n floats (real domain) or n complex values (transformed).
 int launch_grid_op(int op, float *a, const float *b, long long n, bool transformed, float scalar, hipStream_t s);
+// Time-lapse merge (grid.hip): n complex points per grid.
+struct Merge4D {
+  const cx<float> *ms[3], *md[3];            // static / dynamic means
+  cx<float> *mu[3];                          // merged means
+  const cx<float> *ss[6], *dd[6], *sd[9];    // static-static, dynamic-dynamic, static-dynamic
+  cx<float> *sig[6];                         // merged covariance (upper triangle order)
+  long long n;
+};
+int launch_merge_4d(const Merge4D &a, bool cov, hipStream_t s);
 // Synchronous: out2 (host) = {sum of squares, count}; `work` device doubles of
 // data_variance_work_doubles(nyp, nzp).
 int data_variance(const float *grid, int nx, int ny, int nz, int nxp, int nyp, int nzp, double *work,

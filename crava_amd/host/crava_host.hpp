@@ -193,6 +193,26 @@ class SeismicParametersHolder {
   double priorVar0_[3][3] = {{0}};
 };
 
+// Static and dynamic parts of a time-lapse prior (State4D, src/state4d.h),
+// transformed grids owned by the caller:
+//   muStatic[3], muDynamic[3]
+//   sigmaStaticStatic[6], sigmaDynamicDynamic[6]  order aa, ab, ar, bb, br, rr
+//   sigmaStaticDynamic[9]                         [3a+b] = Cov(static a, dynamic b)
+class State4D {
+ public:
+  State4D(Context *ctx, std::vector<FFTGrid *> muStatic, std::vector<FFTGrid *> muDynamic,
+          std::vector<FFTGrid *> sigmaStaticStatic, std::vector<FFTGrid *> sigmaDynamicDynamic,
+          std::vector<FFTGrid *> sigmaStaticDynamic);
+  // src/state4d.cpp:90-159: the current state's means and covariances
+  void merge(SeismicParametersHolder &current);
+  // src/state4d.cpp:161-236: sigma in the order aa, ab, ar, bb, br, rr
+  void mergeCov(const std::vector<FFTGrid *> &sigma);
+
+ private:
+  Context *ctx_;
+  std::vector<FFTGrid *> ms_, md_, ss_, dd_, sd_;
+};
+
 // The posterior core of Crava on the constant-thickness path.
 class Crava {
  public:

@@ -160,4 +160,54 @@ void SeismicParametersHolder::invFFTAllGrids() {
   invFFTCovGrids();
 }
 
+// ---------------------------------------------------------------------------
+// State4D (src/state4d.cpp:90-236)
+State4D::State4D(Context *ctx, std::vector<FFTGrid *> muStatic, std::vector<FFTGrid *> muDynamic,
+                 std::vector<FFTGrid *> sigmaStaticStatic, std::vector<FFTGrid *> sigmaDynamicDynamic,
+                 std::vector<FFTGrid *> sigmaStaticDynamic)
+    : ctx_(ctx), ms_(std::move(muStatic)), md_(std::move(muDynamic)), ss_(std::move(sigmaStaticStatic)),
+      dd_(std::move(sigmaDynamicDynamic)), sd_(std::move(sigmaStaticDynamic)) {
+  if (ms_.size() != 3 || md_.size() != 3 || ss_.size() != 6 || dd_.size() != 6 || sd_.size() != 9)
+    throw Error(CRAVA_HIP_E_INVALID, "State4D needs 3 + 3 means and 6 + 6 + 9 covariance grids");
+}
+
+static void need_transformed(const std::vector<FFTGrid *> &g) {
+  for (const FFTGrid *x : g)
+    if (!x || !x->getIsTransformed()) throw Error(CRAVA_HIP_E_STATE, "State4D merges transformed grids");
+}
+
+void State4D::merge(SeismicParametersHolder &current) {
+  need_transformed(ms_);
+  need_transformed(md_);
+  FFTGrid *mu[3] = {current.GetMuAlpha(), current.GetMuBeta(), current.GetMuRho()};
+  void *s[3], *d[3], *o[3];
+  for (int l = 0; l < 3; ++l) {
+    s[l] = ms_[l]->devicePtr();
+    d[l] = md_[l]->devicePtr();
+    o[l] = mu[l]->devicePtr();
+  }
+  const FFTGrid *g = ms_[0];
+  check(crava_hip_merge_mean_4d(ctx_->handle(), g->getNxp(), g->getNyp(), g->getNzp(), s, d, o), "merge");
+  for (FFTGrid *m : mu) m->setIsTransformed(true);
+  mergeCov({current.GetCovAlpha(), current.GetCrCovAlphaBeta(), current.GetCrCovAlphaRho(), current.GetCovBeta(),
+            current.GetCrCovBetaRho(), current.GetCovRho()});
+}
+
+void State4D::mergeCov(const std::vector<FFTGrid *> &sigma) {
+  if (sigma.size() != 6) throw Error(CRAVA_HIP_E_INVALID, "mergeCov needs 6 grids");
+  need_transformed(ss_);
+  need_transformed(dd_);
+  need_transformed(sd_);
+  void *a[6], *b[6], *c[9], *o[6];
+  for (int q = 0; q < 6; ++q) {
+    a[q] = ss_[q]->devicePtr();
+    b[q] = dd_[q]->devicePtr();
+    o[q] = sigma[q]->devicePtr();
+  }
+  for (int q = 0; q < 9; ++q) c[q] = sd_[q]->devicePtr();
+  const FFTGrid *g = ss_[0];
+  check(crava_hip_merge_cov_4d(ctx_->handle(), g->getNxp(), g->getNyp(), g->getNzp(), a, b, c, o), "mergeCov");
+  for (FFTGrid *x : sigma) x->setIsTransformed(true);
+}
+
 }  // namespace crava_hip

@@ -333,6 +333,25 @@ int crava_hip_batch_counts(crava_hip_batch *batch, long long *n_fail, long long 
 
 int crava_hip_batch_destroy(crava_hip_batch *batch);
 
+/* ---- Time-lapse (4D) merge ---------------------------------------------- */
+
+/* Merge the static and dynamic parts of the prior into the current state's
+ * means, in place of the mean loop of State4D::merge (src/state4d.cpp:90-146):
+ * mu[l] = mu_static[l] + mu_dynamic[l], transformed grids (nxp/2+1 x nyp x
+ * nzp complex float each).  mu may alias either input.  Float, bit-exact. */
+int crava_hip_merge_mean_4d(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, const void *const *mu_static,
+                            const void *const *mu_dynamic, void *const *mu);
+
+/* The covariance merge of State4D::mergeCov (src/state4d.cpp:161-236), which
+ * Crava::computePostMeanResidAndFFTCov calls in place of FFTCovGrids when 4D
+ * is active (src/crava.cpp:992-1001): sigma = Sigma_ss + Sigma_dd +
+ * Sigma_sd + Sigma_sd^H, per point, in the reference's float order.
+ *   ss[6], dd[6]  upper triangles in the order aa, ab, ar, bb, br, rr
+ *   sd[9]         static-dynamic blocks, sd[3a+b] = Cov(static a, dynamic b)
+ *   sigma[6]      output, the order of ss (may alias ss or dd) */
+int crava_hip_merge_cov_4d(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, const void *const *ss,
+                           const void *const *dd, const void *const *sd, void *const *sigma);
+
 /* ---- Forward modelling ------------------------------------------------- */
 
 /* Synthetic seismic per angle from the elastic parameter grids, in place of

@@ -64,6 +64,8 @@ def orc():
         L.orc_data_variance.argtypes = [VP, I, I, I, I, I, I, C.POINTER(C.c_longlong)]
         L.orc_data_variance.restype = C.c_float
         L.orc_facies_geomodel.argtypes = [I, I, I, I, VP, PP]
+        L.orc_merge_mean_4d.argtypes = [LL, PP, PP, PP]
+        L.orc_merge_cov_4d.argtypes = [LL, PP, PP, PP, PP]
         L.orc_synthetic_seismic.argtypes = [I, I, I, I, I, I, I, VP, VP, F, PP, PP, PP, VP, VP, VP, PP]
         L.orc_posterior.restype = C.c_longlong
         L.orc_posterior_f64.restype = C.c_longlong
@@ -295,4 +297,24 @@ def synthetic_seismic(nx, ny, nz, nxp, nyp, nzp, A, wavelets, dz, alpha, beta, r
         ref().ref_synthetic_seismic(*args, i_range[0], i_range[1])
     else:
         orc().orc_synthetic_seismic(*args)
+    return out
+
+
+# ------------------------------------------------------------ time-lapse merge
+def merge_mean_4d(ms, md):
+    """State4D::merge mean loop on lists of 3 complex64 arrays; returns 3 arrays."""
+    ms = [np.ascontiguousarray(x, dtype=np.complex64) for x in ms]
+    md = [np.ascontiguousarray(x, dtype=np.complex64) for x in md]
+    out = [np.empty_like(ms[0]) for _ in range(3)]
+    orc().orc_merge_mean_4d(ms[0].size, _ptrs(ms), _ptrs(md), _ptrs(out))
+    return out
+
+
+def merge_cov_4d(ss, dd, sd):
+    """State4D::mergeCov on complex64 arrays: ss[6], dd[6] (aa ab ar bb br rr),
+    sd[9] (sd[3a+b]); returns the 6 merged grids."""
+    cv = lambda L_: [np.ascontiguousarray(x, dtype=np.complex64) for x in L_]
+    ss, dd, sd = cv(ss), cv(dd), cv(sd)
+    out = [np.empty_like(ss[0]) for _ in range(6)]
+    orc().orc_merge_cov_4d(ss[0].size, _ptrs(ss), _ptrs(dd), _ptrs(sd), _ptrs(out))
     return out

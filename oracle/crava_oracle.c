@@ -1223,3 +1223,61 @@ void orc_synthetic_seismic(int nx, int ny, int nz, int nxp, int nyp, int nzp, in
   }
   free(imp); free(r); free(R); free(W); free(L);
 }
+
+/* ========================================================================
+ * Time-lapse merge, State4D::merge mean loop (src/state4d.cpp:117-136) and
+ * State4D::mergeCov (src/state4d.cpp:161-236), n complex points per grid.
+ * ====================================================================== */
+void orc_merge_mean_4d(long long n, float **ms, float **md, float **mu)
+{
+  for (long long p = 0; p < n; p++) {
+    ocpx full[6], cur[3];
+    for (int l = 0; l < 3; l++) {
+      full[l].re = ms[l][2 * p]; full[l].im = ms[l][2 * p + 1];
+      full[l + 3].re = md[l][2 * p]; full[l + 3].im = md[l][2 * p + 1];
+    }
+    for (int l = 0; l < 3; l++) {
+      cur[l].re = full[l].re + full[l + 3].re;
+      cur[l].im = full[l].im + full[l + 3].im;
+    }
+    for (int l = 0; l < 3; l++) { mu[l][2 * p] = cur[l].re; mu[l][2 * p + 1] = cur[l].im; }
+  }
+}
+
+/* ss[6], dd[6]: upper triangles aa ab ar bb br rr; sd[9]: sd[3a+b]; sigma[6] */
+void orc_merge_cov_4d(long long n, float **ss, float **dd, float **sd, float **sigma)
+{
+  for (long long p = 0; p < n; p++) {
+    ocpx F[6][6], C[3][3];
+    int q = 0;
+    for (int a = 0; a < 3; a++)
+      for (int b = a; b < 3; b++, q++) {
+        F[a][b].re = ss[q][2 * p]; F[a][b].im = ss[q][2 * p + 1];
+        F[a + 3][b + 3].re = dd[q][2 * p]; F[a + 3][b + 3].im = dd[q][2 * p + 1];
+      }
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        F[a][b + 3].re = sd[3 * a + b][2 * p];
+        F[a][b + 3].im = sd[3 * a + b][2 * p + 1];
+      }
+    for (int l = 0; l < 6; l++)
+      for (int m = l + 1; m < 6; m++) {
+        F[m][l].re = F[l][m].re;
+        F[m][l].im = -F[l][m].im;
+      }
+    for (int l = 0; l < 3; l++)
+      for (int m = 0; m < 3; m++) {
+        C[l][m].re = F[l][m].re;
+        C[l][m].re += F[l + 3][m + 3].re;
+        C[l][m].re += F[l + 3][m].re;
+        C[l][m].re += F[l][m + 3].re;
+        C[l][m].im = F[l][m].im;
+        C[l][m].im += F[l + 3][m + 3].im;
+        C[l][m].im += F[l + 3][m].im;
+        C[l][m].im += F[l][m + 3].im;
+      }
+    q = 0;
+    for (int a = 0; a < 3; a++)
+      for (int b = a; b < 3; b++, q++) { sigma[q][2 * p] = C[a][b].re; sigma[q][2 * p + 1] = C[a][b].im; }
+  }
+}

@@ -190,6 +190,41 @@ int main(int argc, char **argv) {
       std::ofstream(dir + "/crava_err.txt") << err;
     }
 
+    // time-lapse merge (State4D::merge) when the test supplies the parts:
+    // 3 + 3 means and 6 + 6 + 9 covariance grids, transformed, in that order
+    {
+      std::ifstream probe(dir + "/state4d_in.bin", std::ios::binary);
+      if (probe) {
+        const size_t csz = 2 * (size_t)(s.nxp / 2 + 1) * s.nyp * s.nzp;
+        const std::vector<float> all = read_bin(dir + "/state4d_in.bin", 27 * csz);
+        std::vector<FFTGrid *> parts(27);
+        for (int g = 0; g < 27; ++g) {
+          parts[g] = new FFTGrid(&ctx, s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
+          parts[g]->createComplexGrid();
+          parts[g]->setAccessMode(FFTGrid::WRITE);
+          for (size_t i = 0; i < csz / 2; ++i)
+            parts[g]->setNextComplex(crava_hip::fftw_complex_f{all[g * csz + 2 * i], all[g * csz + 2 * i + 1]});
+          parts[g]->endAccess();
+        }
+        auto range = [&](int a, int b) { return std::vector<FFTGrid *>(parts.begin() + a, parts.begin() + b); };
+        crava_hip::State4D st(&ctx, range(0, 3), range(3, 6), range(6, 12), range(12, 18), range(18, 27));
+        st.merge(sph);
+        std::vector<float> out;
+        for (int c = 0; c < 3; ++c) {
+          const std::vector<float> v = drain_complex(sph.mu(c));
+          out.insert(out.end(), v.begin(), v.end());
+        }
+        FFTGrid *sig[6] = {sph.GetCovAlpha(), sph.GetCrCovAlphaBeta(), sph.GetCrCovAlphaRho(), sph.GetCovBeta(),
+                           sph.GetCrCovBetaRho(), sph.GetCovRho()};
+        for (FFTGrid *g : sig) {
+          const std::vector<float> v = drain_complex(g);
+          out.insert(out.end(), v.begin(), v.end());
+        }
+        write_bin(dir + "/state4d_out.bin", out);
+        for (FFTGrid *g : parts) delete g;
+      }
+    }
+
     // a few cursor / random-access checks against the values just written
     FFTGrid *m0 = sph.mu(0);
     m0->setAccessMode(FFTGrid::RANDOMACCESS);

@@ -79,6 +79,14 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
         f.write(struct.pack(">4d2i7d3i", 1.0, 2.0, 25.0, 25.0, c.nx, c.ny, 0, 0, 1, 0, 0, 1, 0.0, rnxp, nyp, nzp))
         f.write(crava_in.astype(">f4").tobytes())
 
+    # State4D parts (small case only): 3 + 3 means, 6 + 6 + 9 covariances
+    cn = nxp // 2 + 1
+    if name == "small":
+        rng4 = np.random.default_rng(44)
+        parts = [(rng4.standard_normal((nzp, nyp, cn)) + 1j * rng4.standard_normal((nzp, nyp, cn)))
+                 .astype(np.complex64) for _ in range(27)]
+        np.concatenate([p.reshape(-1) for p in parts]).view(np.float32).tofile(os.path.join(d, "state4d_in.bin"))
+
     r = subprocess.run([BIN, d], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
 
@@ -143,6 +151,14 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
     caa = np.abs(cv[0].view(np.complex64)[:, :, :cn].real)
     noise_level = int((caa <= 1e-5 * caa.max()).sum())
     assert simfail <= noise_level and sim_ref <= noise_level, (simfail, sim_ref, noise_level)
+    # State4D::merge
+    if name == "small":
+        got = np.fromfile(os.path.join(d, "state4d_out.bin"), dtype=np.complex64).reshape(9, -1)
+        mu_ref = oracle.merge_mean_4d(parts[0:3], parts[3:6])
+        cov_ref = oracle.merge_cov_4d(parts[6:12], parts[12:18], parts[18:27])
+        for q, ref in enumerate(mu_ref + cov_ref):
+            assert np.array_equal(got[q], ref.reshape(-1)), q
+
     # CRAVA binary files
     assert np.array_equal(_read(os.path.join(d, "crava_read.bin"), shape), crava_in)
     raw = open(os.path.join(d, "mu0.crava"), "rb").read()
