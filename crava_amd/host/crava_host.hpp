@@ -133,6 +133,17 @@ class FFTGrid {
   // other grid dimensions) is appended to errText and leaves the grid as it was
   void readCravaFile(const std::string &fileName, std::string &errText, bool nopadding = false);
 
+  // STORM grid file (src/fftgrid.cpp:2118-2181, header of
+  // Simbox::getStormHeader, src/simbox.cpp:408-427): geometry of the header.
+  struct StormHeader {
+    double xmin = 0.0, lx = 1.0, ymin = 0.0, ly = 1.0, lz = 1.0, angle = 0.0;  // angle in radians
+    std::string topName, botName;  // surfaces (flat == false)
+  };
+  // writes fileName + ".storm" (binary, big-endian floats) or ".txt" (ascii);
+  // padding: the padded nxp x nyp x nzp cube instead of nx x ny x nz
+  void writeStormFile(const std::string &fileName, const StormHeader &h, bool ascii, bool padding = false,
+                      bool flat = false, bool scientific = false) const;
+
  private:
   int abiCubeType() const;
   void op(int code, const FFTGrid *other, float scalar);

@@ -5,6 +5,8 @@
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
+#include <sstream>
 #include <stdexcept>
 
 namespace crava_hip {
@@ -283,6 +285,68 @@ void FFTGrid::readCravaFile(const std::string &fileName, std::string &errText, b
   } catch (const std::exception &e) {
     errText += std::string("Error: ") + e.what() + "\n";
   }
+}
+
+// ---------------------------------------------------------------------------
+// STORM grid files (src/fftgrid.cpp:2118-2181; header src/simbox.cpp:408-427;
+// numbers formatted as NRLib::ToString(x, 6): fixed, 6 decimals)
+namespace {
+std::string fixed6(double v) {
+  std::ostringstream o;
+  o << std::fixed << std::setprecision(6) << v;
+  return o.str();
+}
+}  // namespace
+
+void FFTGrid::writeStormFile(const std::string &fileName, const StormHeader &h, bool ascii, bool padding, bool flat,
+                             bool scientific) const {
+  const int nx = padding ? nxp_ : nx_, ny = padding ? nyp_ : ny_, nz = padding ? nzp_ : nz_;
+  if (!flat && h.topName.empty()) throw Error(CRAVA_HIP_E_INVALID, "writeStormFile needs surface names unless flat");
+  std::string header = ascii ? "storm_petro_ascii\n" : "storm_petro_binary\n";
+  header += "0 " + std::to_string(cubetype_) + " " + fixed6(-99999.0) + "\n";   // RMISSING
+  header += "FFTGrid\n";
+  if (!flat)
+    header += fixed6(h.xmin) + " " + fixed6(h.lx) + " " + fixed6(h.ymin) + " " + fixed6(h.ly) + " " + h.topName + " " +
+              h.botName + " 0.0 0.0\n";
+  else
+    header += fixed6(h.xmin) + " " + fixed6(h.lx) + " " + fixed6(h.ymin) + " " + fixed6(h.ly) + " 0.0 " + fixed6(h.lz) +
+              " 0.0 0.0\n";
+  header += fixed6(h.lz) + " " + fixed6(h.angle * 180 / 3.14159265358979323846) + "\n\n";
+  header += std::to_string(nx) + " " + std::to_string(ny) + " " + std::to_string(nz) + "\n";
+
+  std::vector<float> v((size_t)rsize_);
+  check(crava_hip_grid_download(ctx_->handle(), v.data(), dev_, sizeof(float) * rsize_), "writeStormFile");
+  const auto at = [&](int i, int j, int k) { return v[(size_t)i + (size_t)rnxp_ * ((size_t)j + (size_t)nyp_ * k)]; };
+  const std::string fname = fileName + (ascii ? ".txt" : ".storm");
+  std::ofstream os(fname, ascii ? std::ios::out : (std::ios::out | std::ios::binary));
+  if (!os) throw Error(CRAVA_HIP_E_INVALID, "cannot open " + fname + " for writing");
+  os << header;
+  if (!ascii) {
+    std::vector<unsigned char> buf((size_t)nx * ny * nz * 4);
+    size_t o = 0;
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const float f = at(i, j, k);
+          uint32_t u;
+          std::memcpy(&u, &f, 4);
+          buf[o++] = (unsigned char)(u >> 24);
+          buf[o++] = (unsigned char)(u >> 16);
+          buf[o++] = (unsigned char)(u >> 8);
+          buf[o++] = (unsigned char)u;
+        }
+    os.write(reinterpret_cast<const char *>(buf.data()), (std::streamsize)buf.size());
+  } else {
+    if (scientific) os << std::scientific;
+    else os << std::fixed << std::setprecision(6);
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx - 1; ++i) os << at(i, j, k) << " ";
+        os << at(nx - 1, j, k) << "\n";
+      }
+  }
+  os << "0\n";
+  if (!os) throw Error(CRAVA_HIP_E_INVALID, "write failed on " + fname);
 }
 
 }  // namespace crava_hip

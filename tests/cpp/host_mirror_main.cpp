@@ -190,6 +190,24 @@ int main(int argc, char **argv) {
       std::ofstream(dir + "/crava_err.txt") << err;
     }
 
+    // STORM files of the posterior mean of alpha: binary with surface names,
+    // ascii flat, and the padded binary cube
+    {
+      crava_hip::FFTGrid::StormHeader sh;
+      sh.xmin = 456000.5;
+      sh.lx = 11200.0;
+      sh.ymin = 6780000.25;
+      sh.ly = 5600.0;
+      sh.lz = 1024.0;
+      sh.angle = 0.3;
+      sh.topName = "top_surface";
+      sh.botName = "base_surface";
+      sph.mu(0)->writeStormFile(dir + "/mu0", sh, false);
+      sph.mu(0)->writeStormFile(dir + "/mu0_flat", sh, true, false, true);
+      sph.mu(0)->writeStormFile(dir + "/mu0_pad", sh, false, true);
+      std::ofstream(dir + "/storm_type.txt") << sph.mu(0)->getType() << "\n";
+    }
+
     // time-lapse merge (State4D::merge) when the test supplies the parts:
     // 3 + 3 means and 6 + 6 + 9 covariance grids, transformed, in that order
     {

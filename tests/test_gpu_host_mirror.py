@@ -151,6 +151,26 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
     caa = np.abs(cv[0].view(np.complex64)[:, :, :cn].real)
     noise_level = int((caa <= 1e-5 * caa.max()).sum())
     assert simfail <= noise_level and sim_ref <= noise_level, (simfail, sim_ref, noise_level)
+    # STORM files: header of Simbox::getStormHeader (NRLib::ToString(x, 6) is
+    # fixed with 6 decimals), values k, j, i, big-endian or "%.6f", then "0"
+    mu0 = _read(os.path.join(d, "out_mu0.bin"), shape)
+    ctype = int(open(os.path.join(d, "storm_type.txt")).read())
+    f6 = lambda v: "%.6f" % v
+
+    def storm_head(kind, geom, n3):
+        return (f"storm_petro_{kind}\n0 {ctype} {f6(-99999.0)}\nFFTGrid\n{geom}\n"
+                f"{f6(1024.0)} {f6(0.3 * 180 / math.pi)}\n\n{n3[0]} {n3[1]} {n3[2]}\n")
+    geo = f"{f6(456000.5)} {f6(11200.0)} {f6(6780000.25)} {f6(5600.0)}"
+    want = storm_head("binary", geo + " top_surface base_surface 0.0 0.0", (c.nx, c.ny, c.nz)).encode()
+    want += mu0[:c.nz, :c.ny, :c.nx].astype(">f4").tobytes() + b"0\n"
+    assert open(os.path.join(d, "mu0.storm"), "rb").read() == want
+    want = storm_head("binary", geo + " top_surface base_surface 0.0 0.0", (nxp, nyp, nzp)).encode()
+    want += mu0[:, :, :nxp].astype(">f4").tobytes() + b"0\n"
+    assert open(os.path.join(d, "mu0_pad.storm"), "rb").read() == want
+    lines = ["%s\n" % " ".join(f6(float(x)) for x in mu0[k, j, :c.nx]) for k in range(c.nz) for j in range(c.ny)]
+    want = storm_head("ascii", geo + f" 0.0 {f6(1024.0)} 0.0 0.0", (c.nx, c.ny, c.nz)) + "".join(lines) + "0\n"
+    assert open(os.path.join(d, "mu0_flat.txt")).read() == want
+
     # State4D::merge
     if name == "small":
         got = np.fromfile(os.path.join(d, "state4d_out.bin"), dtype=np.complex64).reshape(9, -1)
