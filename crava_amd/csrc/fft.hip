@@ -13,6 +13,7 @@
 #include "fft_static.hpp"
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace crava {
@@ -30,8 +31,17 @@ __device__ __forceinline__ int fdiv(int x, unsigned mul) { return mul ? (int)__u
 // has half as many threads as the widest pass has butterflies, so each
 // thread runs up to two independent butterflies and the active ones fill
 // whole waves.
-template <int R, int DIR, typename T>
-__device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw, const PassStep &st, int tx, int ldsp) {
+// Marks a pass that reads its butterflies from LDS.
+struct FromLds {};
+
+// ld: where the pass reads its inputs -- FromLds, or a functor ld(c, e)
+// giving element e of pencil c (the first pass of a kernel that loads its
+// butterflies straight from global memory; no barrier is then needed before
+// the LDS writes).
+template <int R, int DIR, typename T, typename Ld = FromLds>
+__device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw, const PassStep &st, int tx, int ldsp,
+                                              const Ld &ld = Ld{}) {
+  constexpr bool kFromLds = std::is_same<Ld, FromLds>::value;
   const int NB = st.NB, Ns = st.Ns;
   const int nthr = blockDim.x;
   const int tot = tx * NB;
@@ -47,7 +57,10 @@ __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw, const
       const int j = b - c * NB;
       const cx<T> *src = buf + c * ldsp;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[t][r] = src[gpadx(j + r * NB)];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (kFromLds) v[t][r] = src[gpadx(j + r * NB)];
+        else v[t][r] = ld(c, j + r * NB);
+      }
       int k = 0;
       if (Ns > 1) {
         k = j - fdiv(j, st.ns_mul) * Ns;
@@ -64,7 +77,7 @@ __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw, const
       cb[t] = c * ldsp;
     }
   }
-  __syncthreads();
+  if constexpr (kFromLds) __syncthreads();
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     if (act[t]) {
@@ -76,9 +89,23 @@ __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw, const
   __syncthreads();
 }
 
-template <int DIR, typename T>
-__device__ void run_passes(cx<T> *buf, const cx<T> *tw, const PassPlan &pl) {
-  for (int p = 0; p < pl.nrad; ++p) {
+template <int DIR, typename T, typename Ld = FromLds>
+__device__ void run_passes(cx<T> *buf, const cx<T> *tw, const PassPlan &pl, const Ld &ld = Ld{}) {
+  int p = 0;
+  if constexpr (!std::is_same<Ld, FromLds>::value) {
+    // the first pass takes its butterflies from ld
+    const PassStep &st = pl.step[0];
+    switch (st.R) {
+      case 2: stockham_pass<2, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
+      case 3: stockham_pass<3, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
+      case 4: stockham_pass<4, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
+      case 5: stockham_pass<5, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
+      case 7: stockham_pass<7, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
+      default: stockham_pass<8, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
+    }
+    p = 1;
+  }
+  for (; p < pl.nrad; ++p) {
     const PassStep &st = pl.step[p];
     switch (st.R) {
       case 2: stockham_pass<2, DIR>(buf, tw, st, pl.tx, pl.ldsp); break;
@@ -152,7 +179,9 @@ __global__ void __launch_bounds__(1024) k_fft_strided(GridSet gs, long long es, 
 // 2c+1 (imaginary part); the two spectra are separated afterwards with
 // A_k = (Z_k + conj Z_{n-k})/2, B_k = (Z_k - conj Z_{n-k})/(2i).
 // ---------------------------------------------------------------------------
-template <typename T, typename SI, typename SO>
+// DIO: the first radix pass loads its butterflies straight from the rows
+// (branch-free, clamped rows) instead of staging the pencils through LDS.
+template <typename T, typename SI, typename SO, bool DIO>
 __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nrows, int rnxp, PassPlan pl,
                                                        const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
@@ -168,6 +197,15 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nro
   const int nthr = blockDim.x;
   const long long row0 = (long long)bx * 2 * pl.tx;
   for (int q = threadIdx.x; q < n; q += nthr) tw[q] = twg[q];
+  if constexpr (DIO) {
+    const auto ld = [&](int c, int e) -> cx<T> {
+      const long long ra = row0 + 2 * c;
+      const SI a = src[min(ra, nrows - 1) * rnxp + e];
+      const SI b = src[min(ra + 1, nrows - 1) * rnxp + e];
+      return mk(ra < nrows ? (T)a : (T)0, ra + 1 < nrows ? (T)b : (T)0);
+    };
+    run_passes<-1>(buf, tw, pl, ld);
+  } else {
   // element e of pencil c is (row 2c, row 2c+1) at e: one whole complex LDS
   // store per lane
   const int tot = pl.tx * n;
@@ -194,6 +232,7 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nro
   }
   __syncthreads();
   run_passes<-1>(buf, tw, pl);
+  }
   const int cn = n / 2 + 1;
   const T hs = (T)0.5 * scale;
   const int tot2 = pl.tx * cn;
@@ -217,7 +256,7 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nro
 // x pass, inverse: half spectra of rows a, b -> Z = A + iB on the full length
 // (Hermitian extension, Im of bins 0 and n/2 ignored as in a halfcomplex
 // c2r), inverse DFT, real part -> row a, imaginary part -> row b.
-template <typename T, typename SI, typename SO>
+template <typename T, typename SI, typename SO, bool DIO>
 __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nrows, int rnxp, PassPlan pl,
                                                        const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
@@ -234,6 +273,22 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nro
   const int nthr = blockDim.x;
   const long long row0 = (long long)bx * 2 * pl.tx;
   for (int q = threadIdx.x; q < n; q += nthr) tw[q] = twg[q];
+  if constexpr (DIO) {
+    const auto ld = [&](int c, int k) -> cx<T> {
+      const bool mirror = k >= cn;
+      const int kk = mirror ? n - k : k;
+      const bool selfc = (kk == 0) || ((n & 1) == 0 && kk == n / 2);
+      const long long ra = row0 + 2 * c;
+      cx<SI> x = reinterpret_cast<const cx<SI> *>(src + min(ra, nrows - 1) * rnxp)[kk];
+      cx<SI> y = reinterpret_cast<const cx<SI> *>(src + min(ra + 1, nrows - 1) * rnxp)[kk];
+      if (ra >= nrows) x = mk((SI)0, (SI)0);
+      if (ra + 1 >= nrows) y = mk((SI)0, (SI)0);
+      if (selfc) { x.y = (SI)0; y.y = (SI)0; }
+      if (mirror) { x.y = -x.y; y.y = -y.y; }
+      return mk((T)x.x - (T)y.y, (T)x.y + (T)y.x);
+    };
+    run_passes<+1>(buf, tw, pl, ld);
+  } else {
   const int tot = pl.tx * n;
   for (int q0 = threadIdx.x; q0 < tot; q0 += kUnroll * nthr) {
     cx<SI> a[kUnroll], b[kUnroll];
@@ -270,6 +325,7 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nro
   }
   __syncthreads();
   run_passes<+1>(buf, tw, pl);
+  }
   // element e of pencil c -> real part to row 2c, imaginary part to row 2c+1
   const int tot2 = pl.tx * n;
   for (int q = threadIdx.x; q < tot2; q += nthr) {
@@ -463,8 +519,8 @@ static int set_lds(F kernel, size_t bytes) {
   return CRAVA_HIP_OK;
 }
 
-template <typename T, typename SI, typename SO>
-static int launch_x(const Fft3D &f, const GridSet &gs, hipStream_t s, bool inverse) {
+template <typename T, typename SI, typename SO, bool DIO>
+static int launch_x_v(const Fft3D &f, const GridSet &gs, hipStream_t s, bool inverse) {
   const long long nrows = (long long)f.nyp * f.nzp;
   const int rnxp = 2 * (f.nxp / 2 + 1);
   const PassPlan &p = f.px;
@@ -472,13 +528,13 @@ static int launch_x(const Fft3D &f, const GridSet &gs, hipStream_t s, bool inver
   const cx<T> *tw = reinterpret_cast<const cx<T> *>(f.twx);
   int rc;
   if (!inverse) {
-    if ((rc = set_lds(k_fft_rows_r2c<T, SI, SO>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_rows_r2c<T, SI, SO>), dim3(blocks, 1, (unsigned)gs.n), dim3(p.threads), p.lds_bytes, s,
-                       gs, nrows, rnxp, p, tw);
+    if ((rc = set_lds(k_fft_rows_r2c<T, SI, SO, DIO>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_rows_r2c<T, SI, SO, DIO>), dim3(blocks, 1, (unsigned)gs.n), dim3(p.threads), p.lds_bytes,
+                       s, gs, nrows, rnxp, p, tw);
   } else {
-    if ((rc = set_lds(k_fft_rows_c2r<T, SI, SO>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_rows_c2r<T, SI, SO>), dim3(blocks, 1, (unsigned)gs.n), dim3(p.threads), p.lds_bytes, s,
-                       gs, nrows, rnxp, p, tw);
+    if ((rc = set_lds(k_fft_rows_c2r<T, SI, SO, DIO>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_rows_c2r<T, SI, SO, DIO>), dim3(blocks, 1, (unsigned)gs.n), dim3(p.threads), p.lds_bytes,
+                       s, gs, nrows, rnxp, p, tw);
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
@@ -502,6 +558,11 @@ static int launch_strided(const PassPlan &p, const void *tw, const GridSet &gs, 
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
+}
+
+template <typename T, typename SI, typename SO>
+static int launch_x(const Fft3D &f, const GridSet &gs, hipStream_t s, bool inverse) {
+  return f.xdio ? launch_x_v<T, SI, SO, true>(f, gs, s, inverse) : launch_x_v<T, SI, SO, false>(f, gs, s, inverse);
 }
 
 // Generic-kernel dispatch on the storage of one pass.
@@ -604,9 +665,13 @@ int Fft3D::set_variant(int axis, int v) {
   const int n = axis == 0 ? nxp : (axis == 1 ? nyp : nzp);
   PassPlan &p = axis == 0 ? px : (axis == 1 ? py : pz);
   if (v < 0) {
-    const int g = -1 - v;
+    int g = -1 - v;
+    // x pass: -7..-12 are -1..-6 with the first radix pass loading its
+    // butterflies straight from the rows
+    const bool dio = axis == 0 && g >= kNumGenericVariants && g < 2 * kNumGenericVariants;
+    if (dio) g -= kNumGenericVariants;
     if (g >= kNumGenericVariants) {
-      set_error("generic variant " + std::to_string(g) + " does not exist");
+      set_error("generic variant " + std::to_string(-1 - v) + " does not exist for this axis");
       return CRAVA_HIP_E_INVALID;
     }
     PassPlan q;
@@ -614,6 +679,7 @@ int Fft3D::set_variant(int axis, int v) {
     if ((rc = build_plan(n, q, g))) return rc;
     p = q;
     var[axis] = v;
+    if (axis == 0) xdio = dio;
     return CRAVA_HIP_OK;
   }
   const FftKind kind = axis == 0 ? kRowsR2C : kStridedFwd;
@@ -646,7 +712,7 @@ int Fft3D::autotune(void *const *grids, int ng, hipStream_t s, int reps, float *
     const int nv = num_variants(axis);
     float best = 1e30f;
     int bestv = var[axis];
-    for (int v = -kNumGenericVariants; v < nv; ++v) {
+    for (int v = -kNumGenericVariants * (axis == 0 ? 2 : 1); v < nv; ++v) {
       if (set_variant(axis, v) != CRAVA_HIP_OK) continue;
       // forward and inverse pass together: one variant serves both directions
       auto run = [&]() {
