@@ -26,21 +26,21 @@ constexpr int kUnroll = 8;
 // x / d with the plan's multiplier (d == 1 when mul == 0).
 __device__ __forceinline__ int fdiv(int x, unsigned mul) { return mul ? (int)__umulhi((unsigned)x, mul) : x; }
 
-// One radix-R Stockham pass over the tx pencils held in LDS.  Butterfly b
-// (pencil b / NB, index j = b mod NB) runs on thread b mod nthr; a workgroup
-// has half as many threads as the widest pass has butterflies, so each
-// thread runs up to two independent butterflies and the active ones fill
-// whole waves.
 // Marks a pass that reads its butterflies from LDS.
 struct FromLds {};
 
+// One radix-R Stockham pass over the tx pencils held in LDS.  Butterfly b
+// (pencil b / NB, index j = b mod NB; with CF pencil b mod tx, index b / tx,
+// txm the multiplier of tx) runs on thread b mod nthr; a workgroup has half
+// as many threads as the widest pass has butterflies, so each thread runs up
+// to two independent butterflies and the active ones fill whole waves.
 // ld: where the pass reads its inputs -- FromLds, or a functor ld(c, e)
 // giving element e of pencil c (the first pass of a kernel that loads its
 // butterflies straight from global memory; no barrier is then needed before
 // the LDS writes).
-template <int R, int DIR, typename T, typename Ld = FromLds>
+template <int R, int DIR, typename T, typename Ld = FromLds, bool CF = false>
 __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw, const PassStep &st, int tx, int ldsp,
-                                              const Ld &ld = Ld{}) {
+                                              const Ld &ld = Ld{}, unsigned txm = 0) {
   constexpr bool kFromLds = std::is_same<Ld, FromLds>::value;
   const int NB = st.NB, Ns = st.Ns;
   const int nthr = blockDim.x;
@@ -53,8 +53,14 @@ __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw, const
     const int b = (int)threadIdx.x + t * nthr;
     act[t] = b < tot;
     if (act[t]) {
-      const int c = fdiv(b, st.nb_mul);
-      const int j = b - c * NB;
+      int c, j;
+      if constexpr (CF) {
+        j = fdiv(b, txm);
+        c = b - j * tx;
+      } else {
+        c = fdiv(b, st.nb_mul);
+        j = b - c * NB;
+      }
       const cx<T> *src = buf + c * ldsp;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -89,19 +95,21 @@ __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw, const
   __syncthreads();
 }
 
-template <int DIR, typename T, typename Ld = FromLds>
+// All passes; the first takes its butterflies from ld unless that is
+// FromLds, with pencil-fastest lanes when CF (strided columns side by side).
+template <int DIR, typename T, typename Ld = FromLds, bool CF = false>
 __device__ void run_passes(cx<T> *buf, const cx<T> *tw, const PassPlan &pl, const Ld &ld = Ld{}) {
   int p = 0;
   if constexpr (!std::is_same<Ld, FromLds>::value) {
-    // the first pass takes its butterflies from ld
     const PassStep &st = pl.step[0];
+    const unsigned txm = pl.tx_mul;
     switch (st.R) {
-      case 2: stockham_pass<2, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
-      case 3: stockham_pass<3, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
-      case 4: stockham_pass<4, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
-      case 5: stockham_pass<5, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
-      case 7: stockham_pass<7, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
-      default: stockham_pass<8, DIR>(buf, tw, st, pl.tx, pl.ldsp, ld); break;
+      case 2: stockham_pass<2, DIR, T, Ld, CF>(buf, tw, st, pl.tx, pl.ldsp, ld, txm); break;
+      case 3: stockham_pass<3, DIR, T, Ld, CF>(buf, tw, st, pl.tx, pl.ldsp, ld, txm); break;
+      case 4: stockham_pass<4, DIR, T, Ld, CF>(buf, tw, st, pl.tx, pl.ldsp, ld, txm); break;
+      case 5: stockham_pass<5, DIR, T, Ld, CF>(buf, tw, st, pl.tx, pl.ldsp, ld, txm); break;
+      case 7: stockham_pass<7, DIR, T, Ld, CF>(buf, tw, st, pl.tx, pl.ldsp, ld, txm); break;
+      default: stockham_pass<8, DIR, T, Ld, CF>(buf, tw, st, pl.tx, pl.ldsp, ld, txm); break;
     }
     p = 1;
   }
@@ -122,7 +130,10 @@ __device__ void run_passes(cx<T> *buf, const cx<T> *tw, const PassPlan &pl, cons
 // Strided complex pass (y or z).  Columns are contiguous; elements `es`
 // apart; a batch dimension `bstride` apart; blockIdx.z selects the grid.
 // ---------------------------------------------------------------------------
-template <int DIR, typename T, typename SI, typename SO>
+// DIO: the first radix pass loads its butterflies straight from the columns
+// (pencil-fastest lanes, branch-free clamped loads) instead of staging the
+// pencils through LDS.
+template <int DIR, typename T, typename SI, typename SO, bool DIO>
 __global__ void __launch_bounds__(1024) k_fft_strided(GridSet gs, long long es, long long ncols, long long bstride,
                                                       PassPlan pl, const cx<T> *__restrict__ twg) {
   extern __shared__ unsigned char smem_raw[];
@@ -142,6 +153,13 @@ __global__ void __launch_bounds__(1024) k_fft_strided(GridSet gs, long long es, 
   cx<SO> *obase = dst + off;
   for (int q = threadIdx.x; q < pl.n; q += nthr) tw[q] = twg[q];
   const int tot = pl.tx * pl.n;
+  if constexpr (DIO) {
+    const auto ld = [&](int c, int e) -> cx<T> {
+      const cx<SI> v = ibase[min(c, ncol - 1) + e * es];
+      return c < ncol ? mk((T)v.x, (T)v.y) : mk((T)0, (T)0);
+    };
+    run_passes<DIR, T, decltype(ld), true>(buf, tw, pl, ld);
+  } else {
   // kUnroll loads in flight per thread before they are stored to LDS: the
   // loads are branch-free (clamped addresses, validity kept aside) and stay
   // in their storage type until the LDS store, so none waits on another
@@ -162,6 +180,7 @@ __global__ void __launch_bounds__(1024) k_fft_strided(GridSet gs, long long es, 
   }
   __syncthreads();
   run_passes<DIR>(buf, tw, pl);
+  }
 #pragma unroll 4
   for (int q = threadIdx.x; q < tot; q += nthr) {
     const int e = fdiv(q, pl.tx_mul);
@@ -540,21 +559,21 @@ static int launch_x_v(const Fft3D &f, const GridSet &gs, hipStream_t s, bool inv
   return CRAVA_HIP_OK;
 }
 
-template <typename T, typename SI, typename SO>
-static int launch_strided(const PassPlan &p, const void *tw, const GridSet &gs, hipStream_t s, bool inverse,
-                          long long es, long long ncols, long long nbatch, long long bstride) {
+template <typename T, typename SI, typename SO, bool DIO>
+static int launch_strided_g(const PassPlan &p, const void *tw, const GridSet &gs, hipStream_t s, bool inverse,
+                            long long es, long long ncols, long long nbatch, long long bstride) {
   const unsigned gx = (unsigned)((ncols + p.tx - 1) / p.tx);
   const dim3 grid(gx, (unsigned)nbatch, (unsigned)gs.n);
   const cx<T> *twp = reinterpret_cast<const cx<T> *>(tw);
   int rc;
   if (!inverse) {
-    if ((rc = set_lds(k_fft_strided<-1, T, SI, SO>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_strided<-1, T, SI, SO>), grid, dim3(p.threads), p.lds_bytes, s, gs, es, ncols, bstride,
-                       p, twp);
+    if ((rc = set_lds(k_fft_strided<-1, T, SI, SO, DIO>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_strided<-1, T, SI, SO, DIO>), grid, dim3(p.threads), p.lds_bytes, s, gs, es, ncols,
+                       bstride, p, twp);
   } else {
-    if ((rc = set_lds(k_fft_strided<1, T, SI, SO>, p.lds_bytes))) return rc;
-    hipLaunchKernelGGL((k_fft_strided<1, T, SI, SO>), grid, dim3(p.threads), p.lds_bytes, s, gs, es, ncols, bstride,
-                       p, twp);
+    if ((rc = set_lds(k_fft_strided<1, T, SI, SO, DIO>, p.lds_bytes))) return rc;
+    hipLaunchKernelGGL((k_fft_strided<1, T, SI, SO, DIO>), grid, dim3(p.threads), p.lds_bytes, s, gs, es, ncols,
+                       bstride, p, twp);
   }
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
@@ -562,7 +581,7 @@ static int launch_strided(const PassPlan &p, const void *tw, const GridSet &gs, 
 
 template <typename T, typename SI, typename SO>
 static int launch_x(const Fft3D &f, const GridSet &gs, hipStream_t s, bool inverse) {
-  return f.xdio ? launch_x_v<T, SI, SO, true>(f, gs, s, inverse) : launch_x_v<T, SI, SO, false>(f, gs, s, inverse);
+  return f.gdio[0] ? launch_x_v<T, SI, SO, true>(f, gs, s, inverse) : launch_x_v<T, SI, SO, false>(f, gs, s, inverse);
 }
 
 // Generic-kernel dispatch on the storage of one pass.
@@ -575,13 +594,21 @@ static int generic_x(const Fft3D &f, int io, const GridSet &gs, hipStream_t s, b
   }
 }
 
+template <typename T, typename SI, typename SO>
+static int launch_strided(const PassPlan &p, const void *tw, const GridSet &gs, hipStream_t s, bool inverse,
+                          long long es, long long ncols, long long nbatch, long long bstride, bool dio) {
+  return dio ? launch_strided_g<T, SI, SO, true>(p, tw, gs, s, inverse, es, ncols, nbatch, bstride)
+             : launch_strided_g<T, SI, SO, false>(p, tw, gs, s, inverse, es, ncols, nbatch, bstride);
+}
+
 static int generic_strided(const PassPlan &p, const void *tw, int io, const GridSet &gs, hipStream_t s, bool inv,
-                           long long es, long long ncols, long long nbatch, long long bstride) {
+                           long long es, long long ncols, long long nbatch, long long bstride, bool dio) {
   switch (io) {
-    case kIoF32: return launch_strided<float, float, float>(p, tw, gs, s, inv, es, ncols, nbatch, bstride);
-    case kIoF64: return launch_strided<double, double, double>(p, tw, gs, s, inv, es, ncols, nbatch, bstride);
-    case kIoF32toF64: return launch_strided<double, float, double>(p, tw, gs, s, inv, es, ncols, nbatch, bstride);
-    default: return launch_strided<double, double, float>(p, tw, gs, s, inv, es, ncols, nbatch, bstride);
+    case kIoF32: return launch_strided<float, float, float>(p, tw, gs, s, inv, es, ncols, nbatch, bstride, dio);
+    case kIoF64: return launch_strided<double, double, double>(p, tw, gs, s, inv, es, ncols, nbatch, bstride, dio);
+    case kIoF32toF64:
+      return launch_strided<double, float, double>(p, tw, gs, s, inv, es, ncols, nbatch, bstride, dio);
+    default: return launch_strided<double, double, float>(p, tw, gs, s, inv, es, ncols, nbatch, bstride, dio);
   }
 }
 
@@ -652,12 +679,12 @@ int Fft3D::pass(int axis, void *const *grids, int ng, bool inverse, const double
   if (axis == 1) {
     if (nyp <= 1) return copy_between(io, gs, cnxp * nyp * nzp, s);
     if (var[1] >= 0) return static_fft_launch(nyp, io, kind, var[1], gs, cnxp, cnxp, nzp, cnxp * nyp, 0, 0, stw[1], s);
-    return generic_strided(py, twy, io, gs, s, inverse, cnxp, cnxp, nzp, cnxp * nyp);
+    return generic_strided(py, twy, io, gs, s, inverse, cnxp, cnxp, nzp, cnxp * nyp, gdio[1]);
   }
   const long long plane = cnxp * nyp;
   if (nzp <= 1) return copy_between(io, gs, plane * nzp, s);
   if (var[2] >= 0) return static_fft_launch(nzp, io, kind, var[2], gs, plane, plane, 1, 0, 0, 0, stw[2], s);
-  return generic_strided(pz, twz, io, gs, s, inverse, plane, plane, 1, 0);
+  return generic_strided(pz, twz, io, gs, s, inverse, plane, plane, 1, 0, gdio[2]);
 }
 
 int Fft3D::set_variant(int axis, int v) {
@@ -666,12 +693,12 @@ int Fft3D::set_variant(int axis, int v) {
   PassPlan &p = axis == 0 ? px : (axis == 1 ? py : pz);
   if (v < 0) {
     int g = -1 - v;
-    // x pass: -7..-12 are -1..-6 with the first radix pass loading its
-    // butterflies straight from the rows
-    const bool dio = axis == 0 && g >= kNumGenericVariants && g < 2 * kNumGenericVariants;
+    // -7..-12 are -1..-6 with the first radix pass loading its butterflies
+    // straight from global memory
+    const bool dio = g >= kNumGenericVariants;
     if (dio) g -= kNumGenericVariants;
     if (g >= kNumGenericVariants) {
-      set_error("generic variant " + std::to_string(-1 - v) + " does not exist for this axis");
+      set_error("generic variant " + std::to_string(v) + " does not exist");
       return CRAVA_HIP_E_INVALID;
     }
     PassPlan q;
@@ -679,7 +706,7 @@ int Fft3D::set_variant(int axis, int v) {
     if ((rc = build_plan(n, q, g))) return rc;
     p = q;
     var[axis] = v;
-    if (axis == 0) xdio = dio;
+    gdio[axis] = dio;
     return CRAVA_HIP_OK;
   }
   const FftKind kind = axis == 0 ? kRowsR2C : kStridedFwd;
@@ -712,7 +739,7 @@ int Fft3D::autotune(void *const *grids, int ng, hipStream_t s, int reps, float *
     const int nv = num_variants(axis);
     float best = 1e30f;
     int bestv = var[axis];
-    for (int v = -kNumGenericVariants * (axis == 0 ? 2 : 1); v < nv; ++v) {
+    for (int v = -2 * kNumGenericVariants; v < nv; ++v) {
       if (set_variant(axis, v) != CRAVA_HIP_OK) continue;
       // forward and inverse pass together: one variant serves both directions
       auto run = [&]() {

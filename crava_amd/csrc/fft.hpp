@@ -97,7 +97,7 @@ class Fft3D {
   // Launch variant per axis (x, y, z): v >= 0 specialised kernel variant v,
   // v < 0 generic kernel variant -1 - v.
   int var[3] = {-1, -1, -1};
-  bool xdio = false;  // generic x pass with direct first-pass loads (variants -7..-12)
+  bool gdio[3] = {false, false, false};  // generic pass with direct first-pass loads (variants -7..-12)
   int set_variant(int axis, int v);
   int num_variants(int axis) const;  // specialised variants available
   // Time all variants (forward + inverse pass) on the set `grids` (overwritten)

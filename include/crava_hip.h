@@ -112,8 +112,8 @@ int crava_hip_fft_pass_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, i
  * upper half of each range loading the first radix pass straight from HBM
  * and storing the last straight to it; v = -1..-6 the runtime-length kernel
  * with radix order (large-first / small-first) x pencils per workgroup
- * (<=64, <=16, <=8), and for the x pass v = -7..-12 the same with the first
- * radix pass loading straight from the rows.
+ * (<=64, <=16, <=8), and v = -7..-12 the same with the first radix pass
+ * loading straight from global memory.
  * variants[3] / ms[3] (host, may be NULL) receive the choice per axis and its
  * time.  No reference counterpart (FFTW_ESTIMATE plans are not measured). */
 int crava_hip_autotune_fft(crava_hip_ctx *ctx, void *scratch, int nxp, int nyp, int nzp, int precision,

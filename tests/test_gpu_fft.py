@@ -101,7 +101,7 @@ def test_every_launch_variant_and_autotuner(ctx, n, dbl):
     # first pass and store the last straight from/to global memory)
     for v in range(-12, 8):
         try:
-            set_fft_variants(ctx, nxp, nyp, nzp, prec, [min(v, 5), max(v, -6), max(v, -6)])
+            set_fft_variants(ctx, nxp, nyp, nzp, prec, [min(v, 5), v, v])
         except L.CravaHipError:
             continue      # variant does not fit (LDS) for this length/precision
         t = torch.from_numpy(g.copy()).cuda()
@@ -113,7 +113,7 @@ def test_every_launch_variant_and_autotuner(ctx, n, dbl):
         assert rel_l2(t.cpu().numpy()[:, :, :nxp], g[:, :, :nxp]) < tol, v
     scratch = torch.zeros(g.shape, dtype=torch.float64 if dbl else torch.float32, device="cuda")
     var, ms = autotune_fft(ctx, scratch, nxp, nyp, nzp)
-    assert -12 <= var[0] <= 7 and all(-6 <= x <= 7 for x in var[1:]) and all(m > 0 for m in ms)
+    assert all(-12 <= x <= 7 for x in var) and all(m > 0 for m in ms)
     t = torch.from_numpy(g.copy()).cuda()
     ctx.r2c(t, nxp, nyp, nzp, L.DATA)
     torch.cuda.synchronize()
@@ -171,11 +171,11 @@ def test_generic_variants_odd_lengths(ctx, n):
     g = seeded_real_grid(nxp, nyp, nzp, seed=5)
     ref = g.copy()
     oracle.fft_r2c(ref, nxp, nyp, nzp, False)
-    # x pass: -7..-12 are -1..-6 with the first radix pass loading straight
-    # from the rows; float and float-storage/double-arithmetic plans
+    # -7..-12 are -1..-6 with the first radix pass loading straight from
+    # global memory; float and float-storage/double-arithmetic plans
     for prec in (L.F32, L.MIXED):
         for v in range(-12, 0):
-            set_fft_variants(ctx, nxp, nyp, nzp, prec, [v, max(v, -6), max(v, -6)])
+            set_fft_variants(ctx, nxp, nyp, nzp, prec, [v, v, v])
             t = torch.from_numpy(g.copy()).cuda()
             ctx.r2c(t, nxp, nyp, nzp, L.DATA, prec)
             torch.cuda.synchronize()
