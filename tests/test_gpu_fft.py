@@ -15,6 +15,8 @@ SIZES = [
     (96, 32, 45),
     (512, 8, 320),    # the large-grid x and z lengths
     (2, 1, 1),        # degenerate
+    (6, 1, 5),        # one-element y axis (a copy between storages)
+    (4, 7, 1),        # one-element z axis
 ]
 
 
@@ -121,7 +123,8 @@ def test_every_launch_variant_and_autotuner(ctx, n, dbl):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [(80, 80, 160), (15, 12, 24), (512, 8, 320)], ids=lambda n: "x".join(map(str, n)))
+@pytest.mark.parametrize("n", [(80, 80, 160), (15, 12, 24), (512, 8, 320), (6, 1, 5), (4, 7, 1)],
+                         ids=lambda n: "x".join(map(str, n)))
 def test_r2c_float_storage_double_arithmetic(ctx, n):
     """CRAVA_HIP_MIXED transforms round each output once: every spectral value,
     including values 1e-9 below the peak, keeps float relative accuracy
