@@ -68,6 +68,28 @@ class SyntParams(C.Structure):
     ]
 
 
+class SwdivParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("nxp", C.c_int), ("nyp", C.c_int), ("nzp", C.c_int),
+        ("ntheta", C.c_int),
+        ("wavelets", VP),
+        ("wavelet_norm", VP),
+        ("A", VP),
+        ("prior_var0", VP),
+        ("corr_t", VP),
+        ("err_var", VP),
+        ("rel_thick", VP),
+        ("shift", VP),
+        ("gain", VP),
+        ("dz", C.c_float),
+        ("lz", C.c_double),
+        ("low_cut", C.c_float),
+        ("high_cut", C.c_float),
+        ("wnc", C.c_float),
+    ]
+
+
 # name -> (restype, argtypes); every function declared in include/crava_hip.h
 SIGNATURES = {
     "crava_hip_version": (I, []),
@@ -113,6 +135,8 @@ SIGNATURES = {
     "crava_hip_autotune_fft": (I, [VP, VP, I, I, I, I, C.POINTER(I), C.POINTER(C.c_float)]),
     "crava_hip_fft_set_variants": (I, [VP, I, I, I, I, C.POINTER(I)]),
     "crava_hip_synthetic_seismic": (I, [VP, C.POINTER(SyntParams), VP, VP, VP, C.POINTER(VP)]),
+    "crava_hip_divide_by_scale_wavelet": (I, [VP, C.POINTER(SwdivParams), C.POINTER(VP)]),
+    "crava_hip_interpolate_seismic": (I, [VP, I, I, I, I, I, I, C.c_float, VP, C.POINTER(I), C.POINTER(I)]),
     "crava_hip_merge_mean_4d": (I, [VP, I, I, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP)]),
     "crava_hip_merge_cov_4d": (I, [VP, I, I, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(VP)]),
 }

@@ -39,6 +39,7 @@ SOURCES = {
     "batch.hip": [],
     "grid.hip": ["-ffp-contract=off"],
     "forward.hip": ["-ffp-contract=off"],
+    "swdiv.hip": ["-ffp-contract=off"],
 }
 
 COMMON = [

@@ -277,6 +277,47 @@ def synthetic_seismic(ctx: Context, nx, ny, nz, nxp, nyp, nzp, A, wavelets, dz, 
                                             ptr_array([_dev_ptr(g) for g in out])), "crava_hip_synthetic_seismic")
 
 
+def divide_by_scale_wavelet(ctx: Context, nx, ny, nz, nxp, nyp, nzp, grids, wavelets, wnorm, A, var0, corr_t,
+                            err_var, rel_thick, dz, lz, low_cut, high_cut, wnc, shift=None, gain=None):
+    """Wavelet division of the non-constant-thickness path
+    (crava_hip_divide_by_scale_wavelet) in place on the float grids `grids`
+    (one per angle).  wavelets [ntheta][nzp], wnorm [ntheta], A [ntheta][3],
+    var0 [3][3], corr_t [nzp], err_var [ntheta], rel_thick [nyp][nxp] and the
+    optional per-angle [ny][nx] shift / gain tables are host arrays."""
+    import numpy as np
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    nt = A.shape[0]
+    keep = [np.ascontiguousarray(wavelets, dtype=np.float32), np.ascontiguousarray(wnorm, dtype=np.float32), A,
+            np.ascontiguousarray(var0, dtype=np.float64), np.ascontiguousarray(corr_t, dtype=np.float32),
+            np.ascontiguousarray(err_var, dtype=np.float32), np.ascontiguousarray(rel_thick, dtype=np.float64)]
+
+    def table(t):
+        if t is None:
+            return None
+        arr = (C.c_void_p * nt)()
+        for l, x in enumerate(t):
+            if x is not None:
+                x = np.ascontiguousarray(x, dtype=np.float32)
+                keep.append(x)
+                arr[l] = x.ctypes.data
+        keep.append(arr)
+        return C.cast(arr, C.c_void_p)
+
+    p = L.SwdivParams(nx, ny, nz, nxp, nyp, nzp, nt, *[k.ctypes.data for k in keep[:7]], table(shift), table(gain),
+                      dz, lz, low_cut, high_cut, wnc)
+    check(lib().crava_hip_divide_by_scale_wavelet(ctx.h, C.byref(p), ptr_array([_dev_ptr(g) for g in grids])),
+          "crava_hip_divide_by_scale_wavelet")
+
+
+def interpolate_seismic(ctx: Context, nx, ny, nz, nxp, nyp, nzp, threshold, grid):
+    """FFTGrid::interpolateSeismic (crava_hip_interpolate_seismic) in place on
+    one float grid; returns (traces flagged, traces with zero energy)."""
+    nf, n0 = C.c_int(0), C.c_int(0)
+    check(lib().crava_hip_interpolate_seismic(ctx.h, nx, ny, nz, nxp, nyp, nzp, threshold, _dev_ptr(grid),
+                                              C.byref(nf), C.byref(n0)), "crava_hip_interpolate_seismic")
+    return nf.value, n0.value
+
+
 def merge_mean_4d(ctx: Context, nxp, nyp, nzp, mu_static, mu_dynamic, mu):
     """crava_hip_merge_mean_4d on transformed float grids (lists of 3)."""
     P = lambda gs: ptr_array([_dev_ptr(g) for g in gs])

@@ -10,6 +10,7 @@
 #include "nll.hpp"
 #include "posterior.hpp"
 #include "simulate.hpp"
+#include "swdiv.hpp"
 #include "tables.hpp"
 
 #include <cmath>
@@ -624,6 +625,145 @@ int crava_hip_synthetic_seismic(crava_hip_ctx *ctx, const crava_hip_synt_params 
   if ((rc = crava::launch_fm_filter(a, ctx->stream))) return rc;
   if ((rc = f->pass(2, out, nt, true, nullptr, ctx->stream))) return rc;
   return crava::launch_fm_padding(a, ctx->stream);
+}
+
+
+// ---------------------------------------------------------------------------
+// Non-constant-thickness data preparation
+
+int crava_hip_divide_by_scale_wavelet(crava_hip_ctx *ctx, const crava_hip_swdiv_params *p, void *const *grids) {
+  if (!ctx || !p || !grids || !p->wavelets || !p->wavelet_norm || !p->A || !p->prior_var0 || !p->corr_t ||
+      !p->err_var || !p->rel_thick) {
+    set_error("NULL argument to crava_hip_divide_by_scale_wavelet");
+    return CRAVA_HIP_E_INVALID;
+  }
+  const int nt = p->ntheta;
+  if (nt < 1 || nt > crava::kSwdMaxTheta) { set_error("ntheta must be 1..8"); return CRAVA_HIP_E_INVALID; }
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1 || p->nx > p->nxp || p->ny > p->nyp || p->nz > p->nzp || p->nzp < 2) {
+    set_error("invalid volume / padded grid sizes");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (!(p->dz > 0.0f) || !(p->lz > 0.0)) { set_error("dz and lz must be positive"); return CRAVA_HIP_E_INVALID; }
+  for (int l = 0; l < nt; ++l)
+    if (!grids[l]) { set_error("NULL grid"); return CRAVA_HIP_E_INVALID; }
+  const size_t ntr_pad = (size_t)p->nxp * p->nyp;
+  for (size_t t = 0; t < ntr_pad; ++t)
+    if (!(p->rel_thick[t] > 0.0) || !std::isfinite(p->rel_thick[t])) {
+      set_error("relative thicknesses must be positive and finite");
+      return CRAVA_HIP_E_INVALID;
+    }
+  crava::Fft3D *f = nullptr;
+  int rc = ctx->plan(p->nxp, p->nyp, p->nzp, 0, &f);
+  if (rc) return rc;
+  // per-angle tables [nt][cn]: W {re, im}, t3, tau2, wg2
+  const int cn = p->nzp / 2 + 1;
+  std::vector<float> tw((size_t)nt * cn * 2), t3((size_t)nt * cn), tau2((size_t)nt * cn), wg2((size_t)nt * cn);
+  crava::SwdArgs a{};
+  for (int l = 0; l < nt; ++l) {
+    const size_t o = (size_t)l * cn;
+    crava::swd_angle_tables(p->nz, p->nzp, p->wavelets + (size_t)l * p->nzp, p->wavelet_norm[l], p->A + 3 * l,
+                            p->prior_var0, p->corr_t, p->dz, p->lz, p->low_cut, p->high_cut, tw.data() + 2 * o,
+                            t3.data() + o, tau2.data() + o, wg2.data() + o, &a.mod_w[l]);
+    a.ev[l] = p->err_var[l];
+    a.wn2[l] = p->wavelet_norm[l] * p->wavelet_norm[l];
+  }
+  const auto al = [](size_t b) { return ((b + 255) / 256) * 256; };
+  const size_t ntr = (size_t)p->nx * p->ny;
+  size_t ntab = 0;
+  for (int l = 0; l < nt; ++l) ntab += ((p->shift && p->shift[l]) ? 1 : 0) + ((p->gain && p->gain[l]) ? 1 : 0);
+  const size_t b_w = al(tw.size() * sizeof(float)), b_t = al(t3.size() * sizeof(float));
+  const size_t b_rel = al(ntr_pad * sizeof(double)), b_tab = al(ntr * sizeof(float));
+  void *ws = nullptr;
+  if ((rc = ctx->workspace(b_w + 3 * b_t + b_rel + ntab * b_tab, &ws))) return rc;
+  char *wp = static_cast<char *>(ws);
+  size_t off = 0;
+  const auto up = [&](const void *src, size_t bytes, size_t slot) -> const void * {
+    void *dst = wp + off;
+    off += slot;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream) == hipSuccess ? dst : nullptr;
+  };
+  a.W = static_cast<const crava::cx<float> *>(up(tw.data(), tw.size() * sizeof(float), b_w));
+  a.t3 = static_cast<const float *>(up(t3.data(), t3.size() * sizeof(float), b_t));
+  a.tau2 = static_cast<const float *>(up(tau2.data(), tau2.size() * sizeof(float), b_t));
+  a.wg2 = static_cast<const float *>(up(wg2.data(), wg2.size() * sizeof(float), b_t));
+  a.rel = static_cast<const double *>(up(p->rel_thick, ntr_pad * sizeof(double), b_rel));
+  bool ok = a.W && a.t3 && a.tau2 && a.wg2 && a.rel;
+  for (int l = 0; l < nt; ++l) {
+    a.shift[l] = (p->shift && p->shift[l]) ? static_cast<const float *>(up(p->shift[l], ntr * sizeof(float), b_tab))
+                                           : nullptr;
+    a.gain[l] = (p->gain && p->gain[l]) ? static_cast<const float *>(up(p->gain[l], ntr * sizeof(float), b_tab))
+                                        : nullptr;
+    ok = ok && (!(p->shift && p->shift[l]) || a.shift[l]) && (!(p->gain && p->gain[l]) || a.gain[l]);
+    a.g[l] = static_cast<float *>(grids[l]);
+  }
+  if (!ok) { set_error("table upload failed"); return CRAVA_HIP_E_HIP; }
+  // the host tables above go out of scope: finish the uploads first
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  a.dz = p->dz;
+  a.wnc = p->wnc;
+  a.low_cut = p->low_cut;
+  a.high_cut = p->high_cut;
+  a.sq = std::sqrt((float)p->nzp);
+  a.inv_n = (float)(1.0 / (double)p->nzp);
+  a.lz = p->lz;
+  a.nt = nt;
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.nz = p->nz;
+  a.nxp = p->nxp;
+  a.nyp = p->nyp;
+  a.nzp = p->nzp;
+  a.rnxp = 2 * (p->nxp / 2 + 1);
+  if (crava::swd_fused_available(p->nzp) && f->stw[2]) return crava::launch_swd_fused(a, f->stw[2], ctx->stream);
+  if ((rc = crava::launch_swd_prepare(a, ctx->stream))) return rc;
+  if ((rc = f->pass(2, grids, nt, false, nullptr, ctx->stream))) return rc;
+  if ((rc = crava::launch_swd_filter(a, ctx->stream))) return rc;
+  if ((rc = f->pass(2, grids, nt, true, nullptr, ctx->stream))) return rc;
+  return crava::launch_swd_finalize(a, ctx->stream);
+}
+
+int crava_hip_interpolate_seismic(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp,
+                                  float energy_threshold, void *grid, int *n_flagged, int *n_zero) {
+  if (!ctx || !grid) { set_error("NULL argument to crava_hip_interpolate_seismic"); return CRAVA_HIP_E_INVALID; }
+  if (nx < 1 || ny < 1 || nz < 1 || nx > nxp || ny > nyp || nz > nzp) {
+    set_error("invalid volume / padded grid sizes");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if ((long long)nx * ny > (1LL << 30)) { set_error("too many traces"); return CRAVA_HIP_E_INVALID; }
+  const int rnxp = 2 * (nxp / 2 + 1);
+  float *g = static_cast<float *>(grid);
+  const size_t ntr = (size_t)nx * ny;
+  void *ws = nullptr;
+  int rc;
+  if ((rc = ctx->workspace(ntr * sizeof(float), &ws))) return rc;
+  if ((rc = crava::launch_isq_energy(g, nx, ny, nz, nyp, rnxp, static_cast<float *>(ws), ctx->stream))) return rc;
+  std::vector<float> energy(ntr);
+  CRAVA_HIP_CHECK(hipMemcpyAsync(energy.data(), ws, ntr * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  crava::IsqSchedule sch;
+  crava::isq_schedule(energy.data(), nx, ny, energy_threshold, &sch);
+  if (n_flagged) *n_flagged = sch.n_flagged;
+  if (n_zero) *n_zero = sch.n_zero;
+  if (sch.imin > sch.imax) {
+    set_error("no trace has energy above the threshold");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (!sch.ops.empty()) {
+    if ((rc = ctx->workspace(sch.ops.size() * sizeof(crava::IsqOp), &ws))) return rc;
+    const crava::IsqOp *dops = static_cast<const crava::IsqOp *>(ws);
+    CRAVA_HIP_CHECK(hipMemcpyAsync(ws, sch.ops.data(), sch.ops.size() * sizeof(crava::IsqOp), hipMemcpyHostToDevice,
+                                   ctx->stream));
+    for (size_t v = 0; v + 1 < sch.level_start.size(); ++v) {
+      const int b = sch.level_start[v], e = sch.level_start[v + 1];
+      if ((rc = crava::launch_isq_interp(g, dops + b, e - b, nx, nyp, nzp, rnxp, ctx->stream))) return rc;
+    }
+  }
+  if ((rc = crava::launch_isq_extrapolate(g, nx, ny, nz, nxp, nyp, nzp, rnxp, sch.imin, sch.imax, sch.jmin, sch.jmax,
+                                          ctx->stream)))
+    return rc;
+  // the schedule's host copy goes out of scope
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return CRAVA_HIP_OK;
 }
 
 }  // extern "C"

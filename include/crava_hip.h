@@ -384,6 +384,59 @@ typedef struct {
 int crava_hip_synthetic_seismic(crava_hip_ctx *ctx, const crava_hip_synt_params *p, const void *alpha,
                                 const void *beta, const void *rho, void *const *out);
 
+/* ---- Non-constant-thickness data preparation --------------------------- */
+
+/* Divides the seismic of every padded trace by the local scale wavelet, in
+ * place of the trace loop of Crava::divideDataByScaleWavelet
+ * (src/crava.cpp:592-714) with Crava::computeAdjustmentFactor (:726-817),
+ * for 1-D wavelets.  Per trace (i < nxp, j < nyp): scale by 1/sqrt(nzp),
+ * taper k > nz (FFTGrid::getDistToBoundary), r2c, multiply by the adjustment
+ * factor inside (lowCut, highCut) at the trace's frequency step
+ * nz 1000 / (relThick lz nzp) and zero outside, c2r, scale by 1/sqrt(nzp).
+ * The local wavelet of trace (i, j) is the angle's wavelet shifted and
+ * scaled with the shift / gain at the mirrored volume trace (iInd, jInd) of
+ * the reference and stretched by relThick(i, j).  All arrays are host
+ * memory; a NULL shift / gain table or entry means shift 0 / gain 1.
+ * Floating point: within rounding of the reference (the factor's
+ * difference-operator spectrum is evaluated in closed form). */
+typedef struct {
+  int nx, ny, nz;                 /* volume */
+  int nxp, nyp, nzp;              /* padded FFT grid */
+  int ntheta;                     /* 1..8 */
+  const float *wavelets;          /* [ntheta][nzp] global 1-D wavelets in FFT order (Wavelet::rAmp_) */
+  const float *wavelet_norm;      /* [ntheta] Wavelet::getNorm() */
+  const float *A;                 /* [ntheta][3] reflection matrix rows (Crava::A_) */
+  const double *prior_var0;       /* [3][3] prior covariance (Crava::priorVar0_) */
+  const float *corr_t;            /* [nzp] SeismicParametersHolder::getPriorCorrTFiltered(nz, nzp) */
+  const float *err_var;           /* [ntheta] errThetaCov_[l][l] as float */
+  const double *rel_thick;        /* [nyp][nxp] Simbox::getRelThick(i, j) of every padded trace, > 0 */
+  const float *const *shift;      /* [ntheta] local time shifts [ny][nx] (Wavelet::getLocalTimeshift) */
+  const float *const *gain;       /* [ntheta] local gains [ny][nx] (Wavelet::getLocalGainFactor) */
+  float dz;                       /* wavelet sampling interval (Wavelet::dz_, ms), > 0 */
+  double lz;                      /* Simbox::getlz(), > 0 */
+  float low_cut, high_cut;        /* Crava::lowCut_, highCut_ (Hz) */
+  float wnc;                      /* Crava::wnc_, white-noise component */
+} crava_hip_swdiv_params;
+
+/* grids[ntheta]: real padded float grids [nzp][nyp][2(nxp/2+1)] in the time
+ * domain, divided in place.  Asynchronous on the context's stream once the
+ * tables are uploaded. */
+int crava_hip_divide_by_scale_wavelet(crava_hip_ctx *ctx, const crava_hip_swdiv_params *p, void *const *grids);
+
+/* FFTGrid::interpolateSeismic (src/fftgrid.cpp:2564-2655) on one real padded
+ * float grid, in place: volume traces whose energy over k < nz is at most
+ * energy_threshold times the mean are rebuilt from their neighbours in the
+ * reference's order (interpolateTrace, :2658-2696), then every padded trace
+ * outside the box of traces above the threshold is its clamped fill trace
+ * times the 3-D taper (extrapolateSeismic, :2699-2733).  Bit-exact.
+ * n_flagged / n_zero (optional) receive the number of traces at or below the
+ * threshold and of those with zero energy.  Synchronises the context's
+ * stream (the schedule is derived from the energies on the host).  Fails
+ * with CRAVA_HIP_E_INVALID when no trace is above the threshold (the
+ * reference then extrapolates from traces outside the volume). */
+int crava_hip_interpolate_seismic(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp,
+                                  float energy_threshold, void *grid, int *n_flagged, int *n_zero);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,12 @@ def orc():
         L.orc_merge_mean_4d.argtypes = [LL, PP, PP, PP]
         L.orc_merge_cov_4d.argtypes = [LL, PP, PP, PP, PP]
         L.orc_synthetic_seismic.argtypes = [I, I, I, I, I, I, I, VP, VP, F, PP, PP, PP, VP, VP, VP, PP]
+        L.orc_divide_by_scale_wavelet.argtypes = [I, I, I, I, I, I, I, VP, VP, VP, VP, VP, VP, VP, PP, PP,
+                                                  F, D, F, F, F, PP, I, I]
+        L.orc_interpolate_seismic.argtypes = [I, I, I, I, I, I, F, VP, C.POINTER(C.c_int)]
+        L.orc_dist_to_boundary.argtypes = [I, I, I]
+        L.orc_dist_to_boundary.restype = C.c_float
+        L.orc_fill_number.argtypes = [I, I, I]
         L.orc_posterior.restype = C.c_longlong
         L.orc_posterior_f64.restype = C.c_longlong
         L.orc_simulate_points.restype = C.c_longlong
@@ -87,7 +93,7 @@ def ref():
         if not ref_available():
             raise FileNotFoundError(REF_SO)
         L = C.CDLL(REF_SO)
-        I, F, LL, PP = C.c_int, C.c_float, C.c_longlong, C.POINTER(VP)
+        I, F, D, LL, PP = C.c_int, C.c_float, C.c_double, C.c_longlong, C.POINTER(VP)
         L.ref_fft3d_r2c.argtypes = [VP, I, I, I, I]
         L.ref_fft3d_c2r.argtypes = [VP, I, I, I, I]
         L.ref_fft1d_r2c.argtypes = [VP, I]
@@ -96,6 +102,8 @@ def ref():
         L.ref_posterior.argtypes = [I, I, I, I, VP, VP, F, VP, VP, VP, VP, VP, VP, PP, PP, PP, VP]
         L.ref_simulate_points.argtypes = [LL, PP, PP]
         L.ref_synthetic_seismic.argtypes = [I, I, I, I, I, I, I, VP, VP, F, PP, PP, PP, VP, VP, VP, PP, I, I]
+        L.ref_divide_by_scale_wavelet.argtypes = [I, I, I, I, I, I, I, VP, VP, VP, VP, VP, VP, VP, PP, PP,
+                                                  F, D, F, F, F, PP, I, I]
         L.ref_posterior.restype = C.c_longlong
         L.ref_simulate_points.restype = C.c_longlong
         L.ref_wall_seconds.restype = C.c_double
@@ -298,6 +306,54 @@ def synthetic_seismic(nx, ny, nz, nxp, nyp, nzp, A, wavelets, dz, alpha, beta, r
     else:
         orc().orc_synthetic_seismic(*args)
     return out
+
+
+# ----------------------------------------------- non-constant-thickness path
+def divide_by_scale_wavelet(nx, ny, nz, nxp, nyp, nzp, grids, wavelets, wnorm, A, var0, corr_t, err_var,
+                            rel_thick, dz, lz, low_cut, high_cut, wnc, shift=None, gain=None,
+                            which: str = "oracle", i_range=(0, 0)):
+    """Crava::divideDataByScaleWavelet on ntheta real padded float grids;
+    returns the divided copies.  rel_thick: [nyp][nxp] float64; shift/gain:
+    None or ntheta entries, each None or [ny][nx].  which="ref" runs the
+    reference's FFTW 2.1.2 with the global wavelet's per-trace round trip.
+    Traces with i in i_range only ((0, 0) = all); the others are left as
+    given."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    nt = A.shape[0]
+    out = [np.array(g, dtype=np.float32, order="C", copy=True) for g in grids]
+    wav = np.ascontiguousarray(wavelets, dtype=np.float32)
+    wn = np.ascontiguousarray(wnorm, dtype=np.float32)
+    v0 = np.ascontiguousarray(var0, dtype=np.float64)
+    ct = np.ascontiguousarray(corr_t, dtype=np.float32)
+    ev = np.ascontiguousarray(err_var, dtype=np.float32)
+    rt = np.ascontiguousarray(rel_thick, dtype=np.float64)
+    assert wav.shape == (nt, nzp) and ct.shape == (nzp,) and rt.shape == (nyp, nxp)
+    keep = [_opt_tables(t, nt, nx, ny) for t in (shift, gain)]
+    args = (nx, ny, nz, nxp, nyp, nzp, nt, wav.ctypes.data, wn.ctypes.data, A.ctypes.data, v0.ctypes.data,
+            ct.ctypes.data, ev.ctypes.data, rt.ctypes.data, keep[0][1], keep[1][1], dz, lz, low_cut, high_cut,
+            wnc, _ptrs(out))
+    if which == "ref":
+        ref().ref_divide_by_scale_wavelet(*args, i_range[0], i_range[1])
+    else:
+        orc().orc_divide_by_scale_wavelet(*args, i_range[0], i_range[1])
+    return out
+
+
+def interpolate_seismic(nx, ny, nz, nxp, nyp, nzp, threshold, grid):
+    """FFTGrid::interpolateSeismic on a real padded float grid; returns
+    (new grid, traces flagged, traces with zero energy)."""
+    g = np.array(grid, dtype=np.float32, order="C", copy=True)
+    nz0 = C.c_int(0)
+    n = orc().orc_interpolate_seismic(nx, ny, nz, nxp, nyp, nzp, threshold, g.ctypes.data, C.byref(nz0))
+    return g, n, nz0.value
+
+
+def dist_to_boundary(i, n, np_):
+    return orc().orc_dist_to_boundary(i, n, np_)
+
+
+def fill_number(i, n, np_):
+    return orc().orc_fill_number(i, n, np_)
 
 
 # ------------------------------------------------------------ time-lapse merge

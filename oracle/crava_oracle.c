@@ -1281,3 +1281,361 @@ void orc_merge_cov_4d(long long n, float **ss, float **dd, float **sd, float **s
       for (int b = a; b < 3; b++, q++) { sigma[q][2 * p] = C[a][b].re; sigma[q][2 * p + 1] = C[a][b].im; }
   }
 }
+
+/* ========================================================================
+ * Wavelet division of the non-constant-thickness path:
+ * Crava::divideDataByScaleWavelet (src/crava.cpp:592-714) with
+ * Crava::computeAdjustmentFactor (:726-817),
+ * computeElasticImpedanceTimeCovariance (:476-487),
+ * computeReflectionCoefficientTimeCovariance (:490-504),
+ * FFTGrid::getDistToBoundary (src/fftgrid.cpp:1374-1412), the Wavelet1D
+ * copy + shiftAndScale (src/wavelet.cpp:53-88, src/wavelet1D.cpp:497-561),
+ * the difference operators (src/wavelet1D.cpp:420-470),
+ * Wavelet::findNormWithinFrequencyBand (src/wavelet.cpp:811-833),
+ * Wavelet::multiplyRAmpByConstant (:466-474), Wavelet::getCAmp (:309-384).
+ * One-dimensional wavelets: the local wavelet is the global one shifted and
+ * scaled with the shift / gain of the mirrored trace (iInd, jInd), read
+ * through the stretch sf = relThick(i, j).
+ * The reference re-derives the global wavelet's spectrum once per trace
+ * through a float c2r / r2c round trip (Wavelet's copy constructor and
+ * fft1DInPlace); this restatement keeps the spectrum of the given samples.
+ * ====================================================================== */
+#define ORC_RMISSING (-99999.0f)                 /* src/definitions.h:35 */
+
+/* FFTGrid::getDistToBoundary, src/fftgrid.cpp:1374-1412 */
+float orc_dist_to_boundary(int i, int n, int np)
+{
+  float dist = 0.0f;
+  if (i < np) {
+    if (i >= n) {
+      const int mn = n < np - n ? n : np - n;
+      const float taper = (float)(mn / 2.1);
+      const int below = np - i, above = i - (n - 1);
+      if (above < below) dist = (float)above / taper;
+      else dist = (float)below / taper;
+    }
+  } else {
+    dist = ORC_RMISSING;
+  }
+  return dist;
+}
+
+/* FFTGrid::getFillNumber, src/fftgrid.cpp:1319-1357 (i < np) */
+int orc_fill_number(int i, int n, int np)
+{
+  if (i >= np) return -99999;
+  if (i < n) return i;
+  const int below = np - i - 1, above = i - n + 1;
+  if (above < below) return (n - above) > n / 2 ? (n - above) : n / 2;
+  return below < n / 2 ? below : n / 2;
+}
+
+/* trace index of the wavelet's local properties, src/crava.cpp:626-642 */
+static int swd_mirror(int i, int n, int np)
+{
+  int r = i;
+  if (r > 3 * n - 1) r = 0;
+  if (r > (np + n) / 2) r = np - r;
+  if (r >= n) r = 2 * n - r - 1;
+  return r;
+}
+
+/* Wavelet::getCAmp(k, scale) for k < nzp/2 + 1, src/wavelet.cpp:328-357,381-383 */
+static ocpx camp_scaled(const ocpx *c, int cn, int k, float scale)
+{
+  ocpx v;
+  const float omega = (float)k / scale;
+  if (omega >= cn) { v.re = 0.0f; v.im = 0.0f; }
+  else {
+    const int omL = (int)floorf(omega);
+    int omU = (int)floorf(omega) + 1;
+    if (omU >= cn) omU -= 1;
+    const float d = omega - (float)omL;
+    v.re = c[omL].re * (1.0f - d) + c[omU].re * d;
+    v.im = (c[omL].im * (1.0f - d) + c[omU].im * d);
+  }
+  v.re /= scale;
+  v.im /= scale;
+  return v;
+}
+
+/*
+ *   wavelets [nt][nzp], wnorm [nt] (Wavelet::getNorm), A [nt][3],
+ *   var0 [3][3] priorVar0_, corr_t [nzp] getPriorCorrTFiltered(nz, nzp),
+ *   err_var [nt] errThetaCov_[l][l], rel_thick [nyp][nxp] Simbox::getRelThick,
+ *   shift / gain: NULL or nt pointers, each NULL or [ny][nx];
+ *   grids [nt] real padded [nzp][nyp][rnxp], divided in place;
+ *   traces [i0, i1) x [0, nyp) only (i1 <= 0: all), the others untouched.
+ */
+void orc_divide_by_scale_wavelet(int nx, int ny, int nz, int nxp, int nyp, int nzp, int nt,
+                                 const float *wavelets, const float *wnorm, const float *A,
+                                 const double *var0, const float *corr_t, const float *err_var,
+                                 const double *rel_thick, const float *const *shift,
+                                 const float *const *gain, float dz, double lz, float low_cut,
+                                 float high_cut, float wnc, float **grids, int i0, int i1)
+{
+  const int rnxp = 2 * (nxp / 2 + 1), cn = nzp / 2 + 1;
+  const long long plane = (long long)rnxp * nyp;
+  const float tol_fac = 0.05f;                                   /* :734 */
+  if (i1 <= 0 || i1 > nxp) i1 = nxp;
+  float *r = (float *)malloc(sizeof(float) * nzp);
+  float *rc = (float *)malloc(sizeof(float) * nzp);
+  float *wt = (float *)malloc(sizeof(float) * nzp);
+  float *fw = (float *)malloc(sizeof(float) * nzp);
+  float *bw = (float *)malloc(sizeof(float) * nzp);
+  float *cw = (float *)malloc(sizeof(float) * nzp);
+  ocpx *W = (ocpx *)malloc(sizeof(ocpx) * cn);
+  ocpx *RC = (ocpx *)malloc(sizeof(ocpx) * cn);
+  ocpx *R = (ocpx *)malloc(sizeof(ocpx) * cn);
+  ocpx *WL = (ocpx *)malloc(sizeof(ocpx) * cn);
+  ocpx *E3 = (ocpx *)malloc(sizeof(ocpx) * cn);
+  const float sq = sqrtf((float)nzp);
+  for (int l = 0; l < nt; l++) {
+    const float *Al = A + 3 * l;
+    float *g = grids[l];
+    spectrum_f(wavelets + (long long)l * nzp, nzp, W);
+    /* rcSpecIntens: computeReflectionCoefficientTimeCovariance, then r2c */
+    double ei_var = 0.0;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) ei_var += (double)(Al[a] * (float)var0[3 * a + b] * Al[b]);
+    for (int k = 0; k < nzp; k++) rc[k] = (float)(ei_var * corr_t[k]);
+    {
+      const float first = rc[0];
+      float prev = rc[nzp - 1];
+      for (int t = 0; t < nzp - 1; t++) {
+        const float curr = rc[t], next = rc[t + 1];
+        rc[t] = 2 * curr - next - prev;
+        prev = curr;
+      }
+      rc[nzp - 1] = 2 * rc[nzp - 1] - first - prev;
+    }
+    spectrum_f(rc, nzp, RC);
+    float mod_w = wnorm[l];                                        /* :765-768 */
+    mod_w *= mod_w;
+    const float maxfreq = (float)((nzp / 2) * 1000.0 * nz) / (float)(lz * nzp);
+    mod_w *= maxfreq / high_cut;
+    const float ev = err_var[l], wn = wnorm[l];
+    const float d_om = (float)(1.0 / ((double)((float)nzp * dz) * 0.001));
+    int lo_i = (int)floorf(low_cut / d_om), hi_i = (int)ceilf(high_cut / d_om);
+    if (lo_i < 0) lo_i = 0;
+    if (hi_i > cn) hi_i = cn;
+    for (int i = i0; i < i1; i++)
+      for (int j = 0; j < nyp; j++) {
+        const int ii = swd_mirror(i, nx, nxp), jj = swd_mirror(j, ny, nyp);
+        const long long tr = (long long)j * rnxp + i;
+        int k;
+        for (k = 0; k < nzp; k++) {                                /* :644-657 */
+          r[k] = g[k * plane + tr] / sq;
+          if (k > nz) {
+            const float dist = orc_dist_to_boundary(k, nz, nzp);
+            const float f = 1 - dist * dist;
+            r[k] *= (f < 0.0f) ? 0.0f : f;
+          }
+        }
+        spectrum_f(r, nzp, R);
+        /* local wavelet (Wavelet::getLocalTimeshift / getLocalGainFactor) */
+        float ts = 0.0f, gn = 1.0f;
+        if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+          if (shift && shift[l]) ts = shift[l][(long long)jj * nx + ii];
+          if (gain && gain[l]) gn = gain[l][(long long)jj * nx + ii];
+        }
+        const float ishift = ts / dz;
+        for (int q = 0; q < cn; q++) {
+          ocpx am, tmp;
+          am.re = (float)(gn * cos(2.0 * (M_PI * (ishift) * q) / (float)nzp));
+          am.im = (float)(gn * sin(-2.0 * (M_PI * (ishift) * q) / (float)nzp));
+          tmp.re = am.re * W[q].re - am.im * W[q].im;
+          tmp.im = am.im * W[q].re + am.re * W[q].im;
+          WL[q] = tmp;
+        }
+        /* errorSmooth3 = central(backward(forward(wLocal))) in time; the
+         * copy takes wLocal back to its samples (invFFT1DInPlace) */
+        c2r_f(WL, nzp, wt);
+        for (int t = 0; t < nzp; t++) wt[t] = (float)(wt[t] * (1.0 / (double)nzp));
+        for (int t = 0; t < nzp; t++) fw[t] = (t == nzp - 1) ? (wt[0] - wt[t]) : (wt[t + 1] - wt[t]);
+        for (int t = 0; t < nzp; t++) bw[t] = (t == 0) ? (fw[0] - fw[nzp - 1]) : (fw[t] - fw[t - 1]);
+        for (int t = 0; t < nzp; t++) {
+          if (t == 0) cw[t] = (float)(0.5 * (bw[t + 1] - bw[nzp - 1]));
+          else if (t == nzp - 1) cw[t] = (float)(0.5 * (bw[0] - bw[t - 1]));
+          else cw[t] = (float)(0.5 * (bw[t + 1] - bw[t - 1]));
+        }
+        spectrum_f(cw, nzp, E3);
+        spectrum_f(wt, nzp, WL);                                   /* wLocal->fft1DInPlace() */
+        double n2 = 0.0;
+        const double fac = 1.0 / (double)nzp;
+        for (int q = lo_i; q < hi_i; q++) n2 += 2.0 * fac * (double)(E3[q].re * E3[q].re + E3[q].im * E3[q].im);
+        if (lo_i == 0) n2 -= fac * (double)(E3[0].re * E3[0].re + E3[0].im * E3[0].im);
+        const float c = 1.0f / (float)sqrt(n2);
+        /* multiplyRAmpByConstant: floats of index < nzp scaled, the rest of
+         * the interleaved half spectrum set to RMISSING */
+        float *e3f = (float *)E3;
+        for (int t = 0; t < 2 * cn; t++) {
+          if (t < nzp) e3f[t] *= c;
+          else e3f[t] = ORC_RMISSING;
+        }
+        const double rel_t = rel_thick[(long long)j * nxp + i];
+        const float sf = (float)(rel_t * 1.0f);                   /* getLocalStretch = 1 */
+        const double delta_f = (double)nz * 1000.0 / (rel_t * lz * (double)nzp);
+        for (k = 0; k < cn; k++) {                                 /* :772-814 */
+          const ocpx w_loc = camp_scaled(WL, cn, k, sf);
+          const float w_loc2 = (w_loc.re * w_loc.re + w_loc.im * w_loc.im);
+          const ocpx w_glob = W[k];
+          const float w_glob2 = (w_glob.re * w_glob.re + w_glob.im * w_glob.im);
+          ocpx e = camp_scaled(WL, cn, k, sf);
+          float s_loc = (e.re * e.re + e.im * e.im) / (wn * wn);
+          e = camp_scaled(E3, cn, k, sf);
+          s_loc += e.re * e.re + e.im * e.im;
+          e = WL[k];
+          float s_glob = (e.re * e.re + e.im * e.im) / (wn * wn);
+          e = E3[k];
+          s_glob += e.re * e.re + e.im * e.im;
+          float sig_loc = ev * (wnc + (1.0f - wnc) * 0.5f * s_loc);
+          const float sig_glob = ev * (wnc + (1.0f - wnc) * 0.5f * s_glob);
+          const float tau2 = fabsf(RC[k].re);
+          float wg2a = w_glob2, wl2a = w_loc2;
+          if (w_glob2 < tol_fac * mod_w) wg2a = (float)(0.5f * (w_glob2 + tol_fac * mod_w));
+          if (w_loc2 < tol_fac * mod_w) wl2a = (float)(0.5f * (w_loc2 + tol_fac * mod_w));
+          if (wl2a < tol_fac * wg2a) {
+            wl2a = (float)(0.5f * (w_loc2 + tol_fac * wg2a));
+            sig_loc *= wl2a / w_loc2;
+          }
+          ocpx adj;
+          adj.re = (w_loc.re / (tau2 * wl2a + sig_loc)) / (wg2a / (tau2 * w_glob2 + sig_glob));
+          adj.im = (w_loc.im / (tau2 * wl2a + sig_loc)) / (wg2a / (tau2 * w_glob2 + sig_glob));
+          if ((delta_f * k < high_cut) && (delta_f * k > low_cut)) {
+            const float tmp = R[k].re * adj.re - R[k].im * adj.im;
+            R[k].im = R[k].im * adj.re + R[k].re * adj.im;
+            R[k].re = tmp;
+          } else {
+            R[k].im = 0.0f;
+            R[k].re = 0.0f;
+          }
+        }
+        c2r_f(R, nzp, r);
+        for (k = 0; k < nzp; k++) g[k * plane + tr] = r[k] / sq;
+      }
+  }
+  free(r); free(rc); free(wt); free(fw); free(bw); free(cw);
+  free(W); free(RC); free(R); free(WL); free(E3);
+}
+
+/* ========================================================================
+ * FFTGrid::interpolateSeismic (src/fftgrid.cpp:2564-2655) with
+ * interpolateTrace (:2658-2696) and extrapolateSeismic (:2699-2733) on a
+ * real padded grid [nzp][nyp][rnxp].  Returns the number of traces flagged
+ * for interpolation; *n_zero receives those with zero energy.
+ * ====================================================================== */
+static int isq_interp_trace(float *g, long long plane, int rnxp, int nx, int ny, int nzp, int index,
+                            short *flags, int i, int j)
+{
+  int nt = 0;
+  const int left = (i > 0 && (flags[index - 1] % 2) == 0);
+  const int right = (i < nx - 1 && (flags[index + 1] % 2) == 0);
+  const int up = (j > 0 && (flags[index - nx] % 2) == 0);
+  const int down = (j < ny - 1 && (flags[index + nx] % 2) == 0);
+  nt = left + right + up + down;
+  if (nt > 1) {
+    const long long t = (long long)j * rnxp + i;
+    for (int k = 0; k < nzp; k++) {
+      const long long o = k * plane + t;
+      float m = left ? g[o - 1] : 0.0f;
+      if (right) m += g[o + 1];
+      if (up) m += g[o - rnxp];
+      if (down) m += g[o + rnxp];
+      g[o] = m / (float)nt;
+    }
+    flags[index]--;
+  }
+  return flags[index];
+}
+
+int orc_interpolate_seismic(int nx, int ny, int nz, int nxp, int nyp, int nzp, float threshold, float *g,
+                            int *n_zero)
+{
+  const int rnxp = 2 * (nxp / 2 + 1);
+  const long long plane = (long long)rnxp * nyp;
+  short *flags = (short *)malloc(sizeof(short) * nx * ny);
+  float *emap = (float *)malloc(sizeof(float) * nx * ny);
+  int flag = 0, imin = nx, imax = 0, jmin = ny, jmax = 0, index = 0;
+  float total = 0.0f;
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      float e = 0.0f;
+      for (int k = 0; k < nz; k++) {
+        const float v = g[k * plane + (long long)j * rnxp + i];
+        e += v * v;
+      }
+      emap[index] = e;
+      total += e;
+      index++;
+    }
+  const float limit = threshold * total / (float)(nx * ny);
+  int n_inter = 0, n_inter0 = 0;
+  index = 0;
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      int cur = 0;
+      if (emap[index] <= limit) {
+        cur = 1;
+        n_inter++;
+        if (emap[index] == 0.0f) n_inter0++;
+      }
+      flags[index] = (short)(flag + cur);
+      if (cur == 1) flag = 2;
+      else {
+        if (i < imin) imin = i;
+        if (i > imax) imax = i;
+        if (j < jmin) jmin = j;
+        if (j > jmax) jmax = j;
+      }
+      index++;
+    }
+  int ci = 0;
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      if ((flags[ci] % 2) == 1) {
+        if ((isq_interp_trace(g, plane, rnxp, nx, ny, nzp, ci, flags, i, j) % 2) == 0) {
+          index = ci - 1;
+          while (index >= 0 && flags[index] > 1) {
+            if ((flags[index] % 2) == 1)
+              isq_interp_trace(g, plane, rnxp, nx, ny, nzp, index, flags, index % nx, index / nx);
+            index--;
+          }
+          if (index < 0) index = 0;
+          index++;
+          while (flags[index - 1] == 0 && index <= ci) {
+            if (flags[index] > 1) flags[index] -= 2;
+            index++;
+          }
+        }
+      }
+      ci++;
+    }
+  /* extrapolateSeismic */
+  for (int j = 0; j < nyp; j++) {
+    int refj = orc_fill_number(j, ny, nyp);
+    if (refj < jmin) refj = jmin;
+    else if (refj > jmax) refj = jmax;
+    for (int i = 0; i < nxp; i++) {
+      if (i < imin || i > imax || j < jmin || j > jmax) {
+        int refi = orc_fill_number(i, nx, nxp);
+        if (refi < imin) refi = imin;
+        else if (refi > imax) refi = imax;
+        for (int k = 0; k < nzp; k++) {
+          const float value = g[k * plane + (long long)refj * rnxp + refi];
+          const float dx = orc_dist_to_boundary(i, nx, nxp);
+          const float dy = orc_dist_to_boundary(j, ny, nyp);
+          const float dzz = orc_dist_to_boundary(k, nz, nzp);
+          double b = 1.0 - dx * dx - dy * dy - dzz * dzz;
+          if (b < 0.0) b = 0.0;
+          const float mult = (float)pow(b, 3);
+          g[k * plane + (long long)j * rnxp + i] = mult * value;
+        }
+      }
+    }
+  }
+  free(flags);
+  free(emap);
+  if (n_zero) *n_zero = n_inter0;
+  return n_inter;
+}

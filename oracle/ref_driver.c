@@ -296,3 +296,213 @@ void ref_synthetic_seismic(int nx, int ny, int nz, int nxp, int nyp, int nzp, in
   fftwnd_destroy_plan(p2);
   free(imp); free(r); free(w); free(loc);
 }
+
+/* Crava::divideDataByScaleWavelet (src/crava.cpp:592-714) with
+ * computeAdjustmentFactor (:726-817) on the reference's FFTW 2.1.2 1-D
+ * transforms, including the global wavelet's state changes: Wavelet's copy
+ * constructor (src/wavelet.cpp:53-88) takes the global wavelet back to its
+ * samples (invFFT1DInPlace) and computeAdjustmentFactor transforms it again
+ * (fft1DInPlace), so from the second trace on its spectrum is re-derived
+ * through a float c2r / r2c round trip per trace.  Arguments as
+ * orc_divide_by_scale_wavelet; traces [i0, i1) x [0, nyp) only (i1 <= 0:
+ * all), so the CPU baseline can time a bounded sample. */
+static int ref_mirror(int i, int n, int np)
+{
+  int r = i;
+  if (r > 3 * n - 1) r = 0;
+  if (r > (np + n) / 2) r = np - r;
+  if (r >= n) r = 2 * n - r - 1;
+  return r;
+}
+
+static float ref_dist(int i, int n, int np)
+{
+  float dist = 0.0f;
+  if (i < np) {
+    if (i >= n) {
+      const int mn = n < np - n ? n : np - n;
+      const float taper = (float)(mn / 2.1);
+      const int below = np - i, above = i - (n - 1);
+      dist = (above < below) ? (float)above / taper : (float)below / taper;
+    }
+  } else {
+    dist = -99999.0f;
+  }
+  return dist;
+}
+
+static fftw_complex ref_camp(const fftw_complex *c, int cn, int k, float scale)
+{
+  fftw_complex v;
+  const float omega = (float)k / scale;
+  if (omega >= cn) { v.re = 0.0f; v.im = 0.0f; }
+  else {
+    const int omL = (int)floor(omega);
+    int omU = (int)floor(omega) + 1;
+    if (omU >= cn) omU -= 1;
+    const float d = omega - (float)omL;
+    v.re = c[omL].re * (1.0f - d) + c[omU].re * d;
+    v.im = (c[omL].im * (1.0f - d) + c[omU].im * d);
+  }
+  v.re /= scale;
+  v.im /= scale;
+  return v;
+}
+
+void ref_divide_by_scale_wavelet(int nx, int ny, int nz, int nxp, int nyp, int nzp, int nt,
+                                 const float *wavelets, const float *wnorm, const float *A,
+                                 const double *var0, const float *corr_t, const float *err_var,
+                                 const double *rel_thick, const float *const *shift,
+                                 const float *const *gain, float dz, double lz, float low_cut,
+                                 float high_cut, float wnc, float **grids, int i0, int i1)
+{
+  const int rnxp = 2 * (nxp / 2 + 1), cn = nzp / 2 + 1, rn = 2 * cn;
+  const long long plane = (long long)rnxp * nyp;
+  const float tol_fac = 0.05f;
+  float *r = (float *)malloc(sizeof(float) * rn), *rc = (float *)malloc(sizeof(float) * rn);
+  float *gw = (float *)malloc(sizeof(float) * rn), *lw = (float *)malloc(sizeof(float) * rn);
+  float *fw = (float *)malloc(sizeof(float) * rn), *bw = (float *)malloc(sizeof(float) * rn);
+  float *cw = (float *)malloc(sizeof(float) * rn);
+  fftw_complex *R = (fftw_complex *)r, *RC = (fftw_complex *)rc, *GW = (fftw_complex *)gw;
+  fftw_complex *LW = (fftw_complex *)lw, *E3 = (fftw_complex *)cw;
+  rfftwnd_plan p1 = rfftwnd_create_plan(1, &nzp, FFTW_REAL_TO_COMPLEX, FFTW_ESTIMATE | FFTW_IN_PLACE);
+  rfftwnd_plan p2 = rfftwnd_create_plan(1, &nzp, FFTW_COMPLEX_TO_REAL, FFTW_ESTIMATE | FFTW_IN_PLACE);
+  const float sq = sqrtf((float)nzp);
+  const double inv_n = 1.0 / (double)nzp;
+  if (i1 <= 0 || i1 > nxp) i1 = nxp;
+  for (int l = 0; l < nt; l++) {
+    const float *Al = A + 3 * l;
+    float *g = grids[l];
+    memcpy(gw, wavelets + (long long)l * nzp, sizeof(float) * nzp);
+    int gw_real = 1;
+    double ei_var = 0.0;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) ei_var += (double)(Al[a] * (float)var0[3 * a + b] * Al[b]);
+    for (int k = 0; k < nzp; k++) rc[k] = (float)(ei_var * corr_t[k]);
+    {
+      const float first = rc[0];
+      float prev = rc[nzp - 1];
+      for (int t = 0; t < nzp - 1; t++) {
+        const float curr = rc[t], next = rc[t + 1];
+        rc[t] = 2 * curr - next - prev;
+        prev = curr;
+      }
+      rc[nzp - 1] = 2 * rc[nzp - 1] - first - prev;
+    }
+    rfftwnd_one_real_to_complex(p1, (fftw_real *)rc, RC);
+    float mod_w = wnorm[l];
+    mod_w *= mod_w;
+    const float maxfreq = (float)((nzp / 2) * 1000.0 * nz) / (float)(lz * nzp);
+    mod_w *= maxfreq / high_cut;
+    const float ev = err_var[l], wn = wnorm[l];
+    const float d_om = (float)(1.0 / ((double)((float)nzp * dz) * 0.001));
+    int lo_i = (int)floor(low_cut / d_om), hi_i = (int)ceil(high_cut / d_om);
+    if (lo_i < 0) lo_i = 0;
+    if (hi_i > cn) hi_i = cn;
+    for (int i = i0; i < i1; i++)
+      for (int j = 0; j < nyp; j++) {
+        const int ii = ref_mirror(i, nx, nxp), jj = ref_mirror(j, ny, nyp);
+        const long long tr = (long long)j * rnxp + i;
+        int k;
+        for (k = 0; k < nzp; k++) {
+          r[k] = g[k * plane + tr] / sq;
+          if (k > nz) {
+            const float dist = ref_dist(k, nz, nzp);
+            const float f = 1 - dist * dist;
+            r[k] *= (f < 0.0f) ? 0.0f : f;
+          }
+        }
+        rfftwnd_one_real_to_complex(p1, (fftw_real *)r, R);
+        /* createLocalWavelet1D: copy of the global wavelet's samples */
+        if (!gw_real) {
+          rfftwnd_one_complex_to_real(p2, GW, (fftw_real *)gw);
+          for (int t = 0; t < nzp; t++) gw[t] = (float)(gw[t] * inv_n);
+          gw_real = 1;
+        }
+        for (int t = 0; t < rn; t++) lw[t] = t < nzp ? gw[t] : 0.0f;
+        float ts = 0.0f, gn = 1.0f;
+        if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+          if (shift && shift[l]) ts = shift[l][(long long)jj * nx + ii];
+          if (gain && gain[l]) gn = gain[l][(long long)jj * nx + ii];
+        }
+        const float ishift = ts / dz;
+        rfftwnd_one_real_to_complex(p1, (fftw_real *)lw, LW);
+        for (int q = 0; q < cn; q++) {
+          fftw_complex am, tmp;
+          am.re = (float)(gn * cos(2.0 * (M_PI * (ishift) * q) / (float)nzp));
+          am.im = (float)(gn * sin(-2.0 * (M_PI * (ishift) * q) / (float)nzp));
+          tmp.re = am.re * LW[q].re - am.im * LW[q].im;
+          tmp.im = am.im * LW[q].re + am.re * LW[q].im;
+          LW[q] = tmp;
+        }
+        /* errorSmooth chain from wLocal's samples */
+        rfftwnd_one_complex_to_real(p2, LW, (fftw_real *)lw);
+        for (int t = 0; t < nzp; t++) lw[t] = (float)(lw[t] * inv_n);
+        for (int t = 0; t < nzp; t++) fw[t] = (t == nzp - 1) ? (lw[0] - lw[t]) : (lw[t + 1] - lw[t]);
+        for (int t = 0; t < nzp; t++) bw[t] = (t == 0) ? (fw[0] - fw[nzp - 1]) : (fw[t] - fw[t - 1]);
+        for (int t = 0; t < nzp; t++) {
+          if (t == 0) cw[t] = (float)(0.5 * (bw[t + 1] - bw[nzp - 1]));
+          else if (t == nzp - 1) cw[t] = (float)(0.5 * (bw[0] - bw[t - 1]));
+          else cw[t] = (float)(0.5 * (bw[t + 1] - bw[t - 1]));
+        }
+        rfftwnd_one_real_to_complex(p1, (fftw_real *)cw, E3);
+        double n2 = 0.0;
+        const double fac = 1.0 / (double)nzp;
+        for (int q = lo_i; q < hi_i; q++) n2 += 2.0 * fac * (double)(E3[q].re * E3[q].re + E3[q].im * E3[q].im);
+        if (lo_i == 0) n2 -= fac * (double)(E3[0].re * E3[0].re + E3[0].im * E3[0].im);
+        const float c = 1.0f / (float)sqrt(n2);
+        for (int t = 0; t < rn; t++) {
+          if (t < nzp) cw[t] *= c;
+          else cw[t] = -99999.0f;
+        }
+        rfftwnd_one_real_to_complex(p1, (fftw_real *)lw, LW);   /* wLocal->fft1DInPlace() */
+        if (gw_real) {                                           /* wGlobal->fft1DInPlace() */
+          rfftwnd_one_real_to_complex(p1, (fftw_real *)gw, GW);
+          gw_real = 0;
+        }
+        const double rel_t = rel_thick[(long long)j * nxp + i];
+        const float sf = (float)(rel_t * 1.0f);
+        const double delta_f = (double)nz * 1000.0 / (rel_t * lz * (double)nzp);
+        for (k = 0; k < cn; k++) {
+          const fftw_complex w_loc = ref_camp(LW, cn, k, sf);
+          const float w_loc2 = (w_loc.re * w_loc.re + w_loc.im * w_loc.im);
+          const fftw_complex w_glob = GW[k];
+          const float w_glob2 = (w_glob.re * w_glob.re + w_glob.im * w_glob.im);
+          fftw_complex e = ref_camp(LW, cn, k, sf);
+          float s_loc = (e.re * e.re + e.im * e.im) / (wn * wn);
+          e = ref_camp(E3, cn, k, sf);
+          s_loc += e.re * e.re + e.im * e.im;
+          e = LW[k];
+          float s_glob = (e.re * e.re + e.im * e.im) / (wn * wn);
+          e = E3[k];
+          s_glob += e.re * e.re + e.im * e.im;
+          float sig_loc = ev * (wnc + (1.0f - wnc) * 0.5f * s_loc);
+          const float sig_glob = ev * (wnc + (1.0f - wnc) * 0.5f * s_glob);
+          const float tau2 = fabsf(RC[k].re);
+          float wg2a = w_glob2, wl2a = w_loc2;
+          if (w_glob2 < tol_fac * mod_w) wg2a = (float)(0.5f * (w_glob2 + tol_fac * mod_w));
+          if (w_loc2 < tol_fac * mod_w) wl2a = (float)(0.5f * (w_loc2 + tol_fac * mod_w));
+          if (wl2a < tol_fac * wg2a) {
+            wl2a = (float)(0.5f * (w_loc2 + tol_fac * wg2a));
+            sig_loc *= wl2a / w_loc2;
+          }
+          fftw_complex adj;
+          adj.re = (w_loc.re / (tau2 * wl2a + sig_loc)) / (wg2a / (tau2 * w_glob2 + sig_glob));
+          adj.im = (w_loc.im / (tau2 * wl2a + sig_loc)) / (wg2a / (tau2 * w_glob2 + sig_glob));
+          if ((delta_f * k < high_cut) && (delta_f * k > low_cut)) {
+            const float tmp = R[k].re * adj.re - R[k].im * adj.im;
+            R[k].im = R[k].im * adj.re + R[k].re * adj.im;
+            R[k].re = tmp;
+          } else {
+            R[k].im = 0.0f;
+            R[k].re = 0.0f;
+          }
+        }
+        rfftwnd_one_complex_to_real(p2, R, (fftw_real *)r);
+        for (k = 0; k < nzp; k++) g[k * plane + tr] = r[k] / sq;
+      }
+  }
+  fftwnd_destroy_plan(p1);
+  fftwnd_destroy_plan(p2);
+  free(r); free(rc); free(gw); free(lw); free(fw); free(bw); free(cw);
+}

@@ -91,3 +91,30 @@ def test_host_mirror_links_only_the_c_abi():
     assert ours and ours <= set(declared_functions())
     assert not any(u.startswith("hip") for u in undef)
     C.CDLL(so)   # loads with its rpath to libcrava_hip.so
+
+
+@pytest.mark.parametrize("cname,pyname", [("crava_hip_post_params", "PostParams"),
+                                          ("crava_hip_batch_volume", "BatchVolume"),
+                                          ("crava_hip_synt_params", "SyntParams"),
+                                          ("crava_hip_swdiv_params", "SwdivParams")])
+def test_struct_layouts_match_the_header(cname, pyname, tmp_path):
+    """Every ctypes mirror of a header struct has the header's size and field
+    offsets (gcc on the header)."""
+    import subprocess
+    py = getattr(L, pyname)
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "crava_hip.h"', "int main(void) {",
+             f'  printf("%zu\\n", sizeof({cname}));']
+    lines += [f'  printf("%zu\\n", offsetof({cname}, {f[0]}));' for f in py._fields_]
+    lines += ["  return 0;", "}"]
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [C.sizeof(py)] + [getattr(py, f[0]).offset for f in py._fields_]
+
+
+def test_preparation_entry_points_reject_null_arguments():
+    lib = L.lib()
+    assert lib.crava_hip_divide_by_scale_wavelet(None, None, None) == L.E_INVALID
+    assert lib.crava_hip_interpolate_seismic(None, 9, 8, 20, 12, 11, 24, 0.1, None, None, None) == L.E_INVALID

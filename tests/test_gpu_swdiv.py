@@ -1,0 +1,141 @@
+"""GPU non-constant-thickness preparation against the oracle:
+crava_hip_divide_by_scale_wavelet and crava_hip_interpolate_seismic.
+
+Division tolerance: the GPU evaluates the adjustment factor's
+difference-operator spectrum in closed form and runs float Stockham
+transforms, the oracle follows the reference's per-trace wavelet transforms
+with double DFTs rounded to float; every sample inside the padded grid
+(x < nxp) agrees to 5e-6 of the angle's largest value and the relative L2
+error is below 3e-6.  The layout's slack column (x >= nxp) holds no data and
+comes back within 2e-5 of its input.  Interpolation is bit-exact.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from crava_amd import _lib as L
+from crava_amd.core import divide_by_scale_wavelet, interpolate_seismic
+from tests.swdiv_cases import isq_grid, swdiv_inputs
+
+CASES = [
+    # nx, ny, nz, nxp, nyp, nzp, ntheta
+    (10, 7, 24, 14, 10, 32, 3),     # runtime-length z (composition path)
+    (9, 6, 30, 12, 9, 45, 2),       # odd z length
+    (20, 10, 40, 25, 14, 50, 8),    # odd nxp (one slack trace), 8 angles
+    (12, 11, 60, 16, 14, 80, 3),    # specialised z length (fused kernel)
+    (13, 9, 50, 15, 12, 64, 4),     # fused, odd nxp
+    (30, 20, 100, 36, 24, 128, 6),
+]
+
+
+def _check(got, want, inp, nxp):
+    a = got[:, :, :nxp].astype(np.float64)
+    b = want[:, :, :nxp].astype(np.float64)
+    assert np.abs(a - b).max() <= 5e-6 * np.abs(b).max()
+    assert np.linalg.norm(a - b) <= 3e-6 * np.linalg.norm(b)
+    s = got[:, :, nxp:].astype(np.float64)
+    r = inp[:, :, nxp:].astype(np.float64)
+    assert np.abs(s - r).max() <= 2e-5 * max(np.abs(r).max(), 1e-30)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[:6])) + f"_nt{c[6]}")
+def test_divide_by_scale_wavelet_vs_oracle(ctx, case):
+    *n, nt = case
+    print(f"swdiv {n} nt={nt}", flush=True)
+    args, kw = swdiv_inputs(*n, nt, seed=sum(n))
+    want = oracle.divide_by_scale_wavelet(*n, *args, **kw)
+    dg = [torch.from_numpy(g).cuda() for g in args[0]]
+    divide_by_scale_wavelet(ctx, *n, dg, *args[1:], **kw)
+    torch.cuda.synchronize()
+    out = [g.cpu().numpy() for g in dg]
+    for l in range(nt):
+        _check(out[l], want[l], args[0][l], n[3])
+    # the same inputs again give the same bits
+    dg2 = [torch.from_numpy(g).cuda() for g in args[0]]
+    divide_by_scale_wavelet(ctx, *n, dg2, *args[1:], **kw)
+    torch.cuda.synchronize()
+    assert all(np.array_equal(a.cpu().numpy(), b) for a, b in zip(dg2, out))
+
+
+@pytest.mark.gpu
+def test_divide_by_scale_wavelet_full_grid_sampled_traces(ctx):
+    """512x512x320 grid (448x448x256 volume), 6 angles, local shifts and
+    gains: sampled x columns of the GPU result against the oracle on those
+    columns (every trace depends on its own samples and tables only)."""
+    n = (448, 448, 256, 512, 512, 320)
+    nt = 6
+    print("swdiv full grid", flush=True)
+    args, kw = swdiv_inputs(*n, nt, seed=21)
+    dg = [torch.from_numpy(g).cuda() for g in args[0]]
+    divide_by_scale_wavelet(ctx, *n, dg, *args[1:], **kw)
+    torch.cuda.synchronize()
+    for i0, i1 in ((0, 2), (223, 224), (446, 450), (470, 471), (510, 512)):
+        want = oracle.divide_by_scale_wavelet(*n, *args, **kw, i_range=(i0, i1))
+        for l in range(nt):
+            got = dg[l][:, :, i0:i1].cpu().numpy().astype(np.float64)
+            ref = want[l][:, :, i0:i1].astype(np.float64)
+            assert np.abs(got - ref).max() <= 5e-6 * np.abs(ref).max(), (i0, l)
+
+
+@pytest.mark.gpu
+def test_divide_by_scale_wavelet_rejects_bad_arguments(ctx):
+    n = (10, 7, 24, 14, 10, 32)
+    args, _ = swdiv_inputs(*n, 2, seed=0)
+    dg = [torch.from_numpy(g).cuda() for g in args[0]]
+    bad = list(args[1:])
+    bad[6] = args[7].copy()
+    bad[6][3, 4] = 0.0                                   # a zero relative thickness
+    with pytest.raises(L.CravaHipError):
+        divide_by_scale_wavelet(ctx, *n, dg, *bad)
+    bad = list(args[1:])
+    bad[7] = 0.0                                         # dz
+    with pytest.raises(L.CravaHipError):
+        divide_by_scale_wavelet(ctx, *n, dg, *bad)
+    with pytest.raises(L.CravaHipError):                 # volume larger than the grid
+        divide_by_scale_wavelet(ctx, 15, 7, 24, 14, 10, 32, dg, *args[1:])
+
+
+ISQ = [
+    # sizes, n_bad, n_weak, cluster
+    ((9, 8, 20, 12, 11, 24), 3, 2, False),
+    ((31, 17, 40, 36, 20, 48), 40, 20, False),
+    ((31, 17, 40, 36, 20, 48), 49, 0, True),     # a 7x7 block: many levels
+    ((64, 50, 100, 75, 60, 128), 300, 100, False),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ISQ, ids=lambda c: "x".join(map(str, c[0])) + f"_{c[1]}_{c[2]}_{int(c[3])}")
+def test_interpolate_seismic_vs_oracle(ctx, case):
+    n, n_bad, n_weak, cluster = case
+    print(f"interpolate {n} bad={n_bad} weak={n_weak} cluster={cluster}", flush=True)
+    g = isq_grid(*n, seed=sum(n) + n_bad, n_bad=n_bad, n_weak=n_weak, cluster=cluster)
+    want, nf, n0 = oracle.interpolate_seismic(*n, 0.1, g)
+    dg = torch.from_numpy(g).cuda()
+    assert interpolate_seismic(ctx, *n, 0.1, dg) == (nf, n0)
+    assert (nf, n0) == (n_bad + n_weak, n_bad)
+    assert np.array_equal(dg.cpu().numpy(), want)
+
+
+@pytest.mark.gpu
+def test_interpolate_seismic_empty_column_and_full_grid(ctx):
+    n = (448, 448, 256, 512, 512, 320)
+    print("interpolate full grid", flush=True)
+    g = isq_grid(*n, seed=5, n_bad=2000, n_weak=500)
+    g[:, :, 447] = 0.0                            # the last volume column is empty
+    want, nf, n0 = oracle.interpolate_seismic(*n, 0.1, g)
+    dg = torch.from_numpy(g).cuda()
+    assert interpolate_seismic(ctx, *n, 0.1, dg) == (nf, n0)
+    assert np.array_equal(dg.cpu().numpy(), want)
+
+
+@pytest.mark.gpu
+def test_interpolate_seismic_rejects_an_empty_volume(ctx):
+    n = (9, 8, 20, 12, 11, 24)
+    dg = torch.zeros((24, 11, 14), device="cuda")
+    with pytest.raises(L.CravaHipError):
+        interpolate_seismic(ctx, *n, 0.1, dg)
+    with pytest.raises(L.CravaHipError):
+        interpolate_seismic(ctx, 13, 8, 20, 12, 11, 24, 0.1, dg)

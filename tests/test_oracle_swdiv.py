@@ -1,0 +1,253 @@
+"""The oracle's non-constant-thickness preparation
+(orc_divide_by_scale_wavelet, orc_interpolate_seismic) pinned against:
+
+* the reference's own comment vectors of FFTGrid::getFillNumber and the
+  pattern of FFTGrid::getDistToBoundary (src/fftgrid.cpp:1321-1324,
+  1376-1379);
+* closed forms of the division that fix each ingredient independently of
+  the restatement's formula: with the local wavelet equal to the global one
+  the factor is W/|W|^2 (deconvolution, its noise terms cancel); an integer
+  time shift of s samples delays the result by s; a gain g scales the result
+  by g when tau2 = 0 and by 1/g when the noise vanishes; a stretch sf reads
+  the spectrum at k/sf scaled by 1/sf (FFT(w(s t)) = W(w/s)/s), with the band
+  at nz 1000/(relThick lz nzp) Hz per bin; and the mirrored wavelet index of
+  the padding traces, stated by hand from src/crava.cpp:626-642;
+* the reference's FFTW 2.1.2 (oracle/_ref, with the reference's per-trace
+  round trip of the global wavelet);
+* hand-computed interpolation and extrapolation results on small grids.
+
+The tolerance-adjustment branches of computeAdjustmentFactor (tolFac) are
+followed as written in src/crava.cpp:794-808 and agree with the FFTW-driven
+driver; no closed form isolates them.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.swdiv_cases import isq_grid, swdiv_inputs
+
+N1 = (10, 7, 24, 14, 10, 32)      # nx, ny, nz, nxp, nyp, nzp: 7.5 Hz per bin at relThick 1
+
+
+def test_fill_number_matches_reference_comment():
+    # src/fftgrid.cpp:1322-1324
+    assert [oracle.fill_number(i, 5, 8) for i in range(8)] == [0, 1, 2, 3, 4, 4, 1, 0]
+    assert [oracle.fill_number(i, 4, 8) for i in range(8)] == [0, 1, 2, 3, 3, 2, 1, 0]
+    assert [oracle.fill_number(i, 3, 8) for i in range(8)] == [0, 1, 2, 2, 1, 1, 1, 0]
+
+
+def test_dist_to_boundary_matches_reference_comment():
+    # src/fftgrid.cpp:1377-1379: 0 inside, p in (0, 1) next to the volume and
+    # at the end, r > 1 in the middle of the padding
+    for n, pattern in ((5, "00000prp"), (4, "0000prrp"), (3, "000prrrp")):
+        d = [oracle.dist_to_boundary(i, n, 8) for i in range(8)]
+        for v, c in zip(d, pattern):
+            assert {"0": v == 0.0, "p": 0.0 < v < 1.0, "r": v > 1.0}[c], (n, d)
+    # taper length min(n, np - n) / 2.1
+    assert oracle.dist_to_boundary(5, 5, 8) == np.float32(1 / np.float32(3 / 2.1))
+    assert oracle.dist_to_boundary(9, 5, 8) == np.float32(-99999.0)
+
+
+def _taper(nz, nzp):
+    f = np.ones(nzp)
+    for k in range(nz + 1, nzp):
+        d = oracle.dist_to_boundary(k, nz, nzp)
+        f[k] = max(1.0 - d * d, 0.0)
+    return f
+
+
+def _closed_form(grid, n, factor, rel):
+    """c2r(band(r2c(taper(d) / sqrt(nzp)) * factor)) / sqrt(nzp) for every
+    padded trace; factor(k, j, i) over the half spectrum (arrays [cn][nyp][nxp])."""
+    nx, ny, nz, nxp, nyp, nzp = n
+    d = grid[:, :, :nxp].astype(np.float64) / np.sqrt(nzp)
+    d *= _taper(nz, nzp)[:, None, None]
+    R = np.fft.rfft(d, axis=0)
+    k = np.arange(nzp // 2 + 1)[:, None, None]
+    df = nz * 1000.0 / (rel[None] * 100.0 * nzp)
+    band = (df * k < 55.0) & (df * k > 5.0)
+    S = np.where(band, R * factor, 0.0)
+    return np.fft.irfft(S, n=nzp, axis=0) * nzp / np.sqrt(nzp)
+
+
+def _check(got, want, n, tol=2e-5):
+    g = got[:, :, :n[3]].astype(np.float64)
+    scale = np.abs(want).max()
+    assert np.abs(g - want).max() <= tol * scale
+    assert np.linalg.norm(g - want) <= tol * np.linalg.norm(want)
+
+
+def _spec(w):
+    return np.fft.rfft(w.astype(np.float64))[:, None, None]
+
+
+def test_deconvolution_when_local_equals_global():
+    args, _ = swdiv_inputs(*N1, 2, seed=1, local=False, rel_range=None, wavelet="delta")
+    out = oracle.divide_by_scale_wavelet(*N1, *args)
+    rel = args[7]
+    for l in range(2):
+        W = _spec(args[1][l])
+        _check(out[l], _closed_form(args[0][l], N1, W / np.abs(W) ** 2, rel), N1)
+
+
+def test_integer_shift_delays_the_result():
+    args, _ = swdiv_inputs(*N1, 1, seed=2, local=False, rel_range=None, wavelet="delta")
+    base = oracle.divide_by_scale_wavelet(*N1, *args)[0]
+    for s in (1, 3, -2):
+        shift = [np.full((N1[1], N1[0]), s * 4.0, np.float32)]          # s samples of dz = 4
+        got = oracle.divide_by_scale_wavelet(*N1, *args, shift=shift)[0]
+        _check(got, np.roll(base[:, :, :N1[3]].astype(np.float64), s, axis=0), N1, tol=4e-6)
+
+
+def test_padding_traces_use_the_mirrored_wavelet_index():
+    nx, ny, _, nxp, nyp, _ = N1
+    args, _ = swdiv_inputs(*N1, 1, seed=3, local=False, rel_range=None, wavelet="delta")
+    # one integer shift per volume trace; the padded trace (i, j) takes the
+    # shift of (mx[i], my[j]) (src/crava.cpp:626-642 with nx 10 / nxp 14, ny 7 / nyp 10)
+    mx = list(range(10)) + [9, 8, 7, 1]
+    my = list(range(7)) + [6, 5, 1]
+    s_vol = (np.arange(nx)[None, :] + 2 * np.arange(ny)[:, None]) % 5
+    got = oracle.divide_by_scale_wavelet(*N1, *args, shift=[(4.0 * s_vol).astype(np.float32)])[0]
+    base = oracle.divide_by_scale_wavelet(*N1, *args)[0][:, :, :nxp].astype(np.float64)
+    want = np.empty_like(base)
+    for j in range(nyp):
+        for i in range(nxp):
+            want[:, j, i] = np.roll(base[:, j, i], s_vol[my[j], mx[i]])
+    _check(got, want, N1, tol=4e-6)
+
+
+def test_gain_limits():
+    nx, ny, _, nxp, nyp, _ = N1
+    mx = list(range(10)) + [9, 8, 7, 1]
+    my = list(range(7)) + [6, 5, 1]
+    args, _ = swdiv_inputs(*N1, 1, seed=4, local=False, rel_range=None, wavelet="delta")
+    g = np.random.default_rng(5).uniform(0.7, 1.3, (ny, nx)).astype(np.float32)
+    gpad = g[np.ix_(my, mx)][None].astype(np.float64)
+    W = _spec(args[1][0])
+    dec = W / np.abs(W) ** 2
+    # tau2 = 0 (no prior correlation): the gain scales the result
+    a0 = list(args)
+    a0[5] = np.zeros_like(args[5])
+    got = oracle.divide_by_scale_wavelet(*N1, *a0, gain=[g])[0]
+    _check(got, _closed_form(args[0][0], N1, dec, args[7]) * gpad, N1)
+    # vanishing noise: the gain divides the result
+    a1 = list(args)
+    a1[5] = np.zeros_like(args[5])
+    a1[5][0] = 1.0
+    a1[6] = np.full_like(args[6], 1e-30)
+    got = oracle.divide_by_scale_wavelet(*N1, *a1, gain=[g])[0]
+    _check(got, _closed_form(args[0][0], N1, dec, args[7]) / gpad, N1)
+
+
+def test_stretch_reads_the_spectrum_at_k_over_sf():
+    args, _ = swdiv_inputs(*N1, 1, seed=6, local=False, rel_range=(0.6, 1.0), wavelet="delta")
+    args = list(args)
+    args[5] = np.zeros_like(args[5])
+    args[5][0] = 1.0
+    args[6] = np.full_like(args[6], 1e-30)       # vanishing noise: factor 1 / conj(w_sf)
+    nzp = N1[5]
+    cn = nzp // 2 + 1
+    W = np.fft.rfft(args[1][0].astype(np.float64))
+    sf = args[7].astype(np.float32)
+    k = np.arange(cn, dtype=np.float32)[:, None, None]
+    om = k / sf[None]                             # float division, as getCAmp
+    lo = np.floor(om).astype(int)
+    hi = np.minimum(lo + 1, cn - 1)
+    d = (om - lo.astype(np.float32)).astype(np.float64)
+    lo = np.minimum(lo, cn - 1)
+    w = (W[lo] * (1 - d) + W[hi] * d) / sf[None].astype(np.float64)
+    out = oracle.divide_by_scale_wavelet(*N1, *args)[0]
+    _check(out, _closed_form(args[0][0], N1, 1.0 / np.conj(w), args[7]), N1)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("n", [N1, (9, 6, 30, 12, 9, 45), (12, 11, 60, 16, 14, 80)])
+def test_oracle_vs_reference_fftw(n):
+    args, kw = swdiv_inputs(*n, 3, seed=sum(n))
+    o = oracle.divide_by_scale_wavelet(*n, *args, **kw)
+    r = oracle.divide_by_scale_wavelet(*n, *args, **kw, which="ref")
+    for l in range(3):
+        _check(o[l], r[l][:, :, :n[3]].astype(np.float64), n, tol=4e-6)
+
+
+def test_trace_range_leaves_other_traces():
+    args, kw = swdiv_inputs(*N1, 2, seed=8)
+    full = oracle.divide_by_scale_wavelet(*N1, *args, **kw)
+    part = oracle.divide_by_scale_wavelet(*N1, *args, **kw, i_range=(3, 5))
+    for l in range(2):
+        assert np.array_equal(part[l][:, :, 3:5], full[l][:, :, 3:5])
+        assert np.array_equal(part[l][:, :, :3], args[0][l][:, :, :3])
+
+
+# ------------------------------------------------------------ interpolateSeismic
+NI = (9, 8, 20, 12, 11, 24)
+
+
+def _extrapolated(g, n, box):
+    """extrapolateSeismic on a copy of g (float ops of src/fftgrid.cpp:2718-2728)."""
+    nx, ny, nz, nxp, nyp, nzp = n
+    imin, imax, jmin, jmax = box
+    out = g.copy()
+    dzs = [oracle.dist_to_boundary(k, nz, nzp) for k in range(nzp)]
+    for j in range(nyp):
+        rj = min(max(oracle.fill_number(j, ny, nyp), jmin), jmax)
+        for i in range(nxp):
+            if imin <= i <= imax and jmin <= j <= jmax:
+                continue
+            ri = min(max(oracle.fill_number(i, nx, nxp), imin), imax)
+            dx, dy = oracle.dist_to_boundary(i, nx, nxp), oracle.dist_to_boundary(j, ny, nyp)
+            for k in range(nzp):
+                b = 1.0 - float(np.float32(dx * dx)) - float(np.float32(dy * dy)) - float(np.float32(dzs[k] * dzs[k]))
+                mult = np.float32(max(b, 0.0) ** 3)
+                out[k, j, i] = mult * g[k, rj, ri]
+    return out
+
+
+def test_interpolate_without_low_energy_traces_only_extrapolates():
+    g = isq_grid(*NI, seed=1, n_bad=0)
+    got, nf, n0 = oracle.interpolate_seismic(*NI, 0.1, g)
+    assert (nf, n0) == (0, 0)
+    assert np.array_equal(got, _extrapolated(g, NI, (0, NI[0] - 1, 0, NI[1] - 1)))
+
+
+def test_interpolate_single_trace_and_pair():
+    g = isq_grid(*NI, seed=2, n_bad=0)
+    i0, j0 = 4, 3
+    g[:, j0, i0] = 0.0              # one interior trace
+    g[:, 6, 2] = 0.0                # two neighbours in a row
+    g[:, 6, 3] = 0.0
+    want = g.copy()
+    f = np.float32
+    want[:, j0, i0] = (((g[:, j0, i0 - 1] + g[:, j0, i0 + 1]) + g[:, j0 - 1, i0]) + g[:, j0 + 1, i0]) / f(4)
+    first = ((g[:, 6, 1] + g[:, 5, 2]) + g[:, 7, 2]) / f(3)      # left, up, down (right is low)
+    want[:, 6, 2] = first
+    want[:, 6, 3] = (((first + g[:, 6, 4]) + g[:, 5, 3]) + g[:, 7, 3]) / f(4)
+    got, nf, n0 = oracle.interpolate_seismic(*NI, 0.1, g)
+    assert (nf, n0) == (3, 3)
+    assert np.array_equal(got, _extrapolated(want, NI, (0, NI[0] - 1, 0, NI[1] - 1)))
+
+
+def test_interpolate_corner_cases():
+    g = isq_grid(*NI, seed=3, n_bad=0)
+    f = np.float32
+    # (0, 0) and (1, 0) low: (0, 0) has one usable neighbour and is never
+    # revisited (its flag has no earlier low trace); (1, 0) averages right and down
+    g[:, 0, 0] = 0.0
+    g[:, 0, 1] *= f(1e-4)
+    want = g.copy()
+    want[:, 0, 1] = (f(0.0) + g[:, 0, 2] + g[:, 1, 1]) / f(2)
+    got, nf, n0 = oracle.interpolate_seismic(*NI, 0.1, g)
+    assert (nf, n0) == (2, 1)
+    assert np.array_equal(got, _extrapolated(want, NI, (0, NI[0] - 1, 0, NI[1] - 1)))
+
+
+def test_interpolate_box_shrinks_to_good_traces():
+    nx, ny = NI[0], NI[1]
+    g = isq_grid(*NI, seed=4, n_bad=0)
+    g[:, :, nx - 1] = 0.0            # the last volume column is empty
+    got, nf, _ = oracle.interpolate_seismic(*NI, 0.1, g)
+    assert nf == ny
+    # each trace of the column has one usable neighbour (left; the one above
+    # is low too), so none is rebuilt; the box of good traces ends at nx - 2
+    # and the column is extrapolated from column nx - 2
+    assert np.array_equal(got, _extrapolated(g, NI, (0, nx - 2, 0, ny - 1)))
