@@ -106,6 +106,8 @@ def parse():
                     "workload (0: skip)")
     ap.add_argument("--diff-steps", type=int, default=50, help="L-BFGS iterations of the differentiation workload "
                     "(0: skip it)")
+    ap.add_argument("--only", choices=["nonconstant_thickness", "time_lapse_merge"],
+                    help="run one secondary workload alone (for profiling) and print its result")
     return ap.parse_args()
 
 
@@ -511,6 +513,70 @@ def forward_modelling(ctx, prob, reps: int, cpu: bool):
     return res
 
 
+def nonconstant_thickness(ctx, reps: int, cpu: bool):
+    """Data preparation of the non-constant-thickness path on the posterior
+    workload's grid (448x448x256 volume in 512x512x320, 6 angles): the
+    wavelet division of every padded trace (crava_hip_divide_by_scale_wavelet,
+    Crava::divideDataByScaleWavelet) with relThick U(0.6, 1), local shifts on
+    three angles and gains on five, then interpolateSeismic of one angle with
+    2000 empty and 500 weak traces (crava_hip_interpolate_seismic).  HIP
+    events around each call on the launch stream (table set-up, uploads and
+    the schedule's host work included).  Division algorithmic bytes: one
+    read and one write of every grid.  The reference's FFTW path
+    (oracle/_ref) is timed on one core on the first x-columns and scaled."""
+    import numpy as np
+    import torch
+    from crava_amd.core import divide_by_scale_wavelet, interpolate_seismic
+    from tests.swdiv_cases import isq_grid, swdiv_inputs
+    n = (448, 448, 256, 512, 512, 320)
+    nt = 6
+    args, kw = swdiv_inputs(*n, nt, seed=21)
+    src = [torch.from_numpy(g).cuda() for g in args[0]]
+    work = [g.clone() for g in src]
+
+    def restore():
+        for w, g in zip(work, src):
+            w.copy_(g)
+
+    ms = timed_kernel(lambda: divide_by_scale_wavelet(ctx, *n, work, *args[1:], **kw), reps, setup=restore)
+    rsize = src[0].numel()
+    alg = 8 * rsize * nt
+    res = {"volume": list(n[:3]), "grid": list(n[3:]), "ntheta": nt,
+           "division": {"ms_per_call": ms, "traces_per_s": n[3] * n[4] * nt / (ms * 1e-3), "alg_bytes": alg,
+                        "alg_GBps": alg / (ms * 1e-3) / 1e9, "frac_of_peak": alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
+           "cpu_reference": None}
+    log(f"wavelet division: {ms:.2f} ms per call ({nt} angles), {res['division']['alg_GBps']:.0f} GB/s")
+    del work, src
+    g0 = torch.from_numpy(isq_grid(*n, seed=5, n_bad=2000, n_weak=500)).cuda()
+    gi = g0.clone()
+    t_isq = timed_kernel(lambda: interpolate_seismic(ctx, *n, 0.1, gi), reps, setup=lambda: gi.copy_(g0))
+    res["interpolation"] = {"ms_per_call": t_isq, "traces_flagged": 2500}
+    log(f"interpolateSeismic: {t_isq:.2f} ms per call")
+    del g0, gi
+    if cpu:
+        try:
+            import oracle
+            if oracle.ref_available():
+                t1 = time.perf_counter()
+                oracle.divide_by_scale_wavelet(*n, *args, **kw, which="ref", i_range=(0, 1))
+                t_one = time.perf_counter() - t1
+                cols = 8
+                t1 = time.perf_counter()
+                oracle.divide_by_scale_wavelet(*n, *args, **kw, which="ref", i_range=(0, 1 + cols))
+                t_more = time.perf_counter() - t1
+                per_col = max(0.0, t_more - t_one) / cols
+                total = t_one + per_col * (n[3] - 1)
+                res["cpu_reference"] = {"seconds_per_call": total, "cores": 1, "kind": "reference",
+                                        "sample": f"traces of the first 1 and {1 + cols} x-columns ({t_one:.1f}s, "
+                                                  f"{t_more:.1f}s, grid copies included), scaled to {n[3]} columns",
+                                        "gpu_speedup": total / (ms * 1e-3)}
+                log(f"wavelet division, reference CPU path: {total:.1f} s per call (1 core)")
+        except Exception as e:  # the baseline is reported, never the target
+            log(f"wavelet division cpu baseline failed: {e}")
+    torch.cuda.empty_cache()
+    return res
+
+
 def time_lapse_merge(ctx, reps: int, cpu: bool):
     """Time-lapse merge (crava_hip_merge_cov_4d / _mean_4d, State4D::mergeCov /
     merge) on 512x512x320 transformed grids: 6 + 6 + 9 covariance parts into
@@ -570,6 +636,13 @@ def main():
 
     torch.cuda.init()
     dev = torch.cuda.current_device()
+    if args.only:
+        ctx = Context()
+        reps = max(args.forward_reps, 1)
+        fn = {"nonconstant_thickness": lambda: nonconstant_thickness(ctx, reps, not args.no_cpu),
+              "time_lapse_merge": lambda: time_lapse_merge(ctx, reps, not args.no_cpu)}[args.only]
+        print(json.dumps({"workload": args.only, "result": fn()}))
+        return
     cfg = synth.config(args.config)
     log(f"config {cfg.name}: grid {cfg.nxp}x{cfg.nyp}x{cfg.nzp}, ntheta={len(cfg.thetas_deg)}")
     t0 = time.time()
@@ -750,6 +823,10 @@ def main():
     if args.forward_reps > 0:
         torch.cuda.empty_cache()
         fwd = forward_modelling(ctx, prob, args.forward_reps, not args.no_cpu)
+    nct = None
+    if args.forward_reps > 0:
+        torch.cuda.empty_cache()
+        nct = nonconstant_thickness(ctx, args.forward_reps, not args.no_cpu)
     cpu = None
     if not args.no_cpu:
         try:
@@ -796,6 +873,7 @@ def main():
         "batching": batch,
         "differentiation": diff,
         "forward_modelling": fwd,
+        "nonconstant_thickness": nct,
         "time_lapse_merge": tl,
     }
     print(json.dumps(out))

@@ -665,7 +665,7 @@ int crava_hip_divide_by_scale_wavelet(crava_hip_ctx *ctx, const crava_hip_swdiv_
                             p->prior_var0, p->corr_t, p->dz, p->lz, p->low_cut, p->high_cut, tw.data() + 2 * o,
                             t3.data() + o, tau2.data() + o, wg2.data() + o, &a.mod_w[l]);
     a.ev[l] = p->err_var[l];
-    a.wn2[l] = p->wavelet_norm[l] * p->wavelet_norm[l];
+    a.iwn2[l] = 1.0f / (p->wavelet_norm[l] * p->wavelet_norm[l]);
   }
   const auto al = [](size_t b) { return ((b + 255) / 256) * 256; };
   const size_t ntr = (size_t)p->nx * p->ny;

@@ -108,10 +108,11 @@ __device__ __forceinline__ float prep(float v, int k, float sq, int nz, int nzp)
 struct SwdTrace {
   double delta_f;   // nz 1000 / (relT lz nzp)
   float sf, ishift, gain, inside;
+  float rsf, iag;   // 1 / sf, 1 / |gain|
 };
 
 __device__ __forceinline__ SwdTrace trace_params(const SwdArgs &a, int l, int i, int j) {
-  SwdTrace p{0.0, 1.0f, 0.0f, 1.0f, 0.0f};
+  SwdTrace p{0.0, 1.0f, 0.0f, 1.0f, 0.0f, 1.0f, 1.0f};
   if (i >= a.nxp) return p;
   p.inside = 1.0f;
   const int ii = mirror(i, a.nx, a.nxp), jj = mirror(j, a.ny, a.nyp);
@@ -125,26 +126,32 @@ __device__ __forceinline__ SwdTrace trace_params(const SwdArgs &a, int l, int i,
   const double rt = a.rel[(long long)j * a.nxp + i];
   p.sf = (float)(rt * 1.0f);
   p.delta_f = (double)a.nz * 1000.0 / (rt * a.lz * (double)a.nzp);
+  p.rsf = 1.0f / p.sf;
+  p.iag = 1.0f / fabsf(p.gain);
   return p;
 }
 
-// Local spectrum at bin m: gain e^{-2 pi i ishift m / nzp} W(m), multiplier
-// rounded to float from double (Wavelet1D::shiftAndScale); an even
-// length's Nyquist bin is real (a c2r ignores its imaginary part).
+// Local spectrum at bin m: gain e^{-2 pi i ishift m / nzp} W(m)
+// (Wavelet1D::shiftAndScale); the phase in turns is formed and reduced to
+// [-1, 1] half-turns in double, so the float sincospi sees an exact small
+// argument.  An even length's Nyquist bin is real (a c2r ignores its
+// imaginary part).
 __device__ __forceinline__ cx<float> local_at(const cx<float> *W, int m, float ishift, float gain, int nzp) {
   const cx<float> w = W[m];
-  double s, c;
-  sincospi(2.0 * (double)ishift * (double)m / (double)nzp, &s, &c);
-  const float mr = (float)((double)gain * c), mi = (float)((double)gain * -s);
+  const double t = 2.0 * (double)ishift * (double)m / (double)nzp;
+  float s, c;
+  sincospif((float)(t - 2.0 * rint(0.5 * t)), &s, &c);
+  const float mr = gain * c, mi = -(gain * s);
   cx<float> v = mk(mr * w.x - mi * w.y, mi * w.x + mr * w.y);
   if (2 * m == nzp) v.y = 0.0f;
   return v;
 }
 
-// Normalised errorSmooth3 at bin m from the local spectrum v there.
-__device__ __forceinline__ cx<float> e3_at(const float *t3, int m, cx<float> v, float again, int nzp) {
-  const float t = t3[m];
-  cx<float> e = mk((-t * v.y) / again, (t * v.x) / again);
+// Normalised errorSmooth3 at bin m from the local spectrum v there
+// (iag = 1 / |gain|).
+__device__ __forceinline__ cx<float> e3_at(const float *t3, int m, cx<float> v, float iag, int nzp) {
+  const float t = t3[m] * iag;
+  cx<float> e = mk(-t * v.y, t * v.x);
   if (2 * m >= nzp) e.x = kRMissing;
   if (2 * m + 1 >= nzp) e.y = kRMissing;
   return e;
@@ -154,9 +161,8 @@ __device__ __forceinline__ cx<float> e3_at(const float *t3, int m, cx<float> v, 
 // (src/crava.cpp:772-814).  Tables of the angle: W, t3, tau2, wg2.
 __device__ __forceinline__ cx<float> adjustment(const cx<float> *W, const float *t3, const float *tau2,
                                                 const float *wg2, const SwdTrace &p, int k, int nzp, float mod_w,
-                                                float ev, float wn2, float wnc) {
+                                                float ev, float iwn2, float wnc) {
   const int cn = nzp / 2 + 1;
-  const float ag = fabsf(p.gain);
   cx<float> wl = mk(0.0f, 0.0f), el = mk(0.0f, 0.0f);
   const float omega = (float)k / p.sf;
   if (!(omega >= (float)cn)) {
@@ -165,23 +171,23 @@ __device__ __forceinline__ cx<float> adjustment(const cx<float> *W, const float 
     if (omU >= cn) omU -= 1;
     const float d = omega - (float)omL;
     const cx<float> a = local_at(W, omL, p.ishift, p.gain, nzp), b = local_at(W, omU, p.ishift, p.gain, nzp);
-    const cx<float> ea = e3_at(t3, omL, a, ag, nzp), eb = e3_at(t3, omU, b, ag, nzp);
+    const cx<float> ea = e3_at(t3, omL, a, p.iag, nzp), eb = e3_at(t3, omU, b, p.iag, nzp);
     wl.x = a.x * (1.0f - d) + b.x * d;
     wl.y = (a.y * (1.0f - d) + b.y * d);
     el.x = ea.x * (1.0f - d) + eb.x * d;
     el.y = (ea.y * (1.0f - d) + eb.y * d);
   }
-  wl.x /= p.sf;
-  wl.y /= p.sf;
-  el.x /= p.sf;
-  el.y /= p.sf;
+  wl.x *= p.rsf;
+  wl.y *= p.rsf;
+  el.x *= p.rsf;
+  el.y *= p.rsf;
   const float wl2 = (wl.x * wl.x + wl.y * wl.y);
   const float wg2k = wg2[k];
-  float s_loc = (wl.x * wl.x + wl.y * wl.y) / wn2;
+  float s_loc = wl2 * iwn2;
   s_loc += el.x * el.x + el.y * el.y;
   const cx<float> c = local_at(W, k, p.ishift, p.gain, nzp);
-  const cx<float> ec = e3_at(t3, k, c, ag, nzp);
-  float s_glob = (c.x * c.x + c.y * c.y) / wn2;
+  const cx<float> ec = e3_at(t3, k, c, p.iag, nzp);
+  float s_glob = (c.x * c.x + c.y * c.y) * iwn2;
   s_glob += ec.x * ec.x + ec.y * ec.y;
   float sig_l = ev * (wnc + (1.0f - wnc) * 0.5f * s_loc);
   const float sig_g = ev * (wnc + (1.0f - wnc) * 0.5f * s_glob);
@@ -193,9 +199,9 @@ __device__ __forceinline__ cx<float> adjustment(const cx<float> *W, const float 
     wl2a = 0.5f * (wl2 + kTolFac * wg2a);
     sig_l *= wl2a / wl2;
   }
-  const float den = t2 * wl2a + sig_l;
-  const float q = wg2a / (t2 * wg2k + sig_g);
-  return mk((wl.x / den) / q, (wl.y / den) / q);
+  // (wLoc / den) / (wGlob2Adj / (tau2 |W|^2 + sigma2Glob)) with one reciprocal
+  const float r = (t2 * wg2k + sig_g) / ((t2 * wl2a + sig_l) * wg2a);
+  return mk(wl.x * r, wl.y * r);
 }
 
 // Filtered bin k of one trace: R f inside the band, 0 outside; the slack
@@ -205,7 +211,7 @@ __device__ __forceinline__ cx<float> filter_bin(const cx<float> *W, const float 
                                                 const SwdArgs &a, int l) {
   if (p.inside == 0.0f) return scl(R, a.inv_n);
   if ((p.delta_f * k < a.high_cut) && (p.delta_f * k > a.low_cut)) {
-    const cx<float> f = adjustment(W, t3, tau2, wg2, p, k, a.nzp, a.mod_w[l], a.ev[l], a.wn2[l], a.wnc);
+    const cx<float> f = adjustment(W, t3, tau2, wg2, p, k, a.nzp, a.mod_w[l], a.ev[l], a.iwn2[l], a.wnc);
     const float tmp = R.x * f.x - R.y * f.y;
     const float im = R.y * f.x + R.x * f.y;
     return mk(tmp, im);

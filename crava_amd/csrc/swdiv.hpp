@@ -21,7 +21,7 @@ struct SwdArgs {
   const double *rel;                 // [nyp][nxp] relative thickness of every padded trace
   float mod_w[kSwdMaxTheta];         // |w|^2 maxfrequency / highCut
   float ev[kSwdMaxTheta];            // error variance of the angle
-  float wn2[kSwdMaxTheta];           // |w| * |w|
+  float iwn2[kSwdMaxTheta];          // 1 / (|w| * |w|)
   float dz, wnc, low_cut, high_cut;
   float sq;                          // sqrtf(nzp)
   float inv_n;                       // 1 / nzp (the layout's slack traces)

@@ -188,4 +188,54 @@ void Crava::computeSyntSeismic(FFTGrid *alpha, FFTGrid *beta, FFTGrid *rho, cons
   }
 }
 
+void Crava::divideDataByScaleWavelet(const SeismicParametersHolder &sph) {
+  const int nt = s_.ntheta;
+  if ((long long)s_.relThick.size() != (long long)s_.nxp * s_.nyp)
+    throw Error(CRAVA_HIP_E_INVALID, "divideDataByScaleWavelet needs relThick for every padded trace");
+  for (int l = 0; l < nt; ++l)
+    if (seisData_[l]->getIsTransformed()) throw Error(CRAVA_HIP_E_STATE, "seismic grids must be in the time domain");
+  const std::vector<float> corrT = sph.getPriorCorrTFiltered(s_.nz, s_.nzp);
+  std::vector<float> errVar(nt);
+  for (int l = 0; l < nt; ++l) errVar[l] = (float)errThetaCov_[(size_t)l * nt + l];
+  const size_t ntr = (size_t)s_.nx * s_.ny;
+  auto table = [&](const std::vector<std::vector<float>> &t, std::vector<const float *> &ptrs) -> const float *const * {
+    if (t.empty()) return nullptr;
+    if ((int)t.size() != nt) throw Error(CRAVA_HIP_E_INVALID, "local wavelet tables need one entry per angle");
+    ptrs.assign(nt, nullptr);
+    for (int l = 0; l < nt; ++l) {
+      if (t[l].empty()) continue;
+      if (t[l].size() != ntr) throw Error(CRAVA_HIP_E_INVALID, "local wavelet table of the wrong size");
+      ptrs[l] = t[l].data();
+    }
+    return ptrs.data();
+  };
+  std::vector<const float *> pshift, pgain;
+  crava_hip_swdiv_params p{};
+  p.nx = s_.nx;
+  p.ny = s_.ny;
+  p.nz = s_.nz;
+  p.nxp = s_.nxp;
+  p.nyp = s_.nyp;
+  p.nzp = s_.nzp;
+  p.ntheta = nt;
+  p.wavelets = s_.wavelets.data();
+  p.wavelet_norm = s_.waveletNorm.data();
+  p.A = s_.A.data();
+  p.prior_var0 = &sph.getPriorVar0()[0][0];
+  p.corr_t = corrT.data();
+  p.err_var = errVar.data();
+  p.rel_thick = s_.relThick.data();
+  p.shift = table(s_.localShift, pshift);
+  p.gain = table(s_.localGain, pgain);
+  p.dz = s_.dz;
+  p.lz = (double)s_.nz * s_.dz;
+  p.low_cut = s_.lowCut;
+  p.high_cut = s_.highCut;
+  p.wnc = s_.wnc;
+  std::vector<void *> grids(nt);
+  for (int l = 0; l < nt; ++l) grids[l] = seisData_[l]->devicePtr();
+  check(crava_hip_divide_by_scale_wavelet(ctx_->handle(), &p, grids.data()), "divideDataByScaleWavelet");
+  for (int l = 0; l < nt; ++l) seisData_[l]->interpolateSeismic(s_.energyThreshold);
+}
+
 }  // namespace crava_hip

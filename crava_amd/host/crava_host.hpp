@@ -144,6 +144,13 @@ class FFTGrid {
   void writeStormFile(const std::string &fileName, const StormHeader &h, bool ascii, bool padding = false,
                       bool flat = false, bool scientific = false) const;
 
+  // FFTGrid::interpolateSeismic (src/fftgrid.cpp:2564-2655) on a real DATA
+  // grid: traces with energy at most energyTreshold times the mean are
+  // rebuilt from their neighbours, the traces outside the box of the others
+  // extrapolated.  In an access mode the host copy is written to the device
+  // first (unless READ) and refreshed after.  Returns the traces flagged.
+  int interpolateSeismic(float energyTreshold);
+
  private:
   int abiCubeType() const;
   void op(int code, const FFTGrid *other, float scalar);
@@ -174,6 +181,10 @@ class SeismicParametersHolder {
                               float corrGradI, float corrGradJ, int lowIntCut, int nzp);
   // :431-499 (host; the 1-D transform in double).
   std::vector<float> computeCircCorrT(const std::vector<float> &priorCorrT, int minIntFq, int nzp) const;
+  // :504-530 with extractParamCorrFromCovAlpha (:748-768): the cyclic
+  // temporal correlation read from the real covAlpha grid at (0, 0, k),
+  // zero where min(k, nzp - k) >= nz.
+  std::vector<float> getPriorCorrTFiltered(int nz, int nzp) const;
   void makeCircCorrTPosDef(std::vector<float> &circCorrT, int minIntFq, int nzp) const;
 
   void FFTAllGrids();       // :282-298, one launch per pass for each set
@@ -245,6 +256,11 @@ class Crava {
     // forward modelling: per angle, empty or [ny*nx] (i fastest) local
     // wavelet time shifts (ms), gains and stretches relThick*localStretch
     std::vector<std::vector<float>> localShift, localGain, stretch;
+    // non-constant thickness: Simbox::getRelThick of every padded trace
+    // [nyp*nxp] (empty: constant thickness) and the energy threshold of
+    // interpolateSeismic (Crava::energyTreshold_)
+    std::vector<double> relThick;
+    float energyThreshold = 0.0f;
   };
 
   Crava(Context *ctx, const Settings &s, std::vector<FFTGrid *> seisData);
@@ -263,6 +279,13 @@ class Crava {
   // for every angle into syn[ntheta] (real grids).  Parameters held in the
   // Fourier domain are transformed back and forth, as the reference does.
   void computeSyntSeismic(FFTGrid *alpha, FFTGrid *beta, FFTGrid *rho, const std::vector<FFTGrid *> &syn);
+  // :592-714 (with computeAdjustmentFactor, :726-817): divides every padded
+  // trace of the real seismic grids by its local scale wavelet, then
+  // interpolates low-energy traces (interpolateSeismic) per angle.  Needs the
+  // error correlation (errThetaCov) and the prior covariance grids in the
+  // time domain.  The local wavelet shifts and gains are localShift /
+  // localGain; the stretch is relThick.
+  void divideDataByScaleWavelet(const SeismicParametersHolder &sph);
 
   const std::vector<float> &getDataVariance() const { return dataVariance_; }
   const std::vector<float> &getErrorVariance() const { return errorVariance_; }

@@ -140,6 +140,20 @@ void FFTGrid::setAccessMode(int mode) {
                            "setAccessMode");
 }
 
+int FFTGrid::interpolateSeismic(float energyTreshold) {
+  if (cubetype_ != DATA || istransformed_) throw Error(CRAVA_HIP_E_STATE, "interpolateSeismic needs a real DATA grid");
+  const bool hostCopy = mode_ != NONE;
+  if (hostCopy && mode_ != READ)
+    check(crava_hip_grid_upload(ctx_->handle(), dev_, host_.data(), sizeof(float) * rsize_), "interpolateSeismic");
+  int flagged = 0, zero = 0;
+  check(crava_hip_interpolate_seismic(ctx_->handle(), nx_, ny_, nz_, nxp_, nyp_, nzp_, energyTreshold, dev_, &flagged,
+                                      &zero),
+        "interpolateSeismic");
+  if (hostCopy)
+    check(crava_hip_grid_download(ctx_->handle(), host_.data(), dev_, sizeof(float) * rsize_), "interpolateSeismic");
+  return flagged;
+}
+
 void FFTGrid::endAccess() {
   if (mode_ == WRITE || mode_ == READANDWRITE || mode_ == RANDOMACCESS)
     check(crava_hip_grid_upload(ctx_->handle(), dev_, host_.data(), sizeof(float) * rsize_), "endAccess");

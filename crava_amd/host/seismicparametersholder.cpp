@@ -57,6 +57,23 @@ static void dft_real(const std::vector<double> &x, std::vector<double> &re, std:
   }
 }
 
+std::vector<float> SeismicParametersHolder::getPriorCorrTFiltered(int nz, int nzp) const {
+  FFTGrid *c = cov_[0];
+  if (!c || c->getIsTransformed())
+    throw Error(CRAVA_HIP_E_STATE, "getPriorCorrTFiltered needs covAlpha in the time domain");
+  c->setAccessMode(FFTGrid::READ);
+  const float constant = c->getRealValue(0, 0, 0);
+  std::vector<float> circ(nzp);
+  for (int k = 0; k < nzp; ++k) circ[k] = c->getRealValue(0, 0, k, true) / constant;
+  c->endAccess();
+  std::vector<float> filtered(nzp);
+  for (int i = 0; i < nzp; ++i) {
+    const int refk = i < nzp / 2 + 1 ? i : nzp - i;
+    filtered[i] = refk < nz ? circ[refk] : 0.0f;
+  }
+  return filtered;
+}
+
 std::vector<float> SeismicParametersHolder::computeCircCorrT(const std::vector<float> &priorCorrT, int minIntFq,
                                                              int nzp) const {
   if (priorCorrT.empty() || priorCorrT[0] == 0.0f) throw Error(CRAVA_HIP_E_INVALID, "priorCorrT[0] must be nonzero");

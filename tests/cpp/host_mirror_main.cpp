@@ -158,8 +158,48 @@ int main(int argc, char **argv) {
     fwd.computeSyntSeismic(sph.mu(0), sph.mu(1), sph.mu(2), syn);
     for (int l = 0; l < nt; ++l) {
       write_bin(dir + "/out_syn" + std::to_string(l) + ".bin", drain_real(syn[l]));
-      delete syn[l];
     }
+
+    // non-constant thickness (Crava::divideDataByScaleWavelet with
+    // interpolateSeismic) on the synthetic seismic, two traces of each angle
+    // emptied first, with a prior holder whose covariance grids are still in
+    // the time domain, as in the reference's Crava constructor
+    {
+      crava_hip::SeismicParametersHolder sph2(&ctx);
+      sph2.allocateGrids(s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
+      sph2.setPriorVar0(sigma0);
+      sph2.initializeCorrelations(s.corrXY, corrT, s.corrGradI, s.corrGradJ, lowIntCut, s.nzp);
+      crava_hip::Crava::Settings s2 = s;
+      s2.stretch.clear();
+      s2.relThick.resize((size_t)s.nxp * s.nyp);
+      for (int j = 0; j < s.nyp; ++j)
+        for (int i = 0; i < s.nxp; ++i)
+          s2.relThick[(size_t)j * s.nxp + i] = 0.7 + 0.3 * (double)((i * 5 + j * 3) % 13) / 12.0;
+      std::vector<float> gain(ntr);
+      for (size_t t = 0; t < ntr; ++t) gain[t] = 0.8f + 0.05f * (float)(t % 9);
+      s2.localGain.assign(nt, {});
+      s2.localGain[0] = gain;
+      s2.energyThreshold = 0.05f;
+      for (int l = 0; l < nt; ++l) {
+        syn[l]->setAccessMode(FFTGrid::RANDOMACCESS);
+        for (int k = 0; k < s.nzp; ++k) {
+          syn[l]->setRealValue(2, 1, k, 0.0f, true);
+          syn[l]->setRealValue(3, 1, k, 0.0f, true);
+        }
+        syn[l]->endAccess();
+      }
+      crava_hip::Crava cnt(&ctx, s2, syn);
+      cnt.computeDataVariance();
+      cnt.setupErrorCorrelation([angRange](float dTheta) { return std::exp(-3.0 * std::fabs(dTheta) / angRange); });
+      cnt.divideDataByScaleWavelet(sph2);
+      std::vector<float> ev(nt);
+      for (int l = 0; l < nt; ++l) ev[l] = (float)cnt.getErrThetaCov()[(size_t)l * nt + l];
+      write_bin(dir + "/swd_errvar.bin", ev);
+      write_bin(dir + "/swd_gain.bin", gain);
+      write_bin(dir + "/swd_corrt.bin", sph2.getPriorCorrTFiltered(s.nz, s.nzp));
+      for (int l = 0; l < nt; ++l) write_bin(dir + "/out_swd" + std::to_string(l) + ".bin", drain_real(syn[l]));
+    }
+    for (int l = 0; l < nt; ++l) delete syn[l];
 
     // CRAVA binary files (FFTGrid::writeCravaFile / readCravaFile): write the
     // posterior mean of alpha, read the file the test wrote, and report a

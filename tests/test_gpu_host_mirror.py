@@ -206,6 +206,31 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
         pad = np.ones(shape[1:], bool)
         pad[:c.ny, :c.nx] = False
         assert np.array_equal(got[:, pad], syn_ref[l][:, pad]), l
+    # divideDataByScaleWavelet + interpolateSeismic of that synthetic seismic
+    # (traces (2, 1) and (3, 1) emptied, a gain table on angle 0, the shift
+    # table on the last angle, relThick 0.7 + 0.3 ((5i + 3j) mod 13) / 12)
+    ev = np.fromfile(os.path.join(d, "swd_errvar.bin"), dtype=np.float32)
+    gain = np.fromfile(os.path.join(d, "swd_gain.bin"), dtype=np.float32).reshape(c.ny, c.nx)
+    corr_f = np.fromfile(os.path.join(d, "swd_corrt.bin"), dtype=np.float32)
+    circ_full = prob.circ_corr_t
+    refk = np.minimum(np.arange(nzp), nzp - np.arange(nzp))
+    want_corr = np.where(refk < c.nz, circ_full[np.minimum(refk, len(circ_full) - 1)] / circ_full[0], 0.0)
+    assert np.abs(corr_f - want_corr).max() < 1e-6
+    rel = np.array([[0.7 + 0.3 * ((i * 5 + j * 3) % 13) / 12.0 for i in range(nxp)] for j in range(nyp)])
+    syn_in = []
+    for l in range(nt):
+        g = _read(os.path.join(d, f"out_syn{l}.bin"), shape).copy()
+        g[:, 1, 2:4] = 0.0
+        syn_in.append(g)
+    div = oracle.divide_by_scale_wavelet(c.nx, c.ny, c.nz, nxp, nyp, nzp, syn_in, prob.wavelets, prob.wnorm, prob.A,
+                                         prob.sigma0, corr_f, ev, rel, c.dt, c.nz * c.dt, c.low_cut, c.high_cut,
+                                         c.wnc, shift=shift, gain=[gain] + [None] * (nt - 1))
+    for l in range(nt):
+        want, nf, n0 = oracle.interpolate_seismic(c.nx, c.ny, c.nz, nxp, nyp, nzp, 0.05, div[l])
+        assert n0 >= 2, l
+        got = _read(os.path.join(d, f"out_swd{l}.bin"), shape)[:, :, :nxp].astype(np.float64)
+        ref = want[:, :, :nxp].astype(np.float64)
+        assert np.abs(got - ref).max() <= 5e-6 * np.abs(ref).max(), l
     for i in range(3):
         sim = _read(os.path.join(d, f"out_sim{i}.bin"), shape)[:, :, :nxp]
         assert np.isfinite(sim).all()
