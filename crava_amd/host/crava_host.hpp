@@ -144,6 +144,30 @@ class FFTGrid {
   void writeStormFile(const std::string &fileName, const StormHeader &h, bool ascii, bool padding = false,
                       bool flat = false, bool scientific = false) const;
 
+  // Simbox geometry of the SEG-Y and SGRI writers (the Simbox accessors of
+  // src/simbox.h): lateral grid and rotation (getx0, gety0, getdx, getdy,
+  // getAngle in radians), GetLZ and getdz, getMinMaxZ, the inline /
+  // crossline transform (getIL0 ... getXLStepY) and the top and base time
+  // surfaces at the trace centres (getTop / getBot, RMISSING where missing),
+  // [ny*nx] with i fastest.
+  struct SimboxGeometry {
+    double xmin = 0.0, ymin = 0.0, dx = 1.0, dy = 1.0, angle = 0.0;
+    double lz = 1.0, dz = 1.0, zmin = 0.0, zmax = 1.0;
+    double il0 = 0.0, xl0 = 0.0, ilStepX = 1.0, ilStepY = 0.0, xlStepX = 0.0, xlStepY = 1.0;
+    std::vector<double> top, bot;
+  };
+  // SEG-Y file fileName + ".segy" of the volume (FFTGrid::writeSegyFile,
+  // src/fftgrid.cpp:2189-2264, with NRLib::SegY): the reference's standard
+  // textual header, its binary header, and one trace per (i, j) with a
+  // defined top, resampled to z0 + k dz (dz the simbox's rounded to whole
+  // ms), sorted by inline and crossline, IBM floats.  traceHeaderFormat:
+  // SEISWORKS 0, IESX 1, SIP 2, CHARISMA 3, SIPX 4 (NRLib::TraceHeaderFormat).
+  void writeSegyFile(const std::string &fileName, const SimboxGeometry &g, float z0, int traceHeaderFormat = 0) const;
+  // SGRI files fileName + ".Sgrh" (text header) and ".Sgri" (big-endian
+  // floats) of the volume on a regular time axis from zmin
+  // (FFTGrid::writeSgriFile, src/fftgrid.cpp:2325-2397).
+  void writeSgriFile(const std::string &fileName, const SimboxGeometry &g, const std::string &label) const;
+
   // FFTGrid::interpolateSeismic (src/fftgrid.cpp:2564-2655) on a real DATA
   // grid: traces with energy at most energyTreshold times the mean are
   // rebuilt from their neighbours, the traces outside the box of the others

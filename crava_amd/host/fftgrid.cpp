@@ -1,7 +1,10 @@
 // FFTGrid / Context of the C++ host mirror (see crava_host.hpp).
 #include "crava_host.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <map>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -361,6 +364,265 @@ void FFTGrid::writeStormFile(const std::string &fileName, const StormHeader &h, 
   }
   os << "0\n";
   if (!os) throw Error(CRAVA_HIP_E_INVALID, "write failed on " + fname);
+}
+
+// ---------------------------------------------------------------------------
+// SEG-Y (src/fftgrid.cpp:2189-2264 with NRLib::SegY, TextualHeader,
+// BinaryHeader, TraceHeader, SegyGeometry) and SGRI (src/fftgrid.cpp:2325-2397)
+namespace {
+
+constexpr double kRMissingD = -99999.000;    // src/definitions.h:35
+constexpr double kWellMissingD = -999.00000;  // src/definitions.h:39
+
+// EBCDIC of the characters of the textual header (code page 037)
+unsigned char ebcdic(char c) {
+  if (c >= '0' && c <= '9') return (unsigned char)(0xF0 + (c - '0'));
+  if (c >= 'A' && c <= 'I') return (unsigned char)(0xC1 + (c - 'A'));
+  if (c >= 'J' && c <= 'R') return (unsigned char)(0xD1 + (c - 'J'));
+  if (c >= 'S' && c <= 'Z') return (unsigned char)(0xE2 + (c - 'S'));
+  return 0x40;   // space
+}
+
+// The reference's standard SEG-Y textual header (TextualHeader::standardHeader):
+// 40 card images of 80 columns, "C n" in the first columns.
+std::string textual_header() {
+  static const char *const body[40] = {
+      "", "", "", "", "", "", "", "", "", "", "", "         THIS SEGY FILE WAS WRITTEN BY", "",
+      "         THE CRAVA PROGRAM", "", "         DEVELOPED BY", "", "         RAGNAR HAUGE", "",
+      "         ANNE RANDI SYVERSVEEN           AND", "", "         ODD KOLBJORNSEN", "", "         AT", "",
+      "         THE NORWEGIAN COMPUTING CENTER", "", "         2003", "", "", "", "", "", "", "", "", "", "", "",
+      " END EBCDIC"};
+  std::string out;
+  for (int l = 0; l < 40; ++l) {
+    char card[81];
+    std::snprintf(card, sizeof(card), "C%2d%-77s", l + 1, body[l]);
+    for (int c = 0; c < 80; ++c) out.push_back((char)ebcdic(card[c]));
+  }
+  return out;
+}
+
+void be16(std::string &b, short v) {
+  b.push_back((char)((unsigned short)v >> 8));
+  b.push_back((char)((unsigned short)v & 0xFF));
+}
+void be32(std::string &b, int v) {
+  for (int s = 24; s >= 0; s -= 8) b.push_back((char)(((unsigned int)v >> s) & 0xFF));
+}
+
+// IEEE single -> IBM single: value = (M / 2^24) 2^(e - 126), M the 24-bit
+// significand with its leading bit, written as F 16^(E - 64) / 2^24 with
+// E = ceil((e - 126) / 4) + 64 and F = M >> (4 (E - 64) - (e - 126)),
+// truncated; zero (either sign) -> 0.  The exponent field is used as is for
+// every bit pattern (no special cases), as NRLib's writer does.
+uint32_t ibm_float(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) == 0) return 0;
+  const int e = (int)((u >> 23) & 0xff);
+  const uint32_t m = (u & 0x7fffffu) | 0x800000u;
+  const int t = e - 126;
+  const int q = (t >= 0) ? (t + 3) / 4 : -((-t) / 4);   // ceil(t / 4)
+  const int shift = 4 * q - t;
+  return (u & 0x80000000u) | ((uint32_t)(q + 64) << 24) | (m >> shift);
+}
+
+// TraceHeaderFormat locations (1-based): scalco, utm x, utm y, inline, crossline
+struct Thf {
+  int scalco, utmx, utmy, il, xl;
+};
+Thf trace_header_format(int code) {
+  switch (code) {
+    case 0: return {71, 73, 77, 9, 21};      // SeisWorks
+    case 1: return {71, 73, 77, 221, 21};    // IESX
+    case 2: return {71, 181, 185, 189, 193}; // SIP
+    case 3: return {71, 73, 77, 5, 21};      // Charisma
+    case 4: return {71, 73, 77, 181, 185};   // SIPX
+    default: throw Error(CRAVA_HIP_E_INVALID, "unknown trace header format");
+  }
+}
+
+// the 240 bytes of a trace header (TraceHeader::Write)
+std::string trace_header(const Thf &f, float utmx, float utmy, int il, int xl, short ns, short dt) {
+  std::string b;
+  int i = 0;
+  while (i < 240) {
+    if (i == f.scalco - 1) { be16(b, 0); i += 2; }
+    else if (i == f.utmx - 1) { be32(b, (int)utmx); i += 4; }
+    else if (i == f.utmy - 1) { be32(b, (int)utmy); i += 4; }
+    else if (i == 114) { be16(b, ns); i += 2; }
+    else if (i == 116) { be16(b, dt); i += 2; }
+    else if (i == f.il - 1) { be32(b, il); i += 4; }
+    else if (i == f.xl - 1) { be32(b, xl); i += 4; }
+    else { b.push_back(0); b.push_back(0); i += 2; }
+  }
+  return b;
+}
+
+struct Surfaces {
+  const FFTGrid::SimboxGeometry &g;
+  int nx;
+  void xy(int i, int j, double &x, double &y) const {   // Simbox::getXYCoord
+    const double rx = ((double)i + 0.5) * g.dx, ry = ((double)j + 0.5) * g.dy;
+    x = rx * std::cos(g.angle) - ry * std::sin(g.angle) + g.xmin;
+    y = rx * std::sin(g.angle) + ry * std::cos(g.angle) + g.ymin;
+  }
+  double top(int i, int j) const { return g.top[(size_t)j * nx + i]; }
+  double bot(int i, int j) const { return g.bot[(size_t)j * nx + i]; }
+  double rel_thick(int i, int j) const {                 // Simbox::getRelThick
+    const double t = top(i, j), b = bot(i, j);
+    return (t != kRMissingD && b != kRMissingD) ? (b - t) / g.lz : 1.0;
+  }
+};
+
+}  // namespace
+
+void FFTGrid::writeSegyFile(const std::string &fileName, const SimboxGeometry &g, float z0,
+                            int traceHeaderFormat) const {
+  if (istransformed_) throw Error(CRAVA_HIP_E_STATE, "writeSegyFile needs a real grid");
+  if ((long long)g.top.size() != (long long)nx_ * ny_ || (long long)g.bot.size() != (long long)nx_ * ny_)
+    throw Error(CRAVA_HIP_E_INVALID, "writeSegyFile needs top and base surfaces for every trace");
+  const Thf thf = trace_header_format(traceHeaderFormat);
+  std::vector<float> v((size_t)rsize_);
+  check(crava_hip_grid_download(ctx_->handle(), v.data(), dev_, sizeof(float) * rsize_), "writeSegyFile");
+  const auto value = [&](int i, int j, int k) -> float {   // getRealValue inside the volume
+    if (i < 0 || j < 0 || k < 0 || i >= nx_ || j >= ny_ || k >= nz_) return (float)kRMissingD;
+    return v[(size_t)i + (size_t)rnxp_ * ((size_t)j + (size_t)nyp_ * k)];
+  };
+  const Surfaces sb{g, nx_};
+  float dz = (float)std::floor(g.dz + 0.5);
+  if (dz == 0) dz = 1.0f;
+  const int segynz = (int)std::ceil((g.zmax - z0) / dz);
+  // traces by their index in the SEG-Y geometry (SegyGeometry::FindIndex);
+  // a later trace at the same index replaces the earlier one
+  struct Trace {
+    float x, y;
+    int il, xl;
+    std::vector<float> s;
+  };
+  std::map<size_t, Trace> traces;
+  const double cr = std::cos(g.angle), sr = std::sin(g.angle);
+  std::vector<float> tr(segynz > 0 ? segynz : 0);
+  for (int j = 0; j < ny_; ++j)
+    for (int i = 0; i < nx_; ++i) {
+      double x, y;
+      sb.xy(i, j, x, y);
+      const double z = sb.top(i, j);
+      if (z == kRMissingD || z == kWellMissingD) continue;
+      const double gdz = g.dz * sb.rel_thick(i, j);
+      const int firstData = (int)std::floor(0.5 + (z - z0) / dz);
+      int endData = (int)std::floor(0.5 + ((z - z0) + nz_ * gdz) / dz);
+      if (endData > segynz) endData = segynz;
+      int k = 0;
+      for (; k < firstData; ++k) tr[k] = 0;
+      for (; k < endData; ++k) {
+        // getRegularZInterpolatedRealValue (src/fftgrid.cpp:2491-2522)
+        const float zz = (float)((double)z0 + (double)dz * k);
+        float t = (float)((zz - z) / gdz);
+        int kg = int(t);
+        t -= kg;
+        float val;
+        if (kg < -1 || kg > nz_ - 1) {
+          val = (float)kRMissingD;
+        } else {
+          if (kg < 0) { kg = 0; t = 0; }
+          else if (kg > nz_ - 2) { kg = nz_ - 2; t = 1; }
+          const float v1 = value(i, j, kg), v2 = value(i, j, kg + 1);
+          val = (1 - t) * v1 + t * v2;
+        }
+        tr[k] = val;
+      }
+      for (; k < segynz; ++k) tr[k] = 0;
+      const float xf = (float)x, yf = (float)y;
+      // SegyGeometry::IsInside / FindILXL / FindIndex
+      const float xi = (float)(((xf - g.xmin) * cr + (yf - g.ymin) * sr) / g.dx);
+      const float yi = (float)((-(xf - g.xmin) * sr + (yf - g.ymin) * cr) / g.dy);
+      if (!(xi >= 0.0f && xi < nx_ && yi >= 0.0f && yi < ny_))
+        throw Error(CRAVA_HIP_E_INVALID, "writeSegyFile: coordinates are outside grid");
+      const int il = (int)(0.5 + g.il0 + (xf - g.xmin) * g.ilStepX + (yf - g.ymin) * g.ilStepY);
+      const int xl = (int)(0.5 + g.xl0 + (xf - g.xmin) * g.xlStepX + (yf - g.ymin) * g.xlStepY);
+      const int ii = (int)(((xf - g.xmin) * cr + (yf - g.ymin) * sr) / g.dx);
+      const int jj = (int)((-(xf - g.xmin) * sr + (yf - g.ymin) * cr) / g.dy);
+      if (ii < 0 || jj < 0 || ii >= nx_ || jj >= ny_) throw Error(CRAVA_HIP_E_INVALID, "writeSegyFile: invalid index");
+      traces[(size_t)ii + (size_t)nx_ * jj] = Trace{xf, yf, il, xl, tr};
+    }
+  // SegY::WriteAllTracesToFile: by inline, crossline, then index
+  std::vector<std::pair<size_t, const Trace *>> order;
+  for (const auto &kv : traces) order.push_back({kv.first, &kv.second});
+  std::sort(order.begin(), order.end(), [](const std::pair<size_t, const Trace *> &a,
+                                           const std::pair<size_t, const Trace *> &b) {
+    if (a.second->il != b.second->il) return a.second->il < b.second->il;
+    if (a.second->xl != b.second->xl) return a.second->xl < b.second->xl;
+    return a.first < b.first;
+  });
+  std::string out = textual_header();
+  // BinaryHeader::Write (written before the geometry is set: 0 traces per line)
+  be32(out, 9999);
+  be32(out, 1);
+  be32(out, 1);
+  be16(out, 0);
+  be16(out, 0);
+  be16(out, (short)(1000 * (double)dz));
+  be16(out, 0);
+  be16(out, (short)segynz);
+  be16(out, 0);
+  be16(out, 1);
+  be16(out, 1);
+  be16(out, 4);
+  for (int q = 0; q < 12; ++q) be16(out, 0);
+  be16(out, 1);
+  out.append(344, '\0');
+  const short dt = (short)(unsigned short)(dz * 1000);
+  for (const auto &o : order) {
+    out += trace_header(thf, o.second->x, o.second->y, o.second->il, o.second->xl, (short)segynz, dt);
+    for (float f : o.second->s) be32(out, (int)ibm_float(f));
+  }
+  const std::string fname = fileName + ".segy";
+  std::ofstream os(fname, std::ios::out | std::ios::binary);
+  if (!os) throw Error(CRAVA_HIP_E_INVALID, "cannot open " + fname + " for writing");
+  os.write(out.data(), (std::streamsize)out.size());
+  if (!os) throw Error(CRAVA_HIP_E_INVALID, "write failed on " + fname);
+}
+
+void FFTGrid::writeSgriFile(const std::string &fileName, const SimboxGeometry &g, const std::string &label) const {
+  if (istransformed_) throw Error(CRAVA_HIP_E_STATE, "writeSgriFile needs a real grid");
+  if ((long long)g.top.size() != (long long)nx_ * ny_ || (long long)g.bot.size() != (long long)nx_ * ny_)
+    throw Error(CRAVA_HIP_E_INVALID, "writeSgriFile needs top and base surfaces for every trace");
+  std::vector<float> v((size_t)rsize_);
+  check(crava_hip_grid_download(ctx_->handle(), v.data(), dev_, sizeof(float) * rsize_), "writeSgriFile");
+  const Surfaces sb{g, nx_};
+  const std::string hname = fileName + ".Sgrh", bname = fileName + ".Sgri";
+  std::ofstream h(hname);
+  if (!h) throw Error(CRAVA_HIP_E_INVALID, "cannot open " + hname + " for writing");
+  h << "NORSAR General Grid Format v1.0\n3\nX (km)\nY (km)\nT (s)\nFFT-grid\n1\n" << label << "\n1 1 1\n";
+  const float dz = (float)g.dz;
+  const int nz = (int)std::ceil((g.zmax - g.zmin) / dz);
+  h << nx_ << " " << ny_ << " " << nz << "\n";
+  h << std::setprecision(10);
+  h << g.dx * 0.001 << " " << g.dy * 0.001 << " " << dz * 0.001 << "\n";
+  h << (g.xmin + 0.5 * g.dx) * 0.001 << " " << (g.ymin + 0.5 * g.dy) * 0.001 << " " << (g.zmin + 0.5 * dz) * 0.001
+    << "\n";
+  h << g.angle << " 0\n" << kRMissingD << "\n" << bname << "\n0\n";
+  if (!h) throw Error(CRAVA_HIP_E_INVALID, "write failed on " + hname);
+  std::string out;
+  out.reserve((size_t)4 * nx_ * ny_ * (nz > 0 ? nz : 0));
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny_; ++j)
+      for (int i = 0; i < nx_; ++i) {
+        const double zt = sb.top(i, j), zb = sb.bot(i, j);
+        const double z = g.zmin + k * dz;
+        float val = (float)kRMissingD;
+        if (!(z < zt || z > zb)) {
+          const int sk = (int)((z - zt) / (g.dz * sb.rel_thick(i, j)) + 0.5);
+          if (sk >= 0 && sk < nz_) val = v[(size_t)i + (size_t)rnxp_ * ((size_t)j + (size_t)nyp_ * sk)];
+        }
+        uint32_t u;
+        std::memcpy(&u, &val, 4);
+        be32(out, (int)u);
+      }
+  std::ofstream b(bname, std::ios::out | std::ios::binary);
+  if (!b) throw Error(CRAVA_HIP_E_INVALID, "cannot open " + bname + " for writing");
+  b.write(out.data(), (std::streamsize)out.size());
+  if (!b) throw Error(CRAVA_HIP_E_INVALID, "write failed on " + bname);
 }
 
 }  // namespace crava_hip

@@ -308,6 +308,52 @@ def synthetic_seismic(nx, ny, nz, nxp, nyp, nzp, A, wavelets, dz, alpha, beta, r
     return out
 
 
+# ------------------------------------------------ SEG-Y / SGRI (reference NRLib)
+REF_SEGY_SO = os.path.join(HERE, "_ref", "libcrava_ref_segy.so")
+_ref_segy = None
+
+
+def ref_segy_available() -> bool:
+    return os.path.exists(REF_SEGY_SO)
+
+
+def _segy_lib():
+    global _ref_segy
+    if _ref_segy is None:
+        L = C.CDLL(REF_SEGY_SO)
+        I, F, D, S = C.c_int, C.c_float, C.c_double, C.c_char_p
+        L.ref_write_segy.argtypes = [S, VP, I, I, I, I, I] + [D] * 15 + [VP, VP, F, I]
+        L.ref_write_sgri.argtypes = [S, S, VP, I, I, I, I, I] + [D] * 9 + [VP, VP]
+        _ref_segy = L
+    return _ref_segy
+
+
+_GEOM_KEYS = ("xmin", "ymin", "dx", "dy", "angle", "lz", "dz", "zmin", "zmax",
+              "il0", "xl0", "ilStepX", "ilStepY", "xlStepX", "xlStepY")
+
+
+def write_segy_ref(path, grid, nx, ny, nz, geom, top, bot, z0, thf=0):
+    """FFTGrid::writeSegyFile through the reference's NRLib SegY: writes
+    path + ".segy".  grid: real padded float [nzp][nyp][rnxp]; geom: dict of
+    the simbox geometry (xmin ... xlStepY); top/bot: [ny][nx] float64."""
+    g = np.ascontiguousarray(grid, dtype=np.float32)
+    t = np.ascontiguousarray(top, dtype=np.float64)
+    b = np.ascontiguousarray(bot, dtype=np.float64)
+    rc = _segy_lib().ref_write_segy(path.encode(), g.ctypes.data, nx, ny, nz, g.shape[2], g.shape[1],
+                                    *[float(geom[k]) for k in _GEOM_KEYS], t.ctypes.data, b.ctypes.data, z0, thf)
+    assert rc == 0
+
+
+def write_sgri_ref(path, label, grid, nx, ny, nz, geom, top, bot):
+    """FFTGrid::writeSgriFile with NRLib's float writer: path + ".Sgrh" / ".Sgri"."""
+    g = np.ascontiguousarray(grid, dtype=np.float32)
+    t = np.ascontiguousarray(top, dtype=np.float64)
+    b = np.ascontiguousarray(bot, dtype=np.float64)
+    rc = _segy_lib().ref_write_sgri(path.encode(), label.encode(), g.ctypes.data, nx, ny, nz, g.shape[2], g.shape[1],
+                                    *[float(geom[k]) for k in _GEOM_KEYS[:9]], t.ctypes.data, b.ctypes.data)
+    assert rc == 0
+
+
 # ----------------------------------------------- non-constant-thickness path
 def divide_by_scale_wavelet(nx, ny, nz, nxp, nyp, nzp, grids, wavelets, wnorm, A, var0, corr_t, err_var,
                             rel_thick, dz, lz, low_cut, high_cut, wnc, shift=None, gain=None,

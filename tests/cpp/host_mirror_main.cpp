@@ -9,6 +9,7 @@
 // mu{0,1,2}.bin, data{l}.bin (raw float32).  Outputs: circ.bin,
 // out_mu{c}.bin, out_res{l}.bin (time domain), out_cov{q}.bin (transformed),
 // out_sim{c}.bin, out_scalars.txt.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -246,6 +247,51 @@ int main(int argc, char **argv) {
       sph.mu(0)->writeStormFile(dir + "/mu0_flat", sh, true, false, true);
       sph.mu(0)->writeStormFile(dir + "/mu0_pad", sh, false, true);
       std::ofstream(dir + "/storm_type.txt") << sph.mu(0)->getType() << "\n";
+    }
+
+    // SEG-Y (every trace header format) and SGRI files of the posterior
+    // mean of alpha on a rotated simbox with uneven top and base surfaces,
+    // one trace without a top; the geometry goes to segy_geom.txt / .bin
+    {
+      crava_hip::FFTGrid::SimboxGeometry g;
+      g.xmin = 456000.5;
+      g.ymin = 6780000.25;
+      g.dx = 25.0;
+      g.dy = 12.5;
+      g.angle = 0.3;
+      g.dz = s.dz;
+      g.lz = s.nz * s.dz;
+      g.il0 = 1001.0;
+      g.xl0 = 2002.0;
+      g.ilStepX = 0.0348;
+      g.ilStepY = -0.0107;
+      g.xlStepX = 0.0214;
+      g.xlStepY = 0.0693;
+      g.top.resize(ntr);
+      g.bot.resize(ntr);
+      double zmin = 1e30, zmax = -1e30;
+      for (int j = 0; j < s.ny; ++j)
+        for (int i = 0; i < s.nx; ++i) {
+          const size_t t = (size_t)j * s.nx + i;
+          g.top[t] = 1800.0 + 3.7 * i - 2.1 * j + 0.37 * ((i * 7 + j * 5) % 11);
+          g.bot[t] = g.top[t] + g.lz * (0.75 + 0.25 * (double)((i * 3 + j * 2) % 7) / 6.0);
+          zmin = std::min(zmin, g.top[t]);
+          zmax = std::max(zmax, g.bot[t]);
+        }
+      g.top[(size_t)1 * s.nx + 2] = -99999.0;   // a trace with no top surface
+      g.zmin = zmin;
+      g.zmax = zmax;
+      const float z0 = (float)(std::floor(zmin / 10.0) * 10.0);
+      for (int f = 0; f < 5; ++f) sph.mu(0)->writeSegyFile(dir + "/mu0_thf" + std::to_string(f), g, z0, f);
+      sph.mu(0)->writeSgriFile(dir + "/mu0", g, "Posterior alpha");
+      std::ofstream gt(dir + "/segy_geom.txt");
+      gt.precision(17);
+      gt << g.xmin << " " << g.ymin << " " << g.dx << " " << g.dy << " " << g.angle << " " << g.lz << " " << g.dz << " "
+         << g.zmin << " " << g.zmax << " " << g.il0 << " " << g.xl0 << " " << g.ilStepX << " " << g.ilStepY << " "
+         << g.xlStepX << " " << g.xlStepY << " " << z0 << "\n";
+      std::ofstream gb(dir + "/segy_geom.bin", std::ios::binary);
+      gb.write(reinterpret_cast<const char *>(g.top.data()), (std::streamsize)(ntr * sizeof(double)));
+      gb.write(reinterpret_cast<const char *>(g.bot.data()), (std::streamsize)(ntr * sizeof(double)));
     }
 
     // time-lapse merge (State4D::merge) when the test supplies the parts:

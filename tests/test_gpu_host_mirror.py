@@ -231,6 +231,30 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
         got = _read(os.path.join(d, f"out_swd{l}.bin"), shape)[:, :, :nxp].astype(np.float64)
         ref = want[:, :, :nxp].astype(np.float64)
         assert np.abs(got - ref).max() <= 5e-6 * np.abs(ref).max(), l
+    # SEG-Y (all five trace header formats) and SGRI files of the posterior
+    # mean of alpha, byte for byte against the reference's NRLib writer on
+    # the same grid and geometry
+    if oracle.ref_segy_available():
+        vals = open(os.path.join(d, "segy_geom.txt")).read().split()
+        keys = ("xmin", "ymin", "dx", "dy", "angle", "lz", "dz", "zmin", "zmax", "il0", "xl0", "ilStepX", "ilStepY",
+                "xlStepX", "xlStepY")
+        geom = {k: float(v) for k, v in zip(keys, vals)}
+        z0 = np.float32(float(vals[15]))
+        ntr = c.nx * c.ny
+        tb = np.fromfile(os.path.join(d, "segy_geom.bin"), dtype=np.float64)
+        top, bot = tb[:ntr].reshape(c.ny, c.nx), tb[ntr:].reshape(c.ny, c.nx)
+        mu0 = _read(os.path.join(d, "out_mu0.bin"), shape)
+        for f in range(5):
+            mine = open(os.path.join(d, f"mu0_thf{f}.segy"), "rb").read()
+            oracle.write_segy_ref(os.path.join(d, f"ref_thf{f}"), mu0, c.nx, c.ny, c.nz, geom, top, bot, z0, f)
+            assert mine == open(os.path.join(d, f"ref_thf{f}.segy"), "rb").read(), f
+            assert len(mine) > 3600 + 240
+        # the SGRI header names its binary file: the reference writes under the same name
+        mine_h = open(os.path.join(d, "mu0.Sgrh")).read()
+        mine_b = open(os.path.join(d, "mu0.Sgri"), "rb").read()
+        oracle.write_sgri_ref(os.path.join(d, "mu0"), "Posterior alpha", mu0, c.nx, c.ny, c.nz, geom, top, bot)
+        assert open(os.path.join(d, "mu0.Sgrh")).read() == mine_h
+        assert open(os.path.join(d, "mu0.Sgri"), "rb").read() == mine_b
     for i in range(3):
         sim = _read(os.path.join(d, f"out_sim{i}.bin"), shape)[:, :, :nxp]
         assert np.isfinite(sim).all()
