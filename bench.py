@@ -527,7 +527,7 @@ def nonconstant_thickness(ctx, reps: int, cpu: bool):
     import numpy as np
     import torch
     from crava_amd.core import divide_by_scale_wavelet, interpolate_seismic
-    from tests.swdiv_cases import isq_grid, swdiv_inputs
+    from crava_amd.synth import isq_grid, swdiv_inputs
     n = (448, 448, 256, 512, 512, 320)
     nt = 6
     args, kw = swdiv_inputs(*n, nt, seed=21)

@@ -16,7 +16,7 @@ import torch
 import oracle
 from crava_amd import _lib as L
 from crava_amd.core import divide_by_scale_wavelet, interpolate_seismic
-from tests.swdiv_cases import isq_grid, swdiv_inputs
+from crava_amd.synth import isq_grid, swdiv_inputs
 
 CASES = [
     # nx, ny, nz, nxp, nyp, nzp, ntheta

@@ -24,7 +24,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests.swdiv_cases import isq_grid, swdiv_inputs
+from crava_amd.synth import isq_grid, swdiv_inputs
 
 N1 = (10, 7, 24, 14, 10, 32)      # nx, ny, nz, nxp, nyp, nzp: 7.5 Hz per bin at relThick 1
 
