@@ -492,6 +492,7 @@ void FFTGrid::writeSegyFile(const std::string &fileName, const SimboxGeometry &g
   float dz = (float)std::floor(g.dz + 0.5);
   if (dz == 0) dz = 1.0f;
   const int segynz = (int)std::ceil((g.zmax - z0) / dz);
+  if (segynz < 1) throw Error(CRAVA_HIP_E_INVALID, "writeSegyFile: z0 must lie above zmax");
   // traces by their index in the SEG-Y geometry (SegyGeometry::FindIndex);
   // a later trace at the same index replaces the earlier one
   struct Trace {
@@ -509,7 +510,9 @@ void FFTGrid::writeSegyFile(const std::string &fileName, const SimboxGeometry &g
       const double z = sb.top(i, j);
       if (z == kRMissingD || z == kWellMissingD) continue;
       const double gdz = g.dz * sb.rel_thick(i, j);
-      const int firstData = (int)std::floor(0.5 + (z - z0) / dz);
+      // a top below zmax would start past the trace (the reference writes out
+      // of bounds there): clamp
+      const int firstData = std::min((int)std::floor(0.5 + (z - z0) / dz), segynz);
       int endData = (int)std::floor(0.5 + ((z - z0) + nz_ * gdz) / dz);
       if (endData > segynz) endData = segynz;
       int k = 0;
