@@ -277,6 +277,19 @@ def synthetic_seismic(ctx: Context, nx, ny, nz, nxp, nyp, nzp, A, wavelets, dz, 
                                             ptr_array([_dev_ptr(g) for g in out])), "crava_hip_synthetic_seismic")
 
 
+def reflection_matrix(theta_rad, vsvp, seismic_type=None):
+    """ModelAVODynamic::setupDefaultReflectionMatrix (crava_hip_reflection_matrix):
+    [ntheta][3] float32 for angles in radians; seismic_type None (all PP) or
+    a list of 0 (PP) / 1 (PS)."""
+    import numpy as np
+    th = np.ascontiguousarray(theta_rad, dtype=np.float32)
+    A = np.zeros((th.size, 3), dtype=np.float32)
+    st = None if seismic_type is None else np.ascontiguousarray(seismic_type, dtype=np.int32)
+    check(lib().crava_hip_reflection_matrix(th.size, th.ctypes.data, None if st is None else st.ctypes.data,
+                                            float(vsvp), A.ctypes.data), "crava_hip_reflection_matrix")
+    return A
+
+
 def divide_by_scale_wavelet(ctx: Context, nx, ny, nz, nxp, nyp, nzp, grids, wavelets, wnorm, A, var0, corr_t,
                             err_var, rel_thick, dz, lz, low_cut, high_cut, wnc, shift=None, gain=None):
     """Wavelet division of the non-constant-thickness path

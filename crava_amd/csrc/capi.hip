@@ -230,6 +230,36 @@ int crava_hip_perk_tables(int ntheta, int nz, int nzp, double lz, float low_cut,
   return CRAVA_HIP_OK;
 }
 
+int crava_hip_reflection_matrix(int ntheta, const float *theta, const int *seismic_type, double vsvp, float *A) {
+  if (ntheta < 1 || !theta || !A) {
+    set_error("invalid arguments to crava_hip_reflection_matrix");
+    return CRAVA_HIP_E_INVALID;
+  }
+  const double vsvp2 = vsvp * vsvp;
+  for (int i = 0; i < ntheta; ++i) {
+    const int type = seismic_type ? seismic_type[i] : 0;
+    const double angle = (double)theta[i];
+    const double sint = std::sin(angle), sint2 = sint * sint;
+    float *a = A + 3 * i;
+    if (type == 0) {            // PP (linearised Zoeppritz, Aki-Richards form)
+      const double tan2t = std::tan(angle) * std::tan(angle);
+      a[0] = (float)((1.0 + tan2t) / 2.0);
+      a[1] = (float)(-4 * vsvp2 * sint2);
+      a[2] = (float)((1.0 - 4.0 * vsvp2 * sint2) / 2.0);
+    } else if (type == 1) {     // PS
+      const double cost = std::cos(angle), cosp = std::sqrt(1 - vsvp2 * sint2);
+      const double fac = 0.5 * sint / cosp;
+      a[0] = 0;
+      a[1] = (float)(4.0 * fac * (vsvp2 * sint2 - vsvp * cost * cosp));
+      a[2] = (float)(fac * (-1.0 + 2 * vsvp2 * sint2 + 2 * vsvp * cost * cosp));
+    } else {
+      set_error("seismic_type must be 0 (PP) or 1 (PS)");
+      return CRAVA_HIP_E_INVALID;
+    }
+  }
+  return CRAVA_HIP_OK;
+}
+
 int crava_hip_post_create(crava_hip_ctx *ctx, const crava_hip_post_params *params, const unsigned char *invert,
                           const float *kWD, const float *e1, const float *e2, const float *e3,
                           crava_hip_post **post) {

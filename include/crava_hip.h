@@ -134,6 +134,14 @@ int crava_hip_perk_tables(int ntheta, int nz, int nzp, double lz, float low_cut,
                           float min_rel_thick, const float *wavelets, const float *wnorm,
                           unsigned char *invert, float *kWD, float *e1, float *e2, float *e3);
 
+/* Reflection matrix of ModelAVODynamic::setupDefaultReflectionMatrix
+ * (src/modelavodynamic.cpp:876-928), host side: row l holds the
+ * coefficients of (ln Vp, ln Vs, ln rho) changes at incidence angle
+ * theta[l] (radians) for the ratio vsvp = Vs/Vp, computed in double and
+ * rounded to float.  seismic_type[l]: 0 PP (ModelSettings::STANDARDSEIS),
+ * 1 PS (PSSEIS); NULL means every angle PP.  A: [ntheta][3]. */
+int crava_hip_reflection_matrix(int ntheta, const float *theta, const int *seismic_type, double vsvp, float *A);
+
 /* Posterior problem: everything that is not a grid. */
 typedef struct {
   int nxp, nyp, nzp, ntheta;   /* ntheta in 1..8 */
