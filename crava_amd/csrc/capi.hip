@@ -978,6 +978,26 @@ int crava_hip_grid_upload(crava_hip_ctx *ctx, void *grid, const void *host, size
   return CRAVA_HIP_OK;
 }
 
+int crava_hip_grid_read_trace(crava_hip_ctx *ctx, const void *grid, int nxp, int nyp, int nzp, int i, int j, int k0,
+                              int nk, float *out) {
+  if (!ctx || !grid || !out || nxp < 1 || nyp < 1 || nzp < 1) {
+    set_error("invalid arguments to crava_hip_grid_read_trace");
+    return CRAVA_HIP_E_INVALID;
+  }
+  const int rnxp = 2 * (nxp / 2 + 1);
+  if (i < 0 || i >= rnxp || j < 0 || j >= nyp || k0 < 0 || nk < 0 || k0 + nk > nzp) {
+    set_error("trace outside the grid in crava_hip_grid_read_trace");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (nk == 0) return CRAVA_HIP_OK;
+  const size_t plane = (size_t)rnxp * nyp * sizeof(float);
+  const char *src = static_cast<const char *>(grid) + (size_t)k0 * plane + ((size_t)j * rnxp + i) * sizeof(float);
+  CRAVA_HIP_CHECK(hipMemcpy2DAsync(out, sizeof(float), src, plane, sizeof(float), (size_t)nk, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return CRAVA_HIP_OK;
+}
+
 int crava_hip_grid_download(crava_hip_ctx *ctx, void *host, const void *grid, size_t bytes) {
   if (!ctx || !grid || !host) { set_error("invalid arguments to crava_hip_grid_download"); return CRAVA_HIP_E_INVALID; }
   CRAVA_HIP_CHECK(hipMemcpyAsync(host, grid, bytes, hipMemcpyDeviceToHost, ctx->stream));

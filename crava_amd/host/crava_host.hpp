@@ -102,6 +102,9 @@ class FFTGrid {
   fftw_complex_f getNextComplex();
   int setNextComplex(fftw_complex_f value);
   float getRealValue(int i, int j, int k, bool extSimbox = false) const;
+  // values k0 <= k < k0 + nk of trace (i, j) of the real grid, read from the
+  // device without an access mode (crava_hip_grid_read_trace)
+  std::vector<float> readTrace(int i, int j, int k0, int nk) const;
   int setRealValue(int i, int j, int k, float value, bool extSimbox = false);
   fftw_complex_f getComplexValue(int i, int j, int k, bool extSimbox = false) const;
 
@@ -209,6 +212,12 @@ class SeismicParametersHolder {
   // temporal correlation read from the real covAlpha grid at (0, 0, k),
   // zero where min(k, nzp - k) >= nz.
   std::vector<float> getPriorCorrTFiltered(int nz, int nzp) const;
+  // :771-806: the prior variances from the covariance grids' values at
+  // the origin (time domain), and the posterior covariance's z profile at
+  // lateral lag (0, 0) over the volume's nz samples.
+  float getOrigin(FFTGrid *grid) const;
+  void updatePriorVar();
+  std::vector<float> createPostCov00(FFTGrid *postCov) const;
   void makeCircCorrTPosDef(std::vector<float> &circCorrT, int minIntFq, int nzp) const;
 
   void FFTAllGrids();       // :282-298, one launch per pass for each set

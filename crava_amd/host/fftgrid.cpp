@@ -203,6 +203,13 @@ float FFTGrid::getRealValue(int i, int j, int k, bool extSimbox) const {
   return host_[(size_t)i + (size_t)rnxp_ * ((size_t)j + (size_t)nyp_ * k)];
 }
 
+std::vector<float> FFTGrid::readTrace(int i, int j, int k0, int nk) const {
+  if (istransformed_) throw Error(CRAVA_HIP_E_STATE, "readTrace needs a real grid");
+  std::vector<float> v((size_t)(nk > 0 ? nk : 0));
+  check(crava_hip_grid_read_trace(ctx_->handle(), dev_, nxp_, nyp_, nzp_, i, j, k0, nk, v.data()), "readTrace");
+  return v;
+}
+
 int FFTGrid::setRealValue(int i, int j, int k, float value, bool extSimbox) {
   if (mode_ == NONE || mode_ == READ) throw Error(CRAVA_HIP_E_STATE, "setRealValue needs WRITE or RANDOMACCESS");
   const bool in = extSimbox ? (i < nxp_ && j < nyp_ && k < nzp_) : (i < nx_ && j < ny_ && k < nz_);

@@ -57,15 +57,36 @@ static void dft_real(const std::vector<double> &x, std::vector<double> &re, std:
   }
 }
 
+float SeismicParametersHolder::getOrigin(FFTGrid *grid) const {
+  if (!grid || grid->getIsTransformed()) throw Error(CRAVA_HIP_E_STATE, "getOrigin needs a grid in the time domain");
+  return grid->readTrace(0, 0, 0, 1)[0];
+}
+
+void SeismicParametersHolder::updatePriorVar() {
+  priorVar0_[0][0] = getOrigin(cov_[0]);
+  priorVar0_[1][1] = getOrigin(cov_[1]);
+  priorVar0_[2][2] = getOrigin(cov_[2]);
+  priorVar0_[0][1] = getOrigin(cov_[3]);
+  priorVar0_[1][0] = priorVar0_[0][1];
+  priorVar0_[2][0] = getOrigin(cov_[4]);
+  priorVar0_[0][2] = priorVar0_[2][0];
+  priorVar0_[2][1] = getOrigin(cov_[5]);
+  priorVar0_[1][2] = priorVar0_[2][1];
+}
+
+std::vector<float> SeismicParametersHolder::createPostCov00(FFTGrid *postCov) const {
+  if (!postCov || postCov->getIsTransformed())
+    throw Error(CRAVA_HIP_E_STATE, "createPostCov00 needs a grid in the time domain");
+  return postCov->readTrace(0, 0, 0, postCov->getNz());
+}
+
 std::vector<float> SeismicParametersHolder::getPriorCorrTFiltered(int nz, int nzp) const {
   FFTGrid *c = cov_[0];
   if (!c || c->getIsTransformed())
     throw Error(CRAVA_HIP_E_STATE, "getPriorCorrTFiltered needs covAlpha in the time domain");
-  c->setAccessMode(FFTGrid::READ);
-  const float constant = c->getRealValue(0, 0, 0);
-  std::vector<float> circ(nzp);
-  for (int k = 0; k < nzp; ++k) circ[k] = c->getRealValue(0, 0, k, true) / constant;
-  c->endAccess();
+  std::vector<float> circ = c->readTrace(0, 0, 0, nzp);
+  const float constant = circ[0];
+  for (int k = 0; k < nzp; ++k) circ[k] = circ[k] / constant;
   std::vector<float> filtered(nzp);
   for (int i = 0; i < nzp; ++i) {
     const int refk = i < nzp / 2 + 1 ? i : nzp - i;

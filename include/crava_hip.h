@@ -66,6 +66,14 @@ int crava_hip_grid_alloc(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, int prec
 int crava_hip_grid_free(void *grid);
 int crava_hip_grid_upload(crava_hip_ctx *ctx, void *grid, const void *host, size_t bytes);
 int crava_hip_grid_download(crava_hip_ctx *ctx, void *host, const void *grid, size_t bytes);
+/* One trace of a real float grid, grid[k][j][i] for k0 <= k < k0 + nk
+ * (i < 2(nxp/2+1)), copied to host memory (synchronous).  Serves the
+ * reference's single-trace reads through FFTGrid::getRealValue
+ * (SeismicParametersHolder::getOrigin, createPostCov00 and
+ * extractParamCorrFromCovAlpha, src/seismicparametersholder.cpp:748-806)
+ * without copying the whole grid. */
+int crava_hip_grid_read_trace(crava_hip_ctx *ctx, const void *grid, int nxp, int nyp, int nzp, int i, int j, int k0,
+                              int nk, float *out);
 
 /* In-place 3-D real-to-half-complex transform of one device grid.
  * Replaces FFTGrid::fftInPlace (src/fftgrid.cpp:1804-1829): grids that are

@@ -329,6 +329,23 @@ int main(int argc, char **argv) {
       }
     }
 
+    // SeismicParametersHolder::createPostCov00 / getOrigin / updatePriorVar
+    // on the covariance grids taken back to the time domain
+    sph.invFFTCovGrids();
+    write_bin(dir + "/postcov00.bin", sph.createPostCov00(sph.cov(0)));
+    {
+      std::vector<float> org(6), pv(9);
+      for (int q = 0; q < 6; ++q) {
+        org[q] = sph.getOrigin(sph.cov(q));
+        write_bin(dir + "/out_covt" + std::to_string(q) + ".bin", drain_real(sph.cov(q)));
+      }
+      sph.updatePriorVar();
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) pv[3 * a + b] = (float)sph.getPriorVar0()[a][b];
+      write_bin(dir + "/origins.bin", org);
+      write_bin(dir + "/priorvar.bin", pv);
+    }
+
     // a few cursor / random-access checks against the values just written
     FFTGrid *m0 = sph.mu(0);
     m0->setAccessMode(FFTGrid::RANDOMACCESS);

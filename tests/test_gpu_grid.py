@@ -114,3 +114,24 @@ def test_facies_geomodel_bitexact(ctx):
     oracle.facies_geomodel(nx, ny, nz, prior, prob)
     for a, r in zip(tp, prob):
         assert np.array_equal(a.cpu().numpy(), r)
+
+
+@pytest.mark.gpu
+def test_grid_read_trace(ctx):
+    """crava_hip_grid_read_trace returns grid[k0:k0+nk, j, i] bit for bit."""
+    import ctypes as C
+    nxp, nyp, nzp = 105, 45, 98
+    rnxp = 2 * (nxp // 2 + 1)
+    g = torch.randn((nzp, nyp, rnxp), device="cuda")
+    h = g.cpu().numpy()
+    lib = L.lib()
+    for i, j, k0, nk in ((0, 0, 0, nzp), (104, 44, 3, 50), (rnxp - 1, 7, 97, 1), (17, 30, 10, 0)):
+        out = np.full(max(nk, 1), np.nan, np.float32)
+        rc = lib.crava_hip_grid_read_trace(ctx.h, C.c_void_p(g.data_ptr()), nxp, nyp, nzp, i, j, k0, nk,
+                                           out.ctypes.data)
+        assert rc == L.OK
+        assert np.array_equal(out[:nk], h[k0:k0 + nk, j, i])
+    out = np.zeros(4, np.float32)
+    for bad in ((rnxp, 0, 0, 1), (0, nyp, 0, 1), (0, 0, nzp - 1, 2), (0, 0, -1, 1)):
+        assert lib.crava_hip_grid_read_trace(ctx.h, C.c_void_p(g.data_ptr()), nxp, nyp, nzp, *bad,
+                                             out.ctypes.data) == L.E_INVALID

@@ -255,6 +255,14 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
         oracle.write_sgri_ref(os.path.join(d, "mu0"), "Posterior alpha", mu0, c.nx, c.ny, c.nz, geom, top, bot)
         assert open(os.path.join(d, "mu0.Sgrh")).read() == mine_h
         assert open(os.path.join(d, "mu0.Sgri"), "rb").read() == mine_b
+    # createPostCov00 / getOrigin / updatePriorVar read single traces of the
+    # time-domain covariance grids (crava_hip_grid_read_trace)
+    covt = [_read(os.path.join(d, f"out_covt{q}.bin"), shape) for q in range(6)]
+    assert np.array_equal(np.fromfile(os.path.join(d, "postcov00.bin"), dtype=np.float32), covt[0][:c.nz, 0, 0])
+    org = np.fromfile(os.path.join(d, "origins.bin"), dtype=np.float32)
+    assert np.array_equal(org, np.array([g[0, 0, 0] for g in covt], np.float32))
+    pv = np.fromfile(os.path.join(d, "priorvar.bin"), dtype=np.float32).reshape(3, 3)
+    assert np.array_equal(pv, org[[[0, 3, 4], [3, 1, 5], [4, 5, 2]]])
     for i in range(3):
         sim = _read(os.path.join(d, f"out_sim{i}.bin"), shape)[:, :, :nxp]
         assert np.isfinite(sim).all()
