@@ -346,6 +346,20 @@ int main(int argc, char **argv) {
       write_bin(dir + "/priorvar.bin", pv);
     }
 
+    // an ill-conditioned prior covariance is rejected (the reference's
+    // robustification is not reproducible); a well-conditioned one is kept
+    {
+      const double bad[3][3] = {{1.0, 0.9999, 0.0}, {0.9999, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+      bool rejected = false;
+      try {
+        sph.setPriorVar0(bad);
+      } catch (const crava_hip::Error &) {
+        rejected = true;
+      }
+      sph.setPriorVar0(sigma0);
+      std::ofstream(dir + "/illcond.txt") << (rejected ? 1 : 0) << "\n";
+    }
+
     // a few cursor / random-access checks against the values just written
     FFTGrid *m0 = sph.mu(0);
     m0->setAccessMode(FFTGrid::RANDOMACCESS);

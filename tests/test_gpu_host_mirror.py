@@ -255,6 +255,7 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
         oracle.write_sgri_ref(os.path.join(d, "mu0"), "Posterior alpha", mu0, c.nx, c.ny, c.nz, geom, top, bot)
         assert open(os.path.join(d, "mu0.Sgrh")).read() == mine_h
         assert open(os.path.join(d, "mu0.Sgri"), "rb").read() == mine_b
+    assert open(os.path.join(d, "illcond.txt")).read().strip() == "1"
     # createPostCov00 / getOrigin / updatePriorVar read single traces of the
     # time-domain covariance grids (crava_hip_grid_read_trace)
     covt = [_read(os.path.join(d, f"out_covt{q}.bin"), shape) for q in range(6)]
