@@ -283,6 +283,66 @@ def test_posterior_dense_time_domain_tiny():
         assert np.allclose(half, full[:, :, : nxp // 2 + 1], rtol=1e-8, atol=1e-12 * np.abs(full).max())
 
 
+def _f64_posterior(scale_noise):
+    prob = synth.make_problem(synth.config("small"))
+    cfg = prob.cfg
+    tabs = oracle.perk_tables(prob.ntheta, cfg.nz, cfg.nzp, prob.lz, cfg.low_cut, cfg.high_cut, 1.0,
+                              prob.wavelets, prob.wnorm)
+    mu, d, cv, ec = fourier_inputs(prob)
+    m64, d64, c64, e64 = as_f64(mu), as_f64(d), as_f64(cv), as_f64([ec])[0]
+    om, od, oc = [g.copy() for g in m64], [g.copy() for g in d64], [g.copy() for g in c64]
+    nf = oracle.posterior_f64(cfg.nxp, cfg.nyp, cfg.nzp, prob.A, prob.err_theta_cov * scale_noise, cfg.wnc,
+                              prob.sigma0, tabs, om, od, oc, e64.copy())
+    return prob, tabs, (m64, d64, c64), (om, od, oc), nf
+
+
+def test_posterior_noise_limit_and_mode_condition():
+    """Noise -> infinity: the posterior is the prior.  At any noise level
+    the posterior mean is the mode of the Gaussian posterior, so it satisfies
+    the information-form optimality condition
+    S^-1 (mu_post - mu) = K^H E^-1 (d - K mu_post)
+    (the oracle computes the update in the covariance form
+    mu + S K^H M^-1 (d - K mu))."""
+    C = lambda g: g.view(np.complex128)
+    prob, tabs, (m0, d0, c0), (m, d, c), nf = _f64_posterior(1e12)
+    assert nf == 0
+    inb = tabs["invert"].astype(bool)
+    for a, b in zip(m, m0):
+        assert np.abs(C(a) - C(b))[inb].max() <= 1e-5 * np.abs(C(b))[inb].max()
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    for q, (a, b) in enumerate(pairs):
+        prior = np.abs(C(c0[q]).real / prob.sigma0[a, b]) * prob.sigma0[a, b]
+        assert np.abs(C(c[q]) - prior)[inb].max() <= 1e-5 * np.abs(prior)[inb].max()
+    prob, tabs, (m0, d0, c0), (m, d, c), nf = _f64_posterior(1.0)
+    assert nf == 0
+    cfg = prob.cfg
+    kw = tabs["kWD"][..., 0].astype(np.float64) + 1j * tabs["kWD"][..., 1]
+    e1 = tabs["e1"][..., 0].astype(np.float64) + 1j * tabs["e1"][..., 1]
+    e2 = tabs["e2"][..., 0].astype(np.float64) + 1j * tabs["e2"][..., 1]
+    mu, dd, cv, ec = fourier_inputs(prob)
+    e64 = as_f64([ec])[0]
+    rng = np.random.default_rng(3)
+    checked = 0
+    while checked < 40:
+        k, j, i = rng.integers(cfg.nzp), rng.integers(cfg.nyp), rng.integers(cfg.nxp // 2 + 1)
+        if not inb[k]:
+            continue
+        K = kw[k][:, None] * prob.A.astype(np.float64)
+        lam = abs(C(e64)[k, j, i].real)
+        E = 0.5 * (1 - cfg.wnc) * prob.err_theta_cov * lam * (np.outer(e1[k], e1[k].conj()) +
+                                                                np.outer(e2[k], e2[k].conj()))
+        E = E + np.diag(cfg.wnc * np.diag(prob.err_theta_cov))
+        S = np.zeros((3, 3))
+        for q, (a, b) in enumerate(pairs):
+            S[a, b] = S[b, a] = abs(C(c0[q])[k, j, i].real / prob.sigma0[a, b]) * prob.sigma0[a, b]
+        r = np.array([C(g)[k, j, i] for g in d])
+        dm = np.array([C(g)[k, j, i] for g in m]) - np.array([C(g)[k, j, i] for g in m0])
+        lhs = np.linalg.solve(S, dm)
+        rhs = K.conj().T @ np.linalg.solve(E, r)
+        assert np.linalg.norm(lhs - rhs) <= 1e-7 * np.linalg.norm(rhs), (k, j, i)
+        checked += 1
+
+
 def test_posterior_prior_passthrough_out_of_band_and_on_failure():
     prob = synth.make_problem(synth.config("small"))
     cfg = prob.cfg
