@@ -60,6 +60,31 @@ def test_divide_by_scale_wavelet_vs_oracle(ctx, case):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n", [(10, 7, 24, 14, 10, 32), (12, 11, 60, 16, 14, 80)], ids=["nzp32", "nzp80"])
+def test_divide_by_scale_wavelet_thin_layers_reach_the_rmissing_bins(ctx, n):
+    """Thin layers (relThick 0.3-0.45) and a high cut just below Nyquist: in-band bins
+    read the stretched spectrum near Nyquist, where the reference's
+    normalised e3 holds RMISSING (Wavelet::multiplyRAmpByConstant); GPU and
+    oracle agree there too."""
+    nt = 2
+    args, kw = swdiv_inputs(*n, nt, seed=7, rel_range=(0.3, 0.45))
+    hc = 500.0 * n[2] / args[9] - 0.5          # just below the Nyquist frequency of the layer sampling
+    args[11] = hc
+    nzp, cn = n[5], n[5] // 2 + 1
+    rel = args[7]
+    df = n[2] * 1000.0 / (rel * args[9] * nzp)
+    k = np.arange(cn)[:, None, None]
+    hit = (df * k > 5.0) & (df * k < hc) & (np.float32(k) / rel.astype(np.float32) >= cn - 2)
+    assert hit.sum() > 0
+    want = oracle.divide_by_scale_wavelet(*n, *args, **kw)
+    dg = [torch.from_numpy(g).cuda() for g in args[0]]
+    divide_by_scale_wavelet(ctx, *n, dg, *args[1:], **kw)
+    torch.cuda.synchronize()
+    for l in range(nt):
+        _check(dg[l].cpu().numpy(), want[l], args[0][l], n[3])
+
+
+@pytest.mark.gpu
 def test_divide_by_scale_wavelet_full_grid_sampled_traces(ctx):
     """512x512x320 grid (448x448x256 volume), 6 angles, local shifts and
     gains: sampled x columns of the GPU result against the oracle on those
