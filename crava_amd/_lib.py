@@ -136,6 +136,7 @@ SIGNATURES = {
     "crava_hip_fft_set_variants": (I, [VP, I, I, I, I, C.POINTER(I)]),
     "crava_hip_synthetic_seismic": (I, [VP, C.POINTER(SyntParams), VP, VP, VP, C.POINTER(VP)]),
     "crava_hip_grid_read_trace": (I, [VP, VP, I, I, I, I, I, I, I, VP]),
+    "crava_hip_complex_noise_reference": (I, [I, I, I, C.POINTER(C.c_uint32), VP]),
     "crava_hip_reflection_matrix": (I, [I, VP, VP, C.c_double, VP]),
     "crava_hip_divide_by_scale_wavelet": (I, [VP, C.POINTER(SwdivParams), C.POINTER(VP)]),
     "crava_hip_interpolate_seismic": (I, [VP, I, I, I, I, I, I, C.c_float, VP, C.POINTER(I), C.POINTER(I)]),

@@ -230,6 +230,15 @@ int crava_hip_perk_tables(int ntheta, int nz, int nzp, double lz, float low_cut,
   return CRAVA_HIP_OK;
 }
 
+int crava_hip_complex_noise_reference(int nxp, int nyp, int nzp, uint32_t *seed, float *host_out) {
+  if (nxp < 1 || nyp < 1 || nzp < 1 || !seed || !host_out) {
+    set_error("invalid arguments to crava_hip_complex_noise_reference");
+    return CRAVA_HIP_E_INVALID;
+  }
+  crava::reference_complex_noise(nxp, nyp, nzp, seed, host_out);
+  return CRAVA_HIP_OK;
+}
+
 int crava_hip_reflection_matrix(int ntheta, const float *theta, const int *seismic_type, double vsvp, float *A) {
   if (ntheta < 1 || !theta || !A) {
     set_error("invalid arguments to crava_hip_reflection_matrix");

@@ -2,6 +2,7 @@
 #include "tables.hpp"
 
 #include <cmath>
+#include <cstdint>
 #include <vector>
 
 namespace crava {
@@ -104,6 +105,95 @@ void perk_vectors(int ntheta, int nz, int nzp, double lz, float low_cut, float h
       e3[o + 1] = 0.0f;
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// The reference's sequential noise: RandomGen (libs/lib/random.cpp:38-119,
+// constants libs/lib/random.h:11-14) driving FFTGrid::fillInComplexNoise
+// (src/fftgrid.cpp:1214-1263).
+
+namespace {
+
+struct RefRandom {
+  uint32_t seed;
+  double unif01() {                        // 32-bit LCG, multiplier 69069, increment 1
+    seed = 69069u * seed + 1u;
+    return (double)seed * (1.0 / 4294967296.0);
+  }
+  static double g(double x) {              // majorising difference of the Marsaglia-Bray method
+    const double a = 17.49731196, b = 2.36785163, c = 2.15787544;
+    const double ax = std::fabs(x);
+    double r = a * std::exp(-x * x / 2.0);
+    if (ax < 1.0) r -= 2.0 * b * (3.0 - x * x) + c * (1.5 - ax);
+    else if (ax < 1.5) r -= b * (3.0 - ax) * (3.0 - ax) + c * (1.5 - ax);
+    else r -= b * (3.0 - ax) * (3.0 - ax);
+    return r;
+  }
+  double rnorm01() {                       // Marsaglia-Bray (Ripley, Stochastic Simulation, p. 84)
+    const double u = unif01();
+    if (u < 0.8638) {
+      const double u1 = unif01(), u2 = unif01(), u3 = unif01();
+      return 2.0 * (u1 + u2 + u3) - 3.0;
+    }
+    if (u < 0.9745) {
+      const double u1 = unif01(), u2 = unif01();
+      return 1.5 * (u1 + u2 - 1.0);
+    }
+    if (u < 0.9973002039) {
+      double x;
+      for (;;) {
+        const double u1 = unif01(), u2 = unif01();
+        x = 6.0 * u1 - 3.0;
+        if (!(0.358 * u2 > g(x))) return x;
+      }
+    }
+    for (;;) {
+      double v1, v2, w;
+      do {
+        const double u1 = unif01(), u2 = unif01();
+        v1 = 2.0 * u1 - 1.0;
+        v2 = 2.0 * u2 - 1.0;
+        w = v1 * v1 + v2 * v2;
+      } while (w >= 1.0);
+      const double c = std::sqrt((9.0 - 2.0 * std::log(w)) / w);
+      const double s = c * v1, t = c * v2;
+      if (!((std::fabs(s) <= 3.0) && (std::fabs(t) <= 3.0))) return std::fabs(s) > 3.0 ? s : t;
+    }
+  }
+};
+
+}  // namespace
+
+void reference_complex_noise(int nxp, int nyp, int nzp, uint32_t *seed, float *re_im) {
+  RefRandom rg{*seed};
+  const int cnxp = nxp / 2 + 1;
+  const long long csize = (long long)cnxp * nyp * nzp;
+  const float sd = float(1 / std::sqrt(2.0));
+  for (long long i = 0; i < csize; ++i) {
+    const int xs = (int)(i % cnxp);
+    // x planes 0 and (when the index is odd, as the reference tests it) cnxp - 1
+    if (xs == 0 || (xs == cnxp - 1 && (i % 2) == 1)) {
+      const long long jk = (i - xs) / cnxp;
+      const int j = (int)(jk % nyp), k = (int)(jk / nyp);
+      const int jc = j == 0 ? 0 : nyp - j, kc = k == 0 ? 0 : nzp - k;
+      const long long jkc = jc + (long long)kc * nyp;
+      if (jkc == jk) {                       // its own conjugate: real
+        re_im[2 * i] = float(rg.rnorm01());
+        re_im[2 * i + 1] = 0;
+      } else if (jkc > jk) {                 // first of a conjugate pair
+        re_im[2 * i] = float(sd * rg.rnorm01());
+        re_im[2 * i + 1] = float(sd * rg.rnorm01());
+      } else {                               // conjugate of an earlier point
+        const long long cc = jkc * cnxp + xs;
+        re_im[2 * i] = re_im[2 * cc];
+        re_im[2 * i + 1] = -re_im[2 * cc + 1];
+      }
+    } else {
+      re_im[2 * i] = float(sd * rg.rnorm01());
+      re_im[2 * i + 1] = float(sd * rg.rnorm01());
+    }
+  }
+  *seed = rg.seed;
 }
 
 }  // namespace crava

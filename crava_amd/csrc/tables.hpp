@@ -1,5 +1,6 @@
 // Host-side per-k preparation of the posterior loop (constant-thickness case).
 #pragma once
+#include <cstdint>
 
 namespace crava {
 
@@ -21,5 +22,10 @@ void perk_vectors(int ntheta, int nz, int nzp, double lz, float low_cut, float h
 // src/wavelet.cpp:246-261): double accumulation rounded to float, DC and
 // Nyquist bins exactly real.
 void wavelet_half_spectrum(const float *w, int nzp, float *re_im);
+
+// FFTGrid::fillInComplexNoise (src/fftgrid.cpp:1214-1263) with the
+// reference's RandomGen stream, on the host: nxp/2+1 x nyp x nzp {re, im}
+// pairs; *seed is the generator state, advanced as the reference's is.
+void reference_complex_noise(int nxp, int nyp, int nzp, uint32_t *seed, float *re_im);
 
 }  // namespace crava

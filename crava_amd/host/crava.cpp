@@ -139,6 +139,24 @@ long long Crava::simulate(SeismicParametersHolder &sph, unsigned long long seed,
   return nfail;
 }
 
+long long Crava::simulate(SeismicParametersHolder &sph, uint32_t &randomGenSeed, FFTGrid *real[3]) {
+  crava_hip_ctx *h = ctx_->handle();
+  void *seeds[3], *cov[6], *mean[3];
+  for (int c = 0; c < 3; ++c) {
+    real[c]->fillInComplexNoise(randomGenSeed);
+    seeds[c] = real[c]->devicePtr();
+    mean[c] = sph.mu(c)->getIsTransformed() ? nullptr : sph.mu(c)->devicePtr();
+  }
+  for (int q = 0; q < 6; ++q) {
+    if (!sph.cov(q)->getIsTransformed()) throw Error(CRAVA_HIP_E_STATE, "simulate needs transformed covariances");
+    cov[q] = sph.cov(q)->devicePtr();
+  }
+  long long nfail = 0;
+  check(crava_hip_simulate(h, s_.nxp, s_.nyp, s_.nzp, cov, seeds, mean, &nfail), "simulate");
+  for (int c = 0; c < 3; ++c) real[c]->setIsTransformed(false);
+  return nfail;
+}
+
 void Crava::computeSyntSeismic(FFTGrid *alpha, FFTGrid *beta, FFTGrid *rho, const std::vector<FFTGrid *> &syn) {
   const int nt = s_.ntheta;
   if ((int)syn.size() != nt) throw Error(CRAVA_HIP_E_INVALID, "computeSyntSeismic needs one grid per angle");

@@ -20,6 +20,7 @@
 #pragma once
 
 #include <complex>
+#include <cstdint>
 #include <functional>
 #include <stdexcept>
 #include <string>
@@ -86,6 +87,10 @@ class FFTGrid {
   void fillInConstant(float value);
   void fillInParamCorr(const std::vector<float> &corrXY, const float *circCorrT, float gradI, float gradJ);
   void fillInErrCorr(const std::vector<float> &corrXY, float gradI, float gradJ);
+  // src/fftgrid.cpp:1214-1263 with the reference's RandomGen stream (host,
+  // then uploaded); randomGenSeed is the generator state, advanced as the
+  // reference advances it.  The grid becomes a transformed PARAMETER grid.
+  void fillInComplexNoise(uint32_t &randomGenSeed);
 
   void multiplyByScalar(float scalar);
   void addScalar(float scalar);
@@ -308,6 +313,10 @@ class Crava {
   // :1416-1599 one realisation: noise (seed, stream) -> real grids real[3].
   long long simulate(SeismicParametersHolder &sph, unsigned long long seed, unsigned long long stream,
                      FFTGrid *real[3]);
+  // The same realisation with the reference's RandomGen stream for the three
+  // noise grids (seed0, seed1, seed2 in turn, :1466-1468): equal to the
+  // reference's realisation for the same generator state.
+  long long simulate(SeismicParametersHolder &sph, uint32_t &randomGenSeed, FFTGrid *real[3]);
   // :1667-1759 forward modelling: synthetic seismic of (alpha, beta, rho)
   // for every angle into syn[ntheta] (real grids).  Parameters held in the
   // Fourier domain are transformed back and forth, as the reference does.

@@ -203,6 +203,14 @@ float FFTGrid::getRealValue(int i, int j, int k, bool extSimbox) const {
   return host_[(size_t)i + (size_t)rnxp_ * ((size_t)j + (size_t)nyp_ * k)];
 }
 
+void FFTGrid::fillInComplexNoise(uint32_t &randomGenSeed) {
+  std::vector<float> v((size_t)csize_ * 2);
+  check(crava_hip_complex_noise_reference(nxp_, nyp_, nzp_, &randomGenSeed, v.data()), "fillInComplexNoise");
+  check(crava_hip_grid_upload(ctx_->handle(), dev_, v.data(), sizeof(float) * v.size()), "fillInComplexNoise");
+  istransformed_ = true;
+  cubetype_ = PARAMETER;
+}
+
 std::vector<float> FFTGrid::readTrace(int i, int j, int k0, int nk) const {
   if (istransformed_) throw Error(CRAVA_HIP_E_STATE, "readTrace needs a real grid");
   std::vector<float> v((size_t)(nk > 0 ? nk : 0));

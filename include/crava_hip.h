@@ -212,6 +212,16 @@ int crava_hip_nll_grad(crava_hip_post *post, int precision, void *const *mu, voi
 int crava_hip_fill_complex_noise(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, uint64_t seed,
                                  uint64_t stream);
 
+/* The reference's own noise, on the host: FFTGrid::fillInComplexNoise
+ * (src/fftgrid.cpp:1214-1263) driven by RandomGen (libs/lib/random.cpp:
+ * 38-119), the 32-bit LCG with Marsaglia-Bray normals, in the reference's
+ * point order.  host_out: (nxp/2+1) x nyp x nzp {re, im} float pairs (the
+ * half-complex grid, to upload with crava_hip_grid_upload); *seed is the
+ * generator state (RandomGen::seed_) and is advanced as the reference
+ * advances it, so consecutive calls continue its stream.  The stream is
+ * sequential; use it where realisations must equal the reference's. */
+int crava_hip_complex_noise_reference(int nxp, int nyp, int nzp, uint32_t *seed, float *host_out);
+
 /* One posterior realisation, in place of the per-realisation body of
  * Crava::simulate (src/crava.cpp:1482-1573):
  *   seed[3]  in: complex noise grids (float); out: real realisation grids

@@ -129,10 +129,15 @@ int main(int argc, char **argv) {
       real[c]->createComplexGrid();
     }
     const long long simfail = crava.simulate(sph, seed, 0, real);
+    for (int c = 0; c < 3; ++c) write_bin(dir + "/out_sim" + std::to_string(c) + ".bin", drain_real(real[c]));
+    // the same with the reference's RandomGen stream from state 23665
+    uint32_t rg = 23665;
+    crava.simulate(sph, rg, real);
     for (int c = 0; c < 3; ++c) {
-      write_bin(dir + "/out_sim" + std::to_string(c) + ".bin", drain_real(real[c]));
+      write_bin(dir + "/out_simref" + std::to_string(c) + ".bin", drain_real(real[c]));
       delete real[c];
     }
+    std::ofstream(dir + "/simref_seed.txt") << rg << "\n";
 
     // forward modelling of the posterior means (Crava::computeSyntSeismic),
     // with a stretch table on angle 0 and a local time shift on the last

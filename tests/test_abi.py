@@ -118,3 +118,25 @@ def test_preparation_entry_points_reject_null_arguments():
     lib = L.lib()
     assert lib.crava_hip_divide_by_scale_wavelet(None, None, None) == L.E_INVALID
     assert lib.crava_hip_interpolate_seismic(None, 9, 8, 20, 12, 11, 24, 0.1, None, None, None) == L.E_INVALID
+
+
+@pytest.mark.parametrize("n", [(10, 6, 8), (9, 7, 5), (64, 50, 40)])
+def test_reference_noise_stream_matches_the_oracle(n):
+    """crava_hip_complex_noise_reference reproduces the reference's RandomGen
+    stream in FFTGrid::fillInComplexNoise's order bit for bit (the oracle's
+    restatement, pinned by the LCG closed form and the normal moments), and
+    hands back the advanced generator state; two calls continue the stream."""
+    nxp, nyp, nzp = n
+    cn = nxp // 2 + 1
+    lib = L.lib()
+    for seed in (23665, 99, 4000000001):
+        s = C.c_uint32(seed)
+        out = np.empty((nzp, nyp, 2 * cn), np.float32)
+        assert lib.crava_hip_complex_noise_reference(nxp, nyp, nzp, C.byref(s), out.ctypes.data) == L.OK
+        want, s_after = oracle.fill_complex_noise(nxp, nyp, nzp, seed)
+        assert np.array_equal(out, want) and s.value == s_after
+        out2 = np.empty_like(out)
+        assert lib.crava_hip_complex_noise_reference(nxp, nyp, nzp, C.byref(s), out2.ctypes.data) == L.OK
+        want2, _ = oracle.fill_complex_noise(nxp, nyp, nzp, s_after)
+        assert np.array_equal(out2, want2)
+    assert lib.crava_hip_complex_noise_reference(0, 1, 1, None, None) == L.E_INVALID

@@ -264,6 +264,24 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
     assert np.array_equal(org, np.array([g[0, 0, 0] for g in covt], np.float32))
     pv = np.fromfile(os.path.join(d, "priorvar.bin"), dtype=np.float32).reshape(3, 3)
     assert np.array_equal(pv, org[[[0, 3, 4], [3, 1, 5], [4, 5, 2]]])
+    # a realisation with the reference's RandomGen stream equals the
+    # reference's: the oracle's noise from the same generator state, its
+    # per-point step, c2r and the posterior mean
+    st = 23665
+    ref_sim = []
+    covh = [_read(os.path.join(d, f"out_cov{q}.bin"), shape) for q in range(6)]
+    for i in range(3):
+        z, st = oracle.fill_complex_noise(nxp, nyp, nzp, st)
+        ref_sim.append(z)
+    oracle.simulate_points([g.view(np.complex64).reshape(-1).copy() for g in covh],
+                           [z.view(np.complex64).reshape(-1) for z in ref_sim])
+    assert int(open(os.path.join(d, "simref_seed.txt")).read()) == st
+    for i in range(3):
+        oracle.fft_c2r(ref_sim[i], nxp, nyp, nzp, False)
+        mui = _read(os.path.join(d, f"out_mu{i}.bin"), shape)
+        want = ref_sim[i][:, :, :nxp] + mui[:, :, :nxp]
+        got = _read(os.path.join(d, f"out_simref{i}.bin"), shape)[:, :, :nxp]
+        assert rel_l2(got, want) < 2e-6, i
     for i in range(3):
         sim = _read(os.path.join(d, f"out_sim{i}.bin"), shape)[:, :, :nxp]
         assert np.isfinite(sim).all()
