@@ -90,6 +90,19 @@ class SwdivParams(C.Structure):
     ]
 
 
+class LocalNoiseParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("nxp", C.c_int), ("nyp", C.c_int), ("nzp", C.c_int),
+        ("ntheta", C.c_int),
+        ("G", VP),
+        ("errThetaCov", VP),
+        ("priorVar0", VP),
+        ("postVar0", VP),
+        ("noiseScale", VP),
+    ]
+
+
 # name -> (restype, argtypes); every function declared in include/crava_hip.h
 SIGNATURES = {
     "crava_hip_version": (I, []),
@@ -140,6 +153,9 @@ SIGNATURES = {
     "crava_hip_reflection_matrix": (I, [I, VP, VP, C.c_double, VP]),
     "crava_hip_divide_by_scale_wavelet": (I, [VP, C.POINTER(SwdivParams), C.POINTER(VP)]),
     "crava_hip_interpolate_seismic": (I, [VP, I, I, I, I, I, I, C.c_float, VP, C.POINTER(I), C.POINTER(I)]),
+    "crava_hip_local_noise_g": (I, [I, VP, VP, VP, VP]),
+    "crava_hip_local_noise_correct": (I, [VP, C.POINTER(LocalNoiseParams), C.POINTER(VP), C.POINTER(VP), VP]),
+    "crava_hip_local_noise_realisation": (I, [VP, I, I, I, I, I, I, VP, C.POINTER(VP)]),
     "crava_hip_merge_mean_4d": (I, [VP, I, I, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP)]),
     "crava_hip_merge_cov_4d": (I, [VP, I, I, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(VP)]),
 }

@@ -40,6 +40,7 @@ SOURCES = {
     "grid.hip": ["-ffp-contract=off"],
     "forward.hip": ["-ffp-contract=off"],
     "swdiv.hip": ["-ffp-contract=off"],
+    "localnoise.hip": ["-ffp-contract=off"],
 }
 
 COMMON = [

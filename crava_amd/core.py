@@ -331,6 +331,47 @@ def interpolate_seismic(ctx: Context, nx, ny, nz, nxp, nyp, nzp, threshold, grid
     return nf.value, n0.value
 
 
+def local_noise_g(err_theta_cov, prior_var0, post_var0):
+    """Crava::computeG on the host (crava_hip_local_noise_g): G [ntheta][3],
+    eigenvectors signed with their largest component positive."""
+    import numpy as np
+    e = np.ascontiguousarray(err_theta_cov, dtype=np.float64)
+    sm = np.ascontiguousarray(prior_var0, dtype=np.float64)
+    smd = np.ascontiguousarray(post_var0, dtype=np.float64)
+    g = np.zeros((e.shape[0], 3))
+    check(lib().crava_hip_local_noise_g(e.shape[0], e.ctypes.data, sm.ctypes.data, smd.ctypes.data, g.ctypes.data),
+          "crava_hip_local_noise_g")
+    return g
+
+
+def local_noise_correct(ctx: Context, nx, ny, nz, nxp, nyp, nzp, G, err_theta_cov, prior_var0, post_var0,
+                        noise_scales, post, mean2, trace_cov=None):
+    """Crava::correctAlphaBetaRho's trace loop (crava_hip_local_noise_correct)
+    in place on the posterior float grids post[3]; noise_scales: ntheta host
+    arrays [ny][nx]; trace_cov: optional device float64 tensor of ny*nx*9
+    that receives the realisation factors."""
+    import numpy as np
+    G = np.ascontiguousarray(G, dtype=np.float64)
+    nt = G.shape[0]
+    keep = [G, np.ascontiguousarray(err_theta_cov, dtype=np.float64),
+            np.ascontiguousarray(prior_var0, dtype=np.float64), np.ascontiguousarray(post_var0, dtype=np.float64)]
+    tabs = [np.ascontiguousarray(t, dtype=np.float64) for t in noise_scales]
+    arr = (C.c_void_p * nt)(*[t.ctypes.data for t in tabs])
+    p = L.LocalNoiseParams(nx, ny, nz, nxp, nyp, nzp, nt, *[k.ctypes.data for k in keep], C.cast(arr, C.c_void_p))
+    check(lib().crava_hip_local_noise_correct(ctx.h, C.byref(p), ptr_array([_dev_ptr(g) for g in post]),
+                                              ptr_array([_dev_ptr(g) for g in mean2]),
+                                              None if trace_cov is None else _dev_ptr(trace_cov)),
+          "crava_hip_local_noise_correct")
+
+
+def local_noise_realisation(ctx: Context, nx, ny, nz, nxp, nyp, nzp, trace_cov, seed):
+    """The local-noise block of Crava::simulate
+    (crava_hip_local_noise_realisation) in place on the float grids seed[3]."""
+    check(lib().crava_hip_local_noise_realisation(ctx.h, nx, ny, nz, nxp, nyp, nzp, _dev_ptr(trace_cov),
+                                                  ptr_array([_dev_ptr(g) for g in seed])),
+          "crava_hip_local_noise_realisation")
+
+
 def merge_mean_4d(ctx: Context, nxp, nyp, nzp, mu_static, mu_dynamic, mu):
     """crava_hip_merge_mean_4d on transformed float grids (lists of 3)."""
     P = lambda gs: ptr_array([_dev_ptr(g) for g in gs])

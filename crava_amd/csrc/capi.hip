@@ -11,6 +11,7 @@
 #include "posterior.hpp"
 #include "simulate.hpp"
 #include "swdiv.hpp"
+#include "localnoise.hpp"
 #include "tables.hpp"
 
 #include <cmath>
@@ -803,6 +804,96 @@ int crava_hip_interpolate_seismic(crava_hip_ctx *ctx, int nx, int ny, int nz, in
   // the schedule's host copy goes out of scope
   CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return CRAVA_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Local-noise correction
+int crava_hip_local_noise_g(int ntheta, const double *errThetaCov, const double *priorVar0, const double *postVar0,
+                            double *G) {
+  if (!errThetaCov || !priorVar0 || !postVar0 || !G) { set_error("NULL argument to crava_hip_local_noise_g"); return CRAVA_HIP_E_INVALID; }
+  if (ntheta < 1 || ntheta > crava::kLncMaxTheta) { set_error("ntheta must be in 1..8"); return CRAVA_HIP_E_INVALID; }
+  const int rc = crava::lnc_compute_g(ntheta, errThetaCov, priorVar0, postVar0, G);
+  if (rc) set_error("crava_hip_local_noise_g failed");
+  return rc;
+}
+
+int crava_hip_local_noise_correct(crava_hip_ctx *ctx, const crava_hip_local_noise *p, void *const *post,
+                                  const void *const *mean2, double *traceCov) {
+  if (!ctx || !p || !post || !mean2 || !p->G || !p->errThetaCov || !p->priorVar0 || !p->postVar0 || !p->noiseScale) {
+    set_error("NULL argument to crava_hip_local_noise_correct");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (p->ntheta < 1 || p->ntheta > crava::kLncMaxTheta) { set_error("ntheta must be in 1..8"); return CRAVA_HIP_E_INVALID; }
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1 || p->nx > p->nxp || p->ny > p->nyp || p->nz > p->nzp) {
+    set_error("invalid volume / padded grid sizes");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if ((long long)p->nx * p->ny > (1LL << 28)) { set_error("too many traces"); return CRAVA_HIP_E_INVALID; }
+  for (int c = 0; c < 3; ++c)
+    if (!post[c] || !mean2[c]) { set_error("NULL grid"); return CRAVA_HIP_E_INVALID; }
+  const int nt = p->ntheta;
+  for (int a = 0; a < nt; ++a)
+    if (!p->noiseScale[a]) { set_error("NULL noise-scale table"); return CRAVA_HIP_E_INVALID; }
+  crava::LncConst c{};
+  for (int r = 0; r < nt; ++r) {
+    for (int q = 0; q < 3; ++q) c.g[r][q] = p->G[r * 3 + q];
+    for (int q = 0; q < nt; ++q) c.e[r][q] = p->errThetaCov[r * nt + q];
+  }
+  double sd[3][3], smd[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q) {
+      c.sm[r][q] = p->priorVar0[r * 3 + q];
+      smd[r][q] = p->postVar0[r * 3 + q];
+      sd[r][q] = c.sm[r][q] - smd[r][q];
+    }
+  crava::lnc_inv_sqrt_host(sd, 0.0000001, c.sdi);  // src/crava.cpp:2294-2307
+  crava::lnc_inv_sqrt_host(smd, 0.0, c.old);       // :2376-2385
+  const size_t ntr = (size_t)p->nx * p->ny;
+  // Grid2D::FindMin(RMISSING), libs/nrlib/grid/grid2d.hpp:182-191
+  const double missing = -99999.000;
+  for (int a = 0; a < nt; ++a) {
+    const double *tab = p->noiseScale[a];
+    double mn = tab[0];
+    for (size_t t = 0; t < ntr; ++t)
+      if ((mn == missing || tab[t] < mn) && tab[t] != missing) mn = tab[t];
+    c.min_scale[a] = mn;
+  }
+  void *ws = nullptr;
+  int rc;
+  const size_t b_scale = sizeof(double) * nt * ntr, b_sn = sizeof(double) * 9 * ntr;
+  if ((rc = ctx->workspace(b_scale + b_sn, &ws))) return rc;
+  double *dscale = static_cast<double *>(ws);
+  double *dsn = reinterpret_cast<double *>(static_cast<char *>(ws) + b_scale);
+  for (int a = 0; a < nt; ++a)
+    CRAVA_HIP_CHECK(hipMemcpyAsync(dscale + (size_t)a * ntr, p->noiseScale[a], sizeof(double) * ntr,
+                                   hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = crava::launch_lnc_trace(c, nt, dscale, p->nx, p->ny, dsn, traceCov, ctx->stream))) return rc;
+  const int rnxp = 2 * (p->nxp / 2 + 1);
+  float *pg[3];
+  const float *mg[3];
+  for (int q = 0; q < 3; ++q) {
+    pg[q] = static_cast<float *>(post[q]);
+    mg[q] = static_cast<const float *>(mean2[q]);
+  }
+  if ((rc = crava::launch_lnc_apply(pg, mg, dsn, p->nx, p->ny, p->nz, p->nyp, rnxp, ctx->stream))) return rc;
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_local_noise_realisation(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp,
+                                      const double *traceCov, void *const *seed) {
+  if (!ctx || !traceCov || !seed || !seed[0] || !seed[1] || !seed[2]) {
+    set_error("NULL argument to crava_hip_local_noise_realisation");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (nx < 1 || ny < 1 || nz < 1 || nx > nxp || ny > nyp || nz > nzp) {
+    set_error("invalid volume / padded grid sizes");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if ((long long)nx * ny > (1LL << 28)) { set_error("too many traces"); return CRAVA_HIP_E_INVALID; }
+  float *g[3];
+  for (int q = 0; q < 3; ++q) g[q] = static_cast<float *>(seed[q]);
+  return crava::launch_lnc_realisation(g, traceCov, nx, ny, nz, nyp, 2 * (nxp / 2 + 1), ctx->stream);
 }
 
 }  // extern "C"

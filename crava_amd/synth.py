@@ -421,3 +421,28 @@ def isq_grid(nx, ny, nz, nxp, nyp, nzp, seed, n_bad, n_weak=0, cluster=False):
     for t in idx[n_bad:n_bad + n_weak]:
         g[:, t // nx, t % nx] *= np.float32(1e-3)
     return g
+
+
+def local_noise_problem(ntheta: int, nx: int, ny: int, seed: int = 0, uniform: float | None = None) -> dict:
+    """A local-noise correction problem: angle error covariance E
+    (variances ~1e-4, correlation exp(-3|dtheta|/30 deg)), prior variance Sm,
+    its Bayesian update Smd = (Sm^-1 + K^T E^-1 K)^-1 for an Aki-Richards
+    like K, and per-angle noise-scale tables [ny][nx] in [1, 3] (all equal
+    to `uniform` when given)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    th = np.linspace(5.0, 40.0, ntheta)
+    ev = 1e-4 * (1.0 + rng.random(ntheta))
+    rho = np.exp(-3.0 * np.abs(th[:, None] - th[None, :]) / 30.0)
+    E = np.sqrt(ev)[:, None] * np.sqrt(ev)[None, :] * rho
+    B = rng.normal(size=(3, 3))
+    Sm = 1e-3 * (B @ B.T + 0.5 * np.eye(3))
+    t = np.radians(th)
+    K = np.stack([0.5 / np.cos(t) ** 2, -4.0 * 0.25 * np.sin(t) ** 2, 0.5 - 2.0 * 0.25 * np.sin(t) ** 2], axis=1)
+    Smd = np.linalg.inv(np.linalg.inv(Sm) + K.T @ np.linalg.inv(E) @ K)
+    Smd = 0.5 * (Smd + Smd.T)
+    if uniform is None:
+        scales = [1.0 + 2.0 * rng.random((ny, nx)) for _ in range(ntheta)]
+    else:
+        scales = [np.full((ny, nx), float(uniform)) for _ in range(ntheta)]
+    return dict(E=E, Sm=Sm, Smd=Smd, scales=scales)

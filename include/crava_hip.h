@@ -463,6 +463,54 @@ int crava_hip_divide_by_scale_wavelet(crava_hip_ctx *ctx, const crava_hip_swdiv_
 int crava_hip_interpolate_seismic(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp,
                                   float energy_threshold, void *grid, int *n_flagged, int *n_zero);
 
+/* ---- local-noise correction (src/crava.cpp:2130-2470, :1547-1566) ---- */
+
+/* Inputs of Crava::correctAlphaBetaRho.  All matrices are host, row-major,
+ * double. */
+typedef struct crava_hip_local_noise {
+  int nx, ny, nz;              /* volume */
+  int nxp, nyp, nzp;           /* padded grid */
+  int ntheta;                  /* 1..8 */
+  const double *G;             /* [ntheta][3], Crava::computeG */
+  const double *errThetaCov;   /* [ntheta][ntheta], errThetaCov_ */
+  const double *priorVar0;     /* [3][3], priorVar0_ */
+  const double *postVar0;      /* [3][3], postVar0_ */
+  /* ntheta host tables [ny][nx]: getLocalNoiseScale(angle)(i, j) at j*nx + i */
+  const double *const *noiseScale;
+} crava_hip_local_noise;
+
+/* Crava::computeG (src/crava.cpp:2130-2211) on the host.  The reference
+ * takes the eigenvectors of Sm^(-1/2) (Sm - Smd)+ Sm^(-1/2) from LAPACK's
+ * dgeev, whose signs G inherits; here each eigenvector is signed so that its
+ * component of largest magnitude is positive.  The correction does not
+ * depend on those signs when every angle's noise scale is uniform; with
+ * varying scales, a caller that needs the reference's result passes the G
+ * of its own computeG to crava_hip_local_noise_correct.  Host only; no
+ * context needed. */
+int crava_hip_local_noise_g(int ntheta, const double *errThetaCov, const double *priorVar0, const double *postVar0,
+                            double *G);
+
+/* Crava::correctAlphaBetaRho's trace loop (src/crava.cpp:2340-2470):
+ * post[3] (alpha, beta, rho: real padded float grids in the time domain)
+ * are corrected in place towards mean2[3] (the prior means) by each
+ * trace's sigmanew of newPosteriorCovPointwise (:2213-2312).  The per-trace
+ * matrices are computed on the device (matrix functions by Jacobi
+ * rotations; their sums differ from BLAS's in rounding, so the corrected
+ * values agree to float rounding rather than bit for bit).  traceCov
+ * (device, [ny][nx][9] row-major, optional) receives (*sigmamdnew_)(i,j),
+ * the realisation factor that crava_hip_local_noise_realisation applies.
+ * Synchronises the context's stream. */
+int crava_hip_local_noise_correct(crava_hip_ctx *ctx, const crava_hip_local_noise *p, void *const *post,
+                                  const void *const *mean2, double *traceCov);
+
+/* The local-noise block of Crava::simulate (src/crava.cpp:1547-1566): seed[3]
+ * (real padded float grids, the inverse-transformed draws, before the
+ * posterior means are added) become traceCov(i,j) times (seed0, seed1,
+ * seed2) at every volume point, in place.  Bit-exact for a given traceCov.
+ * Asynchronous on the context's stream. */
+int crava_hip_local_noise_realisation(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp,
+                                      const double *traceCov, void *const *seed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,7 +96,8 @@ def test_host_mirror_links_only_the_c_abi():
 @pytest.mark.parametrize("cname,pyname", [("crava_hip_post_params", "PostParams"),
                                           ("crava_hip_batch_volume", "BatchVolume"),
                                           ("crava_hip_synt_params", "SyntParams"),
-                                          ("crava_hip_swdiv_params", "SwdivParams")])
+                                          ("crava_hip_swdiv_params", "SwdivParams"),
+                                          ("crava_hip_local_noise", "LocalNoiseParams")])
 def test_struct_layouts_match_the_header(cname, pyname, tmp_path):
     """Every ctypes mirror of a header struct has the header's size and field
     offsets (gcc on the header)."""

@@ -1,0 +1,146 @@
+"""The local-noise oracle (oracle/local_noise.py) pinned by what the
+mathematics fixes, and the host computeG helper of the C-ABI checked against
+it.
+
+* computeG builds G so that the original posterior is reproduced:
+  G^T (G Sm G^T + E)^-1 G = Sm^-1 (Sm - Smd) Sm^-1 (ntheta >= 3).
+* Unit noise scales leave the posterior unchanged (sigmanew = I).
+* A uniform scale s has the closed form
+  Sm - Smd_new = Sm^1/2 V diag(l / (l + s (1 - l))) V^T Sm^1/2,
+  V diag(l) V^T = Sm^-1/2 (Sm - Smd) Sm^-1/2.
+* Any scales: Smd_new = (Sm^-1 + G^T (D E D)^-1 G)^-1 (information form).
+* The per-point loops: permutation matrices on small integers (exact).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from crava_amd import synth
+from oracle import local_noise as ln
+
+
+def _prob(nt, **kw):
+    return synth.local_noise_problem(nt, 4, 3, seed=nt, **kw)
+
+
+@pytest.mark.parametrize("nt", [3, 4, 6])
+def test_g_reproduces_the_original_posterior(nt):
+    p = _prob(nt)
+    G = ln.compute_g(p["E"], p["Sm"], p["Smd"])
+    assert G.shape == (nt, 3)
+    lhs = G.T @ np.linalg.inv(G @ p["Sm"] @ G.T + p["E"]) @ G
+    ism = np.linalg.inv(p["Sm"])
+    rhs = ism @ (p["Sm"] - p["Smd"]) @ ism
+    assert np.allclose(lhs, rhs, rtol=1e-8, atol=1e-8 * np.abs(rhs).max())
+
+
+@pytest.mark.parametrize("nt", [3, 5])
+def test_unit_scales_leave_the_posterior_unchanged(nt):
+    p = _prob(nt)
+    G = ln.compute_g(p["E"], p["Sm"], p["Smd"])
+    sn, smdn = ln.new_posterior_cov_pointwise(G, np.ones(nt), p["E"], p["Sm"], p["Smd"])
+    assert np.allclose(sn, np.eye(3), atol=1e-8)
+    assert np.allclose(smdn @ smdn, p["Smd"], rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("s", [0.5, 2.0, 7.0])
+def test_uniform_scale_matches_the_closed_form(s):
+    nt = 4
+    p = _prob(nt)
+    G = ln.compute_g(p["E"], p["Sm"], p["Smd"])
+    sn, smdn = ln.new_posterior_cov_pointwise(G, np.full(nt, s), p["E"], p["Sm"], p["Smd"])
+    half = np.real(sl.sqrtm(p["Sm"]))
+    ihalf = np.linalg.inv(half)
+    lam, V = np.linalg.eigh(ihalf @ (p["Sm"] - p["Smd"]) @ ihalf)
+    dn = half @ V @ np.diag(lam / (lam + s * (1.0 - lam))) @ V.T @ half
+    assert np.allclose(smdn @ smdn, p["Sm"] - dn, rtol=1e-8, atol=1e-12)
+    sq = np.real(sl.sqrtm(p["Sm"] - p["Smd"]))
+    x = sn @ sq
+    assert np.allclose(x @ x, dn, rtol=1e-8, atol=1e-12)
+
+
+@pytest.mark.parametrize("nt", [3, 6])
+def test_varying_scales_match_the_information_form(nt):
+    p = _prob(nt)
+    G = ln.compute_g(p["E"], p["Sm"], p["Smd"])
+    rng = np.random.default_rng(7)
+    for _ in range(5):
+        sc = 1.0 + 3.0 * rng.random(nt)
+        sn, smdn = ln.new_posterior_cov_pointwise(G, sc, p["E"], p["Sm"], p["Smd"])
+        D = np.diag(np.sqrt(sc))
+        post = np.linalg.inv(np.linalg.inv(p["Sm"]) + G.T @ np.linalg.inv(D @ p["E"] @ D) @ G)
+        assert np.allclose(smdn @ smdn, post, rtol=1e-8, atol=1e-12)
+        sq = np.real(sl.sqrtm(p["Sm"] - p["Smd"]))
+        x = sn @ sq
+        assert np.allclose(x @ x, p["Sm"] - post, rtol=1e-7, atol=1e-12)
+
+
+def test_sort3x3_is_a_stable_descending_sort():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        ev = rng.integers(-3, 3, size=3).astype(np.float64)
+        V = rng.normal(size=(3, 3))
+        h1, h2 = ln.sort3x3(ev, V)
+        idx = np.argsort(-ev, kind="stable")
+        assert np.array_equal(h1, ev[idx])
+        assert np.array_equal(h2, V[:, idx])
+
+
+def test_eigen_vectors_diagonalise():
+    rng = np.random.default_rng(5)
+    B = rng.normal(size=(4, 4))
+    M = B @ B.T
+    w, V = ln.eigen_vectors(M)
+    assert np.allclose(M @ V, V * w, atol=1e-12)
+    assert np.allclose(np.linalg.norm(V, axis=0), 1.0)
+
+
+def test_min_scale_skips_missing():
+    t = np.array([[ln.RMISSING, 3.0], [2.0, ln.RMISSING]])
+    assert ln.min_scale(t) == 2.0
+    assert ln.min_scale(np.full((2, 2), ln.RMISSING)) == ln.RMISSING
+
+
+def test_per_point_loops_on_permutations():
+    nz, nyp, rnxp = 5, 3, 6
+    rng = np.random.default_rng(2)
+    post = [rng.integers(-50, 50, size=(nz, nyp, rnxp)).astype(np.float32) for _ in range(3)]
+    mean = [rng.integers(-50, 50, size=(nz, nyp, rnxp)).astype(np.float32) for _ in range(3)]
+    P = np.array([[0, 1, 0], [0, 0, 2], [1, 0, 0]], dtype=np.float64)
+    sn = np.broadcast_to(P, (2, 4, 3, 3)).copy()
+    want = [mean[r] + sum(P[r, c] * (post[c] - mean[c]) for c in range(3)) for r in range(3)]
+    got = [g.copy() for g in post]
+    ln.apply_correction(got, mean, sn, 4, 2, nz)
+    for r in range(3):
+        assert np.array_equal(got[r][:nz, :2, :4], want[r][:nz, :2, :4])
+        assert np.array_equal(got[r][:, 2:, :], post[r][:, 2:, :])
+        assert np.array_equal(got[r][:, :, 4:], post[r][:, :, 4:])
+    seed = [g.copy() for g in post]
+    ln.transform_realisation(seed, sn, 4, 2, nz)
+    for r in range(3):
+        w = sum(P[r, c] * post[c] for c in range(3))
+        assert np.array_equal(seed[r][:nz, :2, :4], w[:nz, :2, :4])
+
+
+@pytest.mark.parametrize("nt", [1, 2, 3, 6, 8])
+def test_host_g_helper_matches_the_oracle_under_its_sign_rule(nt):
+    from crava_amd import core
+    p = _prob(nt)
+    G = core.local_noise_g(p["E"], p["Sm"], p["Smd"])
+    ref = ln.compute_g(p["E"], p["Sm"], p["Smd"], resign=True)
+    assert np.allclose(G, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+    # G Sm G^T does not depend on the signs
+    plain = ln.compute_g(p["E"], p["Sm"], p["Smd"])
+    assert np.allclose(G @ p["Sm"] @ G.T, plain @ p["Sm"] @ plain.T, rtol=1e-9,
+                       atol=1e-12 * np.abs(plain).max() ** 2)
+
+
+def test_host_g_helper_rejects_bad_arguments():
+    from crava_amd import _lib as L
+    e = np.eye(9)
+    sm = np.eye(3)
+    g = np.zeros((9, 3))
+    assert L.lib().crava_hip_local_noise_g(9, e.ctypes.data, sm.ctypes.data, sm.ctypes.data, g.ctypes.data) != 0
+    assert L.lib().crava_hip_local_noise_g(3, None, sm.ctypes.data, sm.ctypes.data, g.ctypes.data) != 0
