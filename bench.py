@@ -106,7 +106,7 @@ def parse():
                     "workload (0: skip)")
     ap.add_argument("--diff-steps", type=int, default=50, help="L-BFGS iterations of the differentiation workload "
                     "(0: skip it)")
-    ap.add_argument("--only", choices=["nonconstant_thickness", "time_lapse_merge"],
+    ap.add_argument("--only", choices=["nonconstant_thickness", "time_lapse_merge", "local_noise"],
                     help="run one secondary workload alone (for profiling) and print its result")
     return ap.parse_args()
 
@@ -577,6 +577,79 @@ def nonconstant_thickness(ctx, reps: int, cpu: bool):
     return res
 
 
+def local_noise(ctx, reps: int, cpu: bool):
+    """Local-noise correction on the posterior workload's grid (448x448x256
+    volume in 512x512x320, 6 angles, noise scales U(1, 3) per trace and
+    angle): crava_hip_local_noise_correct (Crava::correctAlphaBetaRho: the
+    per-trace matrices of newPosteriorCovPointwise and the correction of the
+    three posterior grids, scale-table upload included) and
+    crava_hip_local_noise_realisation (the local-noise block of
+    Crava::simulate).  HIP events on the launch stream.  Algorithmic bytes of
+    the correction's grid pass: 6 float reads and 3 float writes per volume
+    point; of the realisation: 3 reads and 3 writes.  The oracle's numpy
+    restatement (a port: Crava does not build here) is timed on one core on
+    the first traces and scaled."""
+    import torch
+    from crava_amd import synth
+    from crava_amd.core import local_noise_correct, local_noise_g, local_noise_realisation
+    n = (448, 448, 256, 512, 512, 320)
+    nx, ny, nz, nxp, nyp, nzp = n
+    nt = 6
+    prob = synth.local_noise_problem(nt, nx, ny, seed=3)
+    G = local_noise_g(prob["E"], prob["Sm"], prob["Smd"])
+    shape = (nzp, nyp, 2 * (nxp // 2 + 1))
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    mean = [8.0 + 0.05 * torch.randn(shape, device="cuda", generator=gen) for _ in range(3)]
+    src = [m + 0.03 * torch.randn(shape, device="cuda", generator=gen) for m in mean]
+    post = [g.clone() for g in src]
+    tc = torch.empty(nx * ny * 9, dtype=torch.float64, device="cuda")
+
+    def restore():
+        for w, g in zip(post, src):
+            w.copy_(g)
+
+    ms = timed_kernel(lambda: local_noise_correct(ctx, *n, G, prob["E"], prob["Sm"], prob["Smd"], prob["scales"],
+                                                  post, mean, tc), reps, setup=restore)
+    pts = nx * ny * nz
+    alg = 36 * pts
+    ms_r = timed_kernel(lambda: local_noise_realisation(ctx, *n, tc, post), reps)
+    alg_r = 24 * pts
+    res = {"volume": [nx, ny, nz], "grid": [nxp, nyp, nzp], "ntheta": nt,
+           "correction": {"ms_per_call": ms, "traces_per_s": nx * ny / (ms * 1e-3), "grid_alg_bytes": alg,
+                          "grid_alg_GBps_over_call": alg / (ms * 1e-3) / 1e9},
+           "realisation": {"ms_per_call": ms_r, "alg_bytes": alg_r, "alg_GBps": alg_r / (ms_r * 1e-3) / 1e9,
+                           "frac_of_peak": alg_r / (ms_r * 1e-3) / 1e9 / PEAK_HBM_GBS},
+           "cpu_port": None}
+    log(f"local-noise correction: {ms:.2f} ms per call; realisation transform {ms_r:.2f} ms "
+        f"({res['realisation']['alg_GBps']:.0f} GB/s)")
+    del post, src, mean
+    if cpu:
+        try:
+            import numpy as np
+            from oracle import local_noise as ln
+            ntr = 64
+            sub = [t[:1, :ntr].copy() for t in prob["scales"]]
+            sub = [np.concatenate([s, np.full((1, 1), t.min())], axis=1) for s, t in zip(sub, prob["scales"])]
+            t1 = time.perf_counter()
+            sn, tcs = ln.trace_matrices(G, sub, prob["E"], prob["Sm"], prob["Smd"])
+            t_tr = (time.perf_counter() - t1) / (ntr + 1)
+            pg = [np.full((nz, 1, ntr + 2), 8.0, np.float32) for _ in range(3)]
+            mg = [np.full((nz, 1, ntr + 2), 7.9, np.float32) for _ in range(3)]
+            t1 = time.perf_counter()
+            ln.apply_correction(pg, mg, sn, ntr + 1, 1, nz)
+            t_ap = (time.perf_counter() - t1) / (ntr + 1)
+            total = (t_tr + t_ap) * nx * ny
+            res["cpu_port"] = {"seconds_per_call": total, "cores": 1, "kind": "port",
+                               "sample": f"{ntr + 1} traces of one row (matrices {t_tr * 1e3:.2f} ms, correction "
+                                         f"{t_ap * 1e3:.2f} ms per trace), scaled to {nx * ny} traces",
+                               "gpu_speedup": total / (ms * 1e-3)}
+            log(f"local-noise correction, oracle port on 1 core: {total:.1f} s per call")
+        except Exception as e:  # the baseline is reported, never the target
+            log(f"local-noise cpu port failed: {e}")
+    torch.cuda.empty_cache()
+    return res
+
+
 def time_lapse_merge(ctx, reps: int, cpu: bool):
     """Time-lapse merge (crava_hip_merge_cov_4d / _mean_4d, State4D::mergeCov /
     merge) on 512x512x320 transformed grids: 6 + 6 + 9 covariance parts into
@@ -640,7 +713,8 @@ def main():
         ctx = Context()
         reps = max(args.forward_reps, 1)
         fn = {"nonconstant_thickness": lambda: nonconstant_thickness(ctx, reps, not args.no_cpu),
-              "time_lapse_merge": lambda: time_lapse_merge(ctx, reps, not args.no_cpu)}[args.only]
+              "time_lapse_merge": lambda: time_lapse_merge(ctx, reps, not args.no_cpu),
+              "local_noise": lambda: local_noise(ctx, reps, not args.no_cpu)}[args.only]
         print(json.dumps({"workload": args.only, "result": fn()}))
         return
     cfg = synth.config(args.config)
@@ -827,6 +901,10 @@ def main():
     if args.forward_reps > 0:
         torch.cuda.empty_cache()
         nct = nonconstant_thickness(ctx, args.forward_reps, not args.no_cpu)
+    lnc = None
+    if args.forward_reps > 0:
+        torch.cuda.empty_cache()
+        lnc = local_noise(ctx, args.forward_reps, not args.no_cpu)
     cpu = None
     if not args.no_cpu:
         try:
@@ -874,6 +952,7 @@ def main():
         "differentiation": diff,
         "forward_modelling": fwd,
         "nonconstant_thickness": nct,
+        "local_noise": lnc,
         "time_lapse_merge": tl,
     }
     print(json.dumps(out))

@@ -18,7 +18,9 @@ Crava::Crava(Context *ctx, const Settings &s, std::vector<FFTGrid *> seisData)
   errThetaCov_.assign((size_t)nt * nt, 0.0);
 }
 
-Crava::~Crava() = default;
+Crava::~Crava() {
+  if (traceCov_) (void)crava_hip_grid_free(traceCov_);
+}
 
 void Crava::computeDataVariance() {
   for (int l = 0; l < s_.ntheta; ++l) {
@@ -122,12 +124,28 @@ long long Crava::computePostMeanResidAndFFTCov(SeismicParametersHolder &sph) {
 
 long long Crava::simulate(SeismicParametersHolder &sph, unsigned long long seed, unsigned long long stream,
                           FFTGrid *real[3]) {
+  void *seeds[3];
+  for (int c = 0; c < 3; ++c) {
+    seeds[c] = real[c]->devicePtr();
+    check(crava_hip_fill_complex_noise(ctx_->handle(), seeds[c], s_.nxp, s_.nyp, s_.nzp, seed, stream * 3 + c),
+          "noise");
+  }
+  return realise(sph, real);
+}
+
+long long Crava::simulate(SeismicParametersHolder &sph, uint32_t &randomGenSeed, FFTGrid *real[3]) {
+  for (int c = 0; c < 3; ++c) real[c]->fillInComplexNoise(randomGenSeed);
+  return realise(sph, real);
+}
+
+// :1482-1573: the per-point step, the inverse transforms, the local-noise
+// factor when correctAlphaBetaRho has run (:1547-1566), then the means.
+long long Crava::realise(SeismicParametersHolder &sph, FFTGrid *real[3]) {
   crava_hip_ctx *h = ctx_->handle();
   void *seeds[3], *cov[6], *mean[3];
   for (int c = 0; c < 3; ++c) {
     seeds[c] = real[c]->devicePtr();
-    check(crava_hip_fill_complex_noise(h, seeds[c], s_.nxp, s_.nyp, s_.nzp, seed, stream * 3 + c), "noise");
-    mean[c] = sph.mu(c)->getIsTransformed() ? nullptr : sph.mu(c)->devicePtr();
+    mean[c] = (traceCov_ || sph.mu(c)->getIsTransformed()) ? nullptr : sph.mu(c)->devicePtr();
   }
   for (int q = 0; q < 6; ++q) {
     if (!sph.cov(q)->getIsTransformed()) throw Error(CRAVA_HIP_E_STATE, "simulate needs transformed covariances");
@@ -135,26 +153,68 @@ long long Crava::simulate(SeismicParametersHolder &sph, unsigned long long seed,
   }
   long long nfail = 0;
   check(crava_hip_simulate(h, s_.nxp, s_.nyp, s_.nzp, cov, seeds, mean, &nfail), "simulate");
+  if (traceCov_) {
+    check(crava_hip_local_noise_realisation(h, s_.nx, s_.ny, s_.nz, s_.nxp, s_.nyp, s_.nzp,
+                                            static_cast<const double *>(traceCov_), seeds),
+          "local noise realisation");
+    for (int c = 0; c < 3; ++c)
+      if (!sph.mu(c)->getIsTransformed())
+        check(crava_hip_grid_op(h, CRAVA_HIP_OP_ADD, seeds[c], sph.mu(c)->devicePtr(), s_.nxp, s_.nyp, s_.nzp, 0,
+                                0.0f),
+              "add mean");
+  }
   for (int c = 0; c < 3; ++c) real[c]->setIsTransformed(false);
   return nfail;
 }
 
-long long Crava::simulate(SeismicParametersHolder &sph, uint32_t &randomGenSeed, FFTGrid *real[3]) {
-  crava_hip_ctx *h = ctx_->handle();
-  void *seeds[3], *cov[6], *mean[3];
+void Crava::correctAlphaBetaRho(const double priorVar0[3][3], const double postVar0[3][3], FFTGrid *post[3],
+                                FFTGrid *mean2[3], const double *G) {
+  const int nt = s_.ntheta;
+  if ((int)s_.localNoiseScale.size() != nt) throw Error(CRAVA_HIP_E_INVALID, "one noise-scale table per angle");
+  for (const auto &t : s_.localNoiseScale)
+    if (t.size() != (size_t)s_.nx * s_.ny) throw Error(CRAVA_HIP_E_INVALID, "noise-scale tables are [ny*nx]");
+  for (int c = 0; c < 3; ++c)
+    if (post[c]->getIsTransformed() || mean2[c]->getIsTransformed())
+      throw Error(CRAVA_HIP_E_STATE, "correctAlphaBetaRho works on time-domain grids");
+  double sm[9], smd[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      sm[r * 3 + c] = priorVar0[r][c];
+      smd[r * 3 + c] = postVar0[r][c];
+    }
+  std::vector<double> g((size_t)nt * 3);
+  if (G)
+    std::copy(G, G + g.size(), g.begin());
+  else
+    check(crava_hip_local_noise_g(nt, errThetaCov_.data(), sm, smd, g.data()), "computeG");
+  if (!traceCov_) {
+    // storage for ny*nx 3x3 double matrices: rows of 18 doubles hold two
+    const int ntr = s_.nx * s_.ny;
+    check(crava_hip_grid_alloc(ctx_->handle(), 16, 1, (ntr + 1) / 2, CRAVA_HIP_F64, &traceCov_), "alloc");
+  }
+  std::vector<const double *> tabs(nt);
+  for (int a = 0; a < nt; ++a) tabs[a] = s_.localNoiseScale[a].data();
+  crava_hip_local_noise p{};
+  p.nx = s_.nx;
+  p.ny = s_.ny;
+  p.nz = s_.nz;
+  p.nxp = s_.nxp;
+  p.nyp = s_.nyp;
+  p.nzp = s_.nzp;
+  p.ntheta = nt;
+  p.G = g.data();
+  p.errThetaCov = errThetaCov_.data();
+  p.priorVar0 = sm;
+  p.postVar0 = smd;
+  p.noiseScale = tabs.data();
+  void *pg[3];
+  const void *mg[3];
   for (int c = 0; c < 3; ++c) {
-    real[c]->fillInComplexNoise(randomGenSeed);
-    seeds[c] = real[c]->devicePtr();
-    mean[c] = sph.mu(c)->getIsTransformed() ? nullptr : sph.mu(c)->devicePtr();
+    pg[c] = post[c]->devicePtr();
+    mg[c] = mean2[c]->devicePtr();
   }
-  for (int q = 0; q < 6; ++q) {
-    if (!sph.cov(q)->getIsTransformed()) throw Error(CRAVA_HIP_E_STATE, "simulate needs transformed covariances");
-    cov[q] = sph.cov(q)->devicePtr();
-  }
-  long long nfail = 0;
-  check(crava_hip_simulate(h, s_.nxp, s_.nyp, s_.nzp, cov, seeds, mean, &nfail), "simulate");
-  for (int c = 0; c < 3; ++c) real[c]->setIsTransformed(false);
-  return nfail;
+  check(crava_hip_local_noise_correct(ctx_->handle(), &p, pg, mg, static_cast<double *>(traceCov_)),
+        "correctAlphaBetaRho");
 }
 
 void Crava::computeSyntSeismic(FFTGrid *alpha, FFTGrid *beta, FFTGrid *rho, const std::vector<FFTGrid *> &syn) {

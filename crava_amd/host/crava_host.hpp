@@ -298,10 +298,14 @@ class Crava {
     // [nyp*nxp] (empty: constant thickness) and the energy threshold of
     // interpolateSeismic (Crava::energyTreshold_)
     std::vector<double> relThick;
+    // local noise: per angle getLocalNoiseScale(angle) as [ny*nx] (i fastest)
+    std::vector<std::vector<double>> localNoiseScale;
     float energyThreshold = 0.0f;
   };
 
   Crava(Context *ctx, const Settings &s, std::vector<FFTGrid *> seisData);
+  Crava(const Crava &) = delete;             // owns device storage
+  Crava &operator=(const Crava &) = delete;
   ~Crava();
 
   void computeDataVariance();                                               // :322-363
@@ -317,6 +321,14 @@ class Crava {
   // noise grids (seed0, seed1, seed2 in turn, :1466-1468): equal to the
   // reference's realisation for the same generator state.
   long long simulate(SeismicParametersHolder &sph, uint32_t &randomGenSeed, FFTGrid *real[3]);
+  // :2340-2470 correctAlphaBetaRho: post[3] (time-domain posterior means)
+  // corrected per trace towards mean2[3] (the prior means) with
+  // Settings::localNoiseScale; keeps the realisation factors (sigmamdnew_)
+  // that both simulate overloads then apply (:1547-1566).  G: the reference's
+  // computeG [ntheta][3] (its LAPACK eigenvector signs), or nullptr for
+  // crava_hip_local_noise_g's.
+  void correctAlphaBetaRho(const double priorVar0[3][3], const double postVar0[3][3], FFTGrid *post[3],
+                           FFTGrid *mean2[3], const double *G = nullptr);
   // :1667-1759 forward modelling: synthetic seismic of (alpha, beta, rho)
   // for every angle into syn[ntheta] (real grids).  Parameters held in the
   // Fourier domain are transformed back and forth, as the reference does.
@@ -341,6 +353,8 @@ class Crava {
   std::vector<float> dataVariance_, errorVariance_;
   std::vector<double> errThetaCov_;   // [ntheta][ntheta]
   long long nFP64_ = 0;
+  void *traceCov_ = nullptr;          // device [ny][nx][9] realisation factors
+  long long realise(SeismicParametersHolder &sph, FFTGrid *real[3]);
 };
 
 }  // namespace crava_hip

@@ -112,6 +112,11 @@ int main(int argc, char **argv) {
       seis.push_back(g);
     }
 
+    // local-noise scales (used by correctAlphaBetaRho below)
+    s.localNoiseScale.assign(nt, std::vector<double>((size_t)s.nx * s.ny));
+    for (int a = 0; a < nt; ++a)
+      for (int j = 0; j < s.ny; ++j)
+        for (int i = 0; i < s.nx; ++i) s.localNoiseScale[a][(size_t)j * s.nx + i] = 1.0 + ((i * 7 + j * 3 + a * 5) % 11) / 5.0;
     crava_hip::Crava crava(&ctx, s, seis);
     crava.computeDataVariance();
     crava.setupErrorCorrelation([angRange](float dTheta) { return std::exp(-3.0 * std::fabs(dTheta) / angRange); });
@@ -133,11 +138,42 @@ int main(int argc, char **argv) {
     // the same with the reference's RandomGen stream from state 23665
     uint32_t rg = 23665;
     crava.simulate(sph, rg, real);
-    for (int c = 0; c < 3; ++c) {
-      write_bin(dir + "/out_simref" + std::to_string(c) + ".bin", drain_real(real[c]));
-      delete real[c];
-    }
+    for (int c = 0; c < 3; ++c) write_bin(dir + "/out_simref" + std::to_string(c) + ".bin", drain_real(real[c]));
     std::ofstream(dir + "/simref_seed.txt") << rg << "\n";
+
+    // local-noise correction (host computeG) of copies of the posterior
+    // means towards the prior means, postVar0 = D sigma0 D with
+    // D = diag(0.6, 0.7, 0.8) (distinct eigenvalues of
+    // sigma0^-1/2 (sigma0 - postVar0) sigma0^-1/2, so that its eigenvectors
+    // are defined up to sign); then the same reference-stream realisation,
+    // now with the local-noise factors
+    {
+      const double dg[3] = {0.6, 0.7, 0.8};
+      double smd[3][3];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) smd[a][b] = (dg[a] * sigma0[a][b]) * dg[b];
+      FFTGrid *post[3], *mean2[3];
+      for (int c = 0; c < 3; ++c) {
+        post[c] = new FFTGrid(&ctx, s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
+        mean2[c] = new FFTGrid(&ctx, s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
+        post[c]->setType(FFTGrid::PARAMETER);
+        mean2[c]->setType(FFTGrid::PARAMETER);
+        post[c]->createRealGrid();
+        mean2[c]->createRealGrid();
+        fill_real(post[c], drain_real(sph.mu(c)));
+        fill_real(mean2[c], read_bin(dir + "/mu" + std::to_string(c) + ".bin", rsize));
+      }
+      crava.correctAlphaBetaRho(sph.getPriorVar0(), smd, post, mean2);
+      for (int c = 0; c < 3; ++c) {
+        write_bin(dir + "/out_lnc" + std::to_string(c) + ".bin", drain_real(post[c]));
+        delete post[c];
+        delete mean2[c];
+      }
+      uint32_t rg2 = 23665;
+      crava.simulate(sph, rg2, real);
+      for (int c = 0; c < 3; ++c) write_bin(dir + "/out_simlnc" + std::to_string(c) + ".bin", drain_real(real[c]));
+    }
+    for (int c = 0; c < 3; ++c) delete real[c];
 
     // forward modelling of the posterior means (Crava::computeSyntSeismic),
     // with a stretch table on angle 0 and a local time shift on the last
@@ -380,7 +416,9 @@ int main(int argc, char **argv) {
     out << "\n";
     for (float x : crava.getErrorVariance()) out << x << " ";
     out << "\n";
+    out.precision(17);
     for (double x : crava.getErrThetaCov()) out << x << " ";
+    out.precision(9);
     out << "\n" << v << " " << outside << "\n";
     for (FFTGrid *g : seis) delete g;
   } catch (const crava_hip::Error &e) {

@@ -22,6 +22,7 @@ import numpy as np
 import pytest
 
 import oracle
+from oracle import local_noise as lnoise
 from crava_amd import _lib as L
 from crava_amd import synth
 from tests.helpers import rel_l2
@@ -281,6 +282,29 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
         mui = _read(os.path.join(d, f"out_mu{i}.bin"), shape)
         want = ref_sim[i][:, :, :nxp] + mui[:, :, :nxp]
         got = _read(os.path.join(d, f"out_simref{i}.bin"), shape)[:, :, :nxp]
+        assert rel_l2(got, want) < 2e-6, i
+    # correctAlphaBetaRho with the host computeG (the oracle re-signs dgeev's
+    # eigenvectors by the same rule), then the realisation with local noise
+    dg = np.array([0.6, 0.7, 0.8])
+    smd = (dg[:, None] * np.asarray(prob.sigma0, dtype=np.float64)) * dg[None, :]
+    Gl = lnoise.compute_g(etc, prob.sigma0, smd, resign=True)
+    tabs = [np.array([[1.0 + ((i * 7 + j * 3 + a * 5) % 11) / 5.0 for i in range(c.nx)] for j in range(c.ny)])
+            for a in range(nt)]
+    sn, tc = lnoise.trace_matrices(Gl, tabs, etc, prob.sigma0, smd)
+    postm = [_read(os.path.join(d, f"out_mu{i}.bin"), shape) for i in range(3)]
+    want = [g.copy() for g in postm]
+    lnoise.apply_correction(want, [g.numpy() for g in prob.mu], sn, c.nx, c.ny, c.nz)
+    for i in range(3):
+        got = _read(os.path.join(d, f"out_lnc{i}.bin"), shape)
+        w = want[i][:c.nz, :c.ny, :c.nx]
+        g = got[:c.nz, :c.ny, :c.nx]
+        assert np.max(np.abs(g.astype(np.float64) - w) / np.spacing(np.abs(w))) <= 1.0, i
+        assert not np.array_equal(g, postm[i][:c.nz, :c.ny, :c.nx]), i
+    draws = [r.copy() for r in ref_sim]
+    lnoise.transform_realisation(draws, tc, c.nx, c.ny, c.nz)
+    for i in range(3):
+        want = draws[i][:, :, :nxp] + postm[i][:, :, :nxp]
+        got = _read(os.path.join(d, f"out_simlnc{i}.bin"), shape)[:, :, :nxp]
         assert rel_l2(got, want) < 2e-6, i
     for i in range(3):
         sim = _read(os.path.join(d, f"out_sim{i}.bin"), shape)[:, :, :nxp]
