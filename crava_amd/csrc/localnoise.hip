@@ -8,6 +8,8 @@
 // positive part), which does not depend on the eigenvectors' signs; the
 // kernels use cyclic Jacobi rotations on the symmetrised matrix, fully
 // unrolled for each angle count so that the matrices stay in registers.
+// The pseudo-inverse takes a Cholesky route when h - 1e-8 I is positive
+// definite (then no eigenvalue is dropped and it is the inverse).
 //
 // The per-point kernels follow the reference's arithmetic: float
 // differences, double sums from the left, one rounding to float (built with
@@ -108,6 +110,50 @@ __host__ __device__ inline void matrix_function(double (&m)[N][N], double thr) {
     }
 }
 
+// Whether every eigenvalue of the symmetrised m exceeds thr: m - thr I has a
+// Cholesky factor exactly then.  On success m holds, in its lower triangle,
+// the Cholesky factor of m itself (the symmetrised m); on failure m is
+// clobbered.
+template <int N>
+__device__ inline bool cholesky_above(double (&m)[N][N], double thr) {
+  double t[N][N];
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int c = 0; c <= r; ++c) {
+      const double v = 0.5 * (m[r][c] + m[c][r]);
+      m[r][c] = v;
+      t[r][c] = (r == c) ? v - thr : v;
+    }
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double dt = t[j][j], dm = m[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) {
+      dt -= t[j][k] * t[j][k];
+      dm -= m[j][k] * m[j][k];
+    }
+    ok = ok && (dt > 0.0) && (dm > 0.0);
+    dt = sqrt(fmax(dt, 1e-300));
+    dm = sqrt(fmax(dm, 1e-300));
+    t[j][j] = dt;
+    m[j][j] = dm;
+#pragma unroll
+    for (int r = j + 1; r < N; ++r) {
+      double xt = t[r][j], xm = m[r][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) {
+        xt -= t[r][k] * t[j][k];
+        xm -= m[r][k] * m[j][k];
+      }
+      t[r][j] = xt / dt;
+      m[r][j] = xm / dm;
+    }
+  }
+  return ok;
+}
+
 // One thread per trace: sigmanew = sqrt(deltanew) (Sm - Smd)^(-1/2) and the
 // realisation factor sqrt(Sm - deltanew) Smd^(-1/2), with
 // deltanew = Sm G^T (G Sm G^T + D E D)^+ G Sm and D = diag(sqrt(scale)).
@@ -130,23 +176,53 @@ __global__ __launch_bounds__(64) void k_lnc_trace(const LncConst c, const double
 #pragma unroll
     for (int s = 0; s < NT; ++s)
       h[r][s] = (gs[r][0] * c.g[s][0] + gs[r][1] * c.g[s][1] + gs[r][2] * c.g[s][2]) + (d[r] * c.e[r][s]) * d[s];
-  matrix_function<NT, kInverse>(h, 0.00000001);
-  // deltanew = (Sm G^T) P (G Sm)
   double dn[3][3];
+  if (cholesky_above<NT>(h, 0.00000001)) {
+    // every eigenvalue of h exceeds the threshold: the pseudo-inverse is the
+    // inverse, and deltanew = (G Sm)^T h^-1 (G Sm) = Y^T Y, Y = L^-1 (G Sm)
+    double y[NT][3];
 #pragma unroll
-  for (int p = 0; p < 3; ++p)
+    for (int r = 0; r < NT; ++r)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      double x = 0.0;
+      for (int q = 0; q < 3; ++q) {
+        double x = gs[r][q];
 #pragma unroll
-      for (int r = 0; r < NT; ++r) {
-        double y = 0.0;
-#pragma unroll
-        for (int s = 0; s < NT; ++s) y += h[r][s] * gs[s][q];
-        x += (c.sm[p][0] * c.g[r][0] + c.sm[p][1] * c.g[r][1] + c.sm[p][2] * c.g[r][2]) * y;
+        for (int k = 0; k < r; ++k) x -= h[r][k] * y[k][q];
+        y[r][q] = x / h[r][r];
       }
-      dn[p][q] = x;
-    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double x = 0.0;
+#pragma unroll
+        for (int r = 0; r < NT; ++r) x += y[r][p] * y[r][q];
+        dn[p][q] = x;
+      }
+  } else {
+    // rebuild h (cholesky_above overwrote it) and take the eigen route
+#pragma unroll
+    for (int r = 0; r < NT; ++r)
+#pragma unroll
+      for (int s = 0; s < NT; ++s)
+        h[r][s] = (gs[r][0] * c.g[s][0] + gs[r][1] * c.g[s][1] + gs[r][2] * c.g[s][2]) + (d[r] * c.e[r][s]) * d[s];
+    matrix_function<NT, kInverse>(h, 0.00000001);
+    // deltanew = (Sm G^T) P (G Sm)
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double x = 0.0;
+#pragma unroll
+        for (int r = 0; r < NT; ++r) {
+          double y = 0.0;
+#pragma unroll
+          for (int s = 0; s < NT; ++s) y += h[r][s] * gs[s][q];
+          x += (c.sm[p][0] * c.g[r][0] + c.sm[p][1] * c.g[r][1] + c.sm[p][2] * c.g[r][2]) * y;
+        }
+        dn[p][q] = x;
+      }
+  }
   double pc[3][3];
 #pragma unroll
   for (int p = 0; p < 3; ++p)

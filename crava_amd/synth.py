@@ -423,15 +423,20 @@ def isq_grid(nx, ny, nz, nxp, nyp, nzp, seed, n_bad, n_weak=0, cluster=False):
     return g
 
 
-def local_noise_problem(ntheta: int, nx: int, ny: int, seed: int = 0, uniform: float | None = None) -> dict:
+def local_noise_problem(ntheta: int, nx: int, ny: int, seed: int = 0, uniform: float | None = None,
+                        duplicate_angle: bool = False) -> dict:
     """A local-noise correction problem: angle error covariance E
     (variances ~1e-4, correlation exp(-3|dtheta|/30 deg)), prior variance Sm,
     its Bayesian update Smd = (Sm^-1 + K^T E^-1 K)^-1 for an Aki-Richards
     like K, and per-angle noise-scale tables [ny][nx] in [1, 3] (all equal
-    to `uniform` when given)."""
+    to `uniform` when given).  duplicate_angle: the last two angles (and their
+    scale tables) coincide, so that every trace's G Sm G^T + D E D is
+    singular and its pseudo-inverse drops an eigenvalue."""
     import numpy as np
     rng = np.random.default_rng(seed)
     th = np.linspace(5.0, 40.0, ntheta)
+    if duplicate_angle:  # the last two angles coincide: E is singular
+        th[-1] = th[-2]
     ev = 1e-4 * (1.0 + rng.random(ntheta))
     rho = np.exp(-3.0 * np.abs(th[:, None] - th[None, :]) / 30.0)
     E = np.sqrt(ev)[:, None] * np.sqrt(ev)[None, :] * rho
@@ -439,10 +444,12 @@ def local_noise_problem(ntheta: int, nx: int, ny: int, seed: int = 0, uniform: f
     Sm = 1e-3 * (B @ B.T + 0.5 * np.eye(3))
     t = np.radians(th)
     K = np.stack([0.5 / np.cos(t) ** 2, -4.0 * 0.25 * np.sin(t) ** 2, 0.5 - 2.0 * 0.25 * np.sin(t) ** 2], axis=1)
-    Smd = np.linalg.inv(np.linalg.inv(Sm) + K.T @ np.linalg.inv(E) @ K)
+    Smd = np.linalg.inv(np.linalg.inv(Sm) + K.T @ np.linalg.pinv(E) @ K)
     Smd = 0.5 * (Smd + Smd.T)
     if uniform is None:
         scales = [1.0 + 2.0 * rng.random((ny, nx)) for _ in range(ntheta)]
+        if duplicate_angle:  # equal scales on the twin angles: G Sm G^T + D E D is singular
+            scales[-1] = scales[-2].copy()
     else:
         scales = [np.full((ny, nx), float(uniform)) for _ in range(ntheta)]
     return dict(E=E, Sm=Sm, Smd=Smd, scales=scales)
