@@ -41,6 +41,7 @@ SOURCES = {
     "forward.hip": ["-ffp-contract=off"],
     "swdiv.hip": ["-ffp-contract=off"],
     "localnoise.hip": ["-ffp-contract=off"],
+    "eig3.cpp": ["-ffp-contract=off"],
 }
 
 COMMON = [

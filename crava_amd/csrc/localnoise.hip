@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "../../include/crava_hip.h"
+#include "eig3.hpp"
 #include "localnoise.hpp"
 
 namespace crava {
@@ -400,9 +401,17 @@ int lnc_compute_g(int ntheta, const double *E, const double *Sm, const double *S
         for (int l = 0; l < 3; ++l) x += sinv[r][k] * sd[k][l] * sinv[l][c];
       a[r][c] = x;
     }
-  for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < r; ++c) a[r][c] = a[c][r] = 0.5 * (a[r][c] + a[c][r]);
-  jacobi<3>(a, v);
+  // LAPACK DGEEV's eigenvectors (signs included) where its restatement
+  // handles the matrix; otherwise Jacobi with the largest-component rule
+  double wr[3];
+  const bool lapack_like = dgeev3(a, wr, v);
+  if (lapack_like) {
+    for (int k = 0; k < 3; ++k) a[k][k] = wr[k];
+  } else {
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < r; ++c) a[r][c] = a[c][r] = 0.5 * (a[r][c] + a[c][r]);
+    jacobi<3>(a, v);
+  }
   // NRLib::Sort3x3: descending, the first maximum wins a tie
   double ev[3], vec[3][3];
   int used[3] = {0, 0, 0};
@@ -417,11 +426,11 @@ int lnc_compute_g(int ntheta, const double *E, const double *Sm, const double *S
     if (best < 0) return CRAVA_HIP_E_INVALID;
     used[best] = 1;
     ev[j] = mx;
-    // sign: the component of largest magnitude is positive
+    // sign: DGEEV's, or the component of largest magnitude positive
     int im = 0;
     for (int k = 1; k < 3; ++k)
       if (std::fabs(v[k][best]) > std::fabs(v[im][best])) im = k;
-    const double sg = v[im][best] < 0.0 ? -1.0 : 1.0;
+    const double sg = (!lapack_like && v[im][best] < 0.0) ? -1.0 : 1.0;
     for (int k = 0; k < 3; ++k) vec[k][j] = sg * v[k][best];
   }
   switch (ntheta) {

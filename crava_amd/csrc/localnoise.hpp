@@ -20,10 +20,12 @@ struct LncConst {
 // the given threshold: l > thr kept).
 void lnc_inv_sqrt_host(const double in[3][3], double thr, double out[3][3]);
 
-// Crava::computeG on the host, with the eigenvectors of
-// Sm^(-1/2) (Sm - Smd)+ Sm^(-1/2) signed so that each one's component of
-// largest magnitude is positive.  E: [ntheta][ntheta], Sm, Smd: [3][3],
-// G: [ntheta][3], all row-major.
+// Crava::computeG on the host.  The eigenvectors of
+// Sm^(-1/2) (Sm - Smd)+ Sm^(-1/2) come from the restatement of LAPACK
+// DGEEV's path (dgeev3) where it handles the matrix, else from Jacobi
+// rotations signed so that each one's component of largest magnitude is
+// positive.  E: [ntheta][ntheta], Sm, Smd: [3][3], G: [ntheta][3],
+// all row-major.
 int lnc_compute_g(int ntheta, const double *E, const double *Sm, const double *Smd, double *G);
 
 // scales: device [ntheta][ny][nx]; sigmanew, trace_cov: device [ny][nx][9]

@@ -481,12 +481,16 @@ typedef struct crava_hip_local_noise {
 
 /* Crava::computeG (src/crava.cpp:2130-2211) on the host.  The reference
  * takes the eigenvectors of Sm^(-1/2) (Sm - Smd)+ Sm^(-1/2) from LAPACK's
- * dgeev, whose signs G inherits; here each eigenvector is signed so that its
- * component of largest magnitude is positive.  The correction does not
- * depend on those signs when every angle's noise scale is uniform; with
- * varying scales, a caller that needs the reference's result passes the G
- * of its own computeG to crava_hip_local_noise_correct.  Host only; no
- * context needed. */
+ * dgeev, whose signs G inherits (with noise scales that vary between angles
+ * the correction depends on them).  Here they come from a restatement of
+ * dgeev's path for a dense 3x3 matrix (balancing, Householder reduction,
+ * Francis double-shift QR, back substitution); a matrix with a zero
+ * off-diagonal entry falls back to Jacobi rotations with each eigenvector's
+ * largest component positive.  Where the QR iteration's last reflections
+ * depend on last-bit rounding the signs can still differ from a given
+ * LAPACK build, so a caller that needs the reference's result exactly
+ * passes the G of its own computeG to crava_hip_local_noise_correct.  Host
+ * only; no context needed. */
 int crava_hip_local_noise_g(int ntheta, const double *errThetaCov, const double *priorVar0, const double *postVar0,
                             double *G);
 
