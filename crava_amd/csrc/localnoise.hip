@@ -253,7 +253,7 @@ __global__ __launch_bounds__(64) void k_lnc_trace(const LncConst c, const double
 }
 
 constexpr int kLncThreads = 256;
-constexpr int kLncKChunk = 32;
+constexpr int kLncKChunk = 16;
 constexpr int kLncBatch = 4;
 
 // post_c <- float(mean_c + s_c0 da + s_c1 db + s_c2 dr), d = float(post - mean)
