@@ -25,7 +25,6 @@ namespace crava {
 namespace {
 
 constexpr double kSafmin = 2.2250738585072014e-308;  // DLAMCH('S')
-constexpr double kEps = 1.1102230246251565e-16;      // DLAMCH('E'), 2^-53
 constexpr double kUlp = 2.2204460492503131e-16;      // DLAMCH('P'), 2^-52
 
 inline double fsign(double a, double b) { return std::signbit(b) ? -std::fabs(a) : std::fabs(a); }
@@ -180,7 +179,6 @@ bool dgeev3(const double m[3][3], double wr[3], double vr[3][3]) {
   const int kexsh = 10;
   int kdefl = 0;
   int I = ihi;
-  double wi[4] = {};
   double WR[4] = {};
   while (I >= ilo) {
     int L = ilo;
@@ -337,7 +335,6 @@ bool dgeev3(const double m[3][3], double wr[3], double vr[3][3]) {
     if (!converged) return false;
     if (L == I) {
       WR[I] = H[I][I];
-      wi[I] = 0.0;
     } else if (L == I - 1) {
       double cs, sn;
       lanv2(H[I - 1][I - 1], H[I - 1][I], H[I][I - 1], H[I][I], cs, sn);
@@ -403,8 +400,6 @@ bool dgeev3(const double m[3][3], double wr[3], double vr[3][3]) {
     for (int r = 1; r <= 3; ++r) vr[r - 1][c - 1] = VR[r][c] * scl;
     wr[c - 1] = WR[c];
   }
-  (void)wi;
-  (void)kEps;
   return true;
 }
 
