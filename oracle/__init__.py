@@ -64,6 +64,8 @@ def orc():
         L.orc_data_variance.argtypes = [VP, I, I, I, I, I, I, C.POINTER(C.c_longlong)]
         L.orc_data_variance.restype = C.c_float
         L.orc_facies_geomodel.argtypes = [I, I, I, I, VP, PP]
+        L.orc_facies_find_density.argtypes = [D, D, D, VP, I, I, VP, VP]
+        L.orc_facies_find_density.restype = C.c_double
         L.orc_merge_mean_4d.argtypes = [LL, PP, PP, PP]
         L.orc_merge_cov_4d.argtypes = [LL, PP, PP, PP, PP]
         L.orc_synthetic_seismic.argtypes = [I, I, I, I, I, I, I, VP, VP, F, PP, PP, PP, VP, VP, VP, PP]
@@ -235,6 +237,15 @@ def facies(nx, ny, nz, nxp, nyp, nzp, alpha, beta, rho, tables, tn, vol, vn, pri
                      tab.ctypes.data, tn[0], tn[1], tn[2], v.ctypes.data, n.ctypes.data, pr.ctypes.data,
                      p_undef, _ptrs(out))
     return out
+
+
+def facies_find_density(vp, vs, rho, table, tn, vol, vn) -> float:
+    """One PosteriorElasticPDF3D::FindDensity lookup; table [tn3][tn2][2*(tn1//2+1)]."""
+    tab = np.ascontiguousarray(table, dtype=np.float32)
+    v = np.ascontiguousarray(vol, dtype=np.float64)
+    n = np.ascontiguousarray(vn, dtype=np.int32)
+    return float(orc().orc_facies_find_density(float(vp), float(vs), float(rho), tab.ctypes.data, tn[0], tn[1],
+                                               v.ctypes.data, n.ctypes.data))
 
 
 # ------------------------------------------------------------ grid fills / operators

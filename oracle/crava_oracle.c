@@ -21,7 +21,9 @@
  *   - posterior: dense brute-force Bayesian inversion on tiny grids (FP64),
  *     Woodbury identity, prior pass-through out of band and on failure;
  *   - RandomGen: LCG closed form, moments of rnorm01;
- *   - facies: brute-force trilinear lookup on hand-built tables;
+ *   - facies: an independent numpy trilinear lookup (clip-then-interpolate
+ *     form), hand-computed node / cell-centre / clamp / negative-entry cases
+ *     and the probability assembly (tests/test_oracle_facies.py);
  *   - NLL: central finite differences and an independent numpy evaluation.
  */
 #include <math.h>
@@ -931,6 +933,14 @@ static double o_find_density(double vp, double vs, double rho, const float *tab,
   value += (wj) * (wk) * (1.0f - wl) * v7;
   value += (wj) * (wk) * (wl) * v8;
   return value;
+}
+
+/* One density lookup (PosteriorElasticPDF3D::FindDensity), exported so the
+ * tests can check it against hand-computed cases and a brute-force lookup. */
+double orc_facies_find_density(double vp, double vs, double rho, const float *tab,
+                               int tn1, int tn2, const double *vol, const int *vn)
+{
+  return o_find_density(vp, vs, rho, tab, tn1, tn2, vol, vn);
 }
 
 void orc_facies(int nx, int ny, int nz, int nxp, int nyp, int nzp,
