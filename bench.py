@@ -2,15 +2,17 @@
 """Posterior Fourier points/s of the CRAVA Fourier-domain inversion core on one MI355X.
 
 Workload (default): 512x512x320 FFT grid (interior 448x448x256), 6 angle
-stacks at 4-40 deg, per-angle S/N 100..20, seeded synthetic float32 inputs
-generated on the GPU.  One step is the whole posterior path of
-Crava::computePostMeanResidAndFFTCov with its transforms:
+stacks at 4-40 deg, per-angle S/N 1e4, 3e3, 1e3, 300, 100, 20, seeded
+synthetic float32 inputs generated on the GPU.  One step is the whole
+posterior path of Crava::computePostMeanResidAndFFTCov with its transforms:
 
   16 forward r2c (6 data, 3 prior means, 6 prior covariances, errCorr)
   + the per-point posterior over all P = 257*512*320 Fourier points
   + 15 inverse c2r (3 posterior means, 6 posterior covariances, 6 residuals)
 
-run twice: the MIXED form (float grids and transforms, FP64 solve where
+run twice: the MIXED form (float grids; data transforms in float, mean and
+covariance transforms with double arithmetic inside each pass; FP32
+square-root solve, FP64 solve for the waves holding a point with
 noise/signal < threshold) and the plain FP64 form (double grids, double
 transforms, double solve).  Each form's step runs two ways: as a one-volume
 crava_hip_batch replayed as a hipGraph (the data/mean and covariance
@@ -94,7 +96,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="precision")
-    ap.add_argument("--threshold", type=float, default=0.1, help="noise/signal below which MIXED uses FP64")
+    ap.add_argument("--threshold", type=float, default=1e-3, help="noise/signal below which MIXED uses FP64")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -134,6 +136,8 @@ class Pipeline:
         self.corr_xy = torch.from_numpy(np.ascontiguousarray(prob.corr_xy, dtype=np.float32)).cuda()
         self.circ_t = torch.from_numpy(np.ascontiguousarray(prob.circ_corr_t, dtype=np.float32)).cuda()
         self.precision = L.F64 if fp64 else L.MIXED
+        # transform precision of the data grids, and of the means and covariances
+        self.data_precision = L.F64 if fp64 else L.F32
         self.cov_precision = L.F64 if fp64 else L.MIXED
         self.L = L
         self.batch = None
@@ -169,14 +173,19 @@ class Pipeline:
                 nf, n64 = self.batch.counts()
                 return nf[0], n64[0]
             return None
-        # data and prior means share the 1/sqrt(N) scaling (DATA / PARAMETER)
-        self.ctx.r2c_many(self.data + self.mu, *n, L.DATA)
-        # covariance spectra span many decades: the mixed form transforms them
-        # with double arithmetic (float storage) so small eigenvalues stay exact
-        self.ctx.r2c_many(self.cov + [self.ec], *n, L.COVARIANCE, self.cov_precision)
+        # data and prior means share the 1/sqrt(N) scaling (DATA / PARAMETER);
+        # the mixed form transforms the means (both ways) and the covariances
+        # (forward) with double arithmetic inside each pass (float storage):
+        # the mean update and the small prior eigenvalues need it; the data,
+        # the error correlation and the posterior covariances do not
+        self.ctx.r2c_many(self.data, *n, L.DATA, self.data_precision)
+        self.ctx.r2c(self.ec, *n, L.COVARIANCE, self.data_precision)
+        self.ctx.r2c_many(self.mu, *n, L.DATA, self.cov_precision)
+        self.ctx.r2c_many(self.cov, *n, L.COVARIANCE, self.cov_precision)
         r = self.post.run(self.precision, self.mu, self.data, self.cov, self.ec, counts=counts)
-        self.ctx.c2r_many(self.mu + self.data, *n, L.DATA)
-        self.ctx.c2r_many(self.cov, *n, L.COVARIANCE)
+        self.ctx.c2r_many(self.mu, *n, L.DATA, self.cov_precision)
+        self.ctx.c2r_many(self.data, *n, L.DATA, self.data_precision)
+        self.ctx.c2r_many(self.cov, *n, L.COVARIANCE, self.data_precision)
         return r
 
     def reset(self):
@@ -827,8 +836,10 @@ def main():
     grid_c_bytes = 8 * cn * nyp * nzp * ng
     real_bytes = 4 * nxp * nyp * nzp * ng
     mixed.reset()
-    ctx.r2c_many(mixed.data + mixed.mu, *n3, L.DATA)
-    ctx.r2c_many(mixed.cov + [mixed.ec], *n3, L.COVARIANCE, L.MIXED)
+    ctx.r2c_many(mixed.data, *n3, L.DATA)
+    ctx.r2c(mixed.ec, *n3, L.COVARIANCE)
+    ctx.r2c_many(mixed.mu, *n3, L.DATA, L.MIXED)
+    ctx.r2c_many(mixed.cov, *n3, L.COVARIANCE, L.MIXED)
 
     def restorer(pipe):
         """The posterior works in place: every timed launch starts from the

@@ -4,12 +4,12 @@
 // Crava::computePostMeanResidAndFFTCov (src/crava.cpp:887-1204).
 //
 // Volumes are spread over a few side streams forked from the origin stream,
-// so small volumes overlap.  Within a volume the data/mean transforms and the
-// covariance transforms go to two different streams: in the mixed form the
-// covariance transforms run double arithmetic (latency-bound) while the float
-// data transforms stream HBM, so the two overlap.  Volumes of one grid shape
-// share transform plans (and the double scratch), so each kind of set of a
-// shape always goes to the same stream.
+// so small volumes overlap.  Within a volume the data transforms and the
+// mean/covariance transforms go to two different streams: in the mixed form
+// the latter run double arithmetic inside each pass while the float data
+// transforms stream HBM, so the two overlap.  Volumes of one grid shape
+// share transform plans, so each kind of set of a shape always goes to the
+// same stream.
 #include "batch.hpp"
 #include "../../include/crava_hip.h"
 
@@ -92,14 +92,19 @@ int BatchExec::init(const BatchVolume *vols, int n) {
   return CRAVA_HIP_OK;
 }
 
-// Transform sets per volume and the stream of each volume.  Plan precision:
-// float grids use float transforms, except that the mixed form transforms
-// the covariance grids forward with double arithmetic (their spectra span
-// many decades); double grids use double transforms.
+// Transform sets per volume and the stream of each volume.  Forward: [0] the
+// data (DATA scaling) and the error correlation (COVARIANCE), [1] the means
+// (DATA) and the covariances (COVARIANCE).  Inverse: [0] the residuals and
+// the covariances, [1] the means.  Plan precision: the float form uses float
+// transforms throughout; the mixed form runs set [1] with double arithmetic
+// inside each pass (float storage) -- the posterior mean update needs it for
+// the means, the small prior eigenvalues for the covariances -- and set [0]
+// in float; the double form uses double transforms.
 int BatchExec::build_sets() {
   const bool f64 = precision_ == CRAVA_HIP_F64;
   const int pa = f64 ? 2 : 0;
   const int pc = f64 ? 2 : (precision_ == CRAVA_HIP_MIXED ? 1 : 0);
+  const bool dc = pc != 0;   // set [1] keeps double scale factors
   work_.assign(vols_.size(), Work{});
   std::map<std::tuple<int, int, int>, std::vector<int>> by_shape;
   for (size_t v = 0; v < vols_.size(); ++v) {
@@ -112,33 +117,31 @@ int BatchExec::build_sets() {
     if ((rc = ctx_->plan(pp.nxp, pp.nyp, pp.nzp, pa, &fa->plan))) return rc;
     if ((rc = ctx_->plan(pp.nxp, pp.nyp, pp.nzp, pc, &fc->plan))) return rc;
     ia->plan = fa->plan;
-    if ((rc = ctx_->plan(pp.nxp, pp.nyp, pp.nzp, pa, &ic->plan))) return rc;
+    ic->plan = fc->plan;
     const double sf = grid_fft_scale(false, f64, N, false), si = grid_fft_scale(false, f64, N, true);
-    const double cf = grid_fft_scale(true, f64, N, false), ci = grid_fft_scale(true, f64, N, true);
+    const double mf = grid_fft_scale(false, dc, N, false), mi = grid_fft_scale(false, dc, N, true);
+    const double cf = grid_fft_scale(true, dc, N, false);
+    const double ef = grid_fft_scale(true, f64, N, false), ei = grid_fft_scale(true, f64, N, true);
     for (int l = 0; l < pp.ntheta; ++l) {
       fa->grids[fa->ng] = bv.data[l];
       fa->scale[fa->ng++] = sf;
-    }
-    for (int c = 0; c < 3; ++c) {
-      fa->grids[fa->ng] = bv.mu[c];
-      fa->scale[fa->ng++] = sf;
-      ia->grids[ia->ng] = bv.mu[c];
-      ia->scale[ia->ng++] = si;
-    }
-    for (int l = 0; l < pp.ntheta; ++l) {
       ia->grids[ia->ng] = bv.data[l];
       ia->scale[ia->ng++] = si;
+    }
+    fa->grids[fa->ng] = bv.errcorr;
+    fa->scale[fa->ng++] = ef;
+    for (int c = 0; c < 3; ++c) {
+      fc->grids[fc->ng] = bv.mu[c];
+      fc->scale[fc->ng++] = mf;
+      ic->grids[ic->ng] = bv.mu[c];
+      ic->scale[ic->ng++] = mi;
     }
     for (int q = 0; q < 6; ++q) {
       fc->grids[fc->ng] = bv.cov[q];
       fc->scale[fc->ng++] = cf;
-      ic->grids[ic->ng] = bv.cov[q];
-      ic->scale[ic->ng++] = ci;
+      ia->grids[ia->ng] = bv.cov[q];
+      ia->scale[ia->ng++] = ei;
     }
-    fc->grids[fc->ng] = bv.errcorr;
-    fc->scale[fc->ng++] = cf;
-    for (Set *st : {fa, fc, ia, ic})
-      if ((rc = st->plan->ensure_scratch(st->ng))) return rc;
     by_shape[std::make_tuple(pp.nxp, pp.nyp, pp.nzp)].push_back((int)v);
   }
   // greedy balance of grid bytes over the side streams, one stream per shape
@@ -147,7 +150,7 @@ int BatchExec::build_sets() {
     size_t bytes = 0;
     for (int v : kv.second) {
       const PostProblem &pp = vols_[v].pd->pp;
-      bytes += real_grid_bytes(pp, f64) * (size_t)(10 + pp.ntheta);
+      bytes += real_grid_bytes(pp, f64) * (size_t)(10 + pp.ntheta);  // every grid of the volume
     }
     groups.emplace_back(bytes, kv.second);
   }
@@ -209,16 +212,22 @@ int BatchExec::launch_post(hipStream_t origin) {
     if ((rc = fork(origin))) return rc;
     for (size_t v = 0; v < vols_.size(); ++v) {
       const BatchVolume &bv = vols_[v];
-      if ((rc = launch_posterior(*bv.pd, precision_, bv.mu, bv.data, bv.cov, bv.errcorr, counters_ + 2 * v,
+      const DeferQueue *q = post_.vq.empty() ? nullptr : &post_.vq[v];
+      if ((rc = launch_posterior(*bv.pd, precision_, bv.mu, bv.data, bv.cov, bv.errcorr, counters_ + 2 * v, q,
                                  side_[work_[v].stream[0]])))
         return rc;
     }
     return join(origin);
   }
-  if (post_.form == kFormSingle || post_.num_classes() < 2) return post_.launch(&origin, 1);
-  if ((rc = fork(origin))) return rc;
-  if ((rc = post_.launch(side_, kStreams))) return rc;
-  return join(origin);
+  if ((rc = post_.reset_queues(origin))) return rc;
+  if (post_.form == kFormSingle || post_.num_classes() < 2) {
+    if ((rc = post_.launch(&origin, 1))) return rc;
+  } else {
+    if ((rc = fork(origin))) return rc;
+    if ((rc = post_.launch(side_, kStreams))) return rc;
+    if ((rc = join(origin))) return rc;
+  }
+  return post_.launch_queued(origin);
 }
 
 int BatchExec::run(hipStream_t s, bool graph) {

@@ -1,7 +1,7 @@
 // Many inversion volumes of uneven size and ntheta as one captured sequence:
-// per volume the forward transforms of (data, mean) and (covariances, error
-// correlation) as two grid sets, one persistent posterior launch over every
-// volume, then the inverse transforms of (mean, residuals) and covariances.
+// per volume the forward transforms of (data, error correlation) and of
+// (means, covariances) as two grid sets, the posterior, then the inverse
+// transforms of (residuals, covariances) and of the means.
 // The sequence runs eagerly or as a hipGraph captured once and replayed.
 #pragma once
 #include "common.hpp"
@@ -45,7 +45,8 @@ class BatchExec {
     double scale[kMaxGrids] = {0};
   };
   struct Work {
-    Set fwd[2], inv[2];   // [0] data + means, [1] covariances (+ error correlation)
+    Set fwd[2], inv[2];   // fwd: [0] data + error correlation, [1] means + covariances;
+                          // inv: [0] residuals + covariances, [1] means
     int stream[2] = {0, 0};
   };
   int build_sets();

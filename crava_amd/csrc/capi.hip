@@ -14,6 +14,7 @@
 #include "localnoise.hpp"
 #include "tables.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -72,6 +73,10 @@ struct crava_hip_post {
   int *nll_klist = nullptr;
   double *nll_partial = nullptr;
   int nll_blocks = 0;
+  // queue of the mixed form (count, then the list), allocated on the first
+  // CRAVA_HIP_MIXED run
+  crava::DeferQueue queue;
+  void *queue_mem = nullptr;
 };
 
 using crava::set_error;
@@ -122,8 +127,10 @@ int crava_hip_synchronize(crava_hip_ctx *ctx) {
   return CRAVA_HIP_OK;
 }
 
+// Float transforms round the factor to float (as FFTGrid); the mixed and
+// double transforms keep it in double.
 static double fft_scale(int cubetype, int precision, long long N, bool inverse) {
-  return crava::grid_fft_scale(cubetype == CRAVA_HIP_COVARIANCE, precision == CRAVA_HIP_F64, N, inverse);
+  return crava::grid_fft_scale(cubetype == CRAVA_HIP_COVARIANCE, precision != CRAVA_HIP_F32, N, inverse);
 }
 
 static int fft_common(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp, int nyp, int nzp, int cubetype,
@@ -180,7 +187,6 @@ int crava_hip_fft_pass_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, i
   crava::Fft3D *f = nullptr;
   int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
   if (rc) return rc;
-  if ((rc = f->ensure_scratch(ngrid))) return rc;
   double sc[crava::kMaxGrids];
   for (int i = 0; i < ngrid; ++i) sc[i] = scale;
   return f->pass(axis, grids, ngrid, inverse != 0, sc, ctx->stream);
@@ -296,6 +302,7 @@ int crava_hip_post_create(crava_hip_ctx *ctx, const crava_hip_post_params *param
   std::vector<crava::PerK<float>> tf(p.nzp);
   std::vector<crava::PerK<double>> td(p.nzp);
   crava::build_perk(pp, invert, kWD, e1, e2, e3, tf.data(), td.data());
+  h->dev.defer_need = crava::defer_capacity(pp, invert);
   const size_t nv = (size_t)p.nzp * p.ntheta * 2;
   h->invert.assign(invert, invert + p.nzp);
   h->kWD.assign(kWD, kWD + nv);
@@ -324,6 +331,7 @@ int crava_hip_post_destroy(crava_hip_post *post) {
   if (post->nll_tab) (void)hipFree(post->nll_tab);
   if (post->nll_klist) (void)hipFree(post->nll_klist);
   if (post->nll_partial) (void)hipFree(post->nll_partial);
+  if (post->queue_mem) (void)hipFree(post->queue_mem);
   delete post;
   return CRAVA_HIP_OK;
 }
@@ -336,10 +344,17 @@ int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, voi
   for (int q = 0; q < 6; ++q) if (!cov[q]) { set_error("NULL covariance grid"); return CRAVA_HIP_E_INVALID; }
   for (int l = 0; l < post->dev.pp.ntheta; ++l) if (!data[l]) { set_error("NULL data grid"); return CRAVA_HIP_E_INVALID; }
   crava_hip_ctx *ctx = post->ctx;
+  if (precision == CRAVA_HIP_MIXED && !post->queue_mem) {
+    const size_t n = (size_t)std::max<long long>(post->dev.defer_need, 1);
+    CRAVA_HIP_CHECK(hipMalloc(&post->queue_mem, 256 + sizeof(int2) * n));
+    post->queue.count = static_cast<unsigned *>(post->queue_mem);
+    post->queue.list = reinterpret_cast<int2 *>(static_cast<unsigned char *>(post->queue_mem) + 256);
+    post->queue.cap = (long long)n;
+  }
   const bool want = n_fail || n_fp64;
   if (want) CRAVA_HIP_CHECK(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
   int rc = crava::launch_posterior(post->dev, precision, mu, data, cov, errcorr, want ? ctx->counters : nullptr,
-                                   ctx->stream);
+                                   &post->queue, ctx->stream);
   if (rc) return rc;
   if (want) {
     unsigned long long h[2] = {0, 0};

@@ -360,25 +360,6 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nro
 // ---------------------------------------------------------------------------
 // Host side: plans and launches.
 // ---------------------------------------------------------------------------
-// Element-wise storage conversion (used when an axis of length 1 makes a pass
-// the identity but the data still has to move between float and double).
-template <typename SI, typename SO>
-__global__ void k_convert(const SI *src, SO *dst, long long n) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    dst[i] = (SO)src[i];
-}
-
-int launch_convert(int io, const void *src, void *dst, long long ncomplex, hipStream_t s) {
-  const long long n = 2 * ncomplex;
-  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
-  if (io == kIoF32toF64)
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(blocks), dim3(256), 0, s, (const float *)src, (double *)dst, n);
-  else if (io == kIoF64toF32)
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks), dim3(256), 0, s, (const double *)src, (float *)dst, n);
-  CRAVA_HIP_CHECK(hipGetLastError());
-  return CRAVA_HIP_OK;
-}
-
 bool factor_radices(int n, PassPlan &p, int order) {
   const RadixList L = make_radices(n);
   p.n = n;
@@ -408,7 +389,6 @@ static const int kGenericTxCap[3] = {64, 16, 8};
 Fft3D::Fft3D(int nx, int ny, int nz, int p) : nxp(nx), nyp(ny), nzp(nz), prec(p), dbl(p >= 1) {}
 
 Fft3D::~Fft3D() {
-  if (scratch) (void)hipFree(scratch);
   if (twx) (void)hipFree(twx);
   if (twy) (void)hipFree(twy);
   if (twz) (void)hipFree(twz);
@@ -507,7 +487,6 @@ int Fft3D::init() {
         (rc = upload_static_twiddles<float>(nyp, &stw[1])) || (rc = upload_static_twiddles<float>(nzp, &stw[2])))
       return rc;
   }
-  if (prec == 1 && (rc = ensure_scratch(1))) return rc;
   // specialised kernels where a length has them (variant 1 until tuned)
   for (int axis = 0; axis < 3; ++axis) {
     var[axis] = -1;
@@ -518,16 +497,6 @@ int Fft3D::init() {
 }
 
 long long Fft3D::complex_elems() const { return (long long)(nxp / 2 + 1) * nyp * nzp; }
-
-int Fft3D::ensure_scratch(int ngrids) {
-  if (prec != 1 || ngrids <= scratch_grids) return CRAVA_HIP_OK;
-  if (scratch) (void)hipFree(scratch);
-  scratch = nullptr;
-  scratch_grids = 0;
-  CRAVA_HIP_CHECK(hipMalloc(&scratch, sizeof(cx<double>) * (size_t)complex_elems() * (size_t)ngrids));
-  scratch_grids = ngrids;
-  return CRAVA_HIP_OK;
-}
 
 template <typename F>
 static int set_lds(F kernel, size_t bytes) {
@@ -589,8 +558,7 @@ static int generic_x(const Fft3D &f, int io, const GridSet &gs, hipStream_t s, b
   switch (io) {
     case kIoF32: return launch_x<float, float, float>(f, gs, s, inv);
     case kIoF64: return launch_x<double, double, double>(f, gs, s, inv);
-    case kIoF32toF64: return launch_x<double, float, double>(f, gs, s, inv);
-    default: return launch_x<double, double, float>(f, gs, s, inv);
+    default: return launch_x<double, float, float>(f, gs, s, inv);
   }
 }
 
@@ -606,46 +574,30 @@ static int generic_strided(const PassPlan &p, const void *tw, int io, const Grid
   switch (io) {
     case kIoF32: return launch_strided<float, float, float>(p, tw, gs, s, inv, es, ncols, nbatch, bstride, dio);
     case kIoF64: return launch_strided<double, double, double>(p, tw, gs, s, inv, es, ncols, nbatch, bstride, dio);
-    case kIoF32toF64:
-      return launch_strided<double, float, double>(p, tw, gs, s, inv, es, ncols, nbatch, bstride, dio);
-    default: return launch_strided<double, double, float>(p, tw, gs, s, inv, es, ncols, nbatch, bstride, dio);
+    default: return launch_strided<double, float, float>(p, tw, gs, s, inv, es, ncols, nbatch, bstride, dio);
   }
 }
 
-// Storage of each pass.  Float and double transforms run in place.  The
-// float-storage transform with double arithmetic keeps the intermediates of
-// grid i in slot i of the plan's double scratch: forward x: grid(f32) ->
-// scratch(f64), y in scratch, z: scratch -> grid(f32); inverse z: grid ->
-// scratch, y in scratch, x: scratch -> grid.
+// Every pass runs in place.  A plan of precision 1 (float storage, double
+// arithmetic) reads float, transforms in double and rounds each output once
+// to float, so the grids stay float between passes and no scratch is needed.
 int Fft3D::make_set(int axis, bool inverse, void *const *grids, int ng, const double *scale, GridSet &gs,
                     int &io) const {
+  (void)axis;
+  (void)inverse;
   if (ng < 1 || ng > kMaxGrids) {
     set_error("a transform set holds 1.." + std::to_string(kMaxGrids) + " grids");
     return CRAVA_HIP_E_INVALID;
   }
-  if (prec == 1 && ng > scratch_grids) {
-    set_error("FFT scratch holds fewer grids than the set");
-    return CRAVA_HIP_E_STATE;
-  }
-  const bool first = inverse ? axis == 2 : axis == 0;
-  io = prec == 0 ? kIoF32 : prec == 2 ? kIoF64 : axis == 1 ? kIoF64 : first ? kIoF32toF64 : kIoF64toF32;
+  io = prec == 0 ? kIoF32 : prec == 2 ? kIoF64 : kIoF32MathF64;
   gs.n = ng;
   for (int i = 0; i < ng; ++i) {
     if (!grids[i]) {
       set_error("NULL grid in a transform set");
       return CRAVA_HIP_E_INVALID;
     }
-    void *slot = prec == 1 ? static_cast<void *>(static_cast<cx<double> *>(scratch) + (size_t)i * complex_elems())
-                           : grids[i];
-    const void *src = grids[i];
-    void *dst = grids[i];
-    if (prec == 1) {
-      if (axis == 1) src = dst = slot;
-      else if (first) dst = slot;
-      else src = slot;
-    }
-    gs.src[i] = src;
-    gs.dst[i] = dst;
+    gs.src[i] = grids[i];
+    gs.dst[i] = grids[i];
     gs.scale[i] = scale ? scale[i] : 1.0;
   }
   for (int i = ng; i < kMaxGrids; ++i) {
@@ -653,14 +605,6 @@ int Fft3D::make_set(int axis, bool inverse, void *const *grids, int ng, const do
     gs.dst[i] = nullptr;
     gs.scale[i] = 0.0;
   }
-  return CRAVA_HIP_OK;
-}
-
-static int copy_between(int io, const GridSet &gs, long long nelem, hipStream_t s) {
-  // a length-1 axis is the identity; still move the data between storages
-  int rc;
-  for (int i = 0; i < gs.n; ++i)
-    if (gs.src[i] != gs.dst[i] && (rc = launch_convert(io, gs.src[i], gs.dst[i], nelem, s))) return rc;
   return CRAVA_HIP_OK;
 }
 
@@ -677,12 +621,12 @@ int Fft3D::pass(int axis, void *const *grids, int ng, bool inverse, const double
   }
   const FftKind kind = inverse ? kStridedInv : kStridedFwd;
   if (axis == 1) {
-    if (nyp <= 1) return copy_between(io, gs, cnxp * nyp * nzp, s);
+    if (nyp <= 1) return CRAVA_HIP_OK;  // a length-1 axis is the identity
     if (var[1] >= 0) return static_fft_launch(nyp, io, kind, var[1], gs, cnxp, cnxp, nzp, cnxp * nyp, 0, 0, stw[1], s);
     return generic_strided(py, twy, io, gs, s, inverse, cnxp, cnxp, nzp, cnxp * nyp, gdio[1]);
   }
   const long long plane = cnxp * nyp;
-  if (nzp <= 1) return copy_between(io, gs, plane * nzp, s);
+  if (nzp <= 1) return CRAVA_HIP_OK;
   if (var[2] >= 0) return static_fft_launch(nzp, io, kind, var[2], gs, plane, plane, 1, 0, 0, 0, stw[2], s);
   return generic_strided(pz, twz, io, gs, s, inverse, plane, plane, 1, 0, gdio[2]);
 }
@@ -730,7 +674,6 @@ int Fft3D::num_variants(int axis) const {
 // share a variant.
 int Fft3D::autotune(void *const *grids, int ng, hipStream_t s, int reps, float *ms_out) {
   int rc;
-  if ((rc = ensure_scratch(ng))) return rc;
   hipEvent_t e0, e1;
   CRAVA_HIP_CHECK(hipEventCreate(&e0));
   CRAVA_HIP_CHECK(hipEventCreate(&e1));
@@ -769,7 +712,6 @@ int Fft3D::autotune(void *const *grids, int ng, hipStream_t s, int reps, float *
 
 int Fft3D::r2c(void *const *grids, int ng, const double *scale, hipStream_t s) {
   int rc;
-  if ((rc = ensure_scratch(ng))) return rc;
   if ((rc = pass(0, grids, ng, false, scale, s))) return rc;
   if ((rc = pass(1, grids, ng, false, nullptr, s))) return rc;
   return pass(2, grids, ng, false, nullptr, s);
@@ -777,7 +719,6 @@ int Fft3D::r2c(void *const *grids, int ng, const double *scale, hipStream_t s) {
 
 int Fft3D::c2r(void *const *grids, int ng, const double *scale, hipStream_t s) {
   int rc;
-  if ((rc = ensure_scratch(ng))) return rc;
   if ((rc = pass(2, grids, ng, true, nullptr, s))) return rc;
   if ((rc = pass(1, grids, ng, true, nullptr, s))) return rc;
   return pass(0, grids, ng, true, scale, s);

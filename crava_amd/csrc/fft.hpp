@@ -49,12 +49,11 @@ inline unsigned div_mul(int d) {
 // false if n has a prime factor > 7.
 bool factor_radices(int n, PassPlan &p, int order);
 
-// Convert ncomplex values between float and double storage (FftIo code).
-int launch_convert(int io, const void *src, void *dst, long long ncomplex, hipStream_t s);
-
 // Scale of a transform (src/fftgrid.cpp:1817,1848-1859): forward 1/sqrt(N)
 // except covariance grids (unscaled); inverse 1/sqrt(N), covariance 1/N.
-// Float transforms round the factor to float as FFTGrid does.
+// Float transforms round the factor to float as FFTGrid does; transforms with
+// double arithmetic (f64) keep it in double, so that a forward + inverse pair
+// of DATA transforms scales by 1/N to double accuracy.
 inline double grid_fft_scale(bool covariance, bool f64, long long N, bool inverse) {
   if (!inverse) {
     if (covariance) return 1.0;
@@ -69,7 +68,8 @@ constexpr int kNumGenericVariants = 6;
 
 class Fft3D {
  public:
-  // prec: 0 float, 1 float storage with double arithmetic, 2 double.
+  // prec: 0 float, 1 float storage with double arithmetic inside each pass
+  // (every pass rounds its outputs once to float), 2 double.
   Fft3D(int nxp, int nyp, int nzp, int prec);
   ~Fft3D();
   int init();  // CRAVA_HIP_OK or an error code
@@ -82,8 +82,6 @@ class Fft3D {
   int c2r(void *grid, hipStream_t s, double scale) { return c2r(&grid, 1, &scale, s); }
   // One pass (axis 0 x rows, 1 y, 2 z) over a set; `scale` applies on the x pass.
   int pass(int axis, void *const *grids, int ng, bool inverse, const double *scale, hipStream_t s);
-  // Make the double scratch of a prec-1 plan hold `ngrids` grids (no-op otherwise).
-  int ensure_scratch(int ngrids);
   long long complex_elems() const;
 
   int nxp, nyp, nzp;
@@ -92,8 +90,6 @@ class Fft3D {
   PassPlan px, py, pz;
   void *twx = nullptr, *twy = nullptr, *twz = nullptr;  // device twiddle tables
   void *stw[3] = {nullptr, nullptr, nullptr};  // per-pass tables of the specialised kernels, per axis
-  void *scratch = nullptr;  // double intermediates of a prec-1 plan, one slot per grid
-  int scratch_grids = 0;
   // Launch variant per axis (x, y, z): v >= 0 specialised kernel variant v,
   // v < 0 generic kernel variant -1 - v.
   int var[3] = {-1, -1, -1};

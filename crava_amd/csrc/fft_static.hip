@@ -324,21 +324,16 @@ static int all_kinds(FftKind kind, int v, const GridSet &gs, long long es, long 
   }
 }
 
-// Storage combinations: float, double, and the two halves of a float-storage
-// transform with double arithmetic and double intermediates: forward = rows
-// float->double, strided double->double, strided double->float; inverse =
-// strided float->double, strided double->double, rows double->float.
+// Storage x arithmetic: float, double, and float storage with double
+// arithmetic (pencils, twiddles and butterflies in double inside the pass;
+// one rounding to float per output).
 template <int N>
 static int launch_store(int io, FftKind kind, int v, const GridSet &gs, long long es, long long ncols,
                         long long nbatch, long long bstride, long long nrows, int rnxp, const void *tw, hipStream_t s) {
   if (io == kIoF32) return all_kinds<N, float, float>(kind, v, gs, es, ncols, nbatch, bstride, nrows, rnxp, tw, s);
   if (io == kIoF64) return all_kinds<N, double, double>(kind, v, gs, es, ncols, nbatch, bstride, nrows, rnxp, tw, s);
-  if (io == kIoF32toF64 && kind == kRowsR2C) return rows_any<N, false, double, float, double>(v, gs, nrows, rnxp, tw, s);
-  if (io == kIoF32toF64 && kind == kStridedInv)
-    return strided_any<N, 1, double, float, double>(v, gs, es, ncols, nbatch, bstride, tw, s);
-  if (io == kIoF64toF32 && kind == kStridedFwd)
-    return strided_any<N, -1, double, double, float>(v, gs, es, ncols, nbatch, bstride, tw, s);
-  if (io == kIoF64toF32 && kind == kRowsC2R) return rows_any<N, true, double, double, float>(v, gs, nrows, rnxp, tw, s);
+  if (io == kIoF32MathF64)
+    return all_kinds<N, double, float>(kind, v, gs, es, ncols, nbatch, bstride, nrows, rnxp, tw, s);
   set_error("unsupported storage combination for this pass");
   return CRAVA_HIP_E_INVALID;
 }

@@ -34,8 +34,9 @@ struct GridSet {
   int n;
 };
 
-// Storage of a pass's input/output (arithmetic is double unless kIoF32).
-enum FftIo { kIoF32 = 0, kIoF64 = 1, kIoF32toF64 = 2, kIoF64toF32 = 3 };
+// Storage and arithmetic of a pass: float, double, or float storage with
+// double arithmetic inside the pass (each pass rounds its outputs once).
+enum FftIo { kIoF32 = 0, kIoF64 = 1, kIoF32MathF64 = 2 };
 
 // Launch a specialised pass over every grid of `gs` (src -> dst, may alias
 // for in place).  Strided: complex, `es` element stride, `ncols` contiguous

@@ -43,16 +43,32 @@ struct PostProblem {
 void build_perk(const PostProblem &pp, const unsigned char *invert, const float *kWD, const float *e1,
                 const float *e2, const float *e3, PerK<float> *tf, PerK<double> *td);
 
+// Queue of the mixed form: the 64-point waves that the FP32 kernel leaves to
+// the FP64 kernel, as {k-plane, first point of the wave}, and their count.
+struct DeferQueue {
+  int2 *list = nullptr;        // device, cap entries
+  unsigned *count = nullptr;   // device
+  long long cap = 0;
+};
+
+// Entries a mixed posterior of this problem can queue: every in-band wave.
+long long defer_capacity(const PostProblem &pp, const unsigned char *invert);
+
 struct PostDevice {
   PostProblem pp;
   PerK<float> *tabf = nullptr;   // device, nzp entries
   PerK<double> *tabd = nullptr;  // device, nzp entries
+  long long defer_need = 0;      // defer_capacity of the problem
 };
 
-// precision: 0 = FP32 (grids complex64), 1 = mixed (complex64 grids, FP64 per
-// point below the noise/signal threshold), 2 = FP64 (complex128 grids).
+// precision: 0 = FP32 (grids complex64, the reference's float algorithm),
+// 1 = mixed (complex64 grids; FP32 square-root information solve, and the
+// FP64 reference-form solve for every wave that holds a point below the
+// noise/signal threshold or whose FP32 factorisations fail; `defer` must
+// hold defer_need entries), 2 = FP64 (complex128 grids).  Launches: FP32 /
+// FP64: one; mixed: queue reset, FP32 kernel, FP64 kernel over the queue.
 int launch_posterior(const PostDevice &pd, int precision, void *const *mu, void *const *data, void *const *cov,
-                     const void *errcorr, unsigned long long *counters, hipStream_t s);
+                     const void *errcorr, unsigned long long *counters, const DeferQueue *defer, hipStream_t s);
 
 // One volume of a batched posterior: its problem and device grids.
 struct PostVolume {
@@ -78,6 +94,7 @@ struct PostBatch {
   int precision = 0, nvol = 0, ppb = 0, blocks = 0;
   int form = kFormPerClass;
   void *args = nullptr;       // device PostArgs per volume
+  void *queues = nullptr;     // mixed form: storage of every volume's queue
   int4 *tiles = nullptr;      // device {volume, k, first point, ntheta}
   long long ntiles = 0;
   long long seg[kMaxTheta + 1] = {0};  // tiles of ntheta c: [seg[c-1], seg[c])
@@ -88,6 +105,13 @@ struct PostBatch {
   int set_launch(int points_per_block, int blocks);
   int num_classes() const;
   int launch(const hipStream_t *streams, int nstreams) const;
+  // Mixed form around launch(): clear every volume's queue before it, and run
+  // the FP64 kernel over every queue after it.
+  int reset_queues(hipStream_t s) const;
+  int launch_queued(hipStream_t s) const;
+  // Mixed form: the queue of each volume (each volume has its own, even
+  // when volumes share a problem).
+  std::vector<DeferQueue> vq;
 };
 
 }  // namespace crava

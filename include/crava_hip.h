@@ -35,7 +35,8 @@ extern "C" {
 
 /* arithmetic of a posterior / transform call */
 #define CRAVA_HIP_F32 0   /* float grids, float solve (the reference's arithmetic) */
-#define CRAVA_HIP_MIXED 1 /* float grids, FP64 solve below the noise/signal threshold */
+#define CRAVA_HIP_MIXED 1 /* float grids; transforms with double arithmetic inside each pass;
+                             FP32 square-root solve, FP64 below the noise/signal threshold */
 #define CRAVA_HIP_F64 2   /* double grids, double transforms and solve */
 
 typedef struct crava_hip_ctx crava_hip_ctx;
@@ -79,9 +80,9 @@ int crava_hip_grid_read_trace(crava_hip_ctx *ctx, const void *grid, int nxp, int
  * Replaces FFTGrid::fftInPlace (src/fftgrid.cpp:1804-1829): grids that are
  * not CRAVA_HIP_COVARIANCE are scaled by 1/sqrt(nxp*nyp*nzp); FFTW's forward
  * sign (-1).  precision: CRAVA_HIP_F32 (float grid, float arithmetic),
- * CRAVA_HIP_MIXED (float grid, double arithmetic: every output is rounded
- * once, so tiny spectral values keep their relative accuracy) or
- * CRAVA_HIP_F64 (double grid). */
+ * CRAVA_HIP_MIXED (float grid, double arithmetic inside each of the three
+ * passes, whose outputs are rounded once to float; scale factors in double)
+ * or CRAVA_HIP_F64 (double grid). */
 int crava_hip_fft_r2c(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int cubetype, int precision);
 
 /* In-place 3-D half-complex-to-real transform.  Replaces
@@ -325,7 +326,8 @@ typedef struct {
  * (main.cpp:279-308).  Per volume, as Crava::computePostMeanResidAndFFTCov
  * (src/crava.cpp:887-1204): forward transforms of data and mean (scaled as
  * DATA/PARAMETER) and of the covariance and error-correlation grids
- * (COVARIANCE; double arithmetic for CRAVA_HIP_MIXED), the posterior, then
+ * (COVARIANCE; CRAVA_HIP_MIXED transforms the means forward and back and the
+ * covariances forward with double arithmetic in each pass), the posterior, then
  * inverse transforms of mean, residuals and covariances.  All transforms of
  * one volume and kind are one launch per pass; the posterior of every volume
  * is one persistent launch.  The grids are registered here and must stay

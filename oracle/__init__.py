@@ -20,6 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libcrava_ref.so")
+REF_RANDOM_SO = os.path.join(HERE, "_ref", "libcrava_ref_random.so")
 
 _orc = None
 _ref = None
@@ -112,6 +113,46 @@ def ref():
         L.ref_chol_cpx.restype = C.c_int
         _ref = L
     return _ref
+
+
+_ref_random = None
+
+
+def ref_random_available() -> bool:
+    return os.path.exists(REF_RANDOM_SO)
+
+
+def _random_lib():
+    global _ref_random
+    if _ref_random is None:
+        L = C.CDLL(REF_RANDOM_SO)
+        L.ref_rnorm01_stream.argtypes = [C.c_uint32, C.c_longlong, VP]
+        L.ref_unif01_stream.argtypes = [C.c_uint32, C.c_longlong, VP]
+        L.ref_fill_complex_noise.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, VP]
+        _ref_random = L
+    return _ref_random
+
+
+def ref_rnorm01_stream(seed: int, n: int) -> np.ndarray:
+    """n draws of the reference's own RandomGen::rnorm01 from `seed`."""
+    out = np.zeros(n, np.float64)
+    _random_lib().ref_rnorm01_stream(seed, n, out.ctypes.data)
+    return out
+
+
+def ref_unif01_stream(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float64)
+    _random_lib().ref_unif01_stream(seed, n, out.ctypes.data)
+    return out
+
+
+def ref_fill_complex_noise(nxp, nyp, nzp, seed: int, extra: int = 0):
+    """FFTGrid::fillInComplexNoise driven by the reference's RandomGen; also
+    the next `extra` rnorm01 draws of the stream."""
+    g = np.zeros((nzp, nyp, 2 * (nxp // 2 + 1)), np.float32)
+    after = np.zeros(max(extra, 1), np.float64)
+    _random_lib().ref_fill_complex_noise(g.ctypes.data, nxp, nyp, nzp, seed, extra, after.ctypes.data)
+    return g, after[:extra]
 
 
 def _ptrs(arrs):

@@ -126,9 +126,11 @@ def test_every_launch_variant_and_autotuner(ctx, n, dbl):
 @pytest.mark.parametrize("n", [(80, 80, 160), (15, 12, 24), (512, 8, 320), (6, 1, 5), (4, 7, 1)],
                          ids=lambda n: "x".join(map(str, n)))
 def test_r2c_float_storage_double_arithmetic(ctx, n):
-    """CRAVA_HIP_MIXED transforms round each output once: every spectral value,
-    including values 1e-9 below the peak, keeps float relative accuracy
-    against the double-arithmetic oracle."""
+    """CRAVA_HIP_MIXED transforms run double arithmetic inside each pass and
+    round each pass's outputs once to float: three roundings of at most
+    2^-24 relative each, so the spectrum is within 3e-7 (normwise) of the
+    double-arithmetic oracle and no worse than the float transform; the
+    inverse returns the input to the same accuracy."""
     nxp, nyp, nzp = n
     print(f"fft mixed {n}", flush=True)
     rnxp = 2 * (nxp // 2 + 1)
@@ -142,8 +144,10 @@ def test_r2c_float_storage_double_arithmetic(ctx, n):
     oracle.fft_r2c(ref, nxp, nyp, nzp, True)
     from crava_amd.core import set_fft_variants
     want = to_complex(ref)
-    big = np.abs(want).max()
-    mask = np.abs(want) > 1e-9 * big
+    t32 = torch.from_numpy(g.copy()).cuda()
+    ctx.r2c(t32, nxp, nyp, nzp, L.COVARIANCE, L.F32)
+    torch.cuda.synchronize()
+    err32 = rel_l2(to_complex(t32.cpu().numpy()), want)
     for v in (None, 0, 3, 4, 7):
         if v is not None:
             try:
@@ -153,14 +157,13 @@ def test_r2c_float_storage_double_arithmetic(ctx, n):
         t = torch.from_numpy(g.copy()).cuda()
         ctx.r2c(t, nxp, nyp, nzp, L.COVARIANCE, L.MIXED)
         torch.cuda.synchronize()
-        got = to_complex(t.cpu().numpy())
-        relerr = np.abs(got[mask] - want[mask]) / np.abs(want[mask])
-        assert np.percentile(relerr, 99) < 1e-6 and relerr.max() < 1e-4, v
+        err = rel_l2(to_complex(t.cpu().numpy()), want)
+        assert err < 3e-7 and err <= max(err32, 1e-7), (v, err, err32)
         # inverse with double arithmetic (COVARIANCE: forward unscaled,
         # inverse 1/N): the round trip returns the input
         ctx.c2r(t, nxp, nyp, nzp, L.COVARIANCE, L.MIXED)
         torch.cuda.synchronize()
-        assert rel_l2(t.cpu().numpy()[:, :, :nxp], g[:, :, :nxp]) < 1e-6, v
+        assert rel_l2(t.cpu().numpy()[:, :, :nxp], g[:, :, :nxp]) < 3e-7, v
 
 
 @pytest.mark.gpu

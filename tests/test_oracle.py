@@ -380,6 +380,63 @@ def test_rnorm01_moments():
     assert abs(np.mean(z ** 4) - 3) < 0.06
 
 
+needs_ref_random = pytest.mark.skipif(not oracle.ref_random_available(),
+                                      reason="oracle/_ref/libcrava_ref_random.so not built (reference sources absent)")
+
+
+@needs_ref_random
+@pytest.mark.parametrize("seed", [23665, 1, 12345, 2 ** 32 - 1])
+def test_rnorm01_and_unif01_bitexact_vs_reference_randomgen(seed):
+    """The oracle's RandomGen restatement against the reference's own
+    libs/lib/random.cpp: 200 000 normals (about 540 from the tail branch,
+    u >= 0.9973) and the uniforms, bit for bit."""
+    import ctypes as C
+    z, _ = oracle.rnorm01_stream(seed, 200000)
+    assert np.array_equal(z, oracle.ref_rnorm01_stream(seed, 200000))
+    s = C.c_uint32(seed)
+    u = np.array([oracle.orc().orc_unif01(C.byref(s)) for _ in range(1000)])
+    assert np.array_equal(u, oracle.ref_unif01_stream(seed, 1000))
+
+
+@needs_ref_random
+@pytest.mark.parametrize("n", [(10, 6, 8), (15, 12, 24), (16, 5, 7), (9, 1, 1)], ids=lambda n: "x".join(map(str, n)))
+def test_complex_noise_bitexact_vs_reference_randomgen(n):
+    """fillInComplexNoise with the oracle's generator equals the same fill
+    driven by the reference's RandomGen, bit for bit, and leaves the stream
+    at the same state (the next draws agree)."""
+    nxp, nyp, nzp = n
+    for seed in (99, 23665):
+        g, s_after = oracle.fill_complex_noise(nxp, nyp, nzp, seed)
+        r, after = oracle.ref_fill_complex_noise(nxp, nyp, nzp, seed, extra=7)
+        assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
+        nxt, _ = oracle.rnorm01_stream(s_after, 7)
+        assert np.array_equal(nxt, after)
+
+
+def test_simulate_points_bitexact_vs_reference_lib_matr():
+    """The oracle's Crava::simulate point loop against the same loop on the
+    reference's own lib_matrCholCpx / lib_matrProdCholVec (oracle/_ref), on
+    Hermitian positive definite covariances and on some that are not."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    rng = np.random.default_rng(4)
+    npts = 4000
+    B = rng.standard_normal((npts, 3, 3)) + 1j * rng.standard_normal((npts, 3, 3))
+    Sg = B @ np.conj(np.swapaxes(B, 1, 2)) * 1e-3
+    Sg[::50] *= -1.0                                   # not positive definite: failure branch
+    Sg[1::97, 2, 2] = 0.0                              # singular pivots
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    cov = [np.ascontiguousarray(Sg[:, a, b]).astype(np.complex64) for a, b in pairs]
+    seeds = [(rng.standard_normal(npts) + 1j * rng.standard_normal(npts)).astype(np.complex64) for _ in range(3)]
+    s1 = [s.copy() for s in seeds]
+    s2 = [s.copy() for s in seeds]
+    n1 = oracle.simulate_points([c.copy() for c in cov], s1)
+    n2 = oracle.simulate_points([c.copy() for c in cov], s2, which="ref")
+    assert n1 == n2 and n1 > 0
+    for a, b in zip(s1, s2):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_complex_noise_hermitian_plane():
     nxp, nyp, nzp = 10, 6, 8
     g, _ = oracle.fill_complex_noise(nxp, nyp, nzp, 99)
