@@ -10,10 +10,10 @@ posterior path of Crava::computePostMeanResidAndFFTCov with its transforms:
   + the per-point posterior over all P = 257*512*320 Fourier points
   + 15 inverse c2r (3 posterior means, 6 posterior covariances, 6 residuals)
 
-run twice: the MIXED form (float grids; data transforms in float, mean and
-covariance transforms with double arithmetic inside each pass; FP32
-square-root solve, FP64 solve for the waves holding a point with
-noise/signal < threshold) and the plain FP64 form (double grids, double
+run twice: the MIXED form (float grids; covariance transforms forward with
+double arithmetic inside each pass, mean transforms in float with mean
+deflation, the rest in float; FP32 square-root solve, FP64 solve for the
+waves holding a point with noise/signal < threshold) and the plain FP64 form (double grids, double
 transforms, double solve).  Each form's step runs two ways: as a one-volume
 crava_hip_batch replayed as a hipGraph (the data/mean and covariance
 transform sets on two streams) and as separate launches on one stream;
@@ -136,8 +136,9 @@ class Pipeline:
         self.corr_xy = torch.from_numpy(np.ascontiguousarray(prob.corr_xy, dtype=np.float32)).cuda()
         self.circ_t = torch.from_numpy(np.ascontiguousarray(prob.circ_corr_t, dtype=np.float32)).cuda()
         self.precision = L.F64 if fp64 else L.MIXED
-        # transform precision of the data grids, and of the means and covariances
+        # transform precision of the data grids, of the means and of the covariances
         self.data_precision = L.F64 if fp64 else L.F32
+        self.mean_precision = L.F64 if fp64 else L.F32_DEFLATE
         self.cov_precision = L.F64 if fp64 else L.MIXED
         self.L = L
         self.batch = None
@@ -174,26 +175,29 @@ class Pipeline:
                 return nf[0], n64[0]
             return None
         # data and prior means share the 1/sqrt(N) scaling (DATA / PARAMETER);
-        # the mixed form transforms the means (both ways) and the covariances
-        # (forward) with double arithmetic inside each pass (float storage):
-        # the mean update and the small prior eigenvalues need it; the data,
-        # the error correlation and the posterior covariances do not
+        # the mixed form transforms the covariances forward with double
+        # arithmetic inside each pass (float storage: the small prior
+        # eigenvalues need it), the means in float with mean deflation (the
+        # mean update needs the accuracy of their variation, not of their
+        # level) and the rest in float
         self.ctx.r2c_many(self.data, *n, L.DATA, self.data_precision)
         self.ctx.r2c(self.ec, *n, L.COVARIANCE, self.data_precision)
-        self.ctx.r2c_many(self.mu, *n, L.DATA, self.cov_precision)
+        self.ctx.r2c_many(self.mu, *n, L.DATA, self.mean_precision)
         self.ctx.r2c_many(self.cov, *n, L.COVARIANCE, self.cov_precision)
         r = self.post.run(self.precision, self.mu, self.data, self.cov, self.ec, counts=counts)
-        self.ctx.c2r_many(self.mu, *n, L.DATA, self.cov_precision)
+        self.ctx.c2r_many(self.mu, *n, L.DATA, self.mean_precision)
         self.ctx.c2r_many(self.data, *n, L.DATA, self.data_precision)
         self.ctx.c2r_many(self.cov, *n, L.COVARIANCE, self.data_precision)
         return r
 
     def reset(self):
-        """Reload the inputs with the library's grid copy (the plain copy the
-        HBM counters are calibrated against)."""
+        """Reload the inputs, the error correlation included, with the
+        library's grid copy (the plain copy the HBM counters are calibrated
+        against)."""
         from crava_amd.core import grid_op
         c = self.prob.cfg
-        for dst, src in zip(self.mu + self.data + self.cov, self.prob.mu + self.prob.data + self.prob.cov):
+        for dst, src in zip(self.mu + self.data + self.cov + [self.ec],
+                            self.prob.mu + self.prob.data + self.prob.cov + [self.prob.errcorr]):
             if self.fp64:
                 dst.copy_(src)
             else:
@@ -228,11 +232,65 @@ def timed_kernel(fn, reps, setup=None):
     return sum(s.elapsed_time(e) for s, e in ev) / reps
 
 
+def cpu_info():
+    """CPU model (from /proc/cpuinfo), logical CPUs and the CPUs this process may use."""
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
+
+
+def cpu_end_to_end_small():
+    """The reference CPU path end to end on one core at the CPU-runnable
+    configuration (64x64x128 cells, FFT grid 80x80x160, 3 angles): 16
+    forward transforms, the per-point loop over all P points, 15 inverse
+    transforms, timed as one wall-clock interval."""
+    import oracle
+    from crava_amd import synth
+    from crava_amd.core import problem_tables
+    kind = "reference" if oracle.ref_available() else "port"
+    which = "ref" if kind == "reference" else "oracle"
+    prob = synth.make_problem(synth.config("plain_cpu"), "cpu")
+    cfg = prob.cfg
+    n = (cfg.nxp, cfg.nyp, cfg.nzp)
+    tabs = problem_tables(prob)
+    mu = [g.numpy().copy() for g in prob.mu]
+    d = [g.numpy().copy() for g in prob.data]
+    cv = [g.numpy().copy() for g in prob.cov]
+    ec = prob.errcorr.numpy().copy()
+    t0 = time.perf_counter()
+    for g in d + mu:
+        oracle.fft_r2c(g, *n, False, which)
+    for g in cv + [ec]:
+        oracle.fft_r2c(g, *n, True, which)
+    oracle.posterior(*n, prob.A, prob.err_theta_cov, cfg.wnc, prob.sigma0, tabs, mu, d, cv, ec, which)
+    for g in mu + d:
+        oracle.fft_c2r(g, *n, False, which)
+    for g in cv:
+        oracle.fft_c2r(g, *n, True, which)
+    dt = time.perf_counter() - t0
+    P = (cfg.nxp // 2 + 1) * cfg.nyp * cfg.nzp
+    return {"workload": f"posterior path {cfg.nxp}x{cfg.nyp}x{cfg.nzp} FFT grid, ntheta={prob.ntheta}",
+            "seconds": dt, "value": P / dt, "unit": "Fourier pts/s", "cores": 1, "kind": kind,
+            "sample": "the whole path, timed end to end (not composed)"}
+
+
 def cpu_baseline(prob, tabs, budget_s):
     """Reference CPU path (FFTW 2.1.2 float + lib_matr.c built from the
     reference sources) on one core, on a bounded sample: one full-size 3-D
     r2c, one c2r, and the per-point loop over a few in-band and out-of-band
-    k-planes; the step time is composed from the measured per-unit costs."""
+    k-planes; the step time is composed from the measured per-unit costs
+    (labelled "composed").  The same path is also timed end to end at the
+    CPU-runnable configuration."""
     import numpy as np
     import oracle
     cfg = prob.cfg
@@ -288,11 +346,13 @@ def cpu_baseline(prob, tabs, budget_s):
         "unit": "Fourier pts/s",
         "cores": 1,
         "kind": kind,
-        "sample": (f"1 full-grid r2c ({t_r2c:.2f}s) + 1 c2r ({t_c2r:.2f}s) of {nxp}x{nyp}x{nzp} and the per-point "
-                   f"loop over up to 4 in-band + 4 out-of-band k-planes ({per_pt['in']*1e6:.2f}/"
+        "sample": (f"composed: 1 full-grid r2c ({t_r2c:.2f}s) + 1 c2r ({t_c2r:.2f}s) of {nxp}x{nyp}x{nzp} and the "
+                   f"per-point loop over up to 4 in-band + 4 out-of-band k-planes ({per_pt['in']*1e6:.2f}/"
                    f"{per_pt['out']*1e6:.3f} us per point); step = 16 r2c + 15 c2r + loop over all P points"),
+        "composed": True,
         "seconds_per_step": t_step,
-        "cpu": platform.processor() or platform.machine(),
+        "cpu": cpu_info(),
+        "end_to_end_small": cpu_end_to_end_small(),
     }
 
 
@@ -838,7 +898,7 @@ def main():
     mixed.reset()
     ctx.r2c_many(mixed.data, *n3, L.DATA)
     ctx.r2c(mixed.ec, *n3, L.COVARIANCE)
-    ctx.r2c_many(mixed.mu, *n3, L.DATA, L.MIXED)
+    ctx.r2c_many(mixed.mu, *n3, L.DATA, L.F32_DEFLATE)
     ctx.r2c_many(mixed.cov, *n3, L.COVARIANCE, L.MIXED)
 
     def restorer(pipe):

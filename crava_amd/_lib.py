@@ -21,6 +21,7 @@ DATA, PARAMETER, COVARIANCE = 0, 1, 2
 OP_ADD, OP_SUBTRACT, OP_MULTIPLY, OP_CHANGE_SIGN, OP_CONJUGATE, OP_MULTIPLY_SCALAR, OP_ADD_SCALAR, \
     OP_FILL_CONSTANT, OP_COPY = range(9)
 F32, MIXED, F64 = 0, 1, 2
+F32_DEFLATE = 3   # transforms only: float arithmetic with mean deflation
 
 _lib = None
 

@@ -4,12 +4,12 @@
 // Crava::computePostMeanResidAndFFTCov (src/crava.cpp:887-1204).
 //
 // Volumes are spread over a few side streams forked from the origin stream,
-// so small volumes overlap.  Within a volume the data transforms and the
-// mean/covariance transforms go to two different streams: in the mixed form
-// the latter run double arithmetic inside each pass while the float data
-// transforms stream HBM, so the two overlap.  Volumes of one grid shape
-// share transform plans, so each kind of set of a shape always goes to the
-// same stream.
+// so small volumes overlap.  Within a volume the two transform sets of each
+// direction go to two different streams: in the mixed form the covariance
+// transforms run double arithmetic inside each pass while the float
+// transforms of the other set stream HBM, so the two overlap.  Volumes of
+// one grid shape share transform plans, so each kind of set of a shape
+// always goes to the same stream.
 #include "batch.hpp"
 #include "../../include/crava_hip.h"
 
@@ -37,6 +37,7 @@ BatchExec::~BatchExec() {
   if (fork_ev_) (void)hipEventDestroy(fork_ev_);
   if (cap_) (void)hipStreamDestroy(cap_);
   if (counters_) (void)hipFree(counters_);
+  if (dcbufs_) (void)hipFree(dcbufs_);
   if (backup_) (void)hipFree(backup_);
 }
 
@@ -71,6 +72,7 @@ int BatchExec::init(const BatchVolume *vols, int n) {
       if (!bv.data[l]) { set_error("volume " + std::to_string(v) + ": NULL data grid"); return CRAVA_HIP_E_INVALID; }
   }
   CRAVA_HIP_CHECK(hipMalloc(&counters_, sizeof(unsigned long long) * 2 * n));
+  CRAVA_HIP_CHECK(hipMalloc(&dcbufs_, sizeof(double) * 4 * kMaxGrids * (size_t)n));
   int rc;
   if ((rc = build_sets())) return rc;
   std::vector<PostVolume> pv(n);
@@ -93,18 +95,20 @@ int BatchExec::init(const BatchVolume *vols, int n) {
 }
 
 // Transform sets per volume and the stream of each volume.  Forward: [0] the
-// data (DATA scaling) and the error correlation (COVARIANCE), [1] the means
-// (DATA) and the covariances (COVARIANCE).  Inverse: [0] the residuals and
+// data (DATA scaling), the error correlation (COVARIANCE) and the means
+// (DATA); [1] the covariances (COVARIANCE).  Inverse: [0] the residuals and
 // the covariances, [1] the means.  Plan precision: the float form uses float
-// transforms throughout; the mixed form runs set [1] with double arithmetic
-// inside each pass (float storage) -- the posterior mean update needs it for
-// the means, the small prior eigenvalues for the covariances -- and set [0]
-// in float; the double form uses double transforms.
+// transforms throughout; the mixed form transforms the covariances forward
+// with double arithmetic inside each pass (float storage: the small prior
+// eigenvalues need it), the means in float with mean deflation (the
+// posterior mean update needs the accuracy of the means' variation, not of
+// their level) and the rest in float; the double form uses double
+// transforms.
 int BatchExec::build_sets() {
   const bool f64 = precision_ == CRAVA_HIP_F64;
   const int pa = f64 ? 2 : 0;
   const int pc = f64 ? 2 : (precision_ == CRAVA_HIP_MIXED ? 1 : 0);
-  const bool dc = pc != 0;   // set [1] keeps double scale factors
+  const bool dc = pc != 0;   // the covariance set keeps double scale factors
   work_.assign(vols_.size(), Work{});
   std::map<std::tuple<int, int, int>, std::vector<int>> by_shape;
   for (size_t v = 0; v < vols_.size(); ++v) {
@@ -117,9 +121,11 @@ int BatchExec::build_sets() {
     if ((rc = ctx_->plan(pp.nxp, pp.nyp, pp.nzp, pa, &fa->plan))) return rc;
     if ((rc = ctx_->plan(pp.nxp, pp.nyp, pp.nzp, pc, &fc->plan))) return rc;
     ia->plan = fa->plan;
-    ic->plan = fc->plan;
+    ic->plan = fa->plan;
+    for (int i = 0; i < 4; ++i) (i < 2 ? w.fwd[i] : w.inv[i - 2]).dc = dcbufs_ + ((size_t)v * 4 + i) * kMaxGrids;
+    const bool deflate = precision_ == CRAVA_HIP_MIXED;
     const double sf = grid_fft_scale(false, f64, N, false), si = grid_fft_scale(false, f64, N, true);
-    const double mf = grid_fft_scale(false, dc, N, false), mi = grid_fft_scale(false, dc, N, true);
+    const double xf = grid_fft_scale(false, true, N, false), xi = grid_fft_scale(false, true, N, true);
     const double cf = grid_fft_scale(true, dc, N, false);
     const double ef = grid_fft_scale(true, f64, N, false), ei = grid_fft_scale(true, f64, N, true);
     for (int l = 0; l < pp.ntheta; ++l) {
@@ -131,10 +137,16 @@ int BatchExec::build_sets() {
     fa->grids[fa->ng] = bv.errcorr;
     fa->scale[fa->ng++] = ef;
     for (int c = 0; c < 3; ++c) {
-      fc->grids[fc->ng] = bv.mu[c];
-      fc->scale[fc->ng++] = mf;
+      if (deflate) {
+        fa->dcmask |= 1u << fa->ng;
+        ic->dcmask |= 1u << ic->ng;
+      }
+      fa->exact[fa->ng] = xf;
+      fa->grids[fa->ng] = bv.mu[c];
+      fa->scale[fa->ng++] = sf;
+      ic->exact[ic->ng] = xi;
       ic->grids[ic->ng] = bv.mu[c];
-      ic->scale[ic->ng++] = mi;
+      ic->scale[ic->ng++] = si;
     }
     for (int q = 0; q < 6; ++q) {
       fc->grids[fc->ng] = bv.cov[q];
@@ -192,7 +204,8 @@ int BatchExec::enqueue(hipStream_t origin, bool transforms_only, bool forward_on
   for (int v = 0; v < n; ++v)
     for (int i = 0; i < 2; ++i) {
       const Set &st = work_[v].fwd[i];
-      if ((rc = st.plan->r2c(st.grids, st.ng, st.scale, side_[work_[v].stream[i]]))) return rc;
+      if ((rc = st.plan->r2c(st.grids, st.ng, st.scale, side_[work_[v].stream[i]], st.dc, st.dcmask, st.exact)))
+        return rc;
     }
   if ((rc = join(origin))) return rc;
   if (forward_only) return CRAVA_HIP_OK;
@@ -201,7 +214,8 @@ int BatchExec::enqueue(hipStream_t origin, bool transforms_only, bool forward_on
   for (int v = 0; v < n; ++v)
     for (int i = 0; i < 2; ++i) {
       const Set &st = work_[v].inv[i];
-      if ((rc = st.plan->c2r(st.grids, st.ng, st.scale, side_[work_[v].stream[i]]))) return rc;
+      if ((rc = st.plan->c2r(st.grids, st.ng, st.scale, side_[work_[v].stream[i]], st.dc, st.dcmask, st.exact)))
+        return rc;
     }
   return join(origin);
 }

@@ -43,10 +43,13 @@ class BatchExec {
     int ng = 0;
     void *grids[kMaxGrids] = {nullptr};
     double scale[kMaxGrids] = {0};
+    double exact[kMaxGrids] = {0};  // level scales of the mean-deflated grids
+    unsigned dcmask = 0;             // grids transformed with mean deflation
+    double *dc = nullptr;            // device, kMaxGrids levels
   };
   struct Work {
-    Set fwd[2], inv[2];   // fwd: [0] data + error correlation, [1] means + covariances;
-                          // inv: [0] residuals + covariances, [1] means
+    Set fwd[2], inv[2];   // fwd: [0] data, error correlation, means; [1] covariances;
+                          // inv: [0] residuals, covariances; [1] means
     int stream[2] = {0, 0};
   };
   int build_sets();
@@ -63,6 +66,7 @@ class BatchExec {
   std::vector<Work> work_;
   PostBatch post_;
   unsigned long long *counters_ = nullptr;  // [nvol][2]
+  double *dcbufs_ = nullptr;                 // [nvol][4][kMaxGrids] deflation levels
   hipStream_t side_[kStreams] = {nullptr};
   hipEvent_t fork_ev_ = nullptr;
   hipEvent_t join_ev_[kStreams] = {nullptr};

@@ -58,6 +58,7 @@ int Context::workspace(size_t bytes, void **out) {
 Context::~Context() {
   plans.clear();
   if (counters) (void)hipFree(counters);
+  if (dcbuf) (void)hipFree(dcbuf);
   if (work) (void)hipFree(work);
 }
 }  // namespace crava
@@ -86,6 +87,11 @@ static int fft_prec(int precision) {
   return precision == CRAVA_HIP_F64 ? 2 : (precision == CRAVA_HIP_MIXED ? 1 : 0);
 }
 
+// exact (double) scale of a transform, the level scale of a deflated one
+static double fft_scale_exact(int cubetype, long long N, bool inverse) {
+  return crava::grid_fft_scale(cubetype == CRAVA_HIP_COVARIANCE, true, N, inverse);
+}
+
 extern "C" {
 
 int crava_hip_version(void) { return 100; }
@@ -99,8 +105,9 @@ int crava_hip_ctx_create(crava_hip_ctx **ctx, void *stream) {
   if (!c) { set_error("out of host memory"); return CRAVA_HIP_E_OOM; }
   c->stream = reinterpret_cast<hipStream_t>(stream);
   hipError_t e = hipMalloc(&c->counters, 4 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc(&c->dcbuf, crava::kMaxGrids * sizeof(double));
   if (e != hipSuccess) {
-    set_error(std::string("hipMalloc(counters): ") + hipGetErrorString(e));
+    set_error(std::string("hipMalloc(context buffers): ") + hipGetErrorString(e));
     delete c;
     return CRAVA_HIP_E_HIP;
   }
@@ -127,10 +134,11 @@ int crava_hip_synchronize(crava_hip_ctx *ctx) {
   return CRAVA_HIP_OK;
 }
 
-// Float transforms round the factor to float (as FFTGrid); the mixed and
-// double transforms keep it in double.
+// Float transforms (deflated or not) round the factor to float as FFTGrid
+// does; the mixed and double transforms keep it in double.
 static double fft_scale(int cubetype, int precision, long long N, bool inverse) {
-  return crava::grid_fft_scale(cubetype == CRAVA_HIP_COVARIANCE, precision != CRAVA_HIP_F32, N, inverse);
+  const bool f64 = precision == CRAVA_HIP_MIXED || precision == CRAVA_HIP_F64;
+  return crava::grid_fft_scale(cubetype == CRAVA_HIP_COVARIANCE, f64, N, inverse);
 }
 
 static int fft_common(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp, int nyp, int nzp, int cubetype,
@@ -142,7 +150,7 @@ static int fft_common(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp
   }
   if (nxp < 2 || nyp < 1 || nzp < 1) { set_error("invalid grid size"); return CRAVA_HIP_E_INVALID; }
   if (cubetype < CRAVA_HIP_DATA || cubetype > CRAVA_HIP_COVARIANCE) { set_error("invalid cubetype"); return CRAVA_HIP_E_INVALID; }
-  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) {
+  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F32_DEFLATE) {
     set_error("invalid transform precision");
     return CRAVA_HIP_E_INVALID;
   }
@@ -151,10 +159,16 @@ static int fft_common(crava_hip_ctx *ctx, void *const *grids, int ngrid, int nxp
   crava::Fft3D *f = nullptr;
   int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
   if (rc) return rc;
-  double scale[crava::kMaxGrids];
-  const double sc = fft_scale(cubetype, precision, (long long)nxp * nyp * nzp, inverse);
-  for (int i = 0; i < ngrid; ++i) scale[i] = sc;
-  return inverse ? f->c2r(grids, ngrid, scale, ctx->stream) : f->r2c(grids, ngrid, scale, ctx->stream);
+  double scale[crava::kMaxGrids], exact[crava::kMaxGrids];
+  const long long N = (long long)nxp * nyp * nzp;
+  const double sc = fft_scale(cubetype, precision, N, inverse), se = fft_scale_exact(cubetype, N, inverse);
+  for (int i = 0; i < ngrid; ++i) {
+    scale[i] = sc;
+    exact[i] = se;
+  }
+  const unsigned mask = precision == CRAVA_HIP_F32_DEFLATE ? (ngrid >= 32 ? ~0u : (1u << ngrid) - 1u) : 0u;
+  return inverse ? f->c2r(grids, ngrid, scale, ctx->stream, ctx->dcbuf, mask, exact)
+                 : f->r2c(grids, ngrid, scale, ctx->stream, ctx->dcbuf, mask, exact);
 }
 
 int crava_hip_fft_r2c(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int cubetype, int precision) {
@@ -201,7 +215,10 @@ int crava_hip_fft_pass(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp
 int crava_hip_autotune_fft(crava_hip_ctx *ctx, void *scratch, int nxp, int nyp, int nzp, int precision,
                            int *variants, float *ms) {
   if (!ctx || !scratch) { set_error("NULL argument to crava_hip_autotune_fft"); return CRAVA_HIP_E_INVALID; }
-  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) { set_error("invalid precision"); return CRAVA_HIP_E_INVALID; }
+  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F32_DEFLATE) {
+    set_error("invalid precision");
+    return CRAVA_HIP_E_INVALID;
+  }
   crava::Fft3D *f = nullptr;
   int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
   if (rc) return rc;
@@ -216,7 +233,10 @@ int crava_hip_autotune_fft(crava_hip_ctx *ctx, void *scratch, int nxp, int nyp, 
 
 int crava_hip_fft_set_variants(crava_hip_ctx *ctx, int nxp, int nyp, int nzp, int precision, const int *variants) {
   if (!ctx || !variants) { set_error("NULL argument to crava_hip_fft_set_variants"); return CRAVA_HIP_E_INVALID; }
-  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) { set_error("invalid precision"); return CRAVA_HIP_E_INVALID; }
+  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F32_DEFLATE) {
+    set_error("invalid precision");
+    return CRAVA_HIP_E_INVALID;
+  }
   crava::Fft3D *f = nullptr;
   int rc = ctx->plan(nxp, nyp, nzp, fft_prec(precision), &f);
   if (rc) return rc;

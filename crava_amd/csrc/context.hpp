@@ -11,6 +11,7 @@ namespace crava {
 struct Context {
   hipStream_t stream = nullptr;
   unsigned long long *counters = nullptr;  // small device counters
+  double *dcbuf = nullptr;                 // levels of a mean-deflated transform set (kMaxGrids)
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<Fft3D>> plans;
   // prec: 0 float, 1 float storage / double arithmetic, 2 double
   int plan(int nxp, int nyp, int nzp, int prec, Fft3D **out);

@@ -210,6 +210,8 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nro
   const SI *src = reinterpret_cast<const SI *>(gs.src[bz]);
   SO *dst = reinterpret_cast<SO *>(gs.dst[bz]);
   const T scale = (T)gs.scale[bz];
+  // mean deflation: subtract c in storage precision (exact near c)
+  const SI off = (gs.dc && ((gs.dcmask >> bz) & 1u)) ? (SI)gs.dc[bz] : (SI)0;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + pl.n;
   const int n = pl.n;
@@ -219,8 +221,8 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nro
   if constexpr (DIO) {
     const auto ld = [&](int c, int e) -> cx<T> {
       const long long ra = row0 + 2 * c;
-      const SI a = src[min(ra, nrows - 1) * rnxp + e];
-      const SI b = src[min(ra + 1, nrows - 1) * rnxp + e];
+      const SI a = src[min(ra, nrows - 1) * rnxp + e] - off;
+      const SI b = src[min(ra + 1, nrows - 1) * rnxp + e] - off;
       return mk(ra < nrows ? (T)a : (T)0, ra + 1 < nrows ? (T)b : (T)0);
     };
     run_passes<-1>(buf, tw, pl, ld);
@@ -238,8 +240,8 @@ __global__ void __launch_bounds__(1024) k_fft_rows_r2c(GridSet gs, long long nro
       const int c = fdiv(q, pl.n_mul);
       const int e = q - c * n;
       const long long ra = row0 + 2 * c;
-      va[u] = src[min(ra, nrows - 1) * rnxp + e];
-      vb[u] = src[min(ra + 1, nrows - 1) * rnxp + e];
+      va[u] = src[min(ra, nrows - 1) * rnxp + e] - off;
+      vb[u] = src[min(ra + 1, nrows - 1) * rnxp + e] - off;
       okb |= (ra < nrows ? 1u : 0u) << (2 * u);
       okb |= (ra + 1 < nrows ? 1u : 0u) << (2 * u + 1);
       slot[u] = q0 + u * nthr < tot ? c * pl.ldsp + gpadx(e) : -1;
@@ -285,6 +287,8 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nro
   const SI *src = reinterpret_cast<const SI *>(gs.src[bz]);
   SO *dst = reinterpret_cast<SO *>(gs.dst[bz]);
   const T scale = (T)gs.scale[bz];
+  // mean deflation: the level removed from the DC bin returns here
+  const T add = (gs.dc && ((gs.dcmask >> bz) & 1u)) ? (T)gs.dc[bz] : (T)0;
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + pl.n;
   const int n = pl.n;
@@ -352,8 +356,8 @@ __global__ void __launch_bounds__(1024) k_fft_rows_c2r(GridSet gs, long long nro
     const int e = q - c * n;
     const long long ra = row0 + 2 * c;
     const cx<T> v = buf[c * pl.ldsp + gpadx(e)];
-    if (ra < nrows) dst[ra * rnxp + e] = (SO)(v.x * scale);
-    if (ra + 1 < nrows) dst[(ra + 1) * rnxp + e] = (SO)(v.y * scale);
+    if (ra < nrows) dst[ra * rnxp + e] = (SO)(v.x * scale + add);
+    if (ra + 1 < nrows) dst[(ra + 1) * rnxp + e] = (SO)(v.y * scale + add);
   }
 }
 
@@ -591,6 +595,8 @@ int Fft3D::make_set(int axis, bool inverse, void *const *grids, int ng, const do
   }
   io = prec == 0 ? kIoF32 : prec == 2 ? kIoF64 : kIoF32MathF64;
   gs.n = ng;
+  gs.dc = nullptr;
+  gs.dcmask = 0;
   for (int i = 0; i < ng; ++i) {
     if (!grids[i]) {
       set_error("NULL grid in a transform set");
@@ -608,10 +614,15 @@ int Fft3D::make_set(int axis, bool inverse, void *const *grids, int ng, const do
   return CRAVA_HIP_OK;
 }
 
-int Fft3D::pass(int axis, void *const *grids, int ng, bool inverse, const double *scale, hipStream_t s) {
+int Fft3D::pass(int axis, void *const *grids, int ng, bool inverse, const double *scale, hipStream_t s,
+                 const double *dc, unsigned dcmask) {
   GridSet gs;
   int io, rc;
   if ((rc = make_set(axis, inverse, grids, ng, scale, gs, io))) return rc;
+  if (axis == 0) {
+    gs.dc = dc;
+    gs.dcmask = dc ? dcmask : 0u;
+  }
   const long long cnxp = nxp / 2 + 1;
   if (axis == 0) {
     if (var[0] >= 0)
@@ -710,18 +721,63 @@ int Fft3D::autotune(void *const *grids, int ng, hipStream_t s, int reps, float *
   return rc;
 }
 
-int Fft3D::r2c(void *const *grids, int ng, const double *scale, hipStream_t s) {
-  int rc;
-  if ((rc = pass(0, grids, ng, false, scale, s))) return rc;
-  if ((rc = pass(1, grids, ng, false, nullptr, s))) return rc;
-  return pass(2, grids, ng, false, nullptr, s);
+// Mean deflation around the passes (one thread per grid of the set; the
+// set's scale holds the exact level scales):
+// 0 before a forward transform: dc[g] = the grid's first real value;
+// 1 after it: DC bin += c * N * scale[g] in double, c = dc[g] as the x
+//   pass subtracted it (rounded to the storage type);
+// 2 before an inverse transform: dc[g] = DC bin * scale[g], DC bin = 0.
+template <typename S>
+__global__ void k_fft_dc(GridSet gs, int which, double n, double *dc, unsigned dcmask) {
+  const int g = threadIdx.x;
+  if (g >= gs.n || !((dcmask >> g) & 1u)) return;
+  S *p = static_cast<S *>(gs.dst[g]);
+  if (which == 0) {
+    dc[g] = (double)p[0];
+  } else if (which == 1) {
+    p[0] = (S)((double)p[0] + (double)(S)dc[g] * n * gs.scale[g]);
+  } else {
+    dc[g] = (double)p[0] * gs.scale[g];
+    p[0] = (S)0;
+    p[1] = (S)0;
+  }
 }
 
-int Fft3D::c2r(void *const *grids, int ng, const double *scale, hipStream_t s) {
+int Fft3D::dc_kernel(int which, void *const *grids, int ng, const double *dscale, double *dc, unsigned dcmask,
+                     hipStream_t s) const {
+  GridSet gs;
+  int io, rc;
+  if ((rc = make_set(0, false, grids, ng, dscale, gs, io))) return rc;
+  const double n = (double)nxp * nyp * nzp;
+  if (prec == 2)
+    hipLaunchKernelGGL(k_fft_dc<double>, dim3(1), dim3(kMaxGrids), 0, s, gs, which, n, dc, dcmask);
+  else
+    hipLaunchKernelGGL(k_fft_dc<float>, dim3(1), dim3(kMaxGrids), 0, s, gs, which, n, dc, dcmask);
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
+int Fft3D::r2c(void *const *grids, int ng, const double *scale, hipStream_t s, double *dc, unsigned dcmask,
+               const double *dscale) {
   int rc;
+  if (!dc) dcmask = 0;
+  if (!dscale) dscale = scale;
+  if (dcmask && (rc = dc_kernel(0, grids, ng, dscale, dc, dcmask, s))) return rc;
+  if ((rc = pass(0, grids, ng, false, scale, s, dc, dcmask))) return rc;
+  if ((rc = pass(1, grids, ng, false, nullptr, s))) return rc;
+  if ((rc = pass(2, grids, ng, false, nullptr, s))) return rc;
+  return dcmask ? dc_kernel(1, grids, ng, dscale, dc, dcmask, s) : CRAVA_HIP_OK;
+}
+
+int Fft3D::c2r(void *const *grids, int ng, const double *scale, hipStream_t s, double *dc, unsigned dcmask,
+               const double *dscale) {
+  int rc;
+  if (!dc) dcmask = 0;
+  if (!dscale) dscale = scale;
+  if (dcmask && (rc = dc_kernel(2, grids, ng, dscale, dc, dcmask, s))) return rc;
   if ((rc = pass(2, grids, ng, true, nullptr, s))) return rc;
   if ((rc = pass(1, grids, ng, true, nullptr, s))) return rc;
-  return pass(0, grids, ng, true, scale, s);
+  return pass(0, grids, ng, true, scale, s, dc, dcmask);
 }
 
 }  // namespace crava

@@ -75,13 +75,27 @@ class Fft3D {
   int init();  // CRAVA_HIP_OK or an error code
   // In-place forward r2c of `ng` (<= kMaxGrids) grids of this size in one
   // launch per pass; grid i's result is multiplied by scale[i] (1 if NULL).
-  int r2c(void *const *grids, int ng, const double *scale, hipStream_t s);
-  // In-place inverse c2r of a set of grids, scaled as r2c.
-  int c2r(void *const *grids, int ng, const double *scale, hipStream_t s);
+  // Mean deflation of the grids whose bit is set in dcmask (dc: device
+  // memory for kMaxGrids doubles, private to this call's stream): the
+  // transform runs on x - c, c = the grid's first value, and c N dscale is
+  // added to the DC bin in double; a float transform then keeps the
+  // accuracy of the grid's variation instead of its level.  dscale: the
+  // exact (double) scale of each grid for the level (scale if NULL), so that
+  // the level survives a forward + inverse pair exactly while the kernels
+  // apply `scale` as their storage type rounds it.
+  int r2c(void *const *grids, int ng, const double *scale, hipStream_t s, double *dc = nullptr,
+          unsigned dcmask = 0, const double *dscale = nullptr);
+  // In-place inverse c2r of a set of grids, scaled as r2c.  Mean deflation:
+  // the DC bin v is zeroed before the passes and v * dscale is added to
+  // every output value.
+  int c2r(void *const *grids, int ng, const double *scale, hipStream_t s, double *dc = nullptr,
+          unsigned dcmask = 0, const double *dscale = nullptr);
   int r2c(void *grid, hipStream_t s, double scale) { return r2c(&grid, 1, &scale, s); }
   int c2r(void *grid, hipStream_t s, double scale) { return c2r(&grid, 1, &scale, s); }
-  // One pass (axis 0 x rows, 1 y, 2 z) over a set; `scale` applies on the x pass.
-  int pass(int axis, void *const *grids, int ng, bool inverse, const double *scale, hipStream_t s);
+  // One pass (axis 0 x rows, 1 y, 2 z) over a set; `scale` applies on the x
+  // pass, and so do dc / dcmask (as in r2c / c2r).
+  int pass(int axis, void *const *grids, int ng, bool inverse, const double *scale, hipStream_t s,
+           const double *dc = nullptr, unsigned dcmask = 0);
   long long complex_elems() const;
 
   int nxp, nyp, nzp;
@@ -103,6 +117,8 @@ class Fft3D {
  private:
   int build_plan(int n, PassPlan &p, int generic_variant);
   int make_set(int axis, bool inverse, void *const *grids, int ng, const double *scale, GridSet &gs, int &io) const;
+  int dc_kernel(int which, void *const *grids, int ng, const double *dscale, double *dc, unsigned dcmask,
+                hipStream_t s) const;
 };
 
 }  // namespace crava

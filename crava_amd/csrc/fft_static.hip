@@ -92,6 +92,8 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nro
   const SI *src = reinterpret_cast<const SI *>(gs.src[bz]);
   SO *dst = reinterpret_cast<SO *>(gs.dst[bz]);
   const T scale = (T)gs.scale[bz];
+  // mean deflation: subtract c in storage precision (exact near c)
+  const SI off = (gs.dc && ((gs.dcmask >> bz) & 1u)) ? (SI)gs.dc[bz] : (SI)0;
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
@@ -103,8 +105,8 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nro
     // branch-free loads (clamped rows, then select), as in the strided kernel
     const auto ld = [&](int c, int e) -> cx<T> {
       const long long ra = row0 + 2 * c;
-      const SI a = src[min(ra, nrows - 1) * rnxp + e];
-      const SI b = src[min(ra + 1, nrows - 1) * rnxp + e];
+      const SI a = src[min(ra, nrows - 1) * rnxp + e] - off;
+      const SI b = src[min(ra + 1, nrows - 1) * rnxp + e] - off;
       return mk(ra < nrows ? (T)a : (T)0, ra + 1 < nrows ? (T)b : (T)0);
     };
     spasses_io<N, TX, THREADS, -1, 0, false, false, T>(buf, tw, ld, LdsIo{});
@@ -121,8 +123,14 @@ __global__ void __launch_bounds__(THREADS) k_srows_r2c(GridSet gs, long long nro
         const int c = q / H, e2 = q % H;
         const long long ra = row0 + 2 * c;
         cx<SI> a = mk((SI)0, (SI)0), b = mk((SI)0, (SI)0);
-        if (ra < nrows) a = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[e2];
-        if (ra + 1 < nrows) b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[e2];
+        if (ra < nrows) {
+          a = reinterpret_cast<const cx<SI> *>(src + ra * rnxp)[e2];
+          a = mk((SI)(a.x - off), (SI)(a.y - off));
+        }
+        if (ra + 1 < nrows) {
+          b = reinterpret_cast<const cx<SI> *>(src + (ra + 1) * rnxp)[e2];
+          b = mk((SI)(b.x - off), (SI)(b.y - off));
+        }
         cx<T> *d = buf + c * PITCH;
         d[padx<T>(2 * e2)] = mk((T)a.x, (T)b.x);
         d[padx<T>(2 * e2 + 1)] = mk((T)a.y, (T)b.y);
@@ -163,6 +171,8 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nro
   const SI *src = reinterpret_cast<const SI *>(gs.src[bz]);
   SO *dst = reinterpret_cast<SO *>(gs.dst[bz]);
   const T scale = (T)gs.scale[bz];
+  // mean deflation: the level removed from the DC bin returns here
+  const T add = (gs.dc && ((gs.dcmask >> bz) & 1u)) ? (T)gs.dc[bz] : (T)0;
   extern __shared__ unsigned char smem_raw[];
   cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *buf = tw + N;
@@ -186,8 +196,8 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nro
     };
     const auto st = [&](int c, int e, cx<T> v) {
       const long long ra = row0 + 2 * c;
-      if (ra < nrows) dst[ra * rnxp + e] = (SO)(v.x * scale);
-      if (ra + 1 < nrows) dst[(ra + 1) * rnxp + e] = (SO)(v.y * scale);
+      if (ra < nrows) dst[ra * rnxp + e] = (SO)(v.x * scale + add);
+      if (ra + 1 < nrows) dst[(ra + 1) * rnxp + e] = (SO)(v.y * scale + add);
     };
     spasses_io<N, TX, THREADS, 1, 0, false, false, T>(buf, tw, ld, st);
     return;
@@ -226,9 +236,10 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nro
       const cx<T> *s0 = buf + c * PITCH;
       const cx<T> v0 = s0[padx<T>(2 * e2)], v1 = s0[padx<T>(2 * e2 + 1)];
       if (ra < nrows)
-        reinterpret_cast<cx<SO> *>(dst + ra * rnxp)[e2] = mk((SO)(v0.x * scale), (SO)(v1.x * scale));
+        reinterpret_cast<cx<SO> *>(dst + ra * rnxp)[e2] = mk((SO)(v0.x * scale + add), (SO)(v1.x * scale + add));
       if (ra + 1 < nrows)
-        reinterpret_cast<cx<SO> *>(dst + (ra + 1) * rnxp)[e2] = mk((SO)(v0.y * scale), (SO)(v1.y * scale));
+        reinterpret_cast<cx<SO> *>(dst + (ra + 1) * rnxp)[e2] =
+            mk((SO)(v0.y * scale + add), (SO)(v1.y * scale + add));
     }
   }
 }

@@ -32,6 +32,13 @@ struct GridSet {
   void *dst[kMaxGrids];
   double scale[kMaxGrids];
   int n;
+  // Mean deflation of the x pass (grids whose bit is set in dcmask): the
+  // forward pass transforms x - c (c = dc[g], a value near the grid's
+  // level, exact by Sterbenz's lemma for values within a factor 2 of it),
+  // the inverse pass adds dc[g] to every output; the DC bin carries c
+  // (see Fft3D::r2c / c2r).  dc is device memory or NULL.
+  const double *dc;
+  unsigned dcmask;
 };
 
 // Storage and arithmetic of a pass: float, double, or float storage with

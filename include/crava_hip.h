@@ -38,6 +38,13 @@ extern "C" {
 #define CRAVA_HIP_MIXED 1 /* float grids; transforms with double arithmetic inside each pass;
                              FP32 square-root solve, FP64 below the noise/signal threshold */
 #define CRAVA_HIP_F64 2   /* double grids, double transforms and solve */
+/* transforms only: float grids and float arithmetic on each grid minus its
+   first value (a value near its level); the level returns exactly through
+   the DC bin (forward) or onto every value (inverse), with the scale
+   factors in double.  For grids whose level dominates their variation (the
+   prior means ln Vp, ln Vs, ln rho): the transform keeps the accuracy of the
+   variation instead of the level's. */
+#define CRAVA_HIP_F32_DEFLATE 3
 
 typedef struct crava_hip_ctx crava_hip_ctx;
 typedef struct crava_hip_post crava_hip_post;
@@ -88,7 +95,8 @@ int crava_hip_fft_r2c(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp,
 /* In-place 3-D half-complex-to-real transform.  Replaces
  * FFTGrid::invFFTInPlace (src/fftgrid.cpp:1832-1862): scaled by 1/N for
  * CRAVA_HIP_COVARIANCE, 1/sqrt(N) otherwise; Im of the self-conjugate x bins
- * is ignored as in FFTW's halfcomplex c2r. */
+ * is ignored as in FFTW's halfcomplex c2r.  precision as crava_hip_fft_r2c,
+ * or CRAVA_HIP_F32_DEFLATE (both directions). */
 int crava_hip_fft_c2r(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp, int cubetype, int precision);
 
 /* The same transforms over `ngrid` (1..16) device grids of one size and

@@ -58,18 +58,19 @@ class _Vol:
 
 def _single_path(ctx, v, precision):
     """The same sequence volume by volume through the single-volume entry
-    points: the mixed form transforms the means (both ways) and the
-    covariances (forward) with double arithmetic in each pass, the rest in
+    points: the mixed form transforms the covariances forward with double
+    arithmetic in each pass and the means with mean deflation, the rest in
     the grids' own precision."""
     c = v.prob.cfg
     n = (c.nxp, c.nyp, c.nzp)
     mc = L.MIXED if precision == L.MIXED else None
+    md = L.F32_DEFLATE if precision == L.MIXED else None
     ctx.r2c_many(v.data, *n, L.DATA)
     ctx.r2c(v.ec, *n, L.COVARIANCE)
-    ctx.r2c_many(v.mu, *n, L.DATA, mc)
+    ctx.r2c_many(v.mu, *n, L.DATA, md)
     ctx.r2c_many(v.cov, *n, L.COVARIANCE, mc)
     nf, n64 = v.post.run(precision, v.mu, v.data, v.cov, v.ec, counts=True)
-    ctx.c2r_many(v.mu, *n, L.DATA, mc)
+    ctx.c2r_many(v.mu, *n, L.DATA, md)
     ctx.c2r_many(v.data, *n, L.DATA)
     ctx.c2r_many(v.cov, *n, L.COVARIANCE)
     return nf, n64

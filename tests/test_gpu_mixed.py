@@ -1,6 +1,7 @@
 """CRAVA_HIP_MIXED (FP32 square-root solve, FP64 for waves below the
-noise/signal threshold; transforms of the means and covariances with double
-arithmetic inside each pass) against the FP64 oracle, in the noise regime of
+noise/signal threshold; covariance transforms with double arithmetic inside
+each pass, mean transforms with mean deflation) against the FP64 oracle, in
+the noise regime of
 the 512x512x320 precision workload: 6 angles at 4-40 deg, per-angle S/N
 1e4, 3e3, 1e3, 300, 100, 20.  The grid is that workload's 320-sample z axis
 with a 64 x 64 lateral FFT grid, so the oracle runs in seconds.
@@ -135,9 +136,9 @@ def test_condition_number_of_the_regime(regime):
 @pytest.mark.gpu
 def test_mixed_path_with_transforms_vs_fp64_oracle(ctx, regime):
     """The whole path in the time domain: forward transforms (data and error
-    correlation in float; means and covariances with double arithmetic in
-    each pass), MIXED posterior at threshold 1e-3, inverse transforms (means
-    with double arithmetic in each pass, residuals and covariances in float)
+    correlation in float, means in float with mean deflation, covariances
+    with double arithmetic in each pass), MIXED posterior at threshold 1e-3,
+    inverse transforms (means deflated, residuals and covariances in float)
     -- against FP64 transforms and the FP64 posterior of the oracle."""
     prob = regime["prob"]
     cfg = prob.cfg
@@ -149,12 +150,12 @@ def test_mixed_path_with_transforms_vs_fp64_oracle(ctx, regime):
     te = prob.errcorr.clone().cuda()
     ctx.r2c_many(td, *n, L.DATA)
     ctx.r2c(te, *n, L.COVARIANCE)
-    ctx.r2c_many(tm, *n, L.DATA, L.MIXED)
+    ctx.r2c_many(tm, *n, L.DATA, L.F32_DEFLATE)
     ctx.r2c_many(tc, *n, L.COVARIANCE, L.MIXED)
     post = Posterior(ctx, synth.post_params(prob, 1e-3), regime["tabs"])
     nf, n64 = post.run(L.MIXED, tm, td, tc, te, counts=True)
     post.close()
-    ctx.c2r_many(tm, *n, L.DATA, L.MIXED)
+    ctx.c2r_many(tm, *n, L.DATA, L.F32_DEFLATE)
     ctx.c2r_many(td, *n, L.DATA)
     ctx.c2r_many(tc, *n, L.COVARIANCE)
     torch.cuda.synchronize()
