@@ -125,6 +125,8 @@ SIGNATURES = {
     "crava_hip_post_destroy": (I, [VP]),
     "crava_hip_post_run": (I, [VP, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP,
                                C.POINTER(LL), C.POINTER(LL)]),
+    "crava_hip_post_run_static": (I, [VP, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP,
+                                      C.POINTER(LL), C.POINTER(LL)]),
     "crava_hip_simulate": (I, [VP, I, I, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(LL)]),
     "crava_hip_simulate_points": (I, [VP, LL, C.POINTER(VP), C.POINTER(VP), C.POINTER(LL)]),
     "crava_hip_fill_complex_noise": (I, [VP, VP, I, I, I, C.c_uint64, C.c_uint64]),

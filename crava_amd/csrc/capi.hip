@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <tuple>
@@ -81,6 +82,45 @@ struct crava_hip_post {
 };
 
 using crava::set_error;
+
+// ---------------------------------------------------------------------------
+// Static queue of crava_hip_post_run_static: the mixed form's FP64 queue (a
+// count, then up to kStaticQueueEntries {k, first point} entries) in a
+// file-scope device array, so that the call allocates nothing.  Every GPU
+// holds its own copy; its address is looked up once per device and kept.
+namespace {
+constexpr long long kStaticQueueEntries = 1LL << 20;   // 8 MiB of entries: 67 M in-band points
+constexpr size_t kStaticQueueHead = 256;
+__device__ __align__(256) unsigned char scratch[kStaticQueueHead + sizeof(int2) * kStaticQueueEntries];
+constexpr int kMaxDevices = 64;
+void *g_scratch_dev[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+
+// The current device's copy of the static queue.
+int static_queue(crava::DeferQueue *q) {
+  int dev = -1;
+  CRAVA_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index out of range for the static queue");
+    return CRAVA_HIP_E_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    CRAVA_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = ptr;
+  }
+  unsigned char *base = static_cast<unsigned char *>(g_scratch_dev[dev]);
+  q->count = reinterpret_cast<unsigned *>(base);
+  q->list = reinterpret_cast<int2 *>(base + kStaticQueueHead);
+  q->cap = kStaticQueueEntries;
+  return CRAVA_HIP_OK;
+}
+
+__global__ void k_zero_counts2(unsigned long long *c) {
+  if (threadIdx.x < 2) c[threadIdx.x] = 0ULL;
+}
+}  // namespace
 
 // transform arithmetic/storage code of a C-ABI precision
 static int fft_prec(int precision) {
@@ -375,6 +415,43 @@ int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, voi
   if (want) CRAVA_HIP_CHECK(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
   int rc = crava::launch_posterior(post->dev, precision, mu, data, cov, errcorr, want ? ctx->counters : nullptr,
                                    &post->queue, ctx->stream);
+  if (rc) return rc;
+  if (want) {
+    unsigned long long h[2] = {0, 0};
+    CRAVA_HIP_CHECK(hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (n_fail) *n_fail = (long long)h[0];
+    if (n_fp64) *n_fp64 = (long long)h[1];
+  }
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_post_run_static(crava_hip_post *post, int precision, void *const *mu, void *const *data,
+                              void *const *cov, const void *errcorr, long long *n_fail, long long *n_fp64) {
+  if (!post || !mu || !data || !cov || !errcorr) {
+    set_error("NULL argument to crava_hip_post_run_static");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) { set_error("invalid precision"); return CRAVA_HIP_E_INVALID; }
+  if (precision == CRAVA_HIP_MIXED && post->dev.defer_need > kStaticQueueEntries) {
+    set_error("the mixed posterior needs a queue of " + std::to_string(post->dev.defer_need) +
+              " entries; the static queue holds " + std::to_string(kStaticQueueEntries));
+    return CRAVA_HIP_E_INVALID;
+  }
+  for (int c = 0; c < 3; ++c) if (!mu[c]) { set_error("NULL mean grid"); return CRAVA_HIP_E_INVALID; }
+  for (int q = 0; q < 6; ++q) if (!cov[q]) { set_error("NULL covariance grid"); return CRAVA_HIP_E_INVALID; }
+  for (int l = 0; l < post->dev.pp.ntheta; ++l) if (!data[l]) { set_error("NULL data grid"); return CRAVA_HIP_E_INVALID; }
+  crava_hip_ctx *ctx = post->ctx;
+  crava::DeferQueue q;
+  int rc;
+  if (precision == CRAVA_HIP_MIXED && (rc = static_queue(&q))) return rc;
+  const bool want = n_fail || n_fp64;
+  if (want) {
+    hipLaunchKernelGGL(k_zero_counts2, dim3(1), dim3(64), 0, ctx->stream, ctx->counters);
+    CRAVA_HIP_CHECK(hipGetLastError());
+  }
+  rc = crava::launch_posterior(post->dev, precision, mu, data, cov, errcorr, want ? ctx->counters : nullptr,
+                               precision == CRAVA_HIP_MIXED ? &q : nullptr, ctx->stream);
   if (rc) return rc;
   if (want) {
     unsigned long long h[2] = {0, 0};

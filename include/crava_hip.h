@@ -191,6 +191,20 @@ int crava_hip_post_destroy(crava_hip_post *post);
 int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, void *const *data, void *const *cov,
                        const void *errcorr, long long *n_fail, long long *n_fp64);
 
+/* crava_hip_post_run that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The mixed form's FP64 queue lives in a static device array
+ * of the library (2^20 entries of one 64-point wave each: problems of up to
+ * 67 108 864 in-band Fourier points; larger ones are refused with
+ * CRAVA_HIP_E_INVALID); each GPU holds its own copy until the process ends,
+ * and the call clears its count before use.  Calls that overlap on one GPU
+ * (concurrent streams or threads) share that array and race.  Same
+ * arguments, results and counts as crava_hip_post_run; without n_fail /
+ * n_fp64 the call does not synchronise. */
+int crava_hip_post_run_static(crava_hip_post *post, int precision, void *const *mu, void *const *data,
+                              void *const *cov, const void *errcorr, long long *n_fail, long long *n_fp64);
+
 /* Negative log marginal likelihood of the data and its gradient (no
  * reference counterpart; the model is the one the posterior loop assumes,
  * src/crava.cpp:1087-1130):
