@@ -4,11 +4,18 @@
 Workload (default): 512x512x320 FFT grid (interior 448x448x256), 6 angle
 stacks at 4-40 deg, per-angle S/N 1e4, 3e3, 1e3, 300, 100, 20, seeded
 synthetic float32 inputs generated on the GPU.  One step is the whole
-posterior path of Crava::computePostMeanResidAndFFTCov with its transforms:
+posterior path of Crava::computePostMeanResidAndFFTCov with its transforms,
+for the separable prior Sigma0 x corr that SeismicParametersHolder::
+initializeCorrelations builds (the six covariance grids are one correlation
+grid times Sigma0_ab, so one correlation transform gives all six spectra):
 
-  16 forward r2c (6 data, 3 prior means, 6 prior covariances, errCorr)
+  11 forward r2c (6 data, 3 prior means, the correlation, errCorr)
   + the per-point posterior over all P = 257*512*320 Fourier points
   + 15 inverse c2r (3 posterior means, 6 posterior covariances, 6 residuals)
+
+The general form (six covariance grids transformed and read: 16 forward
+transforms, as a time-lapse merged prior needs) is timed too and reported
+under "general_prior".
 
 run twice: the MIXED form (float grids; covariance transforms forward with
 double arithmetic inside each pass, mean transforms in float with mean
@@ -116,7 +123,7 @@ def parse():
 class Pipeline:
     """Working grids + posterior handle for one precision form."""
 
-    def __init__(self, ctx, prob, params, tables, fp64: bool):
+    def __init__(self, ctx, prob, params, tables, fp64: bool, separable: bool = True):
         import numpy as np
         import torch
         from crava_amd import _lib as L
@@ -127,6 +134,10 @@ class Pipeline:
         self.data = [g.to(dt).clone() for g in prob.data]
         self.cov = [g.to(dt).clone() for g in prob.cov]
         self.cov0 = [g.clone() for g in self.cov]
+        # separable prior (initializeCorrelations): the one correlation grid
+        self.separable = separable
+        self.corr0 = prob.corr.to(dt).clone()
+        self.corr = self.corr0.clone()
         self.ec0 = prob.errcorr.to(dt).clone()
         self.ec = self.ec0.clone()
         self.post = Posterior(ctx, params, tables)
@@ -149,24 +160,29 @@ class Pipeline:
         form's double-arithmetic covariance transforms overlap the float data
         transforms), the posterior, the inverse sets."""
         from crava_amd.core import Batch
-        self.batch = Batch(self.ctx, self.precision, [(self.post, self.mu, self.data, self.cov, self.ec)])
+        vol = (self.post, self.mu, self.data, self.cov, self.ec) + ((self.corr,) if self.separable else ())
+        self.batch = Batch(self.ctx, self.precision, [vol])
         return self.batch.autotune()
 
     def step(self, counts=False):
         """One pass of the path; each group of same-kind grids is one launch per transform pass."""
         L, c = self.L, self.prob.cfg
         n = (c.nxp, c.nyp, c.nzp)
-        # fill the prior covariance and error-correlation grids afresh
+        # fill the prior (the six covariance grids, or the one correlation grid
+        # of the separable form) and the error correlation afresh
         # (initializeCorrelations, one write per grid): every step then solves
         # the same problem (the FP64 point count depends on them; the previous
         # step's means and residuals as inputs do not change the work).  The
         # FP64 form copies its double grids back.
+        prior = [self.corr] if self.separable else self.cov
         if self.fp64:
-            for d, s_ in zip(self.cov + [self.ec], self.cov0 + [self.ec0]):
+            prior0 = [self.corr0] if self.separable else self.cov0
+            for d, s_ in zip(prior + [self.ec], prior0 + [self.ec0]):
                 d.copy_(s_)
         else:
             from crava_amd.core import fill_err_corr, fill_param_corr
-            fill_param_corr(self.ctx, self.cov, self.cov_scales, *n, self.corr_xy, self.circ_t)
+            fill_param_corr(self.ctx, prior, [1.0] if self.separable else self.cov_scales, *n, self.corr_xy,
+                            self.circ_t)
             fill_err_corr(self.ctx, self.ec, *n, self.corr_xy)
         if self.batch is not None:
             self.batch.run(graph=True)
@@ -183,8 +199,9 @@ class Pipeline:
         self.ctx.r2c_many(self.data, *n, L.DATA, self.data_precision)
         self.ctx.r2c(self.ec, *n, L.COVARIANCE, self.data_precision)
         self.ctx.r2c_many(self.mu, *n, L.DATA, self.mean_precision)
-        self.ctx.r2c_many(self.cov, *n, L.COVARIANCE, self.cov_precision)
-        r = self.post.run(self.precision, self.mu, self.data, self.cov, self.ec, counts=counts)
+        self.ctx.r2c_many(prior, *n, L.COVARIANCE, self.cov_precision)
+        r = self.post.run(self.precision, self.mu, self.data, self.cov, self.ec, counts=counts,
+                          corr=self.corr if self.separable else None)
         self.ctx.c2r_many(self.mu, *n, L.DATA, self.mean_precision)
         self.ctx.c2r_many(self.data, *n, L.DATA, self.data_precision)
         self.ctx.c2r_many(self.cov, *n, L.COVARIANCE, self.data_precision)
@@ -196,8 +213,8 @@ class Pipeline:
         against)."""
         from crava_amd.core import grid_op
         c = self.prob.cfg
-        for dst, src in zip(self.mu + self.data + self.cov + [self.ec],
-                            self.prob.mu + self.prob.data + self.prob.cov + [self.prob.errcorr]):
+        for dst, src in zip(self.mu + self.data + self.cov + [self.ec, self.corr],
+                            self.prob.mu + self.prob.data + self.prob.cov + [self.prob.errcorr, self.prob.corr]):
             if self.fp64:
                 dst.copy_(src)
             else:
@@ -359,7 +376,8 @@ def cpu_baseline(prob, tabs, budget_s):
 def batching(ctx, volumes: int, reps: int, seed: int = 3):
     """The batching workload: `volumes` inversion volumes of uneven size
     (nx, ny in 32..192, nz in 64..256, ntheta 3..8, own wavelet, ranges and
-    S/N), MIXED form, autotuned, replayed as one hipGraph per batch."""
+    S/N), each with its separable prior, MIXED form, autotuned, replayed as
+    one hipGraph per batch."""
     import numpy as np
     import torch
     from crava_amd import _lib as L
@@ -375,12 +393,13 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
         c = prob.cfg
         tabs = problem_tables(prob)
         posts.append(Posterior(ctx, synth.post_params(prob, 0.1), tabs))
-        pristine = prob.mu + prob.data + prob.cov + [prob.errcorr]
+        pristine = prob.mu + prob.data + prob.cov + [prob.errcorr, prob.corr]
         work = [g.clone() for g in pristine]
         nt = prob.ntheta
         nts.append(nt)
         src.append((pristine, work))
-        vols.append((posts[-1], work[:3], work[3:3 + nt], work[3 + nt:9 + nt], work[9 + nt]))
+        # separable prior (initializeCorrelations): one correlation transform per volume
+        vols.append((posts[-1], work[:3], work[3:3 + nt], work[3 + nt:9 + nt], work[9 + nt], work[10 + nt]))
         cn = c.nxp // 2 + 1
         P += cn * c.nyp * c.nzp
         P_in += int(tabs["invert"].sum()) * cn * c.nyp
@@ -885,6 +904,31 @@ def main():
         log(f"fp64: {ms_f64:.3f} ms/step (graph {ms_f64_graph:.3f}, one stream {ms_f64_direct:.3f}), "
             f"{P/ms_f64*1e3:.4g} pts/s")
 
+    # the general prior (six covariance grids transformed and read, as for a
+    # time-lapse merged prior), one stream, both forms: time and accuracy
+    general = None
+    if f64 is not None:
+        keep_m, keep_f = mixed.batch, f64.batch
+        mixed.batch = f64.batch = None
+        mixed.separable = f64.separable = False
+        f64.reset()
+        f64.step()
+        mixed.reset()
+        mixed.step()
+        torch.cuda.synchronize()
+        g_err = errors(mixed, f64)
+        mixed.reset()
+        f64.reset()
+        g_mixed = run(mixed, args.steps, args.warmup) * 1e3
+        g_f64 = run(f64, max(1, args.steps), 1) * 1e3
+        general = {"ms_per_step_mixed": g_mixed, "ms_per_step_fp64": g_f64, "speedup_vs_fp64": g_f64 / g_mixed,
+                   "value": P / (g_mixed * 1e-3), "rel_err_vs_fp64": g_err, "transforms_per_step": 31}
+        log(f"general prior: mixed {g_mixed:.3f} ms, fp64 {g_f64:.3f} ms ({g_f64 / g_mixed:.2f}x), errors {g_err}")
+        mixed.separable = f64.separable = True
+        mixed.batch, f64.batch = keep_m, keep_f
+        mixed.reset()
+        f64.reset()
+
     # per-kernel timing (HIP events on the launch stream), on the launches the
     # step makes: each forward pass over the set of nt data + 3 mean grids
     gset = mixed.data + mixed.mu
@@ -916,6 +960,10 @@ def main():
     setup32 = restorer(mixed)
     t_post = timed_kernel(lambda: mixed.post.run(L.MIXED, mixed.mu, mixed.data, mixed.cov, mixed.ec),
                           args.kernel_reps, setup32)
+    # the separable form's posterior (its transformed correlation grid)
+    ctx.r2c(mixed.corr, *n3, L.COVARIANCE, L.MIXED)
+    t_post_sep = timed_kernel(lambda: mixed.post.run(L.MIXED, mixed.mu, mixed.data, mixed.cov, mixed.ec,
+                                                     corr=mixed.corr), args.kernel_reps, setup32)
     t_post32 = timed_kernel(lambda: mixed.post.run(L.F32, mixed.mu, mixed.data, mixed.cov, mixed.ec),
                             args.kernel_reps, setup32)
     del setup32
@@ -928,11 +976,15 @@ def main():
                                 setup64)
         del setup64
     post_bytes = n_in * 8 * (19 + 2 * nt) + (P - n_in) * 96
+    # separable: reads mu 3, d nt, corr 1, errCorr 1; writes mu 3, Sigma 6, r nt (in band);
+    # reads corr, writes Sigma 6 (out of band)
+    sep_bytes = n_in * 8 * (14 + 2 * nt) + (P - n_in) * 56
     kernels = {
         "fft_x_r2c": {"ms": t_x, "alg_bytes": real_bytes + grid_c_bytes},
         "fft_y": {"ms": t_y, "alg_bytes": 2 * grid_c_bytes},
         "fft_z": {"ms": t_z, "alg_bytes": 2 * grid_c_bytes},
         "posterior_mixed": {"ms": t_post, "alg_bytes": post_bytes},
+        "posterior_mixed_separable": {"ms": t_post_sep, "alg_bytes": sep_bytes},
         "posterior_f32": {"ms": t_post32, "alg_bytes": post_bytes},
     }
     if t_post64:
@@ -997,7 +1049,11 @@ def main():
         "dtype": "f32+f64 (mixed)",
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"posterior path {nxp}x{nyp}x{nzp} FFT grid, ntheta={nt}", "fourier_points": P,
-                   "in_band_points": n_in, "transforms_per_step": 31, "parallelism": "single GPU"},
+                   "in_band_points": n_in, "transforms_per_step": 26,
+                   "prior": "separable: Sigma0 x one correlation grid, as initializeCorrelations builds it "
+                            "(one correlation transform instead of six; the six-grid form is under general_prior)",
+                   "parallelism": "single GPU"},
+        "general_prior": general,
         "ms_per_step_graph": ms_mixed_graph,
         "ms_per_step_one_stream": ms_mixed_direct,
         "graph_launch": graph_tune,

@@ -53,6 +53,7 @@ class BatchVolume(C.Structure):
         ("data", VP * 8),
         ("cov", VP * 6),
         ("errcorr", VP),
+        ("corr", VP),
     ]
 
 class SyntParams(C.Structure):
@@ -127,6 +128,8 @@ SIGNATURES = {
                                C.POINTER(LL), C.POINTER(LL)]),
     "crava_hip_post_run_static": (I, [VP, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), VP,
                                       C.POINTER(LL), C.POINTER(LL)]),
+    "crava_hip_post_run_separable": (I, [VP, I, C.POINTER(VP), C.POINTER(VP), VP, VP, C.POINTER(VP),
+                                         C.POINTER(LL), C.POINTER(LL)]),
     "crava_hip_simulate": (I, [VP, I, I, I, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(LL)]),
     "crava_hip_simulate_points": (I, [VP, LL, C.POINTER(VP), C.POINTER(VP), C.POINTER(LL)]),
     "crava_hip_fill_complex_noise": (I, [VP, VP, I, I, I, C.c_uint64, C.c_uint64]),

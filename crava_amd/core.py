@@ -164,14 +164,21 @@ class Posterior:
         except Exception:
             pass
 
-    def run(self, precision: int, mu, data, cov, errcorr, counts: bool = False):
+    def run(self, precision: int, mu, data, cov, errcorr, counts: bool = False, corr=None):
+        """crava_hip_post_run; with `corr` (the transformed correlation grid of
+        a separable prior) crava_hip_post_run_separable, `cov` then receiving
+        the posterior covariance."""
         mu_p = ptr_array([_dev_ptr(t) for t in mu])
         d_p = ptr_array([_dev_ptr(t) for t in data])
         c_p = ptr_array([_dev_ptr(t) for t in cov])
         nf, n64 = C.c_longlong(0), C.c_longlong(0)
-        check(lib().crava_hip_post_run(self.h, precision, mu_p, d_p, c_p, _dev_ptr(errcorr),
-                                       C.byref(nf) if counts else None, C.byref(n64) if counts else None),
-              "crava_hip_post_run")
+        cnt = (C.byref(nf) if counts else None, C.byref(n64) if counts else None)
+        if corr is None:
+            check(lib().crava_hip_post_run(self.h, precision, mu_p, d_p, c_p, _dev_ptr(errcorr), *cnt),
+                  "crava_hip_post_run")
+        else:
+            check(lib().crava_hip_post_run_separable(self.h, precision, mu_p, d_p, _dev_ptr(corr), _dev_ptr(errcorr),
+                                                     c_p, *cnt), "crava_hip_post_run_separable")
         return (nf.value, n64.value) if counts else None
 
 
@@ -191,15 +198,19 @@ def set_fft_variants(ctx: Context, nxp: int, nyp: int, nzp: int, precision: int,
 
 
 class Batch:
-    """crava_hip_batch over volumes given as (Posterior, mu[3], data[nt], cov[6], errcorr)
-    of time-domain GPU grids.  The grids are registered: keep them alive."""
+    """crava_hip_batch over volumes given as (Posterior, mu[3], data[nt], cov[6], errcorr[, corr])
+    of time-domain GPU grids (corr: a separable prior's correlation grid, cov
+    then outputs only).  The grids are registered: keep them alive."""
 
     def __init__(self, ctx: Context, precision: int, volumes):
         self.ctx = ctx
         self.volumes = list(volumes)
         arr = (L.BatchVolume * len(self.volumes))()
-        for i, (post, mu, data, cov, ec) in enumerate(self.volumes):
+        for i, vol in enumerate(self.volumes):
+            post, mu, data, cov, ec = vol[:5]
+            corr = vol[5] if len(vol) > 5 else None
             v = arr[i]
+            v.corr = _dev_ptr(corr) if corr is not None else None
             v.post = post.h.value
             for c in range(3):
                 v.mu[c] = _dev_ptr(mu[c])

@@ -82,6 +82,7 @@ int BatchExec::init(const BatchVolume *vols, int n) {
     for (int l = 0; l < kMaxTheta; ++l) pv[v].data[l] = vols_[v].data[l];
     for (int q = 0; q < 6; ++q) pv[v].cov[q] = vols_[v].cov[q];
     pv[v].errcorr = vols_[v].errcorr;
+    pv[v].corr = vols_[v].corr;
     pv[v].counters = counters_ + 2 * v;
   }
   if ((rc = post_.init(precision_, pv.data(), n))) return rc;
@@ -148,9 +149,15 @@ int BatchExec::build_sets() {
       ic->grids[ic->ng] = bv.mu[c];
       ic->scale[ic->ng++] = si;
     }
-    for (int q = 0; q < 6; ++q) {
-      fc->grids[fc->ng] = bv.cov[q];
+    if (bv.corr) {  // separable prior: one correlation transform
+      fc->grids[fc->ng] = bv.corr;
       fc->scale[fc->ng++] = cf;
+    }
+    for (int q = 0; q < 6; ++q) {
+      if (!bv.corr) {
+        fc->grids[fc->ng] = bv.cov[q];
+        fc->scale[fc->ng++] = cf;
+      }
       ia->grids[ia->ng] = bv.cov[q];
       ia->scale[ia->ng++] = ei;
     }
@@ -228,7 +235,7 @@ int BatchExec::launch_post(hipStream_t origin) {
       const BatchVolume &bv = vols_[v];
       const DeferQueue *q = post_.vq.empty() ? nullptr : &post_.vq[v];
       if ((rc = launch_posterior(*bv.pd, precision_, bv.mu, bv.data, bv.cov, bv.errcorr, counters_ + 2 * v, q,
-                                 side_[work_[v].stream[0]])))
+                                 side_[work_[v].stream[0]], bv.corr)))
         return rc;
     }
     return join(origin);
@@ -289,7 +296,8 @@ int BatchExec::set_launch(int ppb, int blocks, int form) {
 int BatchExec::backup_grids(hipStream_t s, bool restore) {
   const bool f64 = precision_ == CRAVA_HIP_F64;
   size_t total = 0;
-  for (const BatchVolume &bv : vols_) total += real_grid_bytes(bv.pd->pp, f64) * (size_t)(10 + bv.pd->pp.ntheta);
+  for (const BatchVolume &bv : vols_)
+    total += real_grid_bytes(bv.pd->pp, f64) * (size_t)(10 + bv.pd->pp.ntheta + (bv.corr ? 1 : 0));
   if (!restore && total > backup_bytes_) {
     if (backup_) (void)hipFree(backup_);
     backup_ = nullptr;
@@ -305,6 +313,7 @@ int BatchExec::backup_grids(hipStream_t s, bool restore) {
     for (int l = 0; l < bv.pd->pp.ntheta; ++l) gl.push_back(bv.data[l]);
     for (int q = 0; q < 6; ++q) gl.push_back(bv.cov[q]);
     gl.push_back(bv.errcorr);
+    if (bv.corr) gl.push_back(bv.corr);
     for (void *g : gl) {
       if (restore) CRAVA_HIP_CHECK(hipMemcpyAsync(g, b, gb, hipMemcpyDeviceToDevice, s));
       else CRAVA_HIP_CHECK(hipMemcpyAsync(b, g, gb, hipMemcpyDeviceToDevice, s));

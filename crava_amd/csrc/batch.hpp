@@ -18,6 +18,7 @@ struct BatchVolume {
   void *data[kMaxTheta] = {nullptr};
   void *cov[6] = {nullptr};
   void *errcorr = nullptr;
+  void *corr = nullptr;   // separable prior: its correlation grid (cov[] are outputs)
 };
 
 class BatchExec {

@@ -396,25 +396,53 @@ int crava_hip_post_destroy(crava_hip_post *post) {
   return CRAVA_HIP_OK;
 }
 
-int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, void *const *data, void *const *cov,
-                       const void *errcorr, long long *n_fail, long long *n_fp64) {
-  if (!post || !mu || !data || !cov || !errcorr) { set_error("NULL argument to crava_hip_post_run"); return CRAVA_HIP_E_INVALID; }
+}  // extern "C"
+
+// Shared body of the posterior entry points: argument checks, the queue of
+// the mixed form (the handle's, allocated on first use, or the static one),
+// the launches and the optional counts.
+static int post_run_impl(const char *name, crava_hip_post *post, int precision, void *const *mu, void *const *data,
+                         void *const *cov, const void *corr, const void *errcorr, long long *n_fail, long long *n_fp64,
+                         bool static_q) {
+  if (!post || !mu || !data || !cov || !errcorr) {
+    set_error(std::string("NULL argument to ") + name);
+    return CRAVA_HIP_E_INVALID;
+  }
   if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) { set_error("invalid precision"); return CRAVA_HIP_E_INVALID; }
+  if (static_q && precision == CRAVA_HIP_MIXED && post->dev.defer_need > kStaticQueueEntries) {
+    set_error("the mixed posterior needs a queue of " + std::to_string(post->dev.defer_need) +
+              " entries; the static queue holds " + std::to_string(kStaticQueueEntries));
+    return CRAVA_HIP_E_INVALID;
+  }
   for (int c = 0; c < 3; ++c) if (!mu[c]) { set_error("NULL mean grid"); return CRAVA_HIP_E_INVALID; }
   for (int q = 0; q < 6; ++q) if (!cov[q]) { set_error("NULL covariance grid"); return CRAVA_HIP_E_INVALID; }
   for (int l = 0; l < post->dev.pp.ntheta; ++l) if (!data[l]) { set_error("NULL data grid"); return CRAVA_HIP_E_INVALID; }
   crava_hip_ctx *ctx = post->ctx;
-  if (precision == CRAVA_HIP_MIXED && !post->queue_mem) {
-    const size_t n = (size_t)std::max<long long>(post->dev.defer_need, 1);
-    CRAVA_HIP_CHECK(hipMalloc(&post->queue_mem, 256 + sizeof(int2) * n));
-    post->queue.count = static_cast<unsigned *>(post->queue_mem);
-    post->queue.list = reinterpret_cast<int2 *>(static_cast<unsigned char *>(post->queue_mem) + 256);
-    post->queue.cap = (long long)n;
+  crava::DeferQueue sq;
+  const crava::DeferQueue *q = nullptr;
+  int rc;
+  if (precision == CRAVA_HIP_MIXED) {
+    if (static_q) {
+      if ((rc = static_queue(&sq))) return rc;
+      q = &sq;
+    } else {
+      if (!post->queue_mem) {
+        const size_t n = (size_t)std::max<long long>(post->dev.defer_need, 1);
+        CRAVA_HIP_CHECK(hipMalloc(&post->queue_mem, 256 + sizeof(int2) * n));
+        post->queue.count = static_cast<unsigned *>(post->queue_mem);
+        post->queue.list = reinterpret_cast<int2 *>(static_cast<unsigned char *>(post->queue_mem) + 256);
+        post->queue.cap = (long long)n;
+      }
+      q = &post->queue;
+    }
   }
   const bool want = n_fail || n_fp64;
-  if (want) CRAVA_HIP_CHECK(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
-  int rc = crava::launch_posterior(post->dev, precision, mu, data, cov, errcorr, want ? ctx->counters : nullptr,
-                                   &post->queue, ctx->stream);
+  if (want) {
+    hipLaunchKernelGGL(k_zero_counts2, dim3(1), dim3(64), 0, ctx->stream, ctx->counters);
+    CRAVA_HIP_CHECK(hipGetLastError());
+  }
+  rc = crava::launch_posterior(post->dev, precision, mu, data, cov, errcorr, want ? ctx->counters : nullptr, q,
+                               ctx->stream, corr);
   if (rc) return rc;
   if (want) {
     unsigned long long h[2] = {0, 0};
@@ -426,41 +454,25 @@ int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, voi
   return CRAVA_HIP_OK;
 }
 
+extern "C" {
+
+int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, void *const *data, void *const *cov,
+                       const void *errcorr, long long *n_fail, long long *n_fp64) {
+  return post_run_impl("crava_hip_post_run", post, precision, mu, data, cov, nullptr, errcorr, n_fail, n_fp64, false);
+}
+
+int crava_hip_post_run_separable(crava_hip_post *post, int precision, void *const *mu, void *const *data,
+                                 const void *corr, const void *errcorr, void *const *cov_out, long long *n_fail,
+                                 long long *n_fp64) {
+  if (!corr) { set_error("NULL correlation grid"); return CRAVA_HIP_E_INVALID; }
+  return post_run_impl("crava_hip_post_run_separable", post, precision, mu, data, cov_out, corr, errcorr, n_fail,
+                       n_fp64, false);
+}
+
 int crava_hip_post_run_static(crava_hip_post *post, int precision, void *const *mu, void *const *data,
                               void *const *cov, const void *errcorr, long long *n_fail, long long *n_fp64) {
-  if (!post || !mu || !data || !cov || !errcorr) {
-    set_error("NULL argument to crava_hip_post_run_static");
-    return CRAVA_HIP_E_INVALID;
-  }
-  if (precision < CRAVA_HIP_F32 || precision > CRAVA_HIP_F64) { set_error("invalid precision"); return CRAVA_HIP_E_INVALID; }
-  if (precision == CRAVA_HIP_MIXED && post->dev.defer_need > kStaticQueueEntries) {
-    set_error("the mixed posterior needs a queue of " + std::to_string(post->dev.defer_need) +
-              " entries; the static queue holds " + std::to_string(kStaticQueueEntries));
-    return CRAVA_HIP_E_INVALID;
-  }
-  for (int c = 0; c < 3; ++c) if (!mu[c]) { set_error("NULL mean grid"); return CRAVA_HIP_E_INVALID; }
-  for (int q = 0; q < 6; ++q) if (!cov[q]) { set_error("NULL covariance grid"); return CRAVA_HIP_E_INVALID; }
-  for (int l = 0; l < post->dev.pp.ntheta; ++l) if (!data[l]) { set_error("NULL data grid"); return CRAVA_HIP_E_INVALID; }
-  crava_hip_ctx *ctx = post->ctx;
-  crava::DeferQueue q;
-  int rc;
-  if (precision == CRAVA_HIP_MIXED && (rc = static_queue(&q))) return rc;
-  const bool want = n_fail || n_fp64;
-  if (want) {
-    hipLaunchKernelGGL(k_zero_counts2, dim3(1), dim3(64), 0, ctx->stream, ctx->counters);
-    CRAVA_HIP_CHECK(hipGetLastError());
-  }
-  rc = crava::launch_posterior(post->dev, precision, mu, data, cov, errcorr, want ? ctx->counters : nullptr,
-                               precision == CRAVA_HIP_MIXED ? &q : nullptr, ctx->stream);
-  if (rc) return rc;
-  if (want) {
-    unsigned long long h[2] = {0, 0};
-    CRAVA_HIP_CHECK(hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    if (n_fail) *n_fail = (long long)h[0];
-    if (n_fp64) *n_fp64 = (long long)h[1];
-  }
-  return CRAVA_HIP_OK;
+  return post_run_impl("crava_hip_post_run_static", post, precision, mu, data, cov, nullptr, errcorr, n_fail, n_fp64,
+                       true);
 }
 
 int crava_hip_nll_grad(crava_hip_post *post, int precision, void *const *mu, void *const *data, void *const *cov,
@@ -1028,6 +1040,7 @@ int crava_hip_batch_create(crava_hip_ctx *ctx, int precision, int nvol, const cr
     for (int l = 0; l < crava::kMaxTheta; ++l) bv[v].data[l] = vols[v].data[l];
     for (int q = 0; q < 6; ++q) bv[v].cov[q] = vols[v].cov[q];
     bv[v].errcorr = vols[v].errcorr;
+    bv[v].corr = vols[v].corr;
   }
   crava_hip_batch *b = new (std::nothrow) crava_hip_batch(ctx, precision);
   if (!b) { set_error("out of host memory"); return CRAVA_HIP_E_OOM; }

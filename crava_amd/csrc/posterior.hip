@@ -33,6 +33,12 @@ struct PostArgs {
   cx<G> *d[kMaxTheta];
   cx<G> *cov[6];
   const cx<G> *ec;
+  // Separable prior Sigma0 (x) corr (SeismicParametersHolder::
+  // initializeCorrelations, src/seismicparametersholder.cpp:197-221, fills
+  // the six grids with the one correlation times Sigma0_ab): the prior is
+  // read from this one grid, S = |Re corr| Sigma0, which is the cleaned
+  // prior of the six grids in exact arithmetic; cov[] are then outputs only.
+  const cx<G> *corr;
   const PerK<float> *tabf;
   const PerK<double> *tabd;
   unsigned long long *counters;  // [0] failed Cholesky, [1] points solved in FP64
@@ -330,11 +336,18 @@ __device__ __forceinline__ bool solve_sqrt(const PerK<float> *__restrict__ tb, c
   return true;
 }
 
-// Cleaned prior S of point p (src/seismicparametersholder.cpp:373-426).
+// Cleaned prior S of point p (src/seismicparametersholder.cpp:373-426), or
+// |Re corr| Sigma0 for a separable prior.
 template <typename G>
 __device__ __forceinline__ void load_prior(const PostArgs<G> &a, long long p, G (&S)[6]) {
   const G *s0 = nullptr;
   if constexpr (sizeof(G) == 4) s0 = a.s0f; else s0 = a.s0d;
+  if (a.corr) {
+    const G c = fabs(a.corr[p].x);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) S[q] = c * s0[q];
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     G v = a.cov[q][p].x;
@@ -695,12 +708,13 @@ static int dispatch(const PostArgs<G> &a, int nt, int nzp, long long cap, hipStr
 
 template <typename G>
 static void fill_args(PostArgs<G> &a, const PostDevice &pd, void *const *mu, void *const *data, void *const *cov,
-                      const void *errcorr, unsigned long long *counters) {
+                      const void *errcorr, unsigned long long *counters, const void *corr = nullptr) {
   const PostProblem &pp = pd.pp;
   for (int c = 0; c < 3; ++c) a.mu[c] = reinterpret_cast<cx<G> *>(mu[c]);
   for (int l = 0; l < kMaxTheta; ++l) a.d[l] = l < pp.ntheta ? reinterpret_cast<cx<G> *>(data[l]) : nullptr;
   for (int q = 0; q < 6; ++q) a.cov[q] = reinterpret_cast<cx<G> *>(cov[q]);
   a.ec = reinterpret_cast<const cx<G> *>(errcorr);
+  a.corr = reinterpret_cast<const cx<G> *>(corr);
   a.tabf = pd.tabf;
   a.tabd = pd.tabd;
   a.counters = counters;
@@ -724,15 +738,16 @@ static void fill_args(PostArgs<G> &a, const PostDevice &pd, void *const *mu, voi
 }
 
 int launch_posterior(const PostDevice &pd, int precision, void *const *mu, void *const *data, void *const *cov,
-                     const void *errcorr, unsigned long long *counters, const DeferQueue *defer, hipStream_t s) {
+                     const void *errcorr, unsigned long long *counters, const DeferQueue *defer, hipStream_t s,
+                     const void *corr) {
   const int nt = pd.pp.ntheta;
   if (precision == CRAVA_HIP_F64) {
     PostArgs<double> a{};
-    fill_args(a, pd, mu, data, cov, errcorr, counters);
+    fill_args(a, pd, mu, data, cov, errcorr, counters, corr);
     return dispatch<double, 0>(a, nt, pd.pp.nzp, 0, s);
   }
   PostArgs<float> a{};
-  fill_args(a, pd, mu, data, cov, errcorr, counters);
+  fill_args(a, pd, mu, data, cov, errcorr, counters, corr);
   if (precision == CRAVA_HIP_MIXED) {
     if (!defer || !defer->list || !defer->count || defer->cap < pd.defer_need) {
       set_error("the mixed posterior needs a queue of " + std::to_string(pd.defer_need) + " entries");
@@ -789,11 +804,11 @@ int PostBatch::init(int prec, const PostVolume *vols, int n) {
     plane[v] = (long long)(pp.nxp / 2 + 1) * pp.nyp;
     if (prec == CRAVA_HIP_F64) {
       PostArgs<double> a{};
-      fill_args(a, *pv.pd, pv.mu, pv.data, pv.cov, pv.errcorr, pv.counters);
+      fill_args(a, *pv.pd, pv.mu, pv.data, pv.cov, pv.errcorr, pv.counters, pv.corr);
       std::memcpy(host.data() + asz * v, &a, sizeof(a));
     } else {
       PostArgs<float> a{};
-      fill_args(a, *pv.pd, pv.mu, pv.data, pv.cov, pv.errcorr, pv.counters);
+      fill_args(a, *pv.pd, pv.mu, pv.data, pv.cov, pv.errcorr, pv.counters, pv.corr);
       a.defer_list = vq[v].list;
       a.defer_count = vq[v].count;
       std::memcpy(host.data() + asz * v, &a, sizeof(a));

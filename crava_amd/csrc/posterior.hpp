@@ -67,8 +67,11 @@ struct PostDevice {
 // noise/signal threshold or whose FP32 factorisations fail; `defer` must
 // hold defer_need entries), 2 = FP64 (complex128 grids).  Launches: FP32 /
 // FP64: one; mixed: queue reset, FP32 kernel, FP64 kernel over the queue.
+// corr: NULL, or the one Fourier-domain correlation grid of a prior
+// Sigma0 (x) corr (the cov grids are then outputs only; see PostArgs::corr).
 int launch_posterior(const PostDevice &pd, int precision, void *const *mu, void *const *data, void *const *cov,
-                     const void *errcorr, unsigned long long *counters, const DeferQueue *defer, hipStream_t s);
+                     const void *errcorr, unsigned long long *counters, const DeferQueue *defer, hipStream_t s,
+                     const void *corr = nullptr);
 
 // One volume of a batched posterior: its problem and device grids.
 struct PostVolume {
@@ -77,6 +80,7 @@ struct PostVolume {
   void *data[kMaxTheta] = {nullptr};
   void *cov[6] = {nullptr};
   const void *errcorr = nullptr;
+  const void *corr = nullptr;               // separable prior: its correlation grid (else NULL)
   unsigned long long *counters = nullptr;  // device [2]: failed Cholesky, FP64 points
 };
 

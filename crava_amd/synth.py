@@ -137,6 +137,7 @@ class Problem:
     data: list = field(default_factory=list)    # nt grids
     cov: list = field(default_factory=list)     # 6 grids (aa, bb, rr, ab, ar, br)
     errcorr: torch.Tensor | None = None
+    corr: torch.Tensor | None = None            # the correlation grid: cov[q] = corr * Sigma0_q
     true_m: list = field(default_factory=list)
 
     @property
@@ -250,11 +251,11 @@ def make_problem(cfg: Config, device: str = "cpu", grids: bool = True) -> Proble
     # covariance grids: (circCorrT x corrXY) in float, times Sigma0_ab in float
     base32 = (torch.from_numpy(cct).to(dev)[:, None, None] * torch.from_numpy(cxy).to(dev)[None, :, :])
     pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    prob.corr = _padded(base32, rnxp, RMISSING)
     prob.cov = []
     for (a, b) in pairs:
         s = np.float32(sig0[a, b])
-        g = _padded(base32, rnxp, RMISSING)
-        prob.cov.append(g * float(s))
+        prob.cov.append(prob.corr * float(s))
     ec = torch.zeros((nzp, nyp, nxp), dtype=torch.float32, device=dev)
     ec[0] = torch.from_numpy(cxy).to(dev)
     prob.errcorr = _padded(ec, rnxp, RMISSING)

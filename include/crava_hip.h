@@ -191,6 +191,22 @@ int crava_hip_post_destroy(crava_hip_post *post);
 int crava_hip_post_run(crava_hip_post *post, int precision, void *const *mu, void *const *data, void *const *cov,
                        const void *errcorr, long long *n_fail, long long *n_fp64);
 
+/* The same posterior for a separable prior covariance Sigma0 (x) corr, the
+ * form SeismicParametersHolder::initializeCorrelations builds
+ * (src/seismicparametersholder.cpp:197-221: the six covariance grids are
+ * one correlation grid times priorVar0_(a,b)).  The prior comes from the
+ * one Fourier-domain correlation grid `corr` (unscaled transform, read
+ * only) as S(p) = |Re corr(p)| Sigma0, which equals the cleaned prior of the
+ * six transformed grids (src/seismicparametersholder.cpp:373-426) in exact
+ * arithmetic; cov_out[6] (order aa, bb, rr, ab, ar, br) receive the
+ * posterior covariance (the cleaned prior out of band and where the
+ * factorisation fails).  mu, data, errcorr, precision and the counts as
+ * crava_hip_post_run.  The caller transforms one correlation grid instead
+ * of six and the kernel reads one prior grid instead of six. */
+int crava_hip_post_run_separable(crava_hip_post *post, int precision, void *const *mu, void *const *data,
+                                 const void *corr, const void *errcorr, void *const *cov_out, long long *n_fail,
+                                 long long *n_fp64);
+
 /* crava_hip_post_run that allocates nothing.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
@@ -341,16 +357,19 @@ typedef struct {
   void *data[8];    /* seismic data, time domain        -> residuals, time domain (first ntheta used) */
   void *cov[6];     /* prior covariance (aa,bb,rr,ab,ar,br), time domain -> posterior covariance, time domain */
   void *errcorr;    /* error correlation, time domain   -> its spectrum (consumed) */
+  void *corr;       /* NULL, or a separable prior's correlation grid, time domain -> its spectrum
+                       (consumed); cov[] are then outputs only (crava_hip_post_run_separable) */
 } crava_hip_batch_volume;
 
 /* Many volumes of uneven size and ntheta inverted as one sequence.  No
  * reference counterpart: the reference inverts one volume per Crava object
  * (main.cpp:279-308).  Per volume, as Crava::computePostMeanResidAndFFTCov
  * (src/crava.cpp:887-1204): forward transforms of data and mean (scaled as
- * DATA/PARAMETER) and of the covariance and error-correlation grids
- * (COVARIANCE; CRAVA_HIP_MIXED transforms the means forward and back and the
- * covariances forward with double arithmetic in each pass), the posterior, then
- * inverse transforms of mean, residuals and covariances.  All transforms of
+ * DATA/PARAMETER) and of the covariance (or separable correlation) and
+ * error-correlation grids (COVARIANCE), the posterior, then inverse
+ * transforms of mean, residuals and covariances.  CRAVA_HIP_MIXED transforms
+ * the covariance (or correlation) grids forward with double arithmetic in
+ * each pass and the means with CRAVA_HIP_F32_DEFLATE.  All transforms of
  * one volume and kind are one launch per pass; the posterior of every volume
  * is one persistent launch.  The grids are registered here and must stay
  * allocated, like `ctx` and every `post`, until crava_hip_batch_destroy. */

@@ -36,21 +36,22 @@ def _problems(n, seed, nxy, nzr):
 class _Vol:
     """Device grids of one volume plus their pristine time-domain inputs."""
 
-    def __init__(self, ctx, prob, thr):
+    def __init__(self, ctx, prob, thr, separable=False):
         self.prob = prob
-        self.src = [g.cuda() for g in prob.mu + prob.data + prob.cov + [prob.errcorr]]
+        self.separable = separable
+        self.src = [g.cuda() for g in prob.mu + prob.data + prob.cov + [prob.errcorr, prob.corr.to(prob.errcorr.dtype)]]
         self.grids = [g.clone() for g in self.src]
         self.post = Posterior(ctx, synth.post_params(prob, thr), problem_tables(prob))
         nt = prob.ntheta
         self.mu, self.data = self.grids[:3], self.grids[3:3 + nt]
-        self.cov, self.ec = self.grids[3 + nt:9 + nt], self.grids[9 + nt]
+        self.cov, self.ec, self.corr = self.grids[3 + nt:9 + nt], self.grids[9 + nt], self.grids[10 + nt]
 
     def reload(self):
         for d, s in zip(self.grids, self.src):
             d.copy_(s)
 
     def entry(self):
-        return (self.post, self.mu, self.data, self.cov, self.ec)
+        return (self.post, self.mu, self.data, self.cov, self.ec) + ((self.corr,) if self.separable else ())
 
     def outputs(self):
         return [g.clone() for g in self.mu + self.data + self.cov]
@@ -68,8 +69,8 @@ def _single_path(ctx, v, precision):
     ctx.r2c_many(v.data, *n, L.DATA)
     ctx.r2c(v.ec, *n, L.COVARIANCE)
     ctx.r2c_many(v.mu, *n, L.DATA, md)
-    ctx.r2c_many(v.cov, *n, L.COVARIANCE, mc)
-    nf, n64 = v.post.run(precision, v.mu, v.data, v.cov, v.ec, counts=True)
+    ctx.r2c_many([v.corr] if v.separable else v.cov, *n, L.COVARIANCE, mc)
+    nf, n64 = v.post.run(precision, v.mu, v.data, v.cov, v.ec, counts=True, corr=v.corr if v.separable else None)
     ctx.c2r_many(v.mu, *n, L.DATA, md)
     ctx.c2r_many(v.data, *n, L.DATA)
     ctx.c2r_many(v.cov, *n, L.COVARIANCE)
@@ -113,16 +114,17 @@ def test_batch_small_uneven_vs_oracle(ctx):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", [L.MIXED, L.F64])
-def test_batch_matches_single_volume_path_and_graph_replays(ctx, precision):
-    print(f"batch vs single path, precision {precision}", flush=True)
+@pytest.mark.parametrize("precision,separable", [(L.MIXED, False), (L.F64, False), (L.MIXED, True), (L.F64, True)],
+                         ids=["mixed", "f64", "mixed-separable", "f64-separable"])
+def test_batch_matches_single_volume_path_and_graph_replays(ctx, precision, separable):
+    print(f"batch vs single path, precision {precision}, separable {separable}", flush=True)
     probs = _problems(8, 5, (32, 192), (64, 256))
     if precision == L.F64:
         for p in probs:
             for lst in (p.mu, p.data, p.cov):
                 lst[:] = [g.double() for g in lst]
             p.errcorr = p.errcorr.double()
-    vols = [_Vol(ctx, p, 0.1) for p in probs]
+    vols = [_Vol(ctx, p, 0.1, separable) for p in probs]
     b = Batch(ctx, precision, [v.entry() for v in vols])
 
     # reference: volume by volume through the single-volume entry points

@@ -1,0 +1,258 @@
+"""The inversion workloads at their full sizes on the GPU, checked on sampled
+k-planes against the oracles run on the same Fourier-domain inputs, plus
+properties that hold everywhere:
+
+* 512x512x320, 6 angles, S/N 1e4..20 (the precision workload): the MIXED
+  path (data and error correlation in float, means with mean deflation,
+  covariances with double arithmetic in each pass, MIXED posterior at
+  threshold 1e-3) -- every output of 7 in-band and 6 out-of-band k-planes
+  within 1e-5 of the FP64 oracle; out of band the means and data pass
+  through bit for bit and the covariance is the cleaned prior bit for bit;
+  in band the posterior variances do not exceed the prior ones; the
+  transforms' round trips;
+* 288^3, 5 angles (the plain GPU drop-in): FP32 path against the float
+  oracle (1e-6 means, 2e-5 the rest, as the small-grid FP32 tests) and MIXED
+  against the FP64 oracle on sampled planes;
+* 288x288x216, 5 angles (the differentiation workload): the likelihood and
+  its adjoint gradient restricted to 8 sampled in-band planes (the
+  per-k frequency gate opened only there) against the FP64 torch oracle on
+  those planes (1e-9 value, 1e-7 gradient), and central differences
+  (relative h = 1e-6) of the full in-band likelihood on all 16 parameters.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from crava_amd import _lib as L
+from crava_amd import synth
+from crava_amd.autograd import nll_grad
+from crava_amd.core import Posterior, problem_tables
+from tests.helpers import rel_l2
+
+PAIRS = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+
+
+def _cplx(g):
+    return g.view(np.complex64 if g.dtype == np.float32 else np.complex128)
+
+
+def _planes(grids, ks):
+    return [np.ascontiguousarray(g[ks].cpu().numpy()) for g in grids]
+
+
+def _sub_tabs(tabs, ks):
+    return {k: np.ascontiguousarray(v[ks]) for k, v in tabs.items()}
+
+
+def _up(g):
+    return np.ascontiguousarray(_cplx(g).astype(np.complex128)).view(np.float64)
+
+
+def _mixed_forward(ctx, prob, n):
+    ctx.r2c_many(prob.data, *n, L.DATA)
+    ctx.r2c(prob.errcorr, *n, L.COVARIANCE)
+    ctx.r2c_many(prob.mu, *n, L.DATA, L.F32_DEFLATE)
+    ctx.r2c_many(prob.cov, *n, L.COVARIANCE, L.MIXED)
+
+
+def _check_vs_fp64(prob, ks, tabs, before, after, tol, label):
+    """after (GPU) vs the FP64 oracle on the sampled planes of `before`."""
+    cfg = prob.cfg
+    nt = prob.ntheta
+    mu, d, cv, ec = before[:3], before[3:3 + nt], before[3 + nt:9 + nt], before[9 + nt]
+    om, od, oc = [_up(g) for g in mu], [_up(g) for g in d], [_up(g) for g in cv]
+    pm = [g.copy() for g in om]
+    nf = oracle.posterior_f64(cfg.nxp, cfg.nyp, len(ks), prob.A, prob.err_theta_cov, cfg.wnc, prob.sigma0,
+                              _sub_tabs(tabs, ks), om, od, oc, _up(ec))
+    gm, gd, gc = after[:3], after[3:3 + nt], after[3 + nt:9 + nt]
+    e = {"mean_update": max(rel_l2(_cplx(a).astype(np.complex128) - _cplx(p), _cplx(b) - _cplx(p))
+                            for a, b, p in zip(gm, om, pm)),
+         "mean": max(rel_l2(_cplx(a), _cplx(b)) for a, b in zip(gm, om)),
+         "cov": max(rel_l2(_cplx(a), _cplx(b)) for a, b in zip(gc, oc)),
+         "residual": max(rel_l2(_cplx(a), _cplx(b)) for a, b in zip(gd, od))}
+    print(f"  {label}: errors vs the FP64 oracle on k-planes {list(ks)}: {e}", flush=True)
+    assert nf == 0
+    for k, v in e.items():
+        assert v < tol, (label, k, v)
+
+
+def _passthrough_and_variance(prob, tabs, ks, before, after):
+    nt = prob.ntheta
+    for i, k in enumerate(ks):
+        if tabs["invert"][k]:
+            for q in range(3):      # posterior variance <= prior variance (relative rounding)
+                a, b = PAIRS[q]
+                s = np.float32(prob.sigma0[a, b])
+                prior = np.abs(_cplx(before[3 + nt + q][i]).real / s) * s
+                post = _cplx(after[3 + nt + q][i]).real
+                assert np.all(post <= prior * (1 + 1e-5) + 1e-30), (k, q)
+                assert np.all(post >= 0), (k, q)
+            continue
+        for g0, g1 in zip(before[:3 + nt], after[:3 + nt]):
+            assert np.array_equal(g0[i].view(np.uint32), g1[i].view(np.uint32)), k
+        for q, (a, b) in enumerate(PAIRS):
+            s = np.float32(prob.sigma0[a, b])
+            v = _cplx(before[3 + nt + q][i]).real
+            want = (np.abs(v / s) * s).astype(np.float32)
+            got = _cplx(after[3 + nt + q][i])
+            assert np.array_equal(got.real.view(np.uint32), want.view(np.uint32)), (k, q)
+            assert np.all(got.imag == 0), (k, q)
+
+
+@pytest.mark.gpu
+def test_precision_workload_full_size():
+    from crava_amd.core import Context
+    ctx = Context()
+    print("512x512x320 precision workload: generating", flush=True)
+    prob = synth.make_problem(synth.config("precision"), "cuda")
+    cfg = prob.cfg
+    n = (cfg.nxp, cfg.nyp, cfg.nzp)
+    tabs = problem_tables(prob)
+    ks_in = [7, 20, 45, 70, 250, 290, 313]
+    ks_out = [0, 3, 71, 160, 249, 319]
+    assert all(tabs["invert"][k] for k in ks_in) and not any(tabs["invert"][k] for k in ks_out)
+    ks = ks_in + ks_out
+    # round trips of one data grid (float) and one mean grid (mean deflation)
+    d0, m0 = prob.data[0].clone(), prob.mu[0].clone()
+    t = d0.clone()
+    ctx.r2c(t, *n, L.DATA)
+    ctx.c2r(t, *n, L.DATA)
+    assert rel_l2(t[:, :, :cfg.nxp].cpu().numpy(), d0[:, :, :cfg.nxp].cpu().numpy()) < 1e-6
+    t = m0.clone()
+    ctx.r2c(t, *n, L.DATA, L.F32_DEFLATE)
+    ctx.c2r(t, *n, L.DATA, L.F32_DEFLATE)
+    dev = (t - m0)[:, :, :cfg.nxp].abs()
+    assert float(torch.sqrt(torch.mean(dev.double() ** 2))) < 2 * 2.0 ** -20 and float(dev.max()) < 4 * 2.0 ** -20
+    del t, d0, m0
+    _mixed_forward(ctx, prob, n)
+    grids = prob.mu + prob.data + prob.cov
+    before = _planes(grids + [prob.errcorr], ks)
+    post = Posterior(ctx, synth.post_params(prob, 1e-3), tabs)
+    nf, n64 = post.run(L.MIXED, prob.mu, prob.data, prob.cov, prob.errcorr, counts=True)
+    post.close()
+    torch.cuda.synchronize()
+    after = _planes(grids, ks)
+    print(f"  {nf} failed factorisations, {n64} FP64 points", flush=True)
+    assert nf == 0
+    m = len(ks_in)
+    _check_vs_fp64(prob, np.array(ks_in), tabs, [b[:m] for b in before], [a[:m] for a in after], 1e-5,
+                   "MIXED 512x512x320")
+    _passthrough_and_variance(prob, tabs, ks, before, after)
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_plain_gpu_workload_full_size():
+    from crava_amd.core import Context
+    ctx = Context()
+    print("288^3 plain GPU workload: generating", flush=True)
+    prob = synth.make_problem(synth.config("plain_gpu"), "cuda")
+    cfg = prob.cfg
+    n = (cfg.nxp, cfg.nyp, cfg.nzp)
+    nt = prob.ntheta
+    tabs = problem_tables(prob)
+    inb = np.nonzero(tabs["invert"])[0]
+    ks = np.array(sorted(set([int(inb[0]), int(inb[len(inb) // 3]), int(inb[len(inb) // 2]), int(inb[-1]), 0, 144])))
+    src = [g.clone() for g in prob.mu + prob.data + prob.cov + [prob.errcorr]]
+    # FP32: the reference's float path against the float oracle
+    ctx.r2c_many(prob.data + prob.mu, *n, L.DATA)
+    ctx.r2c_many(prob.cov + [prob.errcorr], *n, L.COVARIANCE)
+    grids = prob.mu + prob.data + prob.cov
+    before = _planes(grids + [prob.errcorr], ks)
+    post = Posterior(ctx, synth.post_params(prob, 1e-3), tabs)
+    nf, _ = post.run(L.F32, prob.mu, prob.data, prob.cov, prob.errcorr, counts=True)
+    torch.cuda.synchronize()
+    after = _planes(grids, ks)
+    mu, d, cv, ec = [g.copy() for g in before[:3]], [g.copy() for g in before[3:3 + nt]], \
+        [g.copy() for g in before[3 + nt:9 + nt]], before[9 + nt].copy()
+    nf_ref = oracle.posterior(cfg.nxp, cfg.nyp, len(ks), prob.A, prob.err_theta_cov, cfg.wnc, prob.sigma0,
+                              _sub_tabs(tabs, ks), mu, d, cv, ec)
+    e32 = {"mean": max(rel_l2(_cplx(a), _cplx(b)) for a, b in zip(after[:3], mu)),
+           "rest": max(rel_l2(_cplx(a), _cplx(b)) for a, b in zip(after[3:], d + cv))}
+    print(f"  FP32 vs the float oracle on k-planes {list(ks)}: {e32}", flush=True)
+    assert nf == nf_ref
+    assert e32["mean"] < 1e-6 and e32["rest"] < 2e-5
+    _passthrough_and_variance(prob, tabs, ks, before, after)
+    # MIXED against the FP64 oracle on the same planes
+    for dst, s in zip(prob.mu + prob.data + prob.cov + [prob.errcorr], src):
+        dst.copy_(s)
+    _mixed_forward(ctx, prob, n)
+    before = _planes(grids + [prob.errcorr], ks)
+    post.run(L.MIXED, prob.mu, prob.data, prob.cov, prob.errcorr)
+    torch.cuda.synchronize()
+    after = _planes(grids, ks)
+    sel = np.array([i for i, k in enumerate(ks) if tabs["invert"][k]])
+    _check_vs_fp64(prob, ks[sel], tabs, [b[sel] for b in before], [a[sel] for a in after], 1e-5, "MIXED 288^3")
+    post.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_differentiation_workload_full_size():
+    from crava_amd.core import Context
+    from oracle import nll_torch
+    ctx = Context()
+    print("288x288x216 differentiation workload: generating", flush=True)
+    prob = synth.make_problem(synth.config("differentiation"), "cuda")
+    cfg = prob.cfg
+    n = (cfg.nxp, cfg.nyp, cfg.nzp)
+    nt = prob.ntheta
+    tabs = problem_tables(prob)
+    mu = [g.double() for g in prob.mu]
+    data = [g.double() for g in prob.data]
+    cv = [g.double() for g in prob.cov]
+    ec = prob.errcorr.double()
+    ctx.r2c_many(mu + data, *n, L.DATA)
+    ctx.r2c_many(cv + [ec], *n, L.COVARIANCE)
+    torch.cuda.synchronize()
+    L0 = np.linalg.cholesky(prob.sigma0)
+    s0 = np.array([1.08, 0.93, 1.05, 0.96, 1.1])[:nt]
+    v0 = np.diag(prob.err_theta_cov) * 1.4
+    l0 = np.array([L0[0, 0], L0[1, 0], L0[1, 1], L0[2, 0], L0[2, 1], L0[2, 2]])
+    # (1) 8 sampled in-band planes: the frequency gate opened only there
+    inb = np.nonzero(tabs["invert"])[0]
+    ks = inb[np.linspace(0, len(inb) - 1, 8).astype(int)]
+    sub = {k: v.copy() for k, v in tabs.items()}
+    sub["invert"] = np.zeros_like(tabs["invert"])
+    sub["invert"][ks] = 1
+    post = Posterior(ctx, synth.post_params(prob), sub)
+    val, grad = nll_grad(post, L.F64, mu, data, cv, ec, s0.tolist(), v0.tolist(), l0.tolist())
+    post.close()
+    host = lambda g: _cplx(np.ascontiguousarray(g.view(cfg.nzp, cfg.nyp, -1).cpu().numpy()))
+    P = nll_torch.prepare(sub, prob.A, prob.err_theta_cov, cfg.wnc, prob.sigma0[0, 0], [host(g) for g in mu],
+                          [host(g) for g in data], host(cv[0]), host(ec), cfg.nxp)
+    th = [torch.tensor(x, dtype=torch.float64, requires_grad=True) for x in (s0, v0, l0)]
+    ref = nll_torch.nll(P, *th)
+    ref.backward()
+    gref = torch.cat([t.grad for t in th]).numpy()
+    print(f"  sampled planes {list(ks)}: NLL {val:.10g} vs oracle {float(ref):.10g}; "
+          f"max gradient difference {np.abs(np.array(grad) - gref).max():.3e} of {np.abs(gref).max():.3e}",
+          flush=True)
+    assert abs(val - float(ref)) <= 1e-9 * abs(float(ref))
+    assert np.abs(np.array(grad) - gref).max() <= 1e-7 * np.abs(gref).max()
+    del P, ref
+    # (2) every in-band plane: the adjoint against central differences
+    post = Posterior(ctx, synth.post_params(prob), tabs)
+    x0 = np.concatenate([s0, v0, l0])
+    f0, g0 = nll_grad(post, L.F64, mu, data, cv, ec, s0.tolist(), v0.tolist(), l0.tolist())
+    g0 = np.array(g0)
+    worst = 0.0
+    for i in range(len(x0)):
+        h = 1e-6 * max(abs(x0[i]), 1e-3 * np.abs(x0[2 * nt:]).max() if i >= 2 * nt else abs(x0[i]))
+        xp, xm = x0.copy(), x0.copy()
+        xp[i] += h
+        xm[i] -= h
+        fp, _ = nll_grad(post, L.F64, mu, data, cv, ec, xp[:nt].tolist(), xp[nt:2 * nt].tolist(),
+                         xp[2 * nt:].tolist())
+        fm, _ = nll_grad(post, L.F64, mu, data, cv, ec, xm[:nt].tolist(), xm[nt:2 * nt].tolist(),
+                         xm[2 * nt:].tolist())
+        fd = (fp - fm) / (2 * h)
+        res = 8 * np.finfo(np.float64).eps * abs(f0) / h      # the difference quotient's own resolution
+        tol = 1e-6 * abs(g0[i]) + 10 * res + 1e-7 * np.abs(g0).max()
+        worst = max(worst, abs(fd - g0[i]) / tol)
+        assert abs(fd - g0[i]) <= tol, (i, fd, g0[i], tol)
+    print(f"  central differences on all {len(x0)} parameters: worst |fd - adjoint| / tolerance = {worst:.3f}",
+          flush=True)
+    post.close()
+    ctx.close()
