@@ -157,6 +157,10 @@ SIGNATURES = {
     "crava_hip_grid_read_trace": (I, [VP, VP, I, I, I, I, I, I, I, VP]),
     "crava_hip_complex_noise_reference": (I, [I, I, I, C.POINTER(C.c_uint32), VP]),
     "crava_hip_reflection_matrix": (I, [I, VP, VP, C.c_double, VP]),
+    "crava_hip_perk_tables_nonconst": (I, [I, I, I, C.c_double, C.c_float, C.c_float, C.c_float, VP, VP, VP, VP, VP,
+                                           VP, VP]),
+    "crava_hip_wavelet_resample": (I, [VP, I, I, C.c_float, C.c_float, I, VP, C.POINTER(C.c_float)]),
+    "crava_hip_wavelet_energy": (I, [I, I, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "crava_hip_divide_by_scale_wavelet": (I, [VP, C.POINTER(SwdivParams), C.POINTER(VP)]),
     "crava_hip_interpolate_seismic": (I, [VP, I, I, I, I, I, I, C.c_float, VP, C.POINTER(I), C.POINTER(I)]),
     "crava_hip_local_noise_g": (I, [I, VP, VP, VP, VP]),

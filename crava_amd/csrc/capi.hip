@@ -297,6 +297,44 @@ int crava_hip_perk_tables(int ntheta, int nz, int nzp, double lz, float low_cut,
   return CRAVA_HIP_OK;
 }
 
+int crava_hip_perk_tables_nonconst(int ntheta, int nz, int nzp, double lz, float low_cut, float high_cut,
+                                   float min_rel_thick, const float *wavelets, const float *wnorm,
+                                   unsigned char *invert, float *kWD, float *e1, float *e2, float *e3) {
+  if (ntheta < 1 || ntheta > crava::kMaxTheta || nzp < 2 || nz < 1 || !wavelets || !wnorm || !invert || !kWD ||
+      !e1 || !e2 || !e3) {
+    set_error("invalid arguments to crava_hip_perk_tables_nonconst");
+    return CRAVA_HIP_E_INVALID;
+  }
+  crava::perk_vectors_nonconst(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wavelets, wnorm, invert, kWD,
+                               e1, e2, e3);
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_wavelet_resample(const float *wavelet, int nw, int center, float dz_wavelet, float dz, int nzp,
+                               float *out, float *norm) {
+  if (!wavelet || !out || nw < 1 || nzp < 2 || !(dz_wavelet > 0.0f) || !(dz > 0.0f)) {
+    set_error("invalid arguments to crava_hip_wavelet_resample");
+    return CRAVA_HIP_E_INVALID;
+  }
+  const float nv = crava::wavelet_resample(wavelet, nw, center, dz_wavelet, dz, nzp, out);
+  if (norm) *norm = nv;
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_wavelet_energy(int ntheta, int nzp, const float *wavelets, const float *A, const double *prior_var0,
+                             const float *corr_t, const float *error_variance, const float *emp_sn,
+                             const unsigned char *match, float *gain, float *model_var, float *signal_var,
+                             float *theo_sn) {
+  if (ntheta < 1 || ntheta > crava::kMaxTheta || nzp < 2 || !wavelets || !A || !prior_var0 || !corr_t ||
+      !error_variance || !emp_sn || !gain || !model_var || !signal_var || !theo_sn) {
+    set_error("invalid arguments to crava_hip_wavelet_energy");
+    return CRAVA_HIP_E_INVALID;
+  }
+  crava::wavelet_energy(ntheta, nzp, wavelets, A, prior_var0, corr_t, error_variance, emp_sn, match, gain, model_var,
+                        signal_var, theo_sn);
+  return CRAVA_HIP_OK;
+}
+
 int crava_hip_complex_noise_reference(int nxp, int nyp, int nzp, uint32_t *seed, float *host_out) {
   if (nxp < 1 || nyp < 1 || nzp < 1 || !seed || !host_out) {
     set_error("invalid arguments to crava_hip_complex_noise_reference");

@@ -1,6 +1,7 @@
 // Host-side per-k preparation (see tables.hpp).
 #include "tables.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -104,6 +105,181 @@ void perk_vectors(int ntheta, int nz, int nzp, double lz, float low_cut, float h
       e3[o] = 1.0f;
       e3[o + 1] = 0.0f;
     }
+  }
+}
+
+// Spectrum value of a real series at k with Wavelet::getCAmp(k, scale)'s
+// stretch (src/wavelet.cpp:327-385): linear interpolation of the half
+// spectrum at k / scale, zero beyond the last bin, divided by scale; float
+// arithmetic as the reference's.
+static c64 camp_scaled(const std::vector<c64> &s, int nzp, int k, float scale) {
+  const int cn = nzp / 2 + 1;
+  const bool mirror = k >= cn;
+  const int kk = mirror ? nzp - k : k;
+  c64 v{0.0f, 0.0f};
+  const float omega = float(kk) / scale;
+  if (!(omega >= (float)cn)) {
+    const int omL = (int)std::floor(omega);
+    int omU = omL + 1;
+    if (omU >= cn) omU -= 1;
+    const float dOmega = omega - float(omL);
+    v.re = s[omL].re * (1.0f - dOmega) + s[omU].re * dOmega;
+    v.im = s[omL].im * (1.0f - dOmega) + s[omU].im * dOmega;
+    if (mirror) v.im = -s[omL].im * (1.0f - dOmega) - s[omU].im * dOmega;
+  }
+  v.re /= scale;
+  v.im /= scale;
+  return v;
+}
+
+// Central o backward o forward difference of a series (the Wavelet1D
+// difference constructors, src/wavelet1D.cpp:420-460), in float.
+static void diff3(const float *w, int nzp, std::vector<float> &fw, std::vector<float> &bw, std::vector<float> &cw) {
+  fw.resize(nzp);
+  bw.resize(nzp);
+  cw.resize(nzp);
+  for (int i = 0; i < nzp; ++i) fw[i] = (i == nzp - 1) ? w[0] - w[i] : w[i + 1] - w[i];
+  for (int i = 0; i < nzp; ++i) bw[i] = (i == 0) ? fw[0] - fw[nzp - 1] : fw[i] - fw[i - 1];
+  for (int i = 0; i < nzp; ++i) {
+    const float up = bw[(i + 1) % nzp], dn = bw[(i + nzp - 1) % nzp];
+    cw[i] = (float)(0.5 * (up - dn));
+  }
+}
+
+void perk_vectors_nonconst(int ntheta, int nz, int nzp, double lz, float low_cut, float high_cut,
+                           float min_rel_thick, const float *wavelets, const float *wnorm, unsigned char *invert,
+                           float *kWD, float *e1, float *e2, float *e3) {
+  // forward-difference operator and its central o backward difference
+  // (diff1Operator / diff3Operator, src/crava.cpp:938-944)
+  std::vector<float> d1(nzp, 0.0f), fw, bw, cw;
+  d1[0] = -1.0f;
+  d1[nzp - 1] = 1.0f;
+  const std::vector<c64> dsp = half_spectrum(d1.data(), nzp);
+  std::vector<float> d2(nzp), d3(nzp);
+  for (int i = 0; i < nzp; ++i) d2[i] = (i == 0) ? d1[0] - d1[nzp - 1] : d1[i] - d1[i - 1];
+  for (int i = 0; i < nzp; ++i) d3[i] = (float)(0.5 * (d2[(i + 1) % nzp] - d2[(i + nzp - 1) % nzp]));
+  const std::vector<c64> d3sp = half_spectrum(d3.data(), nzp);
+  std::vector<std::vector<c64>> wsp(ntheta);
+  std::vector<float> e3norm(ntheta);
+  for (int l = 0; l < ntheta; ++l) {
+    const float *w = wavelets + (long long)l * nzp;
+    wsp[l] = half_spectrum(w, nzp);
+    diff3(w, nzp, fw, bw, cw);   // errorSmooth3, whose norm scales errMult2
+    double n2 = 0.0;
+    for (int i = 0; i < nzp; ++i) n2 += (double)(cw[i] * cw[i]);
+    e3norm[l] = (float)std::sqrt(n2);
+  }
+  for (int k = 0; k < nzp; ++k) {
+    const int kk = k < nzp - k ? k : nzp - k;
+    const float rf = (float)((nz * 1000.0f) / (lz * nzp) * kk);
+    invert[k] = (rf > low_cut * min_rel_thick && rf < high_cut) ? 1 : 0;
+    const c64 kD = camp(dsp, nzp, k), kD3 = camp(d3sp, nzp, k);
+    for (int l = 0; l < ntheta; ++l) {
+      const long long o = ((long long)k * ntheta + l) * 2;
+      // K = D A: the data were divided by the wavelet (src/crava.cpp:1060-1064)
+      kWD[o] = kD.re;
+      kWD[o + 1] = kD.im;
+      e1[o] = 1.0f / wnorm[l];                       // :1067-1071
+      e1[o + 1] = 0.0f;
+      const float m = 1.0f / e3norm[l];              // :1073-1080
+      e2[o] = kD3.re * m;
+      e2[o + 1] = kD3.im * m;
+      // fillInverseAbskWRobust, :1361-1384
+      const c64 v = camp(wsp[l], nzp, k), vf = camp_scaled(wsp[l], nzp, k, 0.999f);
+      const float mod = v.re * v.re + v.im * v.im, modf = vf.re * vf.re + vf.im * vf.im;
+      const float mx = std::max(mod, modf);
+      e3[o] = mx > 0.0f ? float(1.0 / std::sqrt(mx)) : wnorm[l] * nzp * nzp * 100.0f;
+      e3[o + 1] = 0.0f;
+    }
+  }
+}
+
+// Wavelet::getWaveletValue (src/wavelet.cpp:1078-1122): simple kriging of the
+// six samples around z with a 20 Hz Ricker correlation, in double.
+static float wavelet_value(float z, const float *w, int center, int nw, float dz) {
+  int ind[6];
+  double val[6], cov[6], C[6][6];
+  ind[2] = (int)std::floor(z / dz);
+  for (int k = 0; k < 6; ++k) ind[k] = ind[2] + k - 2;
+  for (int k = 0; k < 6; ++k) {
+    const int i = ind[k] + center;
+    val[k] = (i > -1 && i < nw) ? (double)w[i] : 0.0;
+  }
+  const double nu = 20.0, pi = 3.14159265358979323846;
+  for (int k = 0; k < 6; ++k) {
+    // lags in float (dz * index, then the difference), as the reference forms them
+    double dt = (double)(dz * (float)ind[k] - z) * 0.001;
+    double f = nu * nu * pi * pi * dt * dt;
+    cov[k] = (1 - 2 * f) * std::exp(-f);
+    for (int l = 0; l <= k; ++l) {
+      dt = (double)(dz * (float)ind[k] - dz * (float)ind[l]) * 0.001;
+      f = nu * nu * pi * pi * dt * dt;
+      C[l][k] = C[k][l] = (1 - 2 * f) * std::exp(-f);
+    }
+  }
+  // Cholesky solve C x = cov
+  double Lm[6][6] = {{0}}, y[6], x[6];
+  for (int i = 0; i < 6; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double s = C[i][j];
+      for (int k = 0; k < j; ++k) s -= Lm[i][k] * Lm[j][k];
+      Lm[i][j] = (i == j) ? std::sqrt(s) : s / Lm[j][j];
+    }
+  }
+  for (int i = 0; i < 6; ++i) {
+    double s = cov[i];
+    for (int k = 0; k < i; ++k) s -= Lm[i][k] * y[k];
+    y[i] = s / Lm[i][i];
+  }
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < 6; ++k) s -= Lm[k][i] * x[k];
+    x[i] = s / Lm[i][i];
+  }
+  double v = 0.0;
+  for (int k = 0; k < 6; ++k) v += val[k] * x[k];
+  return (float)v;
+}
+
+float wavelet_resample(const float *w, int nw, int center, float dzw, float dz, int nzp, float *out) {
+  for (int k = 0; k < nzp; ++k) {
+    const float z = k < nzp / 2 + 1 ? (float)(dz * k) : (float)(dz * (k - nzp));
+    out[k] = wavelet_value(z, w, center, nw, dzw);
+  }
+  double n2 = 0.0;
+  for (int k = 0; k < nzp; ++k) n2 += (double)(out[k] * out[k]);
+  return (float)std::sqrt(n2);
+}
+
+void wavelet_energy(int ntheta, int nzp, const float *wavelets, const float *A, const double *prior_var0,
+                    const float *corr_t, const float *error_variance, const float *emp_sn,
+                    const unsigned char *match, float *gain, float *model_var, float *signal_var,
+                    float *theo_sn) {
+  std::vector<float> wd(nzp);
+  for (int l = 0; l < ntheta; ++l) {
+    const float *w = wavelets + (long long)l * nzp;
+    // errorSmooth = forward difference of the wavelet (src/crava.cpp:411-419)
+    for (int i = 0; i < nzp; ++i) wd[i] = (i == nzp - 1) ? w[0] - w[i] : w[i + 1] - w[i];
+    // paramVar = A_l Sigma0 A_l^T, float accumulation (:423-429)
+    float pv = 0.0f;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) pv += (float)prior_var0[a * 3 + b] * A[l * 3 + a] * A[l * 3 + b];
+    // computeWDCorrMVar (:1763-1775): w_d^T circulant(corrT) w_d, float accumulation
+    float var = 0.0f;
+    for (int i = 0; i < nzp; ++i)
+      for (int j = 0; j < nzp; ++j) var += wd[i] * corr_t[i > j ? i - j : j - i] * wd[j];
+    float mv = var * pv;
+    float sv = error_variance[l] + mv;
+    float g = 1.0f;
+    if (match && match[l]) {                          // energy matching, :443-459
+      g = std::sqrt((error_variance[l] / mv) * (emp_sn[l] - 1.0f));
+      mv *= g * g;
+      sv = error_variance[l] + mv;
+    }
+    gain[l] = g;
+    model_var[l] = mv;
+    signal_var[l] = sv;
+    theo_sn[l] = sv / error_variance[l];
   }
 }
 

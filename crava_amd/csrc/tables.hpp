@@ -17,6 +17,31 @@ void perk_vectors(int ntheta, int nz, int nzp, double lz, float low_cut, float h
                   const float *wavelets, const float *wnorm, unsigned char *invert, float *kWD, float *e1,
                   float *e2, float *e3);
 
+// The same per-k vectors for a simbox that is not of constant thickness
+// (src/crava.cpp:1058-1083; the data were divided by the wavelets first):
+//   kWD[k][l] : kD(k)                            (K = D A)
+//   e1[k][l]  : 1 / |w_l|
+//   e2[k][l]  : kD3(k) / |e3_l|, kD3 the spectrum of central(backward(d))
+//   e3[k][l]  : fillInverseAbskWRobust (:1361-1384): 1/sqrt(max(|W_l(k)|^2,
+//               |W_l(k, 0.999)|^2)), or |w_l| nzp^2 100 where both vanish.
+void perk_vectors_nonconst(int ntheta, int nz, int nzp, double lz, float low_cut, float high_cut,
+                           float min_rel_thick, const float *wavelets, const float *wnorm, unsigned char *invert,
+                           float *kWD, float *e1, float *e2, float *e3);
+
+// Wavelet::resample (src/wavelet.cpp:416-463): nzp samples in FFT order at
+// spacing dz from a wavelet of nw samples at spacing dzw centred on sample
+// `center`, each by the six-sample Ricker-correlation kriging of
+// Wavelet::getWaveletValue; returns Wavelet::norm_ (sqrt of the sum of
+// squares).
+float wavelet_resample(const float *w, int nw, int center, float dzw, float dz, int nzp, float *out);
+
+// Crava::computeVariances' model variance and energy-matching gain
+// (src/crava.cpp:400-466, computeWDCorrMVar :1763-1775).
+void wavelet_energy(int ntheta, int nzp, const float *wavelets, const float *A, const double *prior_var0,
+                    const float *corr_t, const float *error_variance, const float *emp_sn,
+                    const unsigned char *match, float *gain, float *model_var, float *signal_var,
+                    float *theo_sn);
+
 // Half spectrum (nzp/2+1 {re, im} pairs) of a real series of nzp samples
 // with FFTW's forward sign and no normalisation (Wavelet::fft1DInPlace,
 // src/wavelet.cpp:246-261): double accumulation rounded to float, DC and

@@ -151,6 +151,45 @@ int crava_hip_perk_tables(int ntheta, int nz, int nzp, double lz, float low_cut,
                           float min_rel_thick, const float *wavelets, const float *wnorm,
                           unsigned char *invert, float *kWD, float *e1, float *e2, float *e3);
 
+/* The per-k vectors for a simbox that is not of constant thickness, host
+ * side (src/crava.cpp:1058-1083, fillInverseAbskWRobust :1361-1384; the
+ * data were divided by the wavelets first, crava_hip_divide_by_scale_wavelet):
+ *   kWD[k][l] = kD(k) (K = D A), e1[k][l] = 1/wnorm[l],
+ *   e2[k][l] = kD3(k) / |e3_l| (kD3 the spectrum of the central o backward
+ *   difference of the forward-difference operator, e3_l = central o backward
+ *   o forward difference of wavelet l, Wavelet1D src/wavelet1D.cpp:420-460),
+ *   e3[k][l] = 1 / sqrt(max(|W_l(k)|^2, |W_l(k, 0.999)|^2)) with
+ *   Wavelet::getCAmp(k, scale) (src/wavelet.cpp:327-385), or
+ *   wnorm[l] nzp^2 100 where both vanish.
+ * Arguments as crava_hip_perk_tables (min_rel_thick = Simbox::getMinRelThick). */
+int crava_hip_perk_tables_nonconst(int ntheta, int nz, int nzp, double lz, float low_cut, float high_cut,
+                                   float min_rel_thick, const float *wavelets, const float *wnorm,
+                                   unsigned char *invert, float *kWD, float *e1, float *e2, float *e3);
+
+/* Wavelet::resample (src/wavelet.cpp:416-463), host side: nzp samples in FFT
+ * order (z = dz k for k <= nzp/2, dz (k - nzp) after) from a wavelet of nw
+ * samples at spacing dz_wavelet with its centre at sample `center`, each by
+ * Wavelet::getWaveletValue's kriging of the six neighbouring samples with a
+ * 20 Hz Ricker correlation (:1078-1122; samples outside the wavelet are 0);
+ * *norm (may be NULL) receives Wavelet::norm_, the square root of the sum of
+ * squares.  Wavelet::scale (:477-486) is a multiplication of `out` by the
+ * gain of crava_hip_wavelet_energy. */
+int crava_hip_wavelet_resample(const float *wavelet, int nw, int center, float dz_wavelet, float dz, int nzp,
+                               float *out, float *norm);
+
+/* Crava::computeVariances' variances and energy-matching gain (src/crava.cpp:
+ * 400-466, computeWDCorrMVar :1763-1775), host side.  wavelets [ntheta][nzp]
+ * in FFT order; A [ntheta][3]; prior_var0 Sigma0 (row-major 3x3); corr_t
+ * [nzp] the filtered temporal correlation (getPriorCorrTFiltered);
+ * error_variance and emp_sn per angle (Crava::setupErrorCorrelation);
+ * match[l] != 0 where the angle's energies are matched (NULL: none).
+ * Outputs per angle: gain (1 when not matched; the caller scales the wavelet
+ * by it), model variance, signal variance and the theoretical S/N. */
+int crava_hip_wavelet_energy(int ntheta, int nzp, const float *wavelets, const float *A, const double *prior_var0,
+                             const float *corr_t, const float *error_variance, const float *emp_sn,
+                             const unsigned char *match, float *gain, float *model_var, float *signal_var,
+                             float *theo_sn);
+
 /* Reflection matrix of ModelAVODynamic::setupDefaultReflectionMatrix
  * (src/modelavodynamic.cpp:876-928), host side: row l holds the
  * coefficients of (ln Vp, ln Vs, ln rho) changes at incidence angle
