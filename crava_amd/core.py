@@ -134,6 +134,51 @@ def perk_tables(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wavelets:
     return {"invert": inv, "kWD": out[0], "e1": out[1], "e2": out[2], "e3": out[3]}
 
 
+def perk_tables_nonconst(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wavelets: np.ndarray,
+                         wnorm: np.ndarray):
+    """Host per-k vectors of a simbox that is not of constant thickness
+    (crava_hip_perk_tables_nonconst): K = D A on data already divided by the
+    wavelet."""
+    wav = np.ascontiguousarray(wavelets, dtype=np.float32)
+    wn = np.ascontiguousarray(wnorm, dtype=np.float32)
+    inv = np.zeros(nzp, dtype=np.uint8)
+    out = [np.zeros((nzp, ntheta, 2), dtype=np.float32) for _ in range(4)]
+    check(lib().crava_hip_perk_tables_nonconst(ntheta, nz, nzp, float(lz), float(low_cut), float(high_cut),
+                                               float(min_rel_thick), wav.ctypes.data, wn.ctypes.data,
+                                               inv.ctypes.data, *[o.ctypes.data for o in out]),
+          "crava_hip_perk_tables_nonconst")
+    return {"invert": inv, "kWD": out[0], "e1": out[1], "e2": out[2], "e3": out[3]}
+
+
+def wavelet_resample(wavelet: np.ndarray, center: int, dz_wavelet: float, dz: float, nzp: int):
+    """Wavelet::resample (crava_hip_wavelet_resample): (nzp samples in FFT
+    order, norm)."""
+    w = np.ascontiguousarray(wavelet, dtype=np.float32)
+    out = np.zeros(nzp, np.float32)
+    norm = C.c_float(0.0)
+    check(lib().crava_hip_wavelet_resample(w.ctypes.data, w.size, int(center), float(dz_wavelet), float(dz), nzp,
+                                           out.ctypes.data, C.byref(norm)), "crava_hip_wavelet_resample")
+    return out, norm.value
+
+
+def wavelet_energy(wavelets, A, prior_var0, corr_t, error_variance, emp_sn, match=None):
+    """Crava::computeVariances (crava_hip_wavelet_energy): dict of per-angle
+    gain, model_var, signal_var and theo_sn."""
+    wav = np.ascontiguousarray(wavelets, dtype=np.float32)
+    nt, nzp = wav.shape
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    pv = np.ascontiguousarray(prior_var0, dtype=np.float64)
+    ct = np.ascontiguousarray(corr_t, dtype=np.float32)
+    ev = np.ascontiguousarray(error_variance, dtype=np.float32)
+    sn = np.ascontiguousarray(emp_sn, dtype=np.float32)
+    m = None if match is None else np.ascontiguousarray(match, dtype=np.uint8)
+    out = [np.zeros(nt, np.float32) for _ in range(4)]
+    check(lib().crava_hip_wavelet_energy(nt, nzp, wav.ctypes.data, A.ctypes.data, pv.ctypes.data, ct.ctypes.data,
+                                         ev.ctypes.data, sn.ctypes.data, None if m is None else m.ctypes.data,
+                                         *[o.ctypes.data for o in out]), "crava_hip_wavelet_energy")
+    return dict(zip(("gain", "model_var", "signal_var", "theo_sn"), out))
+
+
 def problem_tables(prob):
     cfg = prob.cfg
     return perk_tables(prob.ntheta, cfg.nz, cfg.nzp, prob.lz, cfg.low_cut, cfg.high_cut, 1.0,

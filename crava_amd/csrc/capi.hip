@@ -316,7 +316,12 @@ int crava_hip_wavelet_resample(const float *wavelet, int nw, int center, float d
     set_error("invalid arguments to crava_hip_wavelet_resample");
     return CRAVA_HIP_E_INVALID;
   }
-  const float nv = crava::wavelet_resample(wavelet, nw, center, dz_wavelet, dz, nzp, out);
+  float nv = 0.0f;
+  if (!crava::wavelet_resample(wavelet, nw, center, dz_wavelet, dz, nzp, out, &nv)) {
+    set_error("crava_hip_wavelet_resample: kriging matrix not positive definite (sample spacing too fine for "
+              "the 20 Hz Ricker correlation)");
+    return CRAVA_HIP_E_INVALID;
+  }
   if (norm) *norm = nv;
   return CRAVA_HIP_OK;
 }

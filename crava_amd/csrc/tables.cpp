@@ -195,8 +195,9 @@ void perk_vectors_nonconst(int ntheta, int nz, int nzp, double lz, float low_cut
 }
 
 // Wavelet::getWaveletValue (src/wavelet.cpp:1078-1122): simple kriging of the
-// six samples around z with a 20 Hz Ricker correlation, in double.
-static float wavelet_value(float z, const float *w, int center, int nw, float dz) {
+// six samples around z with a 20 Hz Ricker correlation, in double.  False
+// where the kriging matrix is not positive definite.
+static bool wavelet_value(float z, const float *w, int center, int nw, float dz, float *value) {
   int ind[6];
   double val[6], cov[6], C[6][6];
   ind[2] = (int)std::floor(z / dz);
@@ -223,6 +224,7 @@ static float wavelet_value(float z, const float *w, int center, int nw, float dz
     for (int j = 0; j <= i; ++j) {
       double s = C[i][j];
       for (int k = 0; k < j; ++k) s -= Lm[i][k] * Lm[j][k];
+      if (i == j && !(s > 0.0)) return false;
       Lm[i][j] = (i == j) ? std::sqrt(s) : s / Lm[j][j];
     }
   }
@@ -238,17 +240,19 @@ static float wavelet_value(float z, const float *w, int center, int nw, float dz
   }
   double v = 0.0;
   for (int k = 0; k < 6; ++k) v += val[k] * x[k];
-  return (float)v;
+  *value = (float)v;
+  return true;
 }
 
-float wavelet_resample(const float *w, int nw, int center, float dzw, float dz, int nzp, float *out) {
+bool wavelet_resample(const float *w, int nw, int center, float dzw, float dz, int nzp, float *out, float *norm) {
   for (int k = 0; k < nzp; ++k) {
     const float z = k < nzp / 2 + 1 ? (float)(dz * k) : (float)(dz * (k - nzp));
-    out[k] = wavelet_value(z, w, center, nw, dzw);
+    if (!wavelet_value(z, w, center, nw, dzw, out + k)) return false;
   }
   double n2 = 0.0;
   for (int k = 0; k < nzp; ++k) n2 += (double)(out[k] * out[k]);
-  return (float)std::sqrt(n2);
+  *norm = (float)std::sqrt(n2);
+  return true;
 }
 
 void wavelet_energy(int ntheta, int nzp, const float *wavelets, const float *A, const double *prior_var0,

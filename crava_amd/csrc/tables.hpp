@@ -31,9 +31,10 @@ void perk_vectors_nonconst(int ntheta, int nz, int nzp, double lz, float low_cut
 // Wavelet::resample (src/wavelet.cpp:416-463): nzp samples in FFT order at
 // spacing dz from a wavelet of nw samples at spacing dzw centred on sample
 // `center`, each by the six-sample Ricker-correlation kriging of
-// Wavelet::getWaveletValue; returns Wavelet::norm_ (sqrt of the sum of
-// squares).
-float wavelet_resample(const float *w, int nw, int center, float dzw, float dz, int nzp, float *out);
+// Wavelet::getWaveletValue; *norm receives Wavelet::norm_ (sqrt of the sum
+// of squares).  Returns false where a kriging matrix is not positive definite
+// (NRLib::CholeskySolve throws there).
+bool wavelet_resample(const float *w, int nw, int center, float dzw, float dz, int nzp, float *out, float *norm);
 
 // Crava::computeVariances' model variance and energy-matching gain
 // (src/crava.cpp:400-466, computeWDCorrMVar :1763-1775).

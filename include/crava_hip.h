@@ -173,7 +173,8 @@ int crava_hip_perk_tables_nonconst(int ntheta, int nz, int nzp, double lz, float
  * 20 Hz Ricker correlation (:1078-1122; samples outside the wavelet are 0);
  * *norm (may be NULL) receives Wavelet::norm_, the square root of the sum of
  * squares.  Wavelet::scale (:477-486) is a multiplication of `out` by the
- * gain of crava_hip_wavelet_energy. */
+ * gain of crava_hip_wavelet_energy.  CRAVA_HIP_E_INVALID where a kriging
+ * matrix is not positive definite (NRLib::CholeskySolve throws there). */
 int crava_hip_wavelet_resample(const float *wavelet, int nw, int center, float dz_wavelet, float dz, int nzp,
                                float *out, float *norm);
 

@@ -67,6 +67,9 @@ def orc():
         L.orc_facies_geomodel.argtypes = [I, I, I, I, VP, PP]
         L.orc_facies_find_density.argtypes = [D, D, D, VP, I, I, VP, VP]
         L.orc_facies_find_density.restype = C.c_double
+        L.orc_perk_tables_nonconst.argtypes = [I, I, I, D, F, F, F, VP, VP, VP, VP, VP, VP, VP]
+        L.orc_wavelet_resample.argtypes = [VP, I, I, F, F, I, VP, VP]
+        L.orc_wavelet_energy.argtypes = [I, I, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]
         L.orc_merge_mean_4d.argtypes = [LL, PP, PP, PP]
         L.orc_merge_cov_4d.argtypes = [LL, PP, PP, PP, PP]
         L.orc_synthetic_seismic.argtypes = [I, I, I, I, I, I, I, VP, VP, F, PP, PP, PP, VP, VP, VP, PP]
@@ -205,6 +208,43 @@ def perk_tables(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wavelets,
                           high_cut, min_rel_thick, wav.ctypes.data, wn.ctypes.data,
                           inv.ctypes.data, *[o.ctypes.data for o in out])
     return {"invert": inv, "kWD": out[0], "e1": out[1], "e2": out[2], "e3": out[3]}
+
+
+def perk_tables_nonconst(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wavelets, wnorm):
+    """Per-k tables of a simbox that is not of constant thickness."""
+    wav = np.ascontiguousarray(wavelets, dtype=np.float32)
+    wn = np.ascontiguousarray(wnorm, dtype=np.float32)
+    inv = np.zeros(nzp, dtype=np.uint8)
+    out = [np.zeros((nzp, ntheta, 2), dtype=np.float32) for _ in range(4)]
+    orc().orc_perk_tables_nonconst(ntheta, nz, nzp, lz, low_cut, high_cut, min_rel_thick, wav.ctypes.data,
+                                   wn.ctypes.data, inv.ctypes.data, *[o.ctypes.data for o in out])
+    return {"invert": inv, "kWD": out[0], "e1": out[1], "e2": out[2], "e3": out[3]}
+
+
+def wavelet_resample(w, center, dzw, dz, nzp):
+    """Wavelet::resample: (nzp samples in FFT order, norm); None where a
+    kriging matrix is not positive definite."""
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    out = np.zeros(nzp, np.float32)
+    norm = np.zeros(1, np.float32)
+    rc = orc().orc_wavelet_resample(w.ctypes.data, w.size, center, dzw, dz, nzp, out.ctypes.data, norm.ctypes.data)
+    return None if rc != 0 else (out, float(norm[0]))
+
+
+def wavelet_energy(wavelets, A, prior_var0, corr_t, error_variance, emp_sn, match=None):
+    """Crava::computeVariances: dict of per-angle gain, model_var, signal_var, theo_sn."""
+    wav = np.ascontiguousarray(wavelets, dtype=np.float32)
+    nt, nzp = wav.shape
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    pv = np.ascontiguousarray(prior_var0, dtype=np.float64)
+    ct = np.ascontiguousarray(corr_t, dtype=np.float32)
+    ev = np.ascontiguousarray(error_variance, dtype=np.float32)
+    sn = np.ascontiguousarray(emp_sn, dtype=np.float32)
+    m = None if match is None else np.ascontiguousarray(match, dtype=np.uint8)
+    out = [np.zeros(nt, np.float32) for _ in range(4)]
+    orc().orc_wavelet_energy(nt, nzp, wav.ctypes.data, A.ctypes.data, pv.ctypes.data, ct.ctypes.data, ev.ctypes.data,
+                             sn.ctypes.data, None if m is None else m.ctypes.data, *[o.ctypes.data for o in out])
+    return dict(zip(("gain", "model_var", "signal_var", "theo_sn"), out))
 
 
 # -------------------------------------------------------------- posterior

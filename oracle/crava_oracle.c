@@ -501,6 +501,220 @@ void orc_perk_tables(int ntheta, int nz, int nzp, float dz, double lz,
 }
 
 /* ========================================================================
+ * Per-k tables for a simbox that is not of constant thickness
+ * (src/crava.cpp:1058-1083): the data were divided by the wavelet first, so
+ * K = D A, the first error multiplier is 1/|w|, the second the spectrum of
+ * the third-difference operator over |errorSmooth3|, and the third the
+ * robust inverse wavelet amplitude of fillInverseAbskWRobust (:1361-1384).
+ * ====================================================================== */
+
+/* Wavelet::getCAmp(k, scale), src/wavelet.cpp:327-385: the stretched
+ * wavelet's spectrum, FFT(w(s t)) = W(omega/s)/s, by linear interpolation of
+ * the half spectrum at k/scale (zero at or beyond cnzp). */
+static ocpx get_camp_scaled(const ocpx *spec, int nzp, int k, float scale)
+{
+  int cnzp = nzp / 2 + 1;
+  int upper = k >= cnzp;
+  int refk = upper ? nzp - k : k;
+  float omega = (float)refk / scale;
+  ocpx v = {0.0f, 0.0f};
+  if (!(omega >= (float)cnzp)) {
+    int omL = (int)floor(omega);
+    int omU = omL + 1;
+    if (omU >= cnzp) omU -= 1;
+    float dOm = omega - (float)omL;
+    v.re = spec[omL].re * (1.0f - dOm) + spec[omU].re * dOm;
+    if (!upper) v.im = spec[omL].im * (1.0f - dOm) + spec[omU].im * dOm;
+    else v.im = -spec[omL].im * (1.0f - dOm) - spec[omU].im * dOm;
+  }
+  v.re /= scale;
+  v.im /= scale;
+  return v;
+}
+
+/* forward, backward and central first differences on the periodic nzp
+ * series, src/wavelet1D.cpp:431-460 */
+static void diff_fwd(const float *x, int n, float *y)
+{
+  for (int i = 0; i < n; i++) y[i] = (i == n - 1) ? (x[0] - x[i]) : (x[i + 1] - x[i]);
+}
+static void diff_bwd(const float *x, int n, float *y)
+{
+  for (int i = 0; i < n; i++) y[i] = (i == 0) ? (x[0] - x[n - 1]) : (x[i] - x[i - 1]);
+}
+static void diff_cen(const float *x, int n, float *y)
+{
+  for (int i = 0; i < n; i++) {
+    float up = (i == n - 1) ? x[0] : x[i + 1];
+    float dn = (i == 0) ? x[n - 1] : x[i - 1];
+    y[i] = (float)(0.5 * (up - dn));
+  }
+}
+
+void orc_perk_tables_nonconst(int ntheta, int nz, int nzp, double lz,
+                              float low_cut, float high_cut, float min_rel_thick,
+                              const float *wavelets, const float *wnorm,
+                              unsigned char *invert, float *kWD, float *e1, float *e2, float *e3)
+{
+  int cnzp = nzp / 2 + 1;
+  float *a = (float *)calloc(nzp, sizeof(float));
+  float *b = (float *)malloc(sizeof(float) * nzp);
+  float *c = (float *)malloc(sizeof(float) * nzp);
+  ocpx *d1spec = (ocpx *)malloc(sizeof(ocpx) * cnzp);
+  ocpx *d3spec = (ocpx *)malloc(sizeof(ocpx) * cnzp);
+  ocpx *wspec = (ocpx *)malloc(sizeof(ocpx) * cnzp * ntheta);
+  float *s3norm = (float *)malloc(sizeof(float) * ntheta);
+  /* diff1Operator, diff2Operator, diff3Operator (src/crava.cpp:938-944) */
+  a[0] = -1.0f; a[nzp - 1] = 1.0f;
+  spectrum_f(a, nzp, d1spec);
+  diff_bwd(a, nzp, b);
+  diff_cen(b, nzp, c);
+  spectrum_f(c, nzp, d3spec);
+  for (int l = 0; l < ntheta; l++) {
+    const float *w = wavelets + (long long)l * nzp;
+    spectrum_f(w, nzp, wspec + l * cnzp);             /* seisWaveletForNorm */
+    diff_fwd(w, nzp, a);                              /* errorSmooth  (:958-961) */
+    diff_bwd(a, nzp, b);                              /* errorSmooth2 */
+    diff_cen(b, nzp, c);                              /* errorSmooth3 */
+    s3norm[l] = find_norm(c, nzp);
+  }
+  for (int k = 0; k < nzp; k++) {
+    int kk = k < nzp - k ? k : nzp - k;
+    float rf = (float)((nz * 1000.0f) / (lz * nzp) * kk);
+    invert[k] = (rf > low_cut * min_rel_thick && rf < high_cut) ? 1 : 0;
+    ocpx kD = get_camp(d1spec, nzp, k);
+    ocpx kD3 = get_camp(d3spec, nzp, k);
+    for (int l = 0; l < ntheta; l++) {
+      long long o = ((long long)k * ntheta + l) * 2;
+      kWD[o] = kD.re; kWD[o + 1] = kD.im;             /* lib_matrFillValueVecCpx(kD) */
+      e1[o] = 1.0f / wnorm[l]; e1[o + 1] = 0.0f;      /* ones / getNorm() */
+      float mult = 1.0f / s3norm[l];
+      e2[o] = kD3.re * mult; e2[o + 1] = kD3.im * mult;
+      /* fillInverseAbskWRobust */
+      ocpx v = get_camp(wspec + l * cnzp, nzp, k);
+      ocpx vf = get_camp_scaled(wspec + l * cnzp, nzp, k, 0.999f);
+      float mod = v.re * v.re + v.im * v.im;
+      float modf = vf.re * vf.re + vf.im * vf.im;
+      float mx = mod > modf ? mod : modf;
+      /* sqrt of a float argument: the float overload under C++'s <math.h> */
+      if (mx > 0.0f) e3[o] = (float)(1.0 / (double)sqrtf(mx));
+      else e3[o] = wnorm[l] * nzp * nzp * 100.0f;
+      e3[o + 1] = 0.0f;
+    }
+  }
+  free(a); free(b); free(c); free(d1spec); free(d3spec); free(wspec); free(s3norm);
+}
+
+/* ========================================================================
+ * Wavelet::resample (src/wavelet.cpp:416-463) with getWaveletValue
+ * (:1078-1122): simple kriging of the six samples around z under a 20 Hz
+ * Ricker correlation, solved by Cholesky (NRLib::CholeskySolve, LAPACK posv
+ * on the upper triangle).  Returns 0, or -1 where the kriging matrix is not
+ * positive definite (the reference throws).
+ * ====================================================================== */
+static double ricker20(double lag_ms)
+{
+  double dt = lag_ms * 0.001;
+  double f = 20.0 * 20.0 * M_PI * M_PI * dt * dt;
+  return (1.0 - 2.0 * f) * exp(-f);
+}
+
+static int krige6(float z, const float *w, int center, int nw, float dzw, float *value)
+{
+  int ind[6];
+  double val[6], rhs[6], U[6][6];
+  ind[2] = (int)floor(z / dzw);
+  for (int k = 0; k < 6; k++) {
+    ind[k] = ind[2] + k - 2;
+    int i = ind[k] + center;
+    val[k] = (i > -1 && i < nw) ? (double)w[i] : 0.0;   /* getArrayValueOrZero */
+  }
+  for (int k = 0; k < 6; k++) {
+    rhs[k] = ricker20((double)(dzw * ind[k] - z));
+    for (int l = 0; l <= k; l++) U[l][k] = ricker20((double)(dzw * ind[k] - dzw * ind[l]));
+  }
+  /* A = U^T U, U upper triangular (posv with uplo = U) */
+  for (int j = 0; j < 6; j++) {
+    double s = U[j][j];
+    for (int i = 0; i < j; i++) s -= U[i][j] * U[i][j];
+    if (!(s > 0.0)) return -1;
+    U[j][j] = sqrt(s);
+    for (int k = j + 1; k < 6; k++) {
+      double t = U[j][k];
+      for (int i = 0; i < j; i++) t -= U[i][j] * U[i][k];
+      U[j][k] = t / U[j][j];
+    }
+  }
+  double y[6], x[6];
+  for (int i = 0; i < 6; i++) {               /* U^T y = rhs */
+    double s = rhs[i];
+    for (int k = 0; k < i; k++) s -= U[k][i] * y[k];
+    y[i] = s / U[i][i];
+  }
+  for (int i = 5; i >= 0; i--) {              /* U x = y */
+    double s = y[i];
+    for (int k = i + 1; k < 6; k++) s -= U[i][k] * x[k];
+    x[i] = s / U[i][i];
+  }
+  double v = 0.0;
+  for (int k = 0; k < 6; k++) v += val[k] * x[k];
+  *value = (float)v;
+  return 0;
+}
+
+int orc_wavelet_resample(const float *w, int nw, int center, float dzw, float dz, int nzp,
+                         float *out, float *norm)
+{
+  for (int k = 0; k < nzp; k++) {
+    float z = (k < nzp / 2 + 1) ? (float)(dz * k) : (float)(dz * (k - nzp));
+    if (krige6(z, w, center, nw, dzw, out + k) != 0) return -1;
+  }
+  double n2 = 0.0;
+  for (int k = 0; k < nzp; k++) n2 += (double)(out[k] * out[k]);
+  *norm = (float)sqrt(n2);
+  return 0;
+}
+
+/* ========================================================================
+ * Crava::computeVariances (src/crava.cpp:400-466) with computeWDCorrMVar
+ * (:1763-1775): per angle, paramVar = A_l Sigma0 A_l^T, the variance of the
+ * differenced wavelet under the temporal correlation, the model and signal
+ * variances, the energy-matching gain and the theoretical S/N.  Float
+ * arithmetic in the reference's order.
+ * ====================================================================== */
+void orc_wavelet_energy(int ntheta, int nzp, const float *wavelets, const float *A,
+                        const double *prior_var0, const float *corr_t, const float *error_variance,
+                        const float *emp_sn, const unsigned char *match,
+                        float *gain, float *model_var, float *signal_var, float *theo_sn)
+{
+  float *wd = (float *)malloc(sizeof(float) * nzp);
+  for (int l = 0; l < ntheta; l++) {
+    diff_fwd(wavelets + (long long)l * nzp, nzp, wd);
+    float pv = 0.0f;
+    for (int i = 0; i < 3; i++)
+      for (int m = 0; m < 3; m++)
+        pv += (float)prior_var0[i * 3 + m] * A[l * 3 + i] * A[l * 3 + m];
+    float wv = 0.0f;
+    for (int i = 0; i < nzp; i++)
+      for (int j = 0; j < nzp; j++) {
+        int ci = i > j ? i - j : j - i;
+        wv += wd[i] * corr_t[ci] * wd[j];
+      }
+    float mv = wv * pv;
+    float sv = error_variance[l] + mv;
+    float g = 1.0f;
+    if (match && match[l]) {
+      g = sqrtf((error_variance[l] / mv) * (emp_sn[l] - 1.0f));
+      mv *= g * g;
+      sv = error_variance[l] + mv;
+    }
+    gain[l] = g; model_var[l] = mv; signal_var[l] = sv;
+    theo_sn[l] = sv / error_variance[l];
+  }
+  free(wd);
+}
+
+/* ========================================================================
  * Per-point posterior loop, src/crava.cpp:1042-1160 (constant thickness),
  * with getNextParameterCovariance (src/seismicparametersholder.cpp:356-427)
  * and getNextErrorVariance (src/crava.cpp:1298-1334).
