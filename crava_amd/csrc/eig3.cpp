@@ -5,9 +5,9 @@
 // DGEEV, and Crava::computeG (src/crava.cpp:2182) uses their signs.
 //
 // The steps are LAPACK's published algorithm for this case:
-//   DGEBAL('B')  no permutation and no scaling when every off-diagonal entry
-//                is nonzero and rows and columns have equal norms (a
-//                symmetric matrix); other matrices are not handled here;
+//   DGEBAL('B')  no permutation when every off-diagonal entry is nonzero;
+//                its radix-2 scaling sweep restated, and matrices it would
+//                scale (as it would not a symmetric one) not handled here;
 //   DGEHD2       one Householder reflector (DLARFG) on column 1;
 //   DORG2R       the 2x2 block of Q = diag(1, I - tau v v^T);
 //   DLAHQR       Francis double-shift QR sweeps on the Hessenberg matrix with
@@ -17,6 +17,8 @@
 //                back transformation, each vector scaled by its largest
 //                entry, then DGEEV's scaling to unit 2-norm.
 // Only real eigenvalues are handled (a symmetric matrix has nothing else).
+#include <algorithm>
+#include <cfloat>
 #include <cmath>
 
 #include "eig3.hpp"
@@ -136,15 +138,45 @@ bool dgeev3(const double m[3][3], double wr[3], double vr[3][3]) {
       if (r != c && m[r][c] == 0.0) return false;  // DGEBAL could permute
       if (!std::isfinite(m[r][c])) return false;
     }
-  // DGEBAL's scaling leaves the matrix alone when the row and column norms
-  // of each index agree within its factor-of-two test
-  for (int i = 1; i <= 3; ++i) {
-    double c = 0.0, r = 0.0;
-    for (int j = 1; j <= 3; ++j) {
-      c += H[j][i] * H[j][i];
-      r += H[i][j] * H[i][j];
+  // DGEBAL's scaling sweep (ILO = 1, IHI = 3: no zero off-diagonal entry, so
+  // no permutation).  The helper proceeds only when the first sweep scales
+  // no index: then DGEBAL's NOCONV stays false and the matrix is unbalanced.
+  {
+    const double radix = 2.0, factor = 0.95;
+    const double sfmin1 = DBL_MIN / (DBL_EPSILON * 0.5 * radix), sfmin2 = sfmin1 * radix, sfmax2 = 1.0 / sfmin2;
+    for (int i = 1; i <= 3; ++i) {
+      double c = 0.0, r = 0.0, ca = 0.0, ra = 0.0;
+      for (int j = 1; j <= 3; ++j) {       // DNRM2 of column i and row i, IDAMAX of each
+        c += H[j][i] * H[j][i];
+        r += H[i][j] * H[i][j];
+        ca = std::max(ca, std::fabs(H[j][i]));
+        ra = std::max(ra, std::fabs(H[i][j]));
+      }
+      c = std::sqrt(c);
+      r = std::sqrt(r);
+      if (c == 0.0 || r == 0.0) continue;
+      double g = r / radix, f = 1.0;
+      const double s = c + r;
+      while (c < g && std::max(f, std::max(c, ca)) < sfmax2 && std::min(r, std::min(g, ra)) > sfmin2) {
+        f *= radix;
+        c *= radix;
+        ca *= radix;
+        r /= radix;
+        g /= radix;
+        ra /= radix;
+      }
+      g = c / radix;
+      while (g >= r && std::max(r, ra) < sfmax2 && std::min(std::min(f, c), std::min(g, ca)) > sfmin2) {
+        f /= radix;
+        c /= radix;
+        g /= radix;
+        ca /= radix;
+        r *= radix;
+        ra *= radix;
+      }
+      if (c + r >= factor * s) continue;   // not worth scaling
+      if (f != 1.0) return false;          // DGEBAL would scale index i
     }
-    if (c < 0.25 * r || r < 0.25 * c) return false;
   }
   // DGEHD2, i = 1 (i = 2 gives tau = 0)
   double alpha = H[2][1], x = H[3][1];

@@ -324,7 +324,9 @@ class Crava {
   // :2340-2470 correctAlphaBetaRho: post[3] (time-domain posterior means)
   // corrected per trace towards mean2[3] (the prior means) with
   // Settings::localNoiseScale; keeps the realisation factors (sigmamdnew_)
-  // that both simulate overloads then apply (:1547-1566).  G: the reference's
+  // that both simulate overloads then apply (:1547-1566).  For the
+  // reference's flow post[c] is sph.mu(c), the grids the realisations add
+  // (postAlpha_ etc., corrected in place, :1568).  G: the reference's
   // computeG [ntheta][3] (its LAPACK eigenvector signs), or nullptr for
   // crava_hip_local_noise_g's.
   void correctAlphaBetaRho(const double priorVar0[3][3], const double postVar0[3][3], FFTGrid *post[3],

@@ -24,7 +24,13 @@
  *   - facies: an independent numpy trilinear lookup (clip-then-interpolate
  *     form), hand-computed node / cell-centre / clamp / negative-entry cases
  *     and the probability assembly (tests/test_oracle_facies.py);
- *   - NLL: central finite differences and an independent numpy evaluation.
+ *   - NLL: central finite differences and an independent numpy evaluation;
+ *   - non-constant-thickness per-k tables, Wavelet::resample and the
+ *     energy-matching variances: closed-form difference-operator spectra,
+ *     delta / zero wavelets, numpy's rfft with the 0.999 stretch, kriging
+ *     exact at nodes and exact on a node-centred 20 Hz Ricker, numpy.linalg
+ *     solves, the Toeplitz quadratic form and theo S/N = empirical S/N after
+ *     matching (tests/test_oracle_wavelet.py).
  */
 #include <math.h>
 #include <stdint.h>

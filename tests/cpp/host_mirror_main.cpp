@@ -141,37 +141,38 @@ int main(int argc, char **argv) {
     for (int c = 0; c < 3; ++c) write_bin(dir + "/out_simref" + std::to_string(c) + ".bin", drain_real(real[c]));
     std::ofstream(dir + "/simref_seed.txt") << rg << "\n";
 
-    // local-noise correction (host computeG) of copies of the posterior
-    // means towards the prior means, postVar0 = D sigma0 D with
-    // D = diag(0.6, 0.7, 0.8) (distinct eigenvalues of
-    // sigma0^-1/2 (sigma0 - postVar0) sigma0^-1/2, so that its eigenvectors
-    // are defined up to sign); then the same reference-stream realisation,
-    // now with the local-noise factors
+    // local-noise correction (host computeG) of the posterior means in
+    // place, towards the prior means, as the reference corrects postAlpha_
+    // etc.: postVar0 = D sigma0 D with D = diag(0.6, 0.7, 0.8) (distinct
+    // eigenvalues of sigma0^-1/2 (sigma0 - postVar0) sigma0^-1/2, so that its
+    // eigenvectors are defined up to sign); then the same reference-stream
+    // realisation, now with the local-noise factors and the corrected means
+    // added (src/crava.cpp:1547-1568)
     {
       const double dg[3] = {0.6, 0.7, 0.8};
       double smd[3][3];
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) smd[a][b] = (dg[a] * sigma0[a][b]) * dg[b];
       FFTGrid *post[3], *mean2[3];
+      std::vector<std::vector<float>> uncorrected(3);
       for (int c = 0; c < 3; ++c) {
-        post[c] = new FFTGrid(&ctx, s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
+        uncorrected[c] = drain_real(sph.mu(c));
+        post[c] = sph.mu(c);
         mean2[c] = new FFTGrid(&ctx, s.nx, s.ny, s.nz, s.nxp, s.nyp, s.nzp);
-        post[c]->setType(FFTGrid::PARAMETER);
         mean2[c]->setType(FFTGrid::PARAMETER);
-        post[c]->createRealGrid();
         mean2[c]->createRealGrid();
-        fill_real(post[c], drain_real(sph.mu(c)));
         fill_real(mean2[c], read_bin(dir + "/mu" + std::to_string(c) + ".bin", rsize));
       }
       crava.correctAlphaBetaRho(sph.getPriorVar0(), smd, post, mean2);
       for (int c = 0; c < 3; ++c) {
-        write_bin(dir + "/out_lnc" + std::to_string(c) + ".bin", drain_real(post[c]));
-        delete post[c];
+        write_bin(dir + "/out_lnc" + std::to_string(c) + ".bin", drain_real(sph.mu(c)));
         delete mean2[c];
       }
       uint32_t rg2 = 23665;
       crava.simulate(sph, rg2, real);
       for (int c = 0; c < 3; ++c) write_bin(dir + "/out_simlnc" + std::to_string(c) + ".bin", drain_real(real[c]));
+      // the steps below work on the uncorrected posterior means
+      for (int c = 0; c < 3; ++c) fill_real(sph.mu(c), uncorrected[c]);
     }
     for (int c = 0; c < 3; ++c) delete real[c];
 

@@ -128,9 +128,12 @@ def test_per_point_loops_on_permutations():
 def test_host_g_helper_matches_lapack_dgeev_on_many_problems():
     """The helper restates DGEEV's path for the 3x3 eigenvectors.  Over 600
     generated problems (ntheta 1..8) its G equals the oracle's (netlib dgeev
-    through scipy) in 576; in the others the QR iteration's last reflections
-    turn on last-bit rounding and some eigenvectors come out with the other
-    sign, which leaves G Sm G^T unchanged."""
+    through scipy) in 576.  The other 24 all have ntheta = 1, where G is one
+    row whose overall sign the QR iteration settles on last-bit rounding: the
+    helper's G is exactly -G there, and the correction, which depends on G
+    only through G^T H^+ G with H = G Sm G^T, is the same.  For ntheta >= 2 a
+    sign flip of one eigenvector would change the correction, so there G
+    must match."""
     from crava_amd import core
     exact = 0
     n = 600
@@ -142,9 +145,8 @@ def test_host_g_helper_matches_lapack_dgeev_on_many_problems():
         if np.allclose(G, ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max()):
             exact += 1
         else:
-            assert np.allclose(np.abs(G), np.abs(ref), rtol=1e-8, atol=1e-10 * np.abs(ref).max()), seed
-            assert np.allclose(G @ p["Sm"] @ G.T, ref @ p["Sm"] @ ref.T, rtol=1e-8,
-                               atol=1e-12 * np.abs(ref).max() ** 2), seed
+            assert nt == 1, seed
+            assert np.allclose(G, -ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max()), seed
     assert exact >= 570, exact
 
 
