@@ -1,4 +1,5 @@
 // Crava (posterior core) of the C++ host mirror (see crava_host.hpp).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -48,15 +49,51 @@ void Crava::setupErrorCorrelation(const std::function<double(float)> &angularCor
     }
 }
 
+void Crava::computeVariances(const std::vector<float> &corrT, const SeismicParametersHolder &sph,
+                             const std::function<double(float)> &angularCorr) {
+  const int nt = s_.ntheta, nzp = s_.nzp;
+  if ((int)corrT.size() < nzp) throw Error(CRAVA_HIP_E_INVALID, "computeVariances needs nzp values of corrT");
+  if (!s_.matchEnergies.empty() && (int)s_.matchEnergies.size() != nt)
+    throw Error(CRAVA_HIP_E_INVALID, "matchEnergies needs one flag per angle");
+  computeDataVariance();
+  setupErrorCorrelation(angularCorr);
+  std::vector<float> gain(nt);
+  modelVariance_.assign(nt, 0.0f);
+  signalVariance_.assign(nt, 0.0f);
+  theoSNRatio_.assign(nt, 0.0f);
+  check(crava_hip_wavelet_energy(nt, nzp, s_.wavelets.data(), s_.A.data(), &sph.getPriorVar0()[0][0], corrT.data(),
+                                 errorVariance_.data(), s_.snr.data(),
+                                 s_.matchEnergies.empty() ? nullptr : s_.matchEnergies.data(), gain.data(),
+                                 modelVariance_.data(), signalVariance_.data(), theoSNRatio_.data()),
+        "computeVariances");
+  // Wavelet::scale (src/wavelet.cpp:477-486): the samples times the gain;
+  // norm_ keeps its value, so the error multipliers that divide by the norm
+  // (fillkWNorm, :1067-1071) see the scaled wavelet over the unscaled norm
+  for (int l = 0; l < nt; ++l) {
+    if (gain[l] == 1.0f) continue;
+    for (int k = 0; k < nzp; ++k) s_.wavelets[(size_t)l * nzp + k] *= gain[l];
+  }
+}
+
 long long Crava::computePostMeanResidAndFFTCov(SeismicParametersHolder &sph) {
   const int nt = s_.ntheta, nzp = s_.nzp;
   crava_hip_ctx *h = ctx_->handle();
-  // per-k vectors (src/crava.cpp:938-1057)
+  // per-k vectors (src/crava.cpp:938-1083)
   std::vector<unsigned char> invert(nzp);
   std::vector<float> kWD((size_t)nzp * nt * 2), e1(kWD.size()), e2(kWD.size()), e3(kWD.size());
-  check(crava_hip_perk_tables(nt, s_.nz, nzp, (double)s_.nz * s_.dz, s_.lowCut, s_.highCut, 1.0f, s_.wavelets.data(),
-                              s_.waveletNorm.data(), invert.data(), kWD.data(), e1.data(), e2.data(), e3.data()),
-        "perk tables");
+  const double lz = (double)s_.nz * s_.dz;
+  if (s_.relThick.empty()) {
+    check(crava_hip_perk_tables(nt, s_.nz, nzp, lz, s_.lowCut, s_.highCut, 1.0f, s_.wavelets.data(),
+                                s_.waveletNorm.data(), invert.data(), kWD.data(), e1.data(), e2.data(), e3.data()),
+          "perk tables");
+  } else {
+    double minRel = s_.minRelThick;
+    if (!(minRel > 0.0)) minRel = *std::min_element(s_.relThick.begin(), s_.relThick.end());
+    check(crava_hip_perk_tables_nonconst(nt, s_.nz, nzp, lz, s_.lowCut, s_.highCut, (float)minRel,
+                                         s_.wavelets.data(), s_.waveletNorm.data(), invert.data(), kWD.data(),
+                                         e1.data(), e2.data(), e3.data()),
+          "perk tables (non-constant thickness)");
+  }
   crava_hip_post_params p;
   std::memset(&p, 0, sizeof(p));
   p.nxp = s_.nxp;

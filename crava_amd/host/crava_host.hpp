@@ -213,9 +213,10 @@ class SeismicParametersHolder {
                               float corrGradI, float corrGradJ, int lowIntCut, int nzp);
   // :431-499 (host; the 1-D transform in double).
   std::vector<float> computeCircCorrT(const std::vector<float> &priorCorrT, int minIntFq, int nzp) const;
-  // :504-530 with extractParamCorrFromCovAlpha (:748-768): the cyclic
-  // temporal correlation read from the real covAlpha grid at (0, 0, k),
-  // zero where min(k, nzp - k) >= nz.
+  // :748-768: the cyclic temporal correlation, the real covAlpha grid at
+  // (0, 0, k) over its value at the origin, k < nzp.
+  std::vector<float> extractParamCorrFromCovAlpha(int nzp) const;
+  // :504-530: extractParamCorrFromCovAlpha, zero where min(k, nzp - k) >= nz.
   std::vector<float> getPriorCorrTFiltered(int nz, int nzp) const;
   // :771-806: the prior variances from the covariance grids' values at
   // the origin (time domain), and the posterior covariance's z profile at
@@ -295,9 +296,13 @@ class Crava {
     // wavelet time shifts (ms), gains and stretches relThick*localStretch
     std::vector<std::vector<float>> localShift, localGain, stretch;
     // non-constant thickness: Simbox::getRelThick of every padded trace
-    // [nyp*nxp] (empty: constant thickness) and the energy threshold of
-    // interpolateSeismic (Crava::energyTreshold_)
+    // [nyp*nxp] (empty: constant thickness), Simbox::getMinRelThick (0: the
+    // smallest relThick) and the energy threshold of interpolateSeismic
+    // (Crava::energyTreshold_)
     std::vector<double> relThick;
+    double minRelThick = 0.0;
+    // energy matching per angle (ModelAVODynamic::getMatchEnergies; empty: none)
+    std::vector<unsigned char> matchEnergies;
     // local noise: per angle getLocalNoiseScale(angle) as [ny*nx] (i fastest)
     std::vector<std::vector<double>> localNoiseScale;
     float energyThreshold = 0.0f;
@@ -310,8 +315,19 @@ class Crava {
 
   void computeDataVariance();                                               // :322-363
   void setupErrorCorrelation(const std::function<double(float)> &angularCorr);  // :366-401
+  // :404-466: computeDataVariance and setupErrorCorrelation, then per angle
+  // the model variance (the differenced wavelet under corrT, times
+  // A Sigma0 A^T), the signal variance and the theoretical S/N; where
+  // Settings::matchEnergies is set, the wavelet is scaled by the
+  // energy-matching gain (Wavelet::scale, which leaves the norm as it was).
+  // corrT: the holder's extractParamCorrFromCovAlpha(nzp).
+  void computeVariances(const std::vector<float> &corrT, const SeismicParametersHolder &sph,
+                        const std::function<double(float)> &angularCorr);
   // :887-1204: forward transforms, per-point posterior, inverse transforms
-  // of the mean and the residuals (the covariance stays transformed).
+  // of the mean and the residuals (the covariance stays transformed).  With
+  // Settings::relThick set (a simbox not of constant thickness) the data
+  // are expected divided by the scale wavelet (divideDataByScaleWavelet)
+  // and the per-k vectors are those of K = D A (:1058-1083).
   // Returns the number of failed factorisations.
   long long computePostMeanResidAndFFTCov(SeismicParametersHolder &sph);
   // :1416-1599 one realisation: noise (seed, stream) -> real grids real[3].
@@ -345,6 +361,11 @@ class Crava {
 
   const std::vector<float> &getDataVariance() const { return dataVariance_; }
   const std::vector<float> &getErrorVariance() const { return errorVariance_; }
+  const std::vector<float> &getModelVariance() const { return modelVariance_; }
+  const std::vector<float> &getSignalVariance() const { return signalVariance_; }
+  const std::vector<float> &getTheoSNRatio() const { return theoSNRatio_; }
+  const std::vector<float> &getWavelets() const { return s_.wavelets; }
+  const std::vector<float> &getWaveletNorm() const { return s_.waveletNorm; }
   const std::vector<double> &getErrThetaCov() const { return errThetaCov_; }
   long long getFP64Points() const { return nFP64_; }
 
@@ -352,7 +373,7 @@ class Crava {
   Context *ctx_;
   Settings s_;
   std::vector<FFTGrid *> seisData_;
-  std::vector<float> dataVariance_, errorVariance_;
+  std::vector<float> dataVariance_, errorVariance_, modelVariance_, signalVariance_, theoSNRatio_;
   std::vector<double> errThetaCov_;   // [ntheta][ntheta]
   long long nFP64_ = 0;
   void *traceCov_ = nullptr;          // device [ny][nx][9] realisation factors

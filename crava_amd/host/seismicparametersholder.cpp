@@ -126,13 +126,18 @@ std::vector<float> SeismicParametersHolder::createPostCov00(FFTGrid *postCov) co
   return postCov->readTrace(0, 0, 0, postCov->getNz());
 }
 
-std::vector<float> SeismicParametersHolder::getPriorCorrTFiltered(int nz, int nzp) const {
+std::vector<float> SeismicParametersHolder::extractParamCorrFromCovAlpha(int nzp) const {
   FFTGrid *c = cov_[0];
   if (!c || c->getIsTransformed())
-    throw Error(CRAVA_HIP_E_STATE, "getPriorCorrTFiltered needs covAlpha in the time domain");
+    throw Error(CRAVA_HIP_E_STATE, "extractParamCorrFromCovAlpha needs covAlpha in the time domain");
   std::vector<float> circ = c->readTrace(0, 0, 0, nzp);
   const float constant = circ[0];
   for (int k = 0; k < nzp; ++k) circ[k] = circ[k] / constant;
+  return circ;
+}
+
+std::vector<float> SeismicParametersHolder::getPriorCorrTFiltered(int nz, int nzp) const {
+  const std::vector<float> circ = extractParamCorrFromCovAlpha(nzp);
   std::vector<float> filtered(nzp);
   for (int i = 0; i < nzp; ++i) {
     const int refk = i < nzp / 2 + 1 ? i : nzp - i;

@@ -231,9 +231,23 @@ int main(int argc, char **argv) {
         }
         syn[l]->endAccess();
       }
+      // energy matching on angle 0 (Crava::computeVariances, in the
+      // constructor before the division)
+      s2.matchEnergies.assign(nt, 0);
+      s2.matchEnergies[0] = 1;
       crava_hip::Crava cnt(&ctx, s2, syn);
-      cnt.computeDataVariance();
-      cnt.setupErrorCorrelation([angRange](float dTheta) { return std::exp(-3.0 * std::fabs(dTheta) / angRange); });
+      const std::vector<float> corrTv = sph2.extractParamCorrFromCovAlpha(s.nzp);
+      cnt.computeVariances(corrTv, sph2,
+                           [angRange](float dTheta) { return std::exp(-3.0 * std::fabs(dTheta) / angRange); });
+      write_bin(dir + "/nc_corrt.bin", corrTv);
+      write_bin(dir + "/nc_errvar.bin", cnt.getErrorVariance());
+      write_bin(dir + "/nc_wavelets.bin", cnt.getWavelets());
+      {
+        std::vector<float> e;
+        for (const auto *v : {&cnt.getModelVariance(), &cnt.getSignalVariance(), &cnt.getTheoSNRatio()})
+          e.insert(e.end(), v->begin(), v->end());
+        write_bin(dir + "/nc_energy.bin", e);
+      }
       cnt.divideDataByScaleWavelet(sph2);
       std::vector<float> ev(nt);
       for (int l = 0; l < nt; ++l) ev[l] = (float)cnt.getErrThetaCov()[(size_t)l * nt + l];
@@ -241,6 +255,14 @@ int main(int argc, char **argv) {
       write_bin(dir + "/swd_gain.bin", gain);
       write_bin(dir + "/swd_corrt.bin", sph2.getPriorCorrTFiltered(s.nz, s.nzp));
       for (int l = 0; l < nt; ++l) write_bin(dir + "/out_swd" + std::to_string(l) + ".bin", drain_real(syn[l]));
+      // the inversion of the divided data with the non-constant-thickness
+      // per-k vectors (K = D A), prior means from the inputs
+      for (int c = 0; c < 3; ++c) fill_real(sph2.mu(c), read_bin(dir + "/mu" + std::to_string(c) + ".bin", rsize));
+      const long long ncfail = cnt.computePostMeanResidAndFFTCov(sph2);
+      for (int c = 0; c < 3; ++c) write_bin(dir + "/nc_mu" + std::to_string(c) + ".bin", drain_real(sph2.mu(c)));
+      for (int l = 0; l < nt; ++l) write_bin(dir + "/nc_res" + std::to_string(l) + ".bin", drain_real(syn[l]));
+      for (int q = 0; q < 6; ++q) write_bin(dir + "/nc_cov" + std::to_string(q) + ".bin", drain_complex(sph2.cov(q)));
+      std::ofstream(dir + "/nc_fail.txt") << ncfail << "\n";
     }
     for (int l = 0; l < nt; ++l) delete syn[l];
 

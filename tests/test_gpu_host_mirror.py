@@ -207,9 +207,30 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
         pad = np.ones(shape[1:], bool)
         pad[:c.ny, :c.nx] = False
         assert np.array_equal(got[:, pad], syn_ref[l][:, pad]), l
+    # computeVariances on that synthetic seismic with energy matching on
+    # angle 0: the unfiltered temporal correlation of covAlpha, the
+    # variances and the gain against the oracle, the scaled wavelet
+    corr_u = np.fromfile(os.path.join(d, "nc_corrt.bin"), dtype=np.float32)
+    circ_full = prob.circ_corr_t
+    refk = np.minimum(np.arange(nzp), nzp - np.arange(nzp))
+    assert np.abs(corr_u - circ_full[np.minimum(refk, len(circ_full) - 1)] / circ_full[0]).max() < 1e-6
+    ev_nc = np.fromfile(os.path.join(d, "nc_errvar.bin"), dtype=np.float32)
+    match = np.zeros(nt, np.uint8)
+    match[0] = 1
+    en = oracle.wavelet_energy(prob.wavelets, prob.A, prob.sigma0, corr_u, ev_nc, np.array(c.snr, np.float32), match)
+    got_e = np.fromfile(os.path.join(d, "nc_energy.bin"), dtype=np.float32).reshape(3, nt)
+    assert np.array_equal(got_e[0], en["model_var"])
+    assert np.array_equal(got_e[1], en["signal_var"])
+    assert np.array_equal(got_e[2], en["theo_sn"])
+    assert abs(en["theo_sn"][0] / np.float32(c.snr[0]) - 1) < 1e-5 and en["gain"][0] != 1.0
+    wav_nc = np.fromfile(os.path.join(d, "nc_wavelets.bin"), dtype=np.float32).reshape(nt, nzp)
+    want_w = prob.wavelets.copy()
+    want_w[0] = want_w[0] * en["gain"][0]
+    assert np.array_equal(wav_nc, want_w)
     # divideDataByScaleWavelet + interpolateSeismic of that synthetic seismic
     # (traces (2, 1) and (3, 1) emptied, a gain table on angle 0, the shift
-    # table on the last angle, relThick 0.7 + 0.3 ((5i + 3j) mod 13) / 12)
+    # table on the last angle, relThick 0.7 + 0.3 ((5i + 3j) mod 13) / 12),
+    # with the energy-matched wavelets
     ev = np.fromfile(os.path.join(d, "swd_errvar.bin"), dtype=np.float32)
     gain = np.fromfile(os.path.join(d, "swd_gain.bin"), dtype=np.float32).reshape(c.ny, c.nx)
     corr_f = np.fromfile(os.path.join(d, "swd_corrt.bin"), dtype=np.float32)
@@ -223,7 +244,7 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
         g = _read(os.path.join(d, f"out_syn{l}.bin"), shape).copy()
         g[:, 1, 2:4] = 0.0
         syn_in.append(g)
-    div = oracle.divide_by_scale_wavelet(c.nx, c.ny, c.nz, nxp, nyp, nzp, syn_in, prob.wavelets, prob.wnorm, prob.A,
+    div = oracle.divide_by_scale_wavelet(c.nx, c.ny, c.nz, nxp, nyp, nzp, syn_in, wav_nc, prob.wnorm, prob.A,
                                          prob.sigma0, corr_f, ev, rel, c.dt, c.nz * c.dt, c.low_cut, c.high_cut,
                                          c.wnc, shift=shift, gain=[gain] + [None] * (nt - 1))
     for l in range(nt):
@@ -232,6 +253,48 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
         got = _read(os.path.join(d, f"out_swd{l}.bin"), shape)[:, :, :nxp].astype(np.float64)
         ref = want[:, :, :nxp].astype(np.float64)
         assert np.abs(got - ref).max() <= 5e-6 * np.abs(ref).max(), l
+    # the inversion of the divided data on the non-constant-thickness per-k
+    # vectors, against the oracle's transforms and loop on the same divided
+    # data.  The residual of this path amplifies transform roundoff (E M^-1
+    # with e3 = 1/|W| far from flat): its bound is twice the spread between
+    # the reference's own float chain (FFTW 2.1.2, lib_matr) and the
+    # oracle's where oracle/_ref is built, else 5e-4
+    etc_nc = (np.sqrt(ev_nc)[:, None] * np.sqrt(ev_nc)[None, :] * ang).astype(np.float32).astype(np.float64)
+    ntabs = oracle.perk_tables_nonconst(nt, c.nz, nzp, c.nz * c.dt, c.low_cut, c.high_cut, float(rel.min()), wav_nc,
+                                        prob.wnorm)
+
+    def nc_chain(which):
+        m = [g.numpy().copy() for g in prob.mu]
+        dd_ = [_read(os.path.join(d, f"out_swd{l}.bin"), shape).copy() for l in range(nt)]
+        cc = [oracle.fill_param_corr(nxp, nyp, nzp, prob.corr_xy, prob.circ_corr_t, 0.0, 0.0,
+                                     np.float32(prob.sigma0[a, b])) for a, b in pairs]
+        ee = oracle.fill_err_corr(nxp, nyp, nzp, prob.corr_xy, 0.0, 0.0)
+        for g in m + dd_:
+            oracle.fft_r2c(g, *n3, False, which)
+        for g in cc + [ee]:
+            oracle.fft_r2c(g, *n3, True, which)
+        nf_ = oracle.posterior(*n3, prob.A, etc_nc, c.wnc, prob.sigma0, ntabs, m, dd_, cc, ee, which=which)
+        for g in m + dd_:
+            oracle.fft_c2r(g, *n3, False, which)
+        return m, dd_, cc, nf_
+
+    ncm, ncd, ncc, nf_nc = nc_chain("oracle")
+    assert int(open(os.path.join(d, "nc_fail.txt")).read()) == nf_nc
+    e_nc = {"mean": max(rel_l2(_read(os.path.join(d, f"nc_mu{i}.bin"), shape)[:, :, :nxp], ncm[i][:, :, :nxp])
+                        for i in range(3)),
+            "residual": max(rel_l2(_read(os.path.join(d, f"nc_res{l}.bin"), shape)[:, :, :nxp], ncd[l][:, :, :nxp])
+                            for l in range(nt)),
+            "cov": max(rel_l2(np.fromfile(os.path.join(d, f"nc_cov{q}.bin"), dtype=np.float32).view(np.complex64),
+                              ncc[q].view(np.complex64).reshape(-1)) for q in range(6))}
+    res_bound = 5e-4
+    if oracle.ref_available():
+        _, rd, _, _ = nc_chain("ref")
+        spread = max(rel_l2(rd[l][:, :, :nxp], ncd[l][:, :, :nxp]) for l in range(nt))
+        res_bound = 2.0 * spread
+        print(f"  reference float chain vs oracle chain: residual {spread:.3g}", flush=True)
+    print(f"  non-constant thickness: errors vs oracle {e_nc}", flush=True)
+    assert e_nc["mean"] < 5e-5 and e_nc["cov"] < 5e-5
+    assert e_nc["residual"] < res_bound
     # SEG-Y (all five trace header formats) and SGRI files of the posterior
     # mean of alpha, byte for byte against the reference's NRLib writer on
     # the same grid and geometry
