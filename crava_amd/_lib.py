@@ -161,6 +161,13 @@ SIGNATURES = {
                                            VP, VP]),
     "crava_hip_wavelet_resample": (I, [VP, I, I, C.c_float, C.c_float, I, VP, C.POINTER(C.c_float)]),
     "crava_hip_wavelet_energy": (I, [I, I, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+    "crava_hip_prior_var0_robustify": (I, [VP, VP, C.POINTER(I)]),
+    "crava_hip_padding_size": (I, [I, C.c_double, C.POINTER(I)]),
+    "crava_hip_estimate_xy_padding": (I, [I, I, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, I, C.c_float, C.c_float, C.c_float, C.POINTER(I), C.POINTER(I),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "crava_hip_estimate_z_padding": (I, [I, C.c_double, C.c_double, C.c_double, I, C.c_float, C.POINTER(I),
+                                         C.POINTER(C.c_double)]),
     "crava_hip_divide_by_scale_wavelet": (I, [VP, C.POINTER(SwdivParams), C.POINTER(VP)]),
     "crava_hip_interpolate_seismic": (I, [VP, I, I, I, I, I, I, C.c_float, VP, C.POINTER(I), C.POINTER(I)]),
     "crava_hip_local_noise_g": (I, [I, VP, VP, VP, VP]),

@@ -179,6 +179,43 @@ def wavelet_energy(wavelets, A, prior_var0, corr_t, error_variance, emp_sn, matc
     return dict(zip(("gain", "model_var", "signal_var", "theo_sn"), out))
 
 
+def prior_var0_robustify(prior_var0):
+    """setCorrelationParameters' lift of an ill-conditioned Sigma0
+    (crava_hip_prior_var0_robustify): (3x3 float64, changed)."""
+    a = np.ascontiguousarray(prior_var0, dtype=np.float64).reshape(3, 3)
+    out = np.zeros((3, 3), np.float64)
+    ch = C.c_int(0)
+    check(lib().crava_hip_prior_var0_robustify(a.ctypes.data, out.ctypes.data, C.byref(ch)),
+          "crava_hip_prior_var0_robustify")
+    return out, bool(ch.value)
+
+
+def padding_size(n: int, pad_fac: float) -> int:
+    """ModelGeneral::setPaddingSize (crava_hip_padding_size)."""
+    out = C.c_int(0)
+    check(lib().crava_hip_padding_size(int(n), float(pad_fac), C.byref(out)), "crava_hip_padding_size")
+    return out.value
+
+
+def estimate_xy_padding(nx, ny, dx, dy, lx, ly, x_pad_fac, y_pad_fac, estimate, range1, range2, angle):
+    """ModelGeneral::estimateXYPaddingSizes: (nxpad, nypad, true x factor, true y factor)."""
+    a, b = C.c_int(0), C.c_int(0)
+    fa, fb = C.c_double(0.0), C.c_double(0.0)
+    check(lib().crava_hip_estimate_xy_padding(nx, ny, dx, dy, lx, ly, x_pad_fac, y_pad_fac, int(bool(estimate)),
+                                              range1, range2, angle, C.byref(a), C.byref(b), C.byref(fa),
+                                              C.byref(fb)), "crava_hip_estimate_xy_padding")
+    return a.value, b.value, fa.value, fb.value
+
+
+def estimate_z_padding(nz, lz, min_rel_thick, z_pad_fac, estimate, default_wavelet_length):
+    """ModelGeneral::estimateZPaddingSize: (nzpad, true z factor)."""
+    a, fa = C.c_int(0), C.c_double(0.0)
+    check(lib().crava_hip_estimate_z_padding(nz, lz, min_rel_thick, z_pad_fac, int(bool(estimate)),
+                                             default_wavelet_length, C.byref(a), C.byref(fa)),
+          "crava_hip_estimate_z_padding")
+    return a.value, fa.value
+
+
 def problem_tables(prob):
     cfg = prob.cfg
     return perk_tables(prob.ntheta, cfg.nz, cfg.nzp, prob.lz, cfg.low_cut, cfg.high_cut, 1.0,

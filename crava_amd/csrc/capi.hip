@@ -340,6 +340,64 @@ int crava_hip_wavelet_energy(int ntheta, int nzp, const float *wavelets, const f
   return CRAVA_HIP_OK;
 }
 
+int crava_hip_padding_size(int n, double pad_fac, int *npad) {
+  if (n < 1 || !npad || !(pad_fac >= 0.0) || !(n * (1.0 + pad_fac) < 1.0e8)) {
+    set_error("invalid arguments to crava_hip_padding_size");
+    return CRAVA_HIP_E_INVALID;
+  }
+  *npad = crava::padding_size(n, pad_fac);
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_estimate_xy_padding(int nx, int ny, double dx, double dy, double lx, double ly, double x_pad_fac,
+                                  double y_pad_fac, int estimate, float range1, float range2, float angle,
+                                  int *nxpad, int *nypad, double *true_x_pad_fac, double *true_y_pad_fac) {
+  if (nx < 1 || ny < 1 || !(dx > 0.0) || !(dy > 0.0) || !(lx > 0.0) || !(ly > 0.0) || !(x_pad_fac >= 0.0) ||
+      !(y_pad_fac >= 0.0) || !(x_pad_fac <= 1.0e4) || !(y_pad_fac <= 1.0e4) || !std::isfinite(range1) ||
+      !std::isfinite(range2) || !std::isfinite(angle) || !nxpad || !nypad || !true_x_pad_fac || !true_y_pad_fac) {
+    set_error("invalid arguments to crava_hip_estimate_xy_padding");
+    return CRAVA_HIP_E_INVALID;
+  }
+  crava::estimate_xy_padding(nx, ny, dx, dy, lx, ly, x_pad_fac, y_pad_fac, estimate != 0, range1, range2, angle, nxpad,
+                             nypad, true_x_pad_fac, true_y_pad_fac);
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_estimate_z_padding(int nz, double lz, double min_rel_thick, double z_pad_fac, int estimate,
+                                 float default_wavelet_length, int *nzpad, double *true_z_pad_fac) {
+  if (nz < 1 || !(lz > 0.0) || !(min_rel_thick > 0.0) || !(z_pad_fac >= 0.0) || !(z_pad_fac <= 1.0e4) ||
+      !(default_wavelet_length >= 0.0f) || !nzpad || !true_z_pad_fac) {
+    set_error("invalid arguments to crava_hip_estimate_z_padding");
+    return CRAVA_HIP_E_INVALID;
+  }
+  crava::estimate_z_padding(nz, lz, min_rel_thick, z_pad_fac, estimate != 0, default_wavelet_length, nzpad,
+                            true_z_pad_fac);
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_prior_var0_robustify(const double *prior_var0, double *out, int *changed) {
+  if (!prior_var0 || !out) {
+    set_error("invalid arguments to crava_hip_prior_var0_robustify");
+    return CRAVA_HIP_E_INVALID;
+  }
+  for (int i = 0; i < 9; ++i)
+    if (!std::isfinite(prior_var0[i])) {
+      set_error("crava_hip_prior_var0_robustify: non-finite prior covariance");
+      return CRAVA_HIP_E_INVALID;
+    }
+  double tmp[9];
+  const int rc = crava::prior_var0_robustify(prior_var0, tmp);
+  if (rc < 0) {
+    set_error("crava_hip_prior_var0_robustify: the prior covariance has an eigenvalue below 0.1 % of the largest "
+              "and a zero off-diagonal entry or rows that DGEEV would balance; its LAPACK eigenvectors, which the "
+              "lift uses, are not reproduced for such a matrix");
+    return CRAVA_HIP_E_INVALID;
+  }
+  for (int i = 0; i < 9; ++i) out[i] = tmp[i];
+  if (changed) *changed = rc;
+  return CRAVA_HIP_OK;
+}
+
 int crava_hip_complex_noise_reference(int nxp, int nyp, int nzp, uint32_t *seed, float *host_out) {
   if (nxp < 1 || nyp < 1 || nzp < 1 || !seed || !host_out) {
     set_error("invalid arguments to crava_hip_complex_noise_reference");

@@ -1,6 +1,8 @@
 // Host-side per-k preparation (see tables.hpp).
 #include "tables.hpp"
 
+#include "eig3.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -285,6 +287,138 @@ void wavelet_energy(int ntheta, int nzp, const float *wavelets, const float *A, 
     signal_var[l] = sv;
     theo_sn[l] = sv / error_variance[l];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Padding sizes (FFTGrid::findClosestFactorableNumber, src/fftgrid.cpp:
+// 2837-2866; ModelGeneral::setPaddingSize and estimate*PaddingSize(s),
+// src/modelgeneral.cpp:581-587,1337-1425), with the reference's float
+// logarithms for the loop bounds.
+
+int closest_factorable(int leastint) {
+  const float lf = std::log((float)leastint);
+  const int m2 = (int)std::ceil((double)lf / std::log(2.0f));
+  const int m3 = (int)std::ceil((double)lf / std::log(3.0f));
+  const int m5 = (int)std::ceil((double)lf / std::log(5.0f));
+  const int m7 = (int)std::ceil((double)lf / std::log(7.0f));
+  int closest = (int)std::pow(2.0f, m2);
+  for (int i = 0; i <= m2; ++i)
+    for (int j = 0; j <= m3; ++j)
+      for (int k = 0; k <= m5; ++k)
+        for (int l = 0; l <= m7; ++l) {
+          const int f = (int)(std::pow(2.0, i) * std::pow(3.0, j) * std::pow(5.0, k) * std::pow(7.0, l));
+          if (f >= leastint && f < closest) closest = f;
+        }
+  return closest;
+}
+
+int padding_size(int n, double pad_fac) {
+  return closest_factorable((int)std::ceil(n * (1.0f + pad_fac)));
+}
+
+void estimate_xy_padding(int nx, int ny, double dx, double dy, double lx, double ly, double x_pad_fac,
+                         double y_pad_fac, bool estimate, float range1, float range2, float angle, int *nxpad,
+                         int *nypad, double *true_x_pad_fac, double *true_y_pad_fac) {
+  if (estimate) {
+    const double factor = 0.5;   // half a lateral range
+    double xpad = factor * std::max(std::fabs(range1 * std::cos(angle)), std::fabs(range2 * std::sin(angle)));
+    double ypad = factor * std::max(std::fabs(range1 * std::sin(angle)), std::fabs(range2 * std::cos(angle)));
+    xpad = std::max(xpad, dx);   // at least one cell
+    ypad = std::max(ypad, dy);
+    x_pad_fac = std::min(1.0, xpad / lx);
+    y_pad_fac = std::min(1.0, ypad / ly);
+  }
+  *nxpad = padding_size(nx, x_pad_fac);
+  *nypad = padding_size(ny, y_pad_fac);
+  *true_x_pad_fac = (double)(*nxpad - nx) / (double)nx;
+  *true_y_pad_fac = (double)(*nypad - ny) / (double)ny;
+}
+
+void estimate_z_padding(int nz, double lz, double min_rel_thick, double z_pad_fac, bool estimate,
+                        float default_wavelet_length, int *nzpad, double *true_z_pad_fac) {
+  const double min_lz = lz * min_rel_thick;
+  if (estimate) {
+    const double zpad = (double)default_wavelet_length / 1.0;
+    z_pad_fac = std::min(1.0, zpad / min_lz);
+  }
+  *nzpad = padding_size(nz, z_pad_fac);
+  *true_z_pad_fac = (double)(*nzpad - nz) / (double)nz;
+}
+
+// ---------------------------------------------------------------------------
+// SeismicParametersHolder::setCorrelationParameters' robustification of
+// Sigma0 (src/seismicparametersholder.cpp:114-135).
+
+namespace {
+
+// eigenvalues of a symmetric 3x3 matrix (cyclic Jacobi), for the condition
+// test only
+void sym3_eigenvalues(const double v[3][3], double ev[3]) {
+  double m[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) m[a][b] = 0.5 * (v[a][b] + v[b][a]);
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    const double off = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
+    if (off <= 1e-300) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        if (m[p][q] == 0.0) continue;
+        const double th = 0.5 * (m[q][q] - m[p][p]) / m[p][q];
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < 3; ++k) {
+          const double kp = m[k][p], kq = m[k][q];
+          m[k][p] = c * kp - sn * kq;
+          m[k][q] = sn * kp + c * kq;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double pk = m[p][k], qk = m[q][k];
+          m[p][k] = c * pk - sn * qk;
+          m[q][k] = sn * pk + c * qk;
+        }
+      }
+  }
+  for (int a = 0; a < 3; ++a) ev[a] = m[a][a];
+}
+
+}  // namespace
+
+int prior_var0_robustify(const double in[9], double out[9]) {
+  double m[3][3], wr[3], vr[3][3];
+  bool diag = true;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      m[r][c] = in[r * 3 + c];
+      out[r * 3 + c] = in[r * 3 + c];
+      if (r != c && m[r][c] != 0.0) diag = false;
+    }
+  bool have_vectors = dgeev3(m, wr, vr);
+  if (!have_vectors && diag) {
+    // DGEEV on a diagonal matrix: DGEBAL isolates every eigenvalue without
+    // exchanges, so the eigenvalues are the diagonal in order and the
+    // eigenvectors the unit vectors
+    for (int r = 0; r < 3; ++r) {
+      wr[r] = m[r][r];
+      for (int c = 0; c < 3; ++c) vr[r][c] = r == c ? 1.0 : 0.0;
+    }
+    have_vectors = true;
+  }
+  if (!have_vectors) sym3_eigenvalues(m, wr);
+  double mx = wr[0];
+  for (int i = 1; i < 3; ++i)
+    if (mx < wr[i]) mx = wr[i];
+  bool lift = false;
+  for (int k = 0; k < 3; ++k) lift = lift || wr[k] < mx * 0.001;
+  if (!lift) return 0;
+  if (!have_vectors) return -1;
+  // the reference forms the update from eVec(k, i) eVec(k, j): row k of the
+  // eigenvector matrix (component k of every eigenvector), kept as written
+  for (int k = 0; k < 3; ++k) {
+    if (!(wr[k] < mx * 0.001)) continue;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) out[i * 3 + j] += vr[k][i] * vr[k][j] * (mx * 0.0011 - wr[k]);
+  }
+  return 1;
 }
 
 // ---------------------------------------------------------------------------

@@ -43,6 +43,28 @@ void wavelet_energy(int ntheta, int nzp, const float *wavelets, const float *A, 
                     const unsigned char *match, float *gain, float *model_var, float *signal_var,
                     float *theo_sn);
 
+// FFTGrid::findClosestFactorableNumber (src/fftgrid.cpp:2837-2866): the
+// smallest 2^a 3^b 5^c 7^d >= leastint within the reference's loop bounds
+// (ceil of float logarithms), else 2^ceil(log2 leastint).
+int closest_factorable(int leastint);
+// ModelGeneral::setPaddingSize (src/modelgeneral.cpp:581-587)
+int padding_size(int n, double pad_fac);
+// ModelGeneral::estimateXYPaddingSizes / estimateZPaddingSize
+// (src/modelgeneral.cpp:1337-1425) without the logging.
+void estimate_xy_padding(int nx, int ny, double dx, double dy, double lx, double ly, double x_pad_fac,
+                         double y_pad_fac, bool estimate, float range1, float range2, float angle, int *nxpad,
+                         int *nypad, double *true_x_pad_fac, double *true_y_pad_fac);
+void estimate_z_padding(int nz, double lz, double min_rel_thick, double z_pad_fac, bool estimate,
+                        float default_wavelet_length, int *nzpad, double *true_z_pad_fac);
+
+// SeismicParametersHolder::setCorrelationParameters' robustification of
+// Sigma0 (src/seismicparametersholder.cpp:114-135) with DGEEV's eigenvalues
+// and eigenvectors (dgeev3; a diagonal Sigma0 in closed form): returns 0
+// when every eigenvalue is at least 0.1 % of the largest (out = in), 1 when
+// out was lifted, -1 when it would be lifted but DGEEV's eigenvectors are
+// not reproduced here (a zero off-diagonal entry or balancing).
+int prior_var0_robustify(const double in[9], double out[9]);
+
 // Half spectrum (nzp/2+1 {re, im} pairs) of a real series of nzp samples
 // with FFTW's forward sign and no normalisation (Wavelet::fft1DInPlace,
 // src/wavelet.cpp:246-261): double accumulation rounded to float, DC and

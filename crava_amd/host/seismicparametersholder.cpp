@@ -29,53 +29,17 @@ void SeismicParametersHolder::allocateGrids(int nx, int ny, int nz, int nxp, int
   }
 }
 
-namespace {
-// eigenvalues of the symmetric part of a 3x3 matrix (cyclic Jacobi)
-void sym3_eigenvalues(const double v[3][3], double ev[3]) {
-  double m[3][3];
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) m[a][b] = 0.5 * (v[a][b] + v[b][a]);
-  for (int sweep = 0; sweep < 50; ++sweep) {
-    const double off = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
-    if (off <= 1e-300) break;
-    for (int p = 0; p < 2; ++p)
-      for (int q = p + 1; q < 3; ++q) {
-        if (m[p][q] == 0.0) continue;
-        const double th = 0.5 * (m[q][q] - m[p][p]) / m[p][q];
-        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
-        for (int k = 0; k < 3; ++k) {            // columns p, q
-          const double kp = m[k][p], kq = m[k][q];
-          m[k][p] = c * kp - sn * kq;
-          m[k][q] = sn * kp + c * kq;
-        }
-        for (int k = 0; k < 3; ++k) {            // rows p, q
-          const double pk = m[p][k], qk = m[q][k];
-          m[p][k] = c * pk - sn * qk;
-          m[q][k] = sn * pk + c * qk;
-        }
-      }
-  }
-  for (int a = 0; a < 3; ++a) ev[a] = m[a][a];
-}
-}  // namespace
 
-// setCorrelationParameters (src/seismicparametersholder.cpp:92-149) keeps
-// Sigma0 as given when every eigenvalue is at least 0.1 % of the largest.
-// Below that the reference adds a correction built from rows of LAPACK
-// dgeev's eigenvector matrix paired with its (unsorted) eigenvalues, which
-// depends on the installed LAPACK's ordering; such a Sigma0 is rejected here.
+// setCorrelationParameters (src/seismicparametersholder.cpp:92-149): Sigma0
+// with the reference's lift of eigenvalues below 0.1 % of the largest
+// (crava_hip_prior_var0_robustify); a lift that library cannot reproduce
+// throws.
 void SeismicParametersHolder::setPriorVar0(const double v[3][3]) {
-  double ev[3];
-  sym3_eigenvalues(v, ev);
-  const double mx = std::max(ev[0], std::max(ev[1], ev[2]));
-  for (double e : ev)
-    if (e < mx * 0.001)
-      throw Error(CRAVA_HIP_E_INVALID,
-                  "prior covariance is ill-conditioned (an eigenvalue below 0.1 % of the largest): the reference's "
-                  "robustification of it depends on its LAPACK's eigenvector order and is not reproduced");
+  double out[9];
+  int changed = 0;
+  check(crava_hip_prior_var0_robustify(&v[0][0], out, &changed), "setPriorVar0");
   for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) priorVar0_[a][b] = v[a][b];
+    for (int b = 0; b < 3; ++b) priorVar0_[a][b] = out[a * 3 + b];
 }
 
 void SeismicParametersHolder::setCovTransformPrecision(int precision) {

@@ -178,6 +178,37 @@ int crava_hip_perk_tables_nonconst(int ntheta, int nz, int nzp, double lz, float
 int crava_hip_wavelet_resample(const float *wavelet, int nw, int center, float dz_wavelet, float dz, int nzp,
                                float *out, float *norm);
 
+/* Padding sizes, host side.  crava_hip_padding_size: ModelGeneral::
+ * setPaddingSize (src/modelgeneral.cpp:581-587), the smallest
+ * 2^a 3^b 5^c 7^d >= ceil(n (1 + pad_fac)) as FFTGrid::
+ * findClosestFactorableNumber finds it (src/fftgrid.cpp:2837-2866, loop
+ * bounds from float logarithms).  crava_hip_estimate_xy_padding /
+ * crava_hip_estimate_z_padding: ModelGeneral::estimateXYPaddingSizes /
+ * estimateZPaddingSize (:1337-1425) -- with estimate != 0 the factors come
+ * from half the lateral variogram range (range1, range2 in m, angle in
+ * radians, at least one cell, at most 1) or from the default wavelet length
+ * over lz * min_rel_thick (at most 1); the outputs are the padded sizes and
+ * the true factors (npad - n) / n. */
+int crava_hip_padding_size(int n, double pad_fac, int *npad);
+int crava_hip_estimate_xy_padding(int nx, int ny, double dx, double dy, double lx, double ly, double x_pad_fac,
+                                  double y_pad_fac, int estimate, float range1, float range2, float angle,
+                                  int *nxpad, int *nypad, double *true_x_pad_fac, double *true_y_pad_fac);
+int crava_hip_estimate_z_padding(int nz, double lz, double min_rel_thick, double z_pad_fac, int estimate,
+                                 float default_wavelet_length, int *nzpad, double *true_z_pad_fac);
+
+/* The robustification of the prior covariance in
+ * SeismicParametersHolder::setCorrelationParameters
+ * (src/seismicparametersholder.cpp:114-135), host side: with DGEEV's
+ * eigenvalues lambda_k and right eigenvectors V (columns), every k with
+ * lambda_k < 0.001 max(lambda) adds (0.0011 max - lambda_k) V(k,i) V(k,j)
+ * to entry (i,j) -- row k of V, as the reference indexes it.  prior_var0
+ * and out: row-major 3x3 (may alias); *changed (may be NULL): 0 when Sigma0
+ * was kept, 1 when lifted.  CRAVA_HIP_E_INVALID for an ill-conditioned
+ * Sigma0 whose DGEEV eigenvectors this library does not reproduce (a zero
+ * off-diagonal entry other than a diagonal Sigma0, or rows DGEBAL would
+ * scale). */
+int crava_hip_prior_var0_robustify(const double *prior_var0, double *out, int *changed);
+
 /* Crava::computeVariances' variances and energy-matching gain (src/crava.cpp:
  * 400-466, computeWDCorrMVar :1763-1775), host side.  wavelets [ntheta][nzp]
  * in FFT order; A [ntheta][3]; prior_var0 Sigma0 (row-major 3x3); corr_t

@@ -247,6 +247,31 @@ def wavelet_energy(wavelets, A, prior_var0, corr_t, error_variance, emp_sn, matc
     return dict(zip(("gain", "model_var", "signal_var", "theo_sn"), out))
 
 
+def prior_var0_robustify(prior_var0):
+    """SeismicParametersHolder::setCorrelationParameters' lift of Sigma0
+    (src/seismicparametersholder.cpp:114-135) with LAPACK's DGEEV (scipy's
+    geev, right eigenvectors in columns), row k of the eigenvector matrix as
+    the reference indexes it: returns (3x3 float64, changed)."""
+    import scipy.linalg as sl
+    a = np.array(prior_var0, dtype=np.float64).reshape(3, 3)
+    w, vr = sl.eig(a.copy(), left=False, right=True)
+    w = w.real
+    vr = vr.real
+    mx = w[0]
+    for i in range(1, 3):
+        if mx < w[i]:
+            mx = w[i]
+    out = a.copy()
+    changed = False
+    for k in range(3):
+        if w[k] < mx * 0.001:
+            changed = True
+            for i in range(3):
+                for j in range(3):
+                    out[i, j] += vr[k, i] * vr[k, j] * (mx * 0.0011 - w[k])
+    return out, changed
+
+
 # -------------------------------------------------------------- posterior
 def posterior(nxp, nyp, nzp, A, etc, wnc, sigma0, tabs, mu, data, cov, errcorr, which: str = "oracle") -> int:
     """Float per-point loop on complex64 grids (in place).  Returns #failed."""
