@@ -162,6 +162,11 @@ SIGNATURES = {
     "crava_hip_wavelet_resample": (I, [VP, I, I, C.c_float, C.c_float, I, VP, C.POINTER(C.c_float)]),
     "crava_hip_wavelet_energy": (I, [I, I, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "crava_hip_prior_var0_robustify": (I, [VP, VP, C.POINTER(I)]),
+    "crava_hip_facies_local_noise": (I, [VP, I, I, I, I, I, I, VP, VP, VP, I, I, VP, I, I, I, VP, VP, VP, VP,
+                                         C.c_float, C.POINTER(VP), VP, VP]),
+    "crava_hip_kde_table": (I, [VP, LL, VP, VP, VP, VP, I, I, I, VP, VP]),
+    "crava_hip_facies_densities": (I, [VP, I, VP, VP, VP, VP, I, I, VP, VP, I, I, I, I, C.POINTER(VP), VP, VP]),
+    "crava_hip_local_noise_sigmanew": (I, [I, VP, VP, VP, VP, VP, VP]),
     "crava_hip_padding_size": (I, [I, C.c_double, C.POINTER(I)]),
     "crava_hip_estimate_xy_padding": (I, [I, I, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                           C.c_double, I, C.c_float, C.c_float, C.c_float, C.POINTER(I), C.POINTER(I),

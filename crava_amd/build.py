@@ -33,6 +33,7 @@ SOURCES = {
     "fft.hip": [],
     "posterior.hip": [],
     "tables.cpp": [],
+    "kde.cpp": ["-ffp-contract=off"],
     "simulate.hip": [],
     "facies.hip": ["-ffp-contract=off"],
     "nll.hip": [],

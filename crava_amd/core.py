@@ -216,6 +216,80 @@ def estimate_z_padding(nz, lz, min_rel_thick, z_pad_fac, estimate, default_wavel
     return a.value, fa.value
 
 
+def kde_table(ctx: Context, d1, d2, d3, sigma, n1: int, n2: int, n3: int, limits, table: torch.Tensor):
+    """A facies density table from samples (crava_hip_kde_table) into the
+    device grid `table` [n3][n2][2(n1/2+1)] float32."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (d1, d2, d3)]
+    sg = np.ascontiguousarray(sigma, dtype=np.float64).reshape(9)
+    lim = np.ascontiguousarray(limits, dtype=np.float64)
+    check(lib().crava_hip_kde_table(ctx.h, len(a[0]), a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data,
+                                    sg.ctypes.data, n1, n2, n3, lim.ctypes.data, _dev_ptr(table)),
+          "crava_hip_kde_table")
+
+
+def facies_densities(ctx: Context, alpha, beta, rho, facies_log, nfac: int, ndens: int = 1, sample_transform=None,
+                     sigma_e=None, no_vs: bool = False, n=(100, 100, 50)):
+    """MakePosteriorElasticPDF3D's density sets (crava_hip_facies_densities):
+    (tables tensor [ndens][nfac][n3][n2][2(n1/2+1)] on the GPU, volumes
+    [ndens][6], vn [ndens][3])."""
+    n1, n2, n3 = n
+    rn = 2 * (n1 // 2 + 1)
+    tabs = torch.zeros((ndens, nfac, n3, n2, rn), dtype=torch.float32, device="cuda")
+    ptrs = ptr_array([int(tabs[d, l].data_ptr()) for d in range(ndens) for l in range(nfac)])
+    a, b, r = (np.ascontiguousarray(x, dtype=np.float32) for x in (alpha, beta, rho))
+    fl = np.ascontiguousarray(facies_log, dtype=np.int32)
+    st = None if sample_transform is None else np.ascontiguousarray(sample_transform, dtype=np.float64)
+    se = None if sigma_e is None else np.ascontiguousarray(sigma_e, dtype=np.float64)
+    vols = np.zeros((ndens, 6), np.float64)
+    vn = np.zeros((ndens, 3), np.int32)
+    check(lib().crava_hip_facies_densities(ctx.h, len(a), a.ctypes.data, b.ctypes.data, r.ctypes.data, fl.ctypes.data,
+                                           nfac, ndens, None if st is None else st.ctypes.data,
+                                           None if se is None else se.ctypes.data, int(no_vs), n1, n2, n3, ptrs,
+                                           vols.ctypes.data, vn.ctypes.data), "crava_hip_facies_densities")
+    return tabs, vols, vn
+
+
+def local_noise_sigmanew(G, err_theta_cov, prior_var0, post_var0, scale):
+    """newPosteriorCovPointwise's sigmanew for one scale vector
+    (crava_hip_local_noise_sigmanew): 3x3 float64."""
+    g = np.ascontiguousarray(G, dtype=np.float64)
+    e = np.ascontiguousarray(err_theta_cov, dtype=np.float64)
+    sm = np.ascontiguousarray(prior_var0, dtype=np.float64)
+    smd = np.ascontiguousarray(post_var0, dtype=np.float64)
+    sc = np.ascontiguousarray(scale, dtype=np.float64)
+    out = np.zeros((3, 3), np.float64)
+    check(lib().crava_hip_local_noise_sigmanew(g.shape[0], g.ctypes.data, e.ctypes.data, sm.ctypes.data,
+                                               smd.ctypes.data, sc.ctypes.data, out.ctypes.data),
+          "crava_hip_local_noise_sigmanew")
+    return out
+
+
+def facies_local_noise(ctx: Context, nx, ny, nz, nxp, nyp, nzp, alpha, beta, rho, tables: torch.Tensor, tn, vols, vn,
+                       noise_scale, prior, p_undefined, argmax=None, counts=None):
+    """Facies probabilities with the local-noise density dimension
+    (crava_hip_facies_local_noise): tables [2^nang][nfac][tn3][tn2][rn1] on
+    the GPU (tn = (tn1, tn2, tn3)), noise_scale a list of nang host [ny][nx]
+    float64 tables.  Returns the nfac + 1 probability grids
+    [nz][ny][2(nx/2+1)]."""
+    nfac = tables.shape[1]
+    tn1, tn2, tn3 = tn
+    nang = len(noise_scale)
+    out = [torch.zeros((nz, ny, 2 * (nx // 2 + 1)), dtype=torch.float32, device="cuda") for _ in range(nfac + 1)]
+    ns = [np.ascontiguousarray(t, dtype=np.float64) for t in noise_scale]
+    ns_arr = (C.c_void_p * max(1, nang))(*[t.ctypes.data for t in ns])
+    v = np.ascontiguousarray(vols, dtype=np.float64)
+    n = np.ascontiguousarray(vn, dtype=np.int32)
+    pr = np.ascontiguousarray(prior, dtype=np.float32)
+    check(lib().crava_hip_facies_local_noise(ctx.h, nx, ny, nz, nxp, nyp, nzp, _dev_ptr(alpha), _dev_ptr(beta),
+                                             _dev_ptr(rho), nfac, nang, _dev_ptr(tables), tn1, tn2, tn3,
+                                             v.ctypes.data, n.ctypes.data, ns_arr, pr.ctypes.data, p_undefined,
+                                             ptr_array([_dev_ptr(o) for o in out]),
+                                             None if argmax is None else _dev_ptr(argmax),
+                                             None if counts is None else _dev_ptr(counts)),
+          "crava_hip_facies_local_noise")
+    return out
+
+
 def problem_tables(prob):
     cfg = prob.cfg
     return perk_tables(prob.ntheta, cfg.nz, cfg.nzp, prob.lz, cfg.low_cut, cfg.high_cut, 1.0,

@@ -11,6 +11,7 @@
 #include "posterior.hpp"
 #include "simulate.hpp"
 #include "swdiv.hpp"
+#include "kde.hpp"
 #include "localnoise.hpp"
 #include "tables.hpp"
 
@@ -761,6 +762,198 @@ int crava_hip_facies(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int ny
   if (counts && nfac >= 1 && nfac <= crava::kMaxFacies)
     CRAVA_HIP_CHECK(hipMemsetAsync(counts, 0, sizeof(unsigned long long) * nfac, ctx->stream));
   return crava::launch_facies(c, ctx->stream);
+}
+
+int crava_hip_facies_local_noise(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp,
+                                 const float *alpha, const float *beta, const float *rho, int nfac, int nang,
+                                 const float *tables, int tn1, int tn2, int tn3, const double *volumes,
+                                 const int *vn, const double *const *noise_scale, const float *prior,
+                                 float p_undefined, float *const *prob, int *argmax, unsigned long long *counts) {
+  if (!ctx || !alpha || !beta || !rho || !tables || !volumes || !vn || !prior || !prob ||
+      (nang > 0 && !noise_scale)) {
+    set_error("NULL argument to crava_hip_facies_local_noise");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (nang < 0 || nang > crava::kMaxDensAngles) { set_error("nang must be 0..8"); return CRAVA_HIP_E_INVALID; }
+  const int ndens = 1 << nang;
+  if (nx < 1 || ny < 1 || nz < 1 || nx > nxp || ny > nyp || nz > nzp || tn1 < 1 || tn2 < 1 || tn3 < 1 ||
+      nfac < 1 || nfac > crava::kMaxFacies) {
+    set_error("invalid sizes in crava_hip_facies_local_noise");
+    return CRAVA_HIP_E_INVALID;
+  }
+  for (int d = 0; d < ndens; ++d)
+    if (vn[3 * d] < 1 || vn[3 * d + 1] < 1 || vn[3 * d + 2] < 1 || vn[3 * d] > tn1 || vn[3 * d + 1] > tn2 ||
+        vn[3 * d + 2] > tn3) {
+      set_error("density lattice larger than its table");
+      return CRAVA_HIP_E_INVALID;
+    }
+  for (int l = 0; l <= nfac; ++l)
+    if (!prob[l]) { set_error("NULL probability grid"); return CRAVA_HIP_E_INVALID; }
+  for (int g = 0; g < nang; ++g)
+    if (!noise_scale[g]) { set_error("NULL noise-scale table"); return CRAVA_HIP_E_INVALID; }
+  // angle weights t = (scale - min) / (max - min) per angle, 0 where the
+  // table is constant (src/faciesprob.cpp:2168-2179), Grid2D::FindMin /
+  // FindMax skipping RMISSING; float(t) as the loop reads them (:2221-2222)
+  const size_t ntr = (size_t)nx * ny;
+  std::vector<crava::FaciesVolume> vols(ndens);
+  for (int d = 0; d < ndens; ++d) {
+    const double *v = volumes + 6 * d;
+    vols[d] = crava::FaciesVolume{v[0], v[1], v[2], v[3], v[4], v[5], vn[3 * d], vn[3 * d + 1], vn[3 * d + 2], 0};
+  }
+  std::vector<float> tw((size_t)nang * ntr);
+  const double missing = -99999.000;
+  for (int g = 0; g < nang; ++g) {
+    const double *tab = noise_scale[g];
+    double mn = tab[0], mx = tab[0];
+    for (size_t t = 0; t < ntr; ++t) {
+      if ((mn == missing || tab[t] < mn) && tab[t] != missing) mn = tab[t];
+      if ((mx == missing || tab[t] > mx) && tab[t] != missing) mx = tab[t];
+    }
+    for (size_t t = 0; t < ntr; ++t) tw[(size_t)g * ntr + t] = mn == mx ? 0.0f : (float)((tab[t] - mn) / (mx - mn));
+  }
+  void *ws = nullptr;
+  const size_t b_vol = sizeof(crava::FaciesVolume) * ndens, b_t = sizeof(float) * tw.size();
+  int rc = ctx->workspace(b_vol + b_t + 256, &ws);
+  if (rc) return rc;
+  auto *dvol = static_cast<crava::FaciesVolume *>(ws);
+  float *dt = reinterpret_cast<float *>(static_cast<char *>(ws) + ((b_vol + 255) / 256) * 256);
+  CRAVA_HIP_CHECK(hipMemcpyAsync(dvol, vols.data(), b_vol, hipMemcpyHostToDevice, ctx->stream));
+  if (b_t) CRAVA_HIP_CHECK(hipMemcpyAsync(dt, tw.data(), b_t, hipMemcpyHostToDevice, ctx->stream));
+  crava::FaciesCall c{nx, ny, nz, nxp, nyp, nzp, alpha, beta, rho, nfac, tables, tn1, tn2, tn3,
+                      volumes, vn, prior, p_undefined, prob, argmax, counts};
+  c.ndens = ndens;
+  c.nang = nang;
+  c.vols = dvol;
+  c.tgrid = dt;
+  if (counts) CRAVA_HIP_CHECK(hipMemsetAsync(counts, 0, sizeof(unsigned long long) * nfac, ctx->stream));
+  if ((rc = crava::launch_facies(c, ctx->stream))) return rc;
+  // the host vectors are staged by the copies; wait so the workspace is not reused early
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_kde_table(crava_hip_ctx *ctx, long long npts, const double *d1, const double *d2, const double *d3,
+                        const double *sigma, int n1, int n2, int n3, const double *limits, void *table) {
+  if (!ctx || !sigma || !limits || !table || npts < 0 || (npts > 0 && (!d1 || !d2 || !d3)) || n1 < 2 || n2 < 1 ||
+      n3 < 1 || (long long)n1 * n2 * n3 > (1LL << 26)) {
+    set_error("invalid arguments to crava_hip_kde_table");
+    return CRAVA_HIP_E_INVALID;
+  }
+  for (int d = 0; d < 3; ++d)
+    if (!std::isfinite(limits[2 * d]) || !std::isfinite(limits[2 * d + 1]) || !(limits[2 * d + 1] > limits[2 * d])) {
+      set_error("crava_hip_kde_table: each dimension needs finite limits with max > min");
+      return CRAVA_HIP_E_INVALID;
+    }
+  const size_t rsize = (size_t)(2 * (n1 / 2 + 1)) * n2 * n3;
+  std::vector<float> hist(rsize), smooth(rsize);
+  double s3[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q) s3[r][q] = sigma[r * 3 + q];
+  if (!crava::kde_smoother(s3, n1, n2, n3, limits, smooth.data())) {
+    set_error("crava_hip_kde_table: the smoothing covariance is not positive definite");
+    return CRAVA_HIP_E_INVALID;
+  }
+  crava::kde_histogram(npts, d1, d2, d3, n1, n2, n3, limits, hist.data());
+  void *ws = nullptr;
+  int rc = ctx->workspace(sizeof(float) * rsize, &ws);
+  if (rc) return rc;
+  CRAVA_HIP_CHECK(hipMemcpyAsync(table, hist.data(), sizeof(float) * rsize, hipMemcpyHostToDevice, ctx->stream));
+  CRAVA_HIP_CHECK(hipMemcpyAsync(ws, smooth.data(), sizeof(float) * rsize, hipMemcpyHostToDevice, ctx->stream));
+  // FFTGrid::fftInPlace of both (PARAMETER: 1/sqrt(N)), multiply, inverse
+  // (1/sqrt(N)) and multiplyByScalar(sqrt(float(N))) (src/posteriorelasticpdf3d.cpp:80-129)
+  void *both[2] = {table, ws};
+  if ((rc = fft_common(ctx, both, 2, n1, n2, n3, CRAVA_HIP_PARAMETER, CRAVA_HIP_F32, false))) return rc;
+  if ((rc = crava_hip_grid_op(ctx, CRAVA_HIP_OP_MULTIPLY, table, ws, n1, n2, n3, 1, 0.0f))) return rc;
+  if ((rc = fft_common(ctx, &table, 1, n1, n2, n3, CRAVA_HIP_PARAMETER, CRAVA_HIP_F32, true))) return rc;
+  if ((rc = crava_hip_grid_op(ctx, CRAVA_HIP_OP_MULTIPLY_SCALAR, table, nullptr, n1, n2, n3, 0,
+                              std::sqrt(float(n1 * n2 * n3)))))
+    return rc;
+  CRAVA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_facies_densities(crava_hip_ctx *ctx, int nsamples, const float *alpha, const float *beta,
+                               const float *rho, const int *facies_log, int nfac, int ndens,
+                               const double *sample_transform, const double *sigma_e, int no_vs, int n1, int n2,
+                               int n3, void *const *tables, double *volumes, int *vn) {
+  if (!ctx || nsamples < 2 || !alpha || !beta || !rho || !facies_log || nfac < 1 || nfac > crava::kMaxFacies ||
+      ndens < 1 || ndens > (1 << crava::kMaxDensAngles) || !tables || !volumes || !vn) {
+    set_error("invalid arguments to crava_hip_facies_densities");
+    return CRAVA_HIP_E_INVALID;
+  }
+  for (int i = 0; i < ndens * nfac; ++i)
+    if (!tables[i]) { set_error("NULL table grid"); return CRAVA_HIP_E_INVALID; }
+  int nobs = 0;
+  for (int i = 0; i < nsamples; ++i) nobs += facies_log[i] != crava::kIMissing;
+  if (nobs < 2) { set_error("crava_hip_facies_densities needs two samples with a facies"); return CRAVA_HIP_E_INVALID; }
+  std::vector<float> a(nsamples), b(nsamples), r(nsamples);
+  std::vector<double> fa, fb, fr;
+  for (int d = 0; d < ndens; ++d) {
+    // the samples of density d: H (alpha, beta, rho) for d > 0
+    // (src/faciesprob.cpp:967-1003), float
+    for (int i = 0; i < nsamples; ++i) {
+      if (d > 0 && sample_transform) {
+        const double *h = sample_transform + 9 * d;
+        const double h1[3] = {(double)alpha[i], (double)beta[i], (double)rho[i]};
+        double h2[3];
+        for (int q = 0; q < 3; ++q) h2[q] = h[q * 3 + 0] * h1[0] + h[q * 3 + 1] * h1[1] + h[q * 3 + 2] * h1[2];
+        a[i] = (float)h2[0];
+        b[i] = (float)h2[1];
+        r[i] = (float)h2[2];
+      } else {
+        a[i] = alpha[i];
+        b[i] = beta[i];
+        r[i] = rho[i];
+      }
+    }
+    double sig[3][3], lim[6];
+    crava::facies_density_setup(nsamples, a.data(), b.data(), r.data(), facies_log,
+                                sigma_e ? sigma_e + 9 * d : nullptr, no_vs != 0, n1, n2, n3, sig, lim);
+    // the Simbox of the lattice (src/faciesprob.cpp:1100-1102, Simbox ctor
+    // src/simbox.cpp:35-66): n = int(0.5 + l / d), base = top + lz
+    const double dA = (lim[1] - lim[0]) / n1, dB = (lim[3] - lim[2]) / n2, dR = (lim[5] - lim[4]) / n3;
+    double *vol = volumes + 6 * d;
+    vol[0] = lim[0];
+    vol[1] = lim[2];
+    vol[2] = dA;
+    vol[3] = dB;
+    vol[4] = lim[4];
+    vol[5] = lim[4] + (lim[5] - lim[4]);
+    vn[3 * d] = int(0.5 + (lim[1] - lim[0]) / dA);
+    vn[3 * d + 1] = int(0.5 + (lim[3] - lim[2]) / dB);
+    vn[3 * d + 2] = int(0.5 + (lim[5] - lim[4]) / dR);
+    double sflat[9];
+    for (int q = 0; q < 9; ++q) sflat[q] = sig[q / 3][q % 3];
+    for (int l = 0; l < nfac; ++l) {
+      fa.clear();
+      fb.clear();
+      fr.clear();
+      for (int i = 0; i < nsamples; ++i)
+        if (facies_log[i] == l) {
+          fa.push_back(a[i]);
+          fb.push_back(b[i]);
+          fr.push_back(r[i]);
+        }
+      int rc = crava_hip_kde_table(ctx, (long long)fa.size(), fa.data(), fb.data(), fr.data(), sflat, n1, n2, n3, lim,
+                                   tables[d * nfac + l]);
+      if (rc) return rc;
+    }
+  }
+  return CRAVA_HIP_OK;
+}
+
+int crava_hip_local_noise_sigmanew(int ntheta, const double *G, const double *errThetaCov, const double *priorVar0,
+                                   const double *postVar0, const double *scale, double *sigmanew) {
+  if (ntheta < 1 || ntheta > crava::kLncMaxTheta || !G || !errThetaCov || !priorVar0 || !postVar0 || !scale ||
+      !sigmanew) {
+    set_error("invalid arguments to crava_hip_local_noise_sigmanew");
+    return CRAVA_HIP_E_INVALID;
+  }
+  for (int a = 0; a < ntheta; ++a)
+    if (!(scale[a] >= 0.0)) { set_error("noise scales must be >= 0"); return CRAVA_HIP_E_INVALID; }
+  crava::LncConst c;
+  crava::lnc_const_host(ntheta, G, errThetaCov, priorVar0, postVar0, c);
+  return crava::lnc_sigmanew_host(c, ntheta, scale, sigmanew);
 }
 
 // ---------------------------------------------------------------------------

@@ -24,11 +24,13 @@ struct FaciesArgs {
   long long tsize;      // floats per table
   int nx, ny, nz, rnxp, nyp, smallrnxp;
   int tn1r, tn2;        // table row pitch (2*(n1/2+1)) and n2
-  int vn0, vn1, vn2;    // volume cell counts
-  double x0, y0, dx, dy, ztop, zbot;
+  FaciesVolume v0;      // the lattice of density 0
   float prior[kMaxFacies];
   float undef_sum;
   int nfac;
+  int ndens, nang;
+  const FaciesVolume *vols;
+  const float *tgrid;
 };
 
 __device__ __forceinline__ float tv(const float *t, int rn, int n2, int a, int b, int c) {
@@ -36,26 +38,27 @@ __device__ __forceinline__ float tv(const float *t, int rn, int n2, int a, int b
   return v > 0.0f ? v : 0.0f;  // std::max<float>(0, v)
 }
 
-__device__ __forceinline__ double find_density(const FaciesArgs &f, const float *t, double vp, double vs, double rho) {
+__device__ __forceinline__ double find_density(const FaciesArgs &f, const FaciesVolume &v, const float *t, double vp,
+                                               double vs, double rho) {
   double value = 0.0;
-  const double rx = (vp - f.x0) * 1.0 + (vs - f.y0) * 0.0;
-  const double ry = -(vp - f.x0) * 0.0 + (vs - f.y0) * 1.0;
-  const double jF = rx / f.dx - 0.5;
-  const double kF = ry / f.dy - 0.5;
-  const double lF = (double)(f.vn2) * (rho - f.ztop) / (f.zbot - f.ztop) - 0.5;
+  const double rx = (vp - v.x0) * 1.0 + (vs - v.y0) * 0.0;
+  const double ry = -(vp - v.x0) * 0.0 + (vs - v.y0) * 1.0;
+  const double jF = rx / v.dx - 0.5;
+  const double kF = ry / v.dy - 0.5;
+  const double lF = (double)(v.n2) * (rho - v.ztop) / (v.zbot - v.ztop) - 0.5;
   int j1, k1, l1, j2, k2, l2;
   float wj, wk, wl;
   j1 = (int)floor(jF);
   if (j1 < 0) { j1 = 0; j2 = 0; wj = 0; }
-  else if (j1 >= f.vn0 - 1) { j1 = f.vn0 - 1; j2 = j1; wj = 0; }
+  else if (j1 >= v.n0 - 1) { j1 = v.n0 - 1; j2 = j1; wj = 0; }
   else { j2 = j1 + 1; wj = (float)(jF - j1); }
   k1 = (int)floor(kF);
   if (k1 < 0) { k1 = 0; k2 = 0; wk = 0; }
-  else if (k1 >= f.vn1 - 1) { k1 = f.vn1 - 1; k2 = k1; wk = 0; }
+  else if (k1 >= v.n1 - 1) { k1 = v.n1 - 1; k2 = k1; wk = 0; }
   else { k2 = k1 + 1; wk = (float)(kF - k1); }
   l1 = (int)floor(lF);
   if (l1 < 0) { l1 = 0; l2 = 0; wl = 0; }
-  else if (l1 >= f.vn2 - 1) { l1 = f.vn2 - 1; l2 = l1; wl = 0; }
+  else if (l1 >= v.n2 - 1) { l1 = v.n2 - 1; l2 = l1; wl = 0; }
   else { l2 = l1 + 1; wl = (float)(lF - l1); }
   const int rn = f.tn1r, n2 = f.tn2;
   const float v1 = tv(t, rn, n2, j1, k1, l1), v2 = tv(t, rn, n2, j1, k1, l2);
@@ -92,8 +95,27 @@ __global__ void __launch_bounds__(256) k_facies(FaciesArgs f) {
       float value[kMaxFacies];
       float sum = f.undef_sum;
       for (int l = 0; l < f.nfac; ++l) {
-        const float v = (float)find_density(f, f.tables + l * f.tsize, a, b, r);
-        const float vs = 0.0f + v;
+        float vs;
+        if (f.ndens == 1) {
+          const float v = (float)find_density(f, f.v0, f.tables + l * f.tsize, a, b, r);
+          vs = 0.0f + v;
+        } else {
+          // FindDensityFromPosteriorPDF (src/faciesprob.cpp:1293-1312): the
+          // density of set d weighted by t_j or 1 - t_j for bit j of d
+          float tt[kMaxDensAngles];
+          for (int g = 0; g < f.nang; ++g) tt[g] = f.tgrid[((long long)g * f.ny + j) * f.nx + k];
+          vs = 0.0f;
+          for (int d = 0; d < f.ndens; ++d) {
+            float v = (float)find_density(f, f.vols[d], f.tables + ((long long)d * f.nfac + l) * f.tsize, a, b, r);
+            int factor = 1;
+            for (int g = 0; g < f.nang; ++g) {
+              if (g > 0) factor *= 2;
+              if ((d & factor) > 0) v *= tt[g];
+              else v *= (1 - tt[g]);
+            }
+            vs += v;
+          }
+        }
         const float dens = vs > 0.0f ? vs : 0.0f;
         value[l] = f.prior[l] * dens;
         sum = sum + value[l];
@@ -139,8 +161,18 @@ int launch_facies(const FaciesCall &c, hipStream_t s) {
   f.rnxp = 2 * (c.nxp / 2 + 1);
   f.nyp = c.nyp;
   f.smallrnxp = 2 * (c.nx / 2 + 1);
-  f.vn0 = c.vn[0]; f.vn1 = c.vn[1]; f.vn2 = c.vn[2];
-  f.x0 = c.vol[0]; f.y0 = c.vol[1]; f.dx = c.vol[2]; f.dy = c.vol[3]; f.ztop = c.vol[4]; f.zbot = c.vol[5];
+  f.v0.n0 = c.vn[0]; f.v0.n1 = c.vn[1]; f.v0.n2 = c.vn[2];
+  f.v0.x0 = c.vol[0]; f.v0.y0 = c.vol[1]; f.v0.dx = c.vol[2]; f.v0.dy = c.vol[3];
+  f.v0.ztop = c.vol[4]; f.v0.zbot = c.vol[5];
+  if (c.ndens < 1 || c.ndens != (1 << c.nang) || c.nang > kMaxDensAngles ||
+      (c.ndens > 1 && (!c.vols || !c.tgrid))) {
+    set_error("facies densities: ndens must be 2^nang (nang <= 8) with volumes and angle weights");
+    return CRAVA_HIP_E_INVALID;
+  }
+  f.ndens = c.ndens;
+  f.nang = c.nang;
+  f.vols = c.vols;
+  f.tgrid = c.tgrid;
   for (int l = 0; l < c.nfac; ++l) f.prior[l] = c.prior[l];
   // float / int as the reference (src/faciesprob.cpp:2183-2185)
   f.undef_sum = c.p_undefined / (c.vn[0] * c.vn[1] * c.vn[2]);

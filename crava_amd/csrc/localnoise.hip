@@ -116,7 +116,7 @@ __host__ __device__ inline void matrix_function(double (&m)[N][N], double thr) {
 // the Cholesky factor of m itself (the symmetrised m); on failure m is
 // clobbered.
 template <int N>
-__device__ inline bool cholesky_above(double (&m)[N][N], double thr) {
+__host__ __device__ inline bool cholesky_above(double (&m)[N][N], double thr) {
   double t[N][N];
 #pragma unroll
   for (int r = 0; r < N; ++r)
@@ -155,17 +155,12 @@ __device__ inline bool cholesky_above(double (&m)[N][N], double thr) {
   return ok;
 }
 
-// One thread per trace: sigmanew = sqrt(deltanew) (Sm - Smd)^(-1/2) and the
-// realisation factor sqrt(Sm - deltanew) Smd^(-1/2), with
-// deltanew = Sm G^T (G Sm G^T + D E D)^+ G Sm and D = diag(sqrt(scale)).
+// The per-trace matrices for the angle factors d = sqrt(scale):
+// sigmanew = sqrt(deltanew) (Sm - Smd)^(-1/2) and the realisation factor
+// sqrt(Sm - deltanew) Smd^(-1/2), with deltanew = Sm G^T (G Sm G^T + D E D)^+ G Sm
+// and D = diag(d).
 template <int NT>
-__global__ __launch_bounds__(64) void k_lnc_trace(const LncConst c, const double *__restrict__ scales, int ntr,
-                                                  double *__restrict__ sigmanew, double *__restrict__ trace_cov) {
-  const int t = blockIdx.x * 64 + threadIdx.x;
-  if (t >= ntr) return;
-  double d[NT];
-#pragma unroll
-  for (int a = 0; a < NT; ++a) d[a] = sqrt(scales[(size_t)a * ntr + t] / c.min_scale[a]);
+__host__ __device__ inline void lnc_matrices(const LncConst &c, const double (&d)[NT], double *sn, double *tc) {
   double gs[NT][3];  // G Sm
 #pragma unroll
   for (int r = 0; r < NT; ++r)
@@ -230,14 +225,12 @@ __global__ __launch_bounds__(64) void k_lnc_trace(const LncConst c, const double
 #pragma unroll
     for (int q = 0; q < 3; ++q) pc[p][q] = c.sm[p][q] - dn[p][q];
   matrix_function<3, kSqrt>(dn, 0.0);
-  double *sn = sigmanew + (size_t)t * 9;
 #pragma unroll
   for (int p = 0; p < 3; ++p)
 #pragma unroll
     for (int q = 0; q < 3; ++q) sn[p * 3 + q] = dn[p][0] * c.sdi[0][q] + dn[p][1] * c.sdi[1][q] + dn[p][2] * c.sdi[2][q];
-  if (trace_cov) {
+  if (tc) {
     matrix_function<3, kSqrt>(pc, 0.0);
-    double *tc = trace_cov + (size_t)t * 9;
     // lib_matr_prod order (libs/lib/lib_matr.c:1218-1224)
 #pragma unroll
     for (int p = 0; p < 3; ++p)
@@ -250,6 +243,18 @@ __global__ __launch_bounds__(64) void k_lnc_trace(const LncConst c, const double
         tc[p * 3 + q] = x;
       }
   }
+}
+
+// One thread per trace, d = sqrt(scale / the angle's smallest scale).
+template <int NT>
+__global__ __launch_bounds__(64) void k_lnc_trace(const LncConst c, const double *__restrict__ scales, int ntr,
+                                                  double *__restrict__ sigmanew, double *__restrict__ trace_cov) {
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= ntr) return;
+  double d[NT];
+#pragma unroll
+  for (int a = 0; a < NT; ++a) d[a] = sqrt(scales[(size_t)a * ntr + t] / c.min_scale[a]);
+  lnc_matrices<NT>(c, d, sigmanew + (size_t)t * 9, trace_cov ? trace_cov + (size_t)t * 9 : nullptr);
 }
 
 constexpr int kLncThreads = 256;
@@ -447,6 +452,48 @@ int lnc_compute_g(int ntheta, const double *E, const double *Sm, const double *S
     CRAVA_LNC_G(7)
     CRAVA_LNC_G(8)
 #undef CRAVA_LNC_G
+    default:
+      return CRAVA_HIP_E_INVALID;
+  }
+  return CRAVA_HIP_OK;
+}
+
+void lnc_const_host(int ntheta, const double *G, const double *E, const double *Sm, const double *Smd, LncConst &c) {
+  c = LncConst{};
+  for (int r = 0; r < ntheta; ++r) {
+    for (int q = 0; q < 3; ++q) c.g[r][q] = G[r * 3 + q];
+    for (int q = 0; q < ntheta; ++q) c.e[r][q] = E[r * ntheta + q];
+  }
+  double sd[3][3], smd[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q) {
+      c.sm[r][q] = Sm[r * 3 + q];
+      smd[r][q] = Smd[r * 3 + q];
+      sd[r][q] = c.sm[r][q] - smd[r][q];
+    }
+  lnc_inv_sqrt_host(sd, 0.0000001, c.sdi);  // src/crava.cpp:2294-2307
+  lnc_inv_sqrt_host(smd, 0.0, c.old);       // :2376-2385
+  for (int a = 0; a < kLncMaxTheta; ++a) c.min_scale[a] = 1.0;
+}
+
+int lnc_sigmanew_host(const LncConst &c, int ntheta, const double *scale, double *sigmanew) {
+  switch (ntheta) {
+#define CRAVA_LNC_H(N)                                         \
+  case N: {                                                    \
+    double d[N];                                               \
+    for (int a = 0; a < N; ++a) d[a] = std::sqrt(scale[a]);    \
+    lnc_matrices<N>(c, d, sigmanew, nullptr);                  \
+    break;                                                     \
+  }
+    CRAVA_LNC_H(1)
+    CRAVA_LNC_H(2)
+    CRAVA_LNC_H(3)
+    CRAVA_LNC_H(4)
+    CRAVA_LNC_H(5)
+    CRAVA_LNC_H(6)
+    CRAVA_LNC_H(7)
+    CRAVA_LNC_H(8)
+#undef CRAVA_LNC_H
     default:
       return CRAVA_HIP_E_INVALID;
   }

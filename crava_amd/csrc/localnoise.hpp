@@ -28,6 +28,13 @@ void lnc_inv_sqrt_host(const double in[3][3], double thr, double out[3][3]);
 // all row-major.
 int lnc_compute_g(int ntheta, const double *E, const double *Sm, const double *Smd, double *G);
 
+// LncConst from G [ntheta][3], errThetaCov [ntheta][ntheta], priorVar0 and
+// postVar0 (row-major); min_scale set to 1.
+void lnc_const_host(int ntheta, const double *G, const double *E, const double *Sm, const double *Smd, LncConst &c);
+// Crava::newPosteriorCovPointwise's sigmanew (row-major 3x3) for one scale
+// vector [ntheta] (D = diag(sqrt(scale))), on the host.
+int lnc_sigmanew_host(const LncConst &c, int ntheta, const double *scale, double *sigmanew);
+
 // scales: device [ntheta][ny][nx]; sigmanew, trace_cov: device [ny][nx][9]
 // (trace_cov may be nullptr).
 int launch_lnc_trace(const LncConst &c, int ntheta, const double *scales, int nx, int ny, double *sigmanew,

@@ -367,6 +367,67 @@ int crava_hip_facies(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int ny
                      const double *volume, const int *vn, const float *prior, float p_undefined, float *const *prob,
                      int *argmax, unsigned long long *counts);
 
+/* crava_hip_facies with the local-noise density dimension
+ * (src/faciesprob.cpp:2160-2265, FindDensityFromPosteriorPDF :1277-1317):
+ * 2^nang density sets, set d weighted per voxel by prod_j (bit j of d ?
+ * t_j : 1 - t_j), t_j = (s_j - min s_j) / (max s_j - min s_j) (0 where s_j
+ * is constant) with s_j the angle's noise-scale table at the voxel's trace.
+ *   nang          0..8; nang = 0 is crava_hip_facies
+ *   tables        device, (2^nang) x nfac tables, set d facies l at index d*nfac + l
+ *   volumes, vn   host [2^nang][6] {x0, y0, dx, dy, ztop, zbot} and [2^nang][3]
+ *                 {nx, ny, nz}: the lattice of each set (undefSum from set 0's)
+ *   noise_scale   host, nang tables [ny][nx] (i fastest; RMISSING skipped in
+ *                 the minimum and maximum)
+ * The reference sizes its weight vectors before it counts the angles
+ * (:2153-2157, a write past the end of empty vectors); this follows the
+ * evident intent, one weight per angle.  Synchronises. */
+int crava_hip_facies_local_noise(crava_hip_ctx *ctx, int nx, int ny, int nz, int nxp, int nyp, int nzp,
+                                 const float *alpha, const float *beta, const float *rho, int nfac, int nang,
+                                 const float *tables, int tn1, int tn2, int tn3, const double *volumes,
+                                 const int *vn, const double *const *noise_scale, const float *prior,
+                                 float p_undefined, float *const *prob, int *argmax, unsigned long long *counts);
+
+/* A facies density table from samples, in place of the PosteriorElasticPDF3D
+ * constructor (src/posteriorelasticpdf3d.cpp:12-138, SetupSmoothingGaussian3D
+ * :360-413): the histogram of the npts points (d1, d2, d3: host double) on
+ * the n1 x n2 x n3 lattice over limits {min1, max1, min2, max2, min3, max3}
+ * (each point 1/npts; points outside dropped) circularly convolved with the
+ * normalised Gaussian exp(-x^T sigma^-1 x / 2) (sigma: host, row-major 3x3)
+ * at the wrapped lattice lags, by the float 3-D transforms with PARAMETER
+ * scaling and a multiplication by sqrt(n1 n2 n3).  table: device float grid
+ * [n3][n2][2(n1/2+1)] (FFTGrid(n1, n2, n3)), overwritten.  Uses the context
+ * workspace; synchronises.  The reference's tables are 100 x 100 x 50
+ * (100 x 1 x 50 without Vs). */
+int crava_hip_kde_table(crava_hip_ctx *ctx, long long npts, const double *d1, const double *d2, const double *d3,
+                        const double *sigma, int n1, int n2, int n3, const double *limits, void *table);
+
+/* The density sets of FaciesProb::MakePosteriorElasticPDF3D
+ * (src/faciesprob.cpp:926-1135) from well samples: for set d the samples
+ * (for d > 0 transformed by sample_transform[d], row-major 3x3, e.g.
+ * crava_hip_local_noise_sigmanew at the set's corner scales; NULL: none),
+ * CalculateVariances, hopt = (4/7)^(1/7) nobs^(-1/7), the smoothing
+ * covariance sigma_e[d] (row-major 3x3, upper triangle read; with no_vs its
+ * alpha/rho 2x2 block in the top-left 2x2; NULL: zero) with each diagonal
+ * raised to at least (hopt sd)^2, the lattice min - 5 sd .. max + 5 sd over
+ * all samples, and one crava_hip_kde_table per facies (samples whose
+ * facies_log is l; IMISSING = -99999 excluded).  Outputs: tables[d*nfac + l]
+ * (device grids as crava_hip_kde_table), volumes [ndens][6] and vn
+ * [ndens][3] for crava_hip_facies_local_noise. */
+int crava_hip_facies_densities(crava_hip_ctx *ctx, int nsamples, const float *alpha, const float *beta,
+                               const float *rho, const int *facies_log, int nfac, int ndens,
+                               const double *sample_transform, const double *sigma_e, int no_vs, int n1, int n2,
+                               int n3, void *const *tables, double *volumes, int *vn);
+
+/* Crava::newPosteriorCovPointwise's sigmanew for one vector of noise scales
+ * (src/crava.cpp:2214-2330), host side: sqrt(deltanew) (Sm - Smd)^(-1/2),
+ * deltanew = Sm G^T (G Sm G^T + D E D)^+ G Sm, D = diag(sqrt(scale)).  G
+ * [ntheta][3], errThetaCov [ntheta][ntheta], priorVar0 / postVar0 and
+ * sigmanew 3x3, all row-major.  MakePosteriorElasticPDF3D's sample
+ * transform of set d uses scale_j = max s_j / min s_j for bit j of d set,
+ * else 1. */
+int crava_hip_local_noise_sigmanew(int ntheta, const double *G, const double *errThetaCov, const double *priorVar0,
+                                   const double *postVar0, const double *scale, double *sigmanew);
+
 /* Prior correlation of the parameter covariance grids, in place of
  * FFTGrid::fillInParamCorr (src/fftgrid.cpp:1119-1164) followed by
  * multiplyByScalar(Sigma0_ab) as in

@@ -67,6 +67,7 @@ def orc():
         L.orc_facies_geomodel.argtypes = [I, I, I, I, VP, PP]
         L.orc_facies_find_density.argtypes = [D, D, D, VP, I, I, VP, VP]
         L.orc_facies_find_density.restype = C.c_double
+        L.orc_facies_densdim.argtypes = [I, I, I, I, I, I, VP, VP, VP, I, I, VP, I, I, I, VP, VP, VP, VP, F, PP]
         L.orc_perk_tables_nonconst.argtypes = [I, I, I, D, F, F, F, VP, VP, VP, VP, VP, VP, VP]
         L.orc_wavelet_resample.argtypes = [VP, I, I, F, F, I, VP, VP]
         L.orc_wavelet_energy.argtypes = [I, I, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]
@@ -342,6 +343,26 @@ def facies(nx, ny, nz, nxp, nyp, nzp, alpha, beta, rho, tables, tn, vol, vn, pri
     orc().orc_facies(nx, ny, nz, nxp, nyp, nzp, alpha.ctypes.data, beta.ctypes.data, rho.ctypes.data, nfac,
                      tab.ctypes.data, tn[0], tn[1], tn[2], v.ctypes.data, n.ctypes.data, pr.ctypes.data,
                      p_undef, _ptrs(out))
+    return out
+
+
+def facies_densdim(nx, ny, nz, nxp, nyp, nzp, alpha, beta, rho, tables, tn, vols, vns, noise, prior, p_undef):
+    """Facies probabilities with 2^nang density sets weighted by the angle
+    weights of the noise-scale tables noise [nang][ny][nx] (float64);
+    tables [ndens][nfac][tn3][tn2][rn1], vols [ndens][6], vns [ndens][3]."""
+    nfac = len(prior)
+    noise = np.ascontiguousarray(noise, dtype=np.float64)
+    nang = 0 if noise.size == 0 else noise.shape[0]
+    sm = 2 * (nx // 2 + 1)
+    out = [np.zeros((nz, ny, sm), dtype=np.float32) for _ in range(nfac + 1)]
+    tab = np.ascontiguousarray(tables, dtype=np.float32)
+    v = np.ascontiguousarray(vols, dtype=np.float64)
+    n = np.ascontiguousarray(vns, dtype=np.int32)
+    pr = np.ascontiguousarray(prior, dtype=np.float32)
+    nz_ = noise if nang else np.zeros(1)
+    orc().orc_facies_densdim(nx, ny, nz, nxp, nyp, nzp, alpha.ctypes.data, beta.ctypes.data, rho.ctypes.data, nfac,
+                             nang, tab.ctypes.data, tn[0], tn[1], tn[2], v.ctypes.data, n.ctypes.data,
+                             nz_.ctypes.data, pr.ctypes.data, p_undef, _ptrs(out))
     return out
 
 

@@ -1202,6 +1202,78 @@ void orc_facies(int nx, int ny, int nz, int nxp, int nyp, int nzp,
       }
 }
 
+/* CalculateFaciesProbFromPosteriorElasticPDF with the local-noise density
+ * dimension (src/faciesprob.cpp:2153-2265, FindDensityFromPosteriorPDF
+ * :1277-1317): ndens = 2^nang density sets (tables [ndens][nfac], lattices
+ * vols [ndens][6] / vns [ndens][3]); the angle weights from the noise-scale
+ * tables noise [nang][ny*nx] (min / max skipping RMISSING, 0 when constant),
+ * with one weight per angle (the reference's own vectors are sized before the
+ * angles are counted). */
+void orc_facies_densdim(int nx, int ny, int nz, int nxp, int nyp, int nzp,
+                        const float *alpha, const float *beta, const float *rho,
+                        int nfac, int nang, const float *tables, int tn1, int tn2, int tn3,
+                        const double *vols, const int *vns, const double *noise,
+                        const float *prior_facies, float p_undefined, float **prob)
+{
+  int ndens = 1 << nang;
+  int rnxp = 2 * (nxp / 2 + 1);
+  int smallrnxp = 2 * (nx / 2 + 1);
+  long long tsize = (long long)2 * (tn1 / 2 + 1) * tn2 * tn3;
+  long long ntr = (long long)nx * ny;
+  float undefSum = p_undefined / (vns[0] * vns[1] * vns[2]);
+  double *tg = (double *)malloc(sizeof(double) * (nang > 0 ? nang : 1) * ntr);
+  for (int g = 0; g < nang; g++) {
+    const double *tab = noise + g * ntr;
+    double mn = tab[0], mx = tab[0];
+    for (long long t = 0; t < ntr; t++) {            /* Grid2D::FindMin / FindMax(RMISSING) */
+      if ((mn == -99999.0 || tab[t] < mn) && tab[t] != -99999.0) mn = tab[t];
+      if ((mx == -99999.0 || tab[t] > mx) && tab[t] != -99999.0) mx = tab[t];
+    }
+    for (long long t = 0; t < ntr; t++) tg[g * ntr + t] = (mn == mx) ? 0.0 : (tab[t] - mn) / (mx - mn);
+  }
+  float value[64], t[8], dv[256];
+  long long in = 0, out = 0;
+  for (int i = 0; i < nzp; i++)
+    for (int j = 0; j < nyp; j++)
+      for (int k = 0; k < rnxp; k++, in++) {
+        float a = alpha[in], b = beta[in], r = rho[in];
+        if (k < smallrnxp && j < ny && i < nz) {
+          float sum = undefSum;
+          for (int l = 0; l < nfac; l++) {
+            float dens;
+            if (k < nx) {
+              for (int g = 0; g < nang; g++) t[g] = (float)tg[g * ntr + (long long)j * nx + k];
+              for (int d = 0; d < ndens; d++)
+                dv[d] = (float)o_find_density(a, b, r, tables + ((long long)d * nfac + l) * tsize, tn1, tn2,
+                                              vols + 6 * d, vns + 3 * d);
+              float valuesum = 0;
+              for (int d = 0; d < ndens; d++) {
+                int factor = 1;
+                for (int g = 0; g < nang; g++) {
+                  if (g > 0) factor *= 2;
+                  if ((d & factor) > 0) dv[d] *= t[g];
+                  else dv[d] *= (1 - t[g]);
+                }
+                valuesum += dv[d];
+              }
+              dens = valuesum > 0.0f ? valuesum : 0.0f;
+            } else {
+              dens = 1.0;
+            }
+            value[l] = prior_facies[l] * dens;
+            sum = sum + value[l];
+          }
+          for (int l = 0; l < nfac; l++) {
+            float help = value[l] / sum;
+            prob[l][out] = (k < nx) ? help : -99999.0f;
+          }
+          prob[nfac][out] = (k < nx) ? undefSum / sum : -99999.0f;
+          out++;
+        }
+      }
+  free(tg);
+}
+
 /* ========================================================================
  * Grid fills and operators (the callers either side of the transforms).
  * ====================================================================== */
