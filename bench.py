@@ -388,11 +388,12 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
     vols, src, posts = [], [], []
     P = P_in = 0
     nts = []
+    fwd_bytes = inv_bytes = post_bytes = 0
     for i in range(volumes):
         prob = synth.make_problem(synth.random_config(rng, i, seed=seed), "cuda")
         c = prob.cfg
         tabs = problem_tables(prob)
-        posts.append(Posterior(ctx, synth.post_params(prob, 0.1), tabs))
+        posts.append(Posterior(ctx, synth.post_params(prob, 1e-3), tabs))
         pristine = prob.mu + prob.data + prob.cov + [prob.errcorr, prob.corr]
         work = [g.clone() for g in pristine]
         nt = prob.ntheta
@@ -401,8 +402,16 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
         # separable prior (initializeCorrelations): one correlation transform per volume
         vols.append((posts[-1], work[:3], work[3:3 + nt], work[3 + nt:9 + nt], work[9 + nt], work[10 + nt]))
         cn = c.nxp // 2 + 1
-        P += cn * c.nyp * c.nzp
-        P_in += int(tabs["invert"].sum()) * cn * c.nyp
+        pts = cn * c.nyp * c.nzp
+        n_in = int(tabs["invert"].sum()) * cn * c.nyp
+        P += pts
+        P_in += n_in
+        # bytes of the three passes of one grid's transform: x reads the real
+        # grid and writes the spectrum, y and z read and write the spectrum
+        passes = 4 * c.nxp * c.nyp * c.nzp + 8 * pts + 4 * 8 * pts
+        fwd_bytes += (nt + 5) * passes        # data, error correlation, means; the correlation grid
+        inv_bytes += (nt + 9) * passes        # residuals, covariances; means
+        post_bytes += n_in * 8 * (14 + 2 * nt) + (pts - n_in) * 56
         del prob
     torch.cuda.synchronize()
     log(f"batching: {volumes} volumes generated in {time.time()-t0:.1f}s, {P} Fourier points")
@@ -435,6 +444,11 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
     ms_graph = timed(True)
     ms_eager = timed(False)
     nf, n64 = b.counts()
+    phases = []
+    for _ in range(reps):
+        reload()
+        phases.append(b.phase_times())
+    ph = [float(np.median([p_[i] for p_ in phases])) for i in range(3)]
     b.close()
     for p in posts:
         p.close()
@@ -446,7 +460,22 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
                                 "form": ["one persistent launch", "one persistent launch per ntheta class",
                                          "one launch per volume"][form],
                                 "ms": post_ms, "pts_per_s": P / (post_ms * 1e-3)},
-           "failed_factorisations": int(sum(nf)), "fp64_points": int(sum(n64))}
+           "failed_factorisations": int(sum(nf)), "fp64_points": int(sum(n64)), "threshold": 1e-3}
+
+    def roof(ms, nbytes, what):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"what": what, "ms": ms, "bytes": nbytes, "GBps": gbs, "frac": gbs / PEAK_HBM_GBS}
+
+    # one eager pass timed per phase (HIP events on the launch stream);
+    # bytes: each transform pass reads and writes its grids, the posterior
+    # its per-point inputs and outputs
+    out["phase_rooflines"] = {
+        "forward_transforms": roof(ph[0], fwd_bytes, "r2c of (ntheta + 5) grids per volume, 3 passes each"),
+        "posterior": roof(ph[1], post_bytes, "separable MIXED posterior: 8 (14 + 2 ntheta) B in band, 56 B out"),
+        "inverse_transforms": roof(ph[2], inv_bytes, "c2r of (ntheta + 9) grids per volume, 3 passes each"),
+    }
+    for k, v in out["phase_rooflines"].items():
+        log(f"batching phase {k}: {v['ms']:.2f} ms, {v['GBps']:.0f} GB/s ({v['frac']:.2f} of peak)")
     log(f"batching: graph {ms_graph:.2f} ms, eager {ms_eager:.2f} ms per batch, {out['value']:.4g} pts/s")
     return out
 

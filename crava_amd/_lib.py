@@ -150,6 +150,7 @@ SIGNATURES = {
     "crava_hip_batch_set_launch": (I, [VP, I, I, I]),
     "crava_hip_batch_run": (I, [VP, I]),
     "crava_hip_batch_counts": (I, [VP, C.POINTER(LL), C.POINTER(LL)]),
+    "crava_hip_batch_phase_times": (I, [VP, C.POINTER(C.c_float)]),
     "crava_hip_batch_destroy": (I, [VP]),
     "crava_hip_autotune_fft": (I, [VP, VP, I, I, I, I, C.POINTER(I), C.POINTER(C.c_float)]),
     "crava_hip_fft_set_variants": (I, [VP, I, I, I, I, C.POINTER(I)]),

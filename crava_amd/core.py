@@ -413,6 +413,13 @@ class Batch:
         check(lib().crava_hip_batch_counts(self.h, nf, n64), "crava_hip_batch_counts")
         return list(nf), list(n64)
 
+    def phase_times(self):
+        """One eager pass timed per phase: (forward transforms, posterior,
+        inverse transforms) in ms."""
+        ms = (C.c_float * 3)()
+        check(lib().crava_hip_batch_phase_times(self.h, ms), "crava_hip_batch_phase_times")
+        return tuple(ms)
+
 
 def synthetic_seismic(ctx: Context, nx, ny, nz, nxp, nyp, nzp, A, wavelets, dz, alpha, beta, rho, out,
                       shift=None, gain=None, stretch=None):

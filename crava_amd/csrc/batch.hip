@@ -200,13 +200,14 @@ int BatchExec::join(hipStream_t origin) {
   return CRAVA_HIP_OK;
 }
 
-int BatchExec::enqueue(hipStream_t origin, bool transforms_only, bool forward_only) {
+int BatchExec::enqueue(hipStream_t origin, bool transforms_only, bool forward_only, hipEvent_t *marks) {
   int rc;
   const int n = (int)vols_.size();
   if (!transforms_only) {
     hipLaunchKernelGGL(k_zero_u64, dim3((2 * n + 255) / 256), dim3(256), 0, origin, counters_, 2 * n);
     CRAVA_HIP_CHECK(hipGetLastError());
   }
+  if (marks) CRAVA_HIP_CHECK(hipEventRecord(marks[0], origin));
   if ((rc = fork(origin))) return rc;
   for (int v = 0; v < n; ++v)
     for (int i = 0; i < 2; ++i) {
@@ -216,7 +217,9 @@ int BatchExec::enqueue(hipStream_t origin, bool transforms_only, bool forward_on
     }
   if ((rc = join(origin))) return rc;
   if (forward_only) return CRAVA_HIP_OK;
+  if (marks) CRAVA_HIP_CHECK(hipEventRecord(marks[1], origin));
   if (!transforms_only && (rc = launch_post(origin))) return rc;
+  if (marks) CRAVA_HIP_CHECK(hipEventRecord(marks[2], origin));
   if ((rc = fork(origin))) return rc;
   for (int v = 0; v < n; ++v)
     for (int i = 0; i < 2; ++i) {
@@ -224,7 +227,24 @@ int BatchExec::enqueue(hipStream_t origin, bool transforms_only, bool forward_on
       if ((rc = st.plan->c2r(st.grids, st.ng, st.scale, side_[work_[v].stream[i]], st.dc, st.dcmask, st.exact)))
         return rc;
     }
-  return join(origin);
+  if ((rc = join(origin))) return rc;
+  if (marks) CRAVA_HIP_CHECK(hipEventRecord(marks[3], origin));
+  return CRAVA_HIP_OK;
+}
+
+int BatchExec::phase_times(hipStream_t s, float *ms) {
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int rc = CRAVA_HIP_OK;
+  for (int i = 0; i < 4 && rc == CRAVA_HIP_OK; ++i)
+    if (hipEventCreate(&ev[i]) != hipSuccess) rc = CRAVA_HIP_E_HIP;
+  if (rc == CRAVA_HIP_OK) rc = enqueue(s, false, false, ev);
+  if (rc == CRAVA_HIP_OK && hipEventSynchronize(ev[3]) != hipSuccess) rc = CRAVA_HIP_E_HIP;
+  for (int i = 0; i < 3 && rc == CRAVA_HIP_OK; ++i)
+    if (hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]) != hipSuccess) rc = CRAVA_HIP_E_HIP;
+  for (int i = 0; i < 4; ++i)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  if (rc == CRAVA_HIP_E_HIP) set_error("batch phase timing: event failure");
+  return rc;
 }
 
 int BatchExec::launch_post(hipStream_t origin) {

@@ -36,6 +36,9 @@ class BatchExec {
   int set_launch(int ppb, int blocks, int form);
   // Per-volume counters of the last run (synchronises `s`).
   int counts(hipStream_t s, long long *n_fail, long long *n_fp64);
+  // One eager pass with events between its phases: ms[0] forward
+  // transforms, ms[1] posterior, ms[2] inverse transforms (synchronises).
+  int phase_times(hipStream_t s, float *ms);
   int nvol() const { return (int)vols_.size(); }
 
  private:
@@ -54,7 +57,7 @@ class BatchExec {
     int stream[2] = {0, 0};
   };
   int build_sets();
-  int enqueue(hipStream_t origin, bool transforms_only, bool forward_only);
+  int enqueue(hipStream_t origin, bool transforms_only, bool forward_only, hipEvent_t *marks = nullptr);
   int launch_post(hipStream_t origin);
   int fork(hipStream_t origin);
   int join(hipStream_t origin);

@@ -1363,6 +1363,11 @@ int crava_hip_batch_run(crava_hip_batch *batch, int use_graph) {
   return batch->exec.run(batch->ctx->stream, use_graph != 0);
 }
 
+int crava_hip_batch_phase_times(crava_hip_batch *batch, float *ms) {
+  if (!batch || !ms) { set_error("NULL argument to crava_hip_batch_phase_times"); return CRAVA_HIP_E_INVALID; }
+  return batch->exec.phase_times(batch->ctx->stream, ms);
+}
+
 int crava_hip_batch_counts(crava_hip_batch *batch, long long *n_fail, long long *n_fp64) {
   if (!batch) { set_error("batch is NULL"); return CRAVA_HIP_E_INVALID; }
   return batch->exec.counts(batch->ctx->stream, n_fail, n_fp64);

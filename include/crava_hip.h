@@ -529,6 +529,13 @@ int crava_hip_batch_set_launch(crava_hip_batch *batch, int points_per_block, int
  * use (and after autotune / set_launch) and replays it. */
 int crava_hip_batch_run(crava_hip_batch *batch, int use_graph);
 
+/* One eager pass of the batch with events between its phases, for the
+ * per-phase rooflines: ms[0] the forward transforms of every volume, ms[1]
+ * the posterior, ms[2] the inverse transforms.  The grids are transformed
+ * and solved as by crava_hip_batch_run; synchronises.  No counterpart in
+ * the reference (one volume per Crava there). */
+int crava_hip_batch_phase_times(crava_hip_batch *batch, float *ms);
+
 /* Per-volume counters of the last run (host arrays [nvol], may be NULL):
  * failed factorisations and points solved in FP64.  Synchronises. */
 int crava_hip_batch_counts(crava_hip_batch *batch, long long *n_fail, long long *n_fp64);
