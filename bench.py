@@ -77,20 +77,33 @@ def kernel_identity(key, variants, nxp, nyp, nzp, ngrid, nt):
 
 def profiled_traffic(name, grid):
     """HBM bytes per launch from the committed counter summaries (TRAFFIC lines
-    of profiles/hbm_traffic_*.txt), or None."""
+    of profiles/hbm_traffic_*.txt), or None.  A launch shape whose other I/O
+    form (LDS-staged or direct first/last pass, the same grid accesses) was
+    the one profiled falls back to that form's figure."""
     import glob
-    best = None
-    for f in glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.txt")):
-        for line in open(f):
-            if not line.startswith("TRAFFIC "):
-                continue
-            body = line[len("TRAFFIC "):].rsplit(" ", 2)
-            if len(body) != 3 or body[0] != name:
-                continue
-            g = int(body[1].split("=")[1])
-            if grid is None or g == grid:
-                best = float(body[2])
-    return best
+
+    def lookup(nm):
+        best = None
+        for f in glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.txt")):
+            for line in open(f):
+                if not line.startswith("TRAFFIC "):
+                    continue
+                body = line[len("TRAFFIC "):].rsplit(" ", 2)
+                if len(body) != 3 or body[0] != nm:
+                    continue
+                g = int(body[1].split("=")[1])
+                if grid is None or g == grid:
+                    best = float(body[2])
+        return best
+
+    found = lookup(name)
+    if found is None and ("k_sfft_strided<" in name or "k_srows_" in name):
+        head, args = name.split("<", 1)
+        a = [x.strip() for x in args.rstrip(">").split(",")]
+        i = 4 if "k_sfft_strided<" in name else 3     # the direct-I/O flag
+        a[i] = "false" if a[i] == "true" else "true"
+        found = lookup(head + "<" + ", ".join(a) + ">")
+    return found
 
 
 def log(msg):
