@@ -45,10 +45,12 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 
-# Launch geometry of the specialised transform kernels' variants
-# (pencils or row pairs per workgroup, threads), as in fft_static.hip.
-# (pencils or row pairs per workgroup, threads, direct global I/O in the first/last pass)
-STRIDED_VARIANTS = [(tx, th, dio) for dio in ("false", "true") for tx, th in ((16, 256), (16, 512), (8, 256), (32, 512))]
+# Launch geometry of the specialised transform kernels' variants, as in
+# fft_static.hip: (pencils or row pairs per workgroup, threads, form) with
+# form "false" (LDS-staged), "true" (direct global I/O in the first and last
+# pass) or "pipe" (direct I/O in persistent workgroups, k_sfft_pipe).
+STRIDED_VARIANTS = ([(tx, th, dio) for dio in ("false", "true") for tx, th in ((16, 256), (16, 512), (8, 256), (32, 512))]
+                    + [(16, 256, "pipe"), (16, 512, "pipe")])
 ROW_VARIANTS = [(tx, th, dio) for dio in ("false", "true") for tx, th in ((8, 256), (16, 512), (4, 256))]
 
 
@@ -69,6 +71,9 @@ def kernel_identity(key, variants, nxp, nyp, nzp, ngrid, nt):
         if v < 0:
             return "crava::k_fft_strided<-1, float, float, float>", None
         tx, th, dio = STRIDED_VARIANTS[v]
+        if dio == "pipe":
+            # persistent: the grid follows the device's occupancy
+            return f"crava::k_sfft_pipe<{n}, {tx}, {th}, -1, float, float, float>", None
         return (f"crava::k_sfft_strided<{n}, {tx}, {th}, -1, {dio}, float, float, float>",
                 -(-ncols // tx) * nbatch * ngrid * th)
     mode = {"posterior_mixed": ("float", 1), "posterior_f32": ("float", 0), "posterior_f64": ("double", 0)}[key]
@@ -128,7 +133,8 @@ def parse():
                     "workload (0: skip)")
     ap.add_argument("--diff-steps", type=int, default=50, help="L-BFGS iterations of the differentiation workload "
                     "(0: skip it)")
-    ap.add_argument("--only", choices=["nonconstant_thickness", "time_lapse_merge", "local_noise"],
+    ap.add_argument("--only", choices=["forward_modelling", "nonconstant_thickness", "time_lapse_merge", "local_noise",
+                                       "batching"],
                     help="run one secondary workload alone (for profiling) and print its result")
     return ap.parse_args()
 
@@ -842,7 +848,10 @@ def main():
     if args.only:
         ctx = Context()
         reps = max(args.forward_reps, 1)
-        fn = {"nonconstant_thickness": lambda: nonconstant_thickness(ctx, reps, not args.no_cpu),
+        fn = {"forward_modelling": lambda: forward_modelling(ctx, synth.make_problem(synth.config(args.config), "cuda"),
+                                                             reps, not args.no_cpu),
+              "nonconstant_thickness": lambda: nonconstant_thickness(ctx, reps, not args.no_cpu),
+              "batching": lambda: batching(ctx, args.batch_volumes, args.batch_reps),
               "time_lapse_merge": lambda: time_lapse_merge(ctx, reps, not args.no_cpu),
               "local_noise": lambda: local_noise(ctx, reps, not args.no_cpu)}[args.only]
         print(json.dumps({"workload": args.only, "result": fn()}))

@@ -169,6 +169,7 @@ SIGNATURES = {
     "crava_hip_facies_densities": (I, [VP, I, VP, VP, VP, VP, I, I, VP, VP, I, I, I, I, C.POINTER(VP), VP, VP]),
     "crava_hip_local_noise_sigmanew": (I, [I, VP, VP, VP, VP, VP, VP]),
     "crava_hip_padding_size": (I, [I, C.c_double, C.POINTER(I)]),
+    "crava_hip_circ_corr_t": (I, [VP, I, I, I, VP]),
     "crava_hip_estimate_xy_padding": (I, [I, I, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                           C.c_double, I, C.c_float, C.c_float, C.c_float, C.POINTER(I), C.POINTER(I),
                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]),

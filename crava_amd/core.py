@@ -190,6 +190,17 @@ def prior_var0_robustify(prior_var0):
     return out, bool(ch.value)
 
 
+def circ_corr_t(prior_corr_t, min_int_fq: int, nzp: int):
+    """SeismicParametersHolder::computeCircCorrT (crava_hip_circ_corr_t):
+    float32 array of nzp lags."""
+    import numpy as np
+    p = np.ascontiguousarray(prior_corr_t, dtype=np.float32)
+    out = np.empty(nzp, np.float32)
+    check(lib().crava_hip_circ_corr_t(p.ctypes.data, int(p.size), int(min_int_fq), int(nzp), out.ctypes.data),
+          "crava_hip_circ_corr_t")
+    return out
+
+
 def padding_size(n: int, pad_fac: float) -> int:
     """ModelGeneral::setPaddingSize (crava_hip_padding_size)."""
     out = C.c_int(0)

@@ -376,6 +376,23 @@ int crava_hip_estimate_z_padding(int nz, double lz, double min_rel_thick, double
   return CRAVA_HIP_OK;
 }
 
+int crava_hip_circ_corr_t(const float *prior_corr_t, int n, int min_int_fq, int nzp, float *out) {
+  if (!prior_corr_t || !out || n < 1 || nzp < 1 || nzp > (1 << 16)) {
+    set_error("invalid arguments to crava_hip_circ_corr_t");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (prior_corr_t[0] == 0.0f) {
+    set_error("crava_hip_circ_corr_t: priorCorrT[0] must be nonzero");
+    return CRAVA_HIP_E_INVALID;
+  }
+  if (!crava::circ_corr_t(prior_corr_t, n, min_int_fq, nzp, out)) {
+    set_error("crava_hip_circ_corr_t: the circular temporal correlation is undefined (c(0) <= 1e-5); increase the "
+              "number of layers");
+    return CRAVA_HIP_E_INVALID;
+  }
+  return CRAVA_HIP_OK;
+}
+
 int crava_hip_prior_var0_robustify(const double *prior_var0, double *out, int *changed) {
   if (!prior_var0 || !out) {
     set_error("invalid arguments to crava_hip_prior_var0_robustify");

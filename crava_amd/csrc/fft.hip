@@ -491,10 +491,13 @@ int Fft3D::init() {
         (rc = upload_static_twiddles<float>(nyp, &stw[1])) || (rc = upload_static_twiddles<float>(nzp, &stw[2])))
       return rc;
   }
-  // specialised kernels where a length has them (variant 1 until tuned)
+  // specialised kernels where a length has them (until tuned: variant 1,
+  // else the first of 0, 5, 4, 3 the length has)
   for (int axis = 0; axis < 3; ++axis) {
     var[axis] = -1;
-    if (num_variants(axis) > 1 && set_variant(axis, 1) != CRAVA_HIP_OK) (void)set_variant(axis, 0);
+    if (num_variants(axis) < 1) continue;
+    for (int v : {1, 0, 5, 4, 3})
+      if (set_variant(axis, v) == CRAVA_HIP_OK) break;
   }
   set_error("");
   return CRAVA_HIP_OK;
@@ -675,7 +678,8 @@ int Fft3D::set_variant(int axis, int v) {
 
 int Fft3D::num_variants(int axis) const {
   const int n = axis == 0 ? nxp : (axis == 1 ? nyp : nzp);
-  if (!static_fft_available(n) || (axis == 0 && n % 2)) return 0;
+  // odd x lengths: the direct-I/O row variants only (static_fft_geometry)
+  if (!static_fft_available(n)) return 0;
   return axis == 0 ? kNumRowVariants : kNumStridedVariants;
 }
 

@@ -9,7 +9,7 @@ namespace crava {
 // Kinds of pass.
 enum FftKind { kStridedFwd = 0, kStridedInv = 1, kRowsR2C = 2, kRowsC2R = 3 };
 
-constexpr int kNumStridedVariants = 8;
+constexpr int kNumStridedVariants = 10;
 constexpr int kNumRowVariants = 6;
 
 struct StaticLaunch {
@@ -18,8 +18,14 @@ struct StaticLaunch {
   size_t lds = 0;
 };
 
-// True if length n has specialised kernels.
+// True if length n has specialised kernels (some launch variants).
 bool static_fft_available(int n);
+// True if length n has specialised kernels in every launch variant (the
+// main lengths; the fused forward-modelling and wavelet-division kernels
+// exist for these).
+bool static_fft_full(int n);
+// True if launch variant `variant` of `kind` exists for length n.
+bool static_variant_available(int n, FftKind kind, int variant);
 // The per-pass twiddle table the specialised kernels of length n read
 // (2n doubles, {re, im} pairs); false if n is not specialised.
 bool static_twiddle_table(int n, double *re_im);

@@ -28,14 +28,19 @@
 //     output grids), the plan's z passes over all angles in one launch,
 //     k_fm_filter, the inverse z passes, k_fm_padding (exact impedance of the
 //     padding traces after the transforms).
-// Built with -ffp-contract=off: the impedance, taper and difference follow
-// the reference's float operations.
+// The impedance, taper and difference follow the reference's float
+// operations (contraction is off in this file's own code).
 #include "../../include/crava_hip.h"
 #include "fft_static.hpp"
 #include "forward.hpp"
 #include "stockham.hpp"
 
 #include <cmath>
+
+// The impedance, taper, difference and filter below keep the reference's
+// float operations (no contraction into FMA); the Stockham passes included
+// above are contracted.
+#pragma clang fp contract(off)
 
 namespace crava {
 
@@ -209,6 +214,10 @@ struct TraceParams {
 // Filter of one bin pair (k, N-k) of a complex column carrying traces pa, pb
 // (Re, Im): separate the two traces' spectra, filter each trace inside the
 // volume with its local wavelet (spectrum W), scale by 1/N and pack again.
+// Without local wavelets (LOCAL false: no stretch, shift or gain grid) the
+// local spectrum is W(k) itself: stretched() at sf = 1, shift 0, gain 1
+// multiplies by 1, adds 0 and divides by 1.
+template <bool LOCAL>
 __device__ __forceinline__ void filter_pair(const cx<float> *W, const TraceParams &pa, const TraceParams &pb, int k,
                                             int N, float inv_n, cx<float> &Zk, cx<float> &Zm) {
   const cx<float> Z = Zk, M = Zm;
@@ -218,8 +227,12 @@ __device__ __forceinline__ void filter_pair(const cx<float> *W, const TraceParam
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const TraceParams &p = *P[h];
-    if (p.inside != 0.0f) S[h] = scl(filt(R[h], stretched(W, k, p.sf, p.ishift, p.gain, N)), inv_n);
-    else S[h] = scl(R[h], inv_n);
+    if (p.inside != 0.0f) {
+      const cx<float> w = LOCAL ? stretched(W, k, p.sf, p.ishift, p.gain, N) : W[k];
+      S[h] = scl(filt(R[h], w), inv_n);
+    } else {
+      S[h] = scl(R[h], inv_n);
+    }
   }
   if (k == (N - k) % N) {
     Zk = mk(S[0].x, S[1].x);
@@ -233,7 +246,7 @@ __device__ __forceinline__ void filter_pair(const cx<float> *W, const TraceParam
 // z length N specialised.  Grid: column blocks x angles.  LDS: twiddles,
 // the angle's wavelet half spectrum, the traces' local parameters, the
 // pencils.
-template <int N, int TX, int THREADS>
+template <int N, int TX, int THREADS, bool LOCAL>
 __global__ void __launch_bounds__(THREADS) k_fm_fused(FwdArgs a, const cx<float> *__restrict__ twg) {
   // the angles of one column block are consecutive work ids on one XCD, so
   // their parameter reads after the first come from that XCD's L2
@@ -326,7 +339,7 @@ __global__ void __launch_bounds__(THREADS) k_fm_fused(FwdArgs a, const cx<float>
       cx<float> *col = buf + c * PITCH;
       const int km = (N - k) % N;
       cx<float> zk = col[padx<float>(k)], zm = col[padx<float>(km)];
-      filter_pair(wsp, tp[2 * c], tp[2 * c + 1], k, N, a.inv_n, zk, zm);
+      filter_pair<LOCAL>(wsp, tp[2 * c], tp[2 * c + 1], k, N, a.inv_n, zk, zm);
       col[padx<float>(k)] = zk;
       if (k != km) col[padx<float>(km)] = zm;
     }
@@ -349,19 +362,32 @@ __global__ void __launch_bounds__(THREADS) k_fm_fused(FwdArgs a, const cx<float>
   spasses_io<N, TX, THREADS, 1, 0, false, true, float>(buf, tw, LdsIo{}, st);
 }
 
-template <int N, int TX, int TH>
-static int launch_fused_v(const FwdArgs &a, const void *tw, hipStream_t s) {
+template <int N, int TX, int TH, bool LOCAL>
+static int launch_fused_l(const FwdArgs &a, const void *tw, hipStream_t s) {
   const size_t lds = sizeof(cx<float>) * ((size_t)N + (size_t)(N / 2 + 1) + (size_t)TX * SP<N, float>::pitch) +
                      sizeof(TraceParams) * 2 * TX;
   if (lds > 65536)
-    CRAVA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_fm_fused<N, TX, TH>),
+    CRAVA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_fm_fused<N, TX, TH, LOCAL>),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const long long ncol = (long long)(a.rnxp / 2) * a.nyp;
   const unsigned gx = (unsigned)((ncol + TX - 1) / TX);
-  hipLaunchKernelGGL((k_fm_fused<N, TX, TH>), dim3(gx, (unsigned)a.nt), dim3(TH), lds, s, a,
+  hipLaunchKernelGGL((k_fm_fused<N, TX, TH, LOCAL>), dim3(gx, (unsigned)a.nt), dim3(TH), lds, s, a,
                      reinterpret_cast<const cx<float> *>(tw));
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
+}
+
+// True if any angle has a stretch, shift or gain grid.
+static bool has_local_wavelets(const FwdArgs &a) {
+  for (int l = 0; l < a.nt; ++l)
+    if (a.sf[l] || a.shift[l] || a.gain[l]) return true;
+  return false;
+}
+
+template <int N, int TX, int TH>
+static int launch_fused_v(const FwdArgs &a, const void *tw, hipStream_t s) {
+  return has_local_wavelets(a) ? launch_fused_l<N, TX, TH, true>(a, tw, s)
+                               : launch_fused_l<N, TX, TH, false>(a, tw, s);
 }
 
 // 16 columns (32 traces) per workgroup of 512 threads: the fastest of
@@ -371,7 +397,7 @@ static int launch_fused_n(const FwdArgs &a, const void *tw, hipStream_t s) {
   return launch_fused_v<N, 16, 512>(a, tw, s);
 }
 
-bool fm_fused_available(int nzp) { return static_fft_available(nzp); }
+bool fm_fused_available(int nzp) { return static_fft_full(nzp); }
 
 int launch_fm_fused(const FwdArgs &a, const void *tw, hipStream_t s) {
   switch (a.nzp) {

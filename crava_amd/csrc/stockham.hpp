@@ -47,10 +47,10 @@ constexpr bool kLds = std::is_same<F, LdsIo>::value;
 // sit side by side in global memory) or element-fastest (c = b / NB).
 // A pass whose inputs come from global memory needs no barrier before its
 // LDS writes (nothing else reads LDS then); a pass writing global memory
-// needs none after.
+// needs none after.  tid is the thread's index in the workgroup.
 template <int N, int TX, int THREADS, int DIR, int P, bool CF, typename T, typename Ld, typename St>
 __device__ __forceinline__ void spass(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw, const Ld &ld,
-                                      const St &st) {
+                                      const St &st, int tid) {
   constexpr int R = SP<N, T>::rl.r[P];
   constexpr int Ns = SP<N, T>::ns(P);
   constexpr int NB = N / R;
@@ -60,7 +60,7 @@ __device__ __forceinline__ void spass(cx<T> *__restrict__ buf, const cx<T> *__re
   cx<T> v[TRIPS][R];
 #pragma unroll
   for (int t = 0; t < TRIPS; ++t) {
-    const int b = (int)threadIdx.x + t * THREADS;
+    const int b = tid + t * THREADS;
     if (TOT % THREADS == 0 || b < TOT) {
       const int c = CF ? b % TX : b / NB, j = CF ? b / TX : b % NB;
 #pragma unroll
@@ -84,7 +84,7 @@ __device__ __forceinline__ void spass(cx<T> *__restrict__ buf, const cx<T> *__re
   if constexpr (kLds<Ld> && kLds<St>) __syncthreads();
 #pragma unroll
   for (int t = 0; t < TRIPS; ++t) {
-    const int b = (int)threadIdx.x + t * THREADS;
+    const int b = tid + t * THREADS;
     if (TOT % THREADS == 0 || b < TOT) {
       const int c = CF ? b % TX : b / NB, j = CF ? b / TX : b % NB;
       const int k = j % Ns;
@@ -104,15 +104,21 @@ __device__ __forceinline__ void spass(cx<T> *__restrict__ buf, const cx<T> *__re
 // passes between are element-fastest.
 template <int N, int TX, int THREADS, int DIR, int P, bool CF0, bool CFL, typename T, typename Ld, typename St>
 __device__ __forceinline__ void spasses_io(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw, const Ld &ld,
-                                           const St &st) {
+                                           const St &st, int tid) {
   constexpr int CNT = SP<N, T>::rl.cnt;
   static_assert(CNT >= 2, "the specialised lengths have at least two passes");
   if constexpr (P < CNT) {
-    if constexpr (P == 0) spass<N, TX, THREADS, DIR, P, CF0, T>(buf, tw, ld, LdsIo{});
-    else if constexpr (P == CNT - 1) spass<N, TX, THREADS, DIR, P, CFL, T>(buf, tw, LdsIo{}, st);
-    else spass<N, TX, THREADS, DIR, P, false, T>(buf, tw, LdsIo{}, LdsIo{});
-    spasses_io<N, TX, THREADS, DIR, P + 1, CF0, CFL, T>(buf, tw, ld, st);
+    if constexpr (P == 0) spass<N, TX, THREADS, DIR, P, CF0, T>(buf, tw, ld, LdsIo{}, tid);
+    else if constexpr (P == CNT - 1) spass<N, TX, THREADS, DIR, P, CFL, T>(buf, tw, LdsIo{}, st, tid);
+    else spass<N, TX, THREADS, DIR, P, false, T>(buf, tw, LdsIo{}, LdsIo{}, tid);
+    spasses_io<N, TX, THREADS, DIR, P + 1, CF0, CFL, T>(buf, tw, ld, st, tid);
   }
+}
+
+template <int N, int TX, int THREADS, int DIR, int P, bool CF0, bool CFL, typename T, typename Ld, typename St>
+__device__ __forceinline__ void spasses_io(cx<T> *__restrict__ buf, const cx<T> *__restrict__ tw, const Ld &ld,
+                                           const St &st) {
+  spasses_io<N, TX, THREADS, DIR, P, CF0, CFL, T>(buf, tw, ld, st, (int)threadIdx.x);
 }
 
 template <int N, int TX, int THREADS, int DIR, int P, typename T>

@@ -50,7 +50,6 @@
 //    of good traces from their clamped fill traces with the 3-D taper.
 //    Bit-exact: float sums in the reference's order, no contraction.
 //
-// Built with -ffp-contract=off.
 #include "../../include/crava_hip.h"
 #include "fft_static.hpp"
 #include "stockham.hpp"
@@ -59,6 +58,10 @@
 
 #include <algorithm>
 #include <cmath>
+
+// This file's own arithmetic keeps the reference's float operations (no
+// contraction into FMA); the Stockham passes included above are contracted.
+#pragma clang fp contract(off)
 
 namespace crava {
 
@@ -107,12 +110,27 @@ __device__ __forceinline__ float prep(float v, int k, float sq, int nz, int nzp)
 // Local parameters of one trace.
 struct SwdTrace {
   double delta_f;   // nz 1000 / (relT lz nzp)
+  double ph;        // phase of the shift per bin, in half-turns: 2 ishift / nzp
   float sf, ishift, gain, inside;
   float rsf, iag;   // 1 / sf, 1 / |gain|
+  float cs, sn;     // cos, sin of pi ph (one bin's phase step)
 };
 
+// sin, cos of pi t for t = ph m: the argument is reduced to [-1, 1]
+// half-turns in double, so the float sincospif sees an exact small argument;
+// no shift (ph = 0) gives (0, 1) without evaluating it.
+__device__ __forceinline__ void phase_at(double ph, int m, float *s, float *c) {
+  if (ph == 0.0) {
+    *s = 0.0f;
+    *c = 1.0f;
+    return;
+  }
+  const double t = ph * (double)m;
+  sincospif((float)(t - 2.0 * rint(0.5 * t)), s, c);
+}
+
 __device__ __forceinline__ SwdTrace trace_params(const SwdArgs &a, int l, int i, int j) {
-  SwdTrace p{0.0, 1.0f, 0.0f, 1.0f, 0.0f, 1.0f, 1.0f};
+  SwdTrace p{0.0, 0.0, 1.0f, 0.0f, 1.0f, 0.0f, 1.0f, 1.0f, 1.0f, 0.0f};
   if (i >= a.nxp) return p;
   p.inside = 1.0f;
   const int ii = mirror(i, a.nx, a.nxp), jj = mirror(j, a.ny, a.nyp);
@@ -128,19 +146,16 @@ __device__ __forceinline__ SwdTrace trace_params(const SwdArgs &a, int l, int i,
   p.delta_f = (double)a.nz * 1000.0 / (rt * a.lz * (double)a.nzp);
   p.rsf = 1.0f / p.sf;
   p.iag = 1.0f / fabsf(p.gain);
+  p.ph = 2.0 * (double)p.ishift / (double)a.nzp;
+  phase_at(p.ph, 1, &p.sn, &p.cs);
   return p;
 }
 
 // Local spectrum at bin m: gain e^{-2 pi i ishift m / nzp} W(m)
-// (Wavelet1D::shiftAndScale); the phase in turns is formed and reduced to
-// [-1, 1] half-turns in double, so the float sincospi sees an exact small
-// argument.  An even length's Nyquist bin is real (a c2r ignores its
-// imaginary part).
-__device__ __forceinline__ cx<float> local_at(const cx<float> *W, int m, float ishift, float gain, int nzp) {
+// (Wavelet1D::shiftAndScale), from the phase's (c, s) at m (phase_at).  An
+// even length's Nyquist bin is real (a c2r ignores its imaginary part).
+__device__ __forceinline__ cx<float> local_at(const cx<float> *W, int m, float c, float s, float gain, int nzp) {
   const cx<float> w = W[m];
-  const double t = 2.0 * (double)ishift * (double)m / (double)nzp;
-  float s, c;
-  sincospif((float)(t - 2.0 * rint(0.5 * t)), &s, &c);
   const float mr = gain * c, mi = -(gain * s);
   cx<float> v = mk(mr * w.x - mi * w.y, mi * w.x + mr * w.y);
   if (2 * m == nzp) v.y = 0.0f;
@@ -170,7 +185,15 @@ __device__ __forceinline__ cx<float> adjustment(const cx<float> *W, const float 
     int omU = omL + 1;
     if (omU >= cn) omU -= 1;
     const float d = omega - (float)omL;
-    const cx<float> a = local_at(W, omL, p.ishift, p.gain, nzp), b = local_at(W, omU, p.ishift, p.gain, nzp);
+    // phase at omL, and at omL + 1 by one step's rotation
+    float sL, cL;
+    phase_at(p.ph, omL, &sL, &cL);
+    float cU = cL, sU = sL;
+    if (omU != omL) {
+      cU = cL * p.cs - sL * p.sn;
+      sU = sL * p.cs + cL * p.sn;
+    }
+    const cx<float> a = local_at(W, omL, cL, sL, p.gain, nzp), b = local_at(W, omU, cU, sU, p.gain, nzp);
     const cx<float> ea = e3_at(t3, omL, a, p.iag, nzp), eb = e3_at(t3, omU, b, p.iag, nzp);
     wl.x = a.x * (1.0f - d) + b.x * d;
     wl.y = (a.y * (1.0f - d) + b.y * d);
@@ -185,7 +208,9 @@ __device__ __forceinline__ cx<float> adjustment(const cx<float> *W, const float 
   const float wg2k = wg2[k];
   float s_loc = wl2 * iwn2;
   s_loc += el.x * el.x + el.y * el.y;
-  const cx<float> c = local_at(W, k, p.ishift, p.gain, nzp);
+  float sk, ck;
+  phase_at(p.ph, k, &sk, &ck);
+  const cx<float> c = local_at(W, k, ck, sk, p.gain, nzp);
   const cx<float> ec = e3_at(t3, k, c, p.iag, nzp);
   float s_glob = (c.x * c.x + c.y * c.y) * iwn2;
   s_glob += ec.x * ec.x + ec.y * ec.y;
@@ -371,7 +396,7 @@ static int launch_fused_v(const SwdArgs &a, const void *tw, hipStream_t s) {
   return CRAVA_HIP_OK;
 }
 
-bool swd_fused_available(int nzp) { return static_fft_available(nzp); }
+bool swd_fused_available(int nzp) { return static_fft_full(nzp); }
 
 int launch_swd_fused(const SwdArgs &a, const void *tw, hipStream_t s) {
   switch (a.nzp) {

@@ -346,6 +346,51 @@ void estimate_z_padding(int nz, double lz, double min_rel_thick, double z_pad_fa
 }
 
 // ---------------------------------------------------------------------------
+// SeismicParametersHolder::computeCircCorrT + makeCircCorrTPosDef
+// (src/seismicparametersholder.cpp:431-499).  The reference's steps in float
+// -- symmetric circular extension, r2c, |.| in float with the lowest bins
+// zeroed, unnormalised c2r, scaling by 1/c(0) -- with each transform
+// evaluated in double and rounded once to float.
+
+bool circ_corr_t(const float *prior, int n, int min_int_fq, int nzp, float *out) {
+#pragma clang fp contract(off)
+  const int cn = nzp / 2 + 1;
+  std::vector<double> c(nzp);
+  for (int k = 0; k < nzp; ++k) {
+    const int refk = k < nzp / 2 + 1 ? k : nzp - k;
+    c[k] = refk < n ? (double)prior[refk] : 0.0;
+  }
+  const double w = 2.0 * 3.14159265358979323846 / (double)nzp;
+  std::vector<float> mag(cn);
+  for (int k = 0; k < cn; ++k) {
+    double sr = 0.0, si = 0.0;
+    for (int t = 0; t < nzp; ++t) {
+      const double a = w * (double)(((long long)k * t) % nzp);
+      sr += c[t] * std::cos(a);
+      si -= c[t] * std::sin(a);
+    }
+    const float re = (float)sr, im = (float)si;
+    const float p2 = re * re + im * im;
+    mag[k] = k <= min_int_fq ? 0.0f : std::sqrt(p2);
+  }
+  // c2r of a real, even half spectrum: bins 1..cn-1 twice, except an even
+  // length's Nyquist bin
+  std::vector<float> r(nzp);
+  for (int t = 0; t < nzp; ++t) {
+    double v = (double)mag[0];
+    for (int k = 1; k < cn; ++k) {
+      const double m = (double)mag[k] * std::cos(w * (double)(((long long)k * t) % nzp));
+      v += (nzp % 2 == 0 && k == nzp / 2) ? m : 2.0 * m;
+    }
+    r[t] = (float)v;
+  }
+  if (!(r[0] > 1.e-5f)) return false;
+  const float scale = 1.0f / r[0];
+  for (int k = 0; k < nzp; ++k) out[k] = r[k] * scale;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // SeismicParametersHolder::setCorrelationParameters' robustification of
 // Sigma0 (src/seismicparametersholder.cpp:114-135).
 

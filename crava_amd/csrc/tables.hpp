@@ -64,6 +64,9 @@ void estimate_z_padding(int nz, double lz, double min_rel_thick, double z_pad_fa
 // out was lifted, -1 when it would be lifted but DGEEV's eigenvectors are
 // not reproduced here (a zero off-diagonal entry or balancing).
 int prior_var0_robustify(const double in[9], double out[9]);
+// computeCircCorrT: nzp floats into out; false where the reference exits
+// (c(0) <= 1e-5).
+bool circ_corr_t(const float *prior, int n, int min_int_fq, int nzp, float *out);
 
 // Half spectrum (nzp/2+1 {re, im} pairs) of a real series of nzp samples
 // with FFTW's forward sign and no normalisation (Wavelet::fft1DInPlace,

@@ -211,7 +211,7 @@ class SeismicParametersHolder {
   // corrXY [nyp*nxp] and the temporal correlation priorCorrT, times Sigma0_ab.
   void initializeCorrelations(const std::vector<float> &priorCorrXY, const std::vector<float> &priorCorrT,
                               float corrGradI, float corrGradJ, int lowIntCut, int nzp);
-  // :431-499 (host; the 1-D transform in double).
+  // :431-499 (host, crava_hip_circ_corr_t).
   std::vector<float> computeCircCorrT(const std::vector<float> &priorCorrT, int minIntFq, int nzp) const;
   // :748-768: the cyclic temporal correlation, the real covAlpha grid at
   // (0, 0, k) over its value at the origin, k < nzp.

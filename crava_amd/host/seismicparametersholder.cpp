@@ -47,26 +47,6 @@ void SeismicParametersHolder::setCovTransformPrecision(int precision) {
     if (g) g->setTransformPrecision(precision);
 }
 
-// Real spectrum of a length-n real sequence by a direct DFT in double
-// (n is a padded trace length, a few hundred; this runs once per inversion).
-static void dft_real(const std::vector<double> &x, std::vector<double> &re, std::vector<double> &im) {
-  const int n = (int)x.size();
-  const int cn = n / 2 + 1;
-  re.assign(cn, 0.0);
-  im.assign(cn, 0.0);
-  const double w = -2.0 * 3.14159265358979323846 / n;
-  for (int k = 0; k < cn; ++k) {
-    double sr = 0.0, si = 0.0;
-    for (int t = 0; t < n; ++t) {
-      const long long m = ((long long)k * t) % n;
-      sr += x[t] * std::cos(w * (double)m);
-      si += x[t] * std::sin(w * (double)m);
-    }
-    re[k] = sr;
-    im[k] = si;
-  }
-}
-
 float SeismicParametersHolder::getOrigin(FFTGrid *grid) const {
   if (!grid || grid->getIsTransformed()) throw Error(CRAVA_HIP_E_STATE, "getOrigin needs a grid in the time domain");
   return grid->readTrace(0, 0, 0, 1)[0];
@@ -112,41 +92,20 @@ std::vector<float> SeismicParametersHolder::getPriorCorrTFiltered(int nz, int nz
 
 std::vector<float> SeismicParametersHolder::computeCircCorrT(const std::vector<float> &priorCorrT, int minIntFq,
                                                              int nzp) const {
-  if (priorCorrT.empty() || priorCorrT[0] == 0.0f) throw Error(CRAVA_HIP_E_INVALID, "priorCorrT[0] must be nonzero");
-  const int n = (int)priorCorrT.size();
+  if (priorCorrT.empty()) throw Error(CRAVA_HIP_E_INVALID, "priorCorrT must not be empty");
   std::vector<float> c(nzp);
-  for (int k = 0; k < nzp; ++k) {
-    const int refk = k < nzp / 2 + 1 ? k : nzp - k;   // symmetric circular extension
-    c[k] = refk < n ? priorCorrT[refk] : 0.0f;
-  }
-  makeCircCorrTPosDef(c, minIntFq, nzp);
+  check(crava_hip_circ_corr_t(priorCorrT.data(), (int)priorCorrT.size(), minIntFq, nzp, c.data()),
+        "computeCircCorrT");
   return c;
 }
 
-// :463-499: |spectrum| with the lowest frequencies (k <= minIntFq) removed,
-// back to a correlation normalised to 1 at lag 0.
+// :463-499 on a circularly extended correlation (c[k] = c[nzp - k], as
+// computeCircCorrT builds it): the C ABI re-extends its first half.
 void SeismicParametersHolder::makeCircCorrTPosDef(std::vector<float> &c, int minIntFq, int nzp) const {
-  std::vector<double> x(c.begin(), c.begin() + nzp), re, im;
-  dft_real(x, re, im);
-  const int cn = nzp / 2 + 1;
-  std::vector<double> mag(cn);
-  for (int k = 0; k < cn; ++k) mag[k] = k <= minIntFq ? 0.0 : std::sqrt(re[k] * re[k] + im[k] * im[k]);
-  // unnormalised inverse of the real, even spectrum (ifft(fft(f)) = n f, as
-  // the reference's 1-D transforms): c(t) = sum_k m_k e^{+2 pi i k t/n}
-  std::vector<double> out(nzp, 0.0);
-  for (int t = 0; t < nzp; ++t) {
-    double s = mag[0];
-    for (int k = 1; k < cn; ++k) {
-      const bool self = (nzp % 2 == 0) && (k == nzp / 2);
-      const double v = mag[k] * std::cos(2.0 * 3.14159265358979323846 * (double)(((long long)k * t) % nzp) / nzp);
-      s += self ? v : 2.0 * v;
-    }
-    out[t] = s;
-  }
-  const float c0 = (float)out[0];
-  if (!(c0 > 1e-5f)) throw Error(CRAVA_HIP_E_INVALID, "circular temporal correlation undefined (increase nz)");
-  const float scale = 1.0f / c0;
-  for (int k = 0; k < nzp; ++k) c[k] = (float)out[k] * scale;
+  if ((int)c.size() < nzp) throw Error(CRAVA_HIP_E_INVALID, "makeCircCorrTPosDef: fewer than nzp values");
+  std::vector<float> out(nzp);
+  check(crava_hip_circ_corr_t(c.data(), nzp / 2 + 1, minIntFq, nzp, out.data()), "makeCircCorrTPosDef");
+  for (int k = 0; k < nzp; ++k) c[k] = out[k];
 }
 
 void SeismicParametersHolder::initializeCorrelations(const std::vector<float> &priorCorrXY,

@@ -196,6 +196,16 @@ int crava_hip_estimate_xy_padding(int nx, int ny, double dx, double dy, double l
 int crava_hip_estimate_z_padding(int nz, double lz, double min_rel_thick, double z_pad_fac, int estimate,
                                  float default_wavelet_length, int *nzpad, double *true_z_pad_fac);
 
+/* SeismicParametersHolder::computeCircCorrT with makeCircCorrTPosDef
+ * (src/seismicparametersholder.cpp:431-499), host side: the prior temporal
+ * correlation prior_corr_t[0..n) extended symmetrically to nzp lags, made
+ * positive definite (|spectrum|, bins k <= min_int_fq zeroed -- the
+ * reference's lowIntCut, src/modelgeneral.cpp:3961) and scaled to 1 at lag 0;
+ * out: nzp floats.  The reference's float steps, each 1-D transform
+ * evaluated in double and rounded once.  CRAVA_HIP_E_INVALID where the
+ * reference asserts (prior_corr_t[0] == 0) or exits (c(0) <= 1e-5). */
+int crava_hip_circ_corr_t(const float *prior_corr_t, int n, int min_int_fq, int nzp, float *out);
+
 /* The robustification of the prior covariance in
  * SeismicParametersHolder::setCorrelationParameters
  * (src/seismicparametersholder.cpp:114-135), host side: with DGEEV's

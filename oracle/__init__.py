@@ -104,6 +104,8 @@ def ref():
         L.ref_fft3d_r2c.argtypes = [VP, I, I, I, I]
         L.ref_fft3d_c2r.argtypes = [VP, I, I, I, I]
         L.ref_fft1d_r2c.argtypes = [VP, I]
+        L.ref_circ_corr_t.argtypes = [VP, I, I, I, VP]
+        L.ref_circ_corr_t.restype = C.c_int
         L.ref_chol_cpx.argtypes = [I, VP]
         L.ref_axeqb_mat_cpx.argtypes = [I, VP, VP, I]
         L.ref_posterior.argtypes = [I, I, I, I, VP, VP, F, VP, VP, VP, VP, VP, VP, PP, PP, PP, VP]
@@ -298,6 +300,15 @@ def posterior_f64(nxp, nyp, nzp, A, etc, wnc, sigma0, tabs, mu, data, cov, errco
                                        s0.ctypes.data, tabs["invert"].ctypes.data, tabs["kWD"].ctypes.data,
                                        tabs["e1"].ctypes.data, tabs["e2"].ctypes.data, tabs["e3"].ctypes.data,
                                        _ptrs(mu), _ptrs(data), _ptrs(cov), errcorr.ctypes.data))
+
+
+def circ_corr_t_ref(prior: np.ndarray, min_int_fq: int, nzp: int) -> tuple[np.ndarray, int]:
+    """computeCircCorrT on the reference's FFTW 1-D transforms (oracle/_ref):
+    (nzp lags, 1 where the reference would exit)."""
+    p = np.ascontiguousarray(prior, dtype=np.float32)
+    out = np.zeros(nzp, np.float32)
+    bad = ref().ref_circ_corr_t(p.ctypes.data, int(p.size), int(min_int_fq), int(nzp), out.ctypes.data)
+    return out, int(bad)
 
 
 def chol_cpx(mat: np.ndarray, which: str = "oracle") -> int:

@@ -50,6 +50,43 @@ void ref_fft3d_c2r(float *g, int nxp, int nyp, int nzp, int is_cov)
   for (long long t = 0; t < rsize; t++) g[t] *= scale;
 }
 
+/* SeismicParametersHolder::computeCircCorrT + makeCircCorrTPosDef
+ * (src/seismicparametersholder.cpp:431-499) on the reference's FFTW 1-D
+ * transforms (FFTGrid::fft1DzInPlace / invFFT1DzInPlace, src/fftgrid.cpp:
+ * 2002-2037): symmetric circular extension, |spectrum| with k <= minIntFq
+ * zeroed, unnormalised inverse, scaled to 1 at lag 0.  out: nzp floats.
+ * Returns 1 where the reference would exit (c(0) <= 1e-5). */
+int ref_circ_corr_t(const float *prior, int n, int min_int_fq, int nzp, float *out)
+{
+  int cn = nzp / 2 + 1, k;
+  fftw_real *c = (fftw_real *)malloc(sizeof(fftw_real) * 2 * cn);
+  for (k = 0; k < 2 * cn; k++) {
+    if (k < nzp) {
+      int refk = k < nzp / 2 + 1 ? k : nzp - k;
+      c[k] = refk < n ? prior[refk] : 0.0f;
+    } else {
+      c[k] = -99999.0f;   /* RMISSING */
+    }
+  }
+  rfftwnd_plan plan = rfftwnd_create_plan(1, &nzp, FFTW_REAL_TO_COMPLEX, FFTW_ESTIMATE | FFTW_IN_PLACE);
+  rfftwnd_one_real_to_complex(plan, c, (fftw_complex *)c);
+  fftwnd_destroy_plan(plan);
+  fftw_complex *f = (fftw_complex *)c;
+  for (k = 0; k < cn; k++) {
+    if (k <= min_int_fq) f[k].re = 0.0f;
+    else f[k].re = (float)sqrtf(f[k].re * f[k].re + f[k].im * f[k].im);
+    f[k].im = 0.0f;
+  }
+  plan = rfftwnd_create_plan(1, &nzp, FFTW_COMPLEX_TO_REAL, FFTW_ESTIMATE | FFTW_IN_PLACE);
+  rfftwnd_one_complex_to_real(plan, f, c);
+  fftwnd_destroy_plan(plan);
+  int bad = !(c[0] > 1.e-5f);
+  float scale = bad ? 0.0f : (float)(1.0f / c[0]);
+  for (k = 0; k < nzp; k++) out[k] = c[k] * scale;
+  free(c);
+  return bad;
+}
+
 /* 1-D r2c (Wavelet::fft1DInPlace, src/wavelet.cpp:246-261) */
 void ref_fft1d_r2c(float *w, int nzp)
 {

@@ -99,9 +99,10 @@ def test_every_launch_variant_and_autotuner(ctx, n, dbl):
     (oracle.fft_r2c_f64 if dbl else lambda a, *r: oracle.fft_r2c(a, *r))(ref, nxp, nyp, nzp, False)
     tol = 1e-13 if dbl else 1e-6
     prec = L.F64 if dbl else L.F32
-    # static variants: strided 0..7, rows 0..5 (the upper halves load the
-    # first pass and store the last straight from/to global memory)
-    for v in range(-12, 8):
+    # static variants: strided 0..9, rows 0..5 (strided 4..7 and rows 3..5
+    # load the first pass and store the last straight from/to global memory,
+    # strided 8..9 do so in persistent workgroups prefetching the next tile)
+    for v in range(-12, 10):
         try:
             set_fft_variants(ctx, nxp, nyp, nzp, prec, [min(v, 5), v, v])
         except L.CravaHipError:
@@ -115,7 +116,7 @@ def test_every_launch_variant_and_autotuner(ctx, n, dbl):
         assert rel_l2(t.cpu().numpy()[:, :, :nxp], g[:, :, :nxp]) < tol, v
     scratch = torch.zeros(g.shape, dtype=torch.float64 if dbl else torch.float32, device="cuda")
     var, ms = autotune_fft(ctx, scratch, nxp, nyp, nzp)
-    assert all(-12 <= x <= 7 for x in var) and all(m > 0 for m in ms)
+    assert all(-12 <= x <= 9 for x in var) and all(m > 0 for m in ms)
     t = torch.from_numpy(g.copy()).cuda()
     ctx.r2c(t, nxp, nyp, nzp, L.DATA)
     torch.cuda.synchronize()
@@ -148,7 +149,7 @@ def test_r2c_float_storage_double_arithmetic(ctx, n):
     ctx.r2c(t32, nxp, nyp, nzp, L.COVARIANCE, L.F32)
     torch.cuda.synchronize()
     err32 = rel_l2(to_complex(t32.cpu().numpy()), want)
-    for v in (None, 0, 3, 4, 7):
+    for v in (None, 0, 3, 4, 7, 8, 9):
         if v is not None:
             try:
                 set_fft_variants(ctx, nxp, nyp, nzp, L.MIXED, [min(v, 5), v, v])
@@ -189,6 +190,81 @@ def test_generic_variants_odd_lengths(ctx, n):
             ctx.c2r(t, nxp, nyp, nzp, L.DATA, prec)
             torch.cuda.synchronize()
             assert rel_l2(t.cpu().numpy()[:, :, :nxp], g[:, :, :nxp]) < 1e-6, (prec, v)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,ng", [((80, 64, 160), 5), ((64, 64, 320), 5), ((64, 64, 64), 1)],
+                         ids=lambda p: "x".join(map(str, p)) if isinstance(p, tuple) else str(p))
+def test_persistent_strided_variants_on_sets(ctx, n, ng):
+    """Strided variants 8 and 9 (persistent workgroups over XCD-contiguous
+    tile ranges, the next tile's first pass prefetched) on a set of grids:
+    partial column blocks (41 complex columns), sets of 5 grids, and a
+    launch with more workgroups than tiles.  Spectra against the FP64
+    oracle, within rounding of the one-workgroup-per-tile form of the same
+    shape (variant 5), and the round trip; float, float storage with double
+    arithmetic, double."""
+    from crava_amd.core import set_fft_variants
+    nxp, nyp, nzp = n
+    print(f"persistent strided {n} x{ng}", flush=True)
+    for prec, dt, tol in ((L.F32, np.float32, 1e-6), (L.MIXED, np.float32, 3e-7), (L.F64, np.float64, 1e-13)):
+        gs = [seeded_real_grid(nxp, nyp, nzp, seed=20 + s, dtype=dt) for s in range(ng)]
+        refs = []
+        for a in gs:
+            r = a.astype(np.float64)
+            oracle.fft_r2c_f64(r, nxp, nyp, nzp, False)
+            refs.append(to_complex(r))
+        base = None
+        for v in (5, 8, 9):
+            set_fft_variants(ctx, nxp, nyp, nzp, prec, [5, v, v])
+            ts = [torch.from_numpy(a.copy()).cuda() for a in gs]
+            ctx.r2c_many(ts, nxp, nyp, nzp, L.DATA, prec)
+            torch.cuda.synchronize()
+            spec = [to_complex(t.cpu().numpy()) for t in ts]
+            for x, r in zip(spec, refs):
+                assert rel_l2(x, r) < tol, (prec, v)
+            if base is None:
+                base = spec
+            else:
+                for x, b in zip(spec, base):
+                    assert rel_l2(x, b) < tol, (prec, v)
+            ctx.c2r_many(ts, nxp, nyp, nzp, L.DATA, prec)
+            torch.cuda.synchronize()
+            for t, a in zip(ts, gs):
+                assert rel_l2(t.cpu().numpy()[:, :, :nxp], a[:, :, :nxp]) < tol, (prec, v)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [(105, 70, 45), (96, 35, 243), (189, 126, 50), (32, 315, 36)],
+                         ids=lambda n: "x".join(map(str, n)))
+def test_extended_lengths_direct_io_variants(ctx, n):
+    """The specialised kernels of the extended 7-smooth lengths (32..320
+    beyond the main ones; odd x lengths included): every direct-I/O variant
+    they carry (strided 4, 5; rows 3, 5), float, float storage with double
+    arithmetic and double, on a set of 3 grids, against the FP64 oracle and
+    through the round trip; variants they do not carry are refused."""
+    from crava_amd.core import set_fft_variants
+    nxp, nyp, nzp = n
+    print(f"extended lengths {n}", flush=True)
+    with pytest.raises(L.CravaHipError):
+        set_fft_variants(ctx, nxp, nyp, nzp, L.F32, [0, 1, 1])
+    for prec, dt, tol in ((L.F32, np.float32, 1e-6), (L.MIXED, np.float32, 3e-7), (L.F64, np.float64, 1e-13)):
+        gs = [seeded_real_grid(nxp, nyp, nzp, seed=40 + s, dtype=dt) for s in range(3)]
+        refs = []
+        for a in gs:
+            r = a.astype(np.float64)
+            oracle.fft_r2c_f64(r, nxp, nyp, nzp, False)
+            refs.append(to_complex(r))
+        for vr, vs in ((3, 4), (5, 5), (3, 5), (5, 4)):
+            set_fft_variants(ctx, nxp, nyp, nzp, prec, [vr, vs, vs])
+            ts = [torch.from_numpy(a.copy()).cuda() for a in gs]
+            ctx.r2c_many(ts, nxp, nyp, nzp, L.DATA, prec)
+            torch.cuda.synchronize()
+            for t, r in zip(ts, refs):
+                assert rel_l2(to_complex(t.cpu().numpy()), r) < tol, (prec, vr, vs)
+            ctx.c2r_many(ts, nxp, nyp, nzp, L.DATA, prec)
+            torch.cuda.synchronize()
+            for t, a in zip(ts, gs):
+                assert rel_l2(t.cpu().numpy()[:, :, :nxp], a[:, :, :nxp]) < tol, (prec, vr, vs)
 
 
 @pytest.mark.gpu
