@@ -214,24 +214,27 @@ struct TraceParams {
 // Filter of one bin pair (k, N-k) of a complex column carrying traces pa, pb
 // (Re, Im): separate the two traces' spectra, filter each trace inside the
 // volume with its local wavelet (spectrum W), scale by 1/N and pack again.
-// Without local wavelets (LOCAL false: no stretch, shift or gain grid) the
-// local spectrum is W(k) itself: stretched() at sf = 1, shift 0, gain 1
-// multiplies by 1, adds 0 and divides by 1.
+// The separation's factor 1/2 and the 1/N are one factor h = 1/(2N) on the
+// (local) wavelet spectrum.  Without local wavelets (LOCAL false: no
+// stretch, shift or gain grid) the local spectrum is W(k) itself:
+// stretched() at sf = 1, shift 0, gain 1 multiplies by 1, adds 0 and divides
+// by 1.
 template <bool LOCAL>
-__device__ __forceinline__ void filter_pair(const cx<float> *W, const TraceParams &pa, const TraceParams &pb, int k,
-                                            int N, float inv_n, cx<float> &Zk, cx<float> &Zm) {
+__device__ __forceinline__ void filter_pair_h(const cx<float> *W, const TraceParams &pa, const TraceParams &pb, int k,
+                                              int N, float h, cx<float> &Zk, cx<float> &Zm) {
+#pragma clang fp contract(fast)
   const cx<float> Z = Zk, M = Zm;
-  const cx<float> R[2] = {mk(0.5f * (Z.x + M.x), 0.5f * (Z.y - M.y)), mk(0.5f * (Z.y + M.y), 0.5f * (M.x - Z.x))};
+  const cx<float> R[2] = {mk(Z.x + M.x, Z.y - M.y), mk(Z.y + M.y, M.x - Z.x)};
   const TraceParams *P[2] = {&pa, &pb};
   cx<float> S[2];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const TraceParams &p = *P[h];
+  for (int q = 0; q < 2; ++q) {
+    const TraceParams &p = *P[q];
     if (p.inside != 0.0f) {
-      const cx<float> w = LOCAL ? stretched(W, k, p.sf, p.ishift, p.gain, N) : W[k];
-      S[h] = scl(filt(R[h], w), inv_n);
+      const cx<float> w = scl(LOCAL ? stretched(W, k, p.sf, p.ishift, p.gain, N) : W[k], h);
+      S[q] = filt(R[q], w);
     } else {
-      S[h] = scl(R[h], inv_n);
+      S[q] = scl(R[q], h);
     }
   }
   if (k == (N - k) % N) {
@@ -242,14 +245,27 @@ __device__ __forceinline__ void filter_pair(const cx<float> *W, const TraceParam
   }
 }
 
-// Fused forward model of TX complex columns (2 TX traces) for one angle,
-// z length N specialised.  Grid: column blocks x angles.  LDS: twiddles,
-// the angle's wavelet half spectrum, the traces' local parameters, the
-// pencils.
+// Fused forward model of TX complex columns (2 TX traces) for one angle, z
+// length N specialised, on the pencil-fastest passes
+// (stockham.hpp cpass): thread tid keeps complex column c = tid % TX and its
+// butterflies j = tid / TX + t (THREADS / TX) through the whole chain, so
+// every LDS offset is a constant from the column's base (pencils N | 1
+// apart, no padding inside a pencil).  Per angle:
+//   1. impedance of rows e < nz, and the taper to nzp beyond, into LDS;
+//   2. the cyclic forward difference imp(e + 1) - imp(e) is the first
+//      forward pass's input hook (no separate pass);
+//   3. forward passes; the filter in place over slots s (bins s and N - s,
+//      slot 0 bins 0 and N/2), each thread on its own column's slots;
+//   4. inverse passes, the last storing the seismic (padding traces their
+//      impedance).
+// The impedance, taper and difference are the reference's float operations;
+// the filter's two scalings are folded into one (filter_pair_h).  The
+// twiddles, the wavelet spectrum and the trace parameters stay in LDS: read
+// from global memory inside the chain they cost more than the fourth
+// resident workgroup that the LDS they take would allow.
 template <int N, int TX, int THREADS, bool LOCAL>
-__global__ void __launch_bounds__(THREADS) k_fm_fused(FwdArgs a, const cx<float> *__restrict__ twg) {
-  // the angles of one column block are consecutive work ids on one XCD, so
-  // their parameter reads after the first come from that XCD's L2
+__global__ void __launch_bounds__(THREADS) k_fm_cols(FwdArgs a, const cx<float> *__restrict__ twg) {
+  static_assert(N % 2 == 0, "the specialised lengths are even");
   const unsigned W = xcd_work_id(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   const unsigned bx = W / (unsigned)a.nt;
   const int l = (int)(W % (unsigned)a.nt);
@@ -258,12 +274,14 @@ __global__ void __launch_bounds__(THREADS) k_fm_fused(FwdArgs a, const cx<float>
   const long long col0 = (long long)bx * TX;
   const int ncb = (int)min((long long)TX, ncol - col0);
   constexpr int NK = N / 2 + 1;
-  constexpr int PITCH = SP<N, float>::pitch;
+  constexpr int PC = N | 1;
+  constexpr int JS = THREADS / TX;
   extern __shared__ unsigned char smem_raw[];
   cx<float> *tw = reinterpret_cast<cx<float> *>(smem_raw);
   cx<float> *wsp = tw + N;
   TraceParams *tp = reinterpret_cast<TraceParams *>(wsp + NK);
   cx<float> *buf = reinterpret_cast<cx<float> *>(tp + 2 * TX);
+  const float hn = 0.5f * a.inv_n;
   for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
   for (int q = threadIdx.x; q < NK; q += THREADS) wsp[q] = a.W[(long long)l * NK + q];
   if (threadIdx.x < 2 * TX) {
@@ -280,6 +298,9 @@ __global__ void __launch_bounds__(THREADS) k_fm_fused(FwdArgs a, const cx<float>
     }
     tp[threadIdx.x] = p;
   }
+  const int c = (int)threadIdx.x % TX, j0 = (int)threadIdx.x / TX;
+  cx<float> *col = buf + c * PC;
+  const long long gc = col0 + min(c, ncb - 1);   // clamped global column
   const cx<float> *al = reinterpret_cast<const cx<float> *>(a.alpha);
   const cx<float> *be = reinterpret_cast<const cx<float> *>(a.beta);
   const cx<float> *rh = reinterpret_cast<const cx<float> *>(a.rho);
@@ -287,71 +308,62 @@ __global__ void __launch_bounds__(THREADS) k_fm_fused(FwdArgs a, const cx<float>
     const cx<float> x = al[o], y = be[o], z = rh[o];
     return mk(impedance(a.A[l], x.x, y.x, z.x), impedance(a.A[l], x.y, y.y, z.y));
   };
-  // 1. impedance of samples k < nz into LDS (trace 2m = Re, 2m+1 = Im); the
-  // trip count is a constant so every thread's loads are in flight at once
-  constexpr int TRIPS = (TX * N + THREADS - 1) / THREADS;
-  cx<float> r[TRIPS];
+  // 1. impedance of rows e < nz (rows j0, j0 + JS, ...), the taper to nzp
+  // in the rows beyond from the column's first and last impedance
+  constexpr int ITRIPS = (N + JS - 1) / JS;
+  const cx<float> i0 = imp2(gc), il = imp2((long long)(a.nz - 1) * ncol + gc);
+  cx<float> r[ITRIPS];
 #pragma unroll
-  for (int t = 0; t < TRIPS; ++t) {
-    const int q = (int)threadIdx.x + t * THREADS;
-    const int e = q / TX, c = q % TX;
+  for (int t = 0; t < ITRIPS; ++t) {
+    const int e = j0 + t * JS;
     r[t] = mk(0.0f, 0.0f);
-    if (e < a.nz && c < ncb) r[t] = imp2((long long)e * ncol + col0 + c);
+    if (e < a.nz) r[t] = imp2((long long)e * ncol + gc);
   }
-#pragma unroll
-  for (int t = 0; t < TRIPS; ++t) {
-    const int q = (int)threadIdx.x + t * THREADS;
-    if (q / TX < a.nz) buf[(q % TX) * PITCH + padx<float>(q / TX)] = r[t];
-  }
-  __syncthreads();
-  // 2. taper to nzp and forward difference (Wavelet1D FIRSTORDERFORWARDDIFF)
   const float fac = 1.0f / (float)(a.nzp - a.nz - 1);
-  const auto imp_at = [&](int c, int e) -> cx<float> {
-    if (e < a.nz) return buf[c * PITCH + padx<float>(e)];
-    const cx<float> i0 = buf[c * PITCH + padx<float>(0)], il = buf[c * PITCH + padx<float>(a.nz - 1)];
-    const float u = (float)(e - a.nz), w = (float)(a.nzp - e - 1);
-    return mk(fac * (u * i0.x + w * il.x), fac * (u * i0.y + w * il.y));
-  };
 #pragma unroll
-  for (int t = 0; t < TRIPS; ++t) {
-    const int q = (int)threadIdx.x + t * THREADS;
-    if ((TX * N) % THREADS == 0 || q < TX * N) {
-      const int e = q / TX, c = q % TX;
-      r[t] = sub(imp_at(c, e == N - 1 ? 0 : e + 1), imp_at(c, e));
+  for (int t = 0; t < ITRIPS; ++t) {
+    const int e = j0 + t * JS;
+    if (e < a.nz) {
+      col[e] = r[t];
+    } else if (N % JS == 0 || e < N) {
+      const float u = (float)(e - a.nz), w = (float)(a.nzp - e - 1);
+      col[e] = mk(fac * (u * i0.x + w * il.x), fac * (u * i0.y + w * il.y));
     }
   }
   __syncthreads();
+  // 2. + 3. the cyclic forward difference as the first pass's input
+  const auto diff = [&](int e) -> cx<float> { return sub(col[e == N - 1 ? 0 : e + 1], col[e]); };
+  cpasses<N, TX, THREADS, -1, 0, true, 1, float>(col, tw, diff, LdsIo{}, j0);
+  // filter over the column's slots
+  {
+    const TraceParams pa = tp[2 * c], pb = tp[2 * c + 1];
+    constexpr int NS = N / 2;
+    constexpr int STRIPS = (NS + JS - 1) / JS;
 #pragma unroll
-  for (int t = 0; t < TRIPS; ++t) {
-    const int q = (int)threadIdx.x + t * THREADS;
-    if ((TX * N) % THREADS == 0 || q < TX * N) buf[(q % TX) * PITCH + padx<float>(q / TX)] = r[t];
-  }
-  __syncthreads();
-  // 3. forward transform of every column
-  spasses<N, TX, THREADS, -1, 0, float>(buf, tw);
-  // 4. filter, bin pairs (k, N-k)
-  constexpr int FTRIPS = (TX * NK + THREADS - 1) / THREADS;
-#pragma unroll
-  for (int t = 0; t < FTRIPS; ++t) {
-    const int q = (int)threadIdx.x + t * THREADS;
-    if ((TX * NK) % THREADS == 0 || q < TX * NK) {
-      const int k = q / TX, c = q % TX;
-      cx<float> *col = buf + c * PITCH;
-      const int km = (N - k) % N;
-      cx<float> zk = col[padx<float>(k)], zm = col[padx<float>(km)];
-      filter_pair<LOCAL>(wsp, tp[2 * c], tp[2 * c + 1], k, N, a.inv_n, zk, zm);
-      col[padx<float>(k)] = zk;
-      if (k != km) col[padx<float>(km)] = zm;
+    for (int t = 0; t < STRIPS; ++t) {
+      const int s = j0 + t * JS;
+      if (NS % JS == 0 || s < NS) {
+        // slot 0: bin 0 pairs with itself
+        cx<float> zk = col[s], zm = col[s == 0 ? 0 : N - s];
+        filter_pair_h<LOCAL>(wsp, pa, pb, s, N, hn, zk, zm);
+        col[s] = zk;
+        if (s != 0) col[N - s] = zm;
+      }
+    }
+    if (j0 == 0) {
+      // slot 0 also holds the Nyquist bin
+      cx<float> zn = col[N / 2], dummy = zn;
+      filter_pair_h<LOCAL>(wsp, pa, pb, N / 2, N, hn, zn, dummy);
+      col[N / 2] = zn;
     }
   }
   __syncthreads();
-  // 5. inverse transform, the last pass storing the seismic; padding traces
-  // receive their impedance
+  // 4. inverse passes, the last storing the seismic
   float *out = a.out[l];
-  const auto st = [&](int c, int e, cx<float> v) {
+  const bool ia = tp[2 * c].inside != 0.0f, ib = tp[2 * c + 1].inside != 0.0f;
+  const auto st = [&](int e, cx<float> v) {
     if (c >= ncb) return;
     const long long o = (long long)e * ncol + col0 + c;
-    const bool ia = tp[2 * c].inside != 0.0f, ib = tp[2 * c + 1].inside != 0.0f;
     if (!ia || !ib) {
       const cx<float> p = imp2(o);
       if (!ia) v.x = p.x;
@@ -359,19 +371,19 @@ __global__ void __launch_bounds__(THREADS) k_fm_fused(FwdArgs a, const cx<float>
     }
     reinterpret_cast<cx<float> *>(out)[o] = v;
   };
-  spasses_io<N, TX, THREADS, 1, 0, false, true, float>(buf, tw, LdsIo{}, st);
+  cpasses<N, TX, THREADS, 1, 0, true, 1, float>(col, tw, LdsIo{}, st, j0);
 }
 
 template <int N, int TX, int TH, bool LOCAL>
-static int launch_fused_l(const FwdArgs &a, const void *tw, hipStream_t s) {
-  const size_t lds = sizeof(cx<float>) * ((size_t)N + (size_t)(N / 2 + 1) + (size_t)TX * SP<N, float>::pitch) +
+static int launch_cols_l(const FwdArgs &a, const void *tw, hipStream_t s) {
+  const size_t lds = sizeof(cx<float>) * ((size_t)N + (size_t)(N / 2 + 1) + (size_t)TX * (N | 1)) +
                      sizeof(TraceParams) * 2 * TX;
   if (lds > 65536)
-    CRAVA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_fm_fused<N, TX, TH, LOCAL>),
+    CRAVA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_fm_cols<N, TX, TH, LOCAL>),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const long long ncol = (long long)(a.rnxp / 2) * a.nyp;
   const unsigned gx = (unsigned)((ncol + TX - 1) / TX);
-  hipLaunchKernelGGL((k_fm_fused<N, TX, TH, LOCAL>), dim3(gx, (unsigned)a.nt), dim3(TH), lds, s, a,
+  hipLaunchKernelGGL((k_fm_cols<N, TX, TH, LOCAL>), dim3(gx, (unsigned)a.nt), dim3(TH), lds, s, a,
                      reinterpret_cast<const cx<float> *>(tw));
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
@@ -386,8 +398,8 @@ static bool has_local_wavelets(const FwdArgs &a) {
 
 template <int N, int TX, int TH>
 static int launch_fused_v(const FwdArgs &a, const void *tw, hipStream_t s) {
-  return has_local_wavelets(a) ? launch_fused_l<N, TX, TH, true>(a, tw, s)
-                               : launch_fused_l<N, TX, TH, false>(a, tw, s);
+  return has_local_wavelets(a) ? launch_cols_l<N, TX, TH, true>(a, tw, s)
+                               : launch_cols_l<N, TX, TH, false>(a, tw, s);
 }
 
 // 16 columns (32 traces) per workgroup of 512 threads: the fastest of

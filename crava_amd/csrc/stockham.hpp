@@ -131,4 +131,80 @@ constexpr size_t lds_bytes(int tx) {
   return sizeof(cx<T>) * ((size_t)N + (size_t)tx * SP<N, T>::pitch);
 }
 
+// Pencil-fastest passes without padding, for the fused per-column kernels.
+// Thread tid keeps pencil c = tid % TX through every pass, working on its
+// butterflies j = tid / TX + t (THREADS / TX); element e of its pencil is
+// col[e ES] in LDS.  With ES = TX (element-major tile, buf[e TX + c]) the
+// TX lanes that share a butterfly index touch TX consecutive elements,
+// which spreads them over the banks with no padding, and every LDS offset a
+// thread uses is a compile-time constant from its pencil's base `col` (no
+// per-element index arithmetic).  Twiddles come from the same per-pass table
+// as spass, in LDS or global memory.
+
+// Pass P over the thread's pencil.  Inputs element j + r NB, from col or
+// ld(e); outputs element (j / Ns) Ns R + j % Ns + r Ns, to col or st(e, v).
+// SYNC puts a workgroup barrier between the reads and the writes (needed
+// whenever both touch the pencils in LDS); a pass writing LDS ends with one.
+template <int N, int TX, int THREADS, int DIR, int P, bool SYNC, int ES, typename T, typename Ld, typename St>
+__device__ __forceinline__ void cpass(cx<T> *__restrict__ col, const cx<T> *__restrict__ tw, const Ld &ld,
+                                      const St &st, int j0) {
+  static_assert(THREADS % TX == 0, "a thread keeps one pencil");
+  constexpr int R = SP<N, T>::rl.r[P];
+  constexpr int Ns = SP<N, T>::ns(P);
+  constexpr int NB = N / R;
+  constexpr int JS = THREADS / TX;
+  constexpr int TRIPS = (NB + JS - 1) / JS;
+  cx<T> v[TRIPS][R];
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int j = j0 + t * JS;
+    if (NB % JS == 0 || j < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (kLds<Ld>) v[t][r] = col[(j + r * NB) * ES];
+        else v[t][r] = ld(j + r * NB);
+      }
+      if constexpr (Ns > 1) {
+        const int k = j % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const cx<T> w = tw[SP<N, T>::off(P) + (r - 1) * Ns + k];
+          v[t][r] = DIR > 0 ? mulc(v[t][r], w) : mul(v[t][r], w);
+        }
+      }
+      Dft<R, DIR, T>::run(v[t]);
+    }
+  }
+  if constexpr (SYNC) __syncthreads();
+#pragma unroll
+  for (int t = 0; t < TRIPS; ++t) {
+    const int j = j0 + t * JS;
+    if (NB % JS == 0 || j < NB) {
+      const int base = (j / Ns) * Ns * R + j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (kLds<St>) col[(base + r * Ns) * ES] = v[t][r];
+        else st(base + r * Ns, v[t][r]);
+      }
+    }
+  }
+  if constexpr (kLds<St>) __syncthreads();
+}
+
+// All passes of one direction.  The first pass reads through ld (SYNC0: it
+// reads LDS, so its writes wait for a barrier), the last writes through st;
+// either may be LdsIo.
+template <int N, int TX, int THREADS, int DIR, int P, bool SYNC0, int ES, typename T, typename Ld, typename St>
+__device__ __forceinline__ void cpasses(cx<T> *__restrict__ col, const cx<T> *__restrict__ tw, const Ld &ld,
+                                        const St &st, int j0) {
+  constexpr int CNT = SP<N, T>::rl.cnt;
+  static_assert(CNT >= 2, "the specialised lengths have at least two passes");
+  if constexpr (P < CNT) {
+    if constexpr (P == 0) cpass<N, TX, THREADS, DIR, P, SYNC0 || kLds<Ld>, ES, T>(col, tw, ld, LdsIo{}, j0);
+    else if constexpr (P == CNT - 1) cpass<N, TX, THREADS, DIR, P, kLds<St>, ES, T>(col, tw, LdsIo{}, st, j0);
+    else cpass<N, TX, THREADS, DIR, P, true, ES, T>(col, tw, LdsIo{}, LdsIo{}, j0);
+    cpasses<N, TX, THREADS, DIR, P + 1, SYNC0, ES, T>(col, tw, ld, st, j0);
+  }
+}
+
 }  // namespace crava

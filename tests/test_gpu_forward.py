@@ -20,6 +20,9 @@ CASES = [
     (12, 9, 60, 16, 12, 80, 3, True),    # specialised z length
     (15, 11, 41, 18, 14, 50, 8, True),   # runtime-length z (50 = 2 5 5), 8 angles
     (30, 20, 130, 36, 24, 160, 6, True),
+    (20, 9, 250, 24, 10, 320, 6, True),   # the 512x512x320 workload's z length
+    (10, 6, 200, 12, 8, 288, 2, False),   # radix-3 passes
+    (9, 7, 400, 10, 8, 512, 3, True),
 ]
 
 
@@ -69,6 +72,33 @@ def test_synthetic_seismic_vs_oracle(ctx, case):
     synthetic_seismic(ctx, *n, A, w, 4.0, *dg, out, **kw)
     torch.cuda.synchronize()
     assert all(torch.equal(a, b) for a, b in zip(first, out))
+
+
+@pytest.mark.gpu
+def test_synthetic_seismic_log_levels(ctx):
+    """Parameters as log(Vp), log(Vs), log(rho) levels with small variations:
+    the impedance's float rounding is of the order of the reflection
+    coefficients' own size, so agreement at the usual bound needs the
+    impedance, taper and difference rounded as the reference rounds them."""
+    n = (20, 9, 250, 24, 10, 320)
+    nt = 6
+    print("forward log levels", flush=True)
+    rng = np.random.default_rng(5)
+    rnxp = 2 * (n[3] // 2 + 1)
+    g = [(lvl + 0.01 * rng.standard_normal((n[5], n[4], rnxp))).astype(np.float32) for lvl in (8.0, 7.3, 0.8)]
+    th = np.deg2rad(np.linspace(4.0, 40.0, nt))
+    k2 = 0.25 * np.sin(th) ** 2 * 4
+    A = np.stack([0.5 * (1 + np.tan(th) ** 2), -k2 * 2, 0.5 * (1 - k2)], axis=1).astype(np.float32)
+    t = np.arange(n[5])
+    t = np.minimum(t, n[5] - t).astype(np.float64)
+    w = np.stack([np.exp(-(t / 3.0) ** 2) * np.cos(0.4 * t) for _ in range(nt)]).astype(np.float32)
+    want = oracle.synthetic_seismic(*n, A, w, 4.0, *g)
+    dg = [torch.from_numpy(x).cuda() for x in g]
+    out = [torch.empty_like(dg[0]) for _ in range(nt)]
+    synthetic_seismic(ctx, *n, A, w, 4.0, *dg, out)
+    torch.cuda.synchronize()
+    for l in range(nt):
+        _check(out[l].cpu().numpy(), want[l], n)
 
 
 @pytest.mark.gpu
