@@ -306,10 +306,20 @@ __global__ void __launch_bounds__(256) k_swd_filter(SwdArgs a) {
 }
 
 // Fused division of TX complex columns (2 TX traces) for one angle, z
-// length N specialised.  Grid: column blocks x angles.  LDS: twiddles, the
-// angle's tables, the traces' parameters, the pencils.
+// length N specialised, on the pencil-fastest passes (stockham.hpp cpass):
+// thread tid keeps complex column c = tid % TX and its butterflies
+// j = tid / TX + t (THREADS / TX) through the whole chain, so every LDS
+// offset is a constant from the column's base (pencils N | 1 apart).  Grid:
+// column blocks x angles.  LDS: the traces' parameters, twiddles, the
+// angle's tables, the pencils.
+//   1. forward transform, the first pass loading straight from global
+//      memory with the scaling and taper;
+//   2. the filter in place over slots s (bins s and N - s, slot 0 bins 0
+//      and N/2), each thread on its own column's slots;
+//   3. inverse transform, the last pass storing the result / sqrt(nzp).
 template <int N, int TX, int THREADS>
 __global__ void __launch_bounds__(THREADS) k_swd_fused(SwdArgs a, const cx<float> *__restrict__ twg) {
+  static_assert(N % 2 == 0, "the specialised lengths are even");
   const unsigned Wid = xcd_work_id(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   const unsigned bx = Wid / (unsigned)a.nt;
   const int l = (int)(Wid % (unsigned)a.nt);
@@ -318,6 +328,8 @@ __global__ void __launch_bounds__(THREADS) k_swd_fused(SwdArgs a, const cx<float
   const long long col0 = (long long)bx * TX;
   const int ncb = (int)min((long long)TX, ncol - col0);
   constexpr int NK = N / 2 + 1;
+  constexpr int PC = N | 1;
+  constexpr int JS = THREADS / TX;
   extern __shared__ unsigned char smem_raw[];
   SwdTrace *tp = reinterpret_cast<SwdTrace *>(smem_raw);
   cx<float> *tw = reinterpret_cast<cx<float> *>(tp + 2 * TX);
@@ -339,52 +351,62 @@ __global__ void __launch_bounds__(THREADS) k_swd_fused(SwdArgs a, const cx<float
     const long long q = col0 + min(c, ncb - 1);
     tp[threadIdx.x] = trace_params(a, l, (int)(q % cnx) * 2 + h, (int)(q / cnx));
   }
+  const int c = (int)threadIdx.x % TX, j0 = (int)threadIdx.x / TX;
+  cx<float> *col = buf + c * PC;
   cx<float> *g = reinterpret_cast<cx<float> *>(a.g[l]);
-  const cx<float> *ibase = g + col0;
+  const long long gc = col0 + min(c, ncb - 1);   // clamped global column
+  const int i0 = (int)(gc % cnx) * 2;
+  const bool pa_in = i0 < a.nxp, pb_in = i0 + 1 < a.nxp;
   // 1. forward transform, the first pass loading with the scaling and taper
   // (branch-free: clamped column, then select)
-  const auto ld = [&](int c, int e) -> cx<float> {
-    const cx<float> v = ibase[min(c, ncb - 1) + (long long)e * ncol];
-    const long long q = col0 + c;
-    const int i0 = (int)(q % cnx) * 2;
+  const auto ld = [&](int e) -> cx<float> {
+    const cx<float> v = g[gc + (long long)e * ncol];
     cx<float> r;
-    r.x = i0 < a.nxp ? prep(v.x, e, a.sq, a.nz, a.nzp) : v.x;
-    r.y = i0 + 1 < a.nxp ? prep(v.y, e, a.sq, a.nz, a.nzp) : v.y;
+    r.x = pa_in ? prep(v.x, e, a.sq, a.nz, a.nzp) : v.x;
+    r.y = pb_in ? prep(v.y, e, a.sq, a.nz, a.nzp) : v.y;
     return c < ncb ? r : mk(0.0f, 0.0f);
   };
-  spasses_io<N, TX, THREADS, -1, 0, true, false, float>(buf, tw, ld, LdsIo{});
-  // 2. filter, bin pairs (k, N-k)
-  constexpr int PITCH = SP<N, float>::pitch;
-  constexpr int FTRIPS = (TX * NK + THREADS - 1) / THREADS;
+  cpasses<N, TX, THREADS, -1, 0, false, 1, float>(col, tw, ld, LdsIo{}, j0);
+  // 2. filter over the column's slots
+  {
+    const SwdTrace pa = tp[2 * c], pb = tp[2 * c + 1];
+    constexpr int NS = N / 2;
+    constexpr int STRIPS = (NS + JS - 1) / JS;
 #pragma unroll
-  for (int t = 0; t < FTRIPS; ++t) {
-    const int q = (int)threadIdx.x + t * THREADS;
-    if ((TX * NK) % THREADS == 0 || q < TX * NK) {
-      const int k = q / TX, c = q % TX;
-      cx<float> *col = buf + c * PITCH;
-      const int km = (N - k) % N;
-      cx<float> zk = col[padx<float>(k)], zm = col[padx<float>(km)];
-      filter_column_pair(wsp, t3, tau2, wg2, tp[2 * c], tp[2 * c + 1], k, a, l, zk, zm);
-      col[padx<float>(k)] = zk;
-      if (k != km) col[padx<float>(km)] = zm;
+    for (int t = 0; t < STRIPS; ++t) {
+      const int s = j0 + t * JS;
+      if (NS % JS == 0 || s < NS) {
+        // slot 0: bin 0 pairs with itself
+        cx<float> zk = col[s], zm = col[s == 0 ? 0 : N - s];
+        filter_column_pair(wsp, t3, tau2, wg2, pa, pb, s, a, l, zk, zm);
+        col[s] = zk;
+        if (s != 0) col[N - s] = zm;
+      }
+    }
+    if (j0 == 0) {
+      // slot 0 also holds the Nyquist bin
+      cx<float> zn = col[N / 2], dummy = zn;
+      filter_column_pair(wsp, t3, tau2, wg2, pa, pb, N / 2, a, l, zn, dummy);
+      col[N / 2] = zn;
     }
   }
   __syncthreads();
   // 3. inverse transform, the last pass storing the result / sqrt(nzp)
-  const auto st = [&](int c, int e, cx<float> v) {
+  const bool ia = tp[2 * c].inside != 0.0f, ib = tp[2 * c + 1].inside != 0.0f;
+  const auto st = [&](int e, cx<float> v) {
     if (c >= ncb) return;
-    if (tp[2 * c].inside != 0.0f) v.x = v.x / a.sq;
-    if (tp[2 * c + 1].inside != 0.0f) v.y = v.y / a.sq;
+    if (ia) v.x = v.x / a.sq;
+    if (ib) v.y = v.y / a.sq;
     g[col0 + c + (long long)e * ncol] = v;
   };
-  spasses_io<N, TX, THREADS, 1, 0, false, true, float>(buf, tw, LdsIo{}, st);
+  cpasses<N, TX, THREADS, 1, 0, true, 1, float>(col, tw, LdsIo{}, st, j0);
 }
 
 template <int N, int TX, int TH>
 static int launch_fused_v(const SwdArgs &a, const void *tw, hipStream_t s) {
   constexpr int NK = N / 2 + 1;
   const size_t lds = sizeof(SwdTrace) * 2 * TX + sizeof(cx<float>) * ((size_t)N + NK) +
-                     sizeof(float) * (3 * NK + (NK & 1)) + sizeof(cx<float>) * (size_t)TX * SP<N, float>::pitch;
+                     sizeof(float) * (3 * NK + (NK & 1)) + sizeof(cx<float>) * (size_t)TX * (N | 1);
   if (lds > 65536)
     CRAVA_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_swd_fused<N, TX, TH>),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

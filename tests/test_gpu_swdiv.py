@@ -26,6 +26,8 @@ CASES = [
     (12, 11, 60, 16, 14, 80, 3),    # specialised z length (fused kernel)
     (13, 9, 50, 15, 12, 64, 4),     # fused, odd nxp
     (30, 20, 100, 36, 24, 128, 6),
+    (20, 9, 250, 24, 10, 320, 6),   # the 512x512x320 workload's z length
+    (11, 6, 200, 15, 8, 288, 2),    # radix-3 passes, odd nxp
 ]
 
 
