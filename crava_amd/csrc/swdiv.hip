@@ -107,6 +107,20 @@ __device__ __forceinline__ float prep(float v, int k, float sq, int nz, int nzp)
   return r;
 }
 
+// prep for the fused kernel: the same scaling and taper with the division
+// by sqrt(nzp) and the boundary distance's division as multiplications by
+// precomputed reciprocals (isq = 1 / sqrt(nzp), itaper = 1 / taper length).
+__device__ __forceinline__ float prep_r(float v, int k, float isq, float itaper, int nz, int nzp) {
+  float r = v * isq;
+  if (k > nz) {
+    const int below = nzp - k, above = k - (nz - 1);
+    const float d = (float)(above < below ? above : below) * itaper;
+    const float f = 1 - d * d;
+    r *= (f < 0.0f) ? 0.0f : f;
+  }
+  return r;
+}
+
 // Local parameters of one trace.
 struct SwdTrace {
   double delta_f;   // nz 1000 / (relT lz nzp)
@@ -155,6 +169,7 @@ __device__ __forceinline__ SwdTrace trace_params(const SwdArgs &a, int l, int i,
 // (Wavelet1D::shiftAndScale), from the phase's (c, s) at m (phase_at).  An
 // even length's Nyquist bin is real (a c2r ignores its imaginary part).
 __device__ __forceinline__ cx<float> local_at(const cx<float> *W, int m, float c, float s, float gain, int nzp) {
+#pragma clang fp contract(fast)
   const cx<float> w = W[m];
   const float mr = gain * c, mi = -(gain * s);
   cx<float> v = mk(mr * w.x - mi * w.y, mi * w.x + mr * w.y);
@@ -165,6 +180,7 @@ __device__ __forceinline__ cx<float> local_at(const cx<float> *W, int m, float c
 // Normalised errorSmooth3 at bin m from the local spectrum v there
 // (iag = 1 / |gain|).
 __device__ __forceinline__ cx<float> e3_at(const float *t3, int m, cx<float> v, float iag, int nzp) {
+#pragma clang fp contract(fast)
   const float t = t3[m] * iag;
   cx<float> e = mk(-t * v.y, t * v.x);
   if (2 * m >= nzp) e.x = kRMissing;
@@ -175,11 +191,12 @@ __device__ __forceinline__ cx<float> e3_at(const float *t3, int m, cx<float> v, 
 // Adjustment factor of one trace at bin k < nzp/2 + 1
 // (src/crava.cpp:772-814).  Tables of the angle: W, t3, tau2, wg2.
 __device__ __forceinline__ cx<float> adjustment(const cx<float> *W, const float *t3, const float *tau2,
-                                                const float *wg2, const SwdTrace &p, int k, int nzp, float mod_w,
-                                                float ev, float iwn2, float wnc) {
+                                                const float *wg2, const SwdTrace &p, int k, float sk, float ck,
+                                                int nzp, float mod_w, float ev, float iwn2, float wnc) {
+#pragma clang fp contract(fast)
   const int cn = nzp / 2 + 1;
   cx<float> wl = mk(0.0f, 0.0f), el = mk(0.0f, 0.0f);
-  const float omega = (float)k / p.sf;
+  const float omega = (float)k * p.rsf;   // k / sf (interpolation is continuous in omega)
   if (!(omega >= (float)cn)) {
     const int omL = (int)floorf(omega);
     int omU = omL + 1;
@@ -208,8 +225,6 @@ __device__ __forceinline__ cx<float> adjustment(const cx<float> *W, const float 
   const float wg2k = wg2[k];
   float s_loc = wl2 * iwn2;
   s_loc += el.x * el.x + el.y * el.y;
-  float sk, ck;
-  phase_at(p.ph, k, &sk, &ck);
   const cx<float> c = local_at(W, k, ck, sk, p.gain, nzp);
   const cx<float> ec = e3_at(t3, k, c, p.iag, nzp);
   float s_glob = (c.x * c.x + c.y * c.y) * iwn2;
@@ -224,19 +239,22 @@ __device__ __forceinline__ cx<float> adjustment(const cx<float> *W, const float 
     wl2a = 0.5f * (wl2 + kTolFac * wg2a);
     sig_l *= wl2a / wl2;
   }
-  // (wLoc / den) / (wGlob2Adj / (tau2 |W|^2 + sigma2Glob)) with one reciprocal
-  const float r = (t2 * wg2k + sig_g) / ((t2 * wl2a + sig_l) * wg2a);
+  // (wLoc / den) / (wGlob2Adj / (tau2 |W|^2 + sigma2Glob)) with one
+  // reciprocal (the hardware's, within 1 ulp)
+  const float r = (t2 * wg2k + sig_g) * __builtin_amdgcn_rcpf((t2 * wl2a + sig_l) * wg2a);
   return mk(wl.x * r, wl.y * r);
 }
 
 // Filtered bin k of one trace: R f inside the band, 0 outside; the slack
 // traces are only scaled by 1/nzp (their transforms are not normalised).
+// (sk, ck): sin and cos of pi ph k, the shift's phase at bin k (phase_at).
 __device__ __forceinline__ cx<float> filter_bin(const cx<float> *W, const float *t3, const float *tau2,
-                                                const float *wg2, const SwdTrace &p, int k, cx<float> R,
-                                                const SwdArgs &a, int l) {
+                                                const float *wg2, const SwdTrace &p, int k, float sk, float ck,
+                                                cx<float> R, const SwdArgs &a, int l) {
+#pragma clang fp contract(fast)
   if (p.inside == 0.0f) return scl(R, a.inv_n);
   if ((p.delta_f * k < a.high_cut) && (p.delta_f * k > a.low_cut)) {
-    const cx<float> f = adjustment(W, t3, tau2, wg2, p, k, a.nzp, a.mod_w[l], a.ev[l], a.iwn2[l], a.wnc);
+    const cx<float> f = adjustment(W, t3, tau2, wg2, p, k, sk, ck, a.nzp, a.mod_w[l], a.ev[l], a.iwn2[l], a.wnc);
     const float tmp = R.x * f.x - R.y * f.y;
     const float im = R.y * f.x + R.x * f.y;
     return mk(tmp, im);
@@ -245,16 +263,19 @@ __device__ __forceinline__ cx<float> filter_bin(const cx<float> *W, const float 
 }
 
 // Separate the spectra of the two traces of a complex column at bins k and
-// N - k, filter each, pack again.
+// N - k, filter each, pack again.  ph_a, ph_b: (sin, cos) of each trace's
+// shift phase at bin k.
 __device__ __forceinline__ void filter_column_pair(const cx<float> *W, const float *t3, const float *tau2,
                                                    const float *wg2, const SwdTrace &pa, const SwdTrace &pb,
-                                                   int k, const SwdArgs &a, int l, cx<float> &Zk, cx<float> &Zm) {
+                                                   int k, float2 ph_a, float2 ph_b, const SwdArgs &a, int l,
+                                                   cx<float> &Zk, cx<float> &Zm) {
+#pragma clang fp contract(fast)
   const int N = a.nzp;
   const cx<float> Z = Zk, M = Zm;
   const cx<float> Ra = mk(0.5f * (Z.x + M.x), 0.5f * (Z.y - M.y));
   const cx<float> Rb = mk(0.5f * (Z.y + M.y), 0.5f * (M.x - Z.x));
-  const cx<float> Sa = filter_bin(W, t3, tau2, wg2, pa, k, Ra, a, l);
-  const cx<float> Sb = filter_bin(W, t3, tau2, wg2, pb, k, Rb, a, l);
+  const cx<float> Sa = filter_bin(W, t3, tau2, wg2, pa, k, ph_a.x, ph_a.y, Ra, a, l);
+  const cx<float> Sb = filter_bin(W, t3, tau2, wg2, pb, k, ph_b.x, ph_b.y, Rb, a, l);
   if (k == (N - k) % N) {
     // DC, and Nyquist of an even length: a c2r reads only the real part
     Zk = mk(Sa.x, Sb.x);
@@ -300,7 +321,10 @@ __global__ void __launch_bounds__(256) k_swd_filter(SwdArgs a) {
   const long long o = (long long)l * cn;
   cx<float> *g = reinterpret_cast<cx<float> *>(a.g[l]);
   cx<float> zk = g[(long long)k * ncol + q], zm = g[(long long)km * ncol + q];
-  filter_column_pair(a.W + o, a.t3 + o, a.tau2 + o, a.wg2 + o, pa, pb, k, a, l, zk, zm);
+  float2 ha, hb;
+  phase_at(pa.ph, k, &ha.x, &ha.y);
+  phase_at(pb.ph, k, &hb.x, &hb.y);
+  filter_column_pair(a.W + o, a.t3 + o, a.tau2 + o, a.wg2 + o, pa, pb, k, ha, hb, a, l, zk, zm);
   g[(long long)k * ncol + q] = zk;
   if (k != km) g[(long long)km * ncol + q] = zm;
 }
@@ -357,13 +381,16 @@ __global__ void __launch_bounds__(THREADS) k_swd_fused(SwdArgs a, const cx<float
   const long long gc = col0 + min(c, ncb - 1);   // clamped global column
   const int i0 = (int)(gc % cnx) * 2;
   const bool pa_in = i0 < a.nxp, pb_in = i0 + 1 < a.nxp;
+  const float isq = 1.0f / a.sq;
+  const int mn = a.nz < a.nzp - a.nz ? a.nz : a.nzp - a.nz;
+  const float itaper = 1.0f / (float)(mn / 2.1);
   // 1. forward transform, the first pass loading with the scaling and taper
   // (branch-free: clamped column, then select)
   const auto ld = [&](int e) -> cx<float> {
     const cx<float> v = g[gc + (long long)e * ncol];
     cx<float> r;
-    r.x = pa_in ? prep(v.x, e, a.sq, a.nz, a.nzp) : v.x;
-    r.y = pb_in ? prep(v.y, e, a.sq, a.nz, a.nzp) : v.y;
+    r.x = pa_in ? prep_r(v.x, e, isq, itaper, a.nz, a.nzp) : v.x;
+    r.y = pb_in ? prep_r(v.y, e, isq, itaper, a.nz, a.nzp) : v.y;
     return c < ncb ? r : mk(0.0f, 0.0f);
   };
   cpasses<N, TX, THREADS, -1, 0, false, 1, float>(col, tw, ld, LdsIo{}, j0);
@@ -372,21 +399,32 @@ __global__ void __launch_bounds__(THREADS) k_swd_fused(SwdArgs a, const cx<float
     const SwdTrace pa = tp[2 * c], pb = tp[2 * c + 1];
     constexpr int NS = N / 2;
     constexpr int STRIPS = (NS + JS - 1) / JS;
+    // the shifts' phases at the slots' bins k = j0 + t JS: exact at k = j0
+    // (phase_at), then rotated by the phase of JS bins
+    float2 ha, hb, da, db;
+    phase_at(pa.ph, j0, &ha.x, &ha.y);
+    phase_at(pb.ph, j0, &hb.x, &hb.y);
+    phase_at(pa.ph, JS, &da.x, &da.y);
+    phase_at(pb.ph, JS, &db.x, &db.y);
 #pragma unroll
     for (int t = 0; t < STRIPS; ++t) {
       const int s = j0 + t * JS;
       if (NS % JS == 0 || s < NS) {
         // slot 0: bin 0 pairs with itself
         cx<float> zk = col[s], zm = col[s == 0 ? 0 : N - s];
-        filter_column_pair(wsp, t3, tau2, wg2, pa, pb, s, a, l, zk, zm);
+        filter_column_pair(wsp, t3, tau2, wg2, pa, pb, s, ha, hb, a, l, zk, zm);
         col[s] = zk;
         if (s != 0) col[N - s] = zm;
       }
+      ha = make_float2(ha.x * da.y + ha.y * da.x, ha.y * da.y - ha.x * da.x);
+      hb = make_float2(hb.x * db.y + hb.y * db.x, hb.y * db.y - hb.x * db.x);
     }
     if (j0 == 0) {
       // slot 0 also holds the Nyquist bin
       cx<float> zn = col[N / 2], dummy = zn;
-      filter_column_pair(wsp, t3, tau2, wg2, pa, pb, N / 2, a, l, zn, dummy);
+      phase_at(pa.ph, N / 2, &ha.x, &ha.y);
+      phase_at(pb.ph, N / 2, &hb.x, &hb.y);
+      filter_column_pair(wsp, t3, tau2, wg2, pa, pb, N / 2, ha, hb, a, l, zn, dummy);
       col[N / 2] = zn;
     }
   }
@@ -395,8 +433,8 @@ __global__ void __launch_bounds__(THREADS) k_swd_fused(SwdArgs a, const cx<float
   const bool ia = tp[2 * c].inside != 0.0f, ib = tp[2 * c + 1].inside != 0.0f;
   const auto st = [&](int e, cx<float> v) {
     if (c >= ncb) return;
-    if (ia) v.x = v.x / a.sq;
-    if (ib) v.y = v.y / a.sq;
+    if (ia) v.x = v.x * isq;
+    if (ib) v.y = v.y * isq;
     g[col0 + c + (long long)e * ncol] = v;
   };
   cpasses<N, TX, THREADS, 1, 0, true, 1, float>(col, tw, LdsIo{}, st, j0);
