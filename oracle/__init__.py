@@ -473,6 +473,7 @@ def _segy_lib():
         I, F, D, S = C.c_int, C.c_float, C.c_double, C.c_char_p
         L.ref_write_segy.argtypes = [S, VP, I, I, I, I, I] + [D] * 15 + [VP, VP, F, I]
         L.ref_write_sgri.argtypes = [S, S, VP, I, I, I, I, I] + [D] * 9 + [VP, VP]
+        L.ref_write_storm.argtypes = [S, VP, I, I, I, I, I, I] + [D] * 6 + [S, S, I, I, I]
         _ref_segy = L
     return _ref_segy
 
@@ -500,6 +501,20 @@ def write_sgri_ref(path, label, grid, nx, ny, nz, geom, top, bot):
     b = np.ascontiguousarray(bot, dtype=np.float64)
     rc = _segy_lib().ref_write_sgri(path.encode(), label.encode(), g.ctypes.data, nx, ny, nz, g.shape[2], g.shape[1],
                                     *[float(geom[k]) for k in _GEOM_KEYS[:9]], t.ctypes.data, b.ctypes.data)
+    assert rc == 0
+
+
+def write_storm_ref(path, grid, nx, ny, nz, cubetype, head, ascii=False, flat=False, scientific=False):
+    """FFTGrid::writeStormFile with Simbox::getStormHeader on NRLib's ToString
+    and WriteBinaryFloat: path + ".storm" (binary) or path + ".txt" (ascii).
+    grid: real padded float [nzp][nyp][rnxp]; (nx, ny, nz) the cube written
+    (the padded sizes for a padded file); head: dict xmin, lx, ymin, ly, lz,
+    angle (radians), top, bot (surface names)."""
+    g = np.ascontiguousarray(grid, dtype=np.float32)
+    rc = _segy_lib().ref_write_storm(path.encode(), g.ctypes.data, nx, ny, nz, g.shape[2], g.shape[1], cubetype,
+                                     *[float(head[k]) for k in ("xmin", "lx", "ymin", "ly", "lz", "angle")],
+                                     head.get("top", "").encode(), head.get("bot", "").encode(), int(flat),
+                                     int(ascii), int(scientific))
     assert rc == 0
 
 

@@ -2,7 +2,8 @@
 // (libs/nrlib/segy, iotools, the vendored boost subset; compiled from
 // /root/reference by oracle/Makefile into oracle/_ref/libcrava_ref_segy.so)
 // in the order FFTGrid::writeSegyFile uses it, and restates
-// FFTGrid::writeSgriFile on NRLib's binary float writer.
+// FFTGrid::writeSgriFile and FFTGrid::writeStormFile (with
+// Simbox::getStormHeader) on NRLib's ToString and binary float writer.
 //
 // TEST INFRASTRUCTURE ONLY (the checker): the product never links or loads
 // it.  The Simbox and FFTGrid glue (trace centres, surfaces, the regular-z
@@ -15,6 +16,8 @@
 #include <vector>
 
 #include "nrlib/iotools/fileio.hpp"
+#include "nrlib/iotools/stringtools.hpp"
+#include "nrlib/math/constants.hpp"
 #include "nrlib/segy/commonheaders.hpp"
 #include "nrlib/segy/segy.hpp"
 #include "nrlib/segy/segygeometry.hpp"
@@ -24,6 +27,7 @@ namespace {
 
 const double kRMissing = -99999.000;    // src/definitions.h:35
 const double kWellMissing = -999.00000; // src/definitions.h:39
+#define RMISSING_STORM -99999.000       // RMISSING as src/definitions.h:35 defines it (a double literal)
 
 struct Box {
   int nx, ny, nz, rnxp, nyp;
@@ -172,6 +176,63 @@ int ref_write_sgri(const char *fileName, const char *label, const float *grid, i
           }
           NRLib::WriteBinaryFloat(binFile, value);
         }
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
+// FFTGrid::writeStormFile, src/fftgrid.cpp:2118-2183, with the header of
+// Simbox::getStormHeader, src/simbox.cpp:408-430, statement for statement on
+// NRLib::ToString and NRLib::WriteBinaryFloat.  Writes path + ".storm"
+// (binary) or path + ".txt" (ascii; fixed, 6 decimals, or scientific).
+// grid: real padded float [nzp][nyp][rnxp]; padding writes the padded cube
+// (nx, ny, nz are then nxp, nyp, nzp); the values are getRealValue(i, j, k,
+// true) of the cube written.
+int ref_write_storm(const char *path, const float *grid, int nx, int ny, int nz, int rnxp, int nyp, int cubetype,
+                    double xmin, double lx, double ymin, double ly, double lz, double angle, const char *topName,
+                    const char *botName, int flat, int ascii, int scientific) {
+  try {
+    const std::string top(topName), bot(botName);
+    std::string header;
+    if (ascii == 0)
+      header = "storm_petro_binary\n";
+    else
+      header = "storm_petro_ascii\n";
+    header += "0 " + NRLib::ToString(cubetype) + " " + NRLib::ToString(RMISSING_STORM, 6) + "\n";
+    header += "FFTGrid\n";
+    if (flat == 0)
+      header += NRLib::ToString(xmin, 6) + " " + NRLib::ToString(lx, 6) + " " + NRLib::ToString(ymin, 6) + " " +
+                NRLib::ToString(ly, 6) + " " + top + " " + bot + " 0.0 0.0\n";
+    else
+      header += NRLib::ToString(xmin, 6) + " " + NRLib::ToString(lx, 6) + " " + NRLib::ToString(ymin, 6) + " " +
+                NRLib::ToString(ly, 6) + " 0.0 " + NRLib::ToString(lz, 6) + " 0.0 0.0\n";
+    header += NRLib::ToString(lz, 6) + " " + NRLib::ToString(angle * 180 / NRLib::Pi, 6) + "\n\n";
+    header += NRLib::ToString(nx) + " " + NRLib::ToString(ny) + " " + NRLib::ToString(nz) + "\n";
+    const auto value = [&](int i, int j, int k) {
+      return grid[(size_t)i + (size_t)rnxp * ((size_t)j + (size_t)nyp * k)];
+    };
+    if (ascii == 0) {
+      std::ofstream binFile(std::string(path) + ".storm", std::ios::out | std::ios::binary);
+      binFile << header;
+      for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+          for (int i = 0; i < nx; i++) NRLib::WriteBinaryFloat(binFile, value(i, j, k));
+      binFile << "0\n";
+    } else {
+      std::ofstream file(std::string(path) + ".txt");
+      file << header;
+      if (scientific)
+        file << std::scientific;
+      else
+        file << std::fixed << std::setprecision(6);
+      for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++) {
+          for (int i = 0; i < nx - 1; i++) file << value(i, j, k) << " ";
+          file << value(nx - 1, j, k) << "\n";
+        }
+      file << "0\n";
+    }
     return 0;
   } catch (...) {
     return 1;

@@ -171,6 +171,18 @@ def test_host_mirror_inversion_vs_oracle(tmp_path, name):
     lines = ["%s\n" % " ".join(f6(float(x)) for x in mu0[k, j, :c.nx]) for k in range(c.nz) for j in range(c.ny)]
     want = storm_head("ascii", geo + f" 0.0 {f6(1024.0)} 0.0 0.0", (c.nx, c.ny, c.nz)) + "".join(lines) + "0\n"
     assert open(os.path.join(d, "mu0_flat.txt")).read() == want
+    if oracle.ref_segy_available():
+        # and byte for byte as the reference's writer on NRLib's ToString and
+        # WriteBinaryFloat writes them
+        head = dict(xmin=456000.5, lx=11200.0, ymin=6780000.25, ly=5600.0, lz=1024.0, angle=0.3,
+                    top="top_surface", bot="base_surface")
+        oracle.write_storm_ref(os.path.join(d, "ref_mu0"), mu0, c.nx, c.ny, c.nz, ctype, head)
+        oracle.write_storm_ref(os.path.join(d, "ref_mu0_pad"), mu0, nxp, nyp, nzp, ctype, head)
+        oracle.write_storm_ref(os.path.join(d, "ref_mu0_flat"), mu0, c.nx, c.ny, c.nz, ctype, head, ascii=True,
+                               flat=True)
+        for a, b in (("mu0.storm", "ref_mu0.storm"), ("mu0_pad.storm", "ref_mu0_pad.storm"),
+                     ("mu0_flat.txt", "ref_mu0_flat.txt")):
+            assert open(os.path.join(d, a), "rb").read() == open(os.path.join(d, b), "rb").read(), a
 
     # State4D::merge
     if name == "small":

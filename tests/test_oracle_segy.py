@@ -3,7 +3,10 @@ the host mirror's FFTGrid::writeSegyFile / writeSgriFile), against what the
 SEG-Y rev. 0 layout and the IBM floating-point format fix: the textual
 header's card images, the binary header's fields, one 240-byte header and
 nz IBM floats per defined trace in inline / crossline order, IBM values of
-known numbers, and the SGRI header and big-endian body."""
+known numbers, and the SGRI header and big-endian body.  The STORM writer
+(FFTGrid::writeStormFile on NRLib's ToString and WriteBinaryFloat) against
+the format's fixed-6-decimal header fields, big-endian floats in k, j, i
+order and the ascii rows without trailing blanks."""
 import os
 import struct
 
@@ -102,3 +105,34 @@ def test_sgri_header_and_body(tmp_path):
     body = np.frombuffer(open(path + ".Sgri", "rb").read(), dtype=">f4").reshape(5, 3, 4)
     # z = zmin + k dz lies inside [top, bot]; sample k of the volume at k
     assert np.array_equal(body.astype(np.float32), g[:5, :3, :4])
+
+
+def test_storm_binary_ascii_and_padded(tmp_path):
+    """Header fields of Simbox::getStormHeader (ToString(x, 6) is fixed with 6
+    decimals, the cube type and sizes plain integers, the angle in degrees),
+    the cube in k, j, i order as big-endian floats or "%.6f" rows, then "0"."""
+    nx, ny, nz = 5, 3, 4
+    rnxp, nyp, nzp = 2 * (6 // 2 + 1), 4, 6
+    rng = np.random.default_rng(3)
+    g = rng.standard_normal((nzp, nyp, rnxp)).astype(np.float32) * np.float32(100.0)
+    g[0, 0, 0] = np.float32(2.0 ** -20)
+    head = dict(xmin=456000.5, lx=11200.0, ymin=6780000.25, ly=5600.0, lz=1024.0, angle=0.3,
+                top="top_surface", bot="base_surface")
+    f6 = lambda v: "%.6f" % v
+    geo = f"{f6(456000.5)} {f6(11200.0)} {f6(6780000.25)} {f6(5600.0)}"
+
+    def storm_head(kind, geom, n3):
+        return (f"storm_petro_{kind}\n0 4 {f6(-99999.0)}\nFFTGrid\n{geom}\n"
+                f"{f6(1024.0)} {f6(0.3 * 180 / np.pi)}\n\n{n3[0]} {n3[1]} {n3[2]}\n")
+    oracle.write_storm_ref(str(tmp_path / "a"), g, nx, ny, nz, 4, head)
+    want = storm_head("binary", geo + " top_surface base_surface 0.0 0.0", (nx, ny, nz)).encode()
+    want += g[:nz, :ny, :nx].astype(">f4").tobytes() + b"0\n"
+    assert open(tmp_path / "a.storm", "rb").read() == want
+    oracle.write_storm_ref(str(tmp_path / "p"), g, 6, nyp, nzp, 4, head)
+    want = storm_head("binary", geo + " top_surface base_surface 0.0 0.0", (6, nyp, nzp)).encode()
+    want += g[:, :, :6].astype(">f4").tobytes() + b"0\n"
+    assert open(tmp_path / "p.storm", "rb").read() == want
+    oracle.write_storm_ref(str(tmp_path / "f"), g, nx, ny, nz, 4, head, ascii=True, flat=True)
+    lines = ["%s\n" % " ".join(f6(float(x)) for x in g[k, j, :nx]) for k in range(nz) for j in range(ny)]
+    want = storm_head("ascii", geo + f" 0.0 {f6(1024.0)} 0.0 0.0", (nx, ny, nz)) + "".join(lines) + "0\n"
+    assert open(tmp_path / "f.txt").read() == want
