@@ -21,76 +21,92 @@ static unsigned blocks_for(long long n) {
   return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 1 << 20));
 }
 
-// FFTGrid::fillInParamCorr (src/fftgrid.cpp:1119-1164) followed by
-// multiplyByScalar(scale[g]) (src/seismicparametersholder.cpp:210-215) for
-// each grid g of the set: the same correlation written to up to 16 grids.
-__global__ void k_fill_param_corr(GridSet gs, int nxp, int nyp, int nzp, const float *__restrict__ corr_xy,
-                                  const float *__restrict__ circ_t, float grad_i, float grad_j) {
-  const int rnxp = 2 * (nxp / 2 + 1);
-  const long long rsize = (long long)rnxp * nyp * nzp;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < rsize;
-       p += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(p % rnxp);
-    const long long r = p / rnxp;
-    const int j = (int)(r % nyp);
-    const int k = (int)(r / nyp);
-    float value;
-    if (i < nxp) {
-      int cycle_i = i - nxp;
-      if (i < -cycle_i) cycle_i = i;
-      int cycle_j = j - nyp;
-      if (j < -cycle_j) cycle_j = j;
-      float sub_k = (float)k + (float)cycle_i * grad_i + (float)cycle_j * grad_j;
-      int base_k = (int)floorf(sub_k);
-      sub_k -= (float)base_k;
-      while (base_k < 0) base_k += nzp;
-      while (base_k >= nzp) base_k -= nzp;
-      value = (1.0f - sub_k) * circ_t[base_k];
-      if (base_k != nzp - 1) value += sub_k * circ_t[base_k + 1];
-      else value += sub_k * circ_t[0];
-      value *= corr_xy[i + nxp * j];
-    } else {
-      value = kRMissing;
+// FFTGrid::fillInParamCorr (src/fftgrid.cpp:1119-1164): the correlation at
+// (i, j, k), i < rnxp.
+__device__ __forceinline__ float param_corr_value(int i, int j, int k, int nxp, int nyp, int nzp,
+                                                  const float *__restrict__ corr_xy, const float *__restrict__ circ_t,
+                                                  float grad_i, float grad_j) {
+  if (i >= nxp) return kRMissing;
+  int cycle_i = i - nxp;
+  if (i < -cycle_i) cycle_i = i;
+  int cycle_j = j - nyp;
+  if (j < -cycle_j) cycle_j = j;
+  float sub_k = (float)k + (float)cycle_i * grad_i + (float)cycle_j * grad_j;
+  int base_k = (int)floorf(sub_k);
+  sub_k -= (float)base_k;
+  while (base_k < 0) base_k += nzp;
+  while (base_k >= nzp) base_k -= nzp;
+  float value = (1.0f - sub_k) * circ_t[base_k];
+  if (base_k != nzp - 1) value += sub_k * circ_t[base_k + 1];
+  else value += sub_k * circ_t[0];
+  value *= corr_xy[i + nxp * j];
+  return value;
+}
+
+// FFTGrid::fillInErrCorr (src/fftgrid.cpp:1065-1116), range 1: the value at
+// (i, j, k), i < rnxp.
+__device__ __forceinline__ float err_corr_value(int i, int j, int k, int nxp, int nyp, int nzp,
+                                                const float *__restrict__ corr_xy, float grad_i, float grad_j) {
+  int cycle_i = i - nxp;
+  if (i < -cycle_i) cycle_i = i;
+  int cycle_j = j - nyp;
+  if (j < -cycle_j) cycle_j = j;
+  float sub_k = (float)k + (float)cycle_i * grad_i + (float)cycle_j * grad_j;
+  if (i >= nxp) return kRMissing;
+  if (fabsf(sub_k) < 1.0f || fabsf(sub_k - (float)nzp) < 1.0f) {
+    int base_k = (int)sub_k;
+    sub_k -= (float)base_k;
+    while (base_k < -1) base_k += nzp;
+    while (base_k >= 1) base_k -= nzp;
+    if (base_k < 0) {
+      base_k = -1 - base_k;
+      sub_k = 1.0f - sub_k;
     }
-    for (int g = 0; g < gs.n; ++g) static_cast<float *>(gs.dst[g])[p] = value * (float)gs.scale[g];
+    return (1.0f - sub_k) * corr_xy[i + nxp * j];
+  }
+  return 0.0f;
+}
+
+// The fills walk the nyp nzp rows (j, k) of the grid, one row per workgroup
+// at a time (grid-stride), so j, k and what depends on them alone are
+// wave-uniform and no per-element index division remains; each thread
+// writes two consecutive x values as one float2 (rnxp is even).
+
+// fillInParamCorr followed by multiplyByScalar(scale[g])
+// (src/seismicparametersholder.cpp:210-215) for each grid g of the set: the
+// same correlation written to up to 16 grids.
+__global__ void __launch_bounds__(256) k_fill_param_corr(GridSet gs, int nxp, int nyp, int nzp,
+                                                         const float *__restrict__ corr_xy,
+                                                         const float *__restrict__ circ_t, float grad_i,
+                                                         float grad_j) {
+  const int rnxp = 2 * (nxp / 2 + 1), cn = rnxp / 2;
+  const long long rows = (long long)nyp * nzp;
+  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int j = (int)(row % nyp), k = (int)(row / nyp);
+    for (int h = threadIdx.x; h < cn; h += blockDim.x) {
+      const float v0 = param_corr_value(2 * h, j, k, nxp, nyp, nzp, corr_xy, circ_t, grad_i, grad_j);
+      const float v1 = param_corr_value(2 * h + 1, j, k, nxp, nyp, nzp, corr_xy, circ_t, grad_i, grad_j);
+      const long long p = row * rnxp + 2 * h;
+      for (int g = 0; g < gs.n; ++g) {
+        const float sc = (float)gs.scale[g];
+        *reinterpret_cast<float2 *>(static_cast<float *>(gs.dst[g]) + p) = make_float2(v0 * sc, v1 * sc);
+      }
+    }
   }
 }
 
-// FFTGrid::fillInErrCorr (src/fftgrid.cpp:1065-1116), range 1.
-__global__ void k_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *__restrict__ corr_xy,
-                                float grad_i, float grad_j) {
-  const int rnxp = 2 * (nxp / 2 + 1);
-  const long long rsize = (long long)rnxp * nyp * nzp;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < rsize;
-       p += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(p % rnxp);
-    const long long r = p / rnxp;
-    const int j = (int)(r % nyp);
-    const int k = (int)(r / nyp);
-    int cycle_i = i - nxp;
-    if (i < -cycle_i) cycle_i = i;
-    int cycle_j = j - nyp;
-    if (j < -cycle_j) cycle_j = j;
-    float sub_k = (float)k + (float)cycle_i * grad_i + (float)cycle_j * grad_j;
-    float value;
-    if (i < nxp) {
-      if (fabsf(sub_k) < 1.0f || fabsf(sub_k - (float)nzp) < 1.0f) {
-        int base_k = (int)sub_k;
-        sub_k -= (float)base_k;
-        while (base_k < -1) base_k += nzp;
-        while (base_k >= 1) base_k -= nzp;
-        if (base_k < 0) {
-          base_k = -1 - base_k;
-          sub_k = 1.0f - sub_k;
-        }
-        value = (1.0f - sub_k) * corr_xy[i + nxp * j];
-      } else {
-        value = 0.0f;
-      }
-    } else {
-      value = kRMissing;
+__global__ void __launch_bounds__(256) k_fill_err_corr(float *grid, int nxp, int nyp, int nzp,
+                                                       const float *__restrict__ corr_xy, float grad_i,
+                                                       float grad_j) {
+  const int rnxp = 2 * (nxp / 2 + 1), cn = rnxp / 2;
+  const long long rows = (long long)nyp * nzp;
+  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int j = (int)(row % nyp), k = (int)(row / nyp);
+    for (int h = threadIdx.x; h < cn; h += blockDim.x) {
+      const float v0 = err_corr_value(2 * h, j, k, nxp, nyp, nzp, corr_xy, grad_i, grad_j);
+      const float v1 = err_corr_value(2 * h + 1, j, k, nxp, nyp, nzp, corr_xy, grad_i, grad_j);
+      *reinterpret_cast<float2 *>(grid + row * rnxp + 2 * h) = make_float2(v0, v1);
     }
-    grid[p] = value;
   }
 }
 
@@ -259,10 +275,15 @@ int launch_merge_4d(const Merge4D &a, bool cov, hipStream_t s) {
   return CRAVA_HIP_OK;
 }
 
+// Workgroups of the row-walking fills: one per row up to 8192 (eight per CU
+// on 1024 CUs' worth of rows, the rest by the grid-stride loop).
+static unsigned fill_blocks(int nyp, int nzp) {
+  return (unsigned)std::max<long long>(1, std::min<long long>((long long)nyp * nzp, 8192));
+}
+
 int launch_fill_param_corr(const GridSet &gs, int nxp, int nyp, int nzp, const float *corr_xy, const float *circ_t,
                            float grad_i, float grad_j, hipStream_t s) {
-  const long long rsize = 2LL * (nxp / 2 + 1) * nyp * nzp;
-  hipLaunchKernelGGL(k_fill_param_corr, dim3(blocks_for(rsize)), dim3(256), 0, s, gs, nxp, nyp, nzp, corr_xy,
+  hipLaunchKernelGGL(k_fill_param_corr, dim3(fill_blocks(nyp, nzp)), dim3(256), 0, s, gs, nxp, nyp, nzp, corr_xy,
                      circ_t, grad_i, grad_j);
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
@@ -270,8 +291,7 @@ int launch_fill_param_corr(const GridSet &gs, int nxp, int nyp, int nzp, const f
 
 int launch_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *corr_xy, float grad_i, float grad_j,
                          hipStream_t s) {
-  const long long rsize = 2LL * (nxp / 2 + 1) * nyp * nzp;
-  hipLaunchKernelGGL(k_fill_err_corr, dim3(blocks_for(rsize)), dim3(256), 0, s, grid, nxp, nyp, nzp, corr_xy,
+  hipLaunchKernelGGL(k_fill_err_corr, dim3(fill_blocks(nyp, nzp)), dim3(256), 0, s, grid, nxp, nyp, nzp, corr_xy,
                      grad_i, grad_j);
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
