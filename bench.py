@@ -48,9 +48,10 @@ PEAK_HBM_GBS = 8000.0
 # Launch geometry of the specialised transform kernels' variants, as in
 # fft_static.hip: (pencils or row pairs per workgroup, threads, form) with
 # form "false" (LDS-staged), "true" (direct global I/O in the first and last
-# pass) or "pipe" (direct I/O in persistent workgroups, k_sfft_pipe).
+# pass), "pipe" (direct I/O in persistent workgroups, k_sfft_pipe) or "cols"
+# (direct I/O on pencil-fastest passes, k_sfft_cols).
 STRIDED_VARIANTS = ([(tx, th, dio) for dio in ("false", "true") for tx, th in ((16, 256), (16, 512), (8, 256), (32, 512))]
-                    + [(16, 256, "pipe"), (16, 512, "pipe")])
+                    + [(16, 256, "pipe"), (16, 512, "pipe"), (16, 512, "cols"), (16, 256, "cols")])
 ROW_VARIANTS = [(tx, th, dio) for dio in ("false", "true") for tx, th in ((8, 256), (16, 512), (4, 256))]
 
 
@@ -74,6 +75,9 @@ def kernel_identity(key, variants, nxp, nyp, nzp, ngrid, nt):
         if dio == "pipe":
             # persistent: the grid follows the device's occupancy
             return f"crava::k_sfft_pipe<{n}, {tx}, {th}, -1, float, float, float>", None
+        if dio == "cols":
+            return (f"crava::k_sfft_cols<{n}, {tx}, {th}, -1, float, float, float>",
+                    -(-ncols // tx) * nbatch * ngrid * th)
         return (f"crava::k_sfft_strided<{n}, {tx}, {th}, -1, {dio}, float, float, float>",
                 -(-ncols // tx) * nbatch * ngrid * th)
     mode = {"posterior_mixed": ("float", 1), "posterior_f32": ("float", 0), "posterior_f64": ("double", 0)}[key]
@@ -84,7 +88,8 @@ def profiled_traffic(name, grid):
     """HBM bytes per launch from the committed counter summaries (TRAFFIC lines
     of profiles/hbm_traffic_*.txt), or None.  A launch shape whose other I/O
     form (LDS-staged or direct first/last pass, the same grid accesses) was
-    the one profiled falls back to that form's figure."""
+    the one profiled falls back to that form's figure; the pencil-fastest
+    strided form to the direct-I/O strided form of its shape."""
     import glob
 
     def lookup(nm):
@@ -102,6 +107,11 @@ def profiled_traffic(name, grid):
         return best
 
     found = lookup(name)
+    if found is None and "k_sfft_cols<" in name:
+        # the same global accesses as the direct-I/O strided form of the shape
+        head, args = name.split("<", 1)
+        a = [x.strip() for x in args.rstrip(">").split(",")]
+        found = lookup("crava::k_sfft_strided<" + ", ".join(a[:4] + ["true"] + a[4:]) + ">")
     if found is None and ("k_sfft_strided<" in name or "k_srows_" in name):
         head, args = name.split("<", 1)
         a = [x.strip() for x in args.rstrip(">").split(",")]

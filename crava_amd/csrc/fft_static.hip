@@ -93,7 +93,8 @@ bool static_fft_geometry(int n, bool dbl, FftKind kind, int variant, StaticLaunc
   const int th = strided ? kSV[variant][1] : kRV[variant][1];
   const size_t eb = dbl ? 16 : 8;
   const size_t len = (size_t)n + (size_t)((n - 1) >> (dbl ? 4 : 5)) + 1;
-  const size_t pitch = len | 1;
+  // padded pencils, or N | 1 apart for the pencil-fastest form
+  const size_t pitch = (strided && kSV[variant][2] == 3) ? (size_t)(n | 1) : (len | 1);
   const size_t lds = eb * ((size_t)n + (size_t)tx * pitch);
   if (lds > max_lds()) return false;
   if (out) { out->tx = tx; out->threads = th; out->lds = lds; }

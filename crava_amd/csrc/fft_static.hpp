@@ -9,7 +9,7 @@ namespace crava {
 // Kinds of pass.
 enum FftKind { kStridedFwd = 0, kStridedInv = 1, kRowsR2C = 2, kRowsC2R = 3 };
 
-constexpr int kNumStridedVariants = 10;
+constexpr int kNumStridedVariants = 12;
 constexpr int kNumRowVariants = 6;
 
 struct StaticLaunch {

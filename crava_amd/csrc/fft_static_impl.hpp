@@ -86,6 +86,43 @@ __global__ void __launch_bounds__(THREADS) k_sfft_strided(GridSet gs, long long 
   }
 }
 
+// Direct-I/O strided pass on the pencil-fastest passes (stockham.hpp cpass):
+// thread tid keeps column tid % TX through every pass, the pencils sit N | 1
+// apart in LDS with no padding, and every LDS offset a thread uses is a
+// compile-time constant from its column's base.  The first pass loads its
+// butterflies straight from global memory (row segments of TX columns), the
+// last stores straight to it.
+template <int N, int TX, int THREADS, int DIR, typename T, typename SI, typename SO>
+__global__ void __launch_bounds__(THREADS) k_sfft_cols(GridSet gs, long long es, long long ncols, long long bstride,
+                                                       const cx<T> *__restrict__ twg) {
+  // (column block, batch, grid) of this workgroup, XCD-contiguous
+  const unsigned W = xcd_work_id(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                                 gridDim.x * gridDim.y * gridDim.z);
+  const unsigned bx = W % gridDim.x, by = (W / gridDim.x) % gridDim.y, bz = W / (gridDim.x * gridDim.y);
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw);
+  cx<T> *buf = tw + N;
+  const long long col0 = (long long)bx * TX;
+  const int ncol = (int)min((long long)TX, ncols - col0);
+  const long long off = (long long)by * bstride + col0;
+  // the first pass reads no twiddles; the barrier that ends it orders these
+  // writes before the later passes read them
+  for (int q = threadIdx.x; q < N; q += THREADS) tw[q] = twg[q];
+  const int c = (int)threadIdx.x % TX, j0 = (int)threadIdx.x / TX;
+  const bool ok = c < ncol;
+  const cx<SI> *ib = reinterpret_cast<const cx<SI> *>(gs.src[bz]) + off + min(c, ncol - 1);
+  cx<SO> *ob = reinterpret_cast<cx<SO> *>(gs.dst[bz]) + off + c;
+  // branch-free loads (clamped column, then select)
+  const auto ld = [&](int e) -> cx<T> {
+    const cx<SI> v = ib[(long long)e * es];
+    return ok ? mk((T)v.x, (T)v.y) : mk((T)0, (T)0);
+  };
+  const auto st = [&](int e, cx<T> v) {
+    if (ok) ob[(long long)e * es] = mk((SO)v.x, (SO)v.y);
+  };
+  cpasses<N, TX, THREADS, DIR, 0, false, 1, T>(buf + c * (N | 1), tw, ld, st, j0);
+}
+
 // Persistent form of the direct-I/O strided pass.  The tiles (column block,
 // batch, grid) are split into 8 contiguous ranges, one per XCD (workgroups
 // are dispatched round-robin over the XCDs); the gridDim.x / 8 workgroups of
@@ -356,9 +393,18 @@ __global__ void __launch_bounds__(THREADS) k_srows_c2r(GridSet gs, long long nro
 // Strided (TX, THREADS, form): form 0 stages the pencils through LDS before
 // and after the passes, form 1 loads the first pass and stores the last
 // straight from and to global memory, form 2 is form 1 in persistent
-// workgroups that prefetch the next tile (k_sfft_pipe).
-constexpr int kSV[kNumStridedVariants][3] = {{16, 256, 0}, {16, 512, 0}, {8, 256, 0}, {32, 512, 0}, {16, 256, 1},
-                                             {16, 512, 1}, {8, 256, 1},  {32, 512, 1}, {16, 256, 2}, {16, 512, 2}};
+// workgroups that prefetch the next tile (k_sfft_pipe), form 3 is form 1 on
+// the pencil-fastest passes (k_sfft_cols).
+constexpr int kSV[kNumStridedVariants][3] = {{16, 256, 0}, {16, 512, 0}, {8, 256, 0},  {32, 512, 0},
+                                             {16, 256, 1}, {16, 512, 1}, {8, 256, 1},  {32, 512, 1},
+                                             {16, 256, 2}, {16, 512, 2}, {16, 512, 3}, {16, 256, 3}};
+
+// LDS bytes of a strided variant: the twiddles and TX pencils (padded
+// pitch, or N | 1 for the pencil-fastest form).
+template <int N, typename T>
+constexpr size_t strided_lds(int tx, int form) {
+  return form == 3 ? sizeof(cx<T>) * ((size_t)N + (size_t)tx * (size_t)(N | 1)) : lds_bytes<N, T>(tx);
+}
 constexpr int kRV[kNumRowVariants][2] = {{8, 256}, {16, 512}, {4, 256}, {8, 256}, {16, 512}, {4, 256}};
 
 template <typename F>
@@ -393,7 +439,7 @@ template <int N, int V, int DIR, typename T, typename SI, typename SO>
 static int launch_strided_v(const GridSet &gs, long long es, long long ncols, long long nbatch, long long bstride,
                             const void *tw, hipStream_t s) {
   constexpr int TX = kSV[V][0], TH = kSV[V][1], FORM = kSV[V][2];
-  const size_t lds = lds_bytes<N, T>(TX);
+  const size_t lds = strided_lds<N, T>(TX, FORM);
   const unsigned gx = (unsigned)((ncols + TX - 1) / TX);
   int rc;
   if constexpr (FORM == 2) {
@@ -404,6 +450,10 @@ static int launch_strided_v(const GridSet &gs, long long es, long long ncols, lo
     if ((rc = persistent_grid(k, TH, lds, tiles, &grid))) return rc;
     hipLaunchKernelGGL(k, dim3(grid), dim3(TH), lds, s, gs, es, ncols, bstride, gx, (unsigned)nbatch,
                        reinterpret_cast<const cx<T> *>(tw));
+  } else if constexpr (FORM == 3) {
+    if ((rc = set_lds_attr(k_sfft_cols<N, TX, TH, DIR, T, SI, SO>, lds))) return rc;
+    hipLaunchKernelGGL((k_sfft_cols<N, TX, TH, DIR, T, SI, SO>), dim3(gx, (unsigned)nbatch, (unsigned)gs.n),
+                       dim3(TH), lds, s, gs, es, ncols, bstride, reinterpret_cast<const cx<T> *>(tw));
   } else {
     constexpr bool DIO = FORM == 1;
     if ((rc = set_lds_attr(k_sfft_strided<N, TX, TH, DIR, DIO, T, SI, SO>, lds))) return rc;
@@ -448,7 +498,9 @@ static int strided_any(int v, const GridSet &gs, long long es, long long ncols, 
     case 6: return launch_strided_v<N, 6, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
     case 7: return launch_strided_v<N, 7, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
     case 8: return launch_strided_v<N, 8, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
-    default: return launch_strided_v<N, 9, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    case 9: return launch_strided_v<N, 9, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    case 10: return launch_strided_v<N, 10, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
+    default: return launch_strided_v<N, 11, DIR, T, SI, SO>(gs, es, ncols, nbatch, bstride, tw, s);
   }
 }
 

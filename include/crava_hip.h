@@ -125,9 +125,10 @@ int crava_hip_fft_pass_many(crava_hip_ctx *ctx, void *const *grids, int ngrid, i
  * size; its contents are destroyed) and keep the fastest in the context's
  * plan; ms[] receives the forward + inverse time per axis.  Variants: v >= 0 a
  * compile-time specialised kernel (lengths 64, 80, 128, 160, 216, 256, 288,
- * 320, 512): 0..7 for the strided y/z passes, 0..5 for the x (row) pass, the
- * upper half of each range loading the first radix pass straight from HBM
- * and storing the last straight to it; v = -1..-6 the runtime-length kernel
+ * 320, 512): 0..11 for the strided y/z passes (4..7 loading the first radix
+ * pass straight from HBM and storing the last straight to it, 8..9 the same
+ * in persistent workgroups, 10..11 the same on pencil-fastest passes), 0..5
+ * for the x (row) pass (3..5 with direct I/O); v = -1..-6 the runtime-length kernel
  * with radix order (large-first / small-first) x pencils per workgroup
  * (<=64, <=16, <=8), and v = -7..-12 the same with the first radix pass
  * loading straight from global memory.

@@ -99,10 +99,11 @@ def test_every_launch_variant_and_autotuner(ctx, n, dbl):
     (oracle.fft_r2c_f64 if dbl else lambda a, *r: oracle.fft_r2c(a, *r))(ref, nxp, nyp, nzp, False)
     tol = 1e-13 if dbl else 1e-6
     prec = L.F64 if dbl else L.F32
-    # static variants: strided 0..9, rows 0..5 (strided 4..7 and rows 3..5
+    # static variants: strided 0..11, rows 0..5 (strided 4..7 and rows 3..5
     # load the first pass and store the last straight from/to global memory,
-    # strided 8..9 do so in persistent workgroups prefetching the next tile)
-    for v in range(-12, 10):
+    # strided 8..9 do so in persistent workgroups prefetching the next tile,
+    # strided 10..11 on pencil-fastest passes)
+    for v in range(-12, 12):
         try:
             set_fft_variants(ctx, nxp, nyp, nzp, prec, [min(v, 5), v, v])
         except L.CravaHipError:
@@ -116,7 +117,7 @@ def test_every_launch_variant_and_autotuner(ctx, n, dbl):
         assert rel_l2(t.cpu().numpy()[:, :, :nxp], g[:, :, :nxp]) < tol, v
     scratch = torch.zeros(g.shape, dtype=torch.float64 if dbl else torch.float32, device="cuda")
     var, ms = autotune_fft(ctx, scratch, nxp, nyp, nzp)
-    assert all(-12 <= x <= 9 for x in var) and all(m > 0 for m in ms)
+    assert all(-12 <= x <= 11 for x in var) and all(m > 0 for m in ms)
     t = torch.from_numpy(g.copy()).cuda()
     ctx.r2c(t, nxp, nyp, nzp, L.DATA)
     torch.cuda.synchronize()
@@ -149,7 +150,7 @@ def test_r2c_float_storage_double_arithmetic(ctx, n):
     ctx.r2c(t32, nxp, nyp, nzp, L.COVARIANCE, L.F32)
     torch.cuda.synchronize()
     err32 = rel_l2(to_complex(t32.cpu().numpy()), want)
-    for v in (None, 0, 3, 4, 7, 8, 9):
+    for v in (None, 0, 3, 4, 7, 8, 9, 10, 11):
         if v is not None:
             try:
                 set_fft_variants(ctx, nxp, nyp, nzp, L.MIXED, [min(v, 5), v, v])
@@ -197,7 +198,8 @@ def test_generic_variants_odd_lengths(ctx, n):
                          ids=lambda p: "x".join(map(str, p)) if isinstance(p, tuple) else str(p))
 def test_persistent_strided_variants_on_sets(ctx, n, ng):
     """Strided variants 8 and 9 (persistent workgroups over XCD-contiguous
-    tile ranges, the next tile's first pass prefetched) on a set of grids:
+    tile ranges, the next tile's first pass prefetched) and 10 and 11
+    (pencil-fastest passes) on a set of grids:
     partial column blocks (41 complex columns), sets of 5 grids, and a
     launch with more workgroups than tiles.  Spectra against the FP64
     oracle, within rounding of the one-workgroup-per-tile form of the same
@@ -214,7 +216,7 @@ def test_persistent_strided_variants_on_sets(ctx, n, ng):
             oracle.fft_r2c_f64(r, nxp, nyp, nzp, False)
             refs.append(to_complex(r))
         base = None
-        for v in (5, 8, 9):
+        for v in (5, 8, 9, 10, 11):
             set_fft_variants(ctx, nxp, nyp, nzp, prec, [5, v, v])
             ts = [torch.from_numpy(a.copy()).cuda() for a in gs]
             ctx.r2c_many(ts, nxp, nyp, nzp, L.DATA, prec)
