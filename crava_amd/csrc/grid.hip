@@ -67,46 +67,48 @@ __device__ __forceinline__ float err_corr_value(int i, int j, int k, int nxp, in
   return 0.0f;
 }
 
-// The fills walk the nyp nzp rows (j, k) of the grid, one row per workgroup
-// at a time (grid-stride), so j, k and what depends on them alone are
-// wave-uniform and no per-element index division remains; each thread
-// writes two consecutive x values as one float2 (rnxp is even).
+// The fills write two consecutive x values per thread as one float2 (rnxp
+// is even, so a pair never crosses a row): consecutive threads take
+// consecutive pairs, and the pair index is split into (x, y, z) with 32-bit
+// arithmetic (IDX = unsigned) whenever the grid has fewer than 2^32 pairs,
+// 64-bit otherwise.
 
 // fillInParamCorr followed by multiplyByScalar(scale[g])
 // (src/seismicparametersholder.cpp:210-215) for each grid g of the set: the
 // same correlation written to up to 16 grids.
+template <typename IDX>
 __global__ void __launch_bounds__(256) k_fill_param_corr(GridSet gs, int nxp, int nyp, int nzp,
                                                          const float *__restrict__ corr_xy,
                                                          const float *__restrict__ circ_t, float grad_i,
                                                          float grad_j) {
-  const int rnxp = 2 * (nxp / 2 + 1), cn = rnxp / 2;
-  const long long rows = (long long)nyp * nzp;
-  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
-    const int j = (int)(row % nyp), k = (int)(row / nyp);
-    for (int h = threadIdx.x; h < cn; h += blockDim.x) {
-      const float v0 = param_corr_value(2 * h, j, k, nxp, nyp, nzp, corr_xy, circ_t, grad_i, grad_j);
-      const float v1 = param_corr_value(2 * h + 1, j, k, nxp, nyp, nzp, corr_xy, circ_t, grad_i, grad_j);
-      const long long p = row * rnxp + 2 * h;
-      for (int g = 0; g < gs.n; ++g) {
-        const float sc = (float)gs.scale[g];
-        *reinterpret_cast<float2 *>(static_cast<float *>(gs.dst[g]) + p) = make_float2(v0 * sc, v1 * sc);
-      }
+  const IDX cn = (IDX)(nxp / 2 + 1);
+  const IDX npair = cn * (IDX)nyp * (IDX)nzp;
+  for (IDX q = (IDX)blockIdx.x * 256u + threadIdx.x; q < npair; q += (IDX)gridDim.x * 256u) {
+    const IDX row = q / cn;
+    const int h = (int)(q - row * cn);
+    const int j = (int)(row % (IDX)nyp), k = (int)(row / (IDX)nyp);
+    const float v0 = param_corr_value(2 * h, j, k, nxp, nyp, nzp, corr_xy, circ_t, grad_i, grad_j);
+    const float v1 = param_corr_value(2 * h + 1, j, k, nxp, nyp, nzp, corr_xy, circ_t, grad_i, grad_j);
+    for (int g = 0; g < gs.n; ++g) {
+      const float sc = (float)gs.scale[g];
+      reinterpret_cast<float2 *>(gs.dst[g])[q] = make_float2(v0 * sc, v1 * sc);
     }
   }
 }
 
+template <typename IDX>
 __global__ void __launch_bounds__(256) k_fill_err_corr(float *grid, int nxp, int nyp, int nzp,
                                                        const float *__restrict__ corr_xy, float grad_i,
                                                        float grad_j) {
-  const int rnxp = 2 * (nxp / 2 + 1), cn = rnxp / 2;
-  const long long rows = (long long)nyp * nzp;
-  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
-    const int j = (int)(row % nyp), k = (int)(row / nyp);
-    for (int h = threadIdx.x; h < cn; h += blockDim.x) {
-      const float v0 = err_corr_value(2 * h, j, k, nxp, nyp, nzp, corr_xy, grad_i, grad_j);
-      const float v1 = err_corr_value(2 * h + 1, j, k, nxp, nyp, nzp, corr_xy, grad_i, grad_j);
-      *reinterpret_cast<float2 *>(grid + row * rnxp + 2 * h) = make_float2(v0, v1);
-    }
+  const IDX cn = (IDX)(nxp / 2 + 1);
+  const IDX npair = cn * (IDX)nyp * (IDX)nzp;
+  for (IDX q = (IDX)blockIdx.x * 256u + threadIdx.x; q < npair; q += (IDX)gridDim.x * 256u) {
+    const IDX row = q / cn;
+    const int h = (int)(q - row * cn);
+    const int j = (int)(row % (IDX)nyp), k = (int)(row / (IDX)nyp);
+    const float v0 = err_corr_value(2 * h, j, k, nxp, nyp, nzp, corr_xy, grad_i, grad_j);
+    const float v1 = err_corr_value(2 * h + 1, j, k, nxp, nyp, nzp, corr_xy, grad_i, grad_j);
+    reinterpret_cast<float2 *>(grid)[q] = make_float2(v0, v1);
   }
 }
 
@@ -275,24 +277,32 @@ int launch_merge_4d(const Merge4D &a, bool cov, hipStream_t s) {
   return CRAVA_HIP_OK;
 }
 
-// Workgroups of the row-walking fills: one per row up to 8192 (eight per CU
-// on 1024 CUs' worth of rows, the rest by the grid-stride loop).
-static unsigned fill_blocks(int nyp, int nzp) {
-  return (unsigned)std::max<long long>(1, std::min<long long>((long long)nyp * nzp, 8192));
-}
+// Value pairs of a padded grid (32-bit indices below 2^32 - 2^28 pairs: the
+// grid-stride increment, at most 2^20 workgroups of 256, cannot overflow).
+static long long fill_pairs(int nxp, int nyp, int nzp) { return (long long)(nxp / 2 + 1) * nyp * nzp; }
 
 int launch_fill_param_corr(const GridSet &gs, int nxp, int nyp, int nzp, const float *corr_xy, const float *circ_t,
                            float grad_i, float grad_j, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_param_corr, dim3(fill_blocks(nyp, nzp)), dim3(256), 0, s, gs, nxp, nyp, nzp, corr_xy,
-                     circ_t, grad_i, grad_j);
+  const long long n = fill_pairs(nxp, nyp, nzp);
+  if (n < (1LL << 32) - 256LL * 1048576LL)
+    hipLaunchKernelGGL(k_fill_param_corr<unsigned>, dim3(blocks_for(n)), dim3(256), 0, s, gs, nxp, nyp, nzp, corr_xy,
+                       circ_t, grad_i, grad_j);
+  else
+    hipLaunchKernelGGL(k_fill_param_corr<unsigned long long>, dim3(blocks_for(n)), dim3(256), 0, s, gs, nxp, nyp, nzp,
+                       corr_xy, circ_t, grad_i, grad_j);
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
 
 int launch_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *corr_xy, float grad_i, float grad_j,
                          hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_err_corr, dim3(fill_blocks(nyp, nzp)), dim3(256), 0, s, grid, nxp, nyp, nzp, corr_xy,
-                     grad_i, grad_j);
+  const long long n = fill_pairs(nxp, nyp, nzp);
+  if (n < (1LL << 32) - 256LL * 1048576LL)
+    hipLaunchKernelGGL(k_fill_err_corr<unsigned>, dim3(blocks_for(n)), dim3(256), 0, s, grid, nxp, nyp, nzp, corr_xy,
+                       grad_i, grad_j);
+  else
+    hipLaunchKernelGGL(k_fill_err_corr<unsigned long long>, dim3(blocks_for(n)), dim3(256), 0, s, grid, nxp, nyp, nzp,
+                       corr_xy, grad_i, grad_j);
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }
