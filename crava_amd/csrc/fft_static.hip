@@ -46,7 +46,7 @@ bool static_variant_available(int n, FftKind kind, int variant) {
   if (!strided && (variant < 0 || variant >= kNumRowVariants)) return false;
   if (static_fft_full(n)) return true;
   if (ext_group(n) < 0) return false;
-  return strided ? (variant == 4 || variant == 5 || variant == 10) : (variant == 3 || variant == 5);
+  return strided ? (variant == 4 || variant == 5 || variant >= 10) : (variant == 3 || variant == 5);
 }
 
 static size_t max_lds() {

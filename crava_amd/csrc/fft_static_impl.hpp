@@ -542,18 +542,20 @@ static int launch_store(int io, FftKind kind, int v, const GridSet &gs, long lon
 }
 
 // The direct-I/O launch shapes only: strided variants 4 and 5 (16 columns,
-// 256 or 512 threads) and 10 (16 columns, 512 threads, pencil-fastest
-// passes), row variants 3 and 5 (8 or 4 row pairs, 256 threads).
+// 256 or 512 threads) and 10 and 11 (the same on pencil-fastest passes),
+// row variants 3 and 5 (8 or 4 row pairs, 256 threads).
 template <int N, typename T, typename S>
 static int ext_kinds(FftKind kind, int v, const GridSet &gs, long long es, long long ncols, long long nbatch,
                      long long bstride, long long nrows, int rnxp, const void *tw, hipStream_t s) {
   switch (kind) {
     case kStridedFwd:
       if (v == 10) return launch_strided_v<N, 10, -1, T, S, S>(gs, es, ncols, nbatch, bstride, tw, s);
+      if (v == 11) return launch_strided_v<N, 11, -1, T, S, S>(gs, es, ncols, nbatch, bstride, tw, s);
       return v == 4 ? launch_strided_v<N, 4, -1, T, S, S>(gs, es, ncols, nbatch, bstride, tw, s)
                     : launch_strided_v<N, 5, -1, T, S, S>(gs, es, ncols, nbatch, bstride, tw, s);
     case kStridedInv:
       if (v == 10) return launch_strided_v<N, 10, 1, T, S, S>(gs, es, ncols, nbatch, bstride, tw, s);
+      if (v == 11) return launch_strided_v<N, 11, 1, T, S, S>(gs, es, ncols, nbatch, bstride, tw, s);
       return v == 4 ? launch_strided_v<N, 4, 1, T, S, S>(gs, es, ncols, nbatch, bstride, tw, s)
                     : launch_strided_v<N, 5, 1, T, S, S>(gs, es, ncols, nbatch, bstride, tw, s);
     case kRowsR2C:

@@ -241,7 +241,7 @@ def test_persistent_strided_variants_on_sets(ctx, n, ng):
 def test_extended_lengths_direct_io_variants(ctx, n):
     """The specialised kernels of the extended 7-smooth lengths (32..320
     beyond the main ones; odd x lengths included): every direct-I/O variant
-    they carry (strided 4, 5, 10; rows 3, 5), float, float storage with double
+    they carry (strided 4, 5, 10, 11; rows 3, 5), float, float storage with double
     arithmetic and double, on a set of 3 grids, against the FP64 oracle and
     through the round trip; variants they do not carry are refused."""
     from crava_amd.core import set_fft_variants
@@ -256,7 +256,7 @@ def test_extended_lengths_direct_io_variants(ctx, n):
             r = a.astype(np.float64)
             oracle.fft_r2c_f64(r, nxp, nyp, nzp, False)
             refs.append(to_complex(r))
-        for vr, vs in ((3, 4), (5, 5), (3, 5), (5, 4), (3, 10)):
+        for vr, vs in ((3, 4), (5, 5), (3, 5), (5, 4), (3, 10), (5, 11)):
             set_fft_variants(ctx, nxp, nyp, nzp, prec, [vr, vs, vs])
             ts = [torch.from_numpy(a.copy()).cuda() for a in gs]
             ctx.r2c_many(ts, nxp, nyp, nzp, L.DATA, prec)
