@@ -494,6 +494,11 @@ int crava_hip_post_create(crava_hip_ctx *ctx, const crava_hip_post_params *param
     e = hipMemcpy(h->dev.tabf, tf.data(), sizeof(crava::PerK<float>) * p.nzp, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(h->dev.tabd, td.data(), sizeof(crava::PerK<double>) * p.nzp, hipMemcpyHostToDevice);
+  // the in-band k-planes first: the solve runs over those, a light kernel
+  // over the rest
+  const std::vector<int> kmap = crava::posterior_kmap(p.nzp, invert, &h->dev.nk_in);
+  if (e == hipSuccess) e = hipMalloc(&h->dev.kmap, sizeof(int) * p.nzp);
+  if (e == hipSuccess) e = hipMemcpy(h->dev.kmap, kmap.data(), sizeof(int) * p.nzp, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     set_error(std::string("posterior table upload: ") + hipGetErrorString(e));
     crava_hip_post_destroy(h);
@@ -507,6 +512,7 @@ int crava_hip_post_destroy(crava_hip_post *post) {
   if (!post) return CRAVA_HIP_OK;
   if (post->dev.tabf) (void)hipFree(post->dev.tabf);
   if (post->dev.tabd) (void)hipFree(post->dev.tabd);
+  if (post->dev.kmap) (void)hipFree(post->dev.kmap);
   if (post->nll_tab) (void)hipFree(post->nll_tab);
   if (post->nll_klist) (void)hipFree(post->nll_klist);
   if (post->nll_partial) (void)hipFree(post->nll_partial);

@@ -52,6 +52,7 @@ struct PostArgs {
   double s0d[6];
   int s0nz[6];
   float thr;
+  const int *kmap;  // k-plane of grid row blockIdx.y (with an offset), or NULL
 };
 
 template <typename T>
@@ -525,11 +526,28 @@ __device__ __forceinline__ void post_any(const PostArgs<G> &a, int k, long long 
 }
 
 template <typename G, int NT, int MODE>
-__global__ void __launch_bounds__(256) k_posterior(PostArgs<G> a) {
+__global__ void __launch_bounds__(256) k_posterior(PostArgs<G> a, int koff) {
   // consecutive workgroups go to different XCDs, which spreads the in-band
   // k-planes (a contiguous range of k) over all of them
   const long long ip = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  post_any<G, NT, MODE>(a, blockIdx.y, ip, ip - (threadIdx.x & 63));
+  const int k = a.kmap ? a.kmap[koff + (int)blockIdx.y] : (int)blockIdx.y;
+  post_any<G, NT, MODE>(a, k, ip, ip - (threadIdx.x & 63));
+}
+
+// Out-of-band k-planes (kmap[koff + blockIdx.y]): the cleaned prior passes
+// through, as post_point / post_point_mixed write it there.  A kernel of its
+// own so these planes, a plain stream of writes, run at the occupancy of a
+// few registers rather than at that of the in-band solve.
+template <typename G>
+__global__ void __launch_bounds__(256) k_posterior_prior(PostArgs<G> a, int koff) {
+  const long long ip = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= a.plane) return;
+  const int k = a.kmap[koff + (int)blockIdx.y];
+  const long long p = (long long)k * a.plane + ip;
+  G S[6];
+  load_prior(a, p, S);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) a.cov[q][p] = mk(S[q], (G)0);
 }
 
 // Mixed form, FP64 kernel: every queued wave; waves stride over the queue.
@@ -668,6 +686,17 @@ void build_perk(const PostProblem &pp, const unsigned char *invert, const float 
   }
 }
 
+std::vector<int> posterior_kmap(int nzp, const unsigned char *invert, int *nk_in) {
+  std::vector<int> m;
+  m.reserve((size_t)nzp);
+  for (int k = 0; k < nzp; ++k)
+    if (invert[k]) m.push_back(k);
+  *nk_in = (int)m.size();
+  for (int k = 0; k < nzp; ++k)
+    if (!invert[k]) m.push_back(k);
+  return m;
+}
+
 long long defer_capacity(const PostProblem &pp, const unsigned char *invert) {
   const long long plane = (long long)(pp.nxp / 2 + 1) * pp.nyp;
   long long nk = 0;
@@ -680,26 +709,32 @@ static unsigned queue_blocks(long long cap) {
   return (unsigned)std::max<long long>(1, std::min<long long>(1024, (cap + 3) / 4));
 }
 
+// With a k-plane map: the solve over the in-band planes, the prior
+// pass-through over the others; without: the solve kernel over every plane.
 template <typename G, int NT, int MODE>
-static void launch_nt(const PostArgs<G> &a, int nzp, long long cap, hipStream_t s) {
+static void launch_nt(const PostArgs<G> &a, int nzp, int nk_in, long long cap, hipStream_t s) {
   const unsigned gx = (unsigned)((a.plane + 255) / 256);
   if constexpr (MODE == 1) hipLaunchKernelGGL(k_zero_queue_count, dim3(1), dim3(1), 0, s, a.defer_count);
-  hipLaunchKernelGGL((k_posterior<G, NT, MODE>), dim3(gx, (unsigned)nzp), dim3(256), 0, s, a);
+  const int nsolve = a.kmap ? nk_in : nzp;
+  if (nsolve > 0)
+    hipLaunchKernelGGL((k_posterior<G, NT, MODE>), dim3(gx, (unsigned)nsolve), dim3(256), 0, s, a, 0);
+  if (a.kmap && nzp > nk_in)
+    hipLaunchKernelGGL((k_posterior_prior<G>), dim3(gx, (unsigned)(nzp - nk_in)), dim3(256), 0, s, a, nk_in);
   if constexpr (MODE == 1)
     hipLaunchKernelGGL((k_posterior_queued<NT>), dim3(queue_blocks(cap)), dim3(256), 0, s, a);
 }
 
 template <typename G, int MODE>
-static int dispatch(const PostArgs<G> &a, int nt, int nzp, long long cap, hipStream_t s) {
+static int dispatch(const PostArgs<G> &a, int nt, int nzp, int nk_in, long long cap, hipStream_t s) {
   switch (nt) {
-    case 1: launch_nt<G, 1, MODE>(a, nzp, cap, s); break;
-    case 2: launch_nt<G, 2, MODE>(a, nzp, cap, s); break;
-    case 3: launch_nt<G, 3, MODE>(a, nzp, cap, s); break;
-    case 4: launch_nt<G, 4, MODE>(a, nzp, cap, s); break;
-    case 5: launch_nt<G, 5, MODE>(a, nzp, cap, s); break;
-    case 6: launch_nt<G, 6, MODE>(a, nzp, cap, s); break;
-    case 7: launch_nt<G, 7, MODE>(a, nzp, cap, s); break;
-    case 8: launch_nt<G, 8, MODE>(a, nzp, cap, s); break;
+    case 1: launch_nt<G, 1, MODE>(a, nzp, nk_in, cap, s); break;
+    case 2: launch_nt<G, 2, MODE>(a, nzp, nk_in, cap, s); break;
+    case 3: launch_nt<G, 3, MODE>(a, nzp, nk_in, cap, s); break;
+    case 4: launch_nt<G, 4, MODE>(a, nzp, nk_in, cap, s); break;
+    case 5: launch_nt<G, 5, MODE>(a, nzp, nk_in, cap, s); break;
+    case 6: launch_nt<G, 6, MODE>(a, nzp, nk_in, cap, s); break;
+    case 7: launch_nt<G, 7, MODE>(a, nzp, nk_in, cap, s); break;
+    case 8: launch_nt<G, 8, MODE>(a, nzp, nk_in, cap, s); break;
     default: set_error("ntheta must be 1..8"); return CRAVA_HIP_E_INVALID;
   }
   CRAVA_HIP_CHECK(hipGetLastError());
@@ -735,6 +770,7 @@ static void fill_args(PostArgs<G> &a, const PostDevice &pd, void *const *mu, voi
     a.s0d[q] = s;
   }
   a.thr = (float)pp.mixed_threshold;
+  a.kmap = pd.kmap;
 }
 
 int launch_posterior(const PostDevice &pd, int precision, void *const *mu, void *const *data, void *const *cov,
@@ -744,7 +780,7 @@ int launch_posterior(const PostDevice &pd, int precision, void *const *mu, void 
   if (precision == CRAVA_HIP_F64) {
     PostArgs<double> a{};
     fill_args(a, pd, mu, data, cov, errcorr, counters, corr);
-    return dispatch<double, 0>(a, nt, pd.pp.nzp, 0, s);
+    return dispatch<double, 0>(a, nt, pd.pp.nzp, pd.nk_in, 0, s);
   }
   PostArgs<float> a{};
   fill_args(a, pd, mu, data, cov, errcorr, counters, corr);
@@ -755,9 +791,9 @@ int launch_posterior(const PostDevice &pd, int precision, void *const *mu, void 
     }
     a.defer_list = defer->list;
     a.defer_count = defer->count;
-    return dispatch<float, 1>(a, nt, pd.pp.nzp, defer->cap, s);
+    return dispatch<float, 1>(a, nt, pd.pp.nzp, pd.nk_in, defer->cap, s);
   }
-  return dispatch<float, 0>(a, nt, pd.pp.nzp, 0, s);
+  return dispatch<float, 0>(a, nt, pd.pp.nzp, pd.nk_in, 0, s);
 }
 
 // ---------------------------------------------------------------------------

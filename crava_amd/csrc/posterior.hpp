@@ -59,7 +59,14 @@ struct PostDevice {
   PerK<float> *tabf = nullptr;   // device, nzp entries
   PerK<double> *tabd = nullptr;  // device, nzp entries
   long long defer_need = 0;      // defer_capacity of the problem
+  // device, nzp entries: the in-band k-planes in increasing order, then the
+  // out-of-band ones (or NULL: one launch over every plane); nk_in in band
+  int *kmap = nullptr;
+  int nk_in = 0;
 };
+
+// kmap of a problem from its frequency gate (invert[k] != 0: in band).
+std::vector<int> posterior_kmap(int nzp, const unsigned char *invert, int *nk_in);
 
 // precision: 0 = FP32 (grids complex64, the reference's float algorithm),
 // 1 = mixed (complex64 grids; FP32 square-root information solve, and the
