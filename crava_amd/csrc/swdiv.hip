@@ -520,10 +520,17 @@ void swd_angle_tables(int nz, int nzp, const float *wavelet, float wnorm, const 
   int lo = (int)std::floor(low_cut / d_om), hi = (int)std::ceil(high_cut / d_om);
   lo = std::max(lo, 0);
   hi = std::min(hi, cn);
-  std::vector<double> d3(cn);
-  for (int m = 0; m < cn; ++m) {
-    const long double t = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)nzp;
-    d3[m] = (double)((2.0L * cosl(t) - 2.0L) * sinl(t));
+  // D3(m) depends on nzp alone: kept per length (and per thread) across
+  // angles and calls
+  thread_local int d3_n = 0;
+  thread_local std::vector<double> d3;
+  if (d3_n != nzp) {
+    d3.assign(cn, 0.0);
+    for (int m = 0; m < cn; ++m) {
+      const long double t = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)nzp;
+      d3[m] = (double)((2.0L * cosl(t) - 2.0L) * sinl(t));
+    }
+    d3_n = nzp;
   }
   const double fac = 1.0 / (double)nzp;
   double n2 = 0.0;

@@ -18,21 +18,44 @@ struct c64 {
 
 // Half spectrum (nzp/2+1 bins) of a real series, forward sign, double
 // accumulation with exactly reduced twiddle angles.
+// cos and sin of 2 pi m / n, m < n, evaluated in long double and rounded to
+// double; kept per length (and per thread), since a workload's calls share
+// a few lengths and the long-double evaluation costs more than the sums.
+struct TrigTable {
+  std::vector<double> cs, sn;
+};
+
+const TrigTable &trig_table(int n) {
+  thread_local std::vector<std::pair<int, TrigTable>> cache;
+  for (const auto &e : cache)
+    if (e.first == n) return e.second;
+  TrigTable t;
+  t.cs.resize(n);
+  t.sn.resize(n);
+  for (int m = 0; m < n; ++m) {
+    const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+    t.cs[m] = (double)cosl(a);
+    t.sn[m] = (double)sinl(a);
+  }
+  if (cache.size() >= 16) cache.erase(cache.begin());
+  cache.emplace_back(n, std::move(t));
+  return cache.back().second;
+}
+
 std::vector<c64> half_spectrum(const float *w, int nzp) {
   const int cn = nzp / 2 + 1;
-  std::vector<double> cs(nzp), sn(nzp);
-  for (int m = 0; m < nzp; ++m) {
-    const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)nzp;
-    cs[m] = (double)cosl(a);
-    sn[m] = (double)sinl(a);
-  }
+  const TrigTable &tt = trig_table(nzp);
+  const double *cs = tt.cs.data(), *sn = tt.sn.data();
   std::vector<c64> out(cn);
   for (int k = 0; k < cn; ++k) {
     double re = 0.0, im = 0.0;
+    // m = k t mod nzp, stepped by k
+    int m = 0;
     for (int t = 0; t < nzp; ++t) {
-      const int m = (int)(((long long)k * t) % nzp);
       re += (double)w[t] * cs[m];
       im -= (double)w[t] * sn[m];
+      m += k;
+      if (m >= nzp) m -= nzp;
     }
     out[k].re = (float)re;
     out[k].im = (float)im;
