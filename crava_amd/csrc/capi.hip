@@ -1290,14 +1290,22 @@ int crava_hip_local_noise_correct(crava_hip_ctx *ctx, const crava_hip_local_nois
   crava::lnc_inv_sqrt_host(sd, 0.0000001, c.sdi);  // src/crava.cpp:2294-2307
   crava::lnc_inv_sqrt_host(smd, 0.0, c.old);       // :2376-2385
   const size_t ntr = (size_t)p->nx * p->ny;
-  // Grid2D::FindMin(RMISSING), libs/nrlib/grid/grid2d.hpp:182-191
+  // Grid2D::FindMin(RMISSING), libs/nrlib/grid/grid2d.hpp:182-191: the
+  // smallest value that is not RMISSING, or RMISSING when every value is.
+  // Written branch-free (a select and a min per value) so the host compiler
+  // vectorises it.
   const double missing = -99999.000;
   for (int a = 0; a < nt; ++a) {
     const double *tab = p->noiseScale[a];
-    double mn = tab[0];
-    for (size_t t = 0; t < ntr; ++t)
-      if ((mn == missing || tab[t] < mn) && tab[t] != missing) mn = tab[t];
-    c.min_scale[a] = mn;
+    double mn = HUGE_VAL;
+    int found = 0;
+    for (size_t t = 0; t < ntr; ++t) {
+      const bool ok = tab[t] != missing;
+      const double v = ok ? tab[t] : HUGE_VAL;
+      mn = v < mn ? v : mn;
+      found |= ok;
+    }
+    c.min_scale[a] = found ? mn : missing;
   }
   void *ws = nullptr;
   int rc;
