@@ -147,3 +147,45 @@ def test_posterior_every_ntheta_vs_oracle(ctx, nt):
     assert n64 > 0
     for a, b in zip(gc, oc):
         assert rel_l2(_cplx(a), _cplx(b)) < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gate", ["none", "all"])
+def test_posterior_frequency_gate_extremes(ctx, gate):
+    """The solve kernel runs over the in-band k-planes and a pass-through
+    kernel over the others: with no plane in band every covariance is the
+    cleaned prior and means and data are untouched (float, mixed and FP64);
+    with every plane in band no pass-through launch happens (float against
+    the float oracle on the same gate, failed factorisations included)."""
+    print(f"posterior gate {gate}", flush=True)
+    prob = synth.make_problem(synth.config("small"), "cpu")
+    cfg = prob.cfg
+    grids = fourier_inputs(prob)
+    tabs = dict(problem_tables(prob))
+    tabs["invert"] = np.full_like(tabs["invert"], 0 if gate == "none" else 1)
+    mu, d, cv, ec = [x if not isinstance(x, list) else [g.copy() for g in x] for x in grids]
+    ec = ec.copy()
+    nf_ref = oracle.posterior(cfg.nxp, cfg.nyp, cfg.nzp, prob.A, prob.err_theta_cov, cfg.wnc, prob.sigma0,
+                              tabs, mu, d, cv, ec)
+    for prec in ((L.F32, L.MIXED, L.F64) if gate == "none" else (L.F32,)):
+        src = grids if prec != L.F64 else [as_f64(grids[0]), as_f64(grids[1]), as_f64(grids[2]),
+                                           as_f64([grids[3]])[0]]
+        tm = [torch.from_numpy(g.copy()).cuda() for g in src[0]]
+        td = [torch.from_numpy(g.copy()).cuda() for g in src[1]]
+        tc = [torch.from_numpy(g.copy()).cuda() for g in src[2]]
+        te = torch.from_numpy(src[3].copy()).cuda()
+        post = Posterior(ctx, synth.post_params(prob, 1e-3), tabs)
+        nf, _ = post.run(prec, tm, td, tc, te, counts=True)
+        torch.cuda.synchronize()
+        post.close()
+        if gate == "none":
+            assert nf == 0 and nf_ref == 0
+            for t, g in zip(tm + td, src[0] + src[1]):
+                assert np.array_equal(t.cpu().numpy(), g)
+        else:
+            assert nf == nf_ref
+        tol = 2e-5 if prec != L.F64 else 1e-6
+        for t, b in zip(tm, mu):
+            assert rel_l2(_cplx(t.cpu().numpy()).astype(np.complex128), _cplx(b)) < tol, prec
+        for t, b in zip(tc, cv):
+            assert rel_l2(_cplx(t.cpu().numpy()).astype(np.complex128), _cplx(b)) < tol, prec
