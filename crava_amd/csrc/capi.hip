@@ -497,6 +497,7 @@ int crava_hip_post_create(crava_hip_ctx *ctx, const crava_hip_post_params *param
   // the in-band k-planes first: the solve runs over those, a light kernel
   // over the rest
   const std::vector<int> kmap = crava::posterior_kmap(p.nzp, invert, &h->dev.nk_in);
+  h->dev.invert.assign(invert, invert + p.nzp);
   if (e == hipSuccess) e = hipMalloc(&h->dev.kmap, sizeof(int) * p.nzp);
   if (e == hipSuccess) e = hipMemcpy(h->dev.kmap, kmap.data(), sizeof(int) * p.nzp, hipMemcpyHostToDevice);
   if (e != hipSuccess) {

@@ -599,6 +599,24 @@ __global__ void __launch_bounds__(256) k_posterior_queued_batch(const PostArgs<f
 // the largest ntheta for every tile; k_posterior_class below is the same
 // loop over the tiles of one ntheta.  Tiles start at multiples of 64 points,
 // so the waves are those of the single-volume kernel.
+// Out-of-band tiles [t0, t1) of a batch: the cleaned prior passes through.
+template <typename G>
+__global__ void __launch_bounds__(256) k_posterior_prior_tiles(const PostArgs<G> *__restrict__ vols,
+                                                               const int4 *__restrict__ tiles, long long t0,
+                                                               long long t1) {
+  for (long long t = t0 + blockIdx.x; t < t1; t += gridDim.x) {
+    const int4 tl = tiles[t];
+    const PostArgs<G> &a = vols[tl.x];
+    const long long ip = (long long)tl.z + threadIdx.x;
+    if (ip >= a.plane) continue;
+    const long long p = (long long)tl.y * a.plane + ip;
+    G S[6];
+    load_prior(a, p, S);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a.cov[q][p] = mk(S[q], (G)0);
+  }
+}
+
 template <typename G, int NT, int MODE>
 __global__ void __launch_bounds__(256) k_posterior_class(const PostArgs<G> *__restrict__ vols,
                                                          const int4 *__restrict__ tiles, long long t0,
@@ -832,9 +850,12 @@ int PostBatch::init(int prec, const PostVolume *vols, int n) {
       off += vq[v].cap;
     }
   }
+  inv.assign(n, {});
   for (int v = 0; v < n; ++v) {
     const PostVolume &pv = vols[v];
     const PostProblem &pp = pv.pd->pp;
+    inv[v] = pv.pd->invert;
+    if ((int)inv[v].size() != pp.nzp) inv[v].assign(pp.nzp, 1);  // no gate kept: every plane through the solve
     nt[v] = pp.ntheta;
     nzp[v] = pp.nzp;
     plane[v] = (long long)(pp.nxp / 2 + 1) * pp.nyp;
@@ -865,17 +886,23 @@ int PostBatch::set_launch(int points_per_block, int nblocks) {
     return CRAVA_HIP_E_INVALID;
   }
   if (points_per_block != ppb || !tiles) {
-    // tiles ordered by ntheta, so each ntheta class is one contiguous segment
+    // in-band tiles ordered by ntheta, so each ntheta class is one
+    // contiguous segment; then every out-of-band tile
     std::vector<int4> h;
     for (int c = 1; c <= kMaxTheta; ++c) {
       seg[c - 1] = (long long)h.size();
       for (int v = 0; v < nvol; ++v) {
         if (nt[v] != c) continue;
         for (int k = 0; k < nzp[v]; ++k)
-          for (long long p0 = 0; p0 < plane[v]; p0 += points_per_block) h.push_back(make_int4(v, k, (int)p0, c));
+          if (inv[v][k])
+            for (long long p0 = 0; p0 < plane[v]; p0 += points_per_block) h.push_back(make_int4(v, k, (int)p0, c));
       }
     }
     seg[kMaxTheta] = (long long)h.size();
+    for (int v = 0; v < nvol; ++v)
+      for (int k = 0; k < nzp[v]; ++k)
+        if (!inv[v][k])
+          for (long long p0 = 0; p0 < plane[v]; p0 += points_per_block) h.push_back(make_int4(v, k, (int)p0, nt[v]));
     if (tiles) (void)hipFree(tiles);
     tiles = nullptr;
     CRAVA_HIP_CHECK(hipMalloc(&tiles, sizeof(int4) * h.size()));
@@ -925,23 +952,38 @@ static int launch_classes(const PostBatch &b, const hipStream_t *streams, int ns
 
 int PostBatch::launch(const hipStream_t *streams, int nstreams) const {
   if (ntiles < 1) return CRAVA_HIP_OK;
+  // the out-of-band tiles, on the last stream
+  const long long p0 = seg[kMaxTheta], np = ntiles - p0;
+  if (np > 0) {
+    const long long g = blocks > 0 ? std::min<long long>(blocks, np) : np;
+    hipStream_t s = streams[nstreams - 1];
+    if (precision == CRAVA_HIP_F64)
+      hipLaunchKernelGGL((k_posterior_prior_tiles<double>), dim3((unsigned)g), dim3((unsigned)ppb), 0, s,
+                         reinterpret_cast<const PostArgs<double> *>(args), tiles, p0, ntiles);
+    else
+      hipLaunchKernelGGL((k_posterior_prior_tiles<float>), dim3((unsigned)g), dim3((unsigned)ppb), 0, s,
+                         reinterpret_cast<const PostArgs<float> *>(args), tiles, p0, ntiles);
+    CRAVA_HIP_CHECK(hipGetLastError());
+  }
   if (form == kFormPerClass) {
     if (precision == CRAVA_HIP_F64) return launch_classes<double, 0>(*this, streams, nstreams);
     if (precision == CRAVA_HIP_MIXED) return launch_classes<float, 1>(*this, streams, nstreams);
     return launch_classes<float, 0>(*this, streams, nstreams);
   }
-  const long long g = blocks > 0 ? std::min<long long>(blocks, ntiles) : ntiles;
+  const long long ns = seg[kMaxTheta];
+  if (ns < 1) return CRAVA_HIP_OK;
+  const long long g = blocks > 0 ? std::min<long long>(blocks, ns) : ns;
   const dim3 grid((unsigned)g), block((unsigned)ppb);
   hipStream_t s = streams[0];
   if (precision == CRAVA_HIP_F64)
     hipLaunchKernelGGL((k_posterior_batch<double, 0>), grid, block, 0, s,
-                       reinterpret_cast<const PostArgs<double> *>(args), tiles, ntiles);
+                       reinterpret_cast<const PostArgs<double> *>(args), tiles, ns);
   else if (precision == CRAVA_HIP_MIXED)
     hipLaunchKernelGGL((k_posterior_batch<float, 1>), grid, block, 0, s,
-                       reinterpret_cast<const PostArgs<float> *>(args), tiles, ntiles);
+                       reinterpret_cast<const PostArgs<float> *>(args), tiles, ns);
   else
     hipLaunchKernelGGL((k_posterior_batch<float, 0>), grid, block, 0, s,
-                       reinterpret_cast<const PostArgs<float> *>(args), tiles, ntiles);
+                       reinterpret_cast<const PostArgs<float> *>(args), tiles, ns);
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }

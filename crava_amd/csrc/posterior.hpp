@@ -63,6 +63,7 @@ struct PostDevice {
   // out-of-band ones (or NULL: one launch over every plane); nk_in in band
   int *kmap = nullptr;
   int nk_in = 0;
+  std::vector<unsigned char> invert;  // host copy of the frequency gate, nzp entries
 };
 
 // kmap of a problem from its frequency gate (invert[k] != 0: in band).
@@ -108,7 +109,10 @@ struct PostBatch {
   void *queues = nullptr;     // mixed form: storage of every volume's queue
   int4 *tiles = nullptr;      // device {volume, k, first point, ntheta}
   long long ntiles = 0;
-  long long seg[kMaxTheta + 1] = {0};  // tiles of ntheta c: [seg[c-1], seg[c])
+  // in-band tiles of ntheta c: [seg[c-1], seg[c]); the out-of-band tiles of
+  // every volume (the prior passes through): [seg[kMaxTheta], ntiles)
+  long long seg[kMaxTheta + 1] = {0};
+  std::vector<std::vector<unsigned char>> inv;  // each volume's frequency gate
   std::vector<int> nt, nzp;
   std::vector<long long> plane;
   ~PostBatch();
