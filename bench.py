@@ -135,6 +135,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--errcorr-grid", action="store_true",
+                    help="fill and transform the whole error-correlation grid every step (default: its spectrum "
+                         "as one Fourier plane, exact for the workload's zero correlation gradients)")
     ap.add_argument("--kernel-reps", type=int, default=5)
     ap.add_argument("--sweep", action="store_true", help="report accuracy over a range of FP64 thresholds")
     ap.add_argument("--batch-volumes", type=int, default=48, help="volumes of the batching workload (0: skip)")
@@ -152,7 +155,7 @@ def parse():
 class Pipeline:
     """Working grids + posterior handle for one precision form."""
 
-    def __init__(self, ctx, prob, params, tables, fp64: bool, separable: bool = True):
+    def __init__(self, ctx, prob, params, tables, fp64: bool, separable: bool = True, ec_plane: bool = True):
         import numpy as np
         import torch
         from crava_amd import _lib as L
@@ -169,6 +172,11 @@ class Pipeline:
         self.corr = self.corr0.clone()
         self.ec0 = prob.errcorr.to(dt).clone()
         self.ec = self.ec0.clone()
+        # zero correlation gradients: fillInErrCorr writes corrXY into plane
+        # k = 0 only, so the grid's transform is one Fourier plane repeated
+        # along kz (crava_hip_err_corr_plane); the posterior reads that plane
+        self.ec_plane = ec_plane
+        self.ecp = torch.empty((prob.cfg.nyp, 2 * (prob.cfg.nxp // 2 + 1)), dtype=dt, device=self.ec.device)
         self.post = Posterior(ctx, params, tables)
         # the prior covariance fill (SeismicParametersHolder::initializeCorrelations)
         pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
@@ -189,7 +197,9 @@ class Pipeline:
         form's double-arithmetic covariance transforms overlap the float data
         transforms), the posterior, the inverse sets."""
         from crava_amd.core import Batch
-        vol = (self.post, self.mu, self.data, self.cov, self.ec) + ((self.corr,) if self.separable else ())
+        self.post.set_errcorr_plane(self.ec_plane)
+        vol = (self.post, self.mu, self.data, self.cov, self.ecp if self.ec_plane else self.ec) + \
+            ((self.corr,) if self.separable else ())
         self.batch = Batch(self.ctx, self.precision, [vol])
         return self.batch.autotune()
 
@@ -204,15 +214,21 @@ class Pipeline:
         # step's means and residuals as inputs do not change the work).  The
         # FP64 form copies its double grids back.
         prior = [self.corr] if self.separable else self.cov
+        self.post.set_errcorr_plane(self.ec_plane)
+        if self.ec_plane:
+            from crava_amd.core import err_corr_plane
+            err_corr_plane(self.ctx, self.ecp, *n, self.corr_xy, L.F64 if self.fp64 else L.MIXED)
+        ec = self.ecp if self.ec_plane else self.ec
         if self.fp64:
             prior0 = [self.corr0] if self.separable else self.cov0
-            for d, s_ in zip(prior + [self.ec], prior0 + [self.ec0]):
+            for d, s_ in zip(prior + ([] if self.ec_plane else [self.ec]), prior0 + [self.ec0]):
                 d.copy_(s_)
         else:
             from crava_amd.core import fill_err_corr, fill_param_corr
             fill_param_corr(self.ctx, prior, [1.0] if self.separable else self.cov_scales, *n, self.corr_xy,
                             self.circ_t)
-            fill_err_corr(self.ctx, self.ec, *n, self.corr_xy)
+            if not self.ec_plane:
+                fill_err_corr(self.ctx, self.ec, *n, self.corr_xy)
         if self.batch is not None:
             self.batch.run(graph=True)
             if counts:
@@ -226,10 +242,11 @@ class Pipeline:
         # mean update needs the accuracy of their variation, not of their
         # level) and the rest in float
         self.ctx.r2c_many(self.data, *n, L.DATA, self.data_precision)
-        self.ctx.r2c(self.ec, *n, L.COVARIANCE, self.data_precision)
+        if not self.ec_plane:
+            self.ctx.r2c(self.ec, *n, L.COVARIANCE, self.data_precision)
         self.ctx.r2c_many(self.mu, *n, L.DATA, self.mean_precision)
         self.ctx.r2c_many(prior, *n, L.COVARIANCE, self.cov_precision)
-        r = self.post.run(self.precision, self.mu, self.data, self.cov, self.ec, counts=counts,
+        r = self.post.run(self.precision, self.mu, self.data, self.cov, ec, counts=counts,
                           corr=self.corr if self.separable else None)
         self.ctx.c2r_many(self.mu, *n, L.DATA, self.mean_precision)
         self.ctx.c2r_many(self.data, *n, L.DATA, self.data_precision)
@@ -881,7 +898,7 @@ def main():
     n_in = int(tabs["invert"].sum()) * cn * nyp
     nt = prob.ntheta
 
-    mixed = Pipeline(ctx, prob, params, tabs, fp64=False)
+    mixed = Pipeline(ctx, prob, params, tabs, fp64=False, ec_plane=not args.errcorr_grid)
     # autotune the transform launch shapes (float and double) on a working grid
     from crava_amd.core import autotune_fft
     tune = {"f32": autotune_fft(ctx, mixed.data[0], nxp, nyp, nzp),
@@ -890,7 +907,7 @@ def main():
     mixed.reset()
     f64 = None
     if not args.no_fp64:
-        f64 = Pipeline(ctx, prob, params, tabs, fp64=True)
+        f64 = Pipeline(ctx, prob, params, tabs, fp64=True, ec_plane=not args.errcorr_grid)
         tune["f64"] = autotune_fft(ctx, f64.data[0], nxp, nyp, nzp)
         log(f"autotuned f64 variants {tune['f64']}")
         f64.reset()
@@ -983,7 +1000,8 @@ def main():
         g_mixed = run(mixed, args.steps, args.warmup) * 1e3
         g_f64 = run(f64, max(1, args.steps), 1) * 1e3
         general = {"ms_per_step_mixed": g_mixed, "ms_per_step_fp64": g_f64, "speedup_vs_fp64": g_f64 / g_mixed,
-                   "value": P / (g_mixed * 1e-3), "rel_err_vs_fp64": g_err, "transforms_per_step": 31}
+                   "value": P / (g_mixed * 1e-3), "rel_err_vs_fp64": g_err,
+                   "transforms_per_step": 30 if mixed.ec_plane else 31}
         log(f"general prior: mixed {g_mixed:.3f} ms, fp64 {g_f64:.3f} ms ({g_f64 / g_mixed:.2f}x), errors {g_err}")
         mixed.separable = f64.separable = True
         mixed.batch, f64.batch = keep_m, keep_f
@@ -1001,6 +1019,10 @@ def main():
     grid_c_bytes = 8 * cn * nyp * nzp * ng
     real_bytes = 4 * nxp * nyp * nzp * ng
     mixed.reset()
+    # the posterior kernels below read the transformed error-correlation grid
+    mixed.post.set_errcorr_plane(False)
+    if f64 is not None:
+        f64.post.set_errcorr_plane(False)
     ctx.r2c_many(mixed.data, *n3, L.DATA)
     ctx.r2c(mixed.ec, *n3, L.COVARIANCE)
     ctx.r2c_many(mixed.mu, *n3, L.DATA, L.F32_DEFLATE)
@@ -1110,7 +1132,10 @@ def main():
         "dtype": "f32+f64 (mixed)",
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"posterior path {nxp}x{nyp}x{nzp} FFT grid, ntheta={nt}", "fourier_points": P,
-                   "in_band_points": n_in, "transforms_per_step": 26,
+                   "in_band_points": n_in, "transforms_per_step": 25 if mixed.ec_plane else 26,
+                   "errcorr": "spectrum as one Fourier plane (zero correlation gradients: the fillInErrCorr grid "
+                              "is corrXY in plane 0, its transform that plane's 2-D transform on every kz plane)"
+                              if mixed.ec_plane else "grid filled and transformed every step",
                    "prior": "separable: Sigma0 x one correlation grid, as initializeCorrelations builds it "
                             "(one correlation transform instead of six; the six-grid form is under general_prior)",
                    "parallelism": "single GPU"},

@@ -142,6 +142,8 @@ SIGNATURES = {
     "crava_hip_fill_param_corr": (I, [VP, C.POINTER(VP), I, C.POINTER(C.c_float), I, I, I, VP, VP, C.c_float,
                                       C.c_float]),
     "crava_hip_fill_err_corr": (I, [VP, VP, I, I, I, VP, C.c_float, C.c_float]),
+    "crava_hip_err_corr_plane": (I, [VP, VP, I, I, I, VP, I]),
+    "crava_hip_post_set_errcorr_plane": (I, [VP, I]),
     "crava_hip_grid_op": (I, [VP, I, VP, VP, I, I, I, I, C.c_float]),
     "crava_hip_data_variance": (I, [VP, VP, I, I, I, I, I, I, C.POINTER(C.c_float), C.POINTER(LL)]),
     "crava_hip_facies_geomodel": (I, [VP, I, I, I, I, C.POINTER(C.c_float), C.POINTER(VP)]),

@@ -103,6 +103,14 @@ def fill_err_corr(ctx: Context, grid, nxp, nyp, nzp, corr_xy, grad_i: float = 0.
           "crava_hip_fill_err_corr")
 
 
+def err_corr_plane(ctx: Context, plane, nxp, nyp, nzp, corr_xy, precision: int):
+    """crava_hip_err_corr_plane: the error-correlation spectrum of zero
+    correlation gradients as one Fourier plane, in place in `plane` (nyp rows
+    of 2 (nxp/2 + 1) floats, or doubles for F64)."""
+    check(lib().crava_hip_err_corr_plane(ctx.h, _dev_ptr(plane), nxp, nyp, nzp, _dev_ptr(corr_xy), precision),
+          "crava_hip_err_corr_plane")
+
+
 def grid_op(ctx: Context, op: int, grid, other, nxp, nyp, nzp, transformed: bool, scalar: float = 0.0):
     check(lib().crava_hip_grid_op(ctx.h, op, _dev_ptr(grid), None if other is None else _dev_ptr(other), nxp, nyp,
                                   nzp, int(transformed), scalar), "crava_hip_grid_op")
@@ -330,6 +338,11 @@ class Posterior:
             self.close()
         except Exception:
             pass
+
+    def set_errcorr_plane(self, plane: bool):
+        """crava_hip_post_set_errcorr_plane: later runs take the error
+        correlation as one Fourier plane (err_corr_plane) or as a grid."""
+        check(lib().crava_hip_post_set_errcorr_plane(self.h, 1 if plane else 0), "crava_hip_post_set_errcorr_plane")
 
     def run(self, precision: int, mu, data, cov, errcorr, counts: bool = False, corr=None):
         """crava_hip_post_run; with `corr` (the transformed correlation grid of

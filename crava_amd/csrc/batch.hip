@@ -28,6 +28,11 @@ static size_t real_grid_bytes(const PostProblem &pp, bool f64) {
   return (size_t)2 * (pp.nxp / 2 + 1) * pp.nyp * pp.nzp * (f64 ? 8 : 4);
 }
 
+// Bytes of a volume's error-correlation buffer: a grid, or one Fourier plane.
+static size_t errcorr_bytes(const PostProblem &pp, bool f64, bool plane) {
+  return plane ? (size_t)2 * (pp.nxp / 2 + 1) * pp.nyp * (f64 ? 8 : 4) : real_grid_bytes(pp, f64);
+}
+
 BatchExec::~BatchExec() {
   drop_graph();
   for (int i = 0; i < kStreams; ++i) {
@@ -71,6 +76,8 @@ int BatchExec::init(const BatchVolume *vols, int n) {
     for (int l = 0; l < bv.pd->pp.ntheta; ++l)
       if (!bv.data[l]) { set_error("volume " + std::to_string(v) + ": NULL data grid"); return CRAVA_HIP_E_INVALID; }
   }
+  ec_plane_.assign(n, 0);
+  for (int v = 0; v < n; ++v) ec_plane_[v] = vols_[v].pd->ec_plane ? 1 : 0;
   CRAVA_HIP_CHECK(hipMalloc(&counters_, sizeof(unsigned long long) * 2 * n));
   CRAVA_HIP_CHECK(hipMalloc(&dcbufs_, sizeof(double) * 4 * kMaxGrids * (size_t)n));
   int rc;
@@ -135,8 +142,10 @@ int BatchExec::build_sets() {
       ia->grids[ia->ng] = bv.data[l];
       ia->scale[ia->ng++] = si;
     }
-    fa->grids[fa->ng] = bv.errcorr;
-    fa->scale[fa->ng++] = ef;
+    if (!ec_plane_[v]) {  // a one-plane spectrum is already transformed
+      fa->grids[fa->ng] = bv.errcorr;
+      fa->scale[fa->ng++] = ef;
+    }
     for (int c = 0; c < 3; ++c) {
       if (deflate) {
         fa->dcmask |= 1u << fa->ng;
@@ -234,7 +243,7 @@ int BatchExec::enqueue(hipStream_t origin, bool transforms_only, bool forward_on
 
 int BatchExec::phase_times(hipStream_t s, float *ms) {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  int rc = CRAVA_HIP_OK;
+  int rc = check_ec_form();
   for (int i = 0; i < 4 && rc == CRAVA_HIP_OK; ++i)
     if (hipEventCreate(&ev[i]) != hipSuccess) rc = CRAVA_HIP_E_HIP;
   if (rc == CRAVA_HIP_OK) rc = enqueue(s, false, false, ev);
@@ -271,7 +280,18 @@ int BatchExec::launch_post(hipStream_t origin) {
   return post_.launch_queued(origin);
 }
 
+// The problems' error-correlation forms must be those the batch was built with.
+int BatchExec::check_ec_form() const {
+  for (size_t v = 0; v < vols_.size(); ++v)
+    if (vols_[v].pd->ec_plane != (ec_plane_[v] != 0)) {
+      set_error("volume " + std::to_string(v) + ": error-correlation form changed after the batch was created");
+      return CRAVA_HIP_E_INVALID;
+    }
+  return CRAVA_HIP_OK;
+}
+
 int BatchExec::run(hipStream_t s, bool graph) {
+  if (int rc = check_ec_form()) return rc;
   if (!graph) return enqueue(s, false, false);
   if (!exec_) {
     CRAVA_HIP_CHECK(hipStreamBeginCapture(cap_, hipStreamCaptureModeRelaxed));
@@ -317,7 +337,8 @@ int BatchExec::backup_grids(hipStream_t s, bool restore) {
   const bool f64 = precision_ == CRAVA_HIP_F64;
   size_t total = 0;
   for (const BatchVolume &bv : vols_)
-    total += real_grid_bytes(bv.pd->pp, f64) * (size_t)(10 + bv.pd->pp.ntheta + (bv.corr ? 1 : 0));
+    total += real_grid_bytes(bv.pd->pp, f64) * (size_t)(9 + bv.pd->pp.ntheta + (bv.corr ? 1 : 0)) +
+             errcorr_bytes(bv.pd->pp, f64, bv.pd->ec_plane);
   if (!restore && total > backup_bytes_) {
     if (backup_) (void)hipFree(backup_);
     backup_ = nullptr;
@@ -332,12 +353,13 @@ int BatchExec::backup_grids(hipStream_t s, bool restore) {
     for (int c = 0; c < 3; ++c) gl.push_back(bv.mu[c]);
     for (int l = 0; l < bv.pd->pp.ntheta; ++l) gl.push_back(bv.data[l]);
     for (int q = 0; q < 6; ++q) gl.push_back(bv.cov[q]);
-    gl.push_back(bv.errcorr);
     if (bv.corr) gl.push_back(bv.corr);
-    for (void *g : gl) {
-      if (restore) CRAVA_HIP_CHECK(hipMemcpyAsync(g, b, gb, hipMemcpyDeviceToDevice, s));
-      else CRAVA_HIP_CHECK(hipMemcpyAsync(b, g, gb, hipMemcpyDeviceToDevice, s));
-      b += gb;
+    gl.push_back(bv.errcorr);  // last: a grid, or one Fourier plane
+    for (size_t i = 0; i < gl.size(); ++i) {
+      const size_t nb = i + 1 < gl.size() ? gb : errcorr_bytes(bv.pd->pp, f64, bv.pd->ec_plane);
+      if (restore) CRAVA_HIP_CHECK(hipMemcpyAsync(gl[i], b, nb, hipMemcpyDeviceToDevice, s));
+      else CRAVA_HIP_CHECK(hipMemcpyAsync(b, gl[i], nb, hipMemcpyDeviceToDevice, s));
+      b += nb;
     }
   }
   return CRAVA_HIP_OK;
@@ -345,6 +367,7 @@ int BatchExec::backup_grids(hipStream_t s, bool restore) {
 
 int BatchExec::autotune(hipStream_t s, int *ppb_out, int *blocks_out, int *form_out, float *ms_out) {
   int rc;
+  if ((rc = check_ec_form())) return rc;
   drop_graph();
   if ((rc = backup_grids(s, false))) return rc;
   // transforms: each plan once, on the largest set that uses it

@@ -62,11 +62,15 @@ class BatchExec {
   int fork(hipStream_t origin);
   int join(hipStream_t origin);
   int backup_grids(hipStream_t s, bool restore);
+  int check_ec_form() const;
   void drop_graph();
 
   Context *ctx_;
   int precision_;
   std::vector<BatchVolume> vols_;
+  // per volume: its error correlation is one Fourier plane (taken from its
+  // problem at init; not transformed here)
+  std::vector<unsigned char> ec_plane_;
   std::vector<Work> work_;
   PostBatch post_;
   unsigned long long *counters_ = nullptr;  // [nvol][2]

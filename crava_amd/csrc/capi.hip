@@ -664,6 +664,7 @@ int crava_hip_nll_grad(crava_hip_post *post, int precision, void *const *mu, voi
   for (int l = 0; l < nt; ++l) c.data[l] = data[l];
   c.cov0 = cov[0];
   c.errcorr = errcorr;
+  c.ec_plane = post->dev.ec_plane;
   c.n_inband = (int)klist.size();
   for (int i = 0; i < crava::kMaxTheta * 3; ++i) c.A[i] = pp.A[i];
   c.sig0[0] = S[0][0]; c.sig0[1] = S[1][1]; c.sig0[2] = S[2][2];
@@ -1439,6 +1440,28 @@ int crava_hip_fill_err_corr(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, in
     return CRAVA_HIP_E_INVALID;
   }
   return crava::launch_fill_err_corr(static_cast<float *>(grid), nxp, nyp, nzp, corr_xy, grad_i, grad_j, ctx->stream);
+}
+
+int crava_hip_err_corr_plane(crava_hip_ctx *ctx, void *plane, int nxp, int nyp, int nzp, const float *corr_xy,
+                             int precision) {
+  if (!ctx || !plane || !corr_xy || !grid_dims_ok(nxp, nyp, nzp) ||
+      (precision != CRAVA_HIP_F32 && precision != CRAVA_HIP_MIXED && precision != CRAVA_HIP_F64)) {
+    set_error("invalid arguments to crava_hip_err_corr_plane");
+    return CRAVA_HIP_E_INVALID;
+  }
+  int rc = crava::launch_fill_err_corr_plane(plane, precision == CRAVA_HIP_F64, nxp, nyp, nzp, corr_xy, ctx->stream);
+  if (rc) return rc;
+  // the unscaled 2-D transform: a one-plane grid's z axis is the identity
+  return fft_common(ctx, &plane, 1, nxp, nyp, 1, CRAVA_HIP_COVARIANCE, precision, false);
+}
+
+int crava_hip_post_set_errcorr_plane(crava_hip_post *post, int plane) {
+  if (!post) {
+    set_error("NULL posterior");
+    return CRAVA_HIP_E_INVALID;
+  }
+  post->dev.ec_plane = plane != 0;
+  return CRAVA_HIP_OK;
 }
 
 int crava_hip_grid_op(crava_hip_ctx *ctx, int op, void *grid, const void *other, int nxp, int nyp, int nzp,

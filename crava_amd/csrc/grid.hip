@@ -112,6 +112,20 @@ __global__ void __launch_bounds__(256) k_fill_err_corr(float *grid, int nxp, int
   }
 }
 
+// Plane k = 0 of fillInErrCorr with zero gradients (the only plane that is
+// not zero then), stored as T: rnxp / 2 pairs of each of the nyp rows.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill_err_corr_plane(T *plane, int nxp, int nyp, int nzp,
+                                                             const float *__restrict__ corr_xy) {
+  const int cn = nxp / 2 + 1;
+  const int npair = cn * nyp;
+  for (int q = (int)(blockIdx.x * 256u + threadIdx.x); q < npair; q += (int)(gridDim.x * 256u)) {
+    const int j = q / cn, h = q - j * cn;
+    plane[2 * q] = (T)err_corr_value(2 * h, j, 0, nxp, nyp, nzp, corr_xy, 0.0f, 0.0f);
+    plane[2 * q + 1] = (T)err_corr_value(2 * h + 1, j, 0, nxp, nyp, nzp, corr_xy, 0.0f, 0.0f);
+  }
+}
+
 // Element-wise operators (src/fftgrid.cpp:1877-1999), over n floats (real
 // domain) or n complex values (transformed).
 __global__ void k_grid_op(int op, float *a, const float *__restrict__ b, long long n, int transformed, float s) {
@@ -303,6 +317,19 @@ int launch_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *co
   else
     hipLaunchKernelGGL(k_fill_err_corr<unsigned long long>, dim3(blocks_for(n)), dim3(256), 0, s, grid, nxp, nyp, nzp,
                        corr_xy, grad_i, grad_j);
+  CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
+int launch_fill_err_corr_plane(void *plane, bool f64, int nxp, int nyp, int nzp, const float *corr_xy,
+                               hipStream_t s) {
+  const long long n = fill_pairs(nxp, nyp, 1);
+  if (f64)
+    hipLaunchKernelGGL(k_fill_err_corr_plane<double>, dim3(blocks_for(n)), dim3(256), 0, s,
+                       static_cast<double *>(plane), nxp, nyp, nzp, corr_xy);
+  else
+    hipLaunchKernelGGL(k_fill_err_corr_plane<float>, dim3(blocks_for(n)), dim3(256), 0, s,
+                       static_cast<float *>(plane), nxp, nyp, nzp, corr_xy);
   CRAVA_HIP_CHECK(hipGetLastError());
   return CRAVA_HIP_OK;
 }

@@ -9,6 +9,10 @@ int launch_fill_param_corr(const GridSet &gs, int nxp, int nyp, int nzp, const f
                            float grad_i, float grad_j, hipStream_t s);
 int launch_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *corr_xy, float grad_i, float grad_j,
                          hipStream_t s);
+// Plane k = 0 of the error-correlation grid with zero gradients, as float or
+// double, into the real-layout plane [nyp][2 (nxp / 2 + 1)].
+int launch_fill_err_corr_plane(void *plane, bool f64, int nxp, int nyp, int nzp, const float *corr_xy,
+                               hipStream_t s);
 // n floats (real domain) or n complex values (transformed).
 int launch_grid_op(int op, float *a, const float *b, long long n, bool transformed, float scalar, hipStream_t s);
 // Time-lapse merge (grid.hip): n complex points per grid.

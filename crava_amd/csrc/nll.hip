@@ -38,6 +38,7 @@ struct NllArgs {
   const cx<G> *d[kMaxTheta];
   const cx<G> *cov0;
   const cx<G> *ec;
+  long long ec_kstride;   // ec of point (k, ip) is ec[k ec_kstride + ip]
   const NllTab *tab;      // per k
   const int *klist;       // in-band planes
   long long plane;
@@ -223,7 +224,7 @@ __global__ void __launch_bounds__(256) k_nll(NllArgs<G> a) {
 #pragma unroll
     for (int l = 0; l < NT; ++l) d[l] = mk((double)a.d[l][p].x, (double)a.d[l][p].y);
     const double cabs = fabs((double)a.cov0[p].x / a.base00);
-    const double lam = fabs((double)a.ec[p].x);
+    const double lam = fabs((double)a.ec[(long long)k * a.ec_kstride + ip].x);
     point_terms<NT>(t, a.A, a.sig0, cabs, lam, mu, d, w, acc);
   }
   // block reduction through LDS, one value at a time
@@ -260,6 +261,7 @@ static int launch_g(const NllCall &c, hipStream_t s) {
   a.nxp_even = (c.nxp % 2) == 0;
   a.plane = (long long)a.cnxp * c.nyp;
   a.total = (long long)c.n_inband * a.plane;
+  a.ec_kstride = c.ec_plane ? 0 : a.plane;
   for (int l = 0; l < kMaxTheta; ++l)
     for (int b = 0; b < 3; ++b) a.A[l][b] = c.A[l * 3 + b];
   for (int q = 0; q < 6; ++q) a.sig0[q] = c.sig0[q];

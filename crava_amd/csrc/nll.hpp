@@ -23,6 +23,7 @@ struct NllCall {
   const void *data[kMaxTheta];
   const void *cov0;
   const void *errcorr;
+  bool ec_plane;  // errcorr is one Fourier plane, the same on every kz plane
   const NllTab *tab;
   const int *klist;
   int n_inband;

@@ -33,6 +33,7 @@ struct PostArgs {
   cx<G> *d[kMaxTheta];
   cx<G> *cov[6];
   const cx<G> *ec;
+  long long ec_kstride;  // ec of point (k, ip) is ec[k ec_kstride + ip]: the plane size, or 0 for one plane
   // Separable prior Sigma0 (x) corr (SeismicParametersHolder::
   // initializeCorrelations, src/seismicparametersholder.cpp:197-221, fills
   // the six grids with the one correlation times Sigma0_ab): the prior is
@@ -394,7 +395,7 @@ __device__ __forceinline__ void post_point(const PostArgs<G> &a, int k, long lon
     for (int b = 0; b < 3; ++b) {
       if constexpr (sizeof(G) == 4) A[l][b] = a.Af[l][b]; else A[l][b] = a.Ad[l][b];
     }
-  const G lam = fabs(a.ec[p].x);
+  const G lam = fabs(a.ec[(long long)k * a.ec_kstride + ip].x);
   cx<G> mu[3], d[NT], post[6];
 #pragma unroll
   for (int c = 0; c < 3; ++c) mu[c] = a.mu[c][p];
@@ -433,7 +434,7 @@ __device__ __forceinline__ void post_point_wide(const PostArgs<float> &a, int k,
     for (int l = 0; l < NT; ++l)
 #pragma unroll
       for (int b = 0; b < 3; ++b) A[l][b] = a.Ad[l][b];
-    const double lam = (double)fabsf(a.ec[p].x);
+    const double lam = (double)fabsf(a.ec[(long long)k * a.ec_kstride + ip].x);
     cx<double> mu[3], d[NT], post[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c) mu[c] = cvt<double>(a.mu[c][p]);
@@ -481,7 +482,7 @@ __device__ __forceinline__ void post_point_mixed(const PostArgs<float> &a, int k
   bool need64 = false;
   float lam = 0.0f;
   if (live) {
-    lam = fabsf(a.ec[p].x);
+    lam = fabsf(a.ec[(long long)k * a.ec_kstride + ip].x);
     // tr(K S K^H) = sum_l |kWD_l|^2 (A S A^T)_ll
     float sig = 0.0f;
 #pragma unroll
@@ -774,6 +775,7 @@ static void fill_args(PostArgs<G> &a, const PostDevice &pd, void *const *mu, voi
   a.defer_list = nullptr;
   a.defer_count = nullptr;
   a.plane = (long long)(pp.nxp / 2 + 1) * pp.nyp;
+  a.ec_kstride = pd.ec_plane ? 0 : a.plane;
   a.nt = pp.ntheta;
   for (int l = 0; l < kMaxTheta; ++l)
     for (int b = 0; b < 3; ++b) {

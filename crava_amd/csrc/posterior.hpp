@@ -64,6 +64,9 @@ struct PostDevice {
   int *kmap = nullptr;
   int nk_in = 0;
   std::vector<unsigned char> invert;  // host copy of the frequency gate, nzp entries
+  // the error-correlation spectrum is one Fourier plane [nyp][nxp/2 + 1],
+  // the same on every kz plane (crava_hip_err_corr_plane)
+  bool ec_plane = false;
 };
 
 // kmap of a problem from its frequency gate (invert[k] != 0: in band).

@@ -289,6 +289,31 @@ int crava_hip_post_run_separable(crava_hip_post *post, int precision, void *cons
                                  const void *corr, const void *errcorr, void *const *cov_out, long long *n_fail,
                                  long long *n_fp64);
 
+/* The error-correlation spectrum as one Fourier plane.  With zero
+ * correlation gradients (no correlation direction: ModelGeneral::
+ * getCorrGradIJ gives 0, 0; src/crava.cpp:144-157) FFTGrid::fillInErrCorr
+ * (src/fftgrid.cpp:1065-1116) writes priorCorrXY into plane k = 0 and zeros
+ * into every other plane, so the grid's unscaled transform (COVARIANCE
+ * fftInPlace, src/crava.cpp:1011) equals the 2-D transform of that plane on
+ * every kz plane.  This fills that plane and transforms it: plane is a
+ * device buffer of nyp rows of 2 (nxp/2 + 1) floats (doubles for
+ * CRAVA_HIP_F64), which becomes nyp rows of nxp/2 + 1 complex values;
+ * precision CRAVA_HIP_F32, CRAVA_HIP_MIXED (float storage, double
+ * arithmetic) or CRAVA_HIP_F64.  With nonzero gradients the grid is not
+ * constant along kz: fill and transform the whole grid instead. */
+int crava_hip_err_corr_plane(crava_hip_ctx *ctx, void *plane, int nxp, int nyp, int nzp, const float *corr_xy,
+                             int precision);
+
+/* plane != 0: every later posterior run of `post` (crava_hip_post_run*,
+ * crava_hip_nll_grad, batches created after this call) takes `errcorr` as the
+ * one Fourier plane of crava_hip_err_corr_plane, read for every kz plane,
+ * instead of a transformed grid; a batch does not transform such a volume's
+ * error correlation.  plane == 0 restores grids.  A batch keeps the form its
+ * volumes had when it was created (running it after a change is an error).
+ * No reference counterpart: it saves the error-correlation grid's fill and
+ * 3-D transform and the posterior's read of it. */
+int crava_hip_post_set_errcorr_plane(crava_hip_post *post, int plane);
+
 /* crava_hip_post_run that allocates nothing.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
@@ -499,7 +524,9 @@ typedef struct {
   void *mu[3];      /* prior mean, time domain         -> posterior mean, time domain */
   void *data[8];    /* seismic data, time domain        -> residuals, time domain (first ntheta used) */
   void *cov[6];     /* prior covariance (aa,bb,rr,ab,ar,br), time domain -> posterior covariance, time domain */
-  void *errcorr;    /* error correlation, time domain   -> its spectrum (consumed) */
+  void *errcorr;    /* error correlation, time domain   -> its spectrum (consumed); or, if `post`
+                       takes one plane (crava_hip_post_set_errcorr_plane), that plane's spectrum
+                       (read only) */
   void *corr;       /* NULL, or a separable prior's correlation grid, time domain -> its spectrum
                        (consumed); cov[] are then outputs only (crava_hip_post_run_separable) */
 } crava_hip_batch_volume;
