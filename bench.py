@@ -190,6 +190,12 @@ class Pipeline:
         self.cov_precision = L.F64 if fp64 else L.MIXED
         self.L = L
         self.batch = None
+        if ec_plane:
+            # valid before the first step: use_graph's autotuner runs the
+            # posterior on it
+            from crava_amd.core import err_corr_plane
+            c = prob.cfg
+            err_corr_plane(ctx, self.ecp, c.nxp, c.nyp, c.nzp, self.corr_xy, L.F64 if fp64 else L.MIXED)
 
     def use_graph(self):
         """Run the step as a one-volume batch replayed as a hipGraph: the
@@ -419,16 +425,19 @@ def cpu_baseline(prob, tabs, budget_s):
     }
 
 
-def batching(ctx, volumes: int, reps: int, seed: int = 3):
+def batching(ctx, volumes: int, reps: int, seed: int = 3, ec_plane: bool = True):
     """The batching workload: `volumes` inversion volumes of uneven size
     (nx, ny in 32..192, nz in 64..256, ntheta 3..8, own wavelet, ranges and
     S/N), each with its separable prior, MIXED form, autotuned, replayed as
-    one hipGraph per batch."""
+    one hipGraph per batch.  ec_plane: each volume's error-correlation
+    spectrum is one Fourier plane (zero correlation gradients), prepared
+    once before the timed replays (read only); else its grid is reloaded
+    and transformed in every replay."""
     import numpy as np
     import torch
     from crava_amd import _lib as L
     from crava_amd import synth
-    from crava_amd.core import Batch, Posterior, problem_tables
+    from crava_amd.core import Batch, Posterior, err_corr_plane, problem_tables
     t0 = time.time()
     rng = np.random.default_rng(seed)
     vols, src, posts = [], [], []
@@ -440,14 +449,22 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
         c = prob.cfg
         tabs = problem_tables(prob)
         posts.append(Posterior(ctx, synth.post_params(prob, 1e-3), tabs))
-        pristine = prob.mu + prob.data + prob.cov + [prob.errcorr, prob.corr]
+        cn = c.nxp // 2 + 1
+        if ec_plane:
+            ecp = torch.empty((c.nyp, 2 * cn), dtype=torch.float32, device="cuda")
+            cxy = torch.from_numpy(np.ascontiguousarray(prob.corr_xy, dtype=np.float32)).cuda()
+            err_corr_plane(ctx, ecp, c.nxp, c.nyp, c.nzp, cxy, L.MIXED)
+            posts[-1].set_errcorr_plane(True)
+            pristine = prob.mu + prob.data + prob.cov + [prob.corr]
+        else:
+            pristine = prob.mu + prob.data + prob.cov + [prob.errcorr, prob.corr]
         work = [g.clone() for g in pristine]
         nt = prob.ntheta
         nts.append(nt)
         src.append((pristine, work))
         # separable prior (initializeCorrelations): one correlation transform per volume
-        vols.append((posts[-1], work[:3], work[3:3 + nt], work[3 + nt:9 + nt], work[9 + nt], work[10 + nt]))
-        cn = c.nxp // 2 + 1
+        ec = ecp if ec_plane else work[9 + nt]
+        vols.append((posts[-1], work[:3], work[3:3 + nt], work[3 + nt:9 + nt], ec, work[-1]))
         pts = cn * c.nyp * c.nzp
         n_in = int(tabs["invert"].sum()) * cn * c.nyp
         P += pts
@@ -455,9 +472,9 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
         # bytes of the three passes of one grid's transform: x reads the real
         # grid and writes the spectrum, y and z read and write the spectrum
         passes = 4 * c.nxp * c.nyp * c.nzp + 8 * pts + 4 * 8 * pts
-        fwd_bytes += (nt + 5) * passes        # data, error correlation, means; the correlation grid
+        fwd_bytes += (nt + (4 if ec_plane else 5)) * passes   # data, (error correlation,) means; correlation
         inv_bytes += (nt + 9) * passes        # residuals, covariances; means
-        post_bytes += n_in * 8 * (14 + 2 * nt) + (pts - n_in) * 56
+        post_bytes += n_in * 8 * ((13 if ec_plane else 14) + 2 * nt) + (pts - n_in) * 56
         del prob
     torch.cuda.synchronize()
     log(f"batching: {volumes} volumes generated in {time.time()-t0:.1f}s, {P} Fourier points")
@@ -506,7 +523,9 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
                                 "form": ["one persistent launch", "one persistent launch per ntheta class",
                                          "one launch per volume"][form],
                                 "ms": post_ms, "pts_per_s": P / (post_ms * 1e-3)},
-           "failed_factorisations": int(sum(nf)), "fp64_points": int(sum(n64)), "threshold": 1e-3}
+           "failed_factorisations": int(sum(nf)), "fp64_points": int(sum(n64)), "threshold": 1e-3,
+           "errcorr": "one Fourier plane per volume (zero correlation gradients), prepared before the timed "
+                      "replays" if ec_plane else "grid reloaded outside and transformed inside every replay"}
 
     def roof(ms, nbytes, what):
         gbs = nbytes / (ms * 1e-3) / 1e9
@@ -516,8 +535,10 @@ def batching(ctx, volumes: int, reps: int, seed: int = 3):
     # bytes: each transform pass reads and writes its grids, the posterior
     # its per-point inputs and outputs
     out["phase_rooflines"] = {
-        "forward_transforms": roof(ph[0], fwd_bytes, "r2c of (ntheta + 5) grids per volume, 3 passes each"),
-        "posterior": roof(ph[1], post_bytes, "separable MIXED posterior: 8 (14 + 2 ntheta) B in band, 56 B out"),
+        "forward_transforms": roof(ph[0], fwd_bytes, f"r2c of (ntheta + {4 if ec_plane else 5}) grids per volume, "
+                                                     "3 passes each"),
+        "posterior": roof(ph[1], post_bytes, f"separable MIXED posterior: 8 ({13 if ec_plane else 14} + 2 ntheta) B "
+                                             "in band, 56 B out"),
         "inverse_transforms": roof(ph[2], inv_bytes, "c2r of (ntheta + 9) grids per volume, 3 passes each"),
     }
     for k, v in out["phase_rooflines"].items():
@@ -878,7 +899,8 @@ def main():
         fn = {"forward_modelling": lambda: forward_modelling(ctx, synth.make_problem(synth.config(args.config), "cuda"),
                                                              reps, not args.no_cpu),
               "nonconstant_thickness": lambda: nonconstant_thickness(ctx, reps, not args.no_cpu),
-              "batching": lambda: batching(ctx, args.batch_volumes, args.batch_reps),
+              "batching": lambda: batching(ctx, args.batch_volumes, args.batch_reps,
+                                           ec_plane=not args.errcorr_grid),
               "time_lapse_merge": lambda: time_lapse_merge(ctx, reps, not args.no_cpu),
               "local_noise": lambda: local_noise(ctx, reps, not args.no_cpu)}[args.only]
         print(json.dumps({"workload": args.only, "result": fn()}))
@@ -1001,7 +1023,7 @@ def main():
         g_f64 = run(f64, max(1, args.steps), 1) * 1e3
         general = {"ms_per_step_mixed": g_mixed, "ms_per_step_fp64": g_f64, "speedup_vs_fp64": g_f64 / g_mixed,
                    "value": P / (g_mixed * 1e-3), "rel_err_vs_fp64": g_err,
-                   "transforms_per_step": 30 if mixed.ec_plane else 31}
+                   "transforms_per_step": 31 if args.errcorr_grid else 30}
         log(f"general prior: mixed {g_mixed:.3f} ms, fp64 {g_f64:.3f} ms ({g_f64 / g_mixed:.2f}x), errors {g_err}")
         mixed.separable = f64.separable = True
         mixed.batch, f64.batch = keep_m, keep_f
@@ -1090,7 +1112,7 @@ def main():
     if args.batch_volumes > 0:
         del mixed, f64
         torch.cuda.empty_cache()
-        batch = batching(ctx, args.batch_volumes, args.batch_reps)
+        batch = batching(ctx, args.batch_volumes, args.batch_reps, ec_plane=not args.errcorr_grid)
     diff = None
     if args.diff_steps > 0:
         torch.cuda.empty_cache()
@@ -1132,10 +1154,10 @@ def main():
         "dtype": "f32+f64 (mixed)",
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"posterior path {nxp}x{nyp}x{nzp} FFT grid, ntheta={nt}", "fourier_points": P,
-                   "in_band_points": n_in, "transforms_per_step": 25 if mixed.ec_plane else 26,
+                   "in_band_points": n_in, "transforms_per_step": 26 if args.errcorr_grid else 25,
                    "errcorr": "spectrum as one Fourier plane (zero correlation gradients: the fillInErrCorr grid "
                               "is corrXY in plane 0, its transform that plane's 2-D transform on every kz plane)"
-                              if mixed.ec_plane else "grid filled and transformed every step",
+                              if not args.errcorr_grid else "grid filled and transformed every step",
                    "prior": "separable: Sigma0 x one correlation grid, as initializeCorrelations builds it "
                             "(one correlation transform instead of six; the six-grid form is under general_prior)",
                    "parallelism": "single GPU"},
