@@ -118,6 +118,11 @@ def profiled_traffic(name, grid):
         i = 4 if "k_sfft_strided<" in name else 3     # the direct-I/O flag
         a[i] = "false" if a[i] == "true" else "true"
         found = lookup(head + "<" + ", ".join(a) + ">")
+    if found is None and "k_sfft_strided<" in name:
+        # the pencil-fastest form of the same shape (no direct-I/O flag)
+        head, args = name.split("<", 1)
+        a = [x.strip() for x in args.rstrip(">").split(",")]
+        found = lookup("crava::k_sfft_cols<" + ", ".join(a[:4] + a[5:]) + ">")
     return found
 
 
