@@ -230,14 +230,27 @@ class Pipeline:
             from crava_amd.core import err_corr_plane
             err_corr_plane(self.ctx, self.ecp, *n, self.corr_xy, L.F64 if self.fp64 else L.MIXED)
         ec = self.ecp if self.ec_plane else self.ec
+        # zero gradients, separable prior, one stream: the correlation
+        # spectrum straight from the transforms of corrXY and circT
+        # (crava_hip_corr_spectrum_separable) instead of the fill and the
+        # 3-D transform (the batch transforms its own correlation grid)
+        factors = self.ec_plane and self.separable and self.batch is None
+        if factors:
+            from crava_amd.core import corr_spectrum_separable
+            corr_spectrum_separable(self.ctx, self.corr, *n, self.corr_xy, self.circ_t,
+                                    L.F64 if self.fp64 else L.MIXED)
         if self.fp64:
             prior0 = [self.corr0] if self.separable else self.cov0
-            for d, s_ in zip(prior + ([] if self.ec_plane else [self.ec]), prior0 + [self.ec0]):
+            pairs = list(zip(prior, prior0)) if not factors else []
+            if not self.ec_plane:
+                pairs.append((self.ec, self.ec0))
+            for d, s_ in pairs:
                 d.copy_(s_)
         else:
             from crava_amd.core import fill_err_corr, fill_param_corr
-            fill_param_corr(self.ctx, prior, [1.0] if self.separable else self.cov_scales, *n, self.corr_xy,
-                            self.circ_t)
+            if not factors:
+                fill_param_corr(self.ctx, prior, [1.0] if self.separable else self.cov_scales, *n, self.corr_xy,
+                                self.circ_t)
             if not self.ec_plane:
                 fill_err_corr(self.ctx, self.ec, *n, self.corr_xy)
         if self.batch is not None:
@@ -256,7 +269,8 @@ class Pipeline:
         if not self.ec_plane:
             self.ctx.r2c(self.ec, *n, L.COVARIANCE, self.data_precision)
         self.ctx.r2c_many(self.mu, *n, L.DATA, self.mean_precision)
-        self.ctx.r2c_many(prior, *n, L.COVARIANCE, self.cov_precision)
+        if not factors:
+            self.ctx.r2c_many(prior, *n, L.COVARIANCE, self.cov_precision)
         r = self.post.run(self.precision, self.mu, self.data, self.cov, ec, counts=counts,
                           corr=self.corr if self.separable else None)
         self.ctx.c2r_many(self.mu, *n, L.DATA, self.mean_precision)

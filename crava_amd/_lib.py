@@ -143,6 +143,7 @@ SIGNATURES = {
                                       C.c_float]),
     "crava_hip_fill_err_corr": (I, [VP, VP, I, I, I, VP, C.c_float, C.c_float]),
     "crava_hip_err_corr_plane": (I, [VP, VP, I, I, I, VP, I]),
+    "crava_hip_corr_spectrum_separable": (I, [VP, VP, I, I, I, VP, VP, I]),
     "crava_hip_post_set_errcorr_plane": (I, [VP, I]),
     "crava_hip_grid_op": (I, [VP, I, VP, VP, I, I, I, I, C.c_float]),
     "crava_hip_data_variance": (I, [VP, VP, I, I, I, I, I, I, C.POINTER(C.c_float), C.POINTER(LL)]),

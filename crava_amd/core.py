@@ -111,6 +111,14 @@ def err_corr_plane(ctx: Context, plane, nxp, nyp, nzp, corr_xy, precision: int):
           "crava_hip_err_corr_plane")
 
 
+def corr_spectrum_separable(ctx: Context, grid, nxp, nyp, nzp, corr_xy, circ_t, precision: int):
+    """crava_hip_corr_spectrum_separable: the transformed correlation grid of
+    a separable prior with zero gradients, from the transforms of corrXY and
+    circT."""
+    check(lib().crava_hip_corr_spectrum_separable(ctx.h, _dev_ptr(grid), nxp, nyp, nzp, _dev_ptr(corr_xy),
+                                                  _dev_ptr(circ_t), precision), "crava_hip_corr_spectrum_separable")
+
+
 def grid_op(ctx: Context, op: int, grid, other, nxp, nyp, nzp, transformed: bool, scalar: float = 0.0):
     check(lib().crava_hip_grid_op(ctx.h, op, _dev_ptr(grid), None if other is None else _dev_ptr(other), nxp, nyp,
                                   nzp, int(transformed), scalar), "crava_hip_grid_op")

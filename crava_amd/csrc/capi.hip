@@ -1455,6 +1455,21 @@ int crava_hip_err_corr_plane(crava_hip_ctx *ctx, void *plane, int nxp, int nyp, 
   return fft_common(ctx, &plane, 1, nxp, nyp, 1, CRAVA_HIP_COVARIANCE, precision, false);
 }
 
+int crava_hip_corr_spectrum_separable(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp,
+                                      const float *corr_xy, const float *circ_t, int precision) {
+  if (!ctx || !grid || !corr_xy || !circ_t || !grid_dims_ok(nxp, nyp, nzp) ||
+      (precision != CRAVA_HIP_F32 && precision != CRAVA_HIP_MIXED && precision != CRAVA_HIP_F64)) {
+    set_error("invalid arguments to crava_hip_corr_spectrum_separable");
+    return CRAVA_HIP_E_INVALID;
+  }
+  const bool f64 = precision == CRAVA_HIP_F64;
+  // plane 0 of the grid: corrXY, then its unscaled 2-D transform in place
+  int rc = crava::launch_fill_err_corr_plane(grid, f64, nxp, nyp, nzp, corr_xy, ctx->stream);
+  if (rc) return rc;
+  if ((rc = fft_common(ctx, &grid, 1, nxp, nyp, 1, CRAVA_HIP_COVARIANCE, precision, false))) return rc;
+  return crava::launch_corr_outer(grid, f64, nxp, nyp, nzp, circ_t, ctx->stream);
+}
+
 int crava_hip_post_set_errcorr_plane(crava_hip_post *post, int plane) {
   if (!post) {
     set_error("NULL posterior");

@@ -126,6 +126,46 @@ __global__ void __launch_bounds__(256) k_fill_err_corr_plane(T *plane, int nxp, 
   }
 }
 
+// Spectrum of a separable correlation grid corrXY(i, j) circT(k) (the grid
+// fillInParamCorr writes with zero gradients): plane 0 holds the 2-D
+// spectrum C of corrXY; plane kz of the output is C T(kz), with
+// T(kz) = sum_m circT[m] exp(-2 pi i kz m / nzp) summed in double by each
+// workgroup (fixed order) and the product rounded once.  Planes kz0 ..
+// kz0 + gridDim.y - 1; a launch that writes plane 0 reads and writes each
+// point in one thread, so it runs after the launch over the other planes.
+template <typename T>
+__global__ void __launch_bounds__(256) k_corr_outer(T *grid, long long plane, int nzp, int kz0,
+                                                    const float *__restrict__ circ_t) {
+  __shared__ double red[2][256];
+  const int kz = kz0 + (int)blockIdx.y;
+  const int tid = (int)threadIdx.x;
+  double sr = 0.0, si = 0.0;
+  for (int m = tid; m < nzp; m += 256) {
+    const long long r = ((long long)kz * m) % nzp;
+    double sn, cs;
+    sincospi(-2.0 * (double)r / (double)nzp, &sn, &cs);
+    sr += (double)circ_t[m] * cs;
+    si += (double)circ_t[m] * sn;
+  }
+  red[0][tid] = sr;
+  red[1][tid] = si;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) {
+      red[0][tid] += red[0][tid + h];
+      red[1][tid] += red[1][tid + h];
+    }
+    __syncthreads();
+  }
+  const double tr = red[0][0], ti = red[1][0];
+  for (long long p = (long long)blockIdx.x * 256 + tid; p < plane; p += (long long)gridDim.x * 256) {
+    const double cr = (double)grid[2 * p], ci = (double)grid[2 * p + 1];
+    const long long o = (long long)kz * plane + p;
+    grid[2 * o] = (T)(cr * tr - ci * ti);
+    grid[2 * o + 1] = (T)(cr * ti + ci * tr);
+  }
+}
+
 // Element-wise operators (src/fftgrid.cpp:1877-1999), over n floats (real
 // domain) or n complex values (transformed).
 __global__ void k_grid_op(int op, float *a, const float *__restrict__ b, long long n, int transformed, float s) {
@@ -331,6 +371,24 @@ int launch_fill_err_corr_plane(void *plane, bool f64, int nxp, int nyp, int nzp,
     hipLaunchKernelGGL(k_fill_err_corr_plane<float>, dim3(blocks_for(n)), dim3(256), 0, s,
                        static_cast<float *>(plane), nxp, nyp, nzp, corr_xy);
   CRAVA_HIP_CHECK(hipGetLastError());
+  return CRAVA_HIP_OK;
+}
+
+int launch_corr_outer(void *grid, bool f64, int nxp, int nyp, int nzp, const float *circ_t, hipStream_t s) {
+  const long long plane = (long long)(nxp / 2 + 1) * nyp;
+  const unsigned gx = (unsigned)std::min<long long>((plane + 255) / 256, 64);
+  // planes 1 .. nzp - 1 from plane 0, then plane 0 in place
+  for (int pass = 0; pass < 2; ++pass) {
+    const int kz0 = pass == 0 ? 1 : 0, ny = pass == 0 ? nzp - 1 : 1;
+    if (ny < 1) continue;
+    if (f64)
+      hipLaunchKernelGGL(k_corr_outer<double>, dim3(gx, (unsigned)ny), dim3(256), 0, s, static_cast<double *>(grid),
+                         plane, nzp, kz0, circ_t);
+    else
+      hipLaunchKernelGGL(k_corr_outer<float>, dim3(gx, (unsigned)ny), dim3(256), 0, s, static_cast<float *>(grid),
+                         plane, nzp, kz0, circ_t);
+    CRAVA_HIP_CHECK(hipGetLastError());
+  }
   return CRAVA_HIP_OK;
 }
 

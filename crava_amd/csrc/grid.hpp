@@ -13,6 +13,9 @@ int launch_fill_err_corr(float *grid, int nxp, int nyp, int nzp, const float *co
 // double, into the real-layout plane [nyp][2 (nxp / 2 + 1)].
 int launch_fill_err_corr_plane(void *plane, bool f64, int nxp, int nyp, int nzp, const float *corr_xy,
                                hipStream_t s);
+// Planes 0 .. nzp - 1 of a separable correlation spectrum from the 2-D
+// spectrum of corrXY in plane 0 and the 1-D transform of circ_t (device).
+int launch_corr_outer(void *grid, bool f64, int nxp, int nyp, int nzp, const float *circ_t, hipStream_t s);
 // n floats (real domain) or n complex values (transformed).
 int launch_grid_op(int op, float *a, const float *b, long long n, bool transformed, float scalar, hipStream_t s);
 // Time-lapse merge (grid.hip): n complex points per grid.

@@ -304,6 +304,20 @@ int crava_hip_post_run_separable(crava_hip_post *post, int precision, void *cons
 int crava_hip_err_corr_plane(crava_hip_ctx *ctx, void *plane, int nxp, int nyp, int nzp, const float *corr_xy,
                              int precision);
 
+/* The transformed correlation grid of a separable prior with zero
+ * correlation gradients, without filling it: fillInParamCorr
+ * (src/fftgrid.cpp:1119-1164) then writes corrXY(i, j) circT(k) (sub_k = k
+ * exactly), whose unscaled 3-D transform (COVARIANCE) is the 2-D transform
+ * of corrXY times the 1-D transform of circT.  grid (device, the FFTGrid
+ * layout, float or double for CRAVA_HIP_F64) receives that spectrum, as
+ * crava_hip_fill_param_corr with scale 1 followed by crava_hip_fft_r2c
+ * would give it up to the fill's rounding of each product; corr_xy
+ * [nyp][nxp] and circ_t [nzp] are device arrays.  The 2-D transform uses
+ * `precision` (CRAVA_HIP_MIXED: double arithmetic), the 1-D one and the
+ * product double arithmetic, rounded once. */
+int crava_hip_corr_spectrum_separable(crava_hip_ctx *ctx, void *grid, int nxp, int nyp, int nzp,
+                                      const float *corr_xy, const float *circ_t, int precision);
+
 /* plane != 0: every later posterior run of `post` (crava_hip_post_run*,
  * crava_hip_nll_grad, batches created after this call) takes `errcorr` as the
  * one Fourier plane of crava_hip_err_corr_plane, read for every kz plane,

@@ -199,3 +199,65 @@ def test_batch_with_planes_matches_single_path(ctx, precision):
     b.close()
     for v in vols:
         v.post.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [(80, 64, 160), (15, 12, 24), (9, 7, 5)])
+@pytest.mark.parametrize("precision", [L.MIXED, L.F64], ids=["mixed", "f64"])
+def test_separable_corr_spectrum_equals_fill_and_transform(ctx, n, precision):
+    """crava_hip_corr_spectrum_separable against fillInParamCorr (zero
+    gradients) + the 3-D transform on the GPU, and against numpy's float64
+    3-D transform of corrXY(i, j) circT(k)."""
+    from crava_amd.core import corr_spectrum_separable, fill_param_corr
+    nxp, nyp, nzp = n
+    print(f"separable correlation spectrum {n} precision {precision}", flush=True)
+    cxy = _corr_xy(nxp, nyp, nzp)
+    ct = np.random.default_rng(nzp + 7).uniform(-0.3, 1.0, nzp).astype(np.float32)
+    cxy_d, ct_d = torch.from_numpy(cxy).cuda(), torch.from_numpy(ct).cuda()
+    rnxp = 2 * (nxp // 2 + 1)
+    g = torch.zeros((nzp, nyp, rnxp), dtype=torch.float32, device="cuda")
+    fill_param_corr(ctx, [g], [1.0], nxp, nyp, nzp, cxy_d, ct_d)
+    if precision == L.F64:
+        g = g.double()
+    ctx.r2c(g, nxp, nyp, nzp, L.COVARIANCE, precision)
+    out = torch.full((nzp, nyp, rnxp), 5.0, dtype=g.dtype, device="cuda")
+    corr_spectrum_separable(ctx, out, nxp, nyp, nzp, cxy_d, ct_d, precision)
+    a, r = _cplx(out), _cplx(g)
+    exact = np.fft.rfftn(ct.astype(np.float64)[:, None, None] * cxy.astype(np.float64)[None, :, :])
+    scale = np.abs(exact).max()
+    # the fill rounds each product to float: the grid path differs from the
+    # exact product of the two transforms by about eps_float / sqrt(N)
+    assert np.abs(a - r).max() <= (2e-6 if precision == L.MIXED else 1e-7) * scale
+    assert np.abs(a - exact).max() <= (2e-6 if precision == L.MIXED else 1e-12) * scale
+
+
+@pytest.mark.gpu
+def test_posterior_on_separable_corr_spectrum(ctx):
+    """The separable mixed posterior on the factor spectrum and on the
+    filled and transformed correlation grid agree to the FP64 bar."""
+    from crava_amd.core import corr_spectrum_separable, fill_param_corr
+    print("posterior on the separable correlation spectrum", flush=True)
+    prob = synth.make_problem(synth.config("small"), "cpu")
+    c = prob.cfg
+    n = (c.nxp, c.nyp, c.nzp)
+    mu, d, cv, ec = fourier_inputs(prob)
+    cxy = torch.from_numpy(np.ascontiguousarray(prob.corr_xy, dtype=np.float32)).cuda()
+    ct = torch.from_numpy(np.ascontiguousarray(prob.circ_corr_t, dtype=np.float32)).cuda()
+    g = torch.zeros((c.nzp, c.nyp, 2 * (c.nxp // 2 + 1)), dtype=torch.float32, device="cuda")
+    fill_param_corr(ctx, [g], [1.0], *n, cxy, ct)
+    ctx.r2c(g, *n, L.COVARIANCE, L.MIXED)
+    f = torch.empty_like(g)
+    corr_spectrum_separable(ctx, f, *n, cxy, ct, L.MIXED)
+    outs = []
+    for corr in (g, f):
+        tm = [torch.from_numpy(x.copy()).cuda() for x in mu]
+        td = [torch.from_numpy(x.copy()).cuda() for x in d]
+        tc = [torch.from_numpy(x.copy()).cuda() for x in cv]
+        post = Posterior(ctx, synth.post_params(prob, 1e-3), problem_tables(prob))
+        post.run(L.MIXED, tm, td, tc, torch.from_numpy(ec.copy()).cuda(), corr=corr)
+        torch.cuda.synchronize()
+        post.close()
+        outs.append(tm + td + tc)
+    for a, r in zip(*outs):
+        a64, r64 = a.double(), r.double()
+        assert float(torch.linalg.vector_norm(a64 - r64) / torch.linalg.vector_norm(r64)) < 1e-5
