@@ -1173,7 +1173,9 @@ def main():
         "dtype": "f32+f64 (mixed)",
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"posterior path {nxp}x{nyp}x{nzp} FFT grid, ntheta={nt}", "fourier_points": P,
-                   "in_band_points": n_in, "transforms_per_step": 26 if args.errcorr_grid else 25,
+                   "in_band_points": n_in, "transforms_per_step": 26 if args.errcorr_grid else (25 if ms_mixed_graph < ms_mixed_direct else 24),
+                   "correlation_spectrum": "filled and transformed (one-volume graph)" if args.errcorr_grid or
+                   ms_mixed_graph < ms_mixed_direct else "FFT2(corrXY) x FFT1(circT), no fill (one stream)",
                    "errcorr": "spectrum as one Fourier plane (zero correlation gradients: the fillInErrCorr grid "
                               "is corrXY in plane 0, its transform that plane's 2-D transform on every kz plane)"
                               if not args.errcorr_grid else "grid filled and transformed every step",
